@@ -1,0 +1,567 @@
+/*
+ * scafes_oracle.c -- CPU restatement of ScaFES's explicit time-stepping path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker; see scafes_oracle.h).  Built with
+ * -O2 -fopenmp -ffp-contract=off so every FP64 operation rounds exactly as
+ * written, in the order the reference's hooks write it.
+ */
+#include "scafes_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static int g_threads = 0;
+
+void oracle_set_threads(int n) { g_threads = n; }
+
+int oracle_get_threads(void)
+{
+#ifdef _OPENMP
+    return g_threads > 0 ? g_threads : omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+#ifdef _OPENMP
+#define NTHR (g_threads > 0 ? g_threads : omp_get_max_threads())
+#endif
+
+/* memAll extent along axis p (DataField ctor: memAll = partition +- stencilWidth,
+ * Problem.hpp:1495-1514). */
+static long ext(const oracle_grid* g, int p)
+{
+    return p < g->dim ? (long)g->n[p] + 2L * g->hw : 1L;
+}
+
+long oracle_nelems(const oracle_grid* g) { return ext(g, 0) * ext(g, 1) * ext(g, 2); }
+
+long oracle_nnodes(const oracle_grid* g) { return (long)g->n[0] * g->n[1] * g->n[2]; }
+
+/* h_p = (x_last - x_first) / (n_p - 1)   (Grid.hpp:547-555). */
+double oracle_gridsize(const oracle_grid* g, int p)
+{
+    return (g->x1[p] - g->x0[p]) / (g->n[p] - 1);
+}
+
+/* Fortran scalar index of node (i,j,k) in memAll (Grid.hpp:955-968). */
+static long pos(const oracle_grid* g, int i, int j, int k)
+{
+    long o0 = g->dim > 0 ? g->hw : 0, o1 = g->dim > 1 ? g->hw : 0, o2 = g->dim > 2 ? g->hw : 0;
+    return (i + o0) + ext(g, 0) * ((j + o1) + ext(g, 1) * (k + o2));
+}
+
+/* Node index in a stencil-0 field (known dfs F, G: no ghost layer). */
+static long npos(const oracle_grid* g, int i, int j, int k)
+{
+    return i + (long)g->n[0] * (j + (long)g->n[1] * k);
+}
+
+/* GLOBAL_BORDER if within nLayersAtBorder of the global edge along any
+ * active axis, else GLOBAL_INNER (Problem.hpp:3902-3980). */
+static int is_border(const oracle_grid* g, int i, int j, int k)
+{
+    int idx[3] = {i, j, k};
+    for (int p = 0; p < g->dim; ++p) {
+        if (idx[p] < g->nlayers || idx[p] > g->n[p] - 1 - g->nlayers) return 1;
+    }
+    return 0;
+}
+
+/* Neighbour memory offset along axis p (Grid::neigh, Grid.hpp:885-921; the
+ * inner-node stencil never reaches the sentinel branch). */
+static long stride(const oracle_grid* g, int p)
+{
+    return p == 0 ? 1L : (p == 1 ? ext(g, 0) : ext(g, 0) * ext(g, 1));
+}
+
+/* coordinates(idxNode) = x_first + h*(i - first)  (Grid.hpp:719-733). */
+static void coords(const oracle_grid* g, int i, int j, int k, double* x)
+{
+    int idx[3] = {i, j, k};
+    for (int p = 0; p < g->dim; ++p) x[p] = g->x0[p] + oracle_gridsize(g, p) * idx[p];
+}
+
+/*---------------------------------------------------------------------------*/
+/* Constant-coefficient heat / Pennes system-test update.
+ * Inner: HeatEqnFDM::updateInner (tests/system/HeatEqnFDM/HeatEqnFDM.hpp:204-218)
+ *        and PennesBioheatEqnFDM::updateInner (PennesBioheatEqnFDM.hpp:153-168).
+ * Border: Dirichlet::updateBorder, vNew = knownDf(G) (Dirichlet/Dirichlet.hpp:124-130).
+ * Copy-old-to-new before the hooks: setVectDfDepToDfIndep (Problem.hpp:6226-6256)
+ * -> DataField::assignValues (DataField.hpp:1147-1158). */
+void oracle_heat7_step(const oracle_grid* g, const double* uo, double* un,
+                       const double* fo, const double* gn, int border_mode,
+                       double border_value, const oracle_heat7_coef* c)
+{
+    const long ne = oracle_nelems(g);
+    memcpy(un, uo, sizeof(double) * (size_t)ne);
+    double hh[3];
+    long st[3];
+    for (int p = 0; p < 3; ++p) {
+        hh[p] = p < g->dim ? oracle_gridsize(g, p) * oracle_gridsize(g, p) : 1.0;
+        st[p] = stride(g, p);
+    }
+    const int n0 = g->n[0], n1 = g->n[1], n2 = g->n[2];
+#pragma omp parallel for collapse(2) schedule(static) num_threads(NTHR)
+    for (int k = 0; k < n2; ++k) {
+        for (int j = 0; j < n1; ++j) {
+            for (int i = 0; i < n0; ++i) {
+                const long m = pos(g, i, j, k);
+                if (!is_border(g, i, j, k)) {
+                    double v = uo[m];
+                    for (int p = 0; p < g->dim; ++p) {
+                        v += c->tau * c->coeff_a * (uo[m - st[p]] + uo[m + st[p]] - 2.0 * uo[m]) /
+                             (hh[p]);
+                    }
+                    if (c->has_perf) v -= c->tau * c->perf_c * c->perf_w * uo[m];
+                    if (c->has_source && fo) v += c->tau * c->rhoc_inv * fo[npos(g, i, j, k)];
+                    un[m] = v;
+                } else {
+                    if (border_mode == ORACLE_BORDER_FIELD) un[m] = gn[npos(g, i, j, k)];
+                    else if (border_mode == ORACLE_BORDER_VALUE) un[m] = border_value;
+                }
+            }
+        }
+    }
+}
+
+/* Known fields of the TimeLinSpaceLin system tests at time t:
+ *   heat   evalInner/evalBorder: tests/system/HeatEqnFDM/Dirichlet/TimeLinSpaceLin/TimeLinSpaceLin.hpp:75-123
+ *   Pennes evalInner/evalBorder: tests/system/PennesBioheatEqnFDM/Dirichlet/TimeLinSpaceLin/TimeLinSpaceLin.hpp:75-124
+ * F, G, Y are stencil-0 fields. */
+static void eval_known_timelin(const oracle_grid* g, int known, double t,
+                               const oracle_heat7_coef* c, double* F, double* G, double* Y)
+{
+    const int n0 = g->n[0], n1 = g->n[1], n2 = g->n[2];
+    (void)c;
+#pragma omp parallel for collapse(2) schedule(static) num_threads(NTHR)
+    for (int k = 0; k < n2; ++k) {
+        for (int j = 0; j < n1; ++j) {
+            for (int i = 0; i < n0; ++i) {
+                double x[3] = {0, 0, 0};
+                coords(g, i, j, k, x);
+                const long m = npos(g, i, j, k);
+                if (!is_border(g, i, j, k)) {
+                    double f = 1.0;
+                    for (int p = 0; p < g->dim; ++p) f += x[p];
+                    f *= 1.0 * 1.0; /* RHO * C */
+                    if (known == ORACLE_KNOWN_PENNES_TIMELIN) {
+                        double tmp = 1.0;
+                        for (int p = 0; p < g->dim; ++p) tmp += x[p];
+                        tmp *= (1.0 + t);
+                        f += c->rhob_cb_w * tmp;
+                    }
+                    F[m] = f;
+                    G[m] = 0.0;
+                    double y = 1.0;
+                    for (int p = 0; p < g->dim; ++p) y += x[p];
+                    y *= (1.0 + t);
+                    Y[m] = y;
+                } else {
+                    F[m] = 0.0;
+                    double gv = 1.0;
+                    for (int p = 0; p < g->dim; ++p) gv += x[p];
+                    gv *= (1.0 + t);
+                    G[m] = gv;
+                    Y[m] = gv;
+                }
+            }
+        }
+    }
+}
+
+/* compErrLinf over all partition nodes (DataField.hpp:1439-1530); b is a
+ * stencil-0 field when bnodes != 0. */
+static double linf_vs_nodes(const oracle_grid* g, const double* a, const double* b)
+{
+    double e = 0.0;
+    for (int k = 0; k < g->n[2]; ++k)
+        for (int j = 0; j < g->n[1]; ++j)
+            for (int i = 0; i < g->n[0]; ++i) {
+                double d = fabs(a[pos(g, i, j, k)] - b[npos(g, i, j, k)]);
+                if (d > e) e = d;
+            }
+    return e;
+}
+
+double oracle_linf_nodes(const oracle_grid* g, const double* a, const double* b)
+{
+    double e = 0.0;
+    for (int k = 0; k < g->n[2]; ++k)
+        for (int j = 0; j < g->n[1]; ++j)
+            for (int i = 0; i < g->n[0]; ++i) {
+                const long m = pos(g, i, j, k);
+                double d = fabs(a[m] - b[m]);
+                if (d > e) e = d;
+            }
+    return e;
+}
+
+/* Restated Problem::iterateOverTime (Problem.hpp:2558-2662) for one unknown
+ * field U and known fields F, G (and exact Y when computing errors):
+ *   n = 0: reset (defaultValue 0 incl. ghosts, :2998-3073), init hooks
+ *          (evalInitPbPhase :2754-2826; TimeLinSpaceLin initInner :126-144),
+ *          old <- new (setValuesAtMemAll);
+ *   n > 0: evalKnownDfs(n) into the "new" known buffers at t = t0 + n*tau
+ *          (:3433-3485, time() :2482-2486), update phase with F_old and G_new;
+ *   each n: compErrLinf vs Y_new, swap pointers, times advance by tau. */
+double oracle_heat7_run(const oracle_grid* g, double* u, int nsteps, double t0,
+                        const oracle_heat7_coef* c, int known, double* linf)
+{
+    const long ne = oracle_nelems(g), nn = oracle_nnodes(g);
+    double* uo = (double*)malloc(sizeof(double) * (size_t)ne);
+    double* un = (double*)malloc(sizeof(double) * (size_t)ne);
+    double* Fo = (double*)calloc((size_t)nn, sizeof(double));
+    double* Fn = (double*)calloc((size_t)nn, sizeof(double));
+    double* Go = (double*)calloc((size_t)nn, sizeof(double));
+    double* Gn = (double*)calloc((size_t)nn, sizeof(double));
+    double* Yo = (double*)calloc((size_t)nn, sizeof(double));
+    double* Yn = (double*)calloc((size_t)nn, sizeof(double));
+    const int timelin = (known == ORACLE_KNOWN_HEAT_TIMELIN || known == ORACLE_KNOWN_PENNES_TIMELIN);
+    double t_unknown_new = t0;
+
+    for (int n = 0; n <= nsteps; ++n) {
+        if (timelin) eval_known_timelin(g, known, t0 + n * c->tau, c, Fn, Gn, Yn);
+        if (n == 0) {
+            if (timelin) {
+                for (long m = 0; m < ne; ++m) un[m] = 0.0;
+                for (int k = 0; k < g->n[2]; ++k)
+                    for (int j = 0; j < g->n[1]; ++j)
+                        for (int i = 0; i < g->n[0]; ++i) {
+                            double x[3] = {0, 0, 0};
+                            coords(g, i, j, k, x);
+                            double y = 1.0;
+                            for (int p = 0; p < g->dim; ++p) y += x[p];
+                            y *= (1.0 + t0);
+                            un[pos(g, i, j, k)] = y;
+                        }
+            } else {
+                memcpy(un, u, sizeof(double) * (size_t)ne);
+            }
+            memcpy(uo, un, sizeof(double) * (size_t)ne);
+            memcpy(Fo, Fn, sizeof(double) * (size_t)nn);
+            memcpy(Go, Gn, sizeof(double) * (size_t)nn);
+            memcpy(Yo, Yn, sizeof(double) * (size_t)nn);
+            t_unknown_new = t0;
+        } else {
+            oracle_heat7_step(g, uo, un, timelin ? Fo : NULL, Gn,
+                              timelin ? ORACLE_BORDER_FIELD : ORACLE_BORDER_VALUE, 0.0, c);
+            t_unknown_new = t_unknown_new + c->tau;
+        }
+        if (linf && timelin) linf[n] = linf_vs_nodes(g, un, Yn);
+        /* swapPointersOf{Known,Unknown}Dfs (:3238-3316) */
+        double* tmp;
+        tmp = uo; uo = un; un = tmp;
+        tmp = Fo; Fo = Fn; Fn = tmp;
+        tmp = Go; Go = Gn; Gn = tmp;
+        tmp = Yo; Yo = Yn; Yn = tmp;
+    }
+    memcpy(u, uo, sizeof(double) * (size_t)ne);
+    free(uo); free(un); free(Fo); free(Fn); free(Go); free(Gn); free(Yo); free(Yn);
+    return t_unknown_new;
+}
+
+/*---------------------------------------------------------------------------*/
+/* Variable-coefficient conservative 7-point heat.  The reference has no
+ * variable-kappa divergence form; this follows the structure of its
+ * node-centred hooks (per-axis accumulation into vNew after the copy,
+ * HeatEqnFDM.hpp:209-215; coefficient read as a known field like
+ * examples/MRIDataPBHEqnFDM/MRIDataPBHEqnFDM.hpp:143-168) with face
+ * diffusivity 0.5*(kappa_C + kappa_nb):
+ *   u' = u + sum_p tau*(0.5(kE+kC)(uE-uC) - 0.5(kC+kW)(uC-uW)) / h_p^2. */
+void oracle_heat7var_step(const oracle_grid* g, const double* uo, double* un,
+                          const double* kap, double tau, int border_mode,
+                          double border_value)
+{
+    /* copy-old-to-new before the hooks (Problem.hpp:6226-6256) */
+    memcpy(un, uo, sizeof(double) * (size_t)oracle_nelems(g));
+    double hh[3];
+    long st[3];
+    for (int p = 0; p < 3; ++p) {
+        hh[p] = p < g->dim ? oracle_gridsize(g, p) * oracle_gridsize(g, p) : 1.0;
+        st[p] = stride(g, p);
+    }
+    const int n0 = g->n[0], n1 = g->n[1], n2 = g->n[2];
+#pragma omp parallel for collapse(2) schedule(static) num_threads(NTHR)
+    for (int k = 0; k < n2; ++k) {
+        for (int j = 0; j < n1; ++j) {
+            for (int i = 0; i < n0; ++i) {
+                const long m = pos(g, i, j, k);
+                if (!is_border(g, i, j, k)) {
+                    const double uc = uo[m], kc = kap[m];
+                    double v = uc;
+                    for (int p = 0; p < g->dim; ++p) {
+                        const double uw = uo[m - st[p]], ue = uo[m + st[p]];
+                        const double kw = kap[m - st[p]], ke = kap[m + st[p]];
+                        v += tau * (0.5 * (ke + kc) * (ue - uc) - 0.5 * (kc + kw) * (uc - uw)) / hh[p];
+                    }
+                    un[m] = v;
+                } else if (border_mode == ORACLE_BORDER_VALUE) {
+                    un[m] = border_value;
+                } else {
+                    un[m] = uo[m];
+                }
+            }
+        }
+    }
+}
+
+void oracle_heat7var_run(const oracle_grid* g, double* u, const double* kap, double tau,
+                         int nsteps, int border_mode, double border_value, double* scratch)
+{
+    const long ne = oracle_nelems(g);
+    double* a = u;
+    double* b = scratch;
+    memcpy(b, a, sizeof(double) * (size_t)ne); /* ghosts identical in both buffers */
+    for (int n = 1; n <= nsteps; ++n) {
+        oracle_heat7var_step(g, a, b, kap, tau, border_mode, border_value);
+        double* t = a; a = b; b = t;
+    }
+    if (a != u) memcpy(u, a, sizeof(double) * (size_t)ne);
+}
+
+/*---------------------------------------------------------------------------*/
+/* Pennes bioheat, TumorHeatEqnFDM::updateInner form
+ * (examples/TumorHeatEqnFDM/TumorHeatEqnFDM.hpp:233-269) with the
+ * piecewise-constant omega/Q replaced by node fields w, Q:
+ *   T' = T + sum_p tau*cdiff*(W+E-2T)/h_p^2 + tau*cperf*w*(T_a - T) + tau*csrc*Q.
+ * Border nodes keep their copied value (Dirichlet at the initial state). */
+void oracle_pennes_step(const oracle_grid* g, const double* to, double* tn, const double* w,
+                        const double* q, const oracle_pennes_coef* c)
+{
+    memcpy(tn, to, sizeof(double) * (size_t)oracle_nelems(g));
+    double hh[3];
+    long st[3];
+    for (int p = 0; p < 3; ++p) {
+        hh[p] = p < g->dim ? oracle_gridsize(g, p) * oracle_gridsize(g, p) : 1.0;
+        st[p] = stride(g, p);
+    }
+    const int n0 = g->n[0], n1 = g->n[1], n2 = g->n[2];
+#pragma omp parallel for collapse(2) schedule(static) num_threads(NTHR)
+    for (int k = 0; k < n2; ++k) {
+        for (int j = 0; j < n1; ++j) {
+            for (int i = 0; i < n0; ++i) {
+                const long m = pos(g, i, j, k);
+                if (!is_border(g, i, j, k)) {
+                    double v = to[m];
+                    for (int p = 0; p < g->dim; ++p)
+                        v += c->tau * c->cdiff * (to[m - st[p]] + to[m + st[p]] - 2.0 * to[m]) / hh[p];
+                    v += c->tau * c->cperf * w[m] * (c->t_a - to[m]);
+                    v += c->tau * c->csrc * q[m];
+                    tn[m] = v;
+                } else {
+                    tn[m] = to[m];
+                }
+            }
+        }
+    }
+}
+
+void oracle_pennes_forward_all(const oracle_grid* g, double* states, const double* w,
+                               const double* q, const oracle_pennes_coef* c, int nsteps)
+{
+    const long ne = oracle_nelems(g);
+    for (int n = 1; n <= nsteps; ++n)
+        oracle_pennes_step(g, states + (size_t)(n - 1) * ne, states + (size_t)n * ne, w, q, c);
+}
+
+/* Discrete adjoint of oracle_pennes_step (inner nodes only; border values
+ * are constants, so their adjoint is dropped):
+ *   lam^{n-1}_i = lam^n_i + sum_p tau*cdiff*(lam_W + lam_E - 2 lam_i)/h_p^2
+ *                 - tau*cperf*w_i*lam^n_i           (lam = 0 on border nodes)
+ *   dJ/dw_i    += lam^n_i * tau*cperf*(T_a - T^{n-1}_i). */
+double oracle_pennes_grad(const oracle_grid* g, const double* states, const double* w,
+                          const double* target, const oracle_pennes_coef* c, int nsteps,
+                          double* grad)
+{
+    const long ne = oracle_nelems(g);
+    double* lam = (double*)calloc((size_t)ne, sizeof(double));
+    double* lam2 = (double*)calloc((size_t)ne, sizeof(double));
+    const double* tN = states + (size_t)nsteps * ne;
+    double hh[3];
+    long st[3];
+    for (int p = 0; p < 3; ++p) {
+        hh[p] = p < g->dim ? oracle_gridsize(g, p) * oracle_gridsize(g, p) : 1.0;
+        st[p] = stride(g, p);
+    }
+    double J = 0.0;
+    for (long m = 0; m < ne; ++m) grad[m] = 0.0;
+    for (int k = 0; k < g->n[2]; ++k)
+        for (int j = 0; j < g->n[1]; ++j)
+            for (int i = 0; i < g->n[0]; ++i) {
+                if (is_border(g, i, j, k)) continue;
+                const long m = pos(g, i, j, k);
+                const double r = tN[m] - target[m];
+                J += 0.5 * r * r;
+                lam[m] = r;
+            }
+    const int n0 = g->n[0], n1 = g->n[1], n2 = g->n[2];
+    for (int n = nsteps; n >= 1; --n) {
+        const double* tp = states + (size_t)(n - 1) * ne;
+#pragma omp parallel for collapse(2) schedule(static) num_threads(NTHR)
+        for (int k = 0; k < n2; ++k)
+            for (int j = 0; j < n1; ++j)
+                for (int i = 0; i < n0; ++i) {
+                    const long m = pos(g, i, j, k);
+                    if (is_border(g, i, j, k)) { lam2[m] = 0.0; continue; }
+                    grad[m] += lam[m] * (c->tau * c->cperf * (c->t_a - tp[m]));
+                    double v = lam[m];
+                    for (int p = 0; p < g->dim; ++p)
+                        v += c->tau * c->cdiff * (lam[m - st[p]] + lam[m + st[p]] - 2.0 * lam[m]) / hh[p];
+                    v -= c->tau * c->cperf * w[m] * lam[m];
+                    lam2[m] = v;
+                }
+        double* t = lam; lam = lam2; lam2 = t;
+    }
+    free(lam);
+    free(lam2);
+    return J;
+}
+
+/*---------------------------------------------------------------------------*/
+/* Yee FDTD in a PEC box (normalised units).  The reference's leapfrog mode
+ * runs updateInner (H from E) then swaps and runs updateInner2 (E from H)
+ * (Problem.hpp:2603-2619); its curl stencil is
+ * dest_old + constant*(dfdl - dfdr) with forward differences for H and
+ * backward differences for E (examples/EMFDTD/stencils/CompleteCPMLCurlStencil.hpp:38-80,
+ * constants +cfln / -cfln at examples/EMFDTD/EMFDTD.hpp:207-212).  Vacuum
+ * (no CPML, sigma = 0) reduces it to the plain Yee curl below. */
+#define YI(i, j, k) ((i) + (long)M * ((j) + (long)M * (k)))
+void oracle_yee_half_h(int N, double s, double* f)
+{
+    const long M = N + 1, nf = M * M * M;
+    double *Ex = f, *Ey = f + nf, *Ez = f + 2 * nf, *Hx = f + 3 * nf, *Hy = f + 4 * nf,
+           *Hz = f + 5 * nf;
+#pragma omp parallel for schedule(static) num_threads(NTHR)
+    for (int k = 0; k <= N; ++k)
+        for (int j = 0; j <= N; ++j)
+            for (int i = 0; i <= N; ++i) {
+                const long m = YI(i, j, k);
+                if (j < N && k < N)
+                    Hx[m] = Hx[m] + s * ((Ey[YI(i, j, k + 1)] - Ey[m]) - (Ez[YI(i, j + 1, k)] - Ez[m]));
+                if (i < N && k < N)
+                    Hy[m] = Hy[m] + s * ((Ez[YI(i + 1, j, k)] - Ez[m]) - (Ex[YI(i, j, k + 1)] - Ex[m]));
+                if (i < N && j < N)
+                    Hz[m] = Hz[m] + s * ((Ex[YI(i, j + 1, k)] - Ex[m]) - (Ey[YI(i + 1, j, k)] - Ey[m]));
+            }
+}
+
+void oracle_yee_half_e(int N, double s, double* f)
+{
+    const long M = N + 1, nf = M * M * M;
+    double *Ex = f, *Ey = f + nf, *Ez = f + 2 * nf, *Hx = f + 3 * nf, *Hy = f + 4 * nf,
+           *Hz = f + 5 * nf;
+#pragma omp parallel for schedule(static) num_threads(NTHR)
+    for (int k = 0; k <= N; ++k)
+        for (int j = 0; j <= N; ++j)
+            for (int i = 0; i <= N; ++i) {
+                const long m = YI(i, j, k);
+                /* tangential E on the PEC walls stays zero */
+                if (i < N && j >= 1 && j < N && k >= 1 && k < N)
+                    Ex[m] = Ex[m] + s * ((Hz[m] - Hz[YI(i, j - 1, k)]) - (Hy[m] - Hy[YI(i, j, k - 1)]));
+                if (j < N && i >= 1 && i < N && k >= 1 && k < N)
+                    Ey[m] = Ey[m] + s * ((Hx[m] - Hx[YI(i, j, k - 1)]) - (Hz[m] - Hz[YI(i - 1, j, k)]));
+                if (k < N && i >= 1 && i < N && j >= 1 && j < N)
+                    Ez[m] = Ez[m] + s * ((Hy[m] - Hy[YI(i - 1, j, k)]) - (Hx[m] - Hx[YI(i, j - 1, k)]));
+            }
+}
+
+void oracle_yee_step(int N, double s, double* f)
+{
+    oracle_yee_half_h(N, s, f);
+    oracle_yee_half_e(N, s, f);
+}
+
+double oracle_yee_energy(int N, const double* f, const double* h_prev)
+{
+    const long M = N + 1, nf = M * M * M;
+    double e = 0.0, h = 0.0;
+    for (long m = 0; m < 3 * nf; ++m) e += f[m] * f[m];
+    for (long m = 0; m < 3 * nf; ++m) h += f[3 * nf + m] * h_prev[m];
+    return 0.5 * e + 0.5 * h;
+}
+#undef YI
+
+/*---------------------------------------------------------------------------*/
+/* 2-D heat, scalar kappa, zero Dirichlet border (ensemble member). */
+static void heat2d_step(int n, double hh, double ck, const double* u, double* v)
+{
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            const long m = i + (long)n * j;
+            if (i == 0 || j == 0 || i == n - 1 || j == n - 1) { v[m] = u[m]; continue; }
+            double x = u[m];
+            x += ck * (u[m - 1] + u[m + 1] - 2.0 * u[m]) / hh;
+            x += ck * (u[m - n] + u[m + n] - 2.0 * u[m]) / hh;
+            v[m] = x;
+        }
+}
+
+static void lap2d(int n, double hh, const double* u, double* out)
+{
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            const long m = i + (long)n * j;
+            if (i == 0 || j == 0 || i == n - 1 || j == n - 1) { out[m] = 0.0; continue; }
+            out[m] = (u[m - 1] + u[m + 1] - 2.0 * u[m]) / hh + (u[m - n] + u[m + n] - 2.0 * u[m]) / hh;
+        }
+}
+
+void oracle_heat2d_run(int n, double h, double kappa, double tau, int nsteps, double* u,
+                       double* u_prev)
+{
+    const long nn = (long)n * n;
+    double* a = (double*)malloc(sizeof(double) * (size_t)nn);
+    double* b = (double*)malloc(sizeof(double) * (size_t)nn);
+    memcpy(a, u, sizeof(double) * (size_t)nn);
+    memcpy(b, u, sizeof(double) * (size_t)nn);
+    const double hh = h * h, ck = tau * kappa;
+    for (int s = 0; s < nsteps; ++s) {
+        heat2d_step(n, hh, ck, a, b);
+        double* t = a; a = b; b = t;
+    }
+    memcpy(u, a, sizeof(double) * (size_t)nn);
+    if (u_prev) memcpy(u_prev, b, sizeof(double) * (size_t)nn);
+    free(a);
+    free(b);
+}
+
+double oracle_heat2d_grad(int n, double h, double kappa, double tau, int nsteps,
+                          const double* u0, const double* target, double* dj)
+{
+    const long nn = (long)n * n;
+    const double hh = h * h, ck = tau * kappa;
+    double* st = (double*)malloc(sizeof(double) * (size_t)nn * (size_t)(nsteps + 1));
+    double* lam = (double*)calloc((size_t)nn, sizeof(double));
+    double* lam2 = (double*)calloc((size_t)nn, sizeof(double));
+    double* lu = (double*)calloc((size_t)nn, sizeof(double));
+    memcpy(st, u0, sizeof(double) * (size_t)nn);
+    for (int s = 1; s <= nsteps; ++s) heat2d_step(n, hh, ck, st + (size_t)(s - 1) * nn, st + (size_t)s * nn);
+    const double* uN = st + (size_t)nsteps * nn;
+    double J = 0.0;
+    for (long m = 0; m < nn; ++m) {
+        const int i = (int)(m % n), j = (int)(m / n);
+        if (i == 0 || j == 0 || i == n - 1 || j == n - 1) continue;
+        const double r = uN[m] - target[m];
+        J += 0.5 * r * r;
+        lam[m] = r;
+    }
+    double g = 0.0;
+    for (int s = nsteps; s >= 1; --s) {
+        lap2d(n, hh, st + (size_t)(s - 1) * nn, lu);
+        for (long m = 0; m < nn; ++m) g += lam[m] * tau * lu[m];
+        lap2d(n, hh, lam, lu);
+        for (long m = 0; m < nn; ++m) {
+            const int i = (int)(m % n), j = (int)(m / n);
+            lam2[m] = (i == 0 || j == 0 || i == n - 1 || j == n - 1) ? 0.0 : lam[m] + ck * lu[m];
+        }
+        double* t = lam; lam = lam2; lam2 = t;
+    }
+    *dj = g;
+    free(st); free(lam); free(lam2); free(lu);
+    return J;
+}

@@ -1,0 +1,157 @@
+/*
+ * scafes_oracle.h -- CPU restatement of ScaFES's explicit time-stepping path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  Nothing in masterthesis_amd/ links or calls it.
+ *
+ * Every routine restates the reference's FP64 arithmetic in the reference's
+ * operation order (compiled with -ffp-contract=off so no FMA contraction
+ * changes the rounding), citing the reference file:line it follows.
+ *
+ * Layout (ScaFES DataField, src/ScaFES_DataField.hpp:622-642, :1187-1199;
+ * src/ScaFES_Grid.hpp:955-968): Fortran order, x fastest.  A field with
+ * stencil width hw has memAll extents n_p + 2*hw on active axes (extent 1
+ * and no halo on axes >= dim).  The trailing time slot of the reference
+ * lump (DataField.hpp:947-953) is kept by the caller as a scalar.
+ *
+ * Pinning status of each routine (how it is checked, see tests/):
+ *   oracle_heat7_*      pinned: exact discrete solutions of the reference's
+ *                       system tests (TimeLinSpaceLin heat/Pennes), the
+ *                       discrete sine eigenmode closed form, brute force.
+ *   oracle_heat7var_*   pinned: reduction to heat7 for constant kappa,
+ *                       operator symmetry, face-assembled numpy operator.
+ *   oracle_pennes_*     pinned: reduction to heat7 (w=0,Q=0), Tumor-form
+ *                       closed-form steady state, adjoint vs central
+ *                       finite differences and dot-product test.
+ *   oracle_yee_*        pinned: exact PEC cavity eigenmode with the Yee
+ *                       dispersion relation, discrete energy invariant,
+ *                       discrete Gauss law.
+ *   oracle_heat2d_*     pinned: 2-D sine eigenmode closed form, gradient vs
+ *                       central finite differences.
+ *   ADOL-C forward-mode gradients of the reference: parity unpinned (ADOL-C
+ *                       is not installed and no reference test checks them).
+ */
+#ifndef SCAFES_ORACLE_H_
+#define SCAFES_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Geometry of one field on the global (single-partition) grid. */
+typedef struct {
+    int dim;          /* 1..3 */
+    int n[3];         /* nodes per axis; axes >= dim must be 1 */
+    int hw;           /* stencil width (ghost layers) of this field */
+    double x0[3];     /* coordNodeFirst */
+    double x1[3];     /* coordNodeLast */
+    int nlayers;      /* nLayersAtBorder (Parameters.hpp:446) */
+} oracle_grid;
+
+/* Border rule for the unknown field at GLOBAL_BORDER nodes. */
+enum { ORACLE_BORDER_KEEP = 0, ORACLE_BORDER_FIELD = 1, ORACLE_BORDER_VALUE = 2 };
+
+/* Known-field evaluator used inside the restated time loop. */
+enum {
+    ORACLE_KNOWN_NONE = 0,       /* F = G = 0 (sine-mode heat)                 */
+    ORACLE_KNOWN_HEAT_TIMELIN,   /* tests/system/HeatEqnFDM/Dirichlet/TimeLinSpaceLin */
+    ORACLE_KNOWN_PENNES_TIMELIN  /* tests/system/PennesBioheatEqnFDM/Dirichlet/TimeLinSpaceLin */
+};
+
+/* Coefficients of the constant-coefficient heat/Pennes node update
+ * (HeatEqnFDM.hpp:204-218, PennesBioheatEqnFDM.hpp:153-168). */
+typedef struct {
+    double tau;
+    double coeff_a;     /* LAMBDA/(RHO*C) */
+    double rhoc_inv;    /* 1/(RHO*C) multiplying F_old */
+    double perf_c;      /* (RHO_BLOOD*C_BLOOD)/(RHO*C)          (if has_perf) */
+    double perf_w;      /* W                                     (if has_perf) */
+    double rhob_cb_w;   /* RHO_BLOOD*C_BLOOD*W inside the Pennes source F */
+    int has_perf;
+    int has_source;
+} oracle_heat7_coef;
+
+long oracle_nelems(const oracle_grid* g);
+long oracle_nnodes(const oracle_grid* g);
+double oracle_gridsize(const oracle_grid* g, int p);
+
+/* One update step (copy-old-to-new, inner stencil, border rule). */
+void oracle_heat7_step(const oracle_grid* g, const double* u_old, double* u_new,
+                       const double* f_old, const double* g_new, int border_mode,
+                       double border_value, const oracle_heat7_coef* c);
+
+/* Full restated iterateOverTime for the heat/Pennes system problems.
+ * u: memAll of the unknown field; on entry ignored if known==TIMELIN (the
+ * init hook is restated), else the initial state; on exit the final state.
+ * linf (nullable, nsteps+1 entries): "Local Linf" of compErrLinf per step
+ * against the exact solution (TIMELIN modes only).  Returns final time. */
+double oracle_heat7_run(const oracle_grid* g, double* u, int nsteps, double t0,
+                        const oracle_heat7_coef* c, int known, double* linf);
+
+/* Variable-coefficient conservative 7-point heat: one step, FP64.
+ * kappa has the same layout (and stencil width) as u. */
+void oracle_heat7var_step(const oracle_grid* g, const double* u_old, double* u_new,
+                          const double* kappa, double tau, int border_mode,
+                          double border_value);
+void oracle_heat7var_run(const oracle_grid* g, double* u, const double* kappa,
+                         double tau, int nsteps, int border_mode, double border_value,
+                         double* scratch);
+
+/* Pennes bioheat with perfusion field w and source field Q (TumorHeatEqnFDM
+ * form, examples/TumorHeatEqnFDM/TumorHeatEqnFDM.hpp:233-269), Dirichlet
+ * borders kept at their initial values.  w, q: same layout as T. */
+typedef struct {
+    double tau;
+    double cdiff;   /* K/(RHO*C) */
+    double cperf;   /* 1/(RHO*C) (w already carries rho_b*c_b) */
+    double csrc;    /* 1/(RHO*C) */
+    double t_a;
+} oracle_pennes_coef;
+
+void oracle_pennes_step(const oracle_grid* g, const double* t_old, double* t_new,
+                        const double* w, const double* q, const oracle_pennes_coef* c);
+/* states: (nsteps+1) * nelems doubles, states[0] = initial state. */
+void oracle_pennes_forward_all(const oracle_grid* g, double* states, const double* w,
+                               const double* q, const oracle_pennes_coef* c, int nsteps);
+/* J = 0.5*sum_{inner}(T^N - target)^2 and dJ/dw (nelems, zero on non-inner
+ * nodes) by the explicit reverse-time discrete adjoint over stored states. */
+double oracle_pennes_grad(const oracle_grid* g, const double* states, const double* w,
+                          const double* target, const oracle_pennes_coef* c, int nsteps,
+                          double* grad);
+
+/* Yee FDTD, vacuum, normalised (c=1, h=1, Courant s=tau/h), PEC box of
+ * N cells per axis; fields stored on (N+1)^3 nodes each, no ghost layer.
+ * order of f: Ex,Ey,Ez,Hx,Hy,Hz, each (N+1)^3.  One leapfrog step = H
+ * half step (updateInner) then E half step (updateInner2), as the
+ * reference's leapfrog mode (Problem.hpp:2603-2619, EMFDTD.hpp:295-386). */
+void oracle_yee_half_h(int ncell, double s, double* f);
+void oracle_yee_half_e(int ncell, double s, double* f);
+void oracle_yee_step(int ncell, double s, double* f);
+/* Discrete energy 0.5*sum(E^n.E^n) + 0.5*sum(H^{n-1/2}.H^{n+1/2}); f holds
+ * E^n and H^{n+1/2} (state right after an H half step), h_prev the three
+ * H components before it.  Invariant of the scheme in exact arithmetic. */
+double oracle_yee_energy(int ncell, const double* f, const double* h_prev);
+
+/* Batched-ensemble member: 2-D heat, scalar kappa, zero Dirichlet border,
+ * n x n nodes (no ghost layer), u_new = u + tau*kappa*lap(u) on inner nodes
+ * (lap with 1/h^2, per-axis accumulation as HeatEqnFDM.hpp:209-215). */
+void oracle_heat2d_run(int n, double h, double kappa, double tau, int nsteps,
+                       double* u, double* u_prev);
+/* J = 0.5*sum (u^N - target)^2, dJ/dkappa by explicit reverse sweep. */
+double oracle_heat2d_grad(int n, double h, double kappa, double tau, int nsteps,
+                          const double* u0, const double* target, double* dj_dkappa);
+
+/* Linf of (a - b) over the nodes of a field (compErrLinf, DataField.hpp:1439-1530). */
+double oracle_linf_nodes(const oracle_grid* g, const double* a, const double* b);
+
+/* Threads used by the OpenMP loops (0 = OpenMP default). */
+void oracle_set_threads(int n);
+int oracle_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

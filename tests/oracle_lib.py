@@ -1,0 +1,213 @@
+"""ctypes binding of the CPU oracle (oracle/build/liboracle.so).
+
+Test infrastructure only: the oracle is the checker for the HIP path and
+is never imported by the masterthesis_amd package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "build", "liboracle.so")
+
+BORDER_KEEP, BORDER_FIELD, BORDER_VALUE = 0, 1, 2
+KNOWN_NONE, KNOWN_HEAT_TIMELIN, KNOWN_PENNES_TIMELIN = 0, 1, 2
+
+
+class Grid(C.Structure):
+    _fields_ = [("dim", C.c_int), ("n", C.c_int * 3), ("hw", C.c_int),
+                ("x0", C.c_double * 3), ("x1", C.c_double * 3), ("nlayers", C.c_int)]
+
+
+class Heat7Coef(C.Structure):
+    _fields_ = [("tau", C.c_double), ("coeff_a", C.c_double), ("rhoc_inv", C.c_double),
+                ("perf_c", C.c_double), ("perf_w", C.c_double), ("rhob_cb_w", C.c_double),
+                ("has_perf", C.c_int), ("has_source", C.c_int)]
+
+
+class PennesCoef(C.Structure):
+    _fields_ = [("tau", C.c_double), ("cdiff", C.c_double), ("cperf", C.c_double),
+                ("csrc", C.c_double), ("t_a", C.c_double)]
+
+
+def make_grid(n, hw=1, x0=None, x1=None, nlayers=1):
+    n = list(n)
+    dim = len(n)
+    nn = n + [1] * (3 - dim)
+    g = Grid()
+    g.dim = dim
+    for p in range(3):
+        g.n[p] = nn[p]
+        g.x0[p] = 0.0 if x0 is None or p >= dim else x0[p]
+        g.x1[p] = 1.0 if x1 is None or p >= dim else x1[p]
+    g.hw = hw
+    g.nlayers = nlayers
+    return g
+
+
+def memall_shape(g):
+    """numpy shape (z, y, x) of a memAll field (Fortran order = C order reversed)."""
+    e = [g.n[p] + 2 * g.hw if p < g.dim else 1 for p in range(3)]
+    return (e[2], e[1], e[0])
+
+
+def nodes_view(g, a):
+    """View of the node block (without ghosts) of a memAll array shaped (z,y,x)."""
+    h = g.hw
+    sl = [slice(h, h + g.n[p]) if p < g.dim else slice(0, 1) for p in range(3)]
+    return a[sl[2], sl[1], sl[0]]
+
+
+def _build():
+    subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            _build()
+        L = C.CDLL(LIB_PATH)
+        dp = C.POINTER(C.c_double)
+        gp = C.POINTER(Grid)
+        L.oracle_nelems.restype = C.c_long
+        L.oracle_nelems.argtypes = [gp]
+        L.oracle_gridsize.restype = C.c_double
+        L.oracle_gridsize.argtypes = [gp, C.c_int]
+        L.oracle_heat7_step.argtypes = [gp, dp, dp, dp, dp, C.c_int, C.c_double, C.POINTER(Heat7Coef)]
+        L.oracle_heat7_run.restype = C.c_double
+        L.oracle_heat7_run.argtypes = [gp, dp, C.c_int, C.c_double, C.POINTER(Heat7Coef), C.c_int, dp]
+        L.oracle_heat7var_step.argtypes = [gp, dp, dp, dp, C.c_double, C.c_int, C.c_double]
+        L.oracle_heat7var_run.argtypes = [gp, dp, dp, C.c_double, C.c_int, C.c_int, C.c_double, dp]
+        L.oracle_pennes_step.argtypes = [gp, dp, dp, dp, dp, C.POINTER(PennesCoef)]
+        L.oracle_pennes_forward_all.argtypes = [gp, dp, dp, dp, C.POINTER(PennesCoef), C.c_int]
+        L.oracle_pennes_grad.restype = C.c_double
+        L.oracle_pennes_grad.argtypes = [gp, dp, dp, dp, C.POINTER(PennesCoef), C.c_int, dp]
+        L.oracle_yee_half_h.argtypes = [C.c_int, C.c_double, dp]
+        L.oracle_yee_half_e.argtypes = [C.c_int, C.c_double, dp]
+        L.oracle_yee_step.argtypes = [C.c_int, C.c_double, dp]
+        L.oracle_yee_energy.restype = C.c_double
+        L.oracle_yee_energy.argtypes = [C.c_int, dp, dp]
+        L.oracle_heat2d_run.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp]
+        L.oracle_heat2d_grad.restype = C.c_double
+        L.oracle_heat2d_grad.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                         dp, dp, dp]
+        L.oracle_linf_nodes.restype = C.c_double
+        L.oracle_linf_nodes.argtypes = [gp, dp, dp]
+        L.oracle_set_threads.argtypes = [C.c_int]
+        L.oracle_get_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# ---------------------------------------------------------------------------
+# Convenience wrappers
+
+
+def heat7_coef(tau, coeff_a=1.0, rhoc_inv=1.0, perf_c=0.0, perf_w=0.0, rhob_cb_w=0.0,
+               has_perf=False, has_source=False):
+    c = Heat7Coef()
+    c.tau, c.coeff_a, c.rhoc_inv = tau, coeff_a, rhoc_inv
+    c.perf_c, c.perf_w, c.rhob_cb_w = perf_c, perf_w, rhob_cb_w
+    c.has_perf, c.has_source = int(has_perf), int(has_source)
+    return c
+
+
+def heat7_run(g, u, nsteps, t0, coef, known):
+    u = np.ascontiguousarray(u, dtype=np.float64).copy()
+    linf = np.zeros(nsteps + 1)
+    t = lib().oracle_heat7_run(C.byref(g), ptr(u), nsteps, t0, C.byref(coef), known, ptr(linf))
+    return u, linf, t
+
+
+def heat7_step(g, uo, f_old=None, g_new=None, border=BORDER_VALUE, border_value=0.0, coef=None):
+    un = np.empty_like(uo)
+    lib().oracle_heat7_step(C.byref(g), ptr(uo), ptr(un), ptr(f_old), ptr(g_new), border,
+                            border_value, C.byref(coef))
+    return un
+
+
+def heat7var_step(g, uo, kappa, tau, border=BORDER_VALUE, border_value=0.0):
+    un = np.empty_like(uo)
+    lib().oracle_heat7var_step(C.byref(g), ptr(uo), ptr(un), ptr(kappa), tau, border, border_value)
+    return un
+
+
+def heat7var_run(g, u, kappa, tau, nsteps, border=BORDER_VALUE, border_value=0.0):
+    u = np.ascontiguousarray(u, dtype=np.float64).copy()
+    scratch = np.empty_like(u)
+    lib().oracle_heat7var_run(C.byref(g), ptr(u), ptr(kappa), tau, nsteps, border,
+                              border_value, ptr(scratch))
+    return u
+
+
+def pennes_coef(tau, cdiff, cperf, csrc, t_a):
+    c = PennesCoef()
+    c.tau, c.cdiff, c.cperf, c.csrc, c.t_a = tau, cdiff, cperf, csrc, t_a
+    return c
+
+
+def pennes_forward_all(g, t0, w, q, coef, nsteps):
+    ne = t0.size
+    st = np.empty((nsteps + 1,) + t0.shape)
+    st[0] = t0
+    lib().oracle_pennes_forward_all(C.byref(g), ptr(st), ptr(w), ptr(q), C.byref(coef), nsteps)
+    assert st[0].size == ne
+    return st
+
+
+def pennes_grad(g, states, w, target, coef, nsteps):
+    grad = np.zeros_like(w)
+    J = lib().oracle_pennes_grad(C.byref(g), ptr(states), ptr(w), ptr(target), C.byref(coef),
+                                 nsteps, ptr(grad))
+    return J, grad
+
+
+def yee_alloc(ncell):
+    m = ncell + 1
+    return np.zeros((6, m, m, m))
+
+
+def yee_step(ncell, s, f, nsteps=1):
+    for _ in range(nsteps):
+        lib().oracle_yee_step(ncell, s, ptr(f))
+    return f
+
+
+def yee_half_h(ncell, s, f):
+    lib().oracle_yee_half_h(ncell, s, ptr(f))
+
+
+def yee_half_e(ncell, s, f):
+    lib().oracle_yee_half_e(ncell, s, ptr(f))
+
+
+def yee_energy(ncell, f, h_prev):
+    return lib().oracle_yee_energy(ncell, ptr(f), ptr(np.ascontiguousarray(h_prev)))
+
+
+def heat2d_run(n, kappa, tau, nsteps, u0):
+    u = np.ascontiguousarray(u0, dtype=np.float64).copy()
+    up = np.empty_like(u)
+    lib().oracle_heat2d_run(n, 1.0 / (n - 1), kappa, tau, nsteps, ptr(u), ptr(up))
+    return u, up
+
+
+def heat2d_grad(n, kappa, tau, nsteps, u0, target):
+    d = C.c_double(0.0)
+    J = lib().oracle_heat2d_grad(n, 1.0 / (n - 1), kappa, tau, nsteps,
+                                 ptr(np.ascontiguousarray(u0, dtype=np.float64)),
+                                 ptr(np.ascontiguousarray(target, dtype=np.float64)), C.byref(d))
+    return J, d.value
