@@ -1,0 +1,207 @@
+/*
+ * scafes_hip.h -- C ABI of the MI355X-native explicit time-stepping engine.
+ *
+ * The reference (ScaFES 2.4.0) has no C ABI: its hot path is the C++ CRTP
+ * engine Problem<OWNPRBLM,CT,DIM> (src/ScaFES_Problem.hpp:264) whose
+ * OpenMP node loops call the user hooks.  Its only device code is an empty
+ * CUDA stub (src/ScaFES_ProblemCuda.cu:5-37, host wrapper
+ * src/ScaFES_ProblemCuda.hpp:84-105).  This header is the slot that stub
+ * was meant to fill: plain pointers and sizes, int status codes, no C++
+ * types, no exceptions.  The C++11 host API (masterthesis_amd/host/ScaFES)
+ * and the Python harness (ctypes) call it.
+ *
+ * Conventions
+ *  - Every function returns SCAFES_OK (0) or a negative code; the text of
+ *    the last error of the calling thread is scafes_last_error().
+ *  - Device pointers come from scafes_malloc (or any HIP allocation on the
+ *    context's device).  All work is ordered on the context's stream;
+ *    functions returning host values synchronise that stream.
+ *  - One host thread per context.
+ *  - Field layout: ScaFES DataField memAll (src/ScaFES_DataField.hpp:622-642,
+ *    src/ScaFES_Grid.hpp:955-968): Fortran order (x fastest), extents
+ *    n_p + 2*hw on active axes.  The reference's trailing time slot
+ *    (DataField.hpp:947-953) is kept by the host, not in device arrays.
+ */
+#ifndef SCAFES_HIP_H_
+#define SCAFES_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCAFES_OK 0
+#define SCAFES_ERR_ARG (-1)
+#define SCAFES_ERR_HIP (-2)
+#define SCAFES_ERR_ALLOC (-3)
+#define SCAFES_ERR_UNSUPPORTED (-4)
+
+typedef struct scafes_ctx scafes_ctx;
+
+enum { SCAFES_F32 = 0, SCAFES_F64 = 1 };
+
+/* Border rule at GLOBAL_BORDER nodes (GridGlobal.hpp:138-151 flags):
+ * KEEP  = the hook leaves the copied old value (Problem.hpp:6226-6256),
+ * FIELD = Dirichlet from a known field G_new (Dirichlet.hpp:124-130),
+ * VALUE = Dirichlet from a constant. */
+enum { SCAFES_BORDER_KEEP = 0, SCAFES_BORDER_FIELD = 1, SCAFES_BORDER_VALUE = 2 };
+
+/* Discrete domain of one partition covering the whole grid
+ * (GridGlobal::discreteDomain(), GridGlobal.hpp:384-422). */
+typedef struct {
+    int dim;       /* 1..3 */
+    int n[3];      /* nodes per axis; 1 on axes >= dim */
+    int nlayers;   /* nLayersAtBorder (Parameters.hpp:446, default 1) */
+    double h[3];   /* gridsize (Grid.hpp:547-555) */
+} scafes_geom;
+
+/* ---- context, errors, memory (Problem owns its lumps: Problem.hpp:1215-1275) */
+const char* scafes_last_error(void);
+int scafes_device_count(int* n);
+int scafes_ctx_create(int device, scafes_ctx** out);
+int scafes_ctx_destroy(scafes_ctx* ctx);
+int scafes_ctx_sync(scafes_ctx* ctx);
+int scafes_ctx_info(scafes_ctx* ctx, int* cu_count, char* name, int name_len);
+/* Device-side wall clock on the context's stream (ScaFES::Timer restart/
+ * elapsed, src/ScaFES_Timer.hpp:171-214, for GPU work): record event slot
+ * 0..7, then read the elapsed milliseconds between two recorded slots. */
+int scafes_event_record(scafes_ctx* ctx, int slot);
+int scafes_event_elapsed(scafes_ctx* ctx, int slot_start, int slot_end, float* ms);
+int scafes_malloc(scafes_ctx* ctx, size_t bytes, void** dptr);
+int scafes_free(scafes_ctx* ctx, void* dptr);
+int scafes_memcpy_h2d(scafes_ctx* ctx, void* dst, const void* src, size_t bytes);
+int scafes_memcpy_d2h(scafes_ctx* ctx, void* dst, const void* src, size_t bytes);
+int scafes_memcpy_d2d(scafes_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Fill count elements of dtype with value (DataField::operator=(CT),
+ * DataField.hpp:1137-1145); a kernel, never a memset node. */
+int scafes_fill(scafes_ctx* ctx, void* dptr, int dtype, size_t count, double value);
+/* Convert between f64 and f32 device arrays (count elements). */
+int scafes_convert(scafes_ctx* ctx, void* dst, int dst_dtype, const void* src, int src_dtype,
+                   size_t count);
+
+/* ---- constant-coefficient heat / Pennes system update ---------------------
+ * Replaces the update phase evalTypeTwoAt (Problem.hpp:5397-5476) for the
+ * hooks HeatEqnFDM::updateInner (tests/system/HeatEqnFDM/HeatEqnFDM.hpp:204-218),
+ * PennesBioheatEqnFDM::updateInner (PennesBioheatEqnFDM.hpp:153-168) and
+ * Dirichlet::updateBorder (Dirichlet.hpp:124-130), FP64 in the reference's
+ * operation order (bit-identical to the CPU path).
+ *   inner:  u' = u + sum_p (tau*a*(W+E-2u))/h_p^2 [- ((tau*perf_c)*perf_w)*u]
+ *                  [+ (tau*rhoc_inv)*F_old]
+ * u_old/u_new: memAll with hw ghost layers; f_old/g_new: stencil-0 fields
+ * (NULL when unused).  Ghost layers of u_new are not written. */
+typedef struct {
+    double tau, coeff_a, rhoc_inv, perf_c, perf_w;
+    int has_perf, has_source;
+    int border_mode;
+    double border_value;
+} scafes_heat7_params;
+int scafes_heat7_step_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const double* u_old,
+                          double* u_new, const double* f_old, const double* g_new,
+                          const scafes_heat7_params* p);
+
+/* ---- variable-coefficient conservative 7-point heat -----------------------
+ *   inner: u' = u + sum_p tau*(0.5(kE+kC)(uE-uC) - 0.5(kC+kW)(uC-uW))/h_p^2
+ * (node-centred hook structure of HeatEqnFDM.hpp:209-215 with a known
+ * coefficient field as in examples/MRIDataPBHEqnFDM/MRIDataPBHEqnFDM.hpp:143-168).
+ * mode: SCAFES_HEATVAR_F64 (bit-identical to the FP64 CPU path),
+ *       SCAFES_HEATVAR_F32 (plain FP32 stream),
+ *       SCAFES_HEATVAR_F32_KAHAN (FP32 stream with a persistent FP32
+ *       compensation field c: u_true ~ u + c).
+ * Runs nsteps steps ping-ponging between (u, u2) and (c, c2); on return
+ * *result_in_second = 1 when the final state is in u2/c2.  kappa has the
+ * layout of u.  cfg: tile configuration index (-1 = autotuned/default). */
+enum { SCAFES_HEATVAR_F64 = 0, SCAFES_HEATVAR_F32 = 1, SCAFES_HEATVAR_F32_KAHAN = 2 };
+int scafes_heat7var_run(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode, void* u,
+                        void* u2, void* c, void* c2, const void* kappa, double tau, int nsteps,
+                        int border_mode, double border_value, int cfg, int* result_in_second);
+/* Number of tile configurations and a description of one. */
+int scafes_heat7var_num_cfgs(void);
+int scafes_heat7var_cfg_desc(int cfg, int* tx, int* ty, int* zchunk);
+/* Project autotuner: times every configuration on the given buffers for
+ * nsteps steps and stores the fastest as the default for this mode. */
+int scafes_autotune_heat7var(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode, void* u,
+                             void* u2, void* c, void* c2, const void* kappa, double tau,
+                             int nsteps, int* best_cfg, float* best_ms);
+/* Synthetic initial state of the 512^3 benchmark workload:
+ * u = prod sin(pi x_p) + amp*(2*U-1), U = splitmix64(seed, node) / 2^53 in
+ * [0,1) on inner nodes (border nodes: the sine product), and
+ * kappa = 1 + 0.5 sin(2 pi x) sin(2 pi y).  dtype of u and kappa. */
+int scafes_heat7var_init(scafes_ctx* ctx, const scafes_geom* g, int hw, int dtype, void* u,
+                         void* kappa, uint64_t seed, double amp);
+
+/* ---- reductions (DataField::compErrLinf DataField.hpp:1439-1530 and
+ *      DataField::checkConv DataField.hpp:1532-1636) ---------------------
+ * a has hw_a ghost layers, b has hw_b (0 for a stencil-0 known field).
+ * op: 0 = max|a-b| over nodes, 1 = sum (a-b)^2 over nodes, 2 = sum a*b.
+ * inner_only: restrict to GLOBAL_INNER nodes.  Result in FP64 on host. */
+int scafes_reduce(scafes_ctx* ctx, const scafes_geom* g, int dtype, const void* a, int hw_a,
+                  const void* b, int hw_b, int op, int inner_only, double* out);
+
+/* ---- Pennes bioheat with perfusion / source fields ---------------------
+ * TumorHeatEqnFDM::updateInner form (examples/TumorHeatEqnFDM/TumorHeatEqnFDM.hpp:233-269)
+ * with omega, Q as node fields; border nodes keep their value:
+ *   T' = T + sum_p (tau*cdiff*(W+E-2T))/h_p^2 + ((tau*cperf)*w)*(T_a-T) + (tau*csrc)*Q.
+ * dtype F64: bit-identical to the FP64 CPU path; F32: FP32 stream. */
+typedef struct {
+    double tau, cdiff, cperf, csrc, t_a;
+} scafes_pennes_params;
+int scafes_pennes_step(scafes_ctx* ctx, const scafes_geom* g, int hw, int dtype,
+                       const void* t_old, void* t_new, const void* w, const void* q,
+                       const scafes_pennes_params* p);
+/* Discrete adjoint step (replaces ADOL-C fov_forward, Problem.hpp:6632-6747):
+ *   lam'_i = lam_i + sum_p tau*cdiff*(lam_W+lam_E-2lam_i)/h_p^2 - tau*cperf*w_i*lam_i
+ *   grad_i += lam_i * tau*cperf*(T_a - t_prev_i)          (inner nodes; FP64 grad)
+ * lam' = 0 on border nodes. */
+int scafes_pennes_adjoint_step(scafes_ctx* ctx, const scafes_geom* g, int hw, int dtype,
+                               const void* lam, void* lam_new, const void* w,
+                               const void* t_prev, double* grad, const scafes_pennes_params* p);
+/* Gradient of J = 0.5 sum_inner (T^N - target)^2 w.r.t. the perfusion field
+ * w by the reverse-time adjoint with binomial checkpointing (Griewank's
+ * treeverse schedule) using at most max_snapshots stored states.
+ * t0 is read only; grad (FP64, layout of T) is overwritten.  stats (nullable,
+ * 4 entries): forward steps executed, adjoint steps, snapshots used,
+ * peak checkpoint bytes. */
+int scafes_pennes_gradient(scafes_ctx* ctx, const scafes_geom* g, int hw, int dtype,
+                           const void* t0, const void* w, const void* q, const void* target,
+                           const scafes_pennes_params* p, int nsteps, int max_snapshots,
+                           double* grad, double* J, double* stats);
+/* Number of forward steps the treeverse schedule runs for (nsteps, snaps). */
+long scafes_checkpoint_forward_steps(int nsteps, int snaps);
+
+/* ---- Yee FDTD, PEC box, vacuum, FP32 ----------------------------------
+ * Leapfrog of the reference's two-phase mode (updateInner then
+ * updateInner2, Problem.hpp:2603-2619; curl stencil of
+ * examples/EMFDTD/stencils/CompleteCPMLCurlStencil.hpp:38-80 without CPML).
+ * f: six FP32 fields Ex,Ey,Ez,Hx,Hy,Hz, each (ncell+1)^3 nodes, x fastest.
+ * energy (nullable, FP64 host array of nsteps/energy_every entries): discrete
+ * energy 0.5 sum E^n.E^n + 0.5 sum H^{n-1/2}.H^{n+1/2}, accumulated in
+ * FP64 inside the H update of every energy_every-th step.
+ * h_prev: scratch of 3 (ncell+1)^3 FP32 fields (needed only with energy). */
+int scafes_yee_run(scafes_ctx* ctx, int ncell, double s, float* f, int nsteps, int fused,
+                   int energy_every, double* energy);
+int scafes_yee_init_pulse(scafes_ctx* ctx, int ncell, float* f, double sigma);
+
+/* ---- batched ensemble of 2-D heat problems -----------------------------
+ * nprob independent n_i x n_i problems (Dirichlet-zero border), scalar
+ * kappa_i and tau_i, nsteps explicit steps each, FP32, in ONE persistent
+ * kernel launch (work queue over problems sorted by size).  Problem i's
+ * state lives at u + off[i] (n_i^2 floats).  With target != NULL also
+ * dJ_i/dkappa_i of J_i = 0.5 sum_inner (u_i^N - target_i)^2 in FP64 via
+ * the commuting-operator adjoint identity dJ/dkappa = N tau <r, L u^{N-1}>.
+ * n, off, kappa, tau, order: device arrays (order = processing order).
+ * J, grad: device FP64 arrays of nprob (nullable when target == NULL).
+ * scratch: device FP32 array of size sum n_i^2 for problems too large for
+ * LDS.  threads: workgroup size (256..1024).  graph != 0 records the launch
+ * into a hipGraph once per context and replays it. */
+int scafes_batch2d_run(scafes_ctx* ctx, int nprob, const int* n, const long* off,
+                       const float* kappa, const float* tau, const int* order, float* u,
+                       float* scratch, const float* target, double* J, double* grad,
+                       int nsteps, int threads, int use_graph);
+int scafes_batch2d_lds_limit(int* max_n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,351 @@
+// scafes_capi.hip -- context, memory, errors and FP64 reductions of the C ABI.
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "scafes_common.hpp"
+
+namespace scafes {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg)
+{
+    set_error(msg);
+    return code;
+}
+
+int check_hip(hipError_t e, const char* what)
+{
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    set_error(m);
+    return e == hipErrorOutOfMemory ? SCAFES_ERR_ALLOC : SCAFES_ERR_HIP;
+}
+
+Ctx* as_ctx(scafes_ctx* c) { return reinterpret_cast<Ctx*>(c); }
+
+int make_geo(const scafes_geom* g, int hw, Geo* out)
+{
+    if (!g || g->dim < 1 || g->dim > 3) return fail(SCAFES_ERR_ARG, "geom: dim must be 1..3");
+    if (hw < 0) return fail(SCAFES_ERR_ARG, "geom: negative ghost width");
+    for (int p = 0; p < 3; ++p) {
+        if (g->n[p] < 1) return fail(SCAFES_ERR_ARG, "geom: n must be >= 1");
+        if (p >= g->dim && g->n[p] != 1) return fail(SCAFES_ERR_ARG, "geom: inactive axis must have n=1");
+    }
+    Geo r;
+    r.nx = g->n[0];
+    r.ny = g->n[1];
+    r.nz = g->n[2];
+    r.dim = g->dim;
+    r.nl0 = g->nlayers;
+    r.nl1 = g->dim > 1 ? g->nlayers : 0;
+    r.nl2 = g->dim > 2 ? g->nlayers : 0;
+    const long h0 = hw, h1 = g->dim > 1 ? hw : 0, h2 = g->dim > 2 ? hw : 0;
+    const long e0 = g->n[0] + 2 * h0, e1 = g->n[1] + 2 * h1;
+    r.e0 = e0;
+    r.e01 = e0 * e1;
+    r.base = h0 + e0 * (h1 + e1 * h2);
+    *out = r;
+    return SCAFES_OK;
+}
+
+// ------------------------------------------------------------------ kernels
+template <typename T>
+__global__ void fill_kernel(T* p, size_t n, T v)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+template <typename D, typename S>
+__global__ void convert_kernel(D* d, const S* s, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        d[i] = (D)s[i];
+}
+
+// Per-block FP64 partials of a node reduction; each thread walks x-rows.
+template <typename T>
+__global__ __launch_bounds__(256) void reduce_kernel(Geo ga, Geo gb, const T* __restrict__ a,
+                                                     const T* __restrict__ b, int op, int inner_only,
+                                                     double* partial)
+{
+    __shared__ double sh[256];
+    double acc = 0.0;
+    const long nrows = (long)ga.ny * ga.nz;
+    const long total = nrows * ga.nx;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % ga.nx);
+        const long r = t / ga.nx;
+        const int j = (int)(r % ga.ny);
+        const int k = (int)(r / ga.ny);
+        if (inner_only && is_border(ga, i, j, k)) continue;
+        const double x = (double)a[at(ga, i, j, k)];
+        const double y = b ? (double)b[at(gb, i, j, k)] : 0.0;
+        if (op == RED_MAX_ABS_DIFF) {
+            const double d = fabs(x - y);
+            acc = d > acc ? d : acc;
+        } else if (op == RED_SUM_SQ_DIFF) {
+            const double d = x - y;
+            acc += d * d;
+        } else {
+            acc += x * y;
+        }
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            if (op == RED_MAX_ABS_DIFF) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+            else sh[threadIdx.x] += sh[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void reduce_final(const double* partial, int n, int op, double* out)
+{
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        if (op == RED_MAX_ABS_DIFF) acc = fmax(acc, partial[i]);
+        else acc += partial[i];
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            if (op == RED_MAX_ABS_DIFF) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+            else sh[threadIdx.x] += sh[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+int reduce_nodes(Ctx* c, const Geo& ga, const Geo& gb, int dtype, const void* a, const void* b,
+                 int op, int inner_only, double* out)
+{
+    const long total = (long)ga.nx * ga.ny * ga.nz;
+    int blocks = ceil_div(total, 256 * 8);
+    if (blocks > c->red_capacity) blocks = c->red_capacity;
+    if (blocks < 1) blocks = 1;
+    if (dtype == SCAFES_F64)
+        reduce_kernel<double><<<blocks, 256, 0, c->stream>>>(ga, gb, (const double*)a,
+                                                             (const double*)b, op, inner_only,
+                                                             c->red_partial);
+    else
+        reduce_kernel<float><<<blocks, 256, 0, c->stream>>>(ga, gb, (const float*)a,
+                                                            (const float*)b, op, inner_only,
+                                                            c->red_partial);
+    SCAFES_HIP(hipGetLastError());
+    reduce_final<<<1, 256, 0, c->stream>>>(c->red_partial, blocks, op, c->red_result);
+    SCAFES_HIP(hipGetLastError());
+    SCAFES_HIP(hipMemcpyAsync(c->host_result, c->red_result, sizeof(double),
+                              hipMemcpyDeviceToHost, c->stream));
+    SCAFES_HIP(hipStreamSynchronize(c->stream));
+    *out = c->host_result[0];
+    return SCAFES_OK;
+}
+
+}  // namespace scafes
+
+using namespace scafes;
+
+extern "C" {
+
+const char* scafes_last_error(void) { return g_last_error.c_str(); }
+
+int scafes_device_count(int* n)
+{
+    if (!n) return fail(SCAFES_ERR_ARG, "null pointer");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) {
+        *n = 0;
+        return check_hip(e, "hipGetDeviceCount");
+    }
+    *n = c;
+    return SCAFES_OK;
+}
+
+int scafes_ctx_create(int device, scafes_ctx** out)
+{
+    if (!out) return fail(SCAFES_ERR_ARG, "null pointer");
+    *out = nullptr;
+    int ndev = 0;
+    SCAFES_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(SCAFES_ERR_ARG, "device index out of range");
+    SCAFES_HIP(hipSetDevice(device));
+    Ctx* c = new Ctx();
+    c->device = device;
+    for (int i = 0; i < 16; ++i) c->tuned[i] = -1;
+    hipDeviceProp_t prop;
+    SCAFES_HIP(hipGetDeviceProperties(&prop, device));
+    c->cu_count = prop.multiProcessorCount;
+    SCAFES_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->red_capacity = 4096;
+    SCAFES_HIP(hipMalloc(&c->red_partial, sizeof(double) * c->red_capacity));
+    SCAFES_HIP(hipMalloc(&c->red_result, sizeof(double) * 8));
+    SCAFES_HIP(hipHostMalloc(&c->host_result, sizeof(double) * 8, hipHostMallocDefault));
+    for (int i = 0; i < 8; ++i) SCAFES_HIP(hipEventCreate(&c->events[i]));
+    *out = reinterpret_cast<scafes_ctx*>(c);
+    return SCAFES_OK;
+}
+
+int scafes_event_record(scafes_ctx* ctx, int slot)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || slot < 0 || slot >= 8) return fail(SCAFES_ERR_ARG, "event_record: bad argument");
+    SCAFES_HIP(hipEventRecord(c->events[slot], c->stream));
+    return SCAFES_OK;
+}
+
+int scafes_event_elapsed(scafes_ctx* ctx, int a, int b, float* ms)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || !ms || a < 0 || a >= 8 || b < 0 || b >= 8)
+        return fail(SCAFES_ERR_ARG, "event_elapsed: bad argument");
+    SCAFES_HIP(hipEventSynchronize(c->events[b]));
+    SCAFES_HIP(hipEventElapsedTime(ms, c->events[a], c->events[b]));
+    return SCAFES_OK;
+}
+
+int scafes_ctx_destroy(scafes_ctx* ctx)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c) return SCAFES_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int i = 0; i < 8; ++i)
+        if (c->events[i]) (void)hipEventDestroy(c->events[i]);
+    if (c->red_partial) (void)hipFree(c->red_partial);
+    if (c->red_result) (void)hipFree(c->red_result);
+    if (c->host_result) (void)hipHostFree(c->host_result);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SCAFES_OK;
+}
+
+int scafes_ctx_sync(scafes_ctx* ctx)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c) return fail(SCAFES_ERR_ARG, "null context");
+    SCAFES_HIP(hipStreamSynchronize(c->stream));
+    return SCAFES_OK;
+}
+
+int scafes_ctx_info(scafes_ctx* ctx, int* cu_count, char* name, int name_len)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c) return fail(SCAFES_ERR_ARG, "null context");
+    if (cu_count) *cu_count = c->cu_count;
+    if (name && name_len > 0) {
+        hipDeviceProp_t prop;
+        SCAFES_HIP(hipGetDeviceProperties(&prop, c->device));
+        std::snprintf(name, (size_t)name_len, "%s", prop.gcnArchName);
+    }
+    return SCAFES_OK;
+}
+
+int scafes_malloc(scafes_ctx* ctx, size_t bytes, void** dptr)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || !dptr) return fail(SCAFES_ERR_ARG, "null pointer");
+    *dptr = nullptr;
+    SCAFES_HIP(hipSetDevice(c->device));
+    SCAFES_HIP(hipMalloc(dptr, bytes ? bytes : 1));
+    return SCAFES_OK;
+}
+
+int scafes_free(scafes_ctx* ctx, void* dptr)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c) return fail(SCAFES_ERR_ARG, "null context");
+    if (!dptr) return SCAFES_OK;
+    SCAFES_HIP(hipStreamSynchronize(c->stream));
+    SCAFES_HIP(hipFree(dptr));
+    return SCAFES_OK;
+}
+
+static int copy(Ctx* c, void* dst, const void* src, size_t bytes, hipMemcpyKind kind)
+{
+    if (!c || (!dst && bytes) || (!src && bytes)) return fail(SCAFES_ERR_ARG, "null pointer");
+    if (!bytes) return SCAFES_OK;
+    SCAFES_HIP(hipMemcpyAsync(dst, src, bytes, kind, c->stream));
+    if (kind != hipMemcpyDeviceToDevice) SCAFES_HIP(hipStreamSynchronize(c->stream));
+    return SCAFES_OK;
+}
+
+int scafes_memcpy_h2d(scafes_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    return copy(as_ctx(ctx), dst, src, bytes, hipMemcpyHostToDevice);
+}
+
+int scafes_memcpy_d2h(scafes_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    return copy(as_ctx(ctx), dst, src, bytes, hipMemcpyDeviceToHost);
+}
+
+int scafes_memcpy_d2d(scafes_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    return copy(as_ctx(ctx), dst, src, bytes, hipMemcpyDeviceToDevice);
+}
+
+int scafes_fill(scafes_ctx* ctx, void* dptr, int dtype, size_t count, double value)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || (!dptr && count)) return fail(SCAFES_ERR_ARG, "null pointer");
+    if (!count) return SCAFES_OK;
+    const int blocks = (int)std::min<size_t>((count + 255) / 256, 65536);
+    if (dtype == SCAFES_F64) fill_kernel<double><<<blocks, 256, 0, c->stream>>>((double*)dptr, count, value);
+    else if (dtype == SCAFES_F32) fill_kernel<float><<<blocks, 256, 0, c->stream>>>((float*)dptr, count, (float)value);
+    else return fail(SCAFES_ERR_ARG, "fill: bad dtype");
+    SCAFES_HIP(hipGetLastError());
+    return SCAFES_OK;
+}
+
+int scafes_convert(scafes_ctx* ctx, void* dst, int dst_dtype, const void* src, int src_dtype,
+                   size_t count)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || ((!dst || !src) && count)) return fail(SCAFES_ERR_ARG, "null pointer");
+    if (!count) return SCAFES_OK;
+    const int blocks = (int)std::min<size_t>((count + 255) / 256, 65536);
+    if (dst_dtype == SCAFES_F32 && src_dtype == SCAFES_F64)
+        convert_kernel<float, double><<<blocks, 256, 0, c->stream>>>((float*)dst, (const double*)src, count);
+    else if (dst_dtype == SCAFES_F64 && src_dtype == SCAFES_F32)
+        convert_kernel<double, float><<<blocks, 256, 0, c->stream>>>((double*)dst, (const float*)src, count);
+    else if (dst_dtype == src_dtype)
+        return scafes_memcpy_d2d(ctx, dst, src, count * (dst_dtype == SCAFES_F64 ? 8 : 4));
+    else
+        return fail(SCAFES_ERR_ARG, "convert: bad dtype");
+    SCAFES_HIP(hipGetLastError());
+    return SCAFES_OK;
+}
+
+int scafes_reduce(scafes_ctx* ctx, const scafes_geom* g, int dtype, const void* a, int hw_a,
+                  const void* b, int hw_b, int op, int inner_only, double* out)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || !a || !out) return fail(SCAFES_ERR_ARG, "reduce: null pointer");
+    if (op < 0 || op > 2) return fail(SCAFES_ERR_ARG, "reduce: bad op");
+    if (op != RED_SUM_PROD && !b && op != RED_SUM_SQ_DIFF && op != RED_MAX_ABS_DIFF)
+        return fail(SCAFES_ERR_ARG, "reduce: b required");
+    if (op == RED_SUM_PROD && !b) return fail(SCAFES_ERR_ARG, "reduce: b required for dot");
+    if (dtype != SCAFES_F32 && dtype != SCAFES_F64) return fail(SCAFES_ERR_ARG, "reduce: bad dtype");
+    Geo ga, gb;
+    int rc = make_geo(g, hw_a, &ga);
+    if (rc) return rc;
+    rc = make_geo(g, hw_b, &gb);
+    if (rc) return rc;
+    return reduce_nodes(c, ga, gb, dtype, a, b, op, inner_only, out);
+}
+
+}  // extern "C"

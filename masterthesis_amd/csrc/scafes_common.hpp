@@ -1,0 +1,93 @@
+// scafes_common.hpp -- shared device/host helpers of the HIP stencil library.
+//
+// Field layout (ScaFES DataField, src/ScaFES_DataField.hpp:622-642 and
+// src/ScaFES_Grid.hpp:955-968): Fortran order, x fastest, memAll extents
+// n_p + 2*hw on active axes.  Kernels address node (i,j,k) as
+//   base + i + e0*j + e01*k,   base = hw + e0*(hw + e1*hw)  (active axes).
+// Stencil-0 fields (known dfs) use the same formula with hw = 0.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/scafes_hip.h"
+
+namespace scafes {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int check_hip(hipError_t e, const char* what);
+
+#define SCAFES_HIP(call)                                        \
+    do {                                                        \
+        hipError_t e_ = (call);                                 \
+        if (e_ != hipSuccess) return ::scafes::check_hip(e_, #call); \
+    } while (0)
+
+// ---------------------------------------------------------------- context
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double* red_partial = nullptr;  // per-block reduction partials (device)
+    double* red_result = nullptr;   // final reduction values (device, 8 slots)
+    double* host_result = nullptr;  // pinned host mirror of red_result
+    int red_capacity = 0;           // number of partial slots
+    int cu_count = 0;
+    // autotuned tile choice per (kernel family, dtype); -1 = not tuned yet
+    int tuned[16];
+    hipEvent_t events[8] = {};
+};
+
+Ctx* as_ctx(scafes_ctx* c);
+
+// ---------------------------------------------------------------- geometry
+struct Geo {
+    int nx, ny, nz;     // nodes per axis (1 on inactive axes)
+    int dim;
+    int nl0, nl1, nl2;  // nLayersAtBorder on active axes, 0 on inactive ones
+    long e0, e01;       // memAll strides
+    long base;          // offset of node (0,0,0)
+};
+
+// Builds the node->memory map of a field with hw ghost layers.
+int make_geo(const scafes_geom* g, int hw, Geo* out);
+
+__host__ __device__ inline long at(const Geo& g, int i, int j, int k)
+{
+    return g.base + i + g.e0 * j + g.e01 * k;
+}
+
+__device__ inline bool is_border(const Geo& g, int i, int j, int k)
+{
+    return i < g.nl0 || i > g.nx - 1 - g.nl0 || j < g.nl1 || j > g.ny - 1 - g.nl1 ||
+           k < g.nl2 || k > g.nz - 1 - g.nl2;
+}
+
+// Maps the hardware dispatch order (round-robin over the 8 XCDs of an
+// MI355X) to a logical tile order in which each XCD owns a contiguous run
+// of tiles, so y-neighbouring tiles share that XCD's L2.
+__device__ inline int xcd_remap(int b, int nb)
+{
+    constexpr int kXcd = 8;
+    const int per = nb / kXcd;
+    const int rem = nb % kXcd;
+    const int x = b % kXcd;
+    const int q = b / kXcd;
+    // XCD x gets per (+1 for x < rem) logical blocks starting at its offset
+    const int off = x * per + (x < rem ? x : rem);
+    if (q < per + (x < rem ? 1 : 0)) return off + q;
+    return b;  // unreachable for a full grid
+}
+
+inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+// Reductions (defined in scafes_capi.hip).
+enum RedOp { RED_MAX_ABS_DIFF = 0, RED_SUM_SQ_DIFF = 1, RED_SUM_PROD = 2 };
+int reduce_nodes(Ctx* c, const Geo& ga, const Geo& gb, int dtype, const void* a, const void* b,
+                 int op, int inner_only, double* out);
+
+}  // namespace scafes

@@ -1,0 +1,640 @@
+// stencil3d.hip -- 2.5D z-streaming 7-point stencil kernels for gfx950.
+//
+// One workgroup owns a TX x TY tile of an xy plane and walks a chunk of z
+// planes.  The centre column of the stencil field lives in a 3-deep
+// register queue (z-1, z, z+1); the current xy plane plus a one-node halo
+// is staged in LDS (double buffered, one barrier per plane), so every
+// field element is fetched from HBM once per sweep apart from the tile
+// halo, which neighbouring tiles on the same XCD bring into L2.  Tiles
+// are mapped XCD-contiguously (xcd_remap).
+//
+// Kernels never read ghost layers: every coordinate is clamped to the node
+// range, and only GLOBAL_INNER nodes (whose neighbours are nodes) use
+// neighbour values.  Ghost layers of the output are never written.
+//
+// Replaces the OpenMP node loop evalTypeTwoAt (src/ScaFES_Problem.hpp:5397-5476),
+// the copy-old-to-new (Problem.hpp:6226-6256) and the per-node index math
+// (DataField.hpp:962-969 -> Grid.hpp:955-968) of the reference.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "scafes_common.hpp"
+
+namespace scafes {
+
+template <typename T>
+struct St7 {
+    T c, w, e, s, n, d, u;  // centre, x-, x+, y-, y+, z-, z+
+};
+
+// ---------------------------------------------------------------- skeleton
+template <class Op, typename T, int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void stream_kernel(Op op, Geo g, int zc, int tiles_x,
+                                                         int tiles_y)
+{
+    constexpr bool TWO = Op::kTwo;
+    __shared__ T sa[2][TY + 2][TX + 2];
+    __shared__ T sb[TWO ? 2 : 1][TWO ? TY + 2 : 1][TWO ? TX + 2 : 1];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int per_plane = tiles_x * tiles_y;
+    const int tz = lb / per_plane;
+    const int r = lb - tz * per_plane;
+    const int tyi = r / tiles_x;
+    const int txi = r - tyi * tiles_x;
+    const int i = txi * TX + tx, j = tyi * TY + ty;
+    const int ic = min(i, g.nx - 1), jc = min(j, g.ny - 1);
+    const int iw = max(ic - 1, 0), ie = min(ic + 1, g.nx - 1);
+    const int js = max(jc - 1, 0), jn = min(jc + 1, g.ny - 1);
+    const int k0 = tz * zc, k1 = min(k0 + zc, g.nz);
+    const bool active = (i < g.nx) && (j < g.ny);
+    const T* __restrict__ A = op.a;
+    const T* __restrict__ B = op.b;
+
+    T am = A[at(g, ic, jc, max(k0 - 1, 0))];
+    T ac = A[at(g, ic, jc, k0)];
+    T bm = T(0), bc = T(0);
+    if constexpr (TWO) {
+        bm = B[at(g, ic, jc, max(k0 - 1, 0))];
+        bc = B[at(g, ic, jc, k0)];
+    }
+    for (int k = k0; k < k1; ++k) {
+        const int kn = min(k + 1, g.nz - 1);
+        const T ap = A[at(g, ic, jc, kn)];
+        T bp = T(0);
+        if constexpr (TWO) bp = B[at(g, ic, jc, kn)];
+        const int buf = (k - k0) & 1;
+        sa[buf][ty + 1][tx + 1] = ac;
+        if (tx == 0) sa[buf][ty + 1][0] = A[at(g, iw, jc, k)];
+        if (tx == TX - 1) sa[buf][ty + 1][TX + 1] = A[at(g, ie, jc, k)];
+        if (ty == 0) sa[buf][0][tx + 1] = A[at(g, ic, js, k)];
+        if (ty == TY - 1) sa[buf][TY + 1][tx + 1] = A[at(g, ic, jn, k)];
+        if constexpr (TWO) {
+            sb[buf][ty + 1][tx + 1] = bc;
+            if (tx == 0) sb[buf][ty + 1][0] = B[at(g, iw, jc, k)];
+            if (tx == TX - 1) sb[buf][ty + 1][TX + 1] = B[at(g, ie, jc, k)];
+            if (ty == 0) sb[buf][0][tx + 1] = B[at(g, ic, js, k)];
+            if (ty == TY - 1) sb[buf][TY + 1][tx + 1] = B[at(g, ic, jn, k)];
+        }
+        __syncthreads();
+        if (active) {
+            St7<T> s{ac, sa[buf][ty + 1][tx], sa[buf][ty + 1][tx + 2], sa[buf][ty][tx + 1],
+                     sa[buf][ty + 2][tx + 1], am, ap};
+            St7<T> t{};
+            if constexpr (TWO)
+                t = St7<T>{bc, sb[buf][ty + 1][tx], sb[buf][ty + 1][tx + 2], sb[buf][ty][tx + 1],
+                           sb[buf][ty + 2][tx + 1], bm, bp};
+            op.apply(g, i, j, k, s, t);
+        }
+        am = ac;
+        ac = ap;
+        if constexpr (TWO) {
+            bm = bc;
+            bc = bp;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- ops
+// Constant-coefficient heat / Pennes system update in the reference's FP64
+// operation order (HeatEqnFDM.hpp:209-217, PennesBioheatEqnFDM.hpp:158-167,
+// Dirichlet.hpp:124-130).  The library is built with -ffp-contract=off.
+struct HeatConstOp {
+    static constexpr bool kTwo = false;
+    const double* a;
+    const double* b;
+    double* out;
+    const double* f_old;
+    const double* g_new;
+    Geo gk;  // layout of the stencil-0 known fields
+    double ctau_a, ctw, ctf, hh0, hh1, hh2, border_value;
+    int has_perf, has_source, border_mode;
+    __device__ void apply(const Geo& g, int i, int j, int k, const St7<double>& s,
+                          const St7<double>&) const
+    {
+        const long m = at(g, i, j, k);
+        if (!is_border(g, i, j, k)) {
+            double v = s.c;
+            v += (ctau_a * ((s.w + s.e) - 2.0 * s.c)) / hh0;
+            if (g.dim > 1) v += (ctau_a * ((s.s + s.n) - 2.0 * s.c)) / hh1;
+            if (g.dim > 2) v += (ctau_a * ((s.d + s.u) - 2.0 * s.c)) / hh2;
+            if (has_perf) v -= ctw * s.c;
+            if (has_source) v += ctf * f_old[at(gk, i, j, k)];
+            out[m] = v;
+        } else if (border_mode == SCAFES_BORDER_FIELD) {
+            out[m] = g_new[at(gk, i, j, k)];
+        } else if (border_mode == SCAFES_BORDER_VALUE) {
+            out[m] = border_value;
+        } else {
+            out[m] = s.c;
+        }
+    }
+};
+
+// Variable-coefficient conservative heat.  MODE 0: FP64 in the CPU path's
+// order; 1: FP32; 2: FP32 with Kahan-compensated accumulation of the
+// per-step increment (u_true ~ u + c).
+template <typename T, int MODE>
+struct HeatVarOp {
+    static constexpr bool kTwo = true;
+    const T* a;  // u
+    const T* b;  // kappa
+    T* out;
+    const T* cin;
+    T* cout;
+    double tau, hh0, hh1, hh2;  // FP64 path
+    float c0, c1, c2;           // 0.5*tau/h_p^2 for the FP32 paths
+    T border_value;
+    int border_mode;
+    __device__ void apply(const Geo& g, int i, int j, int k, const St7<T>& s,
+                          const St7<T>& q) const
+    {
+        const long m = at(g, i, j, k);
+        if (!is_border(g, i, j, k)) {
+            if constexpr (MODE == 0) {
+                double v = s.c;
+                v += (tau * (0.5 * (q.e + q.c) * (s.e - s.c) - 0.5 * (q.c + q.w) * (s.c - s.w))) / hh0;
+                if (g.dim > 1)
+                    v += (tau * (0.5 * (q.n + q.c) * (s.n - s.c) - 0.5 * (q.c + q.s) * (s.c - s.s))) / hh1;
+                if (g.dim > 2)
+                    v += (tau * (0.5 * (q.u + q.c) * (s.u - s.c) - 0.5 * (q.c + q.d) * (s.c - s.d))) / hh2;
+                out[m] = v;
+            } else {
+                float d = c0 * __builtin_fmaf(q.e + q.c, s.e - s.c, -((q.c + q.w) * (s.c - s.w)));
+                if (g.dim > 1)
+                    d = __builtin_fmaf(c1, __builtin_fmaf(q.n + q.c, s.n - s.c, -((q.c + q.s) * (s.c - s.s))), d);
+                if (g.dim > 2)
+                    d = __builtin_fmaf(c2, __builtin_fmaf(q.u + q.c, s.u - s.c, -((q.c + q.d) * (s.c - s.d))), d);
+                if constexpr (MODE == 2) {
+                    const float y = d + cin[m];
+                    const float t = s.c + y;
+                    cout[m] = y - (t - s.c);
+                    out[m] = t;
+                } else {
+                    out[m] = s.c + d;
+                }
+            }
+        } else {
+            out[m] = border_mode == SCAFES_BORDER_VALUE ? border_value : s.c;
+            if constexpr (MODE == 2) cout[m] = border_mode == SCAFES_BORDER_VALUE ? 0.0f : cin[m];
+        }
+    }
+};
+
+// Pennes bioheat, TumorHeatEqnFDM::updateInner form with node fields w, Q
+// (examples/TumorHeatEqnFDM/TumorHeatEqnFDM.hpp:258-267); border nodes keep.
+template <typename T>
+struct PennesOp {
+    static constexpr bool kTwo = false;
+    const T* a;  // T_old
+    const T* b;
+    T* out;
+    const T* w;
+    const T* q;
+    double ctd, ctp, ctq, ta, hh0, hh1, hh2;  // FP64 path
+    float fd0, fd1, fd2, fp, fq, fta;         // FP32 path
+    __device__ void apply(const Geo& g, int i, int j, int k, const St7<T>& s, const St7<T>&) const
+    {
+        const long m = at(g, i, j, k);
+        if (is_border(g, i, j, k)) {
+            out[m] = s.c;
+            return;
+        }
+        if constexpr (sizeof(T) == 8) {
+            double v = s.c;
+            v += (ctd * ((s.w + s.e) - 2.0 * s.c)) / hh0;
+            if (g.dim > 1) v += (ctd * ((s.s + s.n) - 2.0 * s.c)) / hh1;
+            if (g.dim > 2) v += (ctd * ((s.d + s.u) - 2.0 * s.c)) / hh2;
+            v += (ctp * w[m]) * (ta - s.c);
+            v += ctq * q[m];
+            out[m] = v;
+        } else {
+            float d = fd0 * ((s.w + s.e) - 2.0f * s.c);
+            if (g.dim > 1) d = __builtin_fmaf(fd1, (s.s + s.n) - 2.0f * s.c, d);
+            if (g.dim > 2) d = __builtin_fmaf(fd2, (s.d + s.u) - 2.0f * s.c, d);
+            d = __builtin_fmaf(fp * w[m], fta - s.c, d);
+            d = __builtin_fmaf(fq, q[m], d);
+            out[m] = s.c + d;
+        }
+    }
+};
+
+// Discrete adjoint of PennesOp (inner nodes), FP64 gradient accumulation.
+template <typename T>
+struct PennesAdjOp {
+    static constexpr bool kTwo = false;
+    const T* a;  // lambda^n
+    const T* b;
+    T* out;      // lambda^{n-1}
+    const T* w;
+    const T* tprev;
+    double* grad;
+    double ctd, ctp, ta, hh0, hh1, hh2;
+    float fd0, fd1, fd2, fp;
+    __device__ void apply(const Geo& g, int i, int j, int k, const St7<T>& s, const St7<T>&) const
+    {
+        const long m = at(g, i, j, k);
+        if (is_border(g, i, j, k)) {
+            out[m] = T(0);
+            return;
+        }
+        grad[m] += (double)s.c * (ctp * (ta - (double)tprev[m]));
+        if constexpr (sizeof(T) == 8) {
+            double v = s.c;
+            v += (ctd * ((s.w + s.e) - 2.0 * s.c)) / hh0;
+            if (g.dim > 1) v += (ctd * ((s.s + s.n) - 2.0 * s.c)) / hh1;
+            if (g.dim > 2) v += (ctd * ((s.d + s.u) - 2.0 * s.c)) / hh2;
+            v -= (ctp * w[m]) * s.c;
+            out[m] = v;
+        } else {
+            float d = fd0 * ((s.w + s.e) - 2.0f * s.c);
+            if (g.dim > 1) d = __builtin_fmaf(fd1, (s.s + s.n) - 2.0f * s.c, d);
+            if (g.dim > 2) d = __builtin_fmaf(fd2, (s.d + s.u) - 2.0f * s.c, d);
+            d = __builtin_fmaf(-fp * w[m], s.c, d);
+            out[m] = s.c + d;
+        }
+    }
+};
+
+// ---------------------------------------------------------------- launch
+struct TileCfg {
+    int tx, ty, zc;
+};
+
+static const TileCfg kCfgs[] = {
+    {64, 4, 64},  {64, 4, 128}, {64, 8, 64},  {64, 8, 128}, {128, 2, 64},
+    {128, 4, 64}, {32, 8, 64},  {64, 4, 32},  {64, 16, 128}, {64, 8, 256},
+};
+static constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
+static constexpr int kDefaultCfg = 2;
+
+template <class Op, typename T, int TX, int TY>
+static int launch_t(Ctx* c, const Op& op, const Geo& g, int zc)
+{
+    const int tiles_x = ceil_div(g.nx, TX), tiles_y = ceil_div(g.ny, TY);
+    const int tiles_z = ceil_div(g.nz, zc);
+    const long nb = (long)tiles_x * tiles_y * tiles_z;
+    if (nb > 0x7fffffffL) return fail(SCAFES_ERR_ARG, "grid too large");
+    stream_kernel<Op, T, TX, TY><<<dim3((unsigned)nb), dim3(TX, TY), 0, c->stream>>>(op, g, zc, tiles_x,
+                                                                                    tiles_y);
+    SCAFES_HIP(hipGetLastError());
+    return SCAFES_OK;
+}
+
+template <class Op, typename T>
+static int launch(Ctx* c, const Op& op, const Geo& g, int cfg)
+{
+    if (cfg < 0 || cfg >= kNumCfgs) cfg = kDefaultCfg;
+    const TileCfg t = kCfgs[cfg];
+    switch (t.tx * 1000 + t.ty) {
+    case 64004: return launch_t<Op, T, 64, 4>(c, op, g, t.zc);
+    case 64008: return launch_t<Op, T, 64, 8>(c, op, g, t.zc);
+    case 128002: return launch_t<Op, T, 128, 2>(c, op, g, t.zc);
+    case 128004: return launch_t<Op, T, 128, 4>(c, op, g, t.zc);
+    case 32008: return launch_t<Op, T, 32, 8>(c, op, g, t.zc);
+    case 64016: return launch_t<Op, T, 64, 16>(c, op, g, t.zc);
+    default: return fail(SCAFES_ERR_ARG, "bad tile configuration");
+    }
+}
+
+static void hh_of(const scafes_geom* g, double hh[3])
+{
+    for (int p = 0; p < 3; ++p) hh[p] = p < g->dim ? g->h[p] * g->h[p] : 1.0;
+}
+
+static int check_stencil_geo(const scafes_geom* g, int hw)
+{
+    if (hw < 1) return fail(SCAFES_ERR_ARG, "unknown field needs stencil width >= 1");
+    if (g->nlayers < 1) return fail(SCAFES_ERR_ARG, "nLayersAtBorder must be >= 1 for a 7-point update");
+    return SCAFES_OK;
+}
+
+int heat7var_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* un,
+                    const void* cin, void* cout, const void* kap, double tau, int border_mode,
+                    double border_value, int cfg)
+{
+    double hh[3];
+    hh_of(g, hh);
+    if (mode == SCAFES_HEATVAR_F64) {
+        HeatVarOp<double, 0> op{};
+        op.a = (const double*)u;
+        op.b = (const double*)kap;
+        op.out = (double*)un;
+        op.tau = tau;
+        op.hh0 = hh[0];
+        op.hh1 = hh[1];
+        op.hh2 = hh[2];
+        op.border_value = border_value;
+        op.border_mode = border_mode;
+        return launch<HeatVarOp<double, 0>, double>(c, op, geo, cfg);
+    }
+    const float c0 = (float)(0.5 * tau / hh[0]), c1 = (float)(0.5 * tau / hh[1]),
+                c2 = (float)(0.5 * tau / hh[2]);
+    if (mode == SCAFES_HEATVAR_F32) {
+        HeatVarOp<float, 1> op{};
+        op.a = (const float*)u;
+        op.b = (const float*)kap;
+        op.out = (float*)un;
+        op.c0 = c0;
+        op.c1 = c1;
+        op.c2 = c2;
+        op.border_value = (float)border_value;
+        op.border_mode = border_mode;
+        return launch<HeatVarOp<float, 1>, float>(c, op, geo, cfg);
+    }
+    HeatVarOp<float, 2> op{};
+    op.a = (const float*)u;
+    op.b = (const float*)kap;
+    op.out = (float*)un;
+    op.cin = (const float*)cin;
+    op.cout = (float*)cout;
+    op.c0 = c0;
+    op.c1 = c1;
+    op.c2 = c2;
+    op.border_value = (float)border_value;
+    op.border_mode = border_mode;
+    return launch<HeatVarOp<float, 2>, float>(c, op, geo, cfg);
+}
+
+int pennes_step_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, const void* t_old,
+                       void* t_new, const void* w, const void* q, const scafes_pennes_params* p)
+{
+    double hh[3];
+    hh_of(g, hh);
+    if (dtype == SCAFES_F64) {
+        PennesOp<double> op{};
+        op.a = (const double*)t_old;
+        op.out = (double*)t_new;
+        op.w = (const double*)w;
+        op.q = (const double*)q;
+        op.ctd = p->tau * p->cdiff;
+        op.ctp = p->tau * p->cperf;
+        op.ctq = p->tau * p->csrc;
+        op.ta = p->t_a;
+        op.hh0 = hh[0];
+        op.hh1 = hh[1];
+        op.hh2 = hh[2];
+        return launch<PennesOp<double>, double>(c, op, geo, -1);
+    }
+    PennesOp<float> op{};
+    op.a = (const float*)t_old;
+    op.out = (float*)t_new;
+    op.w = (const float*)w;
+    op.q = (const float*)q;
+    op.fd0 = (float)(p->tau * p->cdiff / hh[0]);
+    op.fd1 = (float)(p->tau * p->cdiff / hh[1]);
+    op.fd2 = (float)(p->tau * p->cdiff / hh[2]);
+    op.fp = (float)(p->tau * p->cperf);
+    op.fq = (float)(p->tau * p->csrc);
+    op.fta = (float)p->t_a;
+    return launch<PennesOp<float>, float>(c, op, geo, -1);
+}
+
+int pennes_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, const void* lam,
+                      void* lam_new, const void* w, const void* tprev, double* grad,
+                      const scafes_pennes_params* p)
+{
+    double hh[3];
+    hh_of(g, hh);
+    if (dtype == SCAFES_F64) {
+        PennesAdjOp<double> op{};
+        op.a = (const double*)lam;
+        op.out = (double*)lam_new;
+        op.w = (const double*)w;
+        op.tprev = (const double*)tprev;
+        op.grad = grad;
+        op.ctd = p->tau * p->cdiff;
+        op.ctp = p->tau * p->cperf;
+        op.ta = p->t_a;
+        op.hh0 = hh[0];
+        op.hh1 = hh[1];
+        op.hh2 = hh[2];
+        return launch<PennesAdjOp<double>, double>(c, op, geo, -1);
+    }
+    PennesAdjOp<float> op{};
+    op.a = (const float*)lam;
+    op.out = (float*)lam_new;
+    op.w = (const float*)w;
+    op.tprev = (const float*)tprev;
+    op.grad = grad;
+    op.ctp = p->tau * p->cperf;
+    op.ta = p->t_a;
+    op.fd0 = (float)(p->tau * p->cdiff / hh[0]);
+    op.fd1 = (float)(p->tau * p->cdiff / hh[1]);
+    op.fd2 = (float)(p->tau * p->cdiff / hh[2]);
+    op.fp = (float)(p->tau * p->cperf);
+    return launch<PennesAdjOp<float>, float>(c, op, geo, -1);
+}
+
+// ---------------------------------------------------------------- init
+__device__ inline uint64_t splitmix64(uint64_t seed, uint64_t idx)
+{
+    uint64_t z = seed + (idx + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+template <typename T>
+__global__ void heatvar_init_kernel(Geo g, double hx, double hy, double hz, T* u, T* kap,
+                                    uint64_t seed, double amp)
+{
+    const long total = (long)g.nx * g.ny * g.nz;
+    const double pi = 3.14159265358979323846;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % g.nx);
+        const long r = t / g.nx;
+        const int j = (int)(r % g.ny);
+        const int k = (int)(r / g.ny);
+        const double x = hx * i, y = hy * j, z = hz * k;
+        double v = sin(pi * x);
+        if (g.dim > 1) v *= sin(pi * y);
+        if (g.dim > 2) v *= sin(pi * z);
+        if (!is_border(g, i, j, k)) {
+            const double un = (double)(splitmix64(seed, (uint64_t)t) >> 11) * (1.0 / 9007199254740992.0);
+            v += amp * (2.0 * un - 1.0);
+        }
+        const long m = at(g, i, j, k);
+        u[m] = (T)v;
+        kap[m] = (T)(1.0 + 0.5 * sin(2.0 * pi * x) * sin(2.0 * pi * y));
+    }
+}
+
+}  // namespace scafes
+
+using namespace scafes;
+
+extern "C" {
+
+int scafes_heat7_step_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const double* u_old,
+                          double* u_new, const double* f_old, const double* g_new,
+                          const scafes_heat7_params* p)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || !g || !u_old || !u_new || !p) return fail(SCAFES_ERR_ARG, "heat7: null pointer");
+    if (u_old == u_new) return fail(SCAFES_ERR_ARG, "heat7: in-place update not allowed");
+    int rc = check_stencil_geo(g, hw);
+    if (rc) return rc;
+    if (p->has_source && !f_old) return fail(SCAFES_ERR_ARG, "heat7: source requested without F");
+    if (p->border_mode == SCAFES_BORDER_FIELD && !g_new)
+        return fail(SCAFES_ERR_ARG, "heat7: border FIELD without G");
+    Geo geo, gk;
+    rc = make_geo(g, hw, &geo);
+    if (rc) return rc;
+    rc = make_geo(g, 0, &gk);
+    if (rc) return rc;
+    double hh[3];
+    hh_of(g, hh);
+    HeatConstOp op{};
+    op.a = u_old;
+    op.out = u_new;
+    op.f_old = f_old;
+    op.g_new = g_new;
+    op.gk = gk;
+    op.ctau_a = p->tau * p->coeff_a;
+    op.ctw = (p->tau * p->perf_c) * p->perf_w;
+    op.ctf = p->tau * p->rhoc_inv;
+    op.hh0 = hh[0];
+    op.hh1 = hh[1];
+    op.hh2 = hh[2];
+    op.has_perf = p->has_perf;
+    op.has_source = p->has_source;
+    op.border_mode = p->border_mode;
+    op.border_value = p->border_value;
+    return launch<HeatConstOp, double>(c, op, geo, -1);
+}
+
+int scafes_heat7var_num_cfgs(void) { return kNumCfgs; }
+
+int scafes_heat7var_cfg_desc(int cfg, int* tx, int* ty, int* zchunk)
+{
+    if (cfg < 0 || cfg >= kNumCfgs) return fail(SCAFES_ERR_ARG, "bad cfg");
+    if (tx) *tx = kCfgs[cfg].tx;
+    if (ty) *ty = kCfgs[cfg].ty;
+    if (zchunk) *zchunk = kCfgs[cfg].zc;
+    return SCAFES_OK;
+}
+
+int scafes_heat7var_run(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode, void* u, void* u2,
+                        void* cmp, void* cmp2, const void* kappa, double tau, int nsteps,
+                        int border_mode, double border_value, int cfg, int* result_in_second)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || !g || !u || !u2 || !kappa) return fail(SCAFES_ERR_ARG, "heat7var: null pointer");
+    if (mode < 0 || mode > 2) return fail(SCAFES_ERR_ARG, "heat7var: bad mode");
+    if (mode == SCAFES_HEATVAR_F32_KAHAN && (!cmp || !cmp2))
+        return fail(SCAFES_ERR_ARG, "heat7var: Kahan mode needs both compensation buffers");
+    if (border_mode == SCAFES_BORDER_FIELD)
+        return fail(SCAFES_ERR_UNSUPPORTED, "heat7var: border FIELD not supported");
+    if (nsteps < 0) return fail(SCAFES_ERR_ARG, "heat7var: negative step count");
+    int rc = check_stencil_geo(g, hw);
+    if (rc) return rc;
+    Geo geo;
+    rc = make_geo(g, hw, &geo);
+    if (rc) return rc;
+    if (cfg < 0) cfg = c->tuned[mode] >= 0 ? c->tuned[mode] : kDefaultCfg;
+    void *a = u, *b = u2, *ca = cmp, *cb = cmp2;
+    for (int s = 0; s < nsteps; ++s) {
+        rc = heat7var_launch(c, geo, g, mode, a, b, ca, cb, kappa, tau, border_mode, border_value, cfg);
+        if (rc) return rc;
+        std::swap(a, b);
+        std::swap(ca, cb);
+    }
+    if (result_in_second) *result_in_second = (nsteps % 2 == 1) ? 1 : 0;
+    return SCAFES_OK;
+}
+
+int scafes_autotune_heat7var(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode, void* u,
+                             void* u2, void* cmp, void* cmp2, const void* kappa, double tau,
+                             int nsteps, int* best_cfg, float* best_ms)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c) return fail(SCAFES_ERR_ARG, "autotune: null context");
+    if (nsteps < 1) nsteps = 1;
+    hipEvent_t e0, e1;
+    SCAFES_HIP(hipEventCreate(&e0));
+    SCAFES_HIP(hipEventCreate(&e1));
+    int best = -1;
+    float bms = 1e30f;
+    for (int cfg = 0; cfg < kNumCfgs; ++cfg) {
+        int which = 0;
+        // warm-up one step, then time nsteps (an even count keeps u in place)
+        int rc = scafes_heat7var_run(ctx, g, hw, mode, u, u2, cmp, cmp2, kappa, tau, 2, SCAFES_BORDER_KEEP,
+                                     0.0, cfg, &which);
+        if (rc) return rc;
+        SCAFES_HIP(hipEventRecord(e0, c->stream));
+        rc = scafes_heat7var_run(ctx, g, hw, mode, u, u2, cmp, cmp2, kappa, tau, 2 * ((nsteps + 1) / 2),
+                                 SCAFES_BORDER_KEEP, 0.0, cfg, &which);
+        if (rc) return rc;
+        SCAFES_HIP(hipEventRecord(e1, c->stream));
+        SCAFES_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SCAFES_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= (float)(2 * ((nsteps + 1) / 2));
+        if (ms < bms) {
+            bms = ms;
+            best = cfg;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->tuned[mode] = best;
+    if (best_cfg) *best_cfg = best;
+    if (best_ms) *best_ms = bms;
+    return SCAFES_OK;
+}
+
+int scafes_heat7var_init(scafes_ctx* ctx, const scafes_geom* g, int hw, int dtype, void* u, void* kappa,
+                         uint64_t seed, double amp)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || !g || !u || !kappa) return fail(SCAFES_ERR_ARG, "init: null pointer");
+    Geo geo;
+    int rc = make_geo(g, hw, &geo);
+    if (rc) return rc;
+    const long total = (long)geo.nx * geo.ny * geo.nz;
+    const int blocks = (int)std::min<long>((total + 255) / 256, 65536);
+    if (dtype == SCAFES_F64)
+        heatvar_init_kernel<double><<<blocks, 256, 0, c->stream>>>(geo, g->h[0], g->h[1], g->h[2],
+                                                                   (double*)u, (double*)kappa, seed, amp);
+    else
+        heatvar_init_kernel<float><<<blocks, 256, 0, c->stream>>>(geo, g->h[0], g->h[1], g->h[2],
+                                                                  (float*)u, (float*)kappa, seed, amp);
+    SCAFES_HIP(hipGetLastError());
+    return SCAFES_OK;
+}
+
+int scafes_pennes_step(scafes_ctx* ctx, const scafes_geom* g, int hw, int dtype, const void* t_old,
+                       void* t_new, const void* w, const void* q, const scafes_pennes_params* p)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || !g || !t_old || !t_new || !w || !q || !p) return fail(SCAFES_ERR_ARG, "pennes: null pointer");
+    if (t_old == t_new) return fail(SCAFES_ERR_ARG, "pennes: in-place update not allowed");
+    int rc = check_stencil_geo(g, hw);
+    if (rc) return rc;
+    Geo geo;
+    rc = make_geo(g, hw, &geo);
+    if (rc) return rc;
+    return pennes_step_launch(c, geo, g, dtype, t_old, t_new, w, q, p);
+}
+
+int scafes_pennes_adjoint_step(scafes_ctx* ctx, const scafes_geom* g, int hw, int dtype, const void* lam,
+                               void* lam_new, const void* w, const void* t_prev, double* grad,
+                               const scafes_pennes_params* p)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || !g || !lam || !lam_new || !w || !t_prev || !grad || !p)
+        return fail(SCAFES_ERR_ARG, "pennes adjoint: null pointer");
+    if (lam == lam_new) return fail(SCAFES_ERR_ARG, "pennes adjoint: in-place update not allowed");
+    int rc = check_stencil_geo(g, hw);
+    if (rc) return rc;
+    Geo geo;
+    rc = make_geo(g, hw, &geo);
+    if (rc) return rc;
+    return pennes_adj_launch(c, geo, g, dtype, lam, lam_new, w, t_prev, grad, p);
+}
+
+}  // extern "C"

@@ -1,0 +1,303 @@
+"""GPU parity of the heat-family kernels against the CPU oracle, through the C ABI.
+
+FP64 kernels follow the CPU path's operation order and are compared
+bit-for-bit (the 1e-12 bar is met with zero difference).  FP32 paths are
+compared against the FP64 oracle with the tolerance written in each test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from masterthesis_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx(hip):
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+def _xs(n):
+    return [np.array([(1.0 / (m - 1)) * i for i in range(m)]) for m in n]
+
+
+def timelin_known(n, t, pennes=False, rhob_cb_w=1.0):
+    """F, G, Y of the TimeLinSpaceLin problems (stencil-0 layout, (z,y,x)),
+    evaluated with the hooks' operation order."""
+    dim = len(n)
+    xs = _xs(n)
+    shape = tuple(reversed(n))
+    mesh = np.meshgrid(*reversed(xs), indexing="ij")[::-1]  # mesh[p] indexed (z,y,x)
+    s = np.ones(shape)
+    for p in range(dim):
+        s = s + mesh[p]
+    border = np.zeros(shape, bool)
+    idx = np.meshgrid(*[np.arange(m) for m in reversed(n)], indexing="ij")[::-1]
+    for p in range(dim):
+        border |= (idx[p] < 1) | (idx[p] > n[p] - 2)
+    F = np.where(border, 0.0, s * (1.0 * 1.0))
+    if pennes:
+        F = np.where(border, 0.0, F + rhob_cb_w * (s * (1.0 + t)))
+    G = np.where(border, s * (1.0 + t), 0.0)
+    Y = s * (1.0 + t)
+    return F, G, Y
+
+
+@pytest.mark.parametrize("n,pennes", [((16, 16, 16), False), ((12, 10, 9), False), ((20, 17), False),
+                                      ((33,), False), ((14, 13, 12), True)])
+def test_timelin_bitexact(ctx, n, pennes):
+    dim = len(n)
+    g = capi.geom(n)
+    og = O.make_grid(n)
+    h2 = sum(1.0 / O.lib().oracle_gridsize(og, p) ** 2 for p in range(dim))
+    tau = 0.8 / (2.0 * h2 + (1.0 if pennes else 0.0))
+    nsteps = 25
+    coef = O.heat7_coef(tau, perf_c=1.0, perf_w=1.0, rhob_cb_w=1.0, has_perf=pennes, has_source=True)
+    ref, linf, _ = O.heat7_run(og, np.zeros(O.memall_shape(og)), nsteps, 0.0, coef,
+                               O.KNOWN_PENNES_TIMELIN if pennes else O.KNOWN_HEAT_TIMELIN)
+    # initial state: (1+sum x)(1+t0) on nodes, ghosts 0
+    _, _, Y0 = timelin_known(n, 0.0)
+    u0 = np.zeros(O.memall_shape(og))
+    O.nodes_view(og, u0)[...] = Y0.reshape(O.nodes_view(og, u0).shape)
+    ne = u0.size
+    a, b = ctx.upload(u0), ctx.upload(u0)
+    Fd, Gd = ctx.alloc(Y0.size), ctx.alloc(Y0.size)
+    p = capi.Heat7Params(tau=tau, coeff_a=1.0, rhoc_inv=1.0, perf_c=1.0, perf_w=1.0,
+                         has_perf=int(pennes), has_source=1, border_mode=capi.BORDER_FIELD,
+                         border_value=0.0)
+    for step in range(1, nsteps + 1):
+        F_old, _, _ = timelin_known(n, (step - 1) * tau, pennes)
+        _, G_new, _ = timelin_known(n, step * tau, pennes)
+        Fd.write(F_old)
+        Gd.write(G_new)
+        capi.check(ctx.L.scafes_heat7_step_f64(ctx.h, capi.C.byref(g), 1, a.ptr, b.ptr, Fd.ptr, Gd.ptr,
+                                               capi.C.byref(p)))
+        a, b = b, a
+    got = a.read().reshape(u0.shape)
+    np.testing.assert_array_equal(O.nodes_view(og, got), O.nodes_view(og, ref))
+    assert got.size == ne
+    # and the device path reproduces the exact discrete solution
+    _, _, Yn = timelin_known(n, nsteps * tau)
+    assert np.abs(O.nodes_view(og, got) - Yn.reshape(O.nodes_view(og, got).shape)).max() < 1e-12
+
+
+def sine_state(og):
+    n = [og.n[p] for p in range(og.dim)]
+    u = np.zeros(O.memall_shape(og))
+    xs = _xs(n)
+    v = np.ones(tuple(reversed(n)))
+    mesh = np.meshgrid(*reversed(xs), indexing="ij")[::-1]
+    for p in range(og.dim):
+        v = v * np.sin(np.pi * mesh[p])
+    O.nodes_view(og, u)[...] = v.reshape(O.nodes_view(og, u).shape)
+    return u, v
+
+
+def test_sine_mode_64cube_bitexact_and_analytic(ctx):
+    """Heat 64^3, FP64, tau = 0.9 h^2/6, 100 steps, u0 = prod sin(pi x), G = 0."""
+    n = 64
+    og = O.make_grid((n, n, n))
+    g = capi.geom((n, n, n))
+    h = 1.0 / (n - 1)
+    tau = 0.9 * h * h / 6.0
+    nsteps = 100
+    u0, prod = sine_state(og)
+    ref, _, _ = O.heat7_run(og, u0, nsteps, 0.0, O.heat7_coef(tau, coeff_a=1.0), O.KNOWN_NONE)
+    a, b = ctx.upload(u0), ctx.upload(u0)
+    p = capi.Heat7Params(tau=tau, coeff_a=1.0, rhoc_inv=1.0, has_perf=0, has_source=0,
+                         border_mode=capi.BORDER_VALUE, border_value=0.0)
+    for _ in range(nsteps):
+        capi.check(ctx.L.scafes_heat7_step_f64(ctx.h, capi.C.byref(g), 1, a.ptr, b.ptr, None, None,
+                                               capi.C.byref(p)))
+        a, b = b, a
+    got = a.read().reshape(u0.shape)
+    np.testing.assert_array_equal(got, ref)
+    # error against the analytic solution exp(-3 pi^2 t) prod sin(pi x) (reported)
+    t = nsteps * tau
+    exact = math.exp(-3 * math.pi ** 2 * t) * prod
+    rel = np.abs(O.nodes_view(og, got) - exact.reshape(O.nodes_view(og, got).shape)).max() / np.abs(exact).max()
+    print(f"\n[config heat64] rel. error vs analytic solution: {rel:.3e}")
+    assert rel < 1e-3
+
+
+def heatvar_inputs(n, seed=0, amp=1e-3):
+    og = O.make_grid(n)
+    xs = _xs(n)
+    mesh = np.meshgrid(*reversed(xs), indexing="ij")[::-1]
+    kap = np.zeros(O.memall_shape(og))
+    K = 1.0 + 0.5 * np.sin(2 * np.pi * mesh[0]) * np.sin(2 * np.pi * mesh[1])
+    O.nodes_view(og, kap)[...] = K.reshape(O.nodes_view(og, kap).shape)
+    u, _ = sine_state(og)
+    rng = np.random.default_rng(seed)
+    noise = amp * (2 * rng.random(tuple(reversed(n))) - 1)
+    inner = np.zeros(tuple(reversed(n)), bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    O.nodes_view(og, u)[...] += np.where(inner, noise, 0.0).reshape(O.nodes_view(og, u).shape)
+    return og, u, kap
+
+
+def run_heatvar(ctx, g, mode, u0, kap, tau, nsteps, cfg=-1, dtype=np.float64):
+    a, b = ctx.upload(u0.astype(dtype)), ctx.upload(u0.astype(dtype))
+    kd = ctx.upload(kap.astype(dtype))
+    c1 = c2 = None
+    if mode == capi.HEATVAR_F32_KAHAN:
+        c1, c2 = ctx.alloc(u0.size, np.float32), ctx.alloc(u0.size, np.float32)
+        c1.fill(0.0)
+        c2.fill(0.0)
+    which = capi.C.c_int(0)
+    capi.check(ctx.L.scafes_heat7var_run(ctx.h, capi.C.byref(g), 1, mode, a.ptr, b.ptr,
+                                         c1.ptr if c1 else None, c2.ptr if c2 else None, kd.ptr, tau,
+                                         nsteps, capi.BORDER_VALUE, 0.0, cfg, capi.C.byref(which)))
+    res = (b if which.value else a).read().reshape(u0.shape)
+    comp = None
+    if c1 is not None:
+        comp = (c2 if which.value else c1).read().reshape(u0.shape)
+    for d in (a, b, kd, c1, c2):
+        if d is not None:
+            d.free()
+    return res, comp
+
+
+@pytest.mark.parametrize("cfg", list(range(10)))
+def test_heatvar_f64_bitexact_all_tiles(ctx, cfg):
+    n = (37, 29, 23)  # not multiples of any tile edge
+    og, u0, kap = heatvar_inputs(n, seed=cfg)
+    g = capi.geom(n)
+    tau = 0.9 / (2.0 * 1.5 * sum((m - 1.0) ** 2 for m in n))  # 1/tau >= 2 kappa_max sum 1/h^2
+    ref = O.heat7var_run(og, u0, kap, tau, 12)
+    got, _ = run_heatvar(ctx, g, capi.HEATVAR_F64, u0, kap, tau, 12, cfg=cfg)
+    np.testing.assert_array_equal(O.nodes_view(og, got), O.nodes_view(og, ref))
+
+
+def test_heatvar_fp32_kahan_error(ctx):
+    """FP32 + Kahan stays within 1e-5 relative (to max|u|) of the FP64 oracle."""
+    n = (48, 48, 48)
+    og, u0, kap = heatvar_inputs(n, seed=3)
+    g = capi.geom(n)
+    h = 1.0 / (n[0] - 1)
+    tau = 0.9 * h * h / (6 * 1.5)
+    nsteps = 300
+    ref = O.heat7var_run(og, u0, kap, tau, nsteps)
+    k32, comp = run_heatvar(ctx, g, capi.HEATVAR_F32_KAHAN, u0, kap, tau, nsteps, dtype=np.float32)
+    p32, _ = run_heatvar(ctx, g, capi.HEATVAR_F32, u0, kap, tau, nsteps, dtype=np.float32)
+    scale = np.abs(O.nodes_view(og, ref)).max()
+    ek = np.abs(O.nodes_view(og, k32.astype(np.float64) + comp) - O.nodes_view(og, ref)).max() / scale
+    ep = np.abs(O.nodes_view(og, p32.astype(np.float64)) - O.nodes_view(og, ref)).max() / scale
+    print(f"\n[heat7var 48^3 x {nsteps}] rel err fp32={ep:.2e} fp32+kahan={ek:.2e}")
+    assert ek < 1e-5
+    assert ek < ep
+
+
+def test_heatvar_init_matches_numpy(ctx):
+    """Device-side synthetic init (sine + splitmix64 noise, kappa field)."""
+    n = (20, 18, 16)
+    og = O.make_grid(n)
+    g = capi.geom(n)
+    ne = capi.memall_count(g, 1)
+    u, k = ctx.alloc(ne), ctx.alloc(ne)
+    u.fill(0.0)
+    k.fill(0.0)
+    capi.check(ctx.L.scafes_heat7var_init(ctx.h, capi.C.byref(g), 1, capi.F64, u.ptr, k.ptr, 7, 1e-3))
+    from masterthesis_amd.workloads import heatvar_init_numpy
+    u_np, k_np = heatvar_init_numpy(n, seed=7, amp=1e-3)
+    got_u = O.nodes_view(og, u.read().reshape(O.memall_shape(og)))
+    got_k = O.nodes_view(og, k.read().reshape(O.memall_shape(og)))
+    np.testing.assert_allclose(got_u, u_np, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(got_k, k_np, rtol=0, atol=1e-15)
+
+
+def test_reductions(ctx):
+    n = (21, 19, 17)
+    og = O.make_grid(n)
+    g = capi.geom(n)
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal(O.memall_shape(og))
+    b = rng.standard_normal(tuple(reversed(n)))  # stencil-0 layout
+    da, db = ctx.upload(a), ctx.upload(b)
+    out = capi.C.c_double(0)
+    capi.check(ctx.L.scafes_reduce(ctx.h, capi.C.byref(g), capi.F64, da.ptr, 1, db.ptr, 0,
+                                   capi.RED_MAX_ABS_DIFF, 0, capi.C.byref(out)))
+    an = O.nodes_view(og, a)
+    assert out.value == np.abs(an - b).max()
+    capi.check(ctx.L.scafes_reduce(ctx.h, capi.C.byref(g), capi.F64, da.ptr, 1, db.ptr, 0,
+                                   capi.RED_SUM_SQ_DIFF, 1, capi.C.byref(out)))
+    inner = (slice(1, -1),) * 3
+    assert abs(out.value - np.sum((an[inner] - b[inner]) ** 2)) < 1e-10 * out.value
+
+
+# ---------------------------------------------------------------------------
+# Pennes field form (forward and adjoint steps)
+
+def pennes_inputs(n, seed=11):
+    og = O.make_grid(n)
+    h = 1.0 / (n[0] - 1)
+    rng = np.random.default_rng(seed)
+    w = 0.5 * np.exp(0.3 * rng.standard_normal(O.memall_shape(og)))
+    x = [np.linspace(0, 1, m) for m in n]
+    Z, Y, X = np.meshgrid(x[2], x[1], x[0], indexing="ij")
+    q = np.zeros(O.memall_shape(og))
+    O.nodes_view(og, q)[...] = 50.0 * np.exp(-((X - .5) ** 2 + (Y - .5) ** 2 + (Z - .5) ** 2) / (2 * .15 ** 2))
+    tau = 0.9 / (6 * 0.5 / h ** 2 + w.max())
+    coef = O.pennes_coef(tau=tau, cdiff=0.5, cperf=1.0, csrc=1.0, t_a=37.0)
+    T0 = np.full(O.memall_shape(og), 37.0)
+    return og, w, q, coef, T0
+
+
+def test_pennes_forward_and_adjoint_bitexact(ctx):
+    n = (19, 17, 15)
+    og, w, q, coef, T0 = pennes_inputs(n)
+    g = capi.geom(n)
+    N = 15
+    st = O.pennes_forward_all(og, T0, w, q, coef, N)
+    rng = np.random.default_rng(4)
+    target = st[-1] + 0.01 * rng.standard_normal(st[-1].shape)
+    J, grad = O.pennes_grad(og, st, w, target, coef, N)
+    pp = capi.PennesParams(tau=coef.tau, cdiff=coef.cdiff, cperf=coef.cperf, csrc=coef.csrc, t_a=coef.t_a)
+    a, b = ctx.upload(T0), ctx.upload(T0)
+    dw, dq = ctx.upload(w), ctx.upload(q)
+    for _ in range(N):
+        capi.check(ctx.L.scafes_pennes_step(ctx.h, capi.C.byref(g), 1, capi.F64, a.ptr, b.ptr, dw.ptr,
+                                            dq.ptr, capi.C.byref(pp)))
+        a, b = b, a
+    np.testing.assert_array_equal(a.read().reshape(T0.shape), st[-1])
+    # adjoint sweep over the oracle's stored states
+    inner = np.zeros(T0.shape, bool)
+    O.nodes_view(og, inner)[1:-1, 1:-1, 1:-1] = True
+    lam0 = np.where(inner, st[-1] - target, 0.0)
+    la, lb = ctx.upload(lam0), ctx.upload(lam0)
+    dg = ctx.upload(np.zeros_like(T0))
+    dprev = ctx.alloc(T0.size)
+    for s in range(N, 0, -1):
+        dprev.write(st[s - 1])
+        capi.check(ctx.L.scafes_pennes_adjoint_step(ctx.h, capi.C.byref(g), 1, capi.F64, la.ptr, lb.ptr,
+                                                    dw.ptr, dprev.ptr, dg.ptr, capi.C.byref(pp)))
+        la, lb = lb, la
+    np.testing.assert_array_equal(dg.read().reshape(T0.shape), grad)
+
+
+def test_pennes_fp32_deviation_form_error(ctx):
+    """FP32 stream in the deviation variable theta = T - T_a (T_a = 0 in the
+    kernel) against the FP64 oracle on absolute temperatures."""
+    n = (32, 32, 32)
+    og, w, q, coef, T0 = pennes_inputs(n)
+    g = capi.geom(n)
+    N = 200
+    st = O.pennes_forward_all(og, T0, w, q, coef, N)
+    pp = capi.PennesParams(tau=coef.tau, cdiff=coef.cdiff, cperf=coef.cperf, csrc=coef.csrc, t_a=0.0)
+    th0 = (T0 - coef.t_a).astype(np.float32)
+    a, b = ctx.upload(th0), ctx.upload(th0)
+    dw, dq = ctx.upload(w.astype(np.float32)), ctx.upload(q.astype(np.float32))
+    for _ in range(N):
+        capi.check(ctx.L.scafes_pennes_step(ctx.h, capi.C.byref(g), 1, capi.F32, a.ptr, b.ptr, dw.ptr,
+                                            dq.ptr, capi.C.byref(pp)))
+        a, b = b, a
+    got = a.read().reshape(T0.shape).astype(np.float64) + coef.t_a
+    rise = np.abs(st[-1] - coef.t_a).max()
+    err = np.abs(got - st[-1]).max() / rise
+    print(f"\n[pennes 32^3 x {N}] fp32 rel err (to max temperature rise) {err:.2e}")
+    assert err < 1e-5
