@@ -118,7 +118,7 @@ int scafes_heat7var_run(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode,
                         int border_mode, double border_value, int cfg, int* result_in_second);
 /* Number of tile configurations and a description of one. */
 int scafes_heat7var_num_cfgs(void);
-int scafes_heat7var_cfg_desc(int cfg, int* tx, int* ty, int* zchunk);
+int scafes_heat7var_cfg_desc(int cfg, int* tx, int* ty, int* zchunk, int* depth);
 /* Project autotuner: times every configuration on the given buffers for
  * nsteps steps and stores the fastest as the default for this mode. */
 int scafes_autotune_heat7var(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode, void* u,

@@ -85,6 +85,15 @@ __device__ inline int xcd_remap(int b, int nb)
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
+// Streaming store of a stencil output: written once per sweep and read only
+// by the next sweep, so it is marked non-temporal to keep L2 for the
+// neighbour-tile halo reuse of the inputs.
+template <typename T>
+__device__ inline void stream_store(T* p, T v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+
 // Reductions (defined in scafes_capi.hip).
 enum RedOp { RED_MAX_ABS_DIFF = 0, RED_SUM_SQ_DIFF = 1, RED_SUM_PROD = 2 };
 int reduce_nodes(Ctx* c, const Geo& ga, const Geo& gb, int dtype, const void* a, const void* b,

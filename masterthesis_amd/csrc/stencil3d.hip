@@ -30,11 +30,20 @@ struct St7 {
 };
 
 // ---------------------------------------------------------------- skeleton
-template <class Op, typename T, int TX, int TY>
+// D = prefetch depth in planes (even, >= 2).  Every value a thread needs for
+// plane k (its centre, the halo value it stages for the tile, and the
+// centre-only fields of the op) is loaded D planes ahead into a register
+// ring, so each plane's barrier waits on loads issued D iterations before.
+// Op interface: kTwo (second stencil field b, e.g. kappa), kNX centre-only
+// fields x[0..kNX-1] of type T, apply(g, i, j, k, a7, b7, xv).
+template <class Op, typename T, int TX, int TY, int D>
 __global__ __launch_bounds__(TX* TY) void stream_kernel(Op op, Geo g, int zc, int tiles_x,
                                                          int tiles_y)
 {
+    static_assert(D >= 2 && D % 2 == 0, "prefetch depth must be even");
     constexpr bool TWO = Op::kTwo;
+    constexpr int NX = Op::kNX;
+    constexpr int NXA = NX > 0 ? NX : 1;
     __shared__ T sa[2][TY + 2][TX + 2];
     __shared__ T sb[TWO ? 2 : 1][TWO ? TY + 2 : 1][TWO ? TX + 2 : 1];
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -46,53 +55,85 @@ __global__ __launch_bounds__(TX* TY) void stream_kernel(Op op, Geo g, int zc, in
     const int txi = r - tyi * tiles_x;
     const int i = txi * TX + tx, j = tyi * TY + ty;
     const int ic = min(i, g.nx - 1), jc = min(j, g.ny - 1);
-    const int iw = max(ic - 1, 0), ie = min(ic + 1, g.nx - 1);
-    const int js = max(jc - 1, 0), jn = min(jc + 1, g.ny - 1);
     const int k0 = tz * zc, k1 = min(k0 + zc, g.nz);
+    const int kcmax = min(k1, g.nz - 1);  // last plane whose centre value is needed
     const bool active = (i < g.nx) && (j < g.ny);
+    // halo duty of this thread: one x-halo column value and/or one y-halo row value
+    const bool hx = (tx == 0) || (tx == TX - 1);
+    const bool hy = (ty == 0) || (ty == TY - 1);
+    const int hxi = (tx == 0) ? max(ic - 1, 0) : min(ic + 1, g.nx - 1);
+    const int hyj = (ty == 0) ? max(jc - 1, 0) : min(jc + 1, g.ny - 1);
+    const int sxh = (tx == 0) ? 0 : TX + 1;
+    const int syh = (ty == 0) ? 0 : TY + 1;
+    const long oc = at(g, ic, jc, 0), ohx = at(g, hxi, jc, 0), ohy = at(g, ic, hyj, 0);
     const T* __restrict__ A = op.a;
     const T* __restrict__ B = op.b;
 
-    T am = A[at(g, ic, jc, max(k0 - 1, 0))];
-    T ac = A[at(g, ic, jc, k0)];
-    T bm = T(0), bc = T(0);
-    if constexpr (TWO) {
-        bm = B[at(g, ic, jc, max(k0 - 1, 0))];
-        bc = B[at(g, ic, jc, k0)];
+    T qa[D], qb[D], qax[D], qbx[D], qay[D], qby[D], qx[NXA][D];
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+        qa[s] = qb[s] = qax[s] = qbx[s] = qay[s] = qby[s] = T(0);
+#pragma unroll
+        for (int x = 0; x < NXA; ++x) qx[x][s] = T(0);
     }
-    for (int k = k0; k < k1; ++k) {
-        const int kn = min(k + 1, g.nz - 1);
-        const T ap = A[at(g, ic, jc, kn)];
-        T bp = T(0);
-        if constexpr (TWO) bp = B[at(g, ic, jc, kn)];
-        const int buf = (k - k0) & 1;
-        sa[buf][ty + 1][tx + 1] = ac;
-        if (tx == 0) sa[buf][ty + 1][0] = A[at(g, iw, jc, k)];
-        if (tx == TX - 1) sa[buf][ty + 1][TX + 1] = A[at(g, ie, jc, k)];
-        if (ty == 0) sa[buf][0][tx + 1] = A[at(g, ic, js, k)];
-        if (ty == TY - 1) sa[buf][TY + 1][tx + 1] = A[at(g, ic, jn, k)];
-        if constexpr (TWO) {
-            sb[buf][ty + 1][tx + 1] = bc;
-            if (tx == 0) sb[buf][ty + 1][0] = B[at(g, iw, jc, k)];
-            if (tx == TX - 1) sb[buf][ty + 1][TX + 1] = B[at(g, ie, jc, k)];
-            if (ty == 0) sb[buf][0][tx + 1] = B[at(g, ic, js, k)];
-            if (ty == TY - 1) sb[buf][TY + 1][tx + 1] = B[at(g, ic, jn, k)];
+    auto fetch = [&](const int slot, const int kk) {
+        if (kk <= kcmax) {
+            const long o = g.e01 * kk;
+            qa[slot] = A[oc + o];
+            if constexpr (TWO) qb[slot] = B[oc + o];
+            if (kk < k1) {
+                if (hx) {
+                    qax[slot] = A[ohx + o];
+                    if constexpr (TWO) qbx[slot] = B[ohx + o];
+                }
+                if (hy) {
+                    qay[slot] = A[ohy + o];
+                    if constexpr (TWO) qby[slot] = B[ohy + o];
+                }
+#pragma unroll
+                for (int x = 0; x < NX; ++x) qx[x][slot] = op.x[x][oc + o];
+            }
         }
-        __syncthreads();
-        if (active) {
-            St7<T> s{ac, sa[buf][ty + 1][tx], sa[buf][ty + 1][tx + 2], sa[buf][ty][tx + 1],
-                     sa[buf][ty + 2][tx + 1], am, ap};
-            St7<T> t{};
-            if constexpr (TWO)
-                t = St7<T>{bc, sb[buf][ty + 1][tx], sb[buf][ty + 1][tx + 2], sb[buf][ty][tx + 1],
-                           sb[buf][ty + 2][tx + 1], bm, bp};
-            op.apply(g, i, j, k, s, t);
-        }
-        am = ac;
-        ac = ap;
-        if constexpr (TWO) {
+    };
+
+    T am = A[oc + g.e01 * max(k0 - 1, 0)];
+    T bm = T(0);
+    if constexpr (TWO) bm = B[oc + g.e01 * max(k0 - 1, 0)];
+#pragma unroll
+    for (int s = 0; s < D; ++s) fetch(s, k0 + s);
+
+    for (int kb = k0; kb < k1; kb += D) {
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+            const int k = kb + s;
+            if (k >= k1) break;  // uniform over the workgroup
+            const int buf = s & 1;
+            const T ac = qa[s], ap = qa[(s + 1) % D];
+            const T bc = qb[s], bp = qb[(s + 1) % D];
+            T xv[NXA];
+#pragma unroll
+            for (int x = 0; x < NXA; ++x) xv[x] = qx[x][s];
+            sa[buf][ty + 1][tx + 1] = ac;
+            if (hx) sa[buf][ty + 1][sxh] = qax[s];
+            if (hy) sa[buf][syh][tx + 1] = qay[s];
+            if constexpr (TWO) {
+                sb[buf][ty + 1][tx + 1] = bc;
+                if (hx) sb[buf][ty + 1][sxh] = qbx[s];
+                if (hy) sb[buf][syh][tx + 1] = qby[s];
+            }
+            fetch(s, k + D);
+            __syncthreads();
+            if (active) {
+                St7<T> sA{ac, sa[buf][ty + 1][tx], sa[buf][ty + 1][tx + 2], sa[buf][ty][tx + 1],
+                          sa[buf][ty + 2][tx + 1], am, ap};
+                St7<T> sB{};
+                if constexpr (TWO)
+                    sB = St7<T>{bc, sb[buf][ty + 1][tx], sb[buf][ty + 1][tx + 2], sb[buf][ty][tx + 1],
+                                sb[buf][ty + 2][tx + 1], bm, bp};
+                op.apply(g, i, j, k, sA, sB, xv);
+            }
+            am = ac;
             bm = bc;
-            bc = bp;
         }
     }
 }
@@ -103,8 +144,10 @@ __global__ __launch_bounds__(TX* TY) void stream_kernel(Op op, Geo g, int zc, in
 // Dirichlet.hpp:124-130).  The library is built with -ffp-contract=off.
 struct HeatConstOp {
     static constexpr bool kTwo = false;
+    static constexpr int kNX = 0;
     const double* a;
     const double* b;
+    const double* x[1];
     double* out;
     const double* f_old;
     const double* g_new;
@@ -112,7 +155,7 @@ struct HeatConstOp {
     double ctau_a, ctw, ctf, hh0, hh1, hh2, border_value;
     int has_perf, has_source, border_mode;
     __device__ void apply(const Geo& g, int i, int j, int k, const St7<double>& s,
-                          const St7<double>&) const
+                          const St7<double>&, const double*) const
     {
         const long m = at(g, i, j, k);
         if (!is_border(g, i, j, k)) {
@@ -139,17 +182,18 @@ struct HeatConstOp {
 template <typename T, int MODE>
 struct HeatVarOp {
     static constexpr bool kTwo = true;
+    static constexpr int kNX = MODE == 2 ? 1 : 0;
     const T* a;  // u
     const T* b;  // kappa
+    const T* x[1];  // x[0] = incoming compensation (Kahan mode)
     T* out;
-    const T* cin;
     T* cout;
     double tau, hh0, hh1, hh2;  // FP64 path
     float c0, c1, c2;           // 0.5*tau/h_p^2 for the FP32 paths
     T border_value;
     int border_mode;
     __device__ void apply(const Geo& g, int i, int j, int k, const St7<T>& s,
-                          const St7<T>& q) const
+                          const St7<T>& q, const T* xv) const
     {
         const long m = at(g, i, j, k);
         if (!is_border(g, i, j, k)) {
@@ -168,7 +212,7 @@ struct HeatVarOp {
                 if (g.dim > 2)
                     d = __builtin_fmaf(c2, __builtin_fmaf(q.u + q.c, s.u - s.c, -((q.c + q.d) * (s.c - s.d))), d);
                 if constexpr (MODE == 2) {
-                    const float y = d + cin[m];
+                    const float y = d + xv[0];
                     const float t = s.c + y;
                     cout[m] = y - (t - s.c);
                     out[m] = t;
@@ -178,7 +222,7 @@ struct HeatVarOp {
             }
         } else {
             out[m] = border_mode == SCAFES_BORDER_VALUE ? border_value : s.c;
-            if constexpr (MODE == 2) cout[m] = border_mode == SCAFES_BORDER_VALUE ? 0.0f : cin[m];
+            if constexpr (MODE == 2) cout[m] = border_mode == SCAFES_BORDER_VALUE ? 0.0f : xv[0];
         }
     }
 };
@@ -188,14 +232,15 @@ struct HeatVarOp {
 template <typename T>
 struct PennesOp {
     static constexpr bool kTwo = false;
+    static constexpr int kNX = 2;
     const T* a;  // T_old
     const T* b;
+    const T* x[2];  // perfusion w, source Q
     T* out;
-    const T* w;
-    const T* q;
     double ctd, ctp, ctq, ta, hh0, hh1, hh2;  // FP64 path
     float fd0, fd1, fd2, fp, fq, fta;         // FP32 path
-    __device__ void apply(const Geo& g, int i, int j, int k, const St7<T>& s, const St7<T>&) const
+    __device__ void apply(const Geo& g, int i, int j, int k, const St7<T>& s, const St7<T>&,
+                          const T* xv) const
     {
         const long m = at(g, i, j, k);
         if (is_border(g, i, j, k)) {
@@ -207,15 +252,15 @@ struct PennesOp {
             v += (ctd * ((s.w + s.e) - 2.0 * s.c)) / hh0;
             if (g.dim > 1) v += (ctd * ((s.s + s.n) - 2.0 * s.c)) / hh1;
             if (g.dim > 2) v += (ctd * ((s.d + s.u) - 2.0 * s.c)) / hh2;
-            v += (ctp * w[m]) * (ta - s.c);
-            v += ctq * q[m];
+            v += (ctp * xv[0]) * (ta - s.c);
+            v += ctq * xv[1];
             out[m] = v;
         } else {
             float d = fd0 * ((s.w + s.e) - 2.0f * s.c);
             if (g.dim > 1) d = __builtin_fmaf(fd1, (s.s + s.n) - 2.0f * s.c, d);
             if (g.dim > 2) d = __builtin_fmaf(fd2, (s.d + s.u) - 2.0f * s.c, d);
-            d = __builtin_fmaf(fp * w[m], fta - s.c, d);
-            d = __builtin_fmaf(fq, q[m], d);
+            d = __builtin_fmaf(fp * xv[0], fta - s.c, d);
+            d = __builtin_fmaf(fq, xv[1], d);
             out[m] = s.c + d;
         }
     }
@@ -225,34 +270,35 @@ struct PennesOp {
 template <typename T>
 struct PennesAdjOp {
     static constexpr bool kTwo = false;
+    static constexpr int kNX = 2;
     const T* a;  // lambda^n
     const T* b;
+    const T* x[2];  // perfusion w, previous forward state T^{n-1}
     T* out;      // lambda^{n-1}
-    const T* w;
-    const T* tprev;
     double* grad;
     double ctd, ctp, ta, hh0, hh1, hh2;
     float fd0, fd1, fd2, fp;
-    __device__ void apply(const Geo& g, int i, int j, int k, const St7<T>& s, const St7<T>&) const
+    __device__ void apply(const Geo& g, int i, int j, int k, const St7<T>& s, const St7<T>&,
+                          const T* xv) const
     {
         const long m = at(g, i, j, k);
         if (is_border(g, i, j, k)) {
             out[m] = T(0);
             return;
         }
-        grad[m] += (double)s.c * (ctp * (ta - (double)tprev[m]));
+        grad[m] += (double)s.c * (ctp * (ta - (double)xv[1]));
         if constexpr (sizeof(T) == 8) {
             double v = s.c;
             v += (ctd * ((s.w + s.e) - 2.0 * s.c)) / hh0;
             if (g.dim > 1) v += (ctd * ((s.s + s.n) - 2.0 * s.c)) / hh1;
             if (g.dim > 2) v += (ctd * ((s.d + s.u) - 2.0 * s.c)) / hh2;
-            v -= (ctp * w[m]) * s.c;
+            v -= (ctp * xv[0]) * s.c;
             out[m] = v;
         } else {
             float d = fd0 * ((s.w + s.e) - 2.0f * s.c);
             if (g.dim > 1) d = __builtin_fmaf(fd1, (s.s + s.n) - 2.0f * s.c, d);
             if (g.dim > 2) d = __builtin_fmaf(fd2, (s.d + s.u) - 2.0f * s.c, d);
-            d = __builtin_fmaf(-fp * w[m], s.c, d);
+            d = __builtin_fmaf(-fp * xv[0], s.c, d);
             out[m] = s.c + d;
         }
     }
@@ -260,25 +306,29 @@ struct PennesAdjOp {
 
 // ---------------------------------------------------------------- launch
 struct TileCfg {
-    int tx, ty, zc;
+    int tx, ty, zc, d;  // tile edge x, y; z planes per workgroup; prefetch depth
 };
 
 static const TileCfg kCfgs[] = {
-    {64, 4, 64},  {64, 4, 128}, {64, 8, 64},  {64, 8, 128}, {128, 2, 64},
-    {128, 4, 64}, {32, 8, 64},  {64, 4, 32},  {64, 16, 128}, {64, 8, 256},
+    {64, 4, 64, 4},   {64, 4, 128, 4}, {64, 8, 64, 4},  {64, 8, 128, 4}, {128, 2, 64, 4},
+    {128, 4, 64, 4},  {32, 8, 64, 4},  {64, 4, 64, 2},  {64, 8, 128, 2}, {64, 16, 128, 4},
+    {64, 8, 256, 4},  {64, 4, 128, 6}, {64, 8, 128, 6}, {128, 4, 128, 6}, {64, 4, 256, 8},
 };
 static constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
-static constexpr int kDefaultCfg = 2;
+static constexpr int kDefaultCfg = 3;
+// Measured on an MI355X at 512^3 (fastest tile per heat7var mode:
+// F64 64x8 zc128 d6, F32 128x2 zc64 d4, F32+Kahan 64x16 zc128 d4).
+static constexpr int kDefaultCfgByMode[3] = {12, 4, 9};
 
-template <class Op, typename T, int TX, int TY>
+template <class Op, typename T, int TX, int TY, int D>
 static int launch_t(Ctx* c, const Op& op, const Geo& g, int zc)
 {
     const int tiles_x = ceil_div(g.nx, TX), tiles_y = ceil_div(g.ny, TY);
     const int tiles_z = ceil_div(g.nz, zc);
     const long nb = (long)tiles_x * tiles_y * tiles_z;
     if (nb > 0x7fffffffL) return fail(SCAFES_ERR_ARG, "grid too large");
-    stream_kernel<Op, T, TX, TY><<<dim3((unsigned)nb), dim3(TX, TY), 0, c->stream>>>(op, g, zc, tiles_x,
-                                                                                    tiles_y);
+    stream_kernel<Op, T, TX, TY, D><<<dim3((unsigned)nb), dim3(TX, TY), 0, c->stream>>>(op, g, zc, tiles_x,
+                                                                                       tiles_y);
     SCAFES_HIP(hipGetLastError());
     return SCAFES_OK;
 }
@@ -288,13 +338,19 @@ static int launch(Ctx* c, const Op& op, const Geo& g, int cfg)
 {
     if (cfg < 0 || cfg >= kNumCfgs) cfg = kDefaultCfg;
     const TileCfg t = kCfgs[cfg];
-    switch (t.tx * 1000 + t.ty) {
-    case 64004: return launch_t<Op, T, 64, 4>(c, op, g, t.zc);
-    case 64008: return launch_t<Op, T, 64, 8>(c, op, g, t.zc);
-    case 128002: return launch_t<Op, T, 128, 2>(c, op, g, t.zc);
-    case 128004: return launch_t<Op, T, 128, 4>(c, op, g, t.zc);
-    case 32008: return launch_t<Op, T, 32, 8>(c, op, g, t.zc);
-    case 64016: return launch_t<Op, T, 64, 16>(c, op, g, t.zc);
+    switch (t.tx * 100000 + t.ty * 100 + t.d) {
+    case 6400404: return launch_t<Op, T, 64, 4, 4>(c, op, g, t.zc);
+    case 6400804: return launch_t<Op, T, 64, 8, 4>(c, op, g, t.zc);
+    case 12800204: return launch_t<Op, T, 128, 2, 4>(c, op, g, t.zc);
+    case 12800404: return launch_t<Op, T, 128, 4, 4>(c, op, g, t.zc);
+    case 3200804: return launch_t<Op, T, 32, 8, 4>(c, op, g, t.zc);
+    case 6400402: return launch_t<Op, T, 64, 4, 2>(c, op, g, t.zc);
+    case 6400802: return launch_t<Op, T, 64, 8, 2>(c, op, g, t.zc);
+    case 6401604: return launch_t<Op, T, 64, 16, 4>(c, op, g, t.zc);
+    case 6400406: return launch_t<Op, T, 64, 4, 6>(c, op, g, t.zc);
+    case 6400806: return launch_t<Op, T, 64, 8, 6>(c, op, g, t.zc);
+    case 12800406: return launch_t<Op, T, 128, 4, 6>(c, op, g, t.zc);
+    case 6400408: return launch_t<Op, T, 64, 4, 8>(c, op, g, t.zc);
     default: return fail(SCAFES_ERR_ARG, "bad tile configuration");
     }
 }
@@ -348,7 +404,7 @@ int heat7var_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, cons
     op.a = (const float*)u;
     op.b = (const float*)kap;
     op.out = (float*)un;
-    op.cin = (const float*)cin;
+    op.x[0] = (const float*)cin;
     op.cout = (float*)cout;
     op.c0 = c0;
     op.c1 = c1;
@@ -367,8 +423,8 @@ int pennes_step_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, 
         PennesOp<double> op{};
         op.a = (const double*)t_old;
         op.out = (double*)t_new;
-        op.w = (const double*)w;
-        op.q = (const double*)q;
+        op.x[0] = (const double*)w;
+        op.x[1] = (const double*)q;
         op.ctd = p->tau * p->cdiff;
         op.ctp = p->tau * p->cperf;
         op.ctq = p->tau * p->csrc;
@@ -381,8 +437,8 @@ int pennes_step_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, 
     PennesOp<float> op{};
     op.a = (const float*)t_old;
     op.out = (float*)t_new;
-    op.w = (const float*)w;
-    op.q = (const float*)q;
+    op.x[0] = (const float*)w;
+    op.x[1] = (const float*)q;
     op.fd0 = (float)(p->tau * p->cdiff / hh[0]);
     op.fd1 = (float)(p->tau * p->cdiff / hh[1]);
     op.fd2 = (float)(p->tau * p->cdiff / hh[2]);
@@ -402,8 +458,8 @@ int pennes_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, c
         PennesAdjOp<double> op{};
         op.a = (const double*)lam;
         op.out = (double*)lam_new;
-        op.w = (const double*)w;
-        op.tprev = (const double*)tprev;
+        op.x[0] = (const double*)w;
+        op.x[1] = (const double*)tprev;
         op.grad = grad;
         op.ctd = p->tau * p->cdiff;
         op.ctp = p->tau * p->cperf;
@@ -416,8 +472,8 @@ int pennes_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, c
     PennesAdjOp<float> op{};
     op.a = (const float*)lam;
     op.out = (float*)lam_new;
-    op.w = (const float*)w;
-    op.tprev = (const float*)tprev;
+    op.x[0] = (const float*)w;
+    op.x[1] = (const float*)tprev;
     op.grad = grad;
     op.ctp = p->tau * p->cperf;
     op.ta = p->t_a;
@@ -509,12 +565,13 @@ int scafes_heat7_step_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const d
 
 int scafes_heat7var_num_cfgs(void) { return kNumCfgs; }
 
-int scafes_heat7var_cfg_desc(int cfg, int* tx, int* ty, int* zchunk)
+int scafes_heat7var_cfg_desc(int cfg, int* tx, int* ty, int* zchunk, int* depth)
 {
     if (cfg < 0 || cfg >= kNumCfgs) return fail(SCAFES_ERR_ARG, "bad cfg");
     if (tx) *tx = kCfgs[cfg].tx;
     if (ty) *ty = kCfgs[cfg].ty;
     if (zchunk) *zchunk = kCfgs[cfg].zc;
+    if (depth) *depth = kCfgs[cfg].d;
     return SCAFES_OK;
 }
 
@@ -535,7 +592,7 @@ int scafes_heat7var_run(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode,
     Geo geo;
     rc = make_geo(g, hw, &geo);
     if (rc) return rc;
-    if (cfg < 0) cfg = c->tuned[mode] >= 0 ? c->tuned[mode] : kDefaultCfg;
+    if (cfg < 0) cfg = c->tuned[mode] >= 0 ? c->tuned[mode] : kDefaultCfgByMode[mode];
     void *a = u, *b = u2, *ca = cmp, *cb = cmp2;
     for (int s = 0; s < nsteps; ++s) {
         rc = heat7var_launch(c, geo, g, mode, a, b, ca, cb, kappa, tau, border_mode, border_value, cfg);
