@@ -40,7 +40,10 @@ extern "C" {
 
 typedef struct scafes_ctx scafes_ctx;
 
-enum { SCAFES_F32 = 0, SCAFES_F64 = 1 };
+/* Arithmetic of a field.  F32_KAHAN: FP32 values plus an FP32 compensation
+ * field that carries the rounding of the accumulated per-step increments
+ * (value ~ hi + lo). */
+enum { SCAFES_F32 = 0, SCAFES_F64 = 1, SCAFES_F32_KAHAN = 2 };
 
 /* Border rule at GLOBAL_BORDER nodes (GridGlobal.hpp:138-151 flags):
  * KEEP  = the hook leaves the copied old value (Problem.hpp:6226-6256),
@@ -150,6 +153,11 @@ typedef struct {
 int scafes_pennes_step(scafes_ctx* ctx, const scafes_geom* g, int hw, int dtype,
                        const void* t_old, void* t_new, const void* w, const void* q,
                        const scafes_pennes_params* p);
+/* FP32 step with Kahan-compensated accumulation: (t_old + c_old) ->
+ * (t_new + c_new); the increment is formed from t_old. */
+int scafes_pennes_step_kahan(scafes_ctx* ctx, const scafes_geom* g, int hw, const float* t_old,
+                             float* t_new, const float* c_old, float* c_new, const float* w,
+                             const float* q, const scafes_pennes_params* p);
 /* Discrete adjoint step (replaces ADOL-C fov_forward, Problem.hpp:6632-6747):
  *   lam'_i = lam_i + sum_p tau*cdiff*(lam_W+lam_E-2lam_i)/h_p^2 - tau*cperf*w_i*lam_i
  *   grad_i += lam_i * tau*cperf*(T_a - t_prev_i)          (inner nodes; FP64 grad)
@@ -160,9 +168,12 @@ int scafes_pennes_adjoint_step(scafes_ctx* ctx, const scafes_geom* g, int hw, in
 /* Gradient of J = 0.5 sum_inner (T^N - target)^2 w.r.t. the perfusion field
  * w by the reverse-time adjoint with binomial checkpointing (Griewank's
  * treeverse schedule) using at most max_snapshots stored states.
- * t0 is read only; grad (FP64, layout of T) is overwritten.  stats (nullable,
- * 4 entries): forward steps executed, adjoint steps, snapshots used,
- * peak checkpoint bytes. */
+ * dtype F64, F32, or F32_KAHAN (FP32 fields, compensated forward whose
+ * checkpoints store (hi, lo) pairs; the adjoint field is plain FP32).  For
+ * FP32 modes t0, w, q, target are FP32 (t0 without compensation).
+ * t0 is read only; grad (FP64, layout of T) is overwritten (ghost layers
+ * zero).  stats (nullable, 4 entries): forward steps executed, adjoint
+ * steps, snapshots used, checkpoint bytes. */
 int scafes_pennes_gradient(scafes_ctx* ctx, const scafes_geom* g, int hw, int dtype,
                            const void* t0, const void* w, const void* q, const void* target,
                            const scafes_pennes_params* p, int nsteps, int max_snapshots,
@@ -170,35 +181,43 @@ int scafes_pennes_gradient(scafes_ctx* ctx, const scafes_geom* g, int hw, int dt
 /* Number of forward steps the treeverse schedule runs for (nsteps, snaps). */
 long scafes_checkpoint_forward_steps(int nsteps, int snaps);
 
-/* ---- Yee FDTD, PEC box, vacuum, FP32 ----------------------------------
+/* ---- Yee FDTD, PEC box, vacuum ------------------------------------------
  * Leapfrog of the reference's two-phase mode (updateInner then
  * updateInner2, Problem.hpp:2603-2619; curl stencil of
- * examples/EMFDTD/stencils/CompleteCPMLCurlStencil.hpp:38-80 without CPML).
- * f: six FP32 fields Ex,Ey,Ez,Hx,Hy,Hz, each (ncell+1)^3 nodes, x fastest.
- * energy (nullable, FP64 host array of nsteps/energy_every entries): discrete
- * energy 0.5 sum E^n.E^n + 0.5 sum H^{n-1/2}.H^{n+1/2}, accumulated in
- * FP64 inside the H update of every energy_every-th step.
- * h_prev: scratch of 3 (ncell+1)^3 FP32 fields (needed only with energy). */
-int scafes_yee_run(scafes_ctx* ctx, int ncell, double s, float* f, int nsteps, int fused,
+ * examples/EMFDTD/stencils/CompleteCPMLCurlStencil.hpp:38-80 without CPML):
+ *   H += s*(forward differences of E);  E += s*(backward differences of H);
+ * tangential E stays zero on the walls.  f: six fields Ex,Ey,Ez,Hx,Hy,Hz of
+ * dtype (F32 stream, or F64 in the CPU path's operation order), each
+ * (ncell+1)^3 nodes, x fastest, updated in place over nsteps steps.
+ * s = Courant number tau/h in (0, 1/sqrt(3)].
+ * energy (host FP64 array of ceil(nsteps/energy_every) entries, used when
+ * energy_every > 0): discrete energy 0.5 sum E^n.E^n + 0.5 sum
+ * H^{n-1/2}.H^{n+1/2} of steps 0, energy_every, ..., accumulated in FP64
+ * inside the H half step. */
+int scafes_yee_run(scafes_ctx* ctx, int ncell, int dtype, double s, void* f, int nsteps,
                    int energy_every, double* energy);
-int scafes_yee_init_pulse(scafes_ctx* ctx, int ncell, float* f, double sigma);
+/* Initial state of the Yee workload: all fields zero except
+ * Ez = exp(-r^2/(2 sigma^2)) at inner Ez nodes, r from the box centre in
+ * cell units (Ez(i,j,k) sits at (i, j, k+1/2)). */
+int scafes_yee_init_pulse(scafes_ctx* ctx, int ncell, int dtype, void* f, double sigma);
 
 /* ---- batched ensemble of 2-D heat problems -----------------------------
- * nprob independent n_i x n_i problems (Dirichlet-zero border), scalar
- * kappa_i and tau_i, nsteps explicit steps each, FP32, in ONE persistent
- * kernel launch (work queue over problems sorted by size).  Problem i's
- * state lives at u + off[i] (n_i^2 floats).  With target != NULL also
- * dJ_i/dkappa_i of J_i = 0.5 sum_inner (u_i^N - target_i)^2 in FP64 via
- * the commuting-operator adjoint identity dJ/dkappa = N tau <r, L u^{N-1}>.
- * n, off, kappa, tau, order: device arrays (order = processing order).
- * J, grad: device FP64 arrays of nprob (nullable when target == NULL).
- * scratch: device FP32 array of size sum n_i^2 for problems too large for
- * LDS.  threads: workgroup size (256..1024).  graph != 0 records the launch
- * into a hipGraph once per context and replays it. */
+ * The reference runs one problem per process; this runs nprob independent
+ * n_i x n_i problems (unit square, h_i = 1/(n_i-1), zero Dirichlet border,
+ * scalar kappa_i and tau_i, the 2-D HeatEqnFDM update) for nsteps explicit
+ * FP32 steps each in ONE persistent kernel launch.  Problem i's state is
+ * n_i^2 floats (x fastest) at u + off[i], updated in place; 2 <= n_i <= 256
+ * (scafes_batch2d_lds_limit).  order: processing order, best sorted by
+ * descending ceil(n_i/32)^2 (workgroups take it in snake order).
+ * With target != NULL (same packing as u) also, in FP64,
+ * J_i = 0.5 sum_inner (u_i^N - target_i)^2 and dJ_i/dkappa_i
+ * = N tau_i <u_i^N - target_i, L u_i^{N-1}> (exact discrete adjoint for a
+ * scalar kappa with fixed tau: the step operator commutes with L).
+ * n, off, kappa, tau, order, u, target, J, grad: device arrays.
+ * use_graph != 0 captures the launch into a hipGraph and replays it. */
 int scafes_batch2d_run(scafes_ctx* ctx, int nprob, const int* n, const long* off,
                        const float* kappa, const float* tau, const int* order, float* u,
-                       float* scratch, const float* target, double* J, double* grad,
-                       int nsteps, int threads, int use_graph);
+                       const float* target, double* J, double* grad, int nsteps, int use_graph);
 int scafes_batch2d_lds_limit(int* max_n);
 
 #ifdef __cplusplus

@@ -1,0 +1,68 @@
+"""torch.autograd.Function over the Pennes adjoint of the C ABI.
+
+J(w) = 0.5 * sum_inner (T^N(w) - target)^2 for the Pennes bioheat forward
+model with perfusion field w; backward returns dJ/dw from the reverse-time
+adjoint with binomial checkpointing (scafes_pennes_gradient).  Tensors are
+host (CPU) tensors in the memAll layout (ghost width 1, shape (z, y, x)
+with n+2 per axis); the device work runs on the given capi.Context.
+The FP32 mode streams the deviation theta = T - T_a on the device.
+"""
+import numpy as np
+import torch
+
+from . import capi
+
+
+class _PennesJ(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, w, t0, q, target, runner):
+        J, grad = runner._run(w.detach().cpu().numpy(), t0.detach().cpu().numpy(), q.detach().cpu().numpy(),
+                              target.detach().cpu().numpy())
+        ctx.save_for_backward(torch.from_numpy(grad))
+        return torch.tensor(J, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, gout):
+        (grad,) = ctx.saved_tensors
+        return gout * grad, None, None, None, None
+
+
+class PennesMisfit:
+    """Callable J(w, T0, Q, target) with an adjoint backward pass."""
+
+    def __init__(self, context, n, tau, cdiff, cperf, csrc, t_a, nsteps, snapshots=16, dtype="f32_kahan"):
+        """dtype: "f64", "f32" or "f32_kahan" (FP32 fields with a compensated forward)."""
+        self.ctx = context
+        self.n = tuple(n)
+        self.nsteps = int(nsteps)
+        self.snapshots = int(snapshots)
+        self.f64 = dtype == "f64"
+        self.code = {"f64": capi.F64, "f32": capi.F32, "f32_kahan": capi.F32_KAHAN}[dtype]
+        self.t_a = float(t_a)
+        self.shift = 0.0 if self.f64 else self.t_a
+        self.params = capi.PennesParams(tau=tau, cdiff=cdiff, cperf=cperf, csrc=csrc, t_a=self.t_a - self.shift)
+        self.geom = capi.geom(self.n)
+        self.last_stats = None
+
+    def _run(self, w, t0, q, target):
+        dt = np.float64 if self.f64 else np.float32
+        code = self.code
+        bufs = [self.ctx.upload(np.ascontiguousarray(a, dtype=dt))
+                for a in (t0 - self.shift, w, q, target - self.shift)]
+        g = self.ctx.alloc(t0.size, np.float64)
+        J = capi.C.c_double(0.0)
+        stats = np.zeros(4)
+        try:
+            capi.check(self.ctx.L.scafes_pennes_gradient(
+                self.ctx.h, capi.C.byref(self.geom), 1, code, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, bufs[3].ptr,
+                capi.C.byref(self.params), self.nsteps, self.snapshots, g.ptr, capi.C.byref(J),
+                stats.ctypes.data_as(capi.C.POINTER(capi.C.c_double))))
+            grad = g.read().reshape(t0.shape)
+        finally:
+            for b in bufs + [g]:
+                b.free()
+        self.last_stats = stats
+        return J.value, grad
+
+    def __call__(self, w, t0, q, target):
+        return _PennesJ.apply(w, t0, q, target, self)

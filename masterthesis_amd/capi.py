@@ -13,7 +13,7 @@ import numpy as np
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libscafes_hip.so")
 
-F32, F64 = 0, 1
+F32, F64, F32_KAHAN = 0, 1, 2
 BORDER_KEEP, BORDER_FIELD, BORDER_VALUE = 0, 1, 2
 HEATVAR_F64, HEATVAR_F32, HEATVAR_F32_KAHAN = 0, 1, 2
 RED_MAX_ABS_DIFF, RED_SUM_SQ_DIFF, RED_SUM_PROD = 0, 1, 2
@@ -86,16 +86,17 @@ def load():
     _sig(L, "scafes_pennes_step", [vp, gp, C.c_int, C.c_int, vp, vp, vp, vp, C.POINTER(PennesParams)])
     _sig(L, "scafes_pennes_adjoint_step", [vp, gp, C.c_int, C.c_int, vp, vp, vp, vp, vp,
                                            C.POINTER(PennesParams)])
+    _sig(L, "scafes_pennes_step_kahan", [vp, gp, C.c_int, vp, vp, vp, vp, vp, vp, C.POINTER(PennesParams)])
     if hasattr(L, "scafes_pennes_gradient"):
         _sig(L, "scafes_pennes_gradient", [vp, gp, C.c_int, C.c_int, vp, vp, vp, vp,
                                            C.POINTER(PennesParams), C.c_int, C.c_int, vp, dp, dp])
         _sig(L, "scafes_checkpoint_forward_steps", [C.c_int, C.c_int], C.c_long)
     if hasattr(L, "scafes_yee_run"):
-        _sig(L, "scafes_yee_run", [vp, C.c_int, C.c_double, vp, C.c_int, C.c_int, C.c_int, dp])
-        _sig(L, "scafes_yee_init_pulse", [vp, C.c_int, vp, C.c_double])
+        _sig(L, "scafes_yee_run", [vp, C.c_int, C.c_int, C.c_double, vp, C.c_int, C.c_int, dp])
+        _sig(L, "scafes_yee_init_pulse", [vp, C.c_int, C.c_int, vp, C.c_double])
     if hasattr(L, "scafes_batch2d_run"):
-        _sig(L, "scafes_batch2d_run", [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
-                                       C.c_int, C.c_int, C.c_int])
+        _sig(L, "scafes_batch2d_run", [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                       C.c_int, C.c_int])
         _sig(L, "scafes_batch2d_lds_limit", [ip])
     _lib = L
     return L

@@ -1,0 +1,289 @@
+// adjoint.hip -- gradient of a final-time misfit w.r.t. the Pennes perfusion
+// field by the reverse-time discrete adjoint with binomial checkpointing.
+//
+// The reference differentiates with ADOL-C: one tape of the update phase
+// replayed in forward vector mode w.r.t. a few geometry parameters
+// (src/ScaFES_Problem.hpp:5841-5967 tape, :6632-6747 fov_forward).  A
+// gradient w.r.t. a coefficient FIELD needs reverse mode: J = 0.5 sum_inner
+// (T^N - target)^2, lam^N = T^N - target, then for n = N..1
+//   grad += lam^n * dT^n/dw (needs T^{n-1}),  lam^{n-1} = (dT^n/dT^{n-1})^T lam^n.
+// T^{n-1} is needed in reverse order; it is regenerated from at most
+// `snaps` stored states with Griewank's binomial (treeverse) schedule:
+// l steps and s snapshots are split so that the right part runs with s-1
+// snapshots and the left part later with s, taking the right part as
+// beta(s-1, r) = C(s-1+r, s-1) steps where r is the smallest repetition
+// number with C(s+r, s) >= l.  Forward cost ~ r*N - C(s+1, r-1) steps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "scafes_common.hpp"
+
+namespace scafes {
+
+// beta(s, r) = C(s + r, s), saturating.
+static long beta(int s, int r)
+{
+    if (s < 0 || r < 0) return 0;
+    double v = 1.0;
+    for (int i = 1; i <= s; ++i) {
+        v = v * (r + i) / i;
+        if (v > 4e18) return (long)4e18;
+    }
+    return (long)(v + 0.5);
+}
+
+// Number of steps the right sub-range takes when l steps are reversed with
+// s spare snapshots.
+static long right_part(long l, int s)
+{
+    if (s <= 0 || l <= 1) return 0;
+    int r = 0;
+    while (beta(s, r) < l) ++r;
+    return std::min(beta(s - 1, r), l - 1);
+}
+
+// lam^N = T^N - target on inner nodes (T^N = hi [+ lo]), 0 elsewhere; the
+// second adjoint buffer and the gradient start at zero.
+template <typename T>
+__global__ void lam_init_kernel(Geo g, const T* __restrict__ t, const T* __restrict__ tlo,
+                                const T* __restrict__ target, T* __restrict__ lam, T* __restrict__ lam2,
+                                double* __restrict__ grad)
+{
+    const long total = (long)g.nx * g.ny * g.nz;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(q % g.nx);
+        const long r = q / g.nx;
+        const int j = (int)(r % g.ny), k = (int)(r / g.ny);
+        const long m = at(g, i, j, k);
+        double v = (double)t[m] - (double)target[m];
+        if (tlo) v += (double)tlo[m];
+        lam[m] = is_border(g, i, j, k) ? T(0) : (T)v;
+        lam2[m] = T(0);
+        grad[m] = 0.0;
+    }
+}
+
+// A forward state is one array (F32, F64) or a (hi, lo) pair stored as two
+// consecutive arrays of ne elements (F32_KAHAN).
+struct Rev {
+    Ctx* c;
+    Geo geo;
+    const scafes_geom* g;
+    int dtype;     // state arithmetic
+    int lam_dtype; // adjoint field arithmetic (F32 or F64)
+    long ne;       // elements per array
+    size_t sbytes; // bytes per state
+    const void* w;
+    const void* q;
+    const void* target;
+    const scafes_pennes_params* p;
+    double* grad;
+    std::vector<void*> slots;  // snapshot states (free list)
+    void* tmp[2];              // forward ping-pong states
+    void* work;                // recomputed state when no snapshot is free
+    void* lam[2];
+    int lcur;
+    int nsteps;
+    long fwd_steps, adj_steps;
+    double J;
+
+    bool kahan() const { return dtype == SCAFES_F32_KAHAN; }
+    const void* lo(const void* s) const { return kahan() ? (const void*)((const float*)s + ne) : nullptr; }
+    void* lo(void* s) const { return kahan() ? (void*)((float*)s + ne) : nullptr; }
+
+    int step(const void* a, void* b)
+    {
+        ++fwd_steps;
+        return pennes_step_launch(c, geo, g, dtype, a, b, w, q, p, lo(a), lo(b));
+    }
+    int copy(void* dst, const void* src)
+    {
+        hipError_t e = hipMemcpyAsync(dst, src, sbytes, hipMemcpyDeviceToDevice, c->stream);
+        return e == hipSuccess ? SCAFES_OK : check_hip(e, "checkpoint copy");
+    }
+    // State after n steps from src, written to dst (src is never written).
+    int advance(const void* src, long n, void* dst)
+    {
+        if (n == 0) return src == dst ? SCAFES_OK : copy(dst, src);
+        const void* x = src;
+        for (long s = 1; s <= n; ++s) {
+            void* out = (s == n) ? dst : tmp[s & 1];
+            if (out == x) out = tmp[(s + 1) & 1];
+            int r = step(x, out);
+            if (r) return r;
+            x = out;
+        }
+        return x == dst ? SCAFES_OK : copy(dst, x);
+    }
+    // Adjoint step n (lam^n -> lam^{n-1}) with T^{n-1} = tprev.
+    int adjoint(int n, const void* tprev)
+    {
+        if (n == nsteps) {
+            // first leaf: T^N from T^{N-1}, then lam^N and J
+            int r = step(tprev, tmp[0]);
+            if (r) return r;
+            const long total = (long)geo.nx * geo.ny * geo.nz;
+            const int blocks = (int)std::min<long>((total + 255) / 256, 65536);
+            if (lam_dtype == SCAFES_F64)
+                lam_init_kernel<double><<<blocks, 256, 0, c->stream>>>(geo, (const double*)tmp[0], nullptr,
+                                                                       (const double*)target, (double*)lam[0],
+                                                                       (double*)lam[1], grad);
+            else
+                lam_init_kernel<float><<<blocks, 256, 0, c->stream>>>(geo, (const float*)tmp[0],
+                                                                      (const float*)lo(tmp[0]), (const float*)target,
+                                                                      (float*)lam[0], (float*)lam[1], grad);
+            SCAFES_HIP(hipGetLastError());
+            r = reduce_nodes(c, geo, geo, lam_dtype, lam[0], nullptr, RED_SUM_SQ_DIFF, 1, &J);
+            if (r) return r;
+            J *= 0.5;
+        }
+        ++adj_steps;
+        int r = pennes_adj_launch(c, geo, g, lam_dtype, lam[lcur], lam[1 - lcur], w, tprev, grad, p, lo(tprev));
+        lcur = 1 - lcur;
+        return r;
+    }
+    // Reverse steps a+1..b given T^a in `base`; s snapshots may be taken.
+    int treeverse(long a, long b, int s, const void* base)
+    {
+        if (b - a == 1) return adjoint((int)b, base);
+        if (s <= 0 || slots.empty()) {
+            for (long n = b; n > a; --n) {
+                const void* tp = base;
+                if (n - 1 > a) {
+                    int r = advance(base, n - 1 - a, work);
+                    if (r) return r;
+                    tp = work;
+                }
+                int r = adjoint((int)n, tp);
+                if (r) return r;
+            }
+            return SCAFES_OK;
+        }
+        const long right = right_part(b - a, s);
+        const long m = b - right;
+        void* slot = slots.back();
+        slots.pop_back();
+        int r = advance(base, m - a, slot);
+        if (!r) r = treeverse(m, b, s - 1, slot);
+        slots.push_back(slot);
+        if (r) return r;
+        return treeverse(a, m, s, base);
+    }
+};
+
+// Counts the forward steps of the schedule without running it.
+static long count_forward(long a, long b, int s, int nsteps)
+{
+    if (b - a == 1) return (b == nsteps) ? 1 : 0;
+    if (s <= 0) {
+        long n = 0;
+        for (long k = b; k > a; --k) n += (k - 1 - a) + ((k == nsteps) ? 1 : 0);
+        return n;
+    }
+    const long right = right_part(b - a, s);
+    const long m = b - right;
+    return (m - a) + count_forward(m, b, s - 1, nsteps) + count_forward(a, m, s, nsteps);
+}
+
+}  // namespace scafes
+
+using namespace scafes;
+
+extern "C" {
+
+long scafes_checkpoint_forward_steps(int nsteps, int snaps)
+{
+    if (nsteps < 1) return 0;
+    return count_forward(0, nsteps, std::max(0, std::min(snaps, nsteps - 1)), nsteps);
+}
+
+int scafes_pennes_gradient(scafes_ctx* ctx, const scafes_geom* g, int hw, int dtype, const void* t0, const void* w,
+                           const void* q, const void* target, const scafes_pennes_params* p, int nsteps,
+                           int max_snapshots, double* grad, double* J, double* stats)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || !g || !t0 || !w || !q || !target || !p || !grad || !J)
+        return fail(SCAFES_ERR_ARG, "pennes gradient: null pointer");
+    if (nsteps < 1) return fail(SCAFES_ERR_ARG, "pennes gradient: need nsteps >= 1");
+    if (dtype != SCAFES_F32 && dtype != SCAFES_F64 && dtype != SCAFES_F32_KAHAN)
+        return fail(SCAFES_ERR_ARG, "pennes gradient: bad dtype");
+    if (hw < 1 || g->nlayers < 1) return fail(SCAFES_ERR_ARG, "pennes gradient: need hw >= 1 and nlayers >= 1");
+    Geo geo;
+    int rc = make_geo(g, hw, &geo);
+    if (rc) return rc;
+    long ne = 1;
+    for (int p2 = 0; p2 < 3; ++p2) ne *= (p2 < g->dim) ? (g->n[p2] + 2 * hw) : 1;
+    const size_t es = dtype == SCAFES_F64 ? 8 : 4;
+    const size_t abytes = (size_t)ne * es;  // one array
+    const int snaps = std::max(0, std::min(max_snapshots, nsteps - 1));
+    Rev R;
+    R.c = c;
+    R.geo = geo;
+    R.g = g;
+    R.dtype = dtype;
+    R.lam_dtype = dtype == SCAFES_F64 ? SCAFES_F64 : SCAFES_F32;
+    R.ne = ne;
+    R.sbytes = dtype == SCAFES_F32_KAHAN ? 2 * abytes : abytes;
+    R.w = w;
+    R.q = q;
+    R.target = target;
+    R.p = p;
+    R.grad = grad;
+    R.lcur = 0;
+    R.nsteps = nsteps;
+    R.fwd_steps = R.adj_steps = 0;
+    R.J = 0.0;
+    R.work = nullptr;
+    std::vector<void*> bufs;
+    auto take = [&](void** out, size_t bytes) -> int {
+        hipError_t e = hipMalloc(out, bytes);
+        if (e != hipSuccess) return check_hip(e, "pennes gradient buffers");
+        bufs.push_back(*out);
+        return SCAFES_OK;
+    };
+    // the initial state as a state (a (t0, 0) pair in the compensated mode)
+    const void* base = t0;
+    void* base_buf = nullptr;
+    if (dtype == SCAFES_F32_KAHAN) {
+        rc = take(&base_buf, R.sbytes);
+        if (!rc) {
+            hipError_t e = hipMemcpyAsync(base_buf, t0, abytes, hipMemcpyDeviceToDevice, c->stream);
+            if (e == hipSuccess) e = hipMemsetAsync((char*)base_buf + abytes, 0, abytes, c->stream);
+            if (e != hipSuccess) rc = check_hip(e, "pennes gradient init");
+        }
+        base = base_buf;
+    }
+    if (!rc) rc = take(&R.tmp[0], R.sbytes);
+    if (!rc) rc = take(&R.tmp[1], R.sbytes);
+    if (!rc) rc = take(&R.work, R.sbytes);
+    if (!rc) rc = take(&R.lam[0], abytes);
+    if (!rc) rc = take(&R.lam[1], abytes);
+    for (int s = 0; !rc && s < snaps; ++s) {
+        void* b = nullptr;
+        rc = take(&b, R.sbytes);
+        if (!rc) R.slots.push_back(b);
+    }
+    if (!rc) {
+        hipError_t e = hipMemsetAsync(grad, 0, (size_t)ne * sizeof(double), c->stream);
+        if (e != hipSuccess) rc = check_hip(e, "pennes gradient zero");
+    }
+    if (!rc) rc = R.treeverse(0, nsteps, (int)R.slots.size(), base);
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = check_hip(e, "pennes gradient");
+    }
+    for (size_t i = 0; i < bufs.size(); ++i) (void)hipFree(bufs[i]);
+    if (rc) return rc;
+    *J = R.J;
+    if (stats) {
+        stats[0] = (double)R.fwd_steps;
+        stats[1] = (double)R.adj_steps;
+        stats[2] = (double)snaps;
+        stats[3] = (double)snaps * (double)R.sbytes;
+    }
+    return SCAFES_OK;
+}
+
+}  // extern "C"

@@ -85,14 +85,15 @@ __device__ inline int xcd_remap(int b, int nb)
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
-// Streaming store of a stencil output: written once per sweep and read only
-// by the next sweep, so it is marked non-temporal to keep L2 for the
-// neighbour-tile halo reuse of the inputs.
-template <typename T>
-__device__ inline void stream_store(T* p, T v)
-{
-    __builtin_nontemporal_store(v, p);
-}
+// Pennes forward / adjoint launches (defined in stencil3d.hip).
+// dtype SCAFES_F32_KAHAN: compensated FP32 forward (cin/cout compensation fields);
+// tprev_lo != null: the adjoint reads the forward state as a (hi, lo) pair.
+int pennes_step_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, const void* t_old,
+                       void* t_new, const void* w, const void* q, const scafes_pennes_params* p,
+                       const void* cin, void* cout);
+int pennes_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, const void* lam,
+                      void* lam_new, const void* w, const void* tprev, double* grad,
+                      const scafes_pennes_params* p, const void* tprev_lo);
 
 // Reductions (defined in scafes_capi.hip).
 enum RedOp { RED_MAX_ABS_DIFF = 0, RED_SUM_SQ_DIFF = 1, RED_SUM_PROD = 2 };

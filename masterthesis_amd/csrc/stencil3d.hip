@@ -229,14 +229,17 @@ struct HeatVarOp {
 
 // Pennes bioheat, TumorHeatEqnFDM::updateInner form with node fields w, Q
 // (examples/TumorHeatEqnFDM/TumorHeatEqnFDM.hpp:258-267); border nodes keep.
-template <typename T>
+// KAHAN (FP32 only): the per-step increment is accumulated with a
+// persistent FP32 compensation field (T_true ~ out + cout).
+template <typename T, bool KAHAN = false>
 struct PennesOp {
     static constexpr bool kTwo = false;
-    static constexpr int kNX = 2;
+    static constexpr int kNX = KAHAN ? 3 : 2;
     const T* a;  // T_old
     const T* b;
-    const T* x[2];  // perfusion w, source Q
+    const T* x[3];  // perfusion w, source Q, incoming compensation (KAHAN)
     T* out;
+    T* cout;
     double ctd, ctp, ctq, ta, hh0, hh1, hh2;  // FP64 path
     float fd0, fd1, fd2, fp, fq, fta;         // FP32 path
     __device__ void apply(const Geo& g, int i, int j, int k, const St7<T>& s, const St7<T>&,
@@ -245,6 +248,7 @@ struct PennesOp {
         const long m = at(g, i, j, k);
         if (is_border(g, i, j, k)) {
             out[m] = s.c;
+            if constexpr (KAHAN) cout[m] = xv[2];
             return;
         }
         if constexpr (sizeof(T) == 8) {
@@ -261,20 +265,28 @@ struct PennesOp {
             if (g.dim > 2) d = __builtin_fmaf(fd2, (s.d + s.u) - 2.0f * s.c, d);
             d = __builtin_fmaf(fp * xv[0], fta - s.c, d);
             d = __builtin_fmaf(fq, xv[1], d);
-            out[m] = s.c + d;
+            if constexpr (KAHAN) {
+                const float y = d + xv[2];
+                const float t = s.c + y;
+                cout[m] = y - (t - s.c);
+                out[m] = t;
+            } else {
+                out[m] = s.c + d;
+            }
         }
     }
 };
 
 // Discrete adjoint of PennesOp (inner nodes), FP64 gradient accumulation.
-template <typename T>
+// LO: the forward state is a (hi, lo) pair from the compensated forward.
+template <typename T, bool LO = false>
 struct PennesAdjOp {
     static constexpr bool kTwo = false;
-    static constexpr int kNX = 2;
+    static constexpr int kNX = LO ? 3 : 2;
     const T* a;  // lambda^n
     const T* b;
-    const T* x[2];  // perfusion w, previous forward state T^{n-1}
-    T* out;      // lambda^{n-1}
+    const T* x[3];  // perfusion w, previous forward state T^{n-1} (hi, lo)
+    T* out;         // lambda^{n-1}
     double* grad;
     double ctd, ctp, ta, hh0, hh1, hh2;
     float fd0, fd1, fd2, fp;
@@ -286,7 +298,9 @@ struct PennesAdjOp {
             out[m] = T(0);
             return;
         }
-        grad[m] += (double)s.c * (ctp * (ta - (double)xv[1]));
+        double tp = (double)xv[1];
+        if constexpr (LO) tp += (double)xv[2];
+        grad[m] += (double)s.c * (ctp * (ta - tp));
         if constexpr (sizeof(T) == 8) {
             double v = s.c;
             v += (ctd * ((s.w + s.e) - 2.0 * s.c)) / hh0;
@@ -415,7 +429,8 @@ int heat7var_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, cons
 }
 
 int pennes_step_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, const void* t_old,
-                       void* t_new, const void* w, const void* q, const scafes_pennes_params* p)
+                       void* t_new, const void* w, const void* q, const scafes_pennes_params* p,
+                       const void* cin, void* cout)
 {
     double hh[3];
     hh_of(g, hh);
@@ -434,23 +449,34 @@ int pennes_step_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, 
         op.hh2 = hh[2];
         return launch<PennesOp<double>, double>(c, op, geo, -1);
     }
+    auto fill32 = [&](auto& op) {
+        op.a = (const float*)t_old;
+        op.out = (float*)t_new;
+        op.x[0] = (const float*)w;
+        op.x[1] = (const float*)q;
+        op.fd0 = (float)(p->tau * p->cdiff / hh[0]);
+        op.fd1 = (float)(p->tau * p->cdiff / hh[1]);
+        op.fd2 = (float)(p->tau * p->cdiff / hh[2]);
+        op.fp = (float)(p->tau * p->cperf);
+        op.fq = (float)(p->tau * p->csrc);
+        op.fta = (float)p->t_a;
+    };
+    if (dtype == SCAFES_F32_KAHAN) {
+        if (!cin || !cout) return fail(SCAFES_ERR_ARG, "pennes: compensated step needs both compensation fields");
+        PennesOp<float, true> op{};
+        fill32(op);
+        op.x[2] = (const float*)cin;
+        op.cout = (float*)cout;
+        return launch<PennesOp<float, true>, float>(c, op, geo, -1);
+    }
     PennesOp<float> op{};
-    op.a = (const float*)t_old;
-    op.out = (float*)t_new;
-    op.x[0] = (const float*)w;
-    op.x[1] = (const float*)q;
-    op.fd0 = (float)(p->tau * p->cdiff / hh[0]);
-    op.fd1 = (float)(p->tau * p->cdiff / hh[1]);
-    op.fd2 = (float)(p->tau * p->cdiff / hh[2]);
-    op.fp = (float)(p->tau * p->cperf);
-    op.fq = (float)(p->tau * p->csrc);
-    op.fta = (float)p->t_a;
+    fill32(op);
     return launch<PennesOp<float>, float>(c, op, geo, -1);
 }
 
 int pennes_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, const void* lam,
                       void* lam_new, const void* w, const void* tprev, double* grad,
-                      const scafes_pennes_params* p)
+                      const scafes_pennes_params* p, const void* tprev_lo)
 {
     double hh[3];
     hh_of(g, hh);
@@ -469,18 +495,27 @@ int pennes_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, c
         op.hh2 = hh[2];
         return launch<PennesAdjOp<double>, double>(c, op, geo, -1);
     }
+    auto fill32 = [&](auto& op) {
+        op.a = (const float*)lam;
+        op.out = (float*)lam_new;
+        op.x[0] = (const float*)w;
+        op.x[1] = (const float*)tprev;
+        op.grad = grad;
+        op.ctp = p->tau * p->cperf;
+        op.ta = p->t_a;
+        op.fd0 = (float)(p->tau * p->cdiff / hh[0]);
+        op.fd1 = (float)(p->tau * p->cdiff / hh[1]);
+        op.fd2 = (float)(p->tau * p->cdiff / hh[2]);
+        op.fp = (float)(p->tau * p->cperf);
+    };
+    if (tprev_lo) {
+        PennesAdjOp<float, true> op{};
+        fill32(op);
+        op.x[2] = (const float*)tprev_lo;
+        return launch<PennesAdjOp<float, true>, float>(c, op, geo, -1);
+    }
     PennesAdjOp<float> op{};
-    op.a = (const float*)lam;
-    op.out = (float*)lam_new;
-    op.x[0] = (const float*)w;
-    op.x[1] = (const float*)tprev;
-    op.grad = grad;
-    op.ctp = p->tau * p->cperf;
-    op.ta = p->t_a;
-    op.fd0 = (float)(p->tau * p->cdiff / hh[0]);
-    op.fd1 = (float)(p->tau * p->cdiff / hh[1]);
-    op.fd2 = (float)(p->tau * p->cdiff / hh[2]);
-    op.fp = (float)(p->tau * p->cperf);
+    fill32(op);
     return launch<PennesAdjOp<float>, float>(c, op, geo, -1);
 }
 
@@ -675,7 +710,24 @@ int scafes_pennes_step(scafes_ctx* ctx, const scafes_geom* g, int hw, int dtype,
     Geo geo;
     rc = make_geo(g, hw, &geo);
     if (rc) return rc;
-    return pennes_step_launch(c, geo, g, dtype, t_old, t_new, w, q, p);
+    if (dtype != SCAFES_F32 && dtype != SCAFES_F64) return fail(SCAFES_ERR_ARG, "pennes: dtype F32 or F64");
+    return pennes_step_launch(c, geo, g, dtype, t_old, t_new, w, q, p, nullptr, nullptr);
+}
+
+int scafes_pennes_step_kahan(scafes_ctx* ctx, const scafes_geom* g, int hw, const float* t_old, float* t_new,
+                             const float* c_old, float* c_new, const float* w, const float* q,
+                             const scafes_pennes_params* p)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || !g || !t_old || !t_new || !c_old || !c_new || !w || !q || !p)
+        return fail(SCAFES_ERR_ARG, "pennes kahan: null pointer");
+    if (t_old == t_new || c_old == c_new) return fail(SCAFES_ERR_ARG, "pennes kahan: in-place update not allowed");
+    int rc = check_stencil_geo(g, hw);
+    if (rc) return rc;
+    Geo geo;
+    rc = make_geo(g, hw, &geo);
+    if (rc) return rc;
+    return pennes_step_launch(c, geo, g, SCAFES_F32_KAHAN, t_old, t_new, w, q, p, c_old, c_new);
 }
 
 int scafes_pennes_adjoint_step(scafes_ctx* ctx, const scafes_geom* g, int hw, int dtype, const void* lam,
@@ -691,7 +743,8 @@ int scafes_pennes_adjoint_step(scafes_ctx* ctx, const scafes_geom* g, int hw, in
     Geo geo;
     rc = make_geo(g, hw, &geo);
     if (rc) return rc;
-    return pennes_adj_launch(c, geo, g, dtype, lam, lam_new, w, t_prev, grad, p);
+    if (dtype != SCAFES_F32 && dtype != SCAFES_F64) return fail(SCAFES_ERR_ARG, "pennes adjoint: dtype F32 or F64");
+    return pennes_adj_launch(c, geo, g, dtype, lam, lam_new, w, t_prev, grad, p, nullptr);
 }
 
 }  // extern "C"

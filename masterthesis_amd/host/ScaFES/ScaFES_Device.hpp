@@ -1,0 +1,155 @@
+// ScaFES_Device.hpp -- C++ side of the C ABI (include/scafes_hip.h).
+//
+// RAII wrappers over the HIP library and the kernel descriptor a problem
+// class provides so that the update phase runs on the GPU.  The reference
+// calls the update hooks per node on the CPU (src/ScaFES_Problem.hpp:5443-5466);
+// arbitrary C++ hooks cannot run on a GPU, so a problem names its update
+// family and coefficients once, by specialising ScaFES::DeviceTraits for the
+// class it passes to Problem as OWNPRBLM, next to the (unchanged) class:
+//
+//   template <typename CT, std::size_t DIM, typename Class>
+//   struct ScaFES::DeviceTraits<HeatEqnFDM<CT, DIM, Class>> {
+//       static const bool available = true;
+//       static void describe(const HeatEqnFDM<CT, DIM, Class>& p, ScaFES::DeviceUpdate& d) {...}
+//   };
+//
+// A problem without a specialisation has no device path, and
+// Problem::iterateOverTime() reports that instead of running anything on
+// the CPU.
+#ifndef SCAFES_HOST_DEVICE_HPP_
+#define SCAFES_HOST_DEVICE_HPP_
+
+#include <cstddef>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "scafes_hip.h"
+
+namespace ScaFES {
+
+enum class DeviceKernel {
+    NONE = 0,
+    HEAT7,     // HeatEqnFDM / PennesBioheatEqnFDM system form, FP64
+    HEAT7VAR,  // conservative variable-kappa heat, FP64 / FP32 / FP32+Kahan
+    PENNES,    // TumorHeatEqnFDM form with perfusion and source fields
+    YEE        // leapfrog Yee FDTD, PEC box, vacuum (updateInner = H, updateInner2 = E)
+};
+enum class DevicePrecision { F64 = 0, F32 = 1, F32_KAHAN = 2 };
+enum class DeviceBorder { KEEP = SCAFES_BORDER_KEEP, KNOWN_FIELD = SCAFES_BORDER_FIELD, VALUE = SCAFES_BORDER_VALUE };
+
+// Everything the engine needs to run a problem's update phase on the GPU.
+// Known-field indices refer to the problem's data-field list (the order of
+// the constructor's nameDatafield vector).
+struct DeviceUpdate {
+    DeviceKernel kernel = DeviceKernel::NONE;
+    DevicePrecision precision = DevicePrecision::F64;
+    int unknown = 0;  // data-field index of the updated field
+    // HEAT7: u' = u + sum_p tau*a*(W+E-2u)/h_p^2 [- tau*perfC*perfW*u] [+ tau*rhoCInv*F_old]
+    double coeffA = 1.0, rhoCInv = 1.0, perfC = 0.0, perfW = 0.0;
+    bool perfusion = false;
+    int sourceKnown = -1;  // F, read from the old known buffer (knownDfOld)
+    DeviceBorder border = DeviceBorder::KEEP;
+    int borderKnown = -1;  // G, read from the new known buffer (knownDf)
+    double borderValue = 0.0;
+    // HEAT7VAR: diffusivity field
+    int kappaKnown = -1;
+    // PENNES: T' = T + sum_p tau*cdiff*(W+E-2T)/h_p^2 + tau*cperf*w*(T_a-T) + tau*csrc*Q
+    int perfusionKnown = -1, sourceFieldKnown = -1;
+    double cdiff = 1.0, cperf = 1.0, csrc = 1.0, tA = 0.0;
+    // YEE: fields Ex,Ey,Ez,Hx,Hy,Hz are data fields yee[0..5]; Courant number
+    // tau/h; discrete energy recorded every energyEvery steps (0 = never)
+    int yee[6] = {0, 1, 2, 3, 4, 5};
+    double courant = 0.5;
+    int energyEvery = 0;
+    // Re-evaluate and upload the known fields every step (evalKnownDfs each
+    // time iterate, Problem.hpp:3433-3485); false when they do not depend on t.
+    bool knownTimeDependent = true;
+};
+
+template <class OWNPRBLM>
+struct DeviceTraits {
+    static const bool available = false;
+    static void describe(const OWNPRBLM&, DeviceUpdate&) {}
+};
+
+inline void deviceCheck(int rc, const char* what)
+{
+    if (rc != SCAFES_OK)
+        throw std::runtime_error(std::string("ScaFES device: ") + what + ": " + scafes_last_error());
+}
+
+class DeviceContext {
+public:
+    explicit DeviceContext(int device = 0) : mCtx(0)
+    {
+        deviceCheck(scafes_ctx_create(device, &mCtx), "context creation");
+    }
+    ~DeviceContext()
+    {
+        if (mCtx) scafes_ctx_destroy(mCtx);
+    }
+    DeviceContext(const DeviceContext&) = delete;
+    DeviceContext& operator=(const DeviceContext&) = delete;
+    scafes_ctx* get() const { return mCtx; }
+    void sync() const { deviceCheck(scafes_ctx_sync(mCtx), "sync"); }
+
+private:
+    scafes_ctx* mCtx;
+};
+
+class DeviceBuffer {
+public:
+    DeviceBuffer() : mCtx(0), mPtr(0), mBytes(0) {}
+    DeviceBuffer(const DeviceContext& ctx, std::size_t bytes) : mCtx(ctx.get()), mPtr(0), mBytes(bytes)
+    {
+        deviceCheck(scafes_malloc(mCtx, bytes, &mPtr), "allocation");
+    }
+    ~DeviceBuffer() { release(); }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept : mCtx(o.mCtx), mPtr(o.mPtr), mBytes(o.mBytes)
+    {
+        o.mPtr = 0;
+    }
+    DeviceBuffer& operator=(DeviceBuffer&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            mCtx = o.mCtx;
+            mPtr = o.mPtr;
+            mBytes = o.mBytes;
+            o.mPtr = 0;
+        }
+        return *this;
+    }
+    void* get() const { return mPtr; }
+    std::size_t bytes() const { return mBytes; }
+    void upload(const void* src, std::size_t bytes)
+    {
+        deviceCheck(scafes_memcpy_h2d(mCtx, mPtr, src, bytes), "upload");
+    }
+    void download(void* dst, std::size_t bytes) const
+    {
+        deviceCheck(scafes_memcpy_d2h(mCtx, dst, mPtr, bytes), "download");
+    }
+    void swap(DeviceBuffer& o)
+    {
+        std::swap(mCtx, o.mCtx);
+        std::swap(mPtr, o.mPtr);
+        std::swap(mBytes, o.mBytes);
+    }
+
+private:
+    void release()
+    {
+        if (mPtr) scafes_free(mCtx, mPtr);
+        mPtr = 0;
+    }
+    scafes_ctx* mCtx;
+    void* mPtr;
+    std::size_t mBytes;
+};
+
+}  // namespace ScaFES
+#endif

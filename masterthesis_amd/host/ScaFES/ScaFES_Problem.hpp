@@ -1,0 +1,713 @@
+// ScaFES_Problem.hpp -- the explicit time-stepping engine of the host API.
+//
+// Same constructor, hook interface and phase order as ScaFES::Problem
+// (src/ScaFES_Problem.hpp:264-1316): a CRTP base class whose subclass
+// provides evalInner/evalBorder (known fields), initInner/initBorder
+// (initial state) and updateInner/updateBorder(/2) (time step).
+// iterateOverTime() (reference :2558-2715) runs n = 0..nTimesteps:
+//   evalKnownDfs(n) -> [n==0: reset, init hooks, old<-new]
+//                      [n>0 : update phase (+ update2 phase with leapfrog)]
+//   -> compErrLinf -> checkConv -> swap old/new -> times += tau.
+// The init and eval hooks run on the host (once, or every step when the
+// known fields depend on t).  The update phase runs on the GPU through the
+// C ABI, using the kernel family the problem declares through
+// ScaFES::DeviceTraits; there is no CPU update path.
+#ifndef SCAFES_HOST_PROBLEM_HPP_
+#define SCAFES_HOST_PROBLEM_HPP_
+
+#include <algorithm>
+#include <cmath>
+#include <iomanip>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ScaFES_DataField.hpp"
+#include "ScaFES_Device.hpp"
+#include "ScaFES_GridGlobal.hpp"
+#include "ScaFES_Parameters.hpp"
+#include "ScaFES_Timer.hpp"
+
+namespace ScaFES {
+
+template <class OWNPRBLM, typename CT = double, std::size_t DIM = 3>
+class Problem {
+public:
+    typedef Ntuple<int, DIM> ScaFES_IntNtuple;
+    template <typename TT>
+    using ScaFES_VectDf = std::vector<DataField<TT, DIM>>;
+
+    Problem(const Parameters& params, const GridGlobal<DIM>& gg, bool useLeapfrog,
+            const std::vector<std::string>& nameDatafield, const std::vector<int>& stencilWidth,
+            const std::vector<bool>& isKnownDf, const std::vector<int>& nLayers = std::vector<int>(),
+            const std::vector<CT>& defaultValue = std::vector<CT>(),
+            const std::vector<WriteHowOften>& writeToFile = std::vector<WriteHowOften>(),
+            const std::vector<bool>& computeError = std::vector<bool>(),
+            const std::vector<CT>& geomparamsInit = std::vector<CT>(),
+            const std::vector<bool>& checkConvergence = std::vector<bool>())
+        : mParams(params), mGG(gg), mUseLeapfrog(useLeapfrog), mNames(nameDatafield),
+          mStencilWidth(stencilWidth), mIsKnownDf(isKnownDf), mNlayers(nLayers),
+          mDefaultValue(defaultValue), mWriteToFile(writeToFile), mComputeError(computeError),
+          mParamsCurr(geomparamsInit), mCheckConvergence(checkConvergence)
+    {
+        // The four update hooks are required even if unused (reference :2849-2862, :2895-2908).
+        (void)&OWNPRBLM::template updateInner<CT>;
+        (void)&OWNPRBLM::template updateBorder<CT>;
+        (void)&OWNPRBLM::template updateInner2<CT>;
+        (void)&OWNPRBLM::template updateBorder2<CT>;
+        const std::size_t nDf = mNames.size();
+        if (nDf == 0) throw std::runtime_error("Problem: no data fields given.");
+        if (mStencilWidth.size() != nDf || mIsKnownDf.size() != nDf)
+            throw std::runtime_error("Problem: sizes of nameDatafield, stencilWidth and isKnownDf differ.");
+        if (mNlayers.empty()) mNlayers.assign(nDf, 0);
+        if (mDefaultValue.empty()) mDefaultValue.assign(nDf, CT(0));
+        if (mWriteToFile.empty()) mWriteToFile.assign(nDf, WriteHowOften::NEVER);
+        if (mComputeError.empty()) mComputeError.assign(nDf, false);
+        if (mCheckConvergence.empty()) mCheckConvergence.assign(nDf, false);
+        if (mNlayers.size() != nDf || mDefaultValue.size() != nDf || mWriteToFile.size() != nDf ||
+            mComputeError.size() != nDf || mCheckConvergence.size() != nDf)
+            throw std::runtime_error("Problem: size of an optional per-field vector differs from nameDatafield.");
+        for (std::size_t ii = 0; ii < nDf; ++ii) {
+            if (mStencilWidth[ii] < 0) throw std::runtime_error("Problem: negative stencil width.");
+            if (mNlayers[ii] != 0)
+                throw std::runtime_error("Problem: border-only data fields (nLayers > 0) are not supported.");
+        }
+        // memAll of each field = partition +- stencil width (reference :1495-1514)
+        const Grid<DIM>& dom = mGG.discreteDomain();
+        for (std::size_t ii = 0; ii < nDf; ++ii) {
+            const ScaFES_IntNtuple sw(mStencilWidth[ii]);
+            Ntuple<double, DIM> x0 = dom.coordNodeFirst(), x1 = dom.coordNodeLast();
+            for (std::size_t p = 0; p < DIM; ++p) {
+                x0[p] -= mStencilWidth[ii] * dom.gridsize(p);
+                x1[p] += mStencilWidth[ii] * dom.gridsize(p);
+            }
+            mMemAll.push_back(Grid<DIM>(dom.idxNodeFirst() - sw, dom.idxNodeLast() + sw, x0, x1));
+        }
+        // lumps and views: unknown fields (old/new), known fields and the
+        // reference solutions of unknown fields (old/new; reference :2198-2392)
+        for (std::size_t ii = 0; ii < nDf; ++ii) {
+            const std::size_t ne = (std::size_t)mMemAll[ii].nNodes().size() + 1;
+            if (!mIsKnownDf[ii]) {
+                mUnknownIndex.push_back((int)ii);
+                mLumpUnknownOld.push_back(std::vector<CT>(ne, CT(0)));
+                mLumpUnknownNew.push_back(std::vector<CT>(ne, CT(0)));
+            }
+            const bool hasMem = mIsKnownDf[ii] || mComputeError[ii];
+            mLumpKnownOld.push_back(std::vector<CT>(hasMem ? ne : 0, CT(0)));
+            mLumpKnownNew.push_back(std::vector<CT>(hasMem ? ne : 0, CT(0)));
+        }
+        for (std::size_t u = 0; u < mUnknownIndex.size(); ++u) {
+            const int ii = mUnknownIndex[u];
+            mVectUnknownDfsDomOld.push_back(DataField<CT, DIM>(mNames[ii] + "Old", &mParams, mGG, mMemAll[ii],
+                                                               mStencilWidth[ii], mLumpUnknownOld[u].data(), 1,
+                                                               0, mWriteToFile[ii]));
+            mVectUnknownDfsDomNew.push_back(DataField<CT, DIM>(mNames[ii], &mParams, mGG, mMemAll[ii],
+                                                               mStencilWidth[ii], mLumpUnknownNew[u].data(), 1,
+                                                               0, mWriteToFile[ii]));
+        }
+        for (std::size_t ii = 0; ii < nDf; ++ii) {
+            const int sw = mIsKnownDf[ii] ? mStencilWidth[ii] : 0;
+            CT* po = mLumpKnownOld[ii].empty() ? 0 : mLumpKnownOld[ii].data();
+            CT* pn = mLumpKnownNew[ii].empty() ? 0 : mLumpKnownNew[ii].data();
+            mVectKnownDfsDomOld.push_back(DataField<CT, DIM>(mNames[ii] + "OldSolDom", &mParams, mGG,
+                                                             mMemAll[ii], sw, po, 1, 0, mWriteToFile[ii]));
+            mVectKnownDfsDomNew.push_back(DataField<CT, DIM>(mNames[ii] + "SolDom", &mParams, mGG,
+                                                             mMemAll[ii], sw, pn, 1, 0, mWriteToFile[ii]));
+        }
+        // node-kind flags (reference setGridGlobalFlags :3902-3980)
+        const Grid<DIM> kindGrid(dom.idxNodeFirst(), dom.idxNodeLast(), dom.coordNodeFirst(), dom.coordNodeLast());
+        mLumpKind.assign((std::size_t)kindGrid.nNodes().size() + 1, 0);
+        mKind = DataField<short, DIM>("kind", &mParams, mGG, kindGrid, 0, mLumpKind.data());
+        const int nl = mParams.nLayersAtBorder();
+        const GridSub<DIM>& part = mGG.partition(0);
+        for (typename GridSub<DIM>::iterator it = part.begin(); it != part.end(); ++it) {
+            short k = GridGlobal<DIM>::SCAFES_GLOBAL_INNER_NODE;
+            for (std::size_t p = 0; p < DIM; ++p)
+                if (dom.idxNodeFirst(p) + nl > (*it).elem(p) || dom.idxNodeLast(p) - nl < (*it).elem(p)) {
+                    k = GridGlobal<DIM>::SCAFES_GLOBAL_BORDER_NODE;
+                    break;
+                }
+            mKind(*it) = k;
+        }
+        const char* names[] = {"trace", "write", "evalKnownDf", "sync", "update", "init", "compErr", "checkConv"};
+        for (int i = 0; i < 8; ++i) mWallClockTimes[names[i]] = 0.0;
+    }
+
+    virtual ~Problem() { releaseDevice(); }
+
+    // -------------------------------------------------------------- accessors
+    const Parameters& params() const { return mParams; }
+    const GridGlobal<DIM>& globalGrid() const { return mGG; }
+    const int& nTimesteps() const { return mParams.nTimesteps(); }
+    const double& tau() const { return mParams.tau(); }
+    const double& gridsize(const int& p) const { return mGG.discreteDomain().gridsize(p); }
+    const ScaFES_IntNtuple& nNodes() const { return mGG.discreteDomain().nNodes(); }
+    const int& nNodes(const int& p) const { return mGG.discreteDomain().nNodes(p); }
+    Ntuple<double, DIM> coordinates(const ScaFES_IntNtuple& idx) const { return mGG.discreteDomain().coordinates(idx); }
+    double time(const int& timestep) const { return mParams.timeIntervalStart() + timestep * mParams.tau(); }
+    ScaFES_IntNtuple connect(const ScaFES_IntNtuple& idx, const int& dir) { return mGG.discreteDomain().neigh(idx, dir); }
+    const short& kind(const ScaFES_IntNtuple& idx) const { return mKind[idx]; }
+    const DataField<short, DIM>& kind() const { return mKind; }
+    const std::vector<CT>& vectParamsCurr() const { return mParamsCurr; }
+    const CT& knownDf(const int& idx, const ScaFES_IntNtuple& node) const { return mVectKnownDfsDomNew[idx](node); }
+    const CT& knownDfNew(const int& idx, const ScaFES_IntNtuple& node) const { return mVectKnownDfsDomNew[idx](node); }
+    const CT& knownDfOld(const int& idx, const ScaFES_IntNtuple& node) const { return mVectKnownDfsDomOld[idx](node); }
+    const ScaFES_VectDf<CT>& vectSol() const { return mVectKnownDfsDomNew; }
+    const ScaFES_VectDf<CT>& vectKnownDfsDomNew() const { return mVectKnownDfsDomNew; }
+    const ScaFES_VectDf<CT>& vectKnownDfsDomOld() const { return mVectKnownDfsDomOld; }
+    const ScaFES_VectDf<CT>& vectUnknownDfsDomNew() const { return mVectUnknownDfsDomNew; }
+    const ScaFES_VectDf<CT>& vectUnknownDfsDomOld() const { return mVectUnknownDfsDomOld; }
+    bool useLeapfrogIntegration() const { return mUseLeapfrog; }
+    bool useAsynchronMode() const { return false; }
+    int myRank() const { return 0; }
+    const std::vector<bool>& isKnownDf() const { return mIsKnownDf; }
+    const std::vector<int>& nLayers() const { return mNlayers; }
+    const std::vector<CT>& defaultValue() const { return mDefaultValue; }
+    const std::map<std::string, double>& wallClockTimes() const { return mWallClockTimes; }
+    // "Local Linf" errors of every compErrLinf call, in call order.
+    const std::vector<double>& errorHistory() const { return mErrors; }
+    // Discrete energies recorded by the Yee kernel (every energyEvery steps).
+    const std::vector<double>& energyHistory() const { return mEnergies; }
+    // Time iterate at which the loop stopped (nTimesteps if it ran to the end).
+    int lastTimeIter() const { return mLastIter; }
+
+    // -------------------------------------------------------------- driver
+    void iterateOverTime()
+    {
+        if (!DeviceTraits<OWNPRBLM>::available)
+            throw std::runtime_error(
+                "ScaFES: this problem declares no device update kernel (specialise "
+                "ScaFES::DeviceTraits for it); the engine has no CPU update path.");
+        DeviceTraits<OWNPRBLM>::describe(static_cast<const OWNPRBLM&>(*this), mDev);
+        validateDevice();
+        Timer timerIterate;
+        if (out()) std::cout << mParams.getPrefix() << " * Iterate over time..." << std::endl;
+        mParams.decreaseLevel();
+        bool cont = true;
+        mLastIter = mParams.nTimesteps();
+        for (int n = 0; n <= mParams.nTimesteps(); ++n) {
+            if (out()) std::cout << mParams.getPrefix() << " * Process TIME ITERATE = " << n << "..." << std::endl;
+            evalKnownDfs(n);
+            if (0 < n) {
+                evalUpdatePhase(n);
+            } else {
+                Timer t;
+                resetValsAtNodesOfUnknownDfs();
+                resetTimesOfUnknownDfs();
+                evalInitPbPhase();
+                deviceSetup();
+                mWallClockTimes["init"] += t.elapsed();
+            }
+            compErrOfUnknownDfs(n);
+            cont = checkConvOfUnknownDfs(n);
+            swapPointersOfKnownDfs();
+            swapPointersOfUnknownDfs();
+            updateTimesOfKnownDfs();
+            updateTimesOfUnknownDfs();
+            if (!cont) {
+                if (out())
+                    std::cout << mParams.getPrefix() << " * Convergence after " << n << " of "
+                              << mParams.nTimesteps() << " timesteps." << std::endl;
+                mLastIter = n;
+                break;
+            }
+        }
+        deviceDownload();
+        const double tIterate = timerIterate.elapsed();
+        mParams.increaseLevel();
+        if (out()) printTimes(tIterate);
+        releaseDevice();
+    }
+
+    // Device buffers hold the context's handle: free them before the context.
+    void releaseDevice()
+    {
+        mUOld = DeviceBuffer();
+        mUNew = DeviceBuffer();
+        mCOld = DeviceBuffer();
+        mCNew = DeviceBuffer();
+        mKnownA = DeviceBuffer();
+        mKnownB = DeviceBuffer();
+        mYee = DeviceBuffer();
+        mRef.clear();
+        mCtx.reset();
+    }
+
+private:
+    bool out() const { return mParams.rankOutput() == 0 && 0 < mParams.indentDepth(); }
+    std::size_t nUnknown() const { return mUnknownIndex.size(); }
+    int unknownSlot(int df) const
+    {
+        for (std::size_t u = 0; u < mUnknownIndex.size(); ++u)
+            if (mUnknownIndex[u] == df) return (int)u;
+        return -1;
+    }
+    bool evalDf() const
+    {
+        for (std::size_t ii = 0; ii < mNames.size(); ++ii)
+            if (mComputeError[ii] || mIsKnownDf[ii]) return true;
+        return false;
+    }
+
+    // ---------------------------------------------------- known data fields
+    // evalKnownDfs (reference :3433-3485): evalInner/evalBorder at every node.
+    void evalKnownDfs(const int& n)
+    {
+        if (!evalDf()) return;
+        if (0 < n && !mDev.knownTimeDependent && !anyComputeError()) return;
+        Timer t;
+        const GridSub<DIM>& part = mGG.partition(0);
+        OWNPRBLM* self = static_cast<OWNPRBLM*>(this);
+        for (typename GridSub<DIM>::iterator it = part.begin(); it != part.end(); ++it) {
+            if (mKind[*it] & GridGlobal<DIM>::SCAFES_GLOBAL_INNER_NODE) self->evalInner(mVectKnownDfsDomNew, *it, n);
+            else self->evalBorder(mVectKnownDfsDomNew, *it, n);
+        }
+        if (0 < n) uploadKnown(n);
+        mWallClockTimes["evalKnownDf"] += t.elapsed();
+    }
+    bool anyComputeError() const
+    {
+        for (std::size_t ii = 0; ii < mNames.size(); ++ii)
+            if (mComputeError[ii] && !mIsKnownDf[ii]) return true;
+        return false;
+    }
+
+    // ---------------------------------------------------- init phase
+    void resetValsAtNodesOfUnknownDfs()
+    {
+        for (std::size_t u = 0; u < nUnknown(); ++u) {
+            mVectUnknownDfsDomOld[u] = mDefaultValue[mUnknownIndex[u]];
+            mVectUnknownDfsDomNew[u] = mDefaultValue[mUnknownIndex[u]];
+        }
+    }
+    void resetTimesOfUnknownDfs()
+    {
+        for (std::size_t u = 0; u < nUnknown(); ++u) mVectUnknownDfsDomNew[u].time() = mParams.timeIntervalStart();
+    }
+    // evalInitPbPhase (reference :2754-2826): init hooks, then old <- new.
+    void evalInitPbPhase()
+    {
+        OWNPRBLM* self = static_cast<OWNPRBLM*>(this);
+        for (std::size_t u = 0; u < nUnknown(); ++u) mVectUnknownDfsDomNew[u].time() = mParams.timeIntervalStart();
+        const GridSub<DIM>& part = mGG.partition(0);
+        for (typename GridSub<DIM>::iterator it = part.begin(); it != part.end(); ++it) {
+            if (mKind[*it] & GridGlobal<DIM>::SCAFES_GLOBAL_INNER_NODE)
+                self->template initInner<CT>(mVectUnknownDfsDomNew, mParamsCurr, *it, 0);
+            else
+                self->template initBorder<CT>(mVectUnknownDfsDomNew, mParamsCurr, *it, 0);
+        }
+        for (std::size_t u = 0; u < nUnknown(); ++u) mVectUnknownDfsDomOld[u].setValuesAtMemAll(mVectUnknownDfsDomNew[u]);
+        for (std::size_t ii = 0; ii < mNames.size(); ++ii)
+            if (!mLumpKnownNew[ii].empty()) mVectKnownDfsDomOld[ii].setValuesAtMemAll(mVectKnownDfsDomNew[ii]);
+    }
+
+    // ---------------------------------------------------- device set-up
+    void validateDevice()
+    {
+        const int nDf = (int)mNames.size();
+        auto known = [&](int ii, const char* what) {
+            if (ii < 0 || ii >= nDf || !mIsKnownDf[ii])
+                throw std::runtime_error(std::string("DeviceUpdate: ") + what + " must name a known data field");
+        };
+        auto unknown = [&](int ii) {
+            if (ii < 0 || ii >= nDf || mIsKnownDf[ii])
+                throw std::runtime_error("DeviceUpdate: the updated field must be an unknown data field");
+            if (mStencilWidth[ii] < 1) throw std::runtime_error("DeviceUpdate: updated field needs stencil width >= 1");
+        };
+        switch (mDev.kernel) {
+        case DeviceKernel::HEAT7:
+            unknown(mDev.unknown);
+            if (mDev.precision != DevicePrecision::F64) throw std::runtime_error("DeviceUpdate: HEAT7 runs in FP64");
+            if (mDev.sourceKnown >= 0) known(mDev.sourceKnown, "sourceKnown");
+            if (mDev.border == DeviceBorder::KNOWN_FIELD) known(mDev.borderKnown, "borderKnown");
+            break;
+        case DeviceKernel::HEAT7VAR:
+            unknown(mDev.unknown);
+            known(mDev.kappaKnown, "kappaKnown");
+            if (mStencilWidth[mDev.kappaKnown] != mStencilWidth[mDev.unknown])
+                throw std::runtime_error("DeviceUpdate: kappa needs the stencil width of the updated field");
+            if (mDev.border == DeviceBorder::KNOWN_FIELD)
+                throw std::runtime_error("DeviceUpdate: HEAT7VAR supports KEEP or VALUE borders");
+            break;
+        case DeviceKernel::PENNES:
+            unknown(mDev.unknown);
+            known(mDev.perfusionKnown, "perfusionKnown");
+            known(mDev.sourceFieldKnown, "sourceFieldKnown");
+            if (mDev.precision == DevicePrecision::F32_KAHAN)
+                throw std::runtime_error("DeviceUpdate: PENNES runs in FP64 or FP32");
+            break;
+        case DeviceKernel::YEE:
+            if (!mUseLeapfrog) throw std::runtime_error("DeviceUpdate: YEE needs the leapfrog mode");
+            for (int c = 0; c < 6; ++c) unknown(mDev.yee[c]);
+            if (DIM != 3) throw std::runtime_error("DeviceUpdate: YEE is three-dimensional");
+            for (std::size_t p = 1; p < DIM; ++p)
+                if (nNodes(p) != nNodes(0)) throw std::runtime_error("DeviceUpdate: YEE needs a cubic grid");
+            break;
+        default:
+            throw std::runtime_error("DeviceUpdate: no kernel selected");
+        }
+    }
+
+    scafes_geom geom() const
+    {
+        scafes_geom g;
+        g.dim = (int)DIM;
+        for (int p = 0; p < 3; ++p) {
+            g.n[p] = p < (int)DIM ? nNodes(p) : 1;
+            g.h[p] = p < (int)DIM ? gridsize(p) : 1.0;
+        }
+        g.nlayers = mParams.nLayersAtBorder();
+        return g;
+    }
+    std::size_t nodesOf(int ii) const { return (std::size_t)mMemAll[ii].nNodes().size(); }
+    bool f32() const { return mDev.precision != DevicePrecision::F64; }
+    double shift() const { return (mDev.kernel == DeviceKernel::PENNES && f32()) ? mDev.tA : 0.0; }
+
+    // host lump (memAll + time slot) -> device array (memAll) of the field's dtype
+    void put(DeviceBuffer& d, const std::vector<CT>& lump, std::size_t n, bool asFloat, double sub = 0.0)
+    {
+        if (asFloat) {
+            std::vector<float> tmp(n);
+            for (std::size_t i = 0; i < n; ++i) tmp[i] = (float)((double)lump[i] - sub);
+            d.upload(tmp.data(), n * sizeof(float));
+        } else {
+            std::vector<double> tmp(n);
+            for (std::size_t i = 0; i < n; ++i) tmp[i] = (double)lump[i] - sub;
+            d.upload(tmp.data(), n * sizeof(double));
+        }
+    }
+    DeviceBuffer alloc(std::size_t n, bool asFloat) { return DeviceBuffer(*mCtx, n * (asFloat ? 4 : 8)); }
+
+    void deviceSetup()
+    {
+        mCtx.reset(new DeviceContext(mParams.device()));
+        const bool fl = f32();
+        if (mDev.kernel == DeviceKernel::YEE) {
+            const std::size_t nn = (std::size_t)nNodes().size();
+            mYee = DeviceBuffer(*mCtx, 6 * nn * sizeof(float));
+            std::vector<float> tmp(6 * nn);
+            for (int c = 0; c < 6; ++c) packNodes(mLumpUnknownNew[unknownSlot(mDev.yee[c])], mDev.yee[c], &tmp[c * nn]);
+            mYee.upload(tmp.data(), tmp.size() * sizeof(float));
+        } else {
+            const int ii = mDev.unknown;
+            const int u = unknownSlot(ii);
+            const std::size_t n = nodesOf(ii);
+            mUOld = alloc(n, fl);
+            mUNew = alloc(n, fl);
+            put(mUOld, mLumpUnknownOld[u], n, fl, shift());
+            put(mUNew, mLumpUnknownNew[u], n, fl, shift());
+            if (mDev.precision == DevicePrecision::F32_KAHAN) {
+                mCOld = alloc(n, true);
+                mCNew = alloc(n, true);
+                deviceCheck(scafes_fill(mCtx->get(), mCOld.get(), SCAFES_F32, n, 0.0), "fill");
+                deviceCheck(scafes_fill(mCtx->get(), mCNew.get(), SCAFES_F32, n, 0.0), "fill");
+            }
+        }
+        uploadKnown(0);
+    }
+
+    // Known fields the kernel reads (and reference solutions for compErr).
+    void uploadKnown(const int& n)
+    {
+        if (!mCtx) return;
+        const bool fl = f32();
+        auto up = [&](int ii, DeviceBuffer& d, bool old, bool asFloat) {
+            const std::size_t nE = nodesOf(ii);
+            if (!d.get()) d = alloc(nE, asFloat);
+            put(d, old ? mLumpKnownOldView(ii) : mLumpKnownNewView(ii), nE, asFloat);
+        };
+        const bool firstOrTime = (n == 0) || mDev.knownTimeDependent;
+        if (firstOrTime) {
+            if (mDev.kernel == DeviceKernel::HEAT7) {
+                // F from the old known buffer (knownDfOld), G from the new one (knownDf)
+                if (mDev.sourceKnown >= 0) up(mDev.sourceKnown, mKnownA, n > 0, false);
+                if (mDev.border == DeviceBorder::KNOWN_FIELD) up(mDev.borderKnown, mKnownB, false, false);
+            } else if (mDev.kernel == DeviceKernel::HEAT7VAR) {
+                up(mDev.kappaKnown, mKnownA, false, fl);
+            } else if (mDev.kernel == DeviceKernel::PENNES) {
+                up(mDev.perfusionKnown, mKnownA, false, fl);
+                up(mDev.sourceFieldKnown, mKnownB, false, fl);
+            }
+        }
+        // reference solutions of unknown fields, for compErrLinf on the device
+        for (std::size_t u = 0; u < nUnknown(); ++u) {
+            const int ii = mUnknownIndex[u];
+            if (mComputeError[ii] && !fl && mDev.kernel != DeviceKernel::YEE) {
+                if (mRef.size() < nUnknown()) mRef.resize(nUnknown());
+                up(ii, mRef[u], false, false);
+            }
+        }
+    }
+    // Known lumps as seen through the (swapped) views.
+    const std::vector<CT>& mLumpKnownOldView(int ii) const { return lumpOf(mVectKnownDfsDomOld[ii], ii); }
+    const std::vector<CT>& mLumpKnownNewView(int ii) const { return lumpOf(mVectKnownDfsDomNew[ii], ii); }
+    const std::vector<CT>& lumpOf(const DataField<CT, DIM>& df, int ii) const
+    {
+        return df.elemData() == mLumpKnownOld[ii].data() ? mLumpKnownOld[ii] : mLumpKnownNew[ii];
+    }
+
+    // node block (without ghosts) of an unknown lump, as FP32
+    void packNodes(const std::vector<CT>& lump, int ii, float* dst) const
+    {
+        const GridSub<DIM>& part = mGG.partition(0);
+        const Grid<DIM>& m = mMemAll[ii];
+        std::size_t k = 0;
+        for (typename GridSub<DIM>::iterator it = part.begin(); it != part.end(); ++it) {
+            unsigned long s = 0;
+            m.idxNodeTuple2Scalar(s, *it);
+            dst[k++] = (float)lump[s];
+        }
+    }
+    void unpackNodes(std::vector<CT>& lump, int ii, const float* src) const
+    {
+        const GridSub<DIM>& part = mGG.partition(0);
+        const Grid<DIM>& m = mMemAll[ii];
+        std::size_t k = 0;
+        for (typename GridSub<DIM>::iterator it = part.begin(); it != part.end(); ++it) {
+            unsigned long s = 0;
+            m.idxNodeTuple2Scalar(s, *it);
+            lump[s] = (CT)src[k++];
+        }
+    }
+
+    // ---------------------------------------------------- update phase
+    void evalUpdatePhase(const int& n)
+    {
+        Timer t;
+        // time of the new iterate = old + tau before the update (reference :5419-5423)
+        for (std::size_t u = 0; u < nUnknown(); ++u)
+            mVectUnknownDfsDomNew[u].time() = mVectUnknownDfsDomOld[u].time() + mParams.tau();
+        scafes_ctx* c = mCtx->get();
+        const scafes_geom g = geom();
+        const int hw = mStencilWidth[mDev.unknown];
+        switch (mDev.kernel) {
+        case DeviceKernel::HEAT7: {
+            scafes_heat7_params p;
+            p.tau = mParams.tau();
+            p.coeff_a = mDev.coeffA;
+            p.rhoc_inv = mDev.rhoCInv;
+            p.perf_c = mDev.perfC;
+            p.perf_w = mDev.perfW;
+            p.has_perf = mDev.perfusion ? 1 : 0;
+            p.has_source = mDev.sourceKnown >= 0 ? 1 : 0;
+            p.border_mode = (int)mDev.border;
+            p.border_value = mDev.borderValue;
+            deviceCheck(scafes_heat7_step_f64(c, &g, hw, (const double*)mUOld.get(), (double*)mUNew.get(),
+                                              (const double*)mKnownA.get(), (const double*)mKnownB.get(), &p),
+                        "heat7 step");
+            break;
+        }
+        case DeviceKernel::HEAT7VAR: {
+            int which = 0;
+            deviceCheck(scafes_heat7var_run(c, &g, hw, (int)mDev.precision, mUOld.get(), mUNew.get(), mCOld.get(),
+                                            mCNew.get(), mKnownA.get(), mParams.tau(), 1, (int)mDev.border,
+                                            mDev.borderValue, -1, &which),
+                        "heat7var step");
+            break;
+        }
+        case DeviceKernel::PENNES: {
+            scafes_pennes_params p;
+            p.tau = mParams.tau();
+            p.cdiff = mDev.cdiff;
+            p.cperf = mDev.cperf;
+            p.csrc = mDev.csrc;
+            p.t_a = mDev.tA - shift();
+            deviceCheck(scafes_pennes_step(c, &g, hw, f32() ? SCAFES_F32 : SCAFES_F64, mUOld.get(), mUNew.get(),
+                                           mKnownA.get(), mKnownB.get(), &p),
+                        "pennes step");
+            break;
+        }
+        case DeviceKernel::YEE: {
+            // updateInner (H) and, after the leapfrog swap, updateInner2 (E):
+            // one fused leapfrog step on the device (reference :2603-2619)
+            double e = 0.0;
+            const bool rec = mDev.energyEvery > 0 && (n % mDev.energyEvery) == 0;
+            deviceCheck(scafes_yee_run(c, nNodes(0) - 1, SCAFES_F32, mDev.courant, mYee.get(), 1, rec ? 1 : 0,
+                                       rec ? &e : 0),
+                        "yee step");
+            if (rec) mEnergies.push_back(e);
+            // the reference swaps unknown pointers and times twice inside a
+            // leapfrog step; the net effect on the time slots is nil
+            break;
+        }
+        default:
+            break;
+        }
+        mCtx->sync();
+        mWallClockTimes["update"] += t.elapsed();
+    }
+
+    // ---------------------------------------------------- error / convergence
+    void compErrOfUnknownDfs(const int& /*n*/)
+    {
+        Timer t;
+        for (std::size_t u = 0; u < nUnknown(); ++u) {
+            const int ii = mUnknownIndex[u];
+            if (!mComputeError[ii]) continue;
+            double e = 0.0;
+            const scafes_geom g = geom();
+            if (!f32() && mDev.kernel != DeviceKernel::YEE && ii == mDev.unknown) {
+                deviceCheck(scafes_reduce(mCtx->get(), &g, SCAFES_F64, mUNew.get(), mStencilWidth[ii], mRef[u].get(),
+                                          mStencilWidth[ii], 0, 0, &e),
+                            "compErrLinf");
+            } else {
+                downloadUnknownTo(u, mVectUnknownDfsDomNew[u]);
+                e = mVectUnknownDfsDomNew[u].compErrLinf(mVectKnownDfsDomNew[ii]);
+            }
+            mErrors.push_back(e);
+            if (out())
+                std::cout << mParams.getPrefix() << " * Local Linf error(" << mVectUnknownDfsDomNew[u].name() << ", "
+                          << mVectKnownDfsDomNew[ii].name() << "): " << std::scientific << e
+                          << std::resetiosflags(std::ios::scientific) << std::endl;
+        }
+        mWallClockTimes["compErr"] += t.elapsed();
+    }
+
+    bool checkConvOfUnknownDfs(const int& n)
+    {
+        Timer t;
+        std::vector<bool> cont;
+        for (std::size_t u = 0; u < nUnknown(); ++u) {
+            const int ii = mUnknownIndex[u];
+            if (!(mCheckConvergence[ii] && n >= mParams.checkConvFirstAtIter() &&
+                  (n - mParams.checkConvFirstAtIter()) % mParams.checkConvAtEveryNIter() == 0))
+                continue;
+            if (mDev.kernel == DeviceKernel::YEE || ii != mDev.unknown)
+                throw std::runtime_error("checkConvergence is supported for the updated field of a heat/Pennes kernel");
+            double d = 0.0;
+            const scafes_geom g = geom();
+            deviceCheck(scafes_reduce(mCtx->get(), &g, f32() ? SCAFES_F32 : SCAFES_F64, mUNew.get(), mStencilWidth[ii],
+                                      mUOld.get(), mStencilWidth[ii], 0, 0, &d),
+                        "checkConv");
+            cont.push_back(d > mParams.threshold());
+            if (out())
+                std::cout << mParams.getPrefix() << " * Is (" << mVectUnknownDfsDomNew[u].name() << ", "
+                          << mVectUnknownDfsDomOld[u].name() << ") converged?: " << !cont.back() << std::endl;
+        }
+        mWallClockTimes["checkConv"] += t.elapsed();
+        if (cont.empty()) return true;
+        for (std::size_t i = 0; i < cont.size(); ++i)
+            if (cont[i]) return true;
+        return false;
+    }
+
+    // ---------------------------------------------------- swaps and times
+    void swapPointersOfKnownDfs()
+    {
+        for (std::size_t ii = 0; ii < mNames.size(); ++ii)
+            mVectKnownDfsDomNew[ii].swapPointerToMemoryWith(mVectKnownDfsDomOld[ii]);
+    }
+    void swapPointersOfUnknownDfs()
+    {
+        for (std::size_t u = 0; u < nUnknown(); ++u)
+            mVectUnknownDfsDomNew[u].swapPointerToMemoryWith(mVectUnknownDfsDomOld[u]);
+        mUOld.swap(mUNew);
+        mCOld.swap(mCNew);
+    }
+    void updateTimesOfKnownDfs()
+    {
+        if (!evalDf()) return;
+        for (std::size_t ii = 0; ii < mNames.size(); ++ii)
+            if (mVectKnownDfsDomNew[ii].elemData())
+                mVectKnownDfsDomNew[ii].time() = mVectKnownDfsDomOld[ii].time() + mParams.tau();
+    }
+    void updateTimesOfUnknownDfs()
+    {
+        for (std::size_t u = 0; u < nUnknown(); ++u)
+            mVectUnknownDfsDomNew[u].time() = mVectUnknownDfsDomOld[u].time() + mParams.tau();
+    }
+
+    // ---------------------------------------------------- download
+    // Copies device state of unknown slot u into the lump the view df points to.
+    void downloadUnknownTo(std::size_t u, DataField<CT, DIM>& df)
+    {
+        const int ii = mUnknownIndex[u];
+        std::vector<CT>& lump = df.elemData() == mLumpUnknownOld[u].data() ? mLumpUnknownOld[u] : mLumpUnknownNew[u];
+        const bool isNew = (&df == &mVectUnknownDfsDomNew[u]);
+        if (mDev.kernel == DeviceKernel::YEE) {
+            const std::size_t nn = (std::size_t)nNodes().size();
+            int c = -1;
+            for (int q = 0; q < 6; ++q)
+                if (mDev.yee[q] == ii) c = q;
+            if (c < 0) return;
+            std::vector<float> tmp(nn);
+            deviceCheck(scafes_memcpy_d2h(mCtx->get(), tmp.data(), (const float*)mYee.get() + c * nn, nn * sizeof(float)),
+                        "download");
+            unpackNodes(lump, ii, tmp.data());
+            return;
+        }
+        if (ii != mDev.unknown) return;
+        const std::size_t n = nodesOf(ii);
+        const DeviceBuffer& d = isNew ? mUNew : mUOld;
+        if (!f32()) {
+            std::vector<double> tmp(n);
+            d.download(tmp.data(), n * sizeof(double));
+            for (std::size_t i = 0; i < n; ++i) lump[i] = (CT)tmp[i];
+        } else {
+            std::vector<float> tmp(n), cmp(n, 0.0f);
+            d.download(tmp.data(), n * sizeof(float));
+            if (mDev.precision == DevicePrecision::F32_KAHAN)
+                (isNew ? mCNew : mCOld).download(cmp.data(), n * sizeof(float));
+            for (std::size_t i = 0; i < n; ++i) lump[i] = (CT)((double)tmp[i] + (double)cmp[i] + shift());
+        }
+    }
+    // After the loop the latest iterate is behind the "old" views (swapped).
+    void deviceDownload()
+    {
+        if (!mCtx) return;
+        mCtx->sync();
+        for (std::size_t u = 0; u < nUnknown(); ++u) downloadUnknownTo(u, mVectUnknownDfsDomOld[u]);
+    }
+
+    void printTimes(double tIterate)
+    {
+        double tSum = 0.0;
+        for (std::map<std::string, double>::const_iterator it = mWallClockTimes.begin(); it != mWallClockTimes.end(); ++it)
+            tSum += it->second;
+        std::cout << mParams.getPrefix() << " * Summary of Wall clock times(iterateOverTime):" << std::endl;
+        for (std::map<std::string, double>::const_iterator it = mWallClockTimes.begin(); it != mWallClockTimes.end(); ++it)
+            std::cout << mParams.getPrefix() << "       t(" << std::setw(11) << it->first << ") = " << std::fixed
+                      << std::setw(8) << std::setprecision(5) << it->second << " s (" << std::setw(5)
+                      << std::setprecision(2) << (tSum > 0 ? it->second / tSum * 100.0 : 0.0) << " %)" << std::endl;
+        std::cout << mParams.getPrefix() << "       t(    iterate) = " << std::setw(8) << std::setprecision(5) << tSum
+                  << " s." << std::endl
+                  << mParams.getPrefix() << " t(iterate) w.barriers= " << std::setw(8) << std::setprecision(5)
+                  << tIterate << " s." << std::resetiosflags(std::ios::fixed) << std::setprecision(6) << std::endl;
+    }
+
+protected:
+    Parameters mParams;
+    const GridGlobal<DIM>& mGG;
+
+private:
+    bool mUseLeapfrog;
+    std::vector<std::string> mNames;
+    std::vector<int> mStencilWidth;
+    std::vector<bool> mIsKnownDf;
+    std::vector<int> mNlayers;
+    std::vector<CT> mDefaultValue;
+    std::vector<WriteHowOften> mWriteToFile;
+    std::vector<bool> mComputeError;
+    std::vector<CT> mParamsCurr;
+    std::vector<bool> mCheckConvergence;
+    std::vector<Grid<DIM>> mMemAll;
+    std::vector<int> mUnknownIndex;
+    std::vector<std::vector<CT>> mLumpUnknownOld, mLumpUnknownNew, mLumpKnownOld, mLumpKnownNew;
+    std::vector<short> mLumpKind;
+    ScaFES_VectDf<CT> mVectUnknownDfsDomOld, mVectUnknownDfsDomNew, mVectKnownDfsDomOld, mVectKnownDfsDomNew;
+    DataField<short, DIM> mKind;
+    std::map<std::string, double> mWallClockTimes;
+    std::vector<double> mErrors, mEnergies;
+    int mLastIter = 0;
+    // device state
+    DeviceUpdate mDev;
+    std::unique_ptr<DeviceContext> mCtx;
+    DeviceBuffer mUOld, mUNew, mCOld, mCNew, mKnownA, mKnownB, mYee;
+    std::vector<DeviceBuffer> mRef;
+};
+
+}  // namespace ScaFES
+#endif

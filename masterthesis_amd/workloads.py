@@ -14,6 +14,42 @@ def splitmix64(seed, idx):
         return z ^ (z >> np.uint64(31))
 
 
+def batch2d_ensemble(nprob, nsteps, seed=0, sizes=None, nmin=32, nmax=256):
+    """Synthetic ensemble of 2-D heat problems (host arrays for scafes_batch2d_run).
+
+    n_i ~ U{nmin..nmax} (seeded), kappa_i ~ U[0.1, 1], tau_i = 0.9 h_i^2 / 4
+    (stable for every kappa <= 1 and independent of kappa_i, so dJ/dkappa is
+    taken at fixed tau), u0 = sin(pi x) sin(pi y) + 0.01 seeded noise on
+    inner nodes, target = u0 * exp(-2 pi^2 * 1.1 kappa_i * nsteps tau_i)
+    (the fundamental mode decayed with a 10 % larger diffusivity).
+    order: descending per-thread block cost ceil(n/32) * ceil(n/16)."""
+    rng = np.random.default_rng(seed)
+    n = np.asarray(sizes if sizes is not None else rng.integers(nmin, nmax + 1, size=nprob), dtype=np.int32)
+    nprob = len(n)
+    kappa = rng.uniform(0.1, 1.0, size=nprob).astype(np.float32)
+    h = 1.0 / (n.astype(np.float64) - 1)
+    tau = (0.9 * h * h / 4.0).astype(np.float32)
+    off = np.zeros(nprob, dtype=np.int64)
+    off[1:] = np.cumsum(n.astype(np.int64) ** 2)[:-1]
+    total = int(np.sum(n.astype(np.int64) ** 2))
+    u0 = np.zeros(total, dtype=np.float32)
+    target = np.zeros(total, dtype=np.float32)
+    for i in range(nprob):
+        m = int(n[i])
+        x = np.linspace(0.0, 1.0, m)
+        s = np.sin(np.pi * x)
+        u = np.outer(s, s)
+        if m > 2:
+            u[1:-1, 1:-1] += 0.01 * rng.standard_normal((m - 2, m - 2))
+        u[0, :] = u[-1, :] = u[:, 0] = u[:, -1] = 0.0
+        dec = np.exp(-2 * np.pi ** 2 * 1.1 * float(kappa[i]) * nsteps * float(tau[i]))
+        u0[off[i]:off[i] + m * m] = u.ravel()
+        target[off[i]:off[i] + m * m] = (u * dec).ravel()
+    cost = ((n + 31) // 32) * ((n + 15) // 16)
+    order = np.argsort(-cost, kind="stable").astype(np.int32)
+    return {"n": n, "off": off, "kappa": kappa, "tau": tau, "u0": u0, "target": target, "order": order}
+
+
 def heatvar_init_numpy(n, seed=0, amp=1e-3):
     """Node arrays (z,y,x) of the variable-kappa heat workload:
     u = prod sin(pi x_p) + amp*(2U-1) on inner nodes, kappa = 1 + 0.5 sin(2 pi x) sin(2 pi y).
