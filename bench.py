@@ -83,6 +83,21 @@ def run_mode(ctx, capi, g, mode, dtype, u_init, k_init, tau, nsteps, warmup, slo
     return ms, res, cres, [a, b, k, c1, c2]
 
 
+def profiled_traffic(path=os.path.join(ROOT, "profiles", "heat7var_kahan_512.md")):
+    """HBM bytes per launch of the headline kernel from the committed
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), and the
+    average launch duration the same profile's kernel trace measured."""
+    try:
+        vals = {}
+        for line in open(path):
+            if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "avg_duration_ns"):
+                k, v = line.split(":", 1)
+                vals[k] = float(v)
+        return vals.get("traffic_bytes_per_launch"), vals.get("avg_duration_ns")
+    except OSError:
+        return None, None
+
+
 def nodes(arr, n):
     return arr.reshape(n + 2, n + 2, n + 2)[1:-1, 1:-1, 1:-1]
 
@@ -113,6 +128,144 @@ def cpu_baseline(u_host, k_host, tau, n, budget_s=12.0):
                       f"{n}^3 input, {steps} steps in {dt:.1f} s"}
 
 
+def bench_yee(ctx, capi, ncell=384, nsteps=2000, energy_every=100):
+    """Yee FDTD, PEC box of ncell^3 cells, vacuum, Courant 0.5, Gaussian Ez
+    pulse (sigma = 8 cells) at the centre, FP32 fields, FP64 energy."""
+    nf = (ncell + 1) ** 3
+    f = ctx.alloc(6 * nf, np.float32)
+    capi.check(ctx.L.scafes_yee_init_pulse(ctx.h, ncell, capi.F32, f.ptr, 8.0))
+    warm = np.zeros(1)
+    capi.check(ctx.L.scafes_yee_run(ctx.h, ncell, capi.F32, 0.5, f.ptr, 2, 0, None))
+    capi.check(ctx.L.scafes_yee_init_pulse(ctx.h, ncell, capi.F32, f.ptr, 8.0))
+    nrec = (nsteps + energy_every - 1) // energy_every
+    e = np.zeros(nrec)
+    ctx.sync()
+    ctx.record(0)
+    capi.check(ctx.L.scafes_yee_run(ctx.h, ncell, capi.F32, 0.5, f.ptr, nsteps, energy_every,
+                                    e.ctypes.data_as(capi.C.POINTER(capi.C.c_double))))
+    ctx.record(1)
+    ms = ctx.elapsed_ms(0, 1)
+    f.free()
+    del warm
+    cells = float(ncell) ** 3
+    per = ms / nsteps
+    return {"workload": f"yee {ncell}^3 cells PEC vacuum, Courant 0.5, {nsteps} leapfrog steps, FP32",
+            "ms_per_step": round(per, 5), "gcell_steps_per_s": round(cells / (per * 1e-3) / 1e9, 3),
+            "algorithmic_bytes_per_cell_step": 48, "moved_bytes_per_cell_step": 72,
+            "hbm_frac_algorithmic": round(48 * cells / (per * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "energy_rel_drift_fp64": float(np.abs(e - e[0]).max() / e[0]), "energy_samples": int(nrec)}
+
+
+def bench_pennes(ctx, capi, n=256, nsteps=2000, snapshots=(62,)):
+    """Pennes bioheat on n^3: k = 0.5, rho c = 1, T_a = 37, log-normal
+    perfusion w (median 0.5, sigma 0.3, i.i.d. seeded), Gaussian source;
+    forward time vs adjoint-gradient time (FP32 fields, FP64 gradient)."""
+    from masterthesis_amd.workloads import pennes_workload
+    wl = pennes_workload(n, seed=0)
+    g = capi.geom((n, n, n))
+    ne = capi.memall_count(g, 1)
+    dt = np.float32
+    th0 = ctx.upload(np.zeros(ne, dtype=dt))  # theta = T - T_a, T0 = T_a
+    w = ctx.upload(wl["w"].astype(dt))
+    q = ctx.upload(wl["q"].astype(dt))
+    pp = capi.PennesParams(tau=wl["tau"], cdiff=0.5, cperf=1.0, csrc=1.0, t_a=0.0)
+    a, b = ctx.alloc(ne, dt), ctx.alloc(ne, dt)
+    gp = capi.C.byref(g)
+
+    def forward(wbuf, steps, out_host=False):
+        capi.check(ctx.L.scafes_memcpy_d2d(ctx.h, a.ptr, th0.ptr, ne * 4))
+        capi.check(ctx.L.scafes_memcpy_d2d(ctx.h, b.ptr, th0.ptr, ne * 4))
+        x, y = a, b
+        for _ in range(steps):
+            capi.check(ctx.L.scafes_pennes_step(ctx.h, gp, 1, capi.F32, x.ptr, y.ptr, wbuf.ptr, q.ptr,
+                                                capi.C.byref(pp)))
+            x, y = y, x
+        return x
+
+    # target from the perturbed perfusion
+    wp = ctx.upload((wl["w"] * wl["pert"]).astype(dt))
+    tgt_src = forward(wp, nsteps)
+    target = ctx.alloc(ne, dt)
+    capi.check(ctx.L.scafes_memcpy_d2d(ctx.h, target.ptr, tgt_src.ptr, ne * 4))
+    wp.free()
+    forward(w, 5)
+    ctx.sync()
+    ctx.record(0)
+    forward(w, nsteps)
+    ctx.record(1)
+    t_fwd = ctx.elapsed_ms(0, 1)
+    out = {"workload": f"pennes {n}^3, {nsteps} steps, FP32 fields (deviation from T_a), FP64 gradient",
+           "fwd_ms": round(t_fwd, 2), "fwd_GLUPS": round(float(n) ** 3 * nsteps / (t_fwd * 1e-3) / 1e9, 2),
+           "gradient": []}
+    grad = ctx.alloc(ne, np.float64)
+    for s in snapshots:
+        J = capi.C.c_double(0)
+        stats = np.zeros(4)
+        ctx.sync()
+        t0 = time.perf_counter()
+        rc = ctx.L.scafes_pennes_gradient(ctx.h, gp, 1, capi.F32, th0.ptr, w.ptr, q.ptr, target.ptr,
+                                          capi.C.byref(pp), nsteps, s, grad.ptr, capi.C.byref(J),
+                                          stats.ctypes.data_as(capi.C.POINTER(capi.C.c_double)))
+        t_grad = (time.perf_counter() - t0) * 1e3
+        if rc != 0:
+            out["gradient"].append({"snapshots": s, "error": ctx.L.scafes_last_error().decode()})
+            continue
+        out["gradient"].append({"snapshots": int(stats[2]), "checkpoint_GB": round(stats[3] / 1e9, 2),
+                                "forward_steps": int(stats[0]), "adjoint_steps": int(stats[1]),
+                                "grad_ms": round(t_grad, 2), "grad_over_fwd": round(t_grad / t_fwd, 3),
+                                "J": J.value})
+    for d in (th0, w, q, a, b, target, grad):
+        d.free()
+    return out
+
+
+def bench_batch(ctx, capi, nprob=4096, nsteps=500):
+    """4096 2-D heat problems, n ~ U{32..256}, kappa ~ U[0.1,1], 500 FP32
+    steps each plus dJ/dkappa, one persistent hipGraph-captured launch."""
+    from masterthesis_amd.workloads import batch2d_ensemble
+    ens = batch2d_ensemble(nprob, nsteps, seed=0)
+    d = {k: ctx.upload(ens[k]) for k in ("n", "off", "kappa", "tau", "order", "target")}
+    u = ctx.upload(ens["u0"])
+    J = ctx.alloc(nprob, np.float64)
+    gr = ctx.alloc(nprob, np.float64)
+
+    def run(graph):
+        capi.check(ctx.L.scafes_batch2d_run(ctx.h, nprob, d["n"].ptr, d["off"].ptr, d["kappa"].ptr, d["tau"].ptr,
+                                            d["order"].ptr, u.ptr, d["target"].ptr, J.ptr, gr.ptr, nsteps, graph))
+
+    run(0)
+    u.write(ens["u0"])
+    ctx.sync()
+    t0 = time.perf_counter()
+    run(1)
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3
+    nodes = float(np.sum(ens["n"].astype(np.int64) ** 2))
+    gsum = float(np.sum(np.abs(gr.read())))
+    for x in list(d.values()) + [u, J, gr]:
+        x.free()
+    return {"workload": f"{nprob} 2-D heat problems n~U{{32..256}}, {nsteps} steps + dJ/dkappa, FP32, one launch",
+            "ms": round(ms, 2), "GLUPS": round(nodes * nsteps / (ms * 1e-3) / 1e9, 2), "nodes": int(nodes),
+            "sum_abs_grad": gsum}
+
+
+def calibrate_copy(ctx, capi, nbytes=1 << 30, reps=3):
+    """Plain copy of known size (reads nbytes, writes nbytes) for scaling the
+    HBM counters of a profile."""
+    src, dst = ctx.alloc(nbytes // 4, np.float32), ctx.alloc(nbytes // 4, np.float32)
+    src.fill(1.0)
+    ctx.sync()
+    ctx.record(0)
+    for _ in range(reps):
+        capi.check(ctx.L.scafes_memcpy_d2d(ctx.h, dst.ptr, src.ptr, nbytes))
+    ctx.record(1)
+    ms = ctx.elapsed_ms(0, 1) / reps
+    src.free()
+    dst.free()
+    return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": round(ms, 4),
+            "GBps": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +275,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--autotune", action="store_true", help="autotune tiles before timing")
+    ap.add_argument("--no-yee", action="store_true", help="skip the Yee FDTD leg")
+    ap.add_argument("--no-pennes", action="store_true", help="skip the Pennes adjoint leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--calibrate-copy", action="store_true", help="also run a 1 GiB copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
@@ -174,8 +331,14 @@ def main():
                    "nodes": n ** 3, "tau": tau, "kernel": "stream_kernel<HeatVarOp<float,2>>"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes_per_lup": BYTES_PER_LUP["f32_kahan"]},
+                     "algorithmic_bytes_per_lup": BYTES_PER_LUP["f32_kahan"],
+                     "algorithmic_bytes_per_launch": BYTES_PER_LUP["f32_kahan"] * n ** 3},
     }
+    traffic, prof_ns = profiled_traffic()
+    if traffic is not None and n == N:
+        out["roofline"]["traffic"] = traffic
+        out["roofline"]["profile_avg_launch_us"] = round(prof_ns / 1e3, 2)
+        out["roofline"]["profile"] = "profiles/heat7var_kahan_512.md"
     if tune:
         out["config"]["autotune"] = tune
 
@@ -207,12 +370,30 @@ def main():
             if d is not None:
                 d.free()
 
+    for d in bufs32:
+        if d is not None:
+            d.free()
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle, OpenMP)...")
         u_host = u0.read()
         k_host = k0.read()
         out["cpu_baseline"] = cpu_baseline(u_host, k_host, tau, n)
+        del u_host, k_host
         log(f"[bench] cpu baseline {out['cpu_baseline']['value']} GLUPS")
+    u0.free()
+    k0.free()
+    if args.calibrate_copy:
+        out["copy_calibration"] = calibrate_copy(ctx, capi)
+        log(f"[bench] copy {out['copy_calibration']}")
+    if not args.no_yee:
+        out["yee"] = bench_yee(ctx, capi)
+        log(f"[bench] yee {out['yee']}")
+    if not args.no_pennes:
+        out["pennes_adjoint"] = bench_pennes(ctx, capi)
+        log(f"[bench] pennes {out['pennes_adjoint']}")
+    if not args.no_batch:
+        out["batch2d"] = bench_batch(ctx, capi)
+        log(f"[bench] batch {out['batch2d']}")
     ctx.close()
     print(json.dumps(out), flush=True)
 

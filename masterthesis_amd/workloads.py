@@ -50,6 +50,25 @@ def batch2d_ensemble(nprob, nsteps, seed=0, sizes=None, nmin=32, nmax=256):
     return {"n": n, "off": off, "kappa": kappa, "tau": tau, "u0": u0, "target": target, "order": order}
 
 
+def pennes_workload(n, seed=0, k=0.5, amp=50.0, width=0.1):
+    """Pennes bioheat workload on n^3 nodes of the unit cube (memAll arrays
+    with one ghost layer, shape (n+2,)*3): log-normal perfusion
+    w = 0.5 exp(0.3 Z) (Z i.i.d. N(0,1), seeded), Gaussian source
+    Q = amp exp(-|x - c|^2 / (2 width^2)), tau = 0.9 / (6 k / h^2 + max w)
+    (rho c = 1), and the multiplicative perturbation 1 + 0.1 N(0,1) of w
+    that generates the synthetic target."""
+    rng = np.random.default_rng(seed)
+    m = n + 2
+    w = 0.5 * np.exp(0.3 * rng.standard_normal((m, m, m)))
+    pert = 1.0 + 0.1 * rng.standard_normal((m, m, m))
+    x = np.arange(-1, n + 1) / (n - 1.0)
+    d2 = (x - 0.5) ** 2
+    q = amp * np.exp(-(d2[:, None, None] + d2[None, :, None] + d2[None, None, :]) / (2 * width ** 2))
+    h = 1.0 / (n - 1)
+    tau = 0.9 / (6 * k / h ** 2 + float(w.max()))
+    return {"w": w.ravel(), "q": q.ravel(), "pert": pert.ravel(), "tau": tau}
+
+
 def heatvar_init_numpy(n, seed=0, amp=1e-3):
     """Node arrays (z,y,x) of the variable-kappa heat workload:
     u = prod sin(pi x_p) + amp*(2U-1) on inner nodes, kappa = 1 + 0.5 sin(2 pi x) sin(2 pi y).
