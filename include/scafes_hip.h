@@ -119,9 +119,13 @@ enum { SCAFES_HEATVAR_F64 = 0, SCAFES_HEATVAR_F32 = 1, SCAFES_HEATVAR_F32_KAHAN 
 int scafes_heat7var_run(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode, void* u,
                         void* u2, void* c, void* c2, const void* kappa, double tau, int nsteps,
                         int border_mode, double border_value, int cfg, int* result_in_second);
-/* Number of tile configurations and a description of one. */
+/* Number of tile configurations and a description of one: workgroup tile,
+ * z planes per workgroup, prefetch depth in planes, and time steps per HBM
+ * sweep (1, or 2 for the temporally blocked kernel; nsteps odd then ends
+ * with one single-step sweep). */
 int scafes_heat7var_num_cfgs(void);
-int scafes_heat7var_cfg_desc(int cfg, int* tx, int* ty, int* zchunk, int* depth);
+int scafes_heat7var_cfg_desc(int cfg, int* tx, int* ty, int* zchunk, int* depth,
+                             int* steps_per_sweep);
 /* Project autotuner: times every configuration on the given buffers for
  * nsteps steps and stores the fastest as the default for this mode. */
 int scafes_autotune_heat7var(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode, void* u,

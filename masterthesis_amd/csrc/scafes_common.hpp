@@ -95,6 +95,13 @@ int pennes_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, c
                       void* lam_new, const void* w, const void* tprev, double* grad,
                       const scafes_pennes_params* p, const void* tprev_lo);
 
+// Two-steps-per-sweep variable-kappa heat (defined in heatvar_tb2.hip).
+int heatvar_tb2_num_cfgs();
+void heatvar_tb2_cfg(int cfg, int* tx, int* ty, int* zc, int* d);
+int heatvar_tb2_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* uo,
+                       const void* cin, void* co, const void* kap, double tau, int border_mode, double border_value,
+                       int cfg);
+
 // Reductions (defined in scafes_capi.hip).
 enum RedOp { RED_MAX_ABS_DIFF = 0, RED_SUM_SQ_DIFF = 1, RED_SUM_PROD = 2 };
 int reduce_nodes(Ctx* c, const Geo& ga, const Geo& gb, int dtype, const void* a, const void* b,

@@ -330,9 +330,13 @@ static const TileCfg kCfgs[] = {
 };
 static constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 static constexpr int kDefaultCfg = 3;
-// Measured on an MI355X at 512^3 (fastest tile per heat7var mode:
-// F64 64x8 zc128 d6, F32 128x2 zc64 d4, F32+Kahan 64x16 zc128 d4).
-static constexpr int kDefaultCfgByMode[3] = {12, 4, 9};
+// Fastest one-step tile per heat7var mode, measured on an MI355X at 512^3
+// (F64 64x8 zc128 d6, F32 128x2 zc64 d4, F32+Kahan 64x16 zc128 d4); used
+// for the odd last step behind two-steps-per-sweep launches.
+static constexpr int kOneStepCfgByMode[3] = {12, 4, 9};
+// Default per mode: the two-steps-per-sweep tile 64x8 zc64 d2 (index 2 of
+// heatvar_tb2.hip's table) was fastest in every mode at 512^3.
+static constexpr int kDefaultCfgByMode[3] = {kNumCfgs + 2, kNumCfgs + 2, kNumCfgs + 2};
 
 template <class Op, typename T, int TX, int TY, int D>
 static int launch_t(Ctx* c, const Op& op, const Geo& g, int zc)
@@ -598,15 +602,29 @@ int scafes_heat7_step_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const d
     return launch<HeatConstOp, double>(c, op, geo, -1);
 }
 
-int scafes_heat7var_num_cfgs(void) { return kNumCfgs; }
+// Configuration index space: [0, kNumCfgs) one-step tiles, then the
+// two-steps-per-sweep tiles of heatvar_tb2.hip.
+int scafes_heat7var_num_cfgs(void) { return kNumCfgs + heatvar_tb2_num_cfgs(); }
 
-int scafes_heat7var_cfg_desc(int cfg, int* tx, int* ty, int* zchunk, int* depth)
+int scafes_heat7var_cfg_desc(int cfg, int* tx, int* ty, int* zchunk, int* depth, int* steps_per_sweep)
 {
-    if (cfg < 0 || cfg >= kNumCfgs) return fail(SCAFES_ERR_ARG, "bad cfg");
-    if (tx) *tx = kCfgs[cfg].tx;
-    if (ty) *ty = kCfgs[cfg].ty;
-    if (zchunk) *zchunk = kCfgs[cfg].zc;
-    if (depth) *depth = kCfgs[cfg].d;
+    if (cfg < 0 || cfg >= scafes_heat7var_num_cfgs()) return fail(SCAFES_ERR_ARG, "bad cfg");
+    int a, b, z, d, st;
+    if (cfg < kNumCfgs) {
+        a = kCfgs[cfg].tx;
+        b = kCfgs[cfg].ty;
+        z = kCfgs[cfg].zc;
+        d = kCfgs[cfg].d;
+        st = 1;
+    } else {
+        heatvar_tb2_cfg(cfg - kNumCfgs, &a, &b, &z, &d);
+        st = 2;
+    }
+    if (tx) *tx = a;
+    if (ty) *ty = b;
+    if (zchunk) *zchunk = z;
+    if (depth) *depth = d;
+    if (steps_per_sweep) *steps_per_sweep = st;
     return SCAFES_OK;
 }
 
@@ -628,14 +646,27 @@ int scafes_heat7var_run(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode,
     rc = make_geo(g, hw, &geo);
     if (rc) return rc;
     if (cfg < 0) cfg = c->tuned[mode] >= 0 ? c->tuned[mode] : kDefaultCfgByMode[mode];
+    if (cfg >= scafes_heat7var_num_cfgs()) return fail(SCAFES_ERR_ARG, "heat7var: bad cfg");
     void *a = u, *b = u2, *ca = cmp, *cb = cmp2;
-    for (int s = 0; s < nsteps; ++s) {
+    int launches = 0;
+    int s = 0;
+    if (cfg >= kNumCfgs) {
+        for (; s + 2 <= nsteps; s += 2, ++launches) {
+            rc = heatvar_tb2_launch(c, geo, g, mode, a, b, ca, cb, kappa, tau, border_mode, border_value,
+                                    cfg - kNumCfgs);
+            if (rc) return rc;
+            std::swap(a, b);
+            std::swap(ca, cb);
+        }
+        cfg = kOneStepCfgByMode[mode];  // odd remainder
+    }
+    for (; s < nsteps; ++s, ++launches) {
         rc = heat7var_launch(c, geo, g, mode, a, b, ca, cb, kappa, tau, border_mode, border_value, cfg);
         if (rc) return rc;
         std::swap(a, b);
         std::swap(ca, cb);
     }
-    if (result_in_second) *result_in_second = (nsteps % 2 == 1) ? 1 : 0;
+    if (result_in_second) *result_in_second = (launches % 2 == 1) ? 1 : 0;
     return SCAFES_OK;
 }
 
@@ -651,7 +682,8 @@ int scafes_autotune_heat7var(scafes_ctx* ctx, const scafes_geom* g, int hw, int 
     SCAFES_HIP(hipEventCreate(&e1));
     int best = -1;
     float bms = 1e30f;
-    for (int cfg = 0; cfg < kNumCfgs; ++cfg) {
+    const int ncfg = scafes_heat7var_num_cfgs();
+    for (int cfg = 0; cfg < ncfg; ++cfg) {
         int which = 0;
         // warm-up one step, then time nsteps (an even count keeps u in place)
         int rc = scafes_heat7var_run(ctx, g, hw, mode, u, u2, cmp, cmp2, kappa, tau, 2, SCAFES_BORDER_KEEP,

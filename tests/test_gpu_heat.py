@@ -141,7 +141,8 @@ def heatvar_inputs(n, seed=0, amp=1e-3):
     return og, u, kap
 
 
-def run_heatvar(ctx, g, mode, u0, kap, tau, nsteps, cfg=-1, dtype=np.float64):
+def run_heatvar(ctx, g, mode, u0, kap, tau, nsteps, cfg=-1, dtype=np.float64, border=capi.BORDER_VALUE,
+                bval=0.0):
     a, b = ctx.upload(u0.astype(dtype)), ctx.upload(u0.astype(dtype))
     kd = ctx.upload(kap.astype(dtype))
     c1 = c2 = None
@@ -152,7 +153,7 @@ def run_heatvar(ctx, g, mode, u0, kap, tau, nsteps, cfg=-1, dtype=np.float64):
     which = capi.C.c_int(0)
     capi.check(ctx.L.scafes_heat7var_run(ctx.h, capi.C.byref(g), 1, mode, a.ptr, b.ptr,
                                          c1.ptr if c1 else None, c2.ptr if c2 else None, kd.ptr, tau,
-                                         nsteps, capi.BORDER_VALUE, 0.0, cfg, capi.C.byref(which)))
+                                         nsteps, border, bval, cfg, capi.C.byref(which)))
     res = (b if which.value else a).read().reshape(u0.shape)
     comp = None
     if c1 is not None:
@@ -163,15 +164,57 @@ def run_heatvar(ctx, g, mode, u0, kap, tau, nsteps, cfg=-1, dtype=np.float64):
     return res, comp
 
 
-@pytest.mark.parametrize("cfg", list(range(10)))
+NUM_CFGS = capi.load().scafes_heat7var_num_cfgs()
+
+
+def cfg_steps_per_sweep(cfg):
+    st = capi.C.c_int(0)
+    capi.check(capi.load().scafes_heat7var_cfg_desc(cfg, None, None, None, None, capi.C.byref(st)))
+    return st.value
+
+
+@pytest.mark.parametrize("cfg", list(range(NUM_CFGS)))
 def test_heatvar_f64_bitexact_all_tiles(ctx, cfg):
+    """Every tile configuration, one- and two-steps-per-sweep, even and odd step counts."""
     n = (37, 29, 23)  # not multiples of any tile edge
     og, u0, kap = heatvar_inputs(n, seed=cfg)
     g = capi.geom(n)
     tau = 0.9 / (2.0 * 1.5 * sum((m - 1.0) ** 2 for m in n))  # 1/tau >= 2 kappa_max sum 1/h^2
-    ref = O.heat7var_run(og, u0, kap, tau, 12)
-    got, _ = run_heatvar(ctx, g, capi.HEATVAR_F64, u0, kap, tau, 12, cfg=cfg)
+    nsteps = 12 + cfg % 2
+    ref = O.heat7var_run(og, u0, kap, tau, nsteps)
+    got, _ = run_heatvar(ctx, g, capi.HEATVAR_F64, u0, kap, tau, nsteps, cfg=cfg)
     np.testing.assert_array_equal(O.nodes_view(og, got), O.nodes_view(og, ref))
+
+
+TB2_CFGS = [c for c in range(NUM_CFGS) if cfg_steps_per_sweep(c) == 2]
+
+
+@pytest.mark.parametrize("mode", [capi.HEATVAR_F32, capi.HEATVAR_F32_KAHAN, capi.HEATVAR_F64])
+@pytest.mark.parametrize("cfg", TB2_CFGS)
+def test_heatvar_two_step_sweep_matches_one_step(ctx, cfg, mode):
+    """Two steps per sweep perform the one-step kernel's arithmetic node by node:
+    identical bits in every precision, including the Kahan compensation, with
+    nonzero kept borders, several z chunks and a tile grid that does not
+    divide the domain."""
+    n = (70, 45, 150)
+    og, u0, kap = heatvar_inputs(n, seed=11)
+    rng = np.random.default_rng(5)
+    border = np.ones(tuple(reversed(n)), bool)
+    border[1:-1, 1:-1, 1:-1] = False
+    O.nodes_view(og, u0)[...] += np.where(border, 0.25 + rng.random(border.shape), 0.0)
+    g = capi.geom(n)
+    tau = 0.9 / (2.0 * 1.5 * sum((m - 1.0) ** 2 for m in n))
+    dt =np.float64 if mode == capi.HEATVAR_F64 else np.float32
+    nsteps = 9
+    one, c_one = run_heatvar(ctx, g, mode, u0, kap, tau, nsteps, cfg=0, dtype=dt, border=capi.BORDER_KEEP)
+    two, c_two = run_heatvar(ctx, g, mode, u0, kap, tau, nsteps, cfg=cfg, dtype=dt, border=capi.BORDER_KEEP)
+    np.testing.assert_array_equal(O.nodes_view(og, two), O.nodes_view(og, one))
+    if mode == capi.HEATVAR_F32_KAHAN:
+        np.testing.assert_array_equal(O.nodes_view(og, c_two), O.nodes_view(og, c_one))
+    # border value 0.5 imposed on every step
+    one, _ = run_heatvar(ctx, g, mode, u0, kap, tau, 4, cfg=0, dtype=dt, border=capi.BORDER_VALUE, bval=0.5)
+    two, _ = run_heatvar(ctx, g, mode, u0, kap, tau, 4, cfg=cfg, dtype=dt, border=capi.BORDER_VALUE, bval=0.5)
+    np.testing.assert_array_equal(O.nodes_view(og, two), O.nodes_view(og, one))
 
 
 def test_heatvar_fp32_kahan_error(ctx):
