@@ -83,6 +83,23 @@ def run_mode(ctx, capi, g, mode, dtype, u_init, k_init, tau, nsteps, warmup, slo
     return ms, res, cres, [a, b, k, c1, c2]
 
 
+def heatvar_kernel(ctx, capi, g, mode):
+    """(configuration index, kernel name, time steps per launch) that
+    scafes_heat7var_run uses by default for this grid and mode."""
+    cfg = capi.C.c_int(-1)
+    capi.check(ctx.L.scafes_heat7var_active_cfg(ctx.h, capi.C.byref(g), 1, mode, capi.C.byref(cfg)))
+    tx, ty, zc, d, st = (capi.C.c_int() for _ in range(5))
+    capi.check(ctx.L.scafes_heat7var_cfg_desc(cfg.value, capi.C.byref(tx), capi.C.byref(ty), capi.C.byref(zc),
+                                              capi.C.byref(d), capi.C.byref(st)))
+    t, m = {capi.HEATVAR_F64: ("double", 0), capi.HEATVAR_F32: ("float", 1),
+            capi.HEATVAR_F32_KAHAN: ("float", 2)}[mode]
+    if st.value == 2:
+        name = f"heatvar_tb2_kernel<{t}, {m}, {tx.value}, {ty.value}, {d.value}>"
+    else:
+        name = f"stream_kernel<scafes::HeatVarOp<{t}, {m}>, {t}, {tx.value}, {ty.value}, {d.value}>"
+    return cfg.value, name, st.value, zc.value
+
+
 def profiled_traffic(path=os.path.join(ROOT, "profiles", "heat7var_kahan_512.md")):
     """HBM bytes per launch of the headline kernel from the committed
     rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), and the
@@ -315,24 +332,30 @@ def main():
             d.free()
         log(f"[bench] autotune: cfg {best.value} {bms.value:.3f} ms/step")
 
+    cfg, kname, sps, zc = heatvar_kernel(ctx, capi, g, capi.HEATVAR_F32_KAHAN)
     ms32, r32, c32, bufs32 = run_mode(ctx, capi, g, capi.HEATVAR_F32_KAHAN, np.float32, u0, k0, tau,
                                       args.steps, args.warmup, slot0=0)
     lups = float(n) ** 3 * args.steps
     glups = lups / (ms32 * 1e-3) / 1e9
-    per_launch_ms = ms32 / args.steps
-    achieved = BYTES_PER_LUP["f32_kahan"] * float(n) ** 3 / (per_launch_ms * 1e-3) / 1e9
-    log(f"[bench] fp32+kahan: {ms32:.1f} ms, {glups:.1f} GLUPS, {achieved:.0f} GB/s")
+    # one launch advances sps steps (an odd step count ends with one single-step launch)
+    launches = args.steps // sps + args.steps % sps
+    per_launch_ms = ms32 / launches
+    alg_per_launch = BYTES_PER_LUP["f32_kahan"] * float(n) ** 3 * sps
+    achieved = alg_per_launch / (per_launch_ms * 1e-3) / 1e9
+    log(f"[bench] fp32+kahan ({kname}): {ms32:.1f} ms, {glups:.1f} GLUPS, {achieved:.0f} GB/s algorithmic")
     out = {
         "metric": METRIC, "value": round(glups, 3), "unit": "GLUPS", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(per_launch_ms, 5),
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms32 / args.steps, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (device-seeded splitmix64 noise on the analytic sine mode)",
         "config": {"workload": f"heat7var {n}^3 nodes, variable kappa, FP32 + Kahan, {args.steps} Euler steps",
-                   "nodes": n ** 3, "tau": tau, "kernel": "stream_kernel<HeatVarOp<float,2>>"},
+                   "nodes": n ** 3, "tau": tau, "kernel": kname, "cfg": cfg, "steps_per_launch": sps,
+                   "z_planes_per_workgroup": zc},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes_per_lup": BYTES_PER_LUP["f32_kahan"],
-                     "algorithmic_bytes_per_launch": BYTES_PER_LUP["f32_kahan"] * n ** 3},
+                     "algorithmic_bytes_per_lup": BYTES_PER_LUP["f32_kahan"], "lups_per_launch": sps * n ** 3,
+                     "algorithmic_bytes_per_launch": alg_per_launch, "launches": launches,
+                     "avg_launch_us": round(per_launch_ms * 1e3, 2)},
     }
     traffic, prof_ns = profiled_traffic()
     if traffic is not None and n == N:
@@ -346,7 +369,9 @@ def main():
         ms64, r64, _, bufs64 = run_mode(ctx, capi, g, capi.HEATVAR_F64, np.float64, u0, k0, tau,
                                         args.steps, 2, slot0=2)
         glups64 = lups / (ms64 * 1e-3) / 1e9
-        ach64 = BYTES_PER_LUP["f64"] * float(n) ** 3 / (ms64 / args.steps * 1e-3) / 1e9
+        _, kname64, sps64, _ = heatvar_kernel(ctx, capi, g, capi.HEATVAR_F64)
+        launches64 = args.steps // sps64 + args.steps % sps64
+        ach64 = BYTES_PER_LUP["f64"] * float(n) ** 3 * sps64 / (ms64 / launches64 * 1e-3) / 1e9
         ref = nodes(r64.read(), n)
         hi = nodes(r32.read(), n).astype(np.float64)
         lo = nodes(c32.read(), n).astype(np.float64)
@@ -360,7 +385,8 @@ def main():
         ref = nodes(r64.read(), n)
         err_p = float(np.abs(nodes(r_p.read(), n).astype(np.float64) - ref).max() / scale)
         del ref
-        out["fp64"] = {"value": round(glups64, 3), "unit": "GLUPS", "hbm_frac": round(ach64 / HBM_PEAK_GBS, 4)}
+        out["fp64"] = {"value": round(glups64, 3), "unit": "GLUPS", "kernel": kname64,
+                       "hbm_frac_algorithmic": round(ach64 / HBM_PEAK_GBS, 4)}
         out["fp32_plain"] = {"value": round(glups_p, 3), "unit": "GLUPS", "rel_err_vs_fp64": err_p}
         out["fp32_kahan_rel_err_vs_fp64"] = err_k
         out["fp32_kahan_hi_only_rel_err_vs_fp64"] = err_hi

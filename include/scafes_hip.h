@@ -126,6 +126,10 @@ int scafes_heat7var_run(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode,
 int scafes_heat7var_num_cfgs(void);
 int scafes_heat7var_cfg_desc(int cfg, int* tx, int* ty, int* zchunk, int* depth,
                              int* steps_per_sweep);
+/* The configuration scafes_heat7var_run uses for cfg = -1 on this grid
+ * (autotuned choice, else the per-mode default; the two-steps-per-sweep
+ * tiles need a 3-D grid). */
+int scafes_heat7var_active_cfg(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode, int* cfg);
 /* Project autotuner: times every configuration on the given buffers for
  * nsteps steps and stores the fastest as the default for this mode. */
 int scafes_autotune_heat7var(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode, void* u,

@@ -79,6 +79,7 @@ def load():
                                     C.c_int, C.c_int, C.c_double, C.c_int, ip])
     _sig(L, "scafes_heat7var_num_cfgs", [])
     _sig(L, "scafes_heat7var_cfg_desc", [C.c_int, ip, ip, ip, ip, ip])
+    _sig(L, "scafes_heat7var_active_cfg", [vp, gp, C.c_int, C.c_int, ip])
     _sig(L, "scafes_autotune_heat7var", [vp, gp, C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_double,
                                          C.c_int, ip, C.POINTER(C.c_float)])
     _sig(L, "scafes_heat7var_init", [vp, gp, C.c_int, C.c_int, vp, vp, C.c_uint64, C.c_double])

@@ -1,23 +1,31 @@
 // heatvar_tb2.hip -- two time steps per HBM sweep for the variable-kappa heat
-// update (temporal blocking), same arithmetic per node as the one-step
-// kernel in stencil3d.hip, so FP64 results stay bit-identical to the CPU path.
+// update (temporal blocking) on 3-D grids, same arithmetic per node as the
+// one-step kernel in stencil3d.hip, so results are bit-identical to it in
+// every precision (and, in FP64, to the CPU path).
 //
 // A TX x TY workgroup streams z planes.  Level 1 (u after one step) is
 // computed for every thread position, i.e. for the output tile plus a
 // one-node ring; level 2 (after two steps) for the (TX-2) x (TY-2) interior,
 // which is the output tile.  Per plane k:
 //   - level-0 plane k (u, kappa, plus a one-node ring) goes to LDS from a
-//     D-deep register prefetch ring; centre-only Kahan compensation too;
-//   - level 1 at plane k from LDS neighbours and the z queue (k-1, k+1);
+//     D-deep register prefetch ring; the centre-only Kahan compensation too;
+//   - level 1 at plane k from LDS neighbours and the z queue (k-1, k+1); its
+//     six face coefficients (0.5*(kE+kC) ... in FP64, kE+kC ... in FP32) stay
+//     in registers, since level 2 at the same node one plane later needs
+//     exactly these values, and the upper z face of plane k is the lower one
+//     of plane k+1;
 //   - level 2 at plane k-1 from the level-1 LDS plane written one plane
 //     before (in-plane neighbours) and a level-1 z queue (k-2, k).
-// One barrier per plane: u planes are double-buffered in LDS; kappa planes are
-// triple-buffered because level 2 reads plane k-1's kappa while the next
-// plane's kappa is being written.  HBM moves the one-step bytes once per two steps
-// (plus the ring, mostly L2 hits from the neighbouring tiles on the same XCD).
+// One barrier per plane; u, kappa and level-1 planes are double-buffered.
+// Loads and stores are buffer accesses: a wave-uniform per-plane resource
+// plus a 32-bit in-plane byte offset per thread, which keeps address
+// arithmetic off the vector ALUs.
+// HBM moves the one-step bytes once per two steps, plus the ring, mostly L2
+// hits from the neighbouring tiles on the same XCD.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 
 #include "scafes_common.hpp"
 
@@ -30,23 +38,41 @@ struct VarCoef {
     int border_mode;
 };
 
-// One explicit step of the conservative form at one node.  MODE 0: FP64 in
-// the CPU path's order; 1: FP32; 2: FP32 + Kahan (cin -> *cout).
+// Face coefficient between the centre and one neighbour: 0.5*(k_nb + k_c)
+// in the FP64 path (the CPU path's grouping), k_nb + k_c in FP32 (the 0.5 is
+// folded into c_p).  IEEE addition is commutative, so the face shared by two
+// nodes has one value.
 template <typename T, int MODE>
-__device__ inline T var_update(const Geo& g, const VarCoef& cf, T uc, T uw, T ue, T us, T un, T ud, T uu, T kc,
-                               T kw, T ke, T ks, T kn, T kd, T ku, T cin, T* cout)
+__device__ inline T face(T a, T b)
+{
+    if constexpr (MODE == 0)
+        return 0.5 * (a + b);
+    else
+        return a + b;
+}
+
+template <typename T>
+struct Faces {
+    T e, w, n, s, u, d;
+};
+
+// One explicit step at one interior node from its six face coefficients.
+// MODE 0: FP64 in the CPU path's order; 1: FP32; 2: FP32 + Kahan (cin -> *cout).
+template <typename T, int MODE>
+__device__ inline T var_update(const VarCoef& cf, T uc, T uw, T ue, T us, T un, T ud, T uu, const Faces<T>& f, T cin,
+                               T* cout)
 {
     if constexpr (MODE == 0) {
         double v = uc;
-        v += (cf.tau * (0.5 * (ke + kc) * (ue - uc) - 0.5 * (kc + kw) * (uc - uw))) / cf.hh0;
-        if (g.dim > 1) v += (cf.tau * (0.5 * (kn + kc) * (un - uc) - 0.5 * (kc + ks) * (uc - us))) / cf.hh1;
-        if (g.dim > 2) v += (cf.tau * (0.5 * (ku + kc) * (uu - uc) - 0.5 * (kc + kd) * (uc - ud))) / cf.hh2;
+        v += (cf.tau * (f.e * (ue - uc) - f.w * (uc - uw))) / cf.hh0;
+        v += (cf.tau * (f.n * (un - uc) - f.s * (uc - us))) / cf.hh1;
+        v += (cf.tau * (f.u * (uu - uc) - f.d * (uc - ud))) / cf.hh2;
         *cout = T(0);
         return (T)v;
     } else {
-        float d = cf.c0 * __builtin_fmaf(ke + kc, ue - uc, -((kc + kw) * (uc - uw)));
-        if (g.dim > 1) d = __builtin_fmaf(cf.c1, __builtin_fmaf(kn + kc, un - uc, -((kc + ks) * (uc - us))), d);
-        if (g.dim > 2) d = __builtin_fmaf(cf.c2, __builtin_fmaf(ku + kc, uu - uc, -((kc + kd) * (uc - ud))), d);
+        float d = cf.c0 * __builtin_fmaf(f.e, ue - uc, -(f.w * (uc - uw)));
+        d = __builtin_fmaf(cf.c1, __builtin_fmaf(f.n, un - uc, -(f.s * (uc - us))), d);
+        d = __builtin_fmaf(cf.c2, __builtin_fmaf(f.u, uu - uc, -(f.d * (uc - ud))), d);
         if constexpr (MODE == 2) {
             const float y = d + cin;
             const float t = uc + y;
@@ -56,6 +82,40 @@ __device__ inline T var_update(const Geo& g, const VarCoef& cf, T uc, T uw, T ue
             *cout = 0.0f;
             return uc + d;
         }
+    }
+}
+
+// Buffer access to one z plane: a wave-uniform resource (base = the plane's
+// first memAll element, range = the in-plane extent incl. ghost planes below
+// it) plus a 32-bit per-thread byte offset; the hardware range check returns
+// 0 for, and drops, accesses beyond the range.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+constexpr int kRsrcWord3 = 0x00020000;  // gfx9 family: 32-bit data format, raw (stride 0) buffer
+
+__device__ inline Rsrc plane_rsrc(const char* plane, uint32_t range)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(plane), (short)0, (int)range, kRsrcWord3);
+}
+
+template <typename T>
+__device__ inline T bload(Rsrc r, uint32_t off)
+{
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+    } else {
+        typedef int v2i __attribute__((ext_vector_type(2)));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+    }
+}
+
+template <typename T>
+__device__ inline void bstore(T v, Rsrc r, uint32_t off)
+{
+    if constexpr (sizeof(T) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)off, 0, 0);
+    } else {
+        typedef int v2i __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2i, v), r, (int)off, 0, 0);
     }
 }
 
@@ -69,7 +129,7 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb2_kernel(Geo g, const T* __r
     constexpr int OX = TX - 2, OY = TY - 2;
     constexpr bool KH = (MODE == 2);
     __shared__ T s0[2][TY + 2][TX + 2];  // level-0 plane with ring
-    __shared__ T sk[3][TY + 2][TX + 2];  // kappa plane with ring (read again one plane later)
+    __shared__ T sk[2][TY + 2][TX + 2];  // kappa plane with ring
     __shared__ T s1[2][TY][TX];          // level-1 plane
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
@@ -80,11 +140,14 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb2_kernel(Geo g, const T* __r
     const int txi = r - tyi * tiles_x;
     const int i = txi * OX - 1 + tx, j = tyi * OY - 1 + ty;
     const int ic = min(max(i, 0), g.nx - 1), jc = min(max(j, 0), g.ny - 1);
-    const bool in_xy = i >= 0 && i < g.nx && j >= 0 && j < g.ny;
+    // in-plane interior (also false outside the grid): border and outside
+    // positions keep (or take the border value) at both levels
+    const bool inner_xy = i >= g.nl0 && i <= g.nx - 1 - g.nl0 && j >= g.nl1 && j <= g.ny - 1 - g.nl1;
     const bool out_thread = tx >= 1 && tx <= TX - 2 && ty >= 1 && ty <= TY - 2 && i < g.nx && j < g.ny;
     const int k0 = tz * zc, k1 = min(k0 + zc, g.nz);
-    const int kstart = k0 - 1;         // first plane with a level-1 value
-    const int kcen = k1 + 1;           // last plane whose centre is needed
+    const int kstart = k0 - 1;  // first plane with a level-1 value
+    const int kcen = k1 + 1;    // last plane whose centre is needed
+    const int klo = g.nl2, khi = g.nz - 1 - g.nl2;  // interior planes
     // halo duty for the level-0 ring
     const bool hx = (tx == 0) || (tx == TX - 1);
     const bool hy = (ty == 0) || (ty == TY - 1);
@@ -92,39 +155,50 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb2_kernel(Geo g, const T* __r
     const int hyj = (ty == 0) ? max(jc - 1, 0) : min(jc + 1, g.ny - 1);
     const int sxh = (tx == 0) ? 0 : TX + 1;
     const int syh = (ty == 0) ? 0 : TY + 1;
-    const long oc = at(g, ic, jc, 0), ohx = at(g, hxi, jc, 0), ohy = at(g, ic, hyj, 0);
-    auto kcl = [&](int kk) { return min(max(kk, 0), g.nz - 1); };
+    constexpr uint32_t B = sizeof(T);
+    const uint32_t oc = (uint32_t)(g.base + ic + g.e0 * jc) * B;
+    const uint32_t ohx = (uint32_t)(g.base + hxi + g.e0 * jc) * B;
+    const uint32_t ohy = (uint32_t)(g.base + ic + g.e0 * hyj) * B;
+    const uint32_t oo = (uint32_t)(g.base + i + g.e0 * j) * B;  // output node (out_thread only)
+    const size_t pb = (size_t)g.e01 * B;                        // plane stride in bytes
+    const uint32_t range = (uint32_t)((g.base + g.e01) * B);    // in-plane offsets stay below this
+    const char* ub = (const char*)u;
+    const char* kb8 = (const char*)kap;
+    const char* cb = (const char*)cin;
+    auto plane = [&](int kk) { return (size_t)min(max(kk, 0), g.nz - 1) * pb; };
 
     T qa[D], qb[D], qc[D], qax[D], qbx[D], qay[D], qby[D];
 #pragma unroll
     for (int s = 0; s < D; ++s) qa[s] = qb[s] = qc[s] = qax[s] = qbx[s] = qay[s] = qby[s] = T(0);
     auto fetch = [&](const int slot, const int kk) {
         if (kk <= kcen) {
-            const long o = g.e01 * kcl(kk);
-            qa[slot] = u[oc + o];
-            qb[slot] = kap[oc + o];
+            const size_t o = plane(kk);
+            const Rsrc ru = plane_rsrc(ub + o, range), rk = plane_rsrc(kb8 + o, range);
+            qa[slot] = bload<T>(ru, oc);
+            qb[slot] = bload<T>(rk, oc);
             if (kk <= k1) {
-                if constexpr (KH) qc[slot] = cin[oc + o];
+                if constexpr (KH) qc[slot] = bload<T>(plane_rsrc(cb + o, range), oc);
                 if (hx) {
-                    qax[slot] = u[ohx + o];
-                    qbx[slot] = kap[ohx + o];
+                    qax[slot] = bload<T>(ru, ohx);
+                    qbx[slot] = bload<T>(rk, ohx);
                 }
                 if (hy) {
-                    qay[slot] = u[ohy + o];
-                    qby[slot] = kap[ohy + o];
+                    qay[slot] = bload<T>(ru, ohy);
+                    qby[slot] = bload<T>(rk, ohy);
                 }
             }
         }
     };
-    T am = u[oc + g.e01 * kcl(kstart - 1)];
-    T bm = kap[oc + g.e01 * kcl(kstart - 1)];
+    T am = bload<T>(plane_rsrc(ub + plane(kstart - 1), range), oc);
+    const T bm = bload<T>(plane_rsrc(kb8 + plane(kstart - 1), range), oc);
 #pragma unroll
     for (int s = 0; s < D; ++s) fetch(s, kstart + s);
 
-    T l1m2 = T(0), l1m1 = T(0), c1m1 = T(0);  // level-1 at k-2, k-1 and compensation at k-1
-    T km2 = T(0), km1 = T(0);                  // kappa centre at k-2, k-1
-    int kcur = 0, kprev = 2;                   // kappa LDS slots of planes k and k-1
+    T l1m2 = T(0), l1m1 = T(0), c1m1 = T(0);  // level 1 at k-2, k-1 and its compensation at k-1
+    Faces<T> fp = {};                          // faces of plane k-1 (level 2 reuses them)
+    T fup = face<T, MODE>(qb[0], bm);          // upper z face of plane k-1 = lower face of plane k
     const T bv = (T)cf.border_value;
+    const bool keep = cf.border_mode != SCAFES_BORDER_VALUE;
     for (int kb = kstart; kb <= k1; kb += D) {
 #pragma unroll
         for (int s = 0; s < D; ++s) {
@@ -135,61 +209,54 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb2_kernel(Geo g, const T* __r
             const T bc = qb[s], bp = qb[(s + 1) % D];
             const T c0 = qc[s];
             s0[buf][ty + 1][tx + 1] = ac;
-            sk[kcur][ty + 1][tx + 1] = bc;
+            sk[buf][ty + 1][tx + 1] = bc;
             if (hx) {
                 s0[buf][ty + 1][sxh] = qax[s];
-                sk[kcur][ty + 1][sxh] = qbx[s];
+                sk[buf][ty + 1][sxh] = qbx[s];
             }
             if (hy) {
                 s0[buf][syh][tx + 1] = qay[s];
-                sk[kcur][syh][tx + 1] = qby[s];
+                sk[buf][syh][tx + 1] = qby[s];
             }
             fetch(s, k + D);
             __syncthreads();
-            // ---- level 1 at (i, j, k)
-            T l1, c1 = T(0);
-            const bool in1 = in_xy && k >= 0 && k < g.nz;
-            if (in1 && !is_border(g, i, j, k)) {
-                l1 = var_update<T, MODE>(g, cf, ac, s0[buf][ty + 1][tx], s0[buf][ty + 1][tx + 2], s0[buf][ty][tx + 1],
-                                         s0[buf][ty + 2][tx + 1], am, ap, bc, sk[kcur][ty + 1][tx],
-                                         sk[kcur][ty + 1][tx + 2], sk[kcur][ty][tx + 1], sk[kcur][ty + 2][tx + 1], bm,
-                                         bp, c0, &c1);
-            } else if (cf.border_mode == SCAFES_BORDER_VALUE) {
-                l1 = bv;
-            } else {
-                l1 = ac;
-                c1 = c0;
+            // ---- level 1 at (i, j, k), computed for every position, kept where not interior
+            Faces<T> f;
+            f.e = face<T, MODE>(sk[buf][ty + 1][tx + 2], bc);
+            f.w = face<T, MODE>(bc, sk[buf][ty + 1][tx]);
+            f.n = face<T, MODE>(sk[buf][ty + 2][tx + 1], bc);
+            f.s = face<T, MODE>(bc, sk[buf][ty][tx + 1]);
+            f.u = face<T, MODE>(bp, bc);
+            f.d = fup;
+            T c1;
+            T l1 = var_update<T, MODE>(cf, ac, s0[buf][ty + 1][tx], s0[buf][ty + 1][tx + 2], s0[buf][ty][tx + 1],
+                                       s0[buf][ty + 2][tx + 1], am, ap, f, c0, &c1);
+            if (!(inner_xy && k >= klo && k <= khi)) {
+                l1 = keep ? ac : bv;
+                c1 = keep ? c0 : T(0);
             }
             s1[buf][ty][tx] = l1;
             // ---- level 2 at (i, j, k-1) from the level-1 plane of the previous iteration
             const int k2 = k - 1;
             if (k2 >= k0 && out_thread) {
-                const long m = at(g, i, j, k2);
-                T v, cv = T(0);
-                if (!is_border(g, i, j, k2)) {
-                    const int pb = buf ^ 1;
-                    v = var_update<T, MODE>(g, cf, l1m1, s1[pb][ty][tx - 1], s1[pb][ty][tx + 1], s1[pb][ty - 1][tx],
-                                            s1[pb][ty + 1][tx], l1m2, l1, km1, sk[kprev][ty + 1][tx],
-                                            sk[kprev][ty + 1][tx + 2], sk[kprev][ty][tx + 1], sk[kprev][ty + 2][tx + 1],
-                                            km2, bc, c1m1, &cv);
-                } else if (cf.border_mode == SCAFES_BORDER_VALUE) {
-                    v = bv;
-                } else {
-                    v = l1m1;
-                    cv = c1m1;
+                const int qb1 = buf ^ 1;
+                T cv;
+                T v = var_update<T, MODE>(cf, l1m1, s1[qb1][ty][tx - 1], s1[qb1][ty][tx + 1], s1[qb1][ty - 1][tx],
+                                          s1[qb1][ty + 1][tx], l1m2, l1, fp, c1m1, &cv);
+                if (!(inner_xy && k2 >= klo && k2 <= khi)) {
+                    v = keep ? l1m1 : bv;
+                    cv = keep ? c1m1 : T(0);
                 }
-                uout[m] = v;
-                if constexpr (KH) cout[m] = cv;
+                const size_t o = (size_t)k2 * pb;
+                bstore<T>(v, plane_rsrc((const char*)uout + o, range), oo);
+                if constexpr (KH) bstore<T>(cv, plane_rsrc((const char*)cout + o, range), oo);
             }
             l1m2 = l1m1;
             l1m1 = l1;
             c1m1 = c1;
-            km2 = km1;
-            km1 = bc;
+            fp = f;
+            fup = f.u;
             am = ac;
-            bm = bc;
-            kprev = kcur;
-            kcur = kcur == 2 ? 0 : kcur + 1;
         }
     }
 }
@@ -199,6 +266,7 @@ struct Tb2Cfg {
 };
 static const Tb2Cfg kTb2Cfgs[] = {
     {64, 16, 64, 2}, {64, 16, 128, 2}, {64, 8, 64, 2}, {128, 8, 64, 2}, {64, 16, 64, 4}, {128, 4, 64, 2},
+    {64, 8, 64, 4},  {64, 8, 32, 2},   {64, 8, 128, 2}, {64, 4, 64, 2},
 };
 static constexpr int kNumTb2 = sizeof(kTb2Cfgs) / sizeof(kTb2Cfgs[0]);
 
@@ -223,6 +291,8 @@ static int tb2_dispatch(Ctx* c, const Geo& g, const void* u, const void* cin, co
     switch (t.tx * 100000 + t.ty * 100 + t.d) {
     case 6401602: return tb2_launch_t<T, MODE, 64, 16, 2>(c, g, u, cin, kap, uo, co, cf, t.zc);
     case 6400802: return tb2_launch_t<T, MODE, 64, 8, 2>(c, g, u, cin, kap, uo, co, cf, t.zc);
+    case 6400804: return tb2_launch_t<T, MODE, 64, 8, 4>(c, g, u, cin, kap, uo, co, cf, t.zc);
+    case 6400402: return tb2_launch_t<T, MODE, 64, 4, 2>(c, g, u, cin, kap, uo, co, cf, t.zc);
     case 12800802: return tb2_launch_t<T, MODE, 128, 8, 2>(c, g, u, cin, kap, uo, co, cf, t.zc);
     case 6401604: return tb2_launch_t<T, MODE, 64, 16, 4>(c, g, u, cin, kap, uo, co, cf, t.zc);
     case 12800402: return tb2_launch_t<T, MODE, 128, 4, 2>(c, g, u, cin, kap, uo, co, cf, t.zc);
@@ -241,15 +311,23 @@ void heatvar_tb2_cfg(int cfg, int* tx, int* ty, int* zc, int* d)
     *d = t.d;
 }
 
+bool heatvar_tb2_supported(const Geo& geo, int mode)
+{
+    // 3-D only; in-plane byte offsets (ghost planes included) must fit 32 bits
+    const long bytes = mode == SCAFES_HEATVAR_F64 ? 8 : 4;
+    return geo.dim == 3 && (geo.base + geo.e01) * bytes < (1L << 32);
+}
+
 // Two steps u -> uo (and Kahan compensation cin -> co).
 int heatvar_tb2_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* uo,
                        const void* cin, void* co, const void* kap, double tau, int border_mode, double border_value,
                        int cfg)
 {
     if (cfg < 0 || cfg >= kNumTb2) return fail(SCAFES_ERR_ARG, "bad two-step tile configuration");
+    if (!heatvar_tb2_supported(geo, mode)) return fail(SCAFES_ERR_UNSUPPORTED, "two-step sweep needs a 3-D grid");
     VarCoef cf;
     double hh[3];
-    for (int p = 0; p < 3; ++p) hh[p] = p < g->dim ? g->h[p] * g->h[p] : 1.0;
+    for (int p = 0; p < 3; ++p) hh[p] = g->h[p] * g->h[p];
     cf.tau = tau;
     cf.hh0 = hh[0];
     cf.hh1 = hh[1];

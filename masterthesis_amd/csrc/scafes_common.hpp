@@ -98,6 +98,7 @@ int pennes_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, c
 // Two-steps-per-sweep variable-kappa heat (defined in heatvar_tb2.hip).
 int heatvar_tb2_num_cfgs();
 void heatvar_tb2_cfg(int cfg, int* tx, int* ty, int* zc, int* d);
+bool heatvar_tb2_supported(const Geo& geo, int mode);  // 3-D grids with 32-bit in-plane offsets
 int heatvar_tb2_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* uo,
                        const void* cin, void* co, const void* kap, double tau, int border_mode, double border_value,
                        int cfg);

@@ -334,9 +334,10 @@ static constexpr int kDefaultCfg = 3;
 // (F64 64x8 zc128 d6, F32 128x2 zc64 d4, F32+Kahan 64x16 zc128 d4); used
 // for the odd last step behind two-steps-per-sweep launches.
 static constexpr int kOneStepCfgByMode[3] = {12, 4, 9};
-// Default per mode: the two-steps-per-sweep tile 64x8 zc64 d2 (index 2 of
-// heatvar_tb2.hip's table) was fastest in every mode at 512^3.
-static constexpr int kDefaultCfgByMode[3] = {kNumCfgs + 2, kNumCfgs + 2, kNumCfgs + 2};
+// Default per mode: the fastest two-steps-per-sweep tile at 512^3 on an
+// MI355X (indices into heatvar_tb2.hip's table): F64 64x8 zc64 d4 (6),
+// F32 64x16 zc64 d2 (0), F32+Kahan 64x8 zc32 d2 (7).
+static constexpr int kDefaultCfgByMode[3] = {kNumCfgs + 6, kNumCfgs + 0, kNumCfgs + 7};
 
 template <class Op, typename T, int TX, int TY, int D>
 static int launch_t(Ctx* c, const Op& op, const Geo& g, int zc)
@@ -628,6 +629,28 @@ int scafes_heat7var_cfg_desc(int cfg, int* tx, int* ty, int* zchunk, int* depth,
     return SCAFES_OK;
 }
 
+// cfg < 0: the autotuned choice for this mode, else the default; a
+// two-steps-per-sweep choice on a grid that kernel does not cover falls back
+// to the mode's one-step tile.
+static int resolve_heatvar_cfg(const Ctx* c, const Geo& geo, int mode, int cfg)
+{
+    if (cfg < 0) cfg = c->tuned[mode] >= 0 ? c->tuned[mode] : kDefaultCfgByMode[mode];
+    if (cfg >= kNumCfgs && !heatvar_tb2_supported(geo, mode)) cfg = kOneStepCfgByMode[mode];
+    return cfg;
+}
+
+int scafes_heat7var_active_cfg(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode, int* cfg)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || !g || !cfg) return fail(SCAFES_ERR_ARG, "heat7var_active_cfg: null pointer");
+    if (mode < 0 || mode > 2) return fail(SCAFES_ERR_ARG, "heat7var_active_cfg: bad mode");
+    Geo geo;
+    const int rc = make_geo(g, hw, &geo);
+    if (rc) return rc;
+    *cfg = resolve_heatvar_cfg(c, geo, mode, -1);
+    return SCAFES_OK;
+}
+
 int scafes_heat7var_run(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode, void* u, void* u2,
                         void* cmp, void* cmp2, const void* kappa, double tau, int nsteps,
                         int border_mode, double border_value, int cfg, int* result_in_second)
@@ -645,8 +668,8 @@ int scafes_heat7var_run(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode,
     Geo geo;
     rc = make_geo(g, hw, &geo);
     if (rc) return rc;
-    if (cfg < 0) cfg = c->tuned[mode] >= 0 ? c->tuned[mode] : kDefaultCfgByMode[mode];
     if (cfg >= scafes_heat7var_num_cfgs()) return fail(SCAFES_ERR_ARG, "heat7var: bad cfg");
+    cfg = resolve_heatvar_cfg(c, geo, mode, cfg);
     void *a = u, *b = u2, *ca = cmp, *cb = cmp2;
     int launches = 0;
     int s = 0;
