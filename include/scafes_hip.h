@@ -198,12 +198,17 @@ long scafes_checkpoint_forward_steps(int nsteps, int snaps);
  * dtype (F32 stream, or F64 in the CPU path's operation order), each
  * (ncell+1)^3 nodes, x fastest, updated in place over nsteps steps.
  * s = Courant number tau/h in (0, 1/sqrt(3)].
+ * f2: NULL, or a second state buffer of the same size; with f2 each step is
+ * one out-of-place sweep doing both half steps (6 field reads + 6 writes
+ * per cell instead of 9 + 6 for the two in-place half-step sweeps) and the
+ * state ping-pongs between f and f2: on return *result_in_second (nullable)
+ * is 1 when the final state is in f2.  Both schedules give identical bits.
  * energy (host FP64 array of ceil(nsteps/energy_every) entries, used when
  * energy_every > 0): discrete energy 0.5 sum E^n.E^n + 0.5 sum
  * H^{n-1/2}.H^{n+1/2} of steps 0, energy_every, ..., accumulated in FP64
  * inside the H half step. */
-int scafes_yee_run(scafes_ctx* ctx, int ncell, int dtype, double s, void* f, int nsteps,
-                   int energy_every, double* energy);
+int scafes_yee_run(scafes_ctx* ctx, int ncell, int dtype, double s, void* f, void* f2, int nsteps,
+                   int energy_every, double* energy, int* result_in_second);
 /* Initial state of the Yee workload: all fields zero except
  * Ez = exp(-r^2/(2 sigma^2)) at inner Ez nodes, r from the box centre in
  * cell units (Ez(i,j,k) sits at (i, j, k+1/2)). */

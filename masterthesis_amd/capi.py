@@ -93,7 +93,7 @@ def load():
                                            C.POINTER(PennesParams), C.c_int, C.c_int, vp, dp, dp])
         _sig(L, "scafes_checkpoint_forward_steps", [C.c_int, C.c_int], C.c_long)
     if hasattr(L, "scafes_yee_run"):
-        _sig(L, "scafes_yee_run", [vp, C.c_int, C.c_int, C.c_double, vp, C.c_int, C.c_int, dp])
+        _sig(L, "scafes_yee_run", [vp, C.c_int, C.c_int, C.c_double, vp, vp, C.c_int, C.c_int, dp, ip])
         _sig(L, "scafes_yee_init_pulse", [vp, C.c_int, C.c_int, vp, C.c_double])
     if hasattr(L, "scafes_batch2d_run"):
         _sig(L, "scafes_batch2d_run", [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp,

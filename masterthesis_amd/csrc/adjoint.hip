@@ -86,6 +86,7 @@ struct Rev {
     void* lam[2];
     int lcur;
     int nsteps;
+    bool pair;  // two adjoint steps per sweep where both states are at hand
     long fwd_steps, adj_steps;
     double J;
 
@@ -144,10 +145,45 @@ struct Rev {
         lcur = 1 - lcur;
         return r;
     }
+    // Adjoint steps n and n-1 in one sweep (lam^n -> lam^{n-2}) with
+    // T^{n-1} = t1 and T^{n-2} = t2; n < nsteps.
+    int adjoint2(int n, const void* t1, const void* t2)
+    {
+        (void)n;
+        adj_steps += 2;
+        int r = pennes_adj2_launch(c, geo, g, lam_dtype, lam[lcur], lam[1 - lcur], w, t1, lo(t1), t2, lo(t2), grad,
+                                   p);
+        lcur = 1 - lcur;
+        return r;
+    }
     // Reverse steps a+1..b given T^a in `base`; s snapshots may be taken.
     int treeverse(long a, long b, int s, const void* base)
     {
         if (b - a == 1) return adjoint((int)b, base);
+        if (pair && s >= b - a - 1 && (long)slots.size() >= b - a - 1) {
+            // Repetition number 1: every intermediate state T^{a+1..b-1} gets
+            // a snapshot (b-a-1 forward steps, as the recursive split would
+            // take), then the steps are reversed two per sweep.
+            std::vector<const void*> st(1, base);  // st[q] = T^{a+q}
+            int r = SCAFES_OK;
+            for (long q = 1; q < b - a && !r; ++q) {
+                void* sl = slots.back();
+                slots.pop_back();
+                r = step(st.back(), sl);
+                st.push_back(sl);
+            }
+            for (long n = b; n > a && !r;) {
+                if (n == nsteps || n - a < 2) {
+                    r = adjoint((int)n, st[n - 1 - a]);
+                    n -= 1;
+                } else {
+                    r = adjoint2((int)n, st[n - 1 - a], st[n - 2 - a]);
+                    n -= 2;
+                }
+            }
+            for (size_t q = st.size() - 1; q >= 1; --q) slots.push_back(const_cast<void*>(st[q]));
+            return r;
+        }
         if (s <= 0 || slots.empty()) {
             for (long n = b; n > a; --n) {
                 const void* tp = base;
@@ -233,6 +269,7 @@ int scafes_pennes_gradient(scafes_ctx* ctx, const scafes_geom* g, int hw, int dt
     R.grad = grad;
     R.lcur = 0;
     R.nsteps = nsteps;
+    R.pair = pennes_adj2_supported(geo);
     R.fwd_steps = R.adj_steps = 0;
     R.J = 0.0;
     R.work = nullptr;
@@ -260,10 +297,12 @@ int scafes_pennes_gradient(scafes_ctx* ctx, const scafes_geom* g, int hw, int dt
     if (!rc) rc = take(&R.work, R.sbytes);
     if (!rc) rc = take(&R.lam[0], abytes);
     if (!rc) rc = take(&R.lam[1], abytes);
-    for (int s = 0; !rc && s < snaps; ++s) {
-        void* b = nullptr;
-        rc = take(&b, R.sbytes);
-        if (!rc) R.slots.push_back(b);
+    // one allocation for the whole snapshot pool (a store-all budget of a
+    // 256^3 x 2000-step FP32 run is 134 GB; HBM holds 288 GB)
+    if (!rc && snaps > 0) {
+        void* pool = nullptr;
+        rc = take(&pool, (size_t)snaps * R.sbytes);
+        for (int s = 0; !rc && s < snaps; ++s) R.slots.push_back((char*)pool + (size_t)s * R.sbytes);
     }
     if (!rc) {
         hipError_t e = hipMemsetAsync(grad, 0, (size_t)ne * sizeof(double), c->stream);

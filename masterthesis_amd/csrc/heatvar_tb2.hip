@@ -103,7 +103,6 @@ __device__ inline T bload(Rsrc r, uint32_t off)
     if constexpr (sizeof(T) == 4) {
         return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
     } else {
-        typedef int v2i __attribute__((ext_vector_type(2)));
         return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
     }
 }

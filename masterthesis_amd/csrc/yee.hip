@@ -10,8 +10,10 @@
 // tangential E zero on the walls.  Fields: Ex,Ey,Ez,Hx,Hy,Hz, each (N+1)^3
 // nodes, x fastest; Ex(i,j,k) sits at (i+1/2, j, k), Hx at (i, j+1/2, k+1/2)...
 //
-// Each half step is a z-streaming in-place sweep (a half step reads only
-// the other field triple, so in place is race free): the centre column
+// Two schedules.  With a second state buffer, one out-of-place sweep per
+// step does both half steps (yee_fused_kernel, 48 B per cell-step).
+// Without one, each half step is a z-streaming in-place sweep (a half step
+// reads only the other field triple, so in place is race free): the centre column
 // rides in a register queue, x+1 / y+1 (or x-1 / y-1) neighbours come
 // from L1/L2 lines the neighbouring lanes and rows just fetched.  The H
 // sweep optionally accumulates the discrete energy
@@ -132,6 +134,194 @@ __global__ __launch_bounds__(TX* TY) void yee_e_kernel(YeeDims d, T s, T* __rest
     }
 }
 
+// Buffer access to one z plane of one field (wave-uniform resource + 32-bit
+// in-plane byte offset per thread; see heatvar_tb2.hip).
+using YRsrc = __amdgpu_buffer_rsrc_t;
+constexpr int kYRsrcWord3 = 0x00020000;
+
+__device__ inline YRsrc yrsrc(const char* p, uint32_t range)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(p), (short)0, (int)range, kYRsrcWord3);
+}
+
+template <typename T>
+__device__ inline T yld(YRsrc r, uint32_t off)
+{
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+    }
+}
+
+template <typename T>
+__device__ inline void yst(T v, YRsrc r, uint32_t off)
+{
+    if constexpr (sizeof(T) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)off, 0, 0);
+    } else {
+        typedef int v2i __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2i, v), r, (int)off, 0, 0);
+    }
+}
+
+// One full leapfrog step in one sweep, out of place (fa -> fb, both holding
+// Ex,Ey,Ez,Hx,Hy,Hz).  Per z plane k: H^{n+1/2}(k) from E^n(k) (LDS plane
+// with a one-node ring on the +x/+y side) and E^n(k+1) (register queue);
+// then, one plane behind, E^{n+1}(k-1) from H^{n+1/2}(k-1) (LDS plane) and
+// H^{n+1/2}(k-2) (registers).  Threads on the -x/-y edge of the TX x TY
+// workgroup compute H for the neighbouring tile's last column/row (needed
+// by the E differences) and store nothing; the (TX-1) x (TY-1) others own
+// their nodes and write every field value there (updated or copied), so fb
+// is a complete state.  Per cell and step HBM moves 6 reads + 6 writes.
+// The arithmetic per value is that of yee_h_kernel / yee_e_kernel, so the
+// result is bit-identical to the two-pass in-place step.
+template <typename T, int TX, int TY, int D>
+__global__ __launch_bounds__(TX* TY) void yee_fused_kernel(YeeDims d, T s, const T* __restrict__ fa,
+                                                            T* __restrict__ fb, int zc, int tiles_x, int tiles_y,
+                                                            double* __restrict__ epart)
+{
+    static_assert(D >= 2 && D % 2 == 0, "prefetch depth must be even");
+    constexpr int OX = TX - 1, OY = TY - 1;
+    __shared__ T sEx[2][TY + 1][TX + 1], sEy[2][TY + 1][TX + 1], sEz[2][TY + 1][TX + 1];
+    __shared__ T sHx[2][TY][TX], sHy[2][TY][TX], sHz[2][TY][TX];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int per = tiles_x * tiles_y;
+    const int tz = lb / per, r = lb - tz * per, tyi = r / tiles_x, txi = r - tyi * tiles_x;
+    const int N = d.N;
+    const int i = txi * OX - 1 + tx, j = tyi * OY - 1 + ty;
+    const int ic = min(max(i, 0), N), jc = min(max(j, 0), N);
+    const int ip = min(ic + 1, N), jp = min(jc + 1, N);
+    const bool out = tx >= 1 && ty >= 1 && i <= N && j <= N;
+    const bool rx = tx == TX - 1, ry = ty == TY - 1;  // +x / +y ring duty
+    const int k0 = tz * zc, k1 = min(k0 + zc, N + 1);
+    constexpr uint32_t B = sizeof(T);
+    const uint32_t oc = (uint32_t)(ic + d.M * jc) * B;
+    const uint32_t ox = (uint32_t)(ip + d.M * jc) * B;
+    const uint32_t oy = (uint32_t)(ic + d.M * jp) * B;
+    const uint32_t range = (uint32_t)(d.M2 * B);
+    const size_t fs = (size_t)d.nf * B, pb = (size_t)d.M2 * B;
+    const char* a8 = (const char*)fa;
+    char* b8 = (char*)fb;
+    auto plane = [&](int kk) { return (size_t)min(max(kk, 0), N) * pb; };
+
+    T qex[D], qey[D], qez[D], qhx[D], qhy[D], qhz[D], rxy[D], rxz[D], ryx[D], ryz[D];
+#pragma unroll
+    for (int q = 0; q < D; ++q) qex[q] = qey[q] = qez[q] = qhx[q] = qhy[q] = qhz[q] = rxy[q] = rxz[q] = ryx[q] =
+        ryz[q] = T(0);
+    auto fetch = [&](const int slot, const int kk) {
+        if (kk <= k1 + 1) {
+            const char* p = a8 + plane(kk);
+            const YRsrc rEx = yrsrc(p, range), rEy = yrsrc(p + fs, range), rEz = yrsrc(p + 2 * fs, range);
+            qex[slot] = yld<T>(rEx, oc);
+            qey[slot] = yld<T>(rEy, oc);
+            qez[slot] = yld<T>(rEz, oc);
+            if (kk <= k1) {
+                qhx[slot] = yld<T>(yrsrc(p + 3 * fs, range), oc);
+                qhy[slot] = yld<T>(yrsrc(p + 4 * fs, range), oc);
+                qhz[slot] = yld<T>(yrsrc(p + 5 * fs, range), oc);
+                if (rx) {
+                    rxy[slot] = yld<T>(rEy, ox);
+                    rxz[slot] = yld<T>(rEz, ox);
+                }
+                if (ry) {
+                    ryx[slot] = yld<T>(rEx, oy);
+                    ryz[slot] = yld<T>(rEz, oy);
+                }
+            }
+        }
+    };
+#pragma unroll
+    for (int q = 0; q < D; ++q) fetch(q, k0 - 1 + q);
+
+    T pex = T(0), pey = T(0), pez = T(0);  // E^n at plane k-1
+    T phx = T(0), phy = T(0), phz = T(0);  // H^{n+1/2} at plane k-1
+    T qqx = T(0), qqy = T(0);              // Hx, Hy^{n+1/2} at plane k-2
+    double esum = 0.0;
+    for (int kb = k0 - 1; kb <= k1; kb += D) {
+#pragma unroll
+        for (int q = 0; q < D; ++q) {
+            const int k = kb + q;
+            if (k > k1) break;  // uniform over the workgroup
+            const int buf = q & 1;
+            const int q1 = (q + 1) % D;
+            const T ex = qex[q], ey = qey[q], ez = qez[q];
+            const T exu = qex[q1], eyu = qey[q1];
+            const T hx = qhx[q], hy = qhy[q], hz = qhz[q];
+            sEx[buf][ty][tx] = ex;
+            sEy[buf][ty][tx] = ey;
+            sEz[buf][ty][tx] = ez;
+            if (rx) {
+                sEy[buf][ty][TX] = rxy[q];
+                sEz[buf][ty][TX] = rxz[q];
+            }
+            if (ry) {
+                sEx[buf][TY][tx] = ryx[q];
+                sEz[buf][TY][tx] = ryz[q];
+            }
+            fetch(q, k + D);
+            __syncthreads();
+            // ---- H^{n+1/2} at plane k (every thread position)
+            const T ey_xp = sEy[buf][ty][tx + 1], ez_xp = sEz[buf][ty][tx + 1];
+            const T ex_yp = sEx[buf][ty + 1][tx], ez_yp = sEz[buf][ty + 1][tx];
+            T hxn = hx, hyn = hy, hzn = hz;
+            if (j < N && k < N) hxn = hx + s * ((eyu - ey) - (ez_yp - ez));
+            if (i < N && k < N) hyn = hy + s * ((ez_xp - ez) - (exu - ex));
+            if (i < N && j < N) hzn = hz + s * ((ex_yp - ex) - (ey_xp - ey));
+            sHx[buf][ty][tx] = hxn;
+            sHy[buf][ty][tx] = hyn;
+            sHz[buf][ty][tx] = hzn;
+            if (out && k >= k0 && k < k1) {
+                const size_t o = (size_t)k * pb;
+                yst<T>(hxn, yrsrc(b8 + o + 3 * fs, range), oc);
+                yst<T>(hyn, yrsrc(b8 + o + 4 * fs, range), oc);
+                yst<T>(hzn, yrsrc(b8 + o + 5 * fs, range), oc);
+                if (epart) {
+                    const double e = (double)ex * ex + (double)ey * ey + (double)ez * ez;
+                    const double h = (double)hx * hxn + (double)hy * hyn + (double)hz * hzn;
+                    esum += 0.5 * e + 0.5 * h;
+                }
+            }
+            // ---- E^{n+1} at plane k-1 from the H plane written one iteration ago
+            const int km = k - 1;
+            if (out && km >= k0) {
+                const int pbf = buf ^ 1;
+                const T hz_ym = sHz[pbf][ty - 1][tx], hx_ym = sHx[pbf][ty - 1][tx];
+                const T hz_xm = sHz[pbf][ty][tx - 1], hy_xm = sHy[pbf][ty][tx - 1];
+                const bool in_k = km >= 1 && km < N, in_j = j >= 1 && j < N, in_i = i >= 1 && i < N;
+                T exn = pex, eyn = pey, ezn = pez;
+                if (i < N && in_j && in_k) exn = pex + s * ((phz - hz_ym) - (phy - qqy));
+                if (j < N && in_i && in_k) eyn = pey + s * ((phx - qqx) - (phz - hz_xm));
+                if (km < N && in_i && in_j) ezn = pez + s * ((phy - hy_xm) - (phx - hx_ym));
+                const size_t o = (size_t)km * pb;
+                yst<T>(exn, yrsrc(b8 + o, range), oc);
+                yst<T>(eyn, yrsrc(b8 + o + fs, range), oc);
+                yst<T>(ezn, yrsrc(b8 + o + 2 * fs, range), oc);
+            }
+            qqx = phx;
+            qqy = phy;
+            phx = hxn;
+            phy = hyn;
+            phz = hzn;
+            pex = ex;
+            pey = ey;
+            pez = ez;
+        }
+    }
+    if (epart) {
+        __shared__ double sh[TX * TY];
+        const int t = tx + TX * ty;
+        sh[t] = esum;
+        __syncthreads();
+        for (int w = (TX * TY) / 2; w > 0; w >>= 1) {
+            if (t < w) sh[t] += sh[t + w];
+            __syncthreads();
+        }
+        if (t == 0) epart[blockIdx.x] = sh[0];
+    }
+}
+
 __global__ __launch_bounds__(256) void energy_final(const double* partial, int n, double* out)
 {
     __shared__ double sh[256];
@@ -164,17 +354,23 @@ __global__ void yee_pulse_kernel(YeeDims d, T* f, double sigma)
     }
 }
 
-constexpr int kYTX = 64, kYTY = 4, kYZC = 64;
+constexpr int kYTX = 64, kYTY = 4, kYZC = 64;         // two-pass in-place tiles
+constexpr int kFTX = 64, kFTY = 8, kFZC = 64, kFD = 2;  // fused out-of-place tile
 
 template <typename T>
-static int yee_run_t(Ctx* c, int N, double s, T* f, int nsteps, int energy_every, double* energy)
+static int yee_run_t(Ctx* c, int N, double s, T* f, T* f2, int nsteps, int energy_every, double* energy,
+                     int* result_in_second)
 {
     YeeDims d;
     d.N = N;
     d.M = N + 1;
     d.M2 = d.M * d.M;
     d.nf = d.M2 * d.M;
-    const int tiles_x = ceil_div(N + 1, kYTX), tiles_y = ceil_div(N + 1, kYTY), tiles_z = ceil_div(N + 1, kYZC);
+    const bool fused = f2 != nullptr;
+    if (fused && d.M2 * (long)sizeof(T) >= (1L << 32)) return fail(SCAFES_ERR_ARG, "yee: plane too large");
+    const int TX = fused ? kFTX : kYTX, TY = fused ? kFTY : kYTY, ZC = fused ? kFZC : kYZC;
+    const int OX = fused ? TX - 1 : TX, OY = fused ? TY - 1 : TY;
+    const int tiles_x = ceil_div(N + 1, OX), tiles_y = ceil_div(N + 1, OY), tiles_z = ceil_div(N + 1, ZC);
     const long nb = (long)tiles_x * tiles_y * tiles_z;
     if (nb > 0x7fffffffL) return fail(SCAFES_ERR_ARG, "yee: grid too large");
     const int nrec = (energy_every > 0) ? (nsteps + energy_every - 1) / energy_every : 0;
@@ -184,20 +380,30 @@ static int yee_run_t(Ctx* c, int N, double s, T* f, int nsteps, int energy_every
         SCAFES_HIP(hipMallocAsync((void**)&epart, sizeof(double) * nb, c->stream));
         SCAFES_HIP(hipMallocAsync((void**)&edev, sizeof(double) * nrec, c->stream));
     }
-    const dim3 blk(kYTX, kYTY);
+    const dim3 blk(TX, TY);
     int rc = SCAFES_OK;
+    T* a = f;
+    T* b = f2;
     for (int st = 0; st < nsteps; ++st) {
         const bool rec = nrec > 0 && (st % energy_every) == 0;
-        yee_h_kernel<T, kYTX, kYTY><<<dim3((unsigned)nb), blk, 0, c->stream>>>(d, (T)s, f, kYZC, tiles_x, tiles_y,
-                                                                              rec ? epart : nullptr);
+        if (fused) {
+            yee_fused_kernel<T, kFTX, kFTY, kFD><<<dim3((unsigned)nb), blk, 0, c->stream>>>(
+                d, (T)s, a, b, ZC, tiles_x, tiles_y, rec ? epart : nullptr);
+            std::swap(a, b);
+        } else {
+            yee_h_kernel<T, kYTX, kYTY><<<dim3((unsigned)nb), blk, 0, c->stream>>>(d, (T)s, f, ZC, tiles_x,
+                                                                                  tiles_y, rec ? epart : nullptr);
+        }
         if (rec) energy_final<<<1, 256, 0, c->stream>>>(epart, (int)nb, edev + st / energy_every);
-        yee_e_kernel<T, kYTX, kYTY><<<dim3((unsigned)nb), blk, 0, c->stream>>>(d, (T)s, f, kYZC, tiles_x, tiles_y);
+        if (!fused)
+            yee_e_kernel<T, kYTX, kYTY><<<dim3((unsigned)nb), blk, 0, c->stream>>>(d, (T)s, f, ZC, tiles_x, tiles_y);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
             rc = check_hip(e, "yee kernels");
             break;
         }
     }
+    if (result_in_second) *result_in_second = (fused && (nsteps % 2 == 1)) ? 1 : 0;
     if (nrec > 0) {
         if (rc == SCAFES_OK) {
             hipError_t e = hipMemcpyAsync(energy, edev, sizeof(double) * nrec, hipMemcpyDeviceToHost, c->stream);
@@ -216,17 +422,21 @@ using namespace scafes;
 
 extern "C" {
 
-int scafes_yee_run(scafes_ctx* ctx, int ncell, int dtype, double s, void* f, int nsteps, int energy_every,
-                   double* energy)
+int scafes_yee_run(scafes_ctx* ctx, int ncell, int dtype, double s, void* f, void* f2, int nsteps,
+                   int energy_every, double* energy, int* result_in_second)
 {
     Ctx* c = as_ctx(ctx);
     if (!c || !f) return fail(SCAFES_ERR_ARG, "yee: null pointer");
+    if (f2 == f) return fail(SCAFES_ERR_ARG, "yee: second state buffer must differ from the first");
     if (ncell < 2 || nsteps < 0) return fail(SCAFES_ERR_ARG, "yee: need ncell >= 2 and nsteps >= 0");
     if (!(s > 0.0) || s > 1.0 / 1.7320508075688772)
         return fail(SCAFES_ERR_ARG, "yee: Courant number must satisfy 0 < s <= 1/sqrt(3)");
     if (energy_every > 0 && !energy) return fail(SCAFES_ERR_ARG, "yee: energy output missing");
-    if (dtype == SCAFES_F32) return yee_run_t<float>(c, ncell, s, (float*)f, nsteps, energy_every, energy);
-    if (dtype == SCAFES_F64) return yee_run_t<double>(c, ncell, s, (double*)f, nsteps, energy_every, energy);
+    if (dtype == SCAFES_F32)
+        return yee_run_t<float>(c, ncell, s, (float*)f, (float*)f2, nsteps, energy_every, energy, result_in_second);
+    if (dtype == SCAFES_F64)
+        return yee_run_t<double>(c, ncell, s, (double*)f, (double*)f2, nsteps, energy_every, energy,
+                                 result_in_second);
     return fail(SCAFES_ERR_ARG, "yee: bad dtype");
 }
 

@@ -232,6 +232,7 @@ public:
         mKnownA = DeviceBuffer();
         mKnownB = DeviceBuffer();
         mYee = DeviceBuffer();
+        mYee2 = DeviceBuffer();
         mRef.clear();
         mCtx.reset();
     }
@@ -388,6 +389,7 @@ private:
         if (mDev.kernel == DeviceKernel::YEE) {
             const std::size_t nn = (std::size_t)nNodes().size();
             mYee = DeviceBuffer(*mCtx, 6 * nn * sizeof(float));
+            mYee2 = DeviceBuffer(*mCtx, 6 * nn * sizeof(float));  // out-of-place fused step
             std::vector<float> tmp(6 * nn);
             for (int c = 0; c < 6; ++c) packNodes(mLumpUnknownNew[unknownSlot(mDev.yee[c])], mDev.yee[c], &tmp[c * nn]);
             mYee.upload(tmp.data(), tmp.size() * sizeof(float));
@@ -525,9 +527,11 @@ private:
             // one fused leapfrog step on the device (reference :2603-2619)
             double e = 0.0;
             const bool rec = mDev.energyEvery > 0 && (n % mDev.energyEvery) == 0;
-            deviceCheck(scafes_yee_run(c, nNodes(0) - 1, SCAFES_F32, mDev.courant, mYee.get(), 1, rec ? 1 : 0,
-                                       rec ? &e : 0),
+            int second = 0;
+            deviceCheck(scafes_yee_run(c, nNodes(0) - 1, SCAFES_F32, mDev.courant, mYee.get(), mYee2.get(), 1,
+                                       rec ? 1 : 0, rec ? &e : 0, &second),
                         "yee step");
+            if (second) std::swap(mYee, mYee2);
             if (rec) mEnergies.push_back(e);
             // the reference swaps unknown pointers and times twice inside a
             // leapfrog step; the net effect on the time slots is nil
@@ -705,7 +709,7 @@ private:
     // device state
     DeviceUpdate mDev;
     std::unique_ptr<DeviceContext> mCtx;
-    DeviceBuffer mUOld, mUNew, mCOld, mCNew, mKnownA, mKnownB, mYee;
+    DeviceBuffer mUOld, mUNew, mCOld, mCNew, mKnownA, mKnownB, mYee, mYee2;
     std::vector<DeviceBuffer> mRef;
 };
 

@@ -35,15 +35,25 @@ def random_pec_state(N, seed):
     return f
 
 
-def run_device(ctx, N, s, f0, nsteps, dtype, energy_every=0):
+def run_device(ctx, N, s, f0, nsteps, dtype, energy_every=0, fused=True):
+    """fused: one out-of-place sweep per step (ping-pong buffers); else the
+    two in-place half-step sweeps."""
     d = ctx.upload(f0.astype(dtype))
+    d2 = ctx.alloc(d.count, dtype) if fused else None
+    if d2 is not None:
+        d2.fill(-7.0)  # every value must be written by the sweep
     code = capi.F64 if dtype == np.float64 else capi.F32
     nrec = (nsteps + energy_every - 1) // energy_every if energy_every else 0
     e = np.zeros(max(nrec, 1))
-    capi.check(ctx.L.scafes_yee_run(ctx.h, N, code, s, d.ptr, nsteps, energy_every,
-                                    e.ctypes.data_as(capi.C.POINTER(capi.C.c_double)) if nrec else None))
-    out = d.read().reshape(f0.shape)
+    second = capi.C.c_int(-1)
+    capi.check(ctx.L.scafes_yee_run(ctx.h, N, code, s, d.ptr, d2.ptr if d2 else None, nsteps, energy_every,
+                                    e.ctypes.data_as(capi.C.POINTER(capi.C.c_double)) if nrec else None,
+                                    capi.C.byref(second)))
+    assert second.value == (1 if fused and nsteps % 2 else 0)
+    out = (d2 if second.value else d).read().reshape(f0.shape)
     d.free()
+    if d2 is not None:
+        d2.free()
     return out, e[:nrec]
 
 
@@ -59,16 +69,31 @@ def oracle_run(N, s, f0, nsteps, energy_every=0):
     return f, np.array(energies)
 
 
+@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("N", [16, 37])
-def test_yee_fp64_bitexact_with_energy(ctx, N):
+def test_yee_fp64_bitexact_with_energy(ctx, N, fused):
     s = 0.5
     f0 = random_pec_state(N, N)
     ref, eref = oracle_run(N, s, f0, 25, energy_every=4)
-    got, e = run_device(ctx, N, s, f0, 25, np.float64, energy_every=4)
+    got, e = run_device(ctx, N, s, f0, 25, np.float64, energy_every=4, fused=fused)
     np.testing.assert_array_equal(got, ref)
     np.testing.assert_allclose(e, eref, rtol=1e-12)
     # the scheme conserves it
     assert np.abs(e - e[0]).max() < 1e-11 * e[0]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("nsteps", [3, 4])
+def test_yee_fused_matches_two_pass(ctx, dtype, nsteps):
+    """The fused sweep does the two half steps' arithmetic value by value:
+    identical bits, over several z chunks and a tile grid that does not
+    divide the box; energies agree to FP64 summation-order round-off."""
+    N, s = 150, 0.55
+    f0 = random_pec_state(N, 9)
+    a, ea = run_device(ctx, N, s, f0, nsteps, dtype, energy_every=1, fused=True)
+    b, eb = run_device(ctx, N, s, f0, nsteps, dtype, energy_every=1, fused=False)
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_allclose(ea, eb, rtol=1e-13)
 
 
 def test_yee_fp32_error_and_energy_drift(ctx):
@@ -112,4 +137,6 @@ def test_yee_init_pulse(ctx):
 def test_yee_rejects_unstable_courant(ctx):
     d = ctx.alloc(6 * 9 ** 3, np.float32)
     with pytest.raises(capi.ScafesError):
-        capi.check(ctx.L.scafes_yee_run(ctx.h, 8, capi.F32, 0.6, d.ptr, 1, 0, None))
+        capi.check(ctx.L.scafes_yee_run(ctx.h, 8, capi.F32, 0.6, d.ptr, None, 1, 0, None, None))
+    with pytest.raises(capi.ScafesError):  # the second buffer must be distinct
+        capi.check(ctx.L.scafes_yee_run(ctx.h, 8, capi.F32, 0.5, d.ptr, d.ptr, 1, 0, None, None))
