@@ -337,6 +337,9 @@ private:
             unknown(mDev.unknown);
             known(mDev.perfusionKnown, "perfusionKnown");
             known(mDev.sourceFieldKnown, "sourceFieldKnown");
+            if (mStencilWidth[mDev.perfusionKnown] != mStencilWidth[mDev.unknown] ||
+                mStencilWidth[mDev.sourceFieldKnown] != mStencilWidth[mDev.unknown])
+                throw std::runtime_error("DeviceUpdate: perfusion and source need the stencil width of the updated field");
             if (mDev.precision == DevicePrecision::F32_KAHAN)
                 throw std::runtime_error("DeviceUpdate: PENNES runs in FP64 or FP32");
             break;
@@ -344,6 +347,8 @@ private:
             if (!mUseLeapfrog) throw std::runtime_error("DeviceUpdate: YEE needs the leapfrog mode");
             for (int c = 0; c < 6; ++c) unknown(mDev.yee[c]);
             if (DIM != 3) throw std::runtime_error("DeviceUpdate: YEE is three-dimensional");
+            if (mDev.precision == DevicePrecision::F32_KAHAN)
+                throw std::runtime_error("DeviceUpdate: YEE runs in FP64 or FP32");
             for (std::size_t p = 1; p < DIM; ++p)
                 if (nNodes(p) != nNodes(0)) throw std::runtime_error("DeviceUpdate: YEE needs a cubic grid");
             break;
@@ -388,11 +393,20 @@ private:
         const bool fl = f32();
         if (mDev.kernel == DeviceKernel::YEE) {
             const std::size_t nn = (std::size_t)nNodes().size();
-            mYee = DeviceBuffer(*mCtx, 6 * nn * sizeof(float));
-            mYee2 = DeviceBuffer(*mCtx, 6 * nn * sizeof(float));  // out-of-place fused step
-            std::vector<float> tmp(6 * nn);
-            for (int c = 0; c < 6; ++c) packNodes(mLumpUnknownNew[unknownSlot(mDev.yee[c])], mDev.yee[c], &tmp[c * nn]);
-            mYee.upload(tmp.data(), tmp.size() * sizeof(float));
+            const std::size_t es = fl ? sizeof(float) : sizeof(double);
+            mYee = DeviceBuffer(*mCtx, 6 * nn * es);
+            mYee2 = DeviceBuffer(*mCtx, 6 * nn * es);  // out-of-place fused step
+            if (fl) {
+                std::vector<float> tmp(6 * nn);
+                for (int c = 0; c < 6; ++c)
+                    packNodes(mLumpUnknownNew[unknownSlot(mDev.yee[c])], mDev.yee[c], &tmp[c * nn]);
+                mYee.upload(tmp.data(), tmp.size() * es);
+            } else {
+                std::vector<double> tmp(6 * nn);
+                for (int c = 0; c < 6; ++c)
+                    packNodes(mLumpUnknownNew[unknownSlot(mDev.yee[c])], mDev.yee[c], &tmp[c * nn]);
+                mYee.upload(tmp.data(), tmp.size() * es);
+            }
         } else {
             const int ii = mDev.unknown;
             const int u = unknownSlot(ii);
@@ -451,8 +465,9 @@ private:
         return df.elemData() == mLumpKnownOld[ii].data() ? mLumpKnownOld[ii] : mLumpKnownNew[ii];
     }
 
-    // node block (without ghosts) of an unknown lump, as FP32
-    void packNodes(const std::vector<CT>& lump, int ii, float* dst) const
+    // node block (without ghosts) of an unknown lump, as FP32 or FP64
+    template <typename T>
+    void packNodes(const std::vector<CT>& lump, int ii, T* dst) const
     {
         const GridSub<DIM>& part = mGG.partition(0);
         const Grid<DIM>& m = mMemAll[ii];
@@ -460,10 +475,11 @@ private:
         for (typename GridSub<DIM>::iterator it = part.begin(); it != part.end(); ++it) {
             unsigned long s = 0;
             m.idxNodeTuple2Scalar(s, *it);
-            dst[k++] = (float)lump[s];
+            dst[k++] = (T)lump[s];
         }
     }
-    void unpackNodes(std::vector<CT>& lump, int ii, const float* src) const
+    template <typename T>
+    void unpackNodes(std::vector<CT>& lump, int ii, const T* src) const
     {
         const GridSub<DIM>& part = mGG.partition(0);
         const Grid<DIM>& m = mMemAll[ii];
@@ -528,8 +544,8 @@ private:
             double e = 0.0;
             const bool rec = mDev.energyEvery > 0 && (n % mDev.energyEvery) == 0;
             int second = 0;
-            deviceCheck(scafes_yee_run(c, nNodes(0) - 1, SCAFES_F32, mDev.courant, mYee.get(), mYee2.get(), 1,
-                                       rec ? 1 : 0, rec ? &e : 0, &second),
+            deviceCheck(scafes_yee_run(c, nNodes(0) - 1, f32() ? SCAFES_F32 : SCAFES_F64, mDev.courant, mYee.get(),
+                                       mYee2.get(), 1, rec ? 1 : 0, rec ? &e : 0, &second),
                         "yee step");
             if (second) std::swap(mYee, mYee2);
             if (rec) mEnergies.push_back(e);
@@ -637,10 +653,19 @@ private:
             for (int q = 0; q < 6; ++q)
                 if (mDev.yee[q] == ii) c = q;
             if (c < 0) return;
-            std::vector<float> tmp(nn);
-            deviceCheck(scafes_memcpy_d2h(mCtx->get(), tmp.data(), (const float*)mYee.get() + c * nn, nn * sizeof(float)),
-                        "download");
-            unpackNodes(lump, ii, tmp.data());
+            if (f32()) {
+                std::vector<float> tmp(nn);
+                deviceCheck(scafes_memcpy_d2h(mCtx->get(), tmp.data(), (const float*)mYee.get() + c * nn,
+                                              nn * sizeof(float)),
+                            "download");
+                unpackNodes(lump, ii, tmp.data());
+            } else {
+                std::vector<double> tmp(nn);
+                deviceCheck(scafes_memcpy_d2h(mCtx->get(), tmp.data(), (const double*)mYee.get() + c * nn,
+                                              nn * sizeof(double)),
+                            "download");
+                unpackNodes(lump, ii, tmp.data());
+            }
             return;
         }
         if (ii != mDev.unknown) return;

@@ -47,3 +47,100 @@ def test_timelin_through_problem(tmp_path, hip, prog, n, pennes):
     np.testing.assert_array_equal(got, O.nodes_view(g, ref).ravel())
     # the "Local Linf" values are those of the CPU path
     np.testing.assert_allclose(linf, rlinf, rtol=0, atol=1e-15)
+
+
+def _pattern(n, a, b, c, m, d):
+    """((a i + b j + c k) mod m) / d on an (nz, ny, nx) node block."""
+    k, j, i = np.meshgrid(*(np.arange(x) for x in reversed(n)), indexing="ij")
+    return ((a * i + b * j + c * k) % m) / d, (i, j, k)
+
+
+def _family_args(n, nsteps, tau):
+    return ["--dim=3", "--nNodes=" + "x".join(map(str, n)), f"--nTimesteps={nsteps}", f"--tau={tau!r}"]
+
+
+@pytest.mark.parametrize("prog", ["heatvar_f64", "heatvar_f32_kahan"])
+def test_heatvar_problem_through_host_api(tmp_path, hip, prog):
+    """Variable-kappa problem class (kappa a known field) on the HEAT7VAR
+    kernel: FP64 equals the oracle bit for bit; FP32 + Kahan within 1e-6 of
+    the state's scale."""
+    n, nsteps = (21, 17, 13), 25
+    g = O.make_grid(n)
+    kap = np.zeros(O.memall_shape(g))
+    O.nodes_view(g, kap)[...] = 1.0 + _pattern(n, 1, 2, 3, 5, 8.0)[0]
+    u0 = np.zeros(O.memall_shape(g))
+    p, (i, j, k) = _pattern(n, 3, 5, 7, 13, 16.0)
+    border = (i == 0) | (j == 0) | (k == 0) | (i == n[0] - 1) | (j == n[1] - 1) | (k == n[2] - 1)
+    O.nodes_view(g, u0)[...] = np.where(border, 0.0, p)
+    tau = 0.9 / (2.0 * 1.5 * sum((m - 1.0) ** 2 for m in n))
+    dump = str(tmp_path / "u.bin")
+    run(prog, _family_args(n, nsteps, tau), dump)
+    ref = O.nodes_view(g, O.heat7var_run(g, u0, kap, tau, nsteps)).ravel()
+    got = np.fromfile(dump + ".0", dtype=np.float64)
+    if prog.endswith("f64"):
+        np.testing.assert_array_equal(got, ref)
+    else:
+        assert np.abs(got - ref).max() < 1e-6 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("prog", ["pennes_f64", "pennes_f32"])
+def test_pennes_problem_through_host_api(tmp_path, hip, prog):
+    """Pennes problem class with perfusion/source known fields on the PENNES
+    kernel: FP64 bit-identical to the oracle; FP32 (stored as T - T_a) within
+    1e-5 of the temperature rise."""
+    n, nsteps = (15, 18, 12), 30
+    g = O.make_grid(n)
+    w, q, t0 = (np.zeros(O.memall_shape(g)) for _ in range(3))
+    O.nodes_view(g, w)[...] = 0.5 + _pattern(n, 1, 1, 1, 4, 8.0)[0]
+    O.nodes_view(g, q)[...] = _pattern(n, 2, 1, 3, 7, 4.0)[0]
+    O.nodes_view(g, t0)[...] = 37.0 + _pattern(n, 1, 2, 1, 5, 4.0)[0]
+    h2 = sum(1.0 / O.lib().oracle_gridsize(g, p) ** 2 for p in range(3))
+    tau = 0.9 / (2.0 * 0.5 * h2 + 1.0)
+    dump = str(tmp_path / "t.bin")
+    run(prog, _family_args(n, nsteps, tau), dump)
+    coef = O.pennes_coef(tau=tau, cdiff=0.5, cperf=1.0, csrc=1.0, t_a=37.0)
+    ref = O.nodes_view(g, O.pennes_forward_all(g, t0, w, q, coef, nsteps)[-1]).ravel()
+    got = np.fromfile(dump + ".0", dtype=np.float64)
+    if prog.endswith("f64"):
+        np.testing.assert_array_equal(got, ref)
+    else:
+        rise = np.abs(ref - O.nodes_view(g, t0).ravel()).max()
+        assert np.abs(got - ref).max() < 1e-5 * rise
+
+
+@pytest.mark.parametrize("prog", ["yee_f64", "yee_f32"])
+def test_yee_problem_through_host_api(tmp_path, hip, prog):
+    """Leapfrog Yee problem class (six unknown fields, updateInner = H,
+    updateInner2 = E) on the fused device step: FP64 fields bit-identical to
+    the oracle's half steps and energies to round-off; FP32 within 1e-5."""
+    N, nsteps = 16, 21
+    tau = 0.5 / N
+    s = tau / (1.0 / N)
+    shape = (N + 1,) * 3
+    f = np.zeros((6,) + shape)
+    specs = [(3, 5, 7, 11, 16.0), (5, 7, 3, 13, 16.0), (7, 3, 5, 9, 16.0),
+             (1, 2, 3, 5, 64.0), (2, 3, 1, 7, 64.0), (3, 1, 2, 6, 64.0)]
+    for c, sp in enumerate(specs):
+        p, (i, j, k) = _pattern(shape, *sp)
+        inn = lambda a: (a >= 1) & (a < N)
+        valid = [(i < N) & inn(j) & inn(k), (j < N) & inn(i) & inn(k), (k < N) & inn(i) & inn(j),
+                 (j < N) & (k < N), (i < N) & (k < N), (i < N) & (j < N)][c]
+        f[c] = np.where(valid, p - (0.25 if c < 3 else 0.0), 0.0)
+    dump = str(tmp_path / "f.bin")
+    out = run(prog, _family_args(shape, nsteps, tau), dump)
+    energies = [float(x) for x in re.findall(r"ENERGY (\S+)", out)]
+    ref = f.copy()
+    eref = []
+    for _ in range(nsteps):
+        hprev = ref[3:].copy()
+        O.yee_half_h(N, s, ref)
+        eref.append(O.yee_energy(N, ref, hprev))
+        O.yee_half_e(N, s, ref)
+    got = np.stack([np.fromfile(f"{dump}.{c}", dtype=np.float64).reshape(shape) for c in range(6)])
+    assert len(energies) == nsteps
+    if prog.endswith("f64"):
+        np.testing.assert_array_equal(got, ref)
+        np.testing.assert_allclose(energies, eref, rtol=1e-12)
+    else:
+        assert np.abs(got - ref).max() < 1e-5 * np.abs(ref).max()
+        np.testing.assert_allclose(energies, eref, rtol=1e-5)
