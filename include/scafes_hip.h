@@ -48,8 +48,27 @@ enum { SCAFES_F32 = 0, SCAFES_F64 = 1, SCAFES_F32_KAHAN = 2 };
 /* Border rule at GLOBAL_BORDER nodes (GridGlobal.hpp:138-151 flags):
  * KEEP  = the hook leaves the copied old value (Problem.hpp:6226-6256),
  * FIELD = Dirichlet from a known field G_new (Dirichlet.hpp:124-130),
- * VALUE = Dirichlet from a constant. */
-enum { SCAFES_BORDER_KEEP = 0, SCAFES_BORDER_FIELD = 1, SCAFES_BORDER_VALUE = 2 };
+ * VALUE = Dirichlet from a constant,
+ * NEUMANN / CAUCHY / RADIATION = the flux border classes of the system tests
+ *   (tests/system/HeatEqnFDM/Neumann/Neumann.hpp:126-168,
+ *    tests/system/HeatEqnFDM/Cauchy/Cauchy.hpp:126-170,
+ *    tests/system/PennesBioheatEqnFDM/Radiation/Radiation.hpp:126-175):
+ *   a border node with index 0 on an active axis takes G_new; otherwise, on
+ *   each axis where its index is n_p - 1, the outer neighbour is replaced by
+ *   the inner one W and a flux term from G_old (A = 2 h_p / lambda):
+ *     NEUMANN   (W + W) - A*G_old - 2u
+ *     CAUCHY    ((W + W) - A*alpha*u) + A*G_old - 2u
+ *     RADIATION ((W + W) - A*eps*sigma*u*u*u*u) + A*G_old - 2u
+ *   (central differences on the other axes, then perfusion and source as
+ *   on inner nodes).  heat7 only. */
+enum {
+    SCAFES_BORDER_KEEP = 0,
+    SCAFES_BORDER_FIELD = 1,
+    SCAFES_BORDER_VALUE = 2,
+    SCAFES_BORDER_NEUMANN = 3,
+    SCAFES_BORDER_CAUCHY = 4,
+    SCAFES_BORDER_RADIATION = 5
+};
 
 /* Discrete domain of one partition covering the whole grid
  * (GridGlobal::discreteDomain(), GridGlobal.hpp:384-422). */
@@ -92,17 +111,19 @@ int scafes_convert(scafes_ctx* ctx, void* dst, int dst_dtype, const void* src, i
  * operation order (bit-identical to the CPU path).
  *   inner:  u' = u + sum_p (tau*a*(W+E-2u))/h_p^2 [- ((tau*perf_c)*perf_w)*u]
  *                  [+ (tau*rhoc_inv)*F_old]
- * u_old/u_new: memAll with hw ghost layers; f_old/g_new: stencil-0 fields
- * (NULL when unused).  Ghost layers of u_new are not written. */
+ * u_old/u_new: memAll with hw ghost layers; f_old/g_old/g_new: stencil-0
+ * fields (NULL when unused; g_old is read by the flux border rules only).
+ * Ghost layers of u_new are not written. */
 typedef struct {
     double tau, coeff_a, rhoc_inv, perf_c, perf_w;
     int has_perf, has_source;
     int border_mode;
     double border_value;
+    double lambda, alpha, epsilon, sigma;  /* flux border coefficients */
 } scafes_heat7_params;
 int scafes_heat7_step_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const double* u_old,
-                          double* u_new, const double* f_old, const double* g_new,
-                          const scafes_heat7_params* p);
+                          double* u_new, const double* f_old, const double* g_old,
+                          const double* g_new, const scafes_heat7_params* p);
 
 /* ---- variable-coefficient conservative 7-point heat -----------------------
  *   inner: u' = u + sum_p tau*(0.5(kE+kC)(uE-uC) - 0.5(kC+kW)(uC-uW))/h_p^2

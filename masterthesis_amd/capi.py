@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libscafes_hip.so")
 
 F32, F64, F32_KAHAN = 0, 1, 2
 BORDER_KEEP, BORDER_FIELD, BORDER_VALUE = 0, 1, 2
+BORDER_NEUMANN, BORDER_CAUCHY, BORDER_RADIATION = 3, 4, 5
 HEATVAR_F64, HEATVAR_F32, HEATVAR_F32_KAHAN = 0, 1, 2
 RED_MAX_ABS_DIFF, RED_SUM_SQ_DIFF, RED_SUM_PROD = 0, 1, 2
 
@@ -30,7 +31,8 @@ class Geom(C.Structure):
 class Heat7Params(C.Structure):
     _fields_ = [("tau", C.c_double), ("coeff_a", C.c_double), ("rhoc_inv", C.c_double),
                 ("perf_c", C.c_double), ("perf_w", C.c_double), ("has_perf", C.c_int),
-                ("has_source", C.c_int), ("border_mode", C.c_int), ("border_value", C.c_double)]
+                ("has_source", C.c_int), ("border_mode", C.c_int), ("border_value", C.c_double),
+                ("lam", C.c_double), ("alpha", C.c_double), ("epsilon", C.c_double), ("sigma", C.c_double)]
 
 
 class PennesParams(C.Structure):
@@ -74,7 +76,7 @@ def load():
     _sig(L, "scafes_memcpy_d2d", [vp, vp, vp, C.c_size_t])
     _sig(L, "scafes_fill", [vp, vp, C.c_int, C.c_size_t, C.c_double])
     _sig(L, "scafes_convert", [vp, vp, C.c_int, vp, C.c_int, C.c_size_t])
-    _sig(L, "scafes_heat7_step_f64", [vp, gp, C.c_int, vp, vp, vp, vp, C.POINTER(Heat7Params)])
+    _sig(L, "scafes_heat7_step_f64", [vp, gp, C.c_int, vp, vp, vp, vp, vp, C.POINTER(Heat7Params)])
     _sig(L, "scafes_heat7var_run", [vp, gp, C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_double,
                                     C.c_int, C.c_int, C.c_double, C.c_int, ip])
     _sig(L, "scafes_heat7var_num_cfgs", [])

@@ -150,10 +150,28 @@ struct HeatConstOp {
     const double* x[1];
     double* out;
     const double* f_old;
+    const double* g_old;
     const double* g_new;
     Geo gk;  // layout of the stencil-0 known fields
     double ctau_a, ctw, ctf, hh0, hh1, hh2, border_value;
+    double a2[3], alpha, epsilon, sigma;  // flux borders: a2[p] = 2 h_p / lambda
     int has_perf, has_source, border_mode;
+    // Second difference along one axis at a flux border node: the outer
+    // neighbour replaced by the inner one w plus the flux term.
+    __device__ double flux_diff(double w, double u, double a, double gold) const
+    {
+        double t = w + w;
+        if (border_mode == SCAFES_BORDER_NEUMANN) {
+            t = t - a * gold;
+        } else if (border_mode == SCAFES_BORDER_CAUCHY) {
+            t = t - a * alpha * u;
+            t = t + a * gold;
+        } else {
+            t = t - a * epsilon * sigma * u * u * u * u;
+            t = t + a * gold;
+        }
+        return t - 2.0 * u;
+    }
     __device__ void apply(const Geo& g, int i, int j, int k, const St7<double>& s,
                           const St7<double>&, const double*) const
     {
@@ -170,6 +188,22 @@ struct HeatConstOp {
             out[m] = g_new[at(gk, i, j, k)];
         } else if (border_mode == SCAFES_BORDER_VALUE) {
             out[m] = border_value;
+        } else if (border_mode >= SCAFES_BORDER_NEUMANN) {
+            const long q = at(gk, i, j, k);
+            if (i == 0 || (g.dim > 1 && j == 0) || (g.dim > 2 && k == 0)) {
+                out[m] = g_new[q];
+                return;
+            }
+            const double gold = g_old[q];
+            double v = s.c;
+            v += (ctau_a * (i == g.nx - 1 ? flux_diff(s.w, s.c, a2[0], gold) : (s.w + s.e) - 2.0 * s.c)) / hh0;
+            if (g.dim > 1)
+                v += (ctau_a * (j == g.ny - 1 ? flux_diff(s.s, s.c, a2[1], gold) : (s.s + s.n) - 2.0 * s.c)) / hh1;
+            if (g.dim > 2)
+                v += (ctau_a * (k == g.nz - 1 ? flux_diff(s.d, s.c, a2[2], gold) : (s.d + s.u) - 2.0 * s.c)) / hh2;
+            if (has_perf) v -= ctw * s.c;
+            if (has_source) v += ctf * f_old[q];
+            out[m] = v;
         } else {
             out[m] = s.c;
         }
@@ -566,7 +600,7 @@ using namespace scafes;
 extern "C" {
 
 int scafes_heat7_step_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const double* u_old,
-                          double* u_new, const double* f_old, const double* g_new,
+                          double* u_new, const double* f_old, const double* g_old, const double* g_new,
                           const scafes_heat7_params* p)
 {
     Ctx* c = as_ctx(ctx);
@@ -575,8 +609,14 @@ int scafes_heat7_step_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const d
     int rc = check_stencil_geo(g, hw);
     if (rc) return rc;
     if (p->has_source && !f_old) return fail(SCAFES_ERR_ARG, "heat7: source requested without F");
+    if (p->border_mode < SCAFES_BORDER_KEEP || p->border_mode > SCAFES_BORDER_RADIATION)
+        return fail(SCAFES_ERR_ARG, "heat7: bad border mode");
     if (p->border_mode == SCAFES_BORDER_FIELD && !g_new)
         return fail(SCAFES_ERR_ARG, "heat7: border FIELD without G");
+    if (p->border_mode >= SCAFES_BORDER_NEUMANN && (!g_new || !g_old))
+        return fail(SCAFES_ERR_ARG, "heat7: flux border needs G_old and G_new");
+    if (p->border_mode >= SCAFES_BORDER_NEUMANN && !(p->lambda != 0.0))
+        return fail(SCAFES_ERR_ARG, "heat7: flux border needs lambda != 0");
     Geo geo, gk;
     rc = make_geo(g, hw, &geo);
     if (rc) return rc;
@@ -588,8 +628,13 @@ int scafes_heat7_step_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const d
     op.a = u_old;
     op.out = u_new;
     op.f_old = f_old;
+    op.g_old = g_old;
     op.g_new = g_new;
     op.gk = gk;
+    for (int q = 0; q < 3; ++q) op.a2[q] = q < g->dim ? 2.0 * g->h[q] / p->lambda : 0.0;
+    op.alpha = p->alpha;
+    op.epsilon = p->epsilon;
+    op.sigma = p->sigma;
     op.ctau_a = p->tau * p->coeff_a;
     op.ctw = (p->tau * p->perf_c) * p->perf_w;
     op.ctf = p->tau * p->rhoc_inv;

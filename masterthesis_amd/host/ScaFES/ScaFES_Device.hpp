@@ -36,7 +36,19 @@ enum class DeviceKernel {
     YEE        // leapfrog Yee FDTD, PEC box, vacuum (updateInner = H, updateInner2 = E)
 };
 enum class DevicePrecision { F64 = 0, F32 = 1, F32_KAHAN = 2 };
-enum class DeviceBorder { KEEP = SCAFES_BORDER_KEEP, KNOWN_FIELD = SCAFES_BORDER_FIELD, VALUE = SCAFES_BORDER_VALUE };
+// KEEP: the copied old value; KNOWN_FIELD: Dirichlet from G_new; VALUE:
+// Dirichlet from a constant; NEUMANN / CAUCHY / RADIATION: the flux border
+// classes of the reference's system tests (tests/system/HeatEqnFDM/Neumann,
+// .../Cauchy, tests/system/PennesBioheatEqnFDM/Radiation): G_new on faces
+// with an index 0, the one-sided flux update with G_old on the far faces.
+enum class DeviceBorder {
+    KEEP = SCAFES_BORDER_KEEP,
+    KNOWN_FIELD = SCAFES_BORDER_FIELD,
+    VALUE = SCAFES_BORDER_VALUE,
+    NEUMANN = SCAFES_BORDER_NEUMANN,
+    CAUCHY = SCAFES_BORDER_CAUCHY,
+    RADIATION = SCAFES_BORDER_RADIATION
+};
 
 // Everything the engine needs to run a problem's update phase on the GPU.
 // Known-field indices refer to the problem's data-field list (the order of
@@ -50,8 +62,9 @@ struct DeviceUpdate {
     bool perfusion = false;
     int sourceKnown = -1;  // F, read from the old known buffer (knownDfOld)
     DeviceBorder border = DeviceBorder::KEEP;
-    int borderKnown = -1;  // G, read from the new known buffer (knownDf)
+    int borderKnown = -1;  // G, read from the new known buffer (knownDf); flux borders also read knownDfOld
     double borderValue = 0.0;
+    double lambda = 1.0, alpha = 1.0, epsilon = 1.0, sigma = 1.0;  // flux borders (HeatEqnFDM.hpp:98-110)
     // HEAT7VAR: diffusivity field
     int kappaKnown = -1;
     // PENNES: T' = T + sum_p tau*cdiff*(W+E-2T)/h_p^2 + tau*cperf*w*(T_a-T) + tau*csrc*Q

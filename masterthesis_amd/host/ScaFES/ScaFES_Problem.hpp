@@ -231,6 +231,7 @@ public:
         mCNew = DeviceBuffer();
         mKnownA = DeviceBuffer();
         mKnownB = DeviceBuffer();
+        mKnownC = DeviceBuffer();
         mYee = DeviceBuffer();
         mYee2 = DeviceBuffer();
         mRef.clear();
@@ -323,10 +324,12 @@ private:
             unknown(mDev.unknown);
             if (mDev.precision != DevicePrecision::F64) throw std::runtime_error("DeviceUpdate: HEAT7 runs in FP64");
             if (mDev.sourceKnown >= 0) known(mDev.sourceKnown, "sourceKnown");
-            if (mDev.border == DeviceBorder::KNOWN_FIELD) known(mDev.borderKnown, "borderKnown");
+            if (mDev.border == DeviceBorder::KNOWN_FIELD || fluxBorder()) known(mDev.borderKnown, "borderKnown");
+            if (fluxBorder() && mDev.lambda == 0.0) throw std::runtime_error("DeviceUpdate: flux borders need lambda != 0");
             break;
         case DeviceKernel::HEAT7VAR:
             unknown(mDev.unknown);
+            if (fluxBorder()) throw std::runtime_error("DeviceUpdate: flux borders are HEAT7 only");
             known(mDev.kappaKnown, "kappaKnown");
             if (mStencilWidth[mDev.kappaKnown] != mStencilWidth[mDev.unknown])
                 throw std::runtime_error("DeviceUpdate: kappa needs the stencil width of the updated field");
@@ -370,6 +373,11 @@ private:
     }
     std::size_t nodesOf(int ii) const { return (std::size_t)mMemAll[ii].nNodes().size(); }
     bool f32() const { return mDev.precision != DevicePrecision::F64; }
+    bool fluxBorder() const
+    {
+        return mDev.border == DeviceBorder::NEUMANN || mDev.border == DeviceBorder::CAUCHY ||
+               mDev.border == DeviceBorder::RADIATION;
+    }
     double shift() const { return (mDev.kernel == DeviceKernel::PENNES && f32()) ? mDev.tA : 0.0; }
 
     // host lump (memAll + time slot) -> device array (memAll) of the field's dtype
@@ -440,7 +448,9 @@ private:
             if (mDev.kernel == DeviceKernel::HEAT7) {
                 // F from the old known buffer (knownDfOld), G from the new one (knownDf)
                 if (mDev.sourceKnown >= 0) up(mDev.sourceKnown, mKnownA, n > 0, false);
-                if (mDev.border == DeviceBorder::KNOWN_FIELD) up(mDev.borderKnown, mKnownB, false, false);
+                if (mDev.border == DeviceBorder::KNOWN_FIELD || fluxBorder())
+                    up(mDev.borderKnown, mKnownB, false, false);
+                if (fluxBorder()) up(mDev.borderKnown, mKnownC, n > 0, false);
             } else if (mDev.kernel == DeviceKernel::HEAT7VAR) {
                 up(mDev.kappaKnown, mKnownA, false, fl);
             } else if (mDev.kernel == DeviceKernel::PENNES) {
@@ -513,8 +523,13 @@ private:
             p.has_source = mDev.sourceKnown >= 0 ? 1 : 0;
             p.border_mode = (int)mDev.border;
             p.border_value = mDev.borderValue;
+            p.lambda = mDev.lambda;
+            p.alpha = mDev.alpha;
+            p.epsilon = mDev.epsilon;
+            p.sigma = mDev.sigma;
             deviceCheck(scafes_heat7_step_f64(c, &g, hw, (const double*)mUOld.get(), (double*)mUNew.get(),
-                                              (const double*)mKnownA.get(), (const double*)mKnownB.get(), &p),
+                                              (const double*)mKnownA.get(), (const double*)mKnownC.get(),
+                                              (const double*)mKnownB.get(), &p),
                         "heat7 step");
             break;
         }
@@ -734,7 +749,7 @@ private:
     // device state
     DeviceUpdate mDev;
     std::unique_ptr<DeviceContext> mCtx;
-    DeviceBuffer mUOld, mUNew, mCOld, mCNew, mKnownA, mKnownB, mYee, mYee2;
+    DeviceBuffer mUOld, mUNew, mCOld, mCNew, mKnownA, mKnownB, mKnownC, mYee, mYee2;
     std::vector<DeviceBuffer> mRef;
 };
 

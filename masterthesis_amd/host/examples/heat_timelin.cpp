@@ -5,7 +5,10 @@
 // (equation class -> boundary class -> concrete problem).  Exact discrete
 // solution y = (1+t)(1+sum x); every "Local Linf" line must be round-off.
 //
-// Build variants: -DSCAFES_DIM=1|2|3, -DEXAMPLE_PENNES_FORM for the Pennes form.
+// Build variants: -DSCAFES_DIM=1|2|3, -DEXAMPLE_PENNES_FORM for the Pennes form,
+// -DEXAMPLE_BORDER=1|2|3 for the Neumann / Cauchy / radiation border classes
+// (tests/system/*/{Neumann,Cauchy,Radiation}/TimeLinSpaceLin; default 0 =
+// Dirichlet).
 #include <iostream>
 
 #include "ScaFES.hpp"
@@ -13,6 +16,9 @@
 
 #ifndef SCAFES_DIM
 #define SCAFES_DIM 3
+#endif
+#ifndef EXAMPLE_BORDER
+#define EXAMPLE_BORDER 0
 #endif
 
 // Equation layer: explicit Euler + 7-point stencil (HeatEqnFDM.hpp:204-218;
@@ -22,6 +28,7 @@ class Equation : public ScaFES::Problem<Equation<CT, DIM, Class>, CT, DIM> {
 public:
     typedef ScaFES::Problem<Equation<CT, DIM, Class>, CT, DIM> Base;
     const double LAMBDA = 1.0, RHO = 1.0, C = 1.0, RHO_BLOOD = 1.0, C_BLOOD = 1.0, W = 1.0;
+    const double ALPHA = 1.0, EPSILON = 1.0, SIGMA = 1.0;
     const double COEFF_A = LAMBDA / (RHO * C);
 
     Equation(ScaFES::Parameters const& params, ScaFES::GridGlobal<DIM> const& gg, bool useLeapfrog,
@@ -119,11 +126,83 @@ public:
     }
 };
 
+// Boundary layer of the flux classes (Neumann.hpp / Cauchy.hpp /
+// Radiation.hpp:126-175): G_new on faces with an index 0, otherwise the
+// outer neighbour on far faces replaced by the inner one plus the flux term
+// built from G_old.
+template <typename CT, std::size_t DIM, typename Class>
+class FluxBorder : public Equation<CT, DIM, FluxBorder<CT, DIM, Class>> {
+public:
+    typedef Equation<CT, DIM, FluxBorder<CT, DIM, Class>> Base;
+    using Base::Base;
+    void evalInner(std::vector<ScaFES::DataField<CT, DIM>>& v, ScaFES::Ntuple<int, DIM> const& x, int const& n)
+    {
+        static_cast<Class*>(this)->evalInner(v, x, n);
+    }
+    void evalBorder(std::vector<ScaFES::DataField<CT, DIM>>& v, ScaFES::Ntuple<int, DIM> const& x, int const& n)
+    {
+        static_cast<Class*>(this)->evalBorder(v, x, n);
+    }
+    template <typename TT>
+    void initInner(std::vector<ScaFES::DataField<TT, DIM>>& v, std::vector<TT> const& p,
+                   ScaFES::Ntuple<int, DIM> const& x, int const& n)
+    {
+        static_cast<Class*>(this)->template initInner<TT>(v, p, x, n);
+    }
+    template <typename TT>
+    void initBorder(std::vector<ScaFES::DataField<TT, DIM>>& v, std::vector<TT> const& p,
+                    ScaFES::Ntuple<int, DIM> const& x, int const& n)
+    {
+        static_cast<Class*>(this)->template initBorder<TT>(v, p, x, n);
+    }
+    template <typename TT>
+    void updateBorder(std::vector<ScaFES::DataField<TT, DIM>>& vNew, std::vector<ScaFES::DataField<TT, DIM>> const& vOld,
+                      ScaFES::Ntuple<int, DIM> const& x, int const&)
+    {
+        for (std::size_t p = 0; p < DIM; ++p)
+            if (x.elem(p) == 0) {
+                vNew[0](x) = this->knownDf(1, x);
+                return;
+            }
+        const TT u = vOld[0](x);
+        vNew[0](x) = u;
+        for (std::size_t p = 0; p < DIM; ++p) {
+            const TT w = vOld[0](this->connect(x, 2 * p));
+            const double a = 2.0 * this->gridsize(p) / this->LAMBDA;
+            TT t;
+            if (x.elem(p) == this->nNodes(p) - 1) {
+#if EXAMPLE_BORDER == 1
+                t = w + w - (a * this->knownDfOld(1, x)) - 2.0 * u;
+#elif EXAMPLE_BORDER == 2
+                t = w + w - (a * this->ALPHA * u) + (a * this->knownDfOld(1, x)) - 2.0 * u;
+#else
+                t = w + w - (a * this->EPSILON * this->SIGMA * u * u * u * u) + (a * this->knownDfOld(1, x)) - 2.0 * u;
+#endif
+            } else {
+                t = w + vOld[0](this->connect(x, 2 * p + 1)) - 2.0 * u;
+            }
+            vNew[0](x) += this->tau() * this->COEFF_A * t / (this->gridsize(p) * this->gridsize(p));
+        }
+#ifdef EXAMPLE_PENNES_FORM
+        vNew[0](x) -= this->tau() * ((this->RHO_BLOOD * this->C_BLOOD) / (this->RHO * this->C)) * this->W * u;
+#endif
+        vNew[0](x) += this->tau() * (1.0 / (this->RHO * this->C)) * this->knownDfOld(0, x);
+    }
+};
+
+#if EXAMPLE_BORDER == 0
+template <typename CT, std::size_t DIM, typename Class>
+using BorderLayer = DirichletBorder<CT, DIM, Class>;
+#else
+template <typename CT, std::size_t DIM, typename Class>
+using BorderLayer = FluxBorder<CT, DIM, Class>;
+#endif
+
 // Concrete problem: y = (1+t)(1+sum x).
 template <typename CT, std::size_t DIM>
-class TimeLinSpaceLin : public DirichletBorder<CT, DIM, TimeLinSpaceLin<CT, DIM>> {
+class TimeLinSpaceLin : public BorderLayer<CT, DIM, TimeLinSpaceLin<CT, DIM>> {
 public:
-    typedef DirichletBorder<CT, DIM, TimeLinSpaceLin<CT, DIM>> Base;
+    typedef BorderLayer<CT, DIM, TimeLinSpaceLin<CT, DIM>> Base;
     using Base::Base;
     void evalInner(std::vector<ScaFES::DataField<CT, DIM>>& v, ScaFES::Ntuple<int, DIM> const& idx, int const& n)
     {
@@ -147,11 +226,39 @@ public:
     {
         ScaFES::Ntuple<double, DIM> x = this->coordinates(idx);
         const double t = this->time(n);
+#if EXAMPLE_BORDER == 0
         v[0](idx) = 0.0;
         v[1](idx) = 1.0;
         for (std::size_t p = 0; p < DIM; ++p) v[1](idx) += x[p];
         v[1](idx) *= (1.0 + t);
         v[2](idx) = v[1](idx);
+#else
+        // F as on inner nodes; G = Dirichlet value on faces with an index 0,
+        // else the flux datum (.../{Neumann,Cauchy,Radiation}/TimeLinSpaceLin)
+        evalInner(v, idx, n);
+        bool near = false;
+        for (std::size_t p = 0; p < DIM; ++p) near = near || idx.elem(p) == 0;
+        if (near) {
+            v[1](idx) = v[2](idx);
+            return;
+        }
+#if EXAMPLE_BORDER == 1
+        (void)x;
+        v[1](idx) = -1.0 * this->LAMBDA * (1.0 + t);
+#elif EXAMPLE_BORDER == 2
+        v[1](idx) = 1.0;
+        for (std::size_t p = 0; p < DIM; ++p) v[1](idx) += x[p];
+        v[1](idx) *= this->ALPHA * (1.0 + t);
+        v[1](idx) += this->LAMBDA * (1.0 + t);
+#else
+        v[1](idx) = 1.0;
+        for (std::size_t p = 0; p < DIM; ++p) v[1](idx) += x[p];
+        v[1](idx) *= (1.0 + t);
+        v[1](idx) = v[1](idx) * v[1](idx) * v[1](idx) * v[1](idx);
+        v[1](idx) *= this->SIGMA * this->EPSILON;
+        v[1](idx) += this->LAMBDA * (1.0 + t);
+#endif
+#endif
     }
     template <typename TT>
     void initInner(std::vector<ScaFES::DataField<TT, DIM>>& v, std::vector<TT> const&, ScaFES::Ntuple<int, DIM> const& idx,
@@ -189,7 +296,19 @@ struct DeviceTraits<Equation<CT, DIM, Class>> {
         d.perfW = p.W;
 #endif
         d.sourceKnown = 0;
+#if EXAMPLE_BORDER == 0
         d.border = ScaFES::DeviceBorder::KNOWN_FIELD;
+#elif EXAMPLE_BORDER == 1
+        d.border = ScaFES::DeviceBorder::NEUMANN;
+#elif EXAMPLE_BORDER == 2
+        d.border = ScaFES::DeviceBorder::CAUCHY;
+#else
+        d.border = ScaFES::DeviceBorder::RADIATION;
+#endif
+        d.lambda = p.LAMBDA;
+        d.alpha = p.ALPHA;
+        d.epsilon = p.EPSILON;
+        d.sigma = p.SIGMA;
         d.borderKnown = 1;
         d.knownTimeDependent = true;
     }

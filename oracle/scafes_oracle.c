@@ -93,8 +93,44 @@ static void coords(const oracle_grid* g, int i, int j, int k, double* x)
  * Border: Dirichlet::updateBorder, vNew = knownDf(G) (Dirichlet/Dirichlet.hpp:124-130).
  * Copy-old-to-new before the hooks: setVectDfDepToDfIndep (Problem.hpp:6226-6256)
  * -> DataField::assignValues (DataField.hpp:1147-1158). */
+/* Flux border node (no index 0, some index n_p - 1): the border classes'
+ * updateBorder in the reference's order: per axis, the one-sided term where
+ * the index is n_p - 1 (outer neighbour replaced by the inner one plus the
+ * flux built from G_old), central differences elsewhere; then perfusion and
+ * source as on inner nodes. */
+static double flux_border_update(const oracle_grid* g, const double* uo, long m, const int* idx,
+                                 const long* st, const double* hh, double gold, double fold,
+                                 int border_mode, const oracle_heat7_coef* c)
+{
+    const double u = uo[m];
+    double v = u;
+    for (int p = 0; p < g->dim; ++p) {
+        const double w = uo[m - st[p]];
+        if (idx[p] == g->n[p] - 1) {
+            const double a2 = 2.0 * oracle_gridsize(g, p) / c->lambda;
+            double t = w + w;
+            if (border_mode == ORACLE_BORDER_NEUMANN) {
+                t = t - a2 * gold;
+            } else if (border_mode == ORACLE_BORDER_CAUCHY) {
+                t = t - a2 * c->alpha * u;
+                t = t + a2 * gold;
+            } else {
+                t = t - a2 * c->epsilon * c->sigma * u * u * u * u;
+                t = t + a2 * gold;
+            }
+            t = t - 2.0 * u;
+            v += c->tau * c->coeff_a * t / (hh[p]);
+        } else {
+            v += c->tau * c->coeff_a * (w + uo[m + st[p]] - 2.0 * u) / (hh[p]);
+        }
+    }
+    if (c->has_perf) v -= c->tau * c->perf_c * c->perf_w * u;
+    if (c->has_source) v += c->tau * c->rhoc_inv * fold;
+    return v;
+}
+
 void oracle_heat7_step(const oracle_grid* g, const double* uo, double* un,
-                       const double* fo, const double* gn, int border_mode,
+                       const double* fo, const double* go, const double* gn, int border_mode,
                        double border_value, const oracle_heat7_coef* c)
 {
     const long ne = oracle_nelems(g);
@@ -121,8 +157,21 @@ void oracle_heat7_step(const oracle_grid* g, const double* uo, double* un,
                     if (c->has_source && fo) v += c->tau * c->rhoc_inv * fo[npos(g, i, j, k)];
                     un[m] = v;
                 } else {
-                    if (border_mode == ORACLE_BORDER_FIELD) un[m] = gn[npos(g, i, j, k)];
-                    else if (border_mode == ORACLE_BORDER_VALUE) un[m] = border_value;
+                    if (border_mode == ORACLE_BORDER_FIELD) {
+                        un[m] = gn[npos(g, i, j, k)];
+                    } else if (border_mode == ORACLE_BORDER_VALUE) {
+                        un[m] = border_value;
+                    } else if (border_mode >= ORACLE_BORDER_NEUMANN) {
+                        const int idx[3] = {i, j, k};
+                        int dir = 0;
+                        for (int p = 0; p < g->dim; ++p)
+                            if (idx[p] == 0) dir = 1;
+                        const long q = npos(g, i, j, k);
+                        un[m] = dir ? gn[q]
+                                    : flux_border_update(g, uo, m, idx, st, hh, go[q],
+                                                         (c->has_source && fo) ? fo[q] : 0.0,
+                                                         border_mode, c);
+                    }
                 }
             }
         }
@@ -133,7 +182,7 @@ void oracle_heat7_step(const oracle_grid* g, const double* uo, double* un,
  *   heat   evalInner/evalBorder: tests/system/HeatEqnFDM/Dirichlet/TimeLinSpaceLin/TimeLinSpaceLin.hpp:75-123
  *   Pennes evalInner/evalBorder: tests/system/PennesBioheatEqnFDM/Dirichlet/TimeLinSpaceLin/TimeLinSpaceLin.hpp:75-124
  * F, G, Y are stencil-0 fields. */
-static void eval_known_timelin(const oracle_grid* g, int known, double t,
+static void eval_known_timelin(const oracle_grid* g, int known, int border, double t,
                                const oracle_heat7_coef* c, double* F, double* G, double* Y)
 {
     const int n0 = g->n[0], n1 = g->n[1], n2 = g->n[2];
@@ -161,13 +210,55 @@ static void eval_known_timelin(const oracle_grid* g, int known, double t,
                     for (int p = 0; p < g->dim; ++p) y += x[p];
                     y *= (1.0 + t);
                     Y[m] = y;
-                } else {
+                } else if (border == ORACLE_BORDER_FIELD) {
                     F[m] = 0.0;
                     double gv = 1.0;
                     for (int p = 0; p < g->dim; ++p) gv += x[p];
                     gv *= (1.0 + t);
                     G[m] = gv;
                     Y[m] = gv;
+                } else {
+                    /* flux classes: F as on inner nodes; G is the Dirichlet
+                     * value on faces with an index 0, else the flux datum
+                     * (HeatEqnFDM/Neumann/TimeLinSpaceLin/TimeLinSpaceLin.hpp:102-139,
+                     *  HeatEqnFDM/Cauchy/TimeLinSpaceLin/TimeLinSpaceLin.hpp:102-144,
+                     *  PennesBioheatEqnFDM/Radiation/TimeLinSpaceLin/TimeLinSpaceLin.hpp:108-158) */
+                    double f = 1.0;
+                    for (int p = 0; p < g->dim; ++p) f += x[p];
+                    f *= 1.0 * 1.0;
+                    if (known == ORACLE_KNOWN_PENNES_TIMELIN) {
+                        double tmp = 1.0;
+                        for (int p = 0; p < g->dim; ++p) tmp += x[p];
+                        tmp *= (1.0 + t);
+                        f += c->rhob_cb_w * tmp;
+                    }
+                    F[m] = f;
+                    const int idx[3] = {i, j, k};
+                    int dir = 0;
+                    for (int p = 0; p < g->dim; ++p)
+                        if (idx[p] == 0) dir = 1;
+                    double gv = 1.0;
+                    if (dir) {
+                        for (int p = 0; p < g->dim; ++p) gv += x[p];
+                        gv *= (1.0 + t);
+                    } else if (border == ORACLE_BORDER_NEUMANN) {
+                        gv = -1.0 * c->lambda * (1.0 + t);
+                    } else if (border == ORACLE_BORDER_CAUCHY) {
+                        for (int p = 0; p < g->dim; ++p) gv += x[p];
+                        gv *= c->alpha * (1.0 + t);
+                        gv += c->lambda * (1.0 + t);
+                    } else {
+                        for (int p = 0; p < g->dim; ++p) gv += x[p];
+                        gv *= (1.0 + t);
+                        gv = gv * gv * gv * gv;
+                        gv *= c->sigma * c->epsilon;
+                        gv += c->lambda * (1.0 + t);
+                    }
+                    G[m] = gv;
+                    double y = 1.0;
+                    for (int p = 0; p < g->dim; ++p) y += x[p];
+                    y *= (1.0 + t);
+                    Y[m] = y;
                 }
             }
         }
@@ -210,7 +301,7 @@ double oracle_linf_nodes(const oracle_grid* g, const double* a, const double* b)
  *          (:3433-3485, time() :2482-2486), update phase with F_old and G_new;
  *   each n: compErrLinf vs Y_new, swap pointers, times advance by tau. */
 double oracle_heat7_run(const oracle_grid* g, double* u, int nsteps, double t0,
-                        const oracle_heat7_coef* c, int known, double* linf)
+                        const oracle_heat7_coef* c, int known, int border, double* linf)
 {
     const long ne = oracle_nelems(g), nn = oracle_nnodes(g);
     double* uo = (double*)malloc(sizeof(double) * (size_t)ne);
@@ -225,7 +316,7 @@ double oracle_heat7_run(const oracle_grid* g, double* u, int nsteps, double t0,
     double t_unknown_new = t0;
 
     for (int n = 0; n <= nsteps; ++n) {
-        if (timelin) eval_known_timelin(g, known, t0 + n * c->tau, c, Fn, Gn, Yn);
+        if (timelin) eval_known_timelin(g, known, border, t0 + n * c->tau, c, Fn, Gn, Yn);
         if (n == 0) {
             if (timelin) {
                 for (long m = 0; m < ne; ++m) un[m] = 0.0;
@@ -248,8 +339,8 @@ double oracle_heat7_run(const oracle_grid* g, double* u, int nsteps, double t0,
             memcpy(Yo, Yn, sizeof(double) * (size_t)nn);
             t_unknown_new = t0;
         } else {
-            oracle_heat7_step(g, uo, un, timelin ? Fo : NULL, Gn,
-                              timelin ? ORACLE_BORDER_FIELD : ORACLE_BORDER_VALUE, 0.0, c);
+            oracle_heat7_step(g, uo, un, timelin ? Fo : NULL, Go, Gn,
+                              timelin ? border : ORACLE_BORDER_VALUE, 0.0, c);
             t_unknown_new = t_unknown_new + c->tau;
         }
         if (linf && timelin) linf[n] = linf_vs_nodes(g, un, Yn);

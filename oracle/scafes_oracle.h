@@ -17,8 +17,9 @@
  *
  * Pinning status of each routine (how it is checked, see tests/):
  *   oracle_heat7_*      pinned: exact discrete solutions of the reference's
- *                       system tests (TimeLinSpaceLin heat/Pennes), the
- *                       discrete sine eigenmode closed form, brute force.
+ *                       system tests (TimeLinSpaceLin heat/Pennes with
+ *                       Dirichlet, Neumann, Cauchy and radiation borders),
+ *                       the discrete sine eigenmode closed form, brute force.
  *   oracle_heat7var_*   pinned: reduction to heat7 for constant kappa,
  *                       operator symmetry, face-assembled numpy operator.
  *   oracle_pennes_*     pinned: reduction to heat7 (w=0,Q=0), Tumor-form
@@ -51,8 +52,22 @@ typedef struct {
     int nlayers;      /* nLayersAtBorder (Parameters.hpp:446) */
 } oracle_grid;
 
-/* Border rule for the unknown field at GLOBAL_BORDER nodes. */
-enum { ORACLE_BORDER_KEEP = 0, ORACLE_BORDER_FIELD = 1, ORACLE_BORDER_VALUE = 2 };
+/* Border rule for the unknown field at GLOBAL_BORDER nodes.  The flux rules
+ * follow the reference's system-test border classes: a border node with an
+ * index 0 on any axis takes the Dirichlet value G_new; otherwise, on every
+ * axis where its index is n_p - 1, the missing outer neighbour is replaced
+ * by the inner one plus a flux term built from G_old:
+ *   NEUMANN   tests/system/HeatEqnFDM/Neumann/Neumann.hpp:126-168
+ *   CAUCHY    tests/system/HeatEqnFDM/Cauchy/Cauchy.hpp:126-170
+ *   RADIATION tests/system/PennesBioheatEqnFDM/Radiation/Radiation.hpp:126-175 */
+enum {
+    ORACLE_BORDER_KEEP = 0,
+    ORACLE_BORDER_FIELD = 1,
+    ORACLE_BORDER_VALUE = 2,
+    ORACLE_BORDER_NEUMANN = 3,
+    ORACLE_BORDER_CAUCHY = 4,
+    ORACLE_BORDER_RADIATION = 5
+};
 
 /* Known-field evaluator used inside the restated time loop. */
 enum {
@@ -72,24 +87,33 @@ typedef struct {
     double rhob_cb_w;   /* RHO_BLOOD*C_BLOOD*W inside the Pennes source F */
     int has_perf;
     int has_source;
+    double lambda;      /* LAMBDA of the flux borders (HeatEqnFDM.hpp:98) */
+    double alpha;       /* ALPHA (Cauchy) */
+    double epsilon;     /* EPSILON (radiation) */
+    double sigma;       /* SIGMA (radiation) */
 } oracle_heat7_coef;
 
 long oracle_nelems(const oracle_grid* g);
 long oracle_nnodes(const oracle_grid* g);
 double oracle_gridsize(const oracle_grid* g, int p);
 
-/* One update step (copy-old-to-new, inner stencil, border rule). */
+/* One update step (copy-old-to-new, inner stencil, border rule).  f_old,
+ * g_old, g_new: stencil-0 known fields (nullable where the rule does not
+ * read them). */
 void oracle_heat7_step(const oracle_grid* g, const double* u_old, double* u_new,
-                       const double* f_old, const double* g_new, int border_mode,
-                       double border_value, const oracle_heat7_coef* c);
+                       const double* f_old, const double* g_old, const double* g_new,
+                       int border_mode, double border_value, const oracle_heat7_coef* c);
 
 /* Full restated iterateOverTime for the heat/Pennes system problems.
  * u: memAll of the unknown field; on entry ignored if known==TIMELIN (the
  * init hook is restated), else the initial state; on exit the final state.
- * linf (nullable, nsteps+1 entries): "Local Linf" of compErrLinf per step
- * against the exact solution (TIMELIN modes only).  Returns final time. */
+ * border: ORACLE_BORDER_FIELD (Dirichlet system tests) or one of the flux
+ * rules; the TIMELIN known fields G then follow that border class's
+ * TimeLinSpaceLin test.  linf (nullable, nsteps+1 entries): "Local Linf"
+ * of compErrLinf per step against the exact solution (TIMELIN modes only).
+ * Returns final time. */
 double oracle_heat7_run(const oracle_grid* g, double* u, int nsteps, double t0,
-                        const oracle_heat7_coef* c, int known, double* linf);
+                        const oracle_heat7_coef* c, int known, int border, double* linf);
 
 /* Variable-coefficient conservative 7-point heat: one step, FP64.
  * kappa has the same layout (and stencil width) as u. */

@@ -13,6 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "oracle", "build", "liboracle.so")
 
 BORDER_KEEP, BORDER_FIELD, BORDER_VALUE = 0, 1, 2
+BORDER_NEUMANN, BORDER_CAUCHY, BORDER_RADIATION = 3, 4, 5
 KNOWN_NONE, KNOWN_HEAT_TIMELIN, KNOWN_PENNES_TIMELIN = 0, 1, 2
 
 
@@ -24,7 +25,8 @@ class Grid(C.Structure):
 class Heat7Coef(C.Structure):
     _fields_ = [("tau", C.c_double), ("coeff_a", C.c_double), ("rhoc_inv", C.c_double),
                 ("perf_c", C.c_double), ("perf_w", C.c_double), ("rhob_cb_w", C.c_double),
-                ("has_perf", C.c_int), ("has_source", C.c_int)]
+                ("has_perf", C.c_int), ("has_source", C.c_int), ("lam", C.c_double),
+                ("alpha", C.c_double), ("epsilon", C.c_double), ("sigma", C.c_double)]
 
 
 class PennesCoef(C.Structure):
@@ -79,9 +81,9 @@ def lib():
         L.oracle_nelems.argtypes = [gp]
         L.oracle_gridsize.restype = C.c_double
         L.oracle_gridsize.argtypes = [gp, C.c_int]
-        L.oracle_heat7_step.argtypes = [gp, dp, dp, dp, dp, C.c_int, C.c_double, C.POINTER(Heat7Coef)]
+        L.oracle_heat7_step.argtypes = [gp, dp, dp, dp, dp, dp, C.c_int, C.c_double, C.POINTER(Heat7Coef)]
         L.oracle_heat7_run.restype = C.c_double
-        L.oracle_heat7_run.argtypes = [gp, dp, C.c_int, C.c_double, C.POINTER(Heat7Coef), C.c_int, dp]
+        L.oracle_heat7_run.argtypes = [gp, dp, C.c_int, C.c_double, C.POINTER(Heat7Coef), C.c_int, C.c_int, dp]
         L.oracle_heat7var_step.argtypes = [gp, dp, dp, dp, C.c_double, C.c_int, C.c_double]
         L.oracle_heat7var_run.argtypes = [gp, dp, dp, C.c_double, C.c_int, C.c_int, C.c_double, dp]
         L.oracle_pennes_step.argtypes = [gp, dp, dp, dp, dp, C.POINTER(PennesCoef)]
@@ -117,24 +119,25 @@ def ptr(a):
 
 
 def heat7_coef(tau, coeff_a=1.0, rhoc_inv=1.0, perf_c=0.0, perf_w=0.0, rhob_cb_w=0.0,
-               has_perf=False, has_source=False):
+               has_perf=False, has_source=False, lam=1.0, alpha=1.0, epsilon=1.0, sigma=1.0):
     c = Heat7Coef()
     c.tau, c.coeff_a, c.rhoc_inv = tau, coeff_a, rhoc_inv
     c.perf_c, c.perf_w, c.rhob_cb_w = perf_c, perf_w, rhob_cb_w
     c.has_perf, c.has_source = int(has_perf), int(has_source)
+    c.lam, c.alpha, c.epsilon, c.sigma = lam, alpha, epsilon, sigma
     return c
 
 
-def heat7_run(g, u, nsteps, t0, coef, known):
+def heat7_run(g, u, nsteps, t0, coef, known, border=BORDER_FIELD):
     u = np.ascontiguousarray(u, dtype=np.float64).copy()
     linf = np.zeros(nsteps + 1)
-    t = lib().oracle_heat7_run(C.byref(g), ptr(u), nsteps, t0, C.byref(coef), known, ptr(linf))
+    t = lib().oracle_heat7_run(C.byref(g), ptr(u), nsteps, t0, C.byref(coef), known, border, ptr(linf))
     return u, linf, t
 
 
-def heat7_step(g, uo, f_old=None, g_new=None, border=BORDER_VALUE, border_value=0.0, coef=None):
+def heat7_step(g, uo, f_old=None, g_new=None, border=BORDER_VALUE, border_value=0.0, coef=None, g_old=None):
     un = np.empty_like(uo)
-    lib().oracle_heat7_step(C.byref(g), ptr(uo), ptr(un), ptr(f_old), ptr(g_new), border,
+    lib().oracle_heat7_step(C.byref(g), ptr(uo), ptr(un), ptr(f_old), ptr(g_old), ptr(g_new), border,
                             border_value, C.byref(coef))
     return un
 

@@ -26,9 +26,11 @@ def _xs(n):
     return [np.array([(1.0 / (m - 1)) * i for i in range(m)]) for m in n]
 
 
-def timelin_known(n, t, pennes=False, rhob_cb_w=1.0):
+def timelin_known(n, t, pennes=False, rhob_cb_w=1.0, border=None):
     """F, G, Y of the TimeLinSpaceLin problems (stencil-0 layout, (z,y,x)),
-    evaluated with the hooks' operation order."""
+    evaluated with the hooks' operation order; border: None (Dirichlet) or
+    one of the flux classes (lambda = alpha = epsilon = sigma = 1)."""
+    bc = border
     dim = len(n)
     xs = _xs(n)
     shape = tuple(reversed(n))
@@ -40,42 +42,73 @@ def timelin_known(n, t, pennes=False, rhob_cb_w=1.0):
     idx = np.meshgrid(*[np.arange(m) for m in reversed(n)], indexing="ij")[::-1]
     for p in range(dim):
         border |= (idx[p] < 1) | (idx[p] > n[p] - 2)
-    F = np.where(border, 0.0, s * (1.0 * 1.0))
-    if pennes:
-        F = np.where(border, 0.0, F + rhob_cb_w * (s * (1.0 + t)))
-    G = np.where(border, s * (1.0 + t), 0.0)
     Y = s * (1.0 + t)
+    if bc is None:
+        F = np.where(border, 0.0, s * (1.0 * 1.0))
+        if pennes:
+            F = np.where(border, 0.0, F + rhob_cb_w * (s * (1.0 + t)))
+        G = np.where(border, s * (1.0 + t), 0.0)
+        return F, G, Y
+    F = s * (1.0 * 1.0)
+    if pennes:
+        F = F + rhob_cb_w * (s * (1.0 + t))
+    near = np.zeros(shape, bool)
+    for p in range(dim):
+        near |= idx[p] == 0
+    if bc == capi.BORDER_NEUMANN:
+        flux = np.full(shape, -1.0 * 1.0 * (1.0 + t))
+    elif bc == capi.BORDER_CAUCHY:
+        flux = s * (1.0 * (1.0 + t)) + 1.0 * (1.0 + t)
+    else:
+        q = s * (1.0 + t)
+        flux = (q * q * q * q) * (1.0 * 1.0) + 1.0 * (1.0 + t)
+    G = np.where(near, Y, np.where(border, flux, 0.0))
     return F, G, Y
 
 
-@pytest.mark.parametrize("n,pennes", [((16, 16, 16), False), ((12, 10, 9), False), ((20, 17), False),
-                                      ((33,), False), ((14, 13, 12), True)])
-def test_timelin_bitexact(ctx, n, pennes):
+@pytest.mark.parametrize("n,pennes,bc", [((16, 16, 16), False, None), ((12, 10, 9), False, None),
+                                         ((20, 17), False, None), ((33,), False, None),
+                                         ((14, 13, 12), True, None),
+                                         ((14, 13, 12), False, capi.BORDER_NEUMANN),
+                                         ((11, 15), False, capi.BORDER_CAUCHY),
+                                         ((13, 12, 14), True, capi.BORDER_NEUMANN),
+                                         ((12, 14, 13), True, capi.BORDER_CAUCHY),
+                                         ((12, 11, 13), True, capi.BORDER_RADIATION),
+                                         ((27,), True, capi.BORDER_RADIATION)])
+def test_timelin_bitexact(ctx, n, pennes, bc):
+    """TimeLinSpaceLin heat/Pennes with Dirichlet or flux borders: FP64 device
+    steps equal the oracle bit for bit and stay on the exact solution."""
     dim = len(n)
     g = capi.geom(n)
     og = O.make_grid(n)
     h2 = sum(1.0 / O.lib().oracle_gridsize(og, p) ** 2 for p in range(dim))
     tau = 0.8 / (2.0 * h2 + (1.0 if pennes else 0.0))
+    if bc is not None:
+        hmin = min(O.lib().oracle_gridsize(og, p) for p in range(dim))
+        tau = 0.4 / (2.0 * h2 + 1.0 + (8.0 * (1.0 + dim) ** 3 / hmin if bc == capi.BORDER_RADIATION else 0.0))
     nsteps = 25
     coef = O.heat7_coef(tau, perf_c=1.0, perf_w=1.0, rhob_cb_w=1.0, has_perf=pennes, has_source=True)
     ref, linf, _ = O.heat7_run(og, np.zeros(O.memall_shape(og)), nsteps, 0.0, coef,
-                               O.KNOWN_PENNES_TIMELIN if pennes else O.KNOWN_HEAT_TIMELIN)
+                               O.KNOWN_PENNES_TIMELIN if pennes else O.KNOWN_HEAT_TIMELIN,
+                               O.BORDER_FIELD if bc is None else bc)
     # initial state: (1+sum x)(1+t0) on nodes, ghosts 0
     _, _, Y0 = timelin_known(n, 0.0)
     u0 = np.zeros(O.memall_shape(og))
     O.nodes_view(og, u0)[...] = Y0.reshape(O.nodes_view(og, u0).shape)
     ne = u0.size
     a, b = ctx.upload(u0), ctx.upload(u0)
-    Fd, Gd = ctx.alloc(Y0.size), ctx.alloc(Y0.size)
+    Fd, God, Gd = ctx.alloc(Y0.size), ctx.alloc(Y0.size), ctx.alloc(Y0.size)
     p = capi.Heat7Params(tau=tau, coeff_a=1.0, rhoc_inv=1.0, perf_c=1.0, perf_w=1.0,
-                         has_perf=int(pennes), has_source=1, border_mode=capi.BORDER_FIELD,
-                         border_value=0.0)
+                         has_perf=int(pennes), has_source=1,
+                         border_mode=capi.BORDER_FIELD if bc is None else bc,
+                         border_value=0.0, lam=1.0, alpha=1.0, epsilon=1.0, sigma=1.0)
     for step in range(1, nsteps + 1):
-        F_old, _, _ = timelin_known(n, (step - 1) * tau, pennes)
-        _, G_new, _ = timelin_known(n, step * tau, pennes)
+        F_old, G_old, _ = timelin_known(n, (step - 1) * tau, pennes, border=bc)
+        _, G_new, _ = timelin_known(n, step * tau, pennes, border=bc)
         Fd.write(F_old)
+        God.write(G_old)
         Gd.write(G_new)
-        capi.check(ctx.L.scafes_heat7_step_f64(ctx.h, capi.C.byref(g), 1, a.ptr, b.ptr, Fd.ptr, Gd.ptr,
+        capi.check(ctx.L.scafes_heat7_step_f64(ctx.h, capi.C.byref(g), 1, a.ptr, b.ptr, Fd.ptr, God.ptr, Gd.ptr,
                                                capi.C.byref(p)))
         a, b = b, a
     got = a.read().reshape(u0.shape)
@@ -112,7 +145,7 @@ def test_sine_mode_64cube_bitexact_and_analytic(ctx):
     p = capi.Heat7Params(tau=tau, coeff_a=1.0, rhoc_inv=1.0, has_perf=0, has_source=0,
                          border_mode=capi.BORDER_VALUE, border_value=0.0)
     for _ in range(nsteps):
-        capi.check(ctx.L.scafes_heat7_step_f64(ctx.h, capi.C.byref(g), 1, a.ptr, b.ptr, None, None,
+        capi.check(ctx.L.scafes_heat7_step_f64(ctx.h, capi.C.byref(g), 1, a.ptr, b.ptr, None, None, None,
                                                capi.C.byref(p)))
         a, b = b, a
     got = a.read().reshape(u0.shape)

@@ -49,6 +49,37 @@ def test_timelin_through_problem(tmp_path, hip, prog, n, pennes):
     np.testing.assert_allclose(linf, rlinf, rtol=0, atol=1e-15)
 
 
+@pytest.mark.parametrize("prog,n,pennes,bc", [("heat_neumann_2d", (13, 11), False, O.BORDER_NEUMANN),
+                                              ("heat_neumann_3d", (10, 9, 11), False, O.BORDER_NEUMANN),
+                                              ("heat_cauchy_3d", (9, 11, 10), False, O.BORDER_CAUCHY),
+                                              ("pennes_neumann_3d", (11, 10, 9), True, O.BORDER_NEUMANN),
+                                              ("pennes_cauchy_3d", (10, 10, 12), True, O.BORDER_CAUCHY),
+                                              ("pennes_radiation_3d", (9, 10, 8), True, O.BORDER_RADIATION),
+                                              ("pennes_radiation_1d", (21,), True, O.BORDER_RADIATION)])
+def test_flux_border_timelin_through_problem(tmp_path, hip, prog, n, pennes, bc):
+    """The reference's Neumann / Cauchy / radiation TimeLinSpaceLin system
+    tests as problem classes: every "Local Linf" is round-off and the final
+    field equals the oracle's bit for bit."""
+    nsteps = 30
+    dim = len(n)
+    g = O.make_grid(n)
+    h2 = sum(1.0 / O.lib().oracle_gridsize(g, p) ** 2 for p in range(dim))
+    hmin = min(O.lib().oracle_gridsize(g, p) for p in range(dim))
+    tau = 0.4 / (2.0 * h2 + 1.0 + (8.0 * (1.0 + dim) ** 3 / hmin if bc == O.BORDER_RADIATION else 0.0))
+    dump = str(tmp_path / "u.bin")
+    args = [f"--dim={dim}", "--nNodes=" + "x".join(map(str, n)), f"--nTimesteps={nsteps}", f"--tau={tau!r}"]
+    out = run(prog, args, dump)
+    linf = [float(x) for x in re.findall(r"Local Linf error\(U, USolDom\): (\S+)", out)]
+    assert len(linf) == nsteps + 1
+    assert max(linf) < 1e-12
+    coef = O.heat7_coef(tau, perf_c=1.0, perf_w=1.0, rhob_cb_w=1.0, has_perf=pennes, has_source=True)
+    ref, rlinf, _ = O.heat7_run(g, np.zeros(O.memall_shape(g)), nsteps, 0.0, coef,
+                                O.KNOWN_PENNES_TIMELIN if pennes else O.KNOWN_HEAT_TIMELIN, bc)
+    got = np.fromfile(dump, dtype=np.float64)
+    np.testing.assert_array_equal(got, O.nodes_view(g, ref).ravel())
+    np.testing.assert_allclose(linf, rlinf, rtol=0, atol=1e-15)
+
+
 def _pattern(n, a, b, c, m, d):
     """((a i + b j + c k) mod m) / d on an (nz, ny, nx) node block."""
     k, j, i = np.meshgrid(*(np.arange(x) for x in reversed(n)), indexing="ij")

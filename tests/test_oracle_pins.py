@@ -4,8 +4,9 @@ mathematics fix.  Each test is chosen so a plausible slip in the oracle
 instead of G_new, a wrong face average, a wrong curl sign) fails it.
 
  * TimeLinSpaceLin heat / Pennes (tests/system/{HeatEqnFDM,PennesBioheatEqnFDM}/
-   Dirichlet/TimeLinSpaceLin): y = (1+t)(1+sum x) is reproduced exactly by
-   explicit Euler + 7-point stencil, so every "Local Linf" must be round-off.
+   {Dirichlet,Neumann,Cauchy,Radiation}/TimeLinSpaceLin): y = (1+t)(1+sum x)
+   is reproduced exactly by explicit Euler + 7-point stencil and the
+   one-sided flux borders, so every "Local Linf" must be round-off.
  * Discrete sine eigenmode: u^n = (1 - tau*lambda_h)^n prod sin(pi x).
  * Var-kappa: symmetry of the discrete operator, reduction to constant
    kappa, face-by-face assembled operator.
@@ -77,6 +78,89 @@ def test_pennes_timelin_needs_f_old():
     bad = O.heat7_coef(tau, perf_c=1.0, perf_w=1.1, rhob_cb_w=1.0, has_perf=True, has_source=True)
     _, linf_bad, _ = O.heat7_run(g, np.zeros(O.memall_shape(g)), 30, 0.0, bad, O.KNOWN_PENNES_TIMELIN)
     assert linf_bad.max() > 1e-4
+
+
+# ---------------------------------------------------------------------------
+# Flux border classes of the reference's system tests (Neumann, Cauchy,
+# radiation; tests/system/*/{Neumann,Cauchy,Radiation}/TimeLinSpaceLin): the
+# one-sided border update with the flux datum G_old keeps y = (1+t)(1+sum x)
+# exact, so every "Local Linf" must be round-off.
+
+FLUX_CASES = [("heat", O.BORDER_NEUMANN), ("heat", O.BORDER_CAUCHY), ("pennes", O.BORDER_NEUMANN),
+              ("pennes", O.BORDER_CAUCHY), ("pennes", O.BORDER_RADIATION)]
+
+
+def flux_coef(form, tau, **kw):
+    if form == "pennes":
+        return O.heat7_coef(tau, perf_c=1.0, perf_w=1.0, rhob_cb_w=1.0, has_perf=True, has_source=True, **kw)
+    return O.heat7_coef(tau, has_source=True, **kw)
+
+
+@pytest.mark.parametrize("form,border", FLUX_CASES)
+@pytest.mark.parametrize("dims", [(11,), (9, 8), (8, 9, 7)])
+def test_flux_border_timelin_exact(dims, form, border):
+    g = O.make_grid(dims)
+    h2 = sum(1.0 / O.lib().oracle_gridsize(g, p) ** 2 for p in range(len(dims)))
+    nsteps = 40
+    tau = 0.4 / (2.0 * h2 + 1.0)
+    if border == O.BORDER_RADIATION:
+        # the linearised radiation term adds 8 eps sigma u^3 / h to the
+        # stiffness (u <= 1 + dim here)
+        hmin = min(O.lib().oracle_gridsize(g, p) for p in range(len(dims)))
+        tau = 0.4 / (2.0 * h2 + 1.0 + 8.0 * (1.0 + len(dims)) ** 3 / hmin)
+    known = O.KNOWN_PENNES_TIMELIN if form == "pennes" else O.KNOWN_HEAT_TIMELIN
+    u, linf, _ = O.heat7_run(g, np.zeros(O.memall_shape(g)), nsteps, 0.0, flux_coef(form, tau), known, border)
+    assert linf.max() < 1e-12
+    # the far faces really are updated by the flux rule (not kept)
+    x = [np.linspace(0, 1, n) for n in dims]
+    y = (1.0 + sum(np.meshgrid(*x, indexing="ij"))) * (1.0 + nsteps * tau)
+    got = O.nodes_view(g, u).reshape(tuple(reversed(dims))).T
+    np.testing.assert_allclose(got[(-1,) * len(dims)], y[(-1,) * len(dims)], rtol=1e-12)
+
+
+def _timelin_known(g, dims, t, form, border, lam=1.0, alpha=1.0, eps_sigma=1.0):
+    """F, G (stencil-0, (z,y,x)) of the flux-border TimeLinSpaceLin tests and
+    the exact state y, computed independently of the oracle."""
+    x = [np.linspace(0, 1, n) for n in dims]
+    mesh = np.meshgrid(*reversed(x), indexing="ij")  # (z, y, x) order, axis 0 = last dim
+    s = 1.0 + sum(mesh)
+    y = s * (1.0 + t)
+    F = s.copy()
+    if form == "pennes":
+        F = F + y
+    idx = np.meshgrid(*(np.arange(n) for n in reversed(dims)), indexing="ij")
+    near = np.zeros(s.shape, bool)
+    for a in idx:
+        near |= (a == 0)
+    if border == O.BORDER_NEUMANN:
+        flux = np.full(s.shape, -lam * (1.0 + t))
+    elif border == O.BORDER_CAUCHY:
+        flux = alpha * y + lam * (1.0 + t)
+    else:
+        flux = eps_sigma * y ** 4 + lam * (1.0 + t)
+    G = np.where(near, y, flux)
+    return F, G, y
+
+
+@pytest.mark.parametrize("form,border", FLUX_CASES)
+def test_flux_border_one_step_sensitivity(form, border):
+    """One oracle step from the exact state with independently evaluated
+    known fields lands on y(t+tau); a flux datum with the wrong sign, or G_new
+    used where the rule reads G_old, does not."""
+    dims = (7, 8, 6)
+    g = O.make_grid(dims)
+    tau, t = 2e-3, 0.3
+    F, Go, y0 = _timelin_known(g, dims, t, form, border)
+    _, Gn, y1 = _timelin_known(g, dims, t + tau, form, border)
+    u = np.zeros(O.memall_shape(g))
+    O.nodes_view(g, u)[...] = y0.reshape(O.nodes_view(g, u).shape)
+    c = flux_coef(form, tau)
+    un = O.heat7_step(g, u, f_old=F.ravel(), g_new=Gn.ravel(), g_old=Go.ravel(), border=border, coef=c)
+    err = np.abs(O.nodes_view(g, un).reshape(y1.shape) - y1).max()
+    assert err < 1e-12
+    for bad in (-Go, Gn):
+        un = O.heat7_step(g, u, f_old=F.ravel(), g_new=Gn.ravel(), g_old=bad.ravel(), border=border, coef=c)
+        assert np.abs(O.nodes_view(g, un).reshape(y1.shape) - y1).max() > 1e-6
 
 
 # ---------------------------------------------------------------------------
