@@ -110,6 +110,19 @@ int heatvar_tb2_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, c
                        const void* cin, void* co, const void* kap, double tau, int border_mode, double border_value,
                        int cfg);
 
+// Yee box of N cells per axis; fields hold N+1 nodes per axis, x fastest.
+struct YeeDims {
+    int N;       // cells per axis
+    long M, M2;  // N+1, (N+1)^2
+    long nf;     // (N+1)^3
+};
+// Two leapfrog steps per sweep, out of place (defined in yee_tb2.hip); ep0 /
+// ep1 (nullable): per-workgroup FP64 energy partials of the two steps.
+bool yee_tb2_supported(const YeeDims& d, int elem_bytes);
+long yee_tb2_blocks(const YeeDims& d);
+template <typename T>
+int yee_tb2_launch(Ctx* c, const YeeDims& d, T s, const T* a, T* b, double* ep0, double* ep1);
+
 // Reductions (defined in scafes_capi.hip).
 enum RedOp { RED_MAX_ABS_DIFF = 0, RED_SUM_SQ_DIFF = 1, RED_SUM_PROD = 2 };
 int reduce_nodes(Ctx* c, const Geo& ga, const Geo& gb, int dtype, const void* a, const void* b,

@@ -28,12 +28,6 @@
 
 namespace scafes {
 
-struct YeeDims {
-    int N;       // cells per axis; fields hold N+1 nodes per axis
-    long M, M2;  // N+1, (N+1)^2
-    long nf;     // (N+1)^3
-};
-
 __device__ inline long yat(const YeeDims& d, int i, int j, int k) { return i + d.M * j + d.M2 * k; }
 
 template <typename T, int TX, int TY>
@@ -368,42 +362,56 @@ static int yee_run_t(Ctx* c, int N, double s, T* f, T* f2, int nsteps, int energ
     d.nf = d.M2 * d.M;
     const bool fused = f2 != nullptr;
     if (fused && d.M2 * (long)sizeof(T) >= (1L << 32)) return fail(SCAFES_ERR_ARG, "yee: plane too large");
+    const bool tb2 = fused && yee_tb2_supported(d, (int)sizeof(T));
     const int TX = fused ? kFTX : kYTX, TY = fused ? kFTY : kYTY, ZC = fused ? kFZC : kYZC;
     const int OX = fused ? TX - 1 : TX, OY = fused ? TY - 1 : TY;
     const int tiles_x = ceil_div(N + 1, OX), tiles_y = ceil_div(N + 1, OY), tiles_z = ceil_div(N + 1, ZC);
     const long nb = (long)tiles_x * tiles_y * tiles_z;
     if (nb > 0x7fffffffL) return fail(SCAFES_ERR_ARG, "yee: grid too large");
+    const long nbp = std::max(nb, tb2 ? yee_tb2_blocks(d) : 0L);  // partial slots per step
     const int nrec = (energy_every > 0) ? (nsteps + energy_every - 1) / energy_every : 0;
     double* epart = nullptr;
     double* edev = nullptr;
     if (nrec > 0) {
-        SCAFES_HIP(hipMallocAsync((void**)&epart, sizeof(double) * nb, c->stream));
+        SCAFES_HIP(hipMallocAsync((void**)&epart, sizeof(double) * 2 * nbp, c->stream));
         SCAFES_HIP(hipMallocAsync((void**)&edev, sizeof(double) * nrec, c->stream));
     }
+    auto rec = [&](int st) { return nrec > 0 && (st % energy_every) == 0; };
     const dim3 blk(TX, TY);
     int rc = SCAFES_OK;
     T* a = f;
     T* b = f2;
-    for (int st = 0; st < nsteps; ++st) {
-        const bool rec = nrec > 0 && (st % energy_every) == 0;
+    int st = 0;
+    if (tb2) {
+        // two steps per sweep; an odd count ends with one fused step
+        for (; st + 2 <= nsteps; st += 2) {
+            const bool r0 = rec(st), r1 = rec(st + 1);
+            rc = yee_tb2_launch<T>(c, d, (T)s, a, b, r0 ? epart : nullptr, r1 ? epart + nbp : nullptr);
+            if (rc) break;
+            const long nbt = yee_tb2_blocks(d);
+            if (r0) energy_final<<<1, 256, 0, c->stream>>>(epart, (int)nbt, edev + st / energy_every);
+            if (r1) energy_final<<<1, 256, 0, c->stream>>>(epart + nbp, (int)nbt, edev + (st + 1) / energy_every);
+            std::swap(a, b);
+        }
+    }
+    for (; rc == SCAFES_OK && st < nsteps; ++st) {
+        const bool r = rec(st);
         if (fused) {
             yee_fused_kernel<T, kFTX, kFTY, kFD><<<dim3((unsigned)nb), blk, 0, c->stream>>>(
-                d, (T)s, a, b, ZC, tiles_x, tiles_y, rec ? epart : nullptr);
+                d, (T)s, a, b, ZC, tiles_x, tiles_y, r ? epart : nullptr);
             std::swap(a, b);
         } else {
             yee_h_kernel<T, kYTX, kYTY><<<dim3((unsigned)nb), blk, 0, c->stream>>>(d, (T)s, f, ZC, tiles_x,
-                                                                                  tiles_y, rec ? epart : nullptr);
+                                                                                  tiles_y, r ? epart : nullptr);
         }
-        if (rec) energy_final<<<1, 256, 0, c->stream>>>(epart, (int)nb, edev + st / energy_every);
+        if (r) energy_final<<<1, 256, 0, c->stream>>>(epart, (int)nb, edev + st / energy_every);
         if (!fused)
             yee_e_kernel<T, kYTX, kYTY><<<dim3((unsigned)nb), blk, 0, c->stream>>>(d, (T)s, f, ZC, tiles_x, tiles_y);
         hipError_t e = hipGetLastError();
-        if (e != hipSuccess) {
-            rc = check_hip(e, "yee kernels");
-            break;
-        }
+        if (e != hipSuccess) rc = check_hip(e, "yee kernels");
     }
-    if (result_in_second) *result_in_second = (fused && (nsteps % 2 == 1)) ? 1 : 0;
+    // every out-of-place launch ends with a swap: the final state is in a
+    if (result_in_second) *result_in_second = (fused && a == f2) ? 1 : 0;
     if (nrec > 0) {
         if (rc == SCAFES_OK) {
             hipError_t e = hipMemcpyAsync(energy, edev, sizeof(double) * nrec, hipMemcpyDeviceToHost, c->stream);

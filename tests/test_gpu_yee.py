@@ -49,7 +49,7 @@ def run_device(ctx, N, s, f0, nsteps, dtype, energy_every=0, fused=True):
     capi.check(ctx.L.scafes_yee_run(ctx.h, N, code, s, d.ptr, d2.ptr if d2 else None, nsteps, energy_every,
                                     e.ctypes.data_as(capi.C.POINTER(capi.C.c_double)) if nrec else None,
                                     capi.C.byref(second)))
-    assert second.value == (1 if fused and nsteps % 2 else 0)
+    assert second.value in ((0, 1) if fused else (0,))
     out = (d2 if second.value else d).read().reshape(f0.shape)
     d.free()
     if d2 is not None:
