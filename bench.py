@@ -168,7 +168,7 @@ def bench_yee(ctx, capi, ncell=384, nsteps=2000, energy_every=100):
     cells = float(ncell) ** 3
     per = ms / nsteps
     return {"workload": f"yee {ncell}^3 cells PEC vacuum, Courant 0.5, {nsteps} leapfrog steps, FP32",
-            "kernel": "yee_tb2_kernel<float, 64, 12, 2> (two leapfrog steps per sweep)",
+            "kernel": "yee_tb2_kernel<float, 64, 16, 4> (two leapfrog steps per sweep)",
             "ms_per_step": round(per, 5), "gcell_steps_per_s": round(cells / (per * 1e-3) / 1e9, 3),
             "algorithmic_bytes_per_cell_step": 48, "moved_bytes_per_cell_step": 24,
             "hbm_frac_algorithmic": round(48 * cells / (per * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

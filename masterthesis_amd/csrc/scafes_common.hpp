@@ -118,10 +118,14 @@ struct YeeDims {
 };
 // Two leapfrog steps per sweep, out of place (defined in yee_tb2.hip); ep0 /
 // ep1 (nullable): per-workgroup FP64 energy partials of the two steps.
+struct Tb2Plan {
+    int tiles_x, tiles_y, zc;  // xy tiles, z planes per workgroup
+    long nb;                   // workgroups
+};
 bool yee_tb2_supported(const YeeDims& d, int elem_bytes);
-long yee_tb2_blocks(const YeeDims& d);
+Tb2Plan yee_tb2_plan(const YeeDims& d, int elem_bytes, int cu_count);
 template <typename T>
-int yee_tb2_launch(Ctx* c, const YeeDims& d, T s, const T* a, T* b, double* ep0, double* ep1);
+int yee_tb2_launch(Ctx* c, const YeeDims& d, const Tb2Plan& pl, T s, const T* a, T* b, double* ep0, double* ep1);
 
 // Reductions (defined in scafes_capi.hip).
 enum RedOp { RED_MAX_ABS_DIFF = 0, RED_SUM_SQ_DIFF = 1, RED_SUM_PROD = 2 };

@@ -368,7 +368,8 @@ static int yee_run_t(Ctx* c, int N, double s, T* f, T* f2, int nsteps, int energ
     const int tiles_x = ceil_div(N + 1, OX), tiles_y = ceil_div(N + 1, OY), tiles_z = ceil_div(N + 1, ZC);
     const long nb = (long)tiles_x * tiles_y * tiles_z;
     if (nb > 0x7fffffffL) return fail(SCAFES_ERR_ARG, "yee: grid too large");
-    const long nbp = std::max(nb, tb2 ? yee_tb2_blocks(d) : 0L);  // partial slots per step
+    const Tb2Plan plan = yee_tb2_plan(d, (int)sizeof(T), c->cu_count);
+    const long nbp = std::max(nb, tb2 ? plan.nb : 0L);  // partial slots per step
     const int nrec = (energy_every > 0) ? (nsteps + energy_every - 1) / energy_every : 0;
     double* epart = nullptr;
     double* edev = nullptr;
@@ -386,9 +387,9 @@ static int yee_run_t(Ctx* c, int N, double s, T* f, T* f2, int nsteps, int energ
         // two steps per sweep; an odd count ends with one fused step
         for (; st + 2 <= nsteps; st += 2) {
             const bool r0 = rec(st), r1 = rec(st + 1);
-            rc = yee_tb2_launch<T>(c, d, (T)s, a, b, r0 ? epart : nullptr, r1 ? epart + nbp : nullptr);
+            rc = yee_tb2_launch<T>(c, d, plan, (T)s, a, b, r0 ? epart : nullptr, r1 ? epart + nbp : nullptr);
             if (rc) break;
-            const long nbt = yee_tb2_blocks(d);
+            const long nbt = plan.nb;
             if (r0) energy_final<<<1, 256, 0, c->stream>>>(epart, (int)nbt, edev + st / energy_every);
             if (r1) energy_final<<<1, 256, 0, c->stream>>>(epart + nbp, (int)nbt, edev + (st + 1) / energy_every);
             std::swap(a, b);

@@ -263,31 +263,59 @@ __global__ __launch_bounds__(TX* TY) void yee_tb2_kernel(YeeDims d, T s, const T
     }
 }
 
-constexpr int kTTX = 64, kTTY = 12, kTZC = 64, kTD = 2;
+// Tile per precision: FP32 64 x 16 with 4 planes in flight; FP64 64 x 12
+// with 2 (its LDS planes take twice the space).  Measured on an MI355X at
+// 384^3 FP32: 64x16 d4 and a whole z column per workgroup 144 Gcell-steps/s,
+// 64x12 d2 with 64-plane chunks 106.
+template <typename T>
+struct TileOf {
+    static constexpr int TX = 64, TY = sizeof(T) == 4 ? 16 : 12, D = sizeof(T) == 4 ? 4 : 2;
+};
 
 }  // namespace
 
 bool yee_tb2_supported(const YeeDims& d, int elem_bytes) { return d.M2 * (long)elem_bytes < (1L << 32); }
 
-long yee_tb2_blocks(const YeeDims& d)
+// Work split: one workgroup per xy tile and z chunk; the chunk count is the
+// smallest that keeps every CU busy for about the same time (a chunk costs
+// its planes plus 5 pipeline fill planes; one workgroup per CU at a time).
+Tb2Plan yee_tb2_plan(const YeeDims& d, int elem_bytes, int cu_count)
 {
-    const long tiles_x = ceil_div(d.N + 1, kTTX - 3), tiles_y = ceil_div(d.N + 1, kTTY - 3);
-    return tiles_x * tiles_y * ceil_div(d.N + 1, kTZC);
+    Tb2Plan pl;
+    const int tx = 64, ty = elem_bytes == 4 ? 16 : 12;
+    pl.tiles_x = ceil_div(d.N + 1, tx - 3);
+    pl.tiles_y = ceil_div(d.N + 1, ty - 3);
+    const long nxy = (long)pl.tiles_x * pl.tiles_y;
+    const long cus = std::max(cu_count, 1);
+    long best = -1;
+    pl.zc = d.N + 1;
+    for (int nz = 1; nz <= 16; ++nz) {
+        const int zc = ceil_div(d.N + 1, nz);
+        const long rounds = (nxy * ceil_div(d.N + 1, zc) + cus - 1) / cus;
+        const long cost = rounds * (zc + 5);
+        if (best < 0 || cost < best) {
+            best = cost;
+            pl.zc = zc;
+        }
+    }
+    pl.nb = nxy * ceil_div(d.N + 1, pl.zc);
+    return pl;
 }
 
 template <typename T>
-int yee_tb2_launch(Ctx* c, const YeeDims& d, T s, const T* a, T* b, double* ep0, double* ep1)
+int yee_tb2_launch(Ctx* c, const YeeDims& d, const Tb2Plan& pl, T s, const T* a, T* b, double* ep0, double* ep1)
 {
-    const int tiles_x = ceil_div(d.N + 1, kTTX - 3), tiles_y = ceil_div(d.N + 1, kTTY - 3);
-    const long nb = yee_tb2_blocks(d);
-    if (nb > 0x7fffffffL) return fail(SCAFES_ERR_ARG, "yee: grid too large");
-    yee_tb2_kernel<T, kTTX, kTTY, kTD><<<dim3((unsigned)nb), dim3(kTTX, kTTY), 0, c->stream>>>(
-        d, s, a, b, kTZC, tiles_x, tiles_y, ep0, ep1);
+    typedef TileOf<T> Tl;
+    if (pl.nb > 0x7fffffffL) return fail(SCAFES_ERR_ARG, "yee: grid too large");
+    yee_tb2_kernel<T, Tl::TX, Tl::TY, Tl::D><<<dim3((unsigned)pl.nb), dim3(Tl::TX, Tl::TY), 0, c->stream>>>(
+        d, s, a, b, pl.zc, pl.tiles_x, pl.tiles_y, ep0, ep1);
     SCAFES_HIP(hipGetLastError());
     return SCAFES_OK;
 }
 
-template int yee_tb2_launch<float>(Ctx*, const YeeDims&, float, const float*, float*, double*, double*);
-template int yee_tb2_launch<double>(Ctx*, const YeeDims&, double, const double*, double*, double*, double*);
+template int yee_tb2_launch<float>(Ctx*, const YeeDims&, const Tb2Plan&, float, const float*, float*, double*,
+                                   double*);
+template int yee_tb2_launch<double>(Ctx*, const YeeDims&, const Tb2Plan&, double, const double*, double*, double*,
+                                    double*);
 
 }  // namespace scafes
