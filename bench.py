@@ -115,6 +115,18 @@ def profiled_traffic(path=os.path.join(ROOT, "profiles", "heat7var_kahan_512.md"
         return None, None
 
 
+def _profile_fields(name, applies):
+    """HBM bytes per launch and average launch time from a committed
+    rocprofv3 summary (only when the run's shape is the profiled one)."""
+    if not applies:
+        return {}
+    traffic, ns = profiled_traffic(os.path.join(ROOT, "profiles", name))
+    if traffic is None:
+        return {}
+    return {"hbm_traffic_bytes_per_launch": traffic, "profile_avg_launch_us": round(ns / 1e3, 2),
+            "profile": "profiles/" + name}
+
+
 def nodes(arr, n):
     return arr.reshape(n + 2, n + 2, n + 2)[1:-1, 1:-1, 1:-1]
 
@@ -172,7 +184,8 @@ def bench_yee(ctx, capi, ncell=384, nsteps=2000, energy_every=100):
             "ms_per_step": round(per, 5), "gcell_steps_per_s": round(cells / (per * 1e-3) / 1e9, 3),
             "algorithmic_bytes_per_cell_step": 48, "moved_bytes_per_cell_step": 24,
             "hbm_frac_algorithmic": round(48 * cells / (per * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "energy_rel_drift_fp64": float(np.abs(e - e[0]).max() / e[0]), "energy_samples": int(nrec)}
+            "energy_rel_drift_fp64": float(np.abs(e - e[0]).max() / e[0]), "energy_samples": int(nrec),
+            **_profile_fields("yee_tb2_384.md", ncell == 384)}
 
 
 def bench_pennes(ctx, capi, n=256, nsteps=2000, snapshots=(62, 1999)):
@@ -217,7 +230,10 @@ def bench_pennes(ctx, capi, n=256, nsteps=2000, snapshots=(62, 1999)):
     t_fwd = ctx.elapsed_ms(0, 1)
     out = {"workload": f"pennes {n}^3, {nsteps} steps, FP32 fields (deviation from T_a), FP64 gradient",
            "fwd_ms": round(t_fwd, 2), "fwd_GLUPS": round(float(n) ** 3 * nsteps / (t_fwd * 1e-3) / 1e9, 2),
-           "gradient": []}
+           "fwd_kernel": "stream_kernel<PennesOp<float>> (16 B/LUP)",
+           "adjoint_kernel": "pennes_adj2_kernel<float> (two reverse steps per sweep, 18 B/step)",
+           "fwd_profile": _profile_fields("pennes_fwd_256.md", n == 256),
+           "adjoint_profile": _profile_fields("pennes_adj2_256.md", n == 256), "gradient": []}
     grad = ctx.alloc(ne, np.float64)
     for s in snapshots:
         J = capi.C.c_double(0)
@@ -300,6 +316,9 @@ def main():
     ap.add_argument("--no-pennes", action="store_true", help="skip the Pennes adjoint leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--calibrate-copy", action="store_true", help="also run a 1 GiB copy (counter calibration)")
+    ap.add_argument("--leg-steps", type=int, default=0,
+                    help="step count of the Yee / Pennes / batch legs (0 = their workloads' counts; "
+                         "short runs are for profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
@@ -416,13 +435,13 @@ def main():
         out["copy_calibration"] = calibrate_copy(ctx, capi)
         log(f"[bench] copy {out['copy_calibration']}")
     if not args.no_yee:
-        out["yee"] = bench_yee(ctx, capi)
+        out["yee"] = bench_yee(ctx, capi, **({"nsteps": args.leg_steps} if args.leg_steps else {}))
         log(f"[bench] yee {out['yee']}")
     if not args.no_pennes:
-        out["pennes_adjoint"] = bench_pennes(ctx, capi)
+        out["pennes_adjoint"] = bench_pennes(ctx, capi, **({"nsteps": args.leg_steps} if args.leg_steps else {}))
         log(f"[bench] pennes {out['pennes_adjoint']}")
     if not args.no_batch:
-        out["batch2d"] = bench_batch(ctx, capi)
+        out["batch2d"] = bench_batch(ctx, capi, **({"nsteps": args.leg_steps} if args.leg_steps else {}))
         log(f"[bench] batch {out['batch2d']}")
     ctx.close()
     print(json.dumps(out), flush=True)
