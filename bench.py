@@ -93,8 +93,8 @@ def heatvar_kernel(ctx, capi, g, mode):
                                               capi.C.byref(d), capi.C.byref(st)))
     t, m = {capi.HEATVAR_F64: ("double", 0), capi.HEATVAR_F32: ("float", 1),
             capi.HEATVAR_F32_KAHAN: ("float", 2)}[mode]
-    if st.value == 2:
-        name = f"heatvar_tb2_kernel<{t}, {m}, {tx.value}, {ty.value}, {d.value}>"
+    if st.value >= 2:
+        name = f"heatvar_tb{st.value}_kernel<{t}, {m}, {tx.value}, {ty.value}, {d.value}>"
     else:
         name = f"stream_kernel<scafes::HeatVarOp<{t}, {m}>, {t}, {tx.value}, {ty.value}, {d.value}>"
     return cfg.value, name, st.value, zc.value

@@ -142,8 +142,8 @@ int scafes_heat7var_run(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode,
                         int border_mode, double border_value, int cfg, int* result_in_second);
 /* Number of tile configurations and a description of one: workgroup tile,
  * z planes per workgroup, prefetch depth in planes, and time steps per HBM
- * sweep (1, or 2 for the temporally blocked kernel; nsteps odd then ends
- * with one single-step sweep). */
+ * sweep (1, or 2 / 3 for the temporally blocked kernels; steps left over
+ * at the end run one per sweep). */
 int scafes_heat7var_num_cfgs(void);
 int scafes_heat7var_cfg_desc(int cfg, int* tx, int* ty, int* zchunk, int* depth,
                              int* steps_per_sweep);

@@ -1,5 +1,5 @@
-// heatvar_tb2.hip -- two time steps per HBM sweep for the variable-kappa heat
-// update (temporal blocking) on 3-D grids, same arithmetic per node as the
+// heatvar_tb2.hip -- two (or three) time steps per HBM sweep for the
+// variable-kappa heat update (temporal blocking) on 3-D grids, same arithmetic per node as the
 // one-step kernel in stencil3d.hip, so results are bit-identical to it in
 // every precision (and, in FP64, to the CPU path).
 //
@@ -260,54 +260,244 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb2_kernel(Geo g, const T* __r
     }
 }
 
-struct Tb2Cfg {
-    int tx, ty, zc, d;
-};
-static const Tb2Cfg kTb2Cfgs[] = {
-    {64, 16, 64, 2}, {64, 16, 128, 2}, {64, 8, 64, 2}, {128, 8, 64, 2}, {64, 16, 64, 4}, {128, 4, 64, 2},
-    {64, 8, 64, 4},  {64, 8, 32, 2},   {64, 8, 128, 2}, {64, 4, 64, 2},
-};
-static constexpr int kNumTb2 = sizeof(kTb2Cfgs) / sizeof(kTb2Cfgs[0]);
-
+// Three steps per sweep.  Threads cover the output tile plus a two-node
+// ring: level 1 (u after one step) on every thread position, level 2 on
+// positions one node inside, level 3 on the (TX-4) x (TY-4) output tile.
+// Iteration kk computes level 1 at plane kk, level 2 at kk-1 and level 3 at
+// kk-2; levels 2 and 3 read the in-plane neighbours of the level below from
+// an LDS plane written one iteration earlier (one barrier per plane, all
+// planes double-buffered).  The face coefficients of a node are the same
+// at every level, so levels 2 and 3 take them from a register queue.
 template <typename T, int MODE, int TX, int TY, int D>
-static int tb2_launch_t(Ctx* c, const Geo& g, const void* u, const void* cin, const void* kap, void* uo, void* co,
-                        const VarCoef& cf, int zc)
+__global__ __launch_bounds__(TX* TY) void heatvar_tb3_kernel(Geo g, const T* __restrict__ u, const T* __restrict__ cin,
+                                                              const T* __restrict__ kap, T* __restrict__ uout,
+                                                              T* __restrict__ cout, VarCoef cf, int zc, int tiles_x,
+                                                              int tiles_y)
 {
-    const int tiles_x = ceil_div(g.nx, TX - 2), tiles_y = ceil_div(g.ny, TY - 2), tiles_z = ceil_div(g.nz, zc);
+    static_assert(D >= 2 && D % 2 == 0, "prefetch depth must be even");
+    constexpr int OX = TX - 4, OY = TY - 4;
+    constexpr bool KH = (MODE == 2);
+    __shared__ T s0[2][TY + 2][TX + 2];  // level-0 plane with ring
+    __shared__ T sk[2][TY + 2][TX + 2];  // kappa plane with ring
+    __shared__ T s1[2][TY][TX];          // level-1 plane
+    __shared__ T s2[2][TY][TX];          // level-2 plane
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int per_plane = tiles_x * tiles_y;
+    const int tz = lb / per_plane;
+    const int r = lb - tz * per_plane;
+    const int tyi = r / tiles_x;
+    const int txi = r - tyi * tiles_x;
+    const int i = txi * OX - 2 + tx, j = tyi * OY - 2 + ty;
+    const int ic = min(max(i, 0), g.nx - 1), jc = min(max(j, 0), g.ny - 1);
+    const bool inner_xy = i >= g.nl0 && i <= g.nx - 1 - g.nl0 && j >= g.nl1 && j <= g.ny - 1 - g.nl1;
+    const bool out_thread = tx >= 2 && tx <= TX - 3 && ty >= 2 && ty <= TY - 3 && i < g.nx && j < g.ny;
+    // clamped neighbour slots of levels 2 and 3 (clamped reads only feed
+    // positions that are never stored)
+    const int txm = max(tx - 1, 0), txp = min(tx + 1, TX - 1), tym = max(ty - 1, 0), typ = min(ty + 1, TY - 1);
+    const int k0 = tz * zc, k1 = min(k0 + zc, g.nz);
+    const int kfirst = k0 - 2;  // first plane with a level-1 value
+    const int klast = k1 + 1;   // last iteration (level 1 at k1 + 1)
+    const int kcen = k1 + 2;    // last plane whose centre is read
+    const int klo = g.nl2, khi = g.nz - 1 - g.nl2;
+    const bool hx = (tx == 0) || (tx == TX - 1);
+    const bool hy = (ty == 0) || (ty == TY - 1);
+    const int hxi = (tx == 0) ? max(ic - 1, 0) : min(ic + 1, g.nx - 1);
+    const int hyj = (ty == 0) ? max(jc - 1, 0) : min(jc + 1, g.ny - 1);
+    const int sxh = (tx == 0) ? 0 : TX + 1;
+    const int syh = (ty == 0) ? 0 : TY + 1;
+    constexpr uint32_t B = sizeof(T);
+    const uint32_t oc = (uint32_t)(g.base + ic + g.e0 * jc) * B;
+    const uint32_t ohx = (uint32_t)(g.base + hxi + g.e0 * jc) * B;
+    const uint32_t ohy = (uint32_t)(g.base + ic + g.e0 * hyj) * B;
+    const uint32_t oo = (uint32_t)(g.base + i + g.e0 * j) * B;  // output node (out_thread only)
+    const size_t pb = (size_t)g.e01 * B;
+    const uint32_t range = (uint32_t)((g.base + g.e01) * B);
+    const char* ub = (const char*)u;
+    const char* kb8 = (const char*)kap;
+    const char* cb = (const char*)cin;
+    auto plane = [&](int kk) { return (size_t)min(max(kk, 0), g.nz - 1) * pb; };
+
+    T qa[D], qb[D], qc[D], qax[D], qbx[D], qay[D], qby[D];
+#pragma unroll
+    for (int s = 0; s < D; ++s) qa[s] = qb[s] = qc[s] = qax[s] = qbx[s] = qay[s] = qby[s] = T(0);
+    auto fetch = [&](const int slot, const int kk) {
+        if (kk <= kcen) {
+            const size_t o = plane(kk);
+            const Rsrc ru = plane_rsrc(ub + o, range), rk = plane_rsrc(kb8 + o, range);
+            qa[slot] = bload<T>(ru, oc);
+            qb[slot] = bload<T>(rk, oc);
+            if (kk <= klast) {
+                if constexpr (KH) qc[slot] = bload<T>(plane_rsrc(cb + o, range), oc);
+                if (hx) {
+                    qax[slot] = bload<T>(ru, ohx);
+                    qbx[slot] = bload<T>(rk, ohx);
+                }
+                if (hy) {
+                    qay[slot] = bload<T>(ru, ohy);
+                    qby[slot] = bload<T>(rk, ohy);
+                }
+            }
+        }
+    };
+    T am = bload<T>(plane_rsrc(ub + plane(kfirst - 1), range), oc);
+    const T bm = bload<T>(plane_rsrc(kb8 + plane(kfirst - 1), range), oc);
+#pragma unroll
+    for (int s = 0; s < D; ++s) fetch(s, kfirst + s);
+
+    T l1m1 = T(0), l1m2 = T(0), c1m1 = T(0);  // level 1 at kk-1, kk-2; compensation at kk-1
+    T l2m1 = T(0), l2m2 = T(0), c2m1 = T(0);  // level 2 at kk-2, kk-3; compensation at kk-2
+    Faces<T> fq1 = {}, fq2 = {};                // faces of planes kk-1, kk-2
+    T fup = face<T, MODE>(qb[0], bm);
+    const T bv = (T)cf.border_value;
+    const bool keep = cf.border_mode != SCAFES_BORDER_VALUE;
+    for (int kb = kfirst; kb <= klast; kb += D) {
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+            const int k = kb + s;
+            if (k > klast) break;  // uniform over the workgroup
+            const int buf = s & 1, pbf = buf ^ 1;
+            const T ac = qa[s], ap = qa[(s + 1) % D];
+            const T bc = qb[s], bp = qb[(s + 1) % D];
+            const T c0 = qc[s];
+            s0[buf][ty + 1][tx + 1] = ac;
+            sk[buf][ty + 1][tx + 1] = bc;
+            if (hx) {
+                s0[buf][ty + 1][sxh] = qax[s];
+                sk[buf][ty + 1][sxh] = qbx[s];
+            }
+            if (hy) {
+                s0[buf][syh][tx + 1] = qay[s];
+                sk[buf][syh][tx + 1] = qby[s];
+            }
+            fetch(s, k + D);
+            __syncthreads();
+            // ---- level 1 at plane k
+            Faces<T> f;
+            f.e = face<T, MODE>(sk[buf][ty + 1][tx + 2], bc);
+            f.w = face<T, MODE>(bc, sk[buf][ty + 1][tx]);
+            f.n = face<T, MODE>(sk[buf][ty + 2][tx + 1], bc);
+            f.s = face<T, MODE>(bc, sk[buf][ty][tx + 1]);
+            f.u = face<T, MODE>(bp, bc);
+            f.d = fup;
+            T c1;
+            T l1 = var_update<T, MODE>(cf, ac, s0[buf][ty + 1][tx], s0[buf][ty + 1][tx + 2], s0[buf][ty][tx + 1],
+                                       s0[buf][ty + 2][tx + 1], am, ap, f, c0, &c1);
+            if (!(inner_xy && k >= klo && k <= khi)) {
+                l1 = keep ? ac : bv;
+                c1 = keep ? c0 : T(0);
+            }
+            s1[buf][ty][tx] = l1;
+            // ---- level 2 at plane k-1 from the level-1 plane written one iteration ago
+            const int k2 = k - 1;
+            T c2;
+            T l2 = var_update<T, MODE>(cf, l1m1, s1[pbf][ty][txm], s1[pbf][ty][txp], s1[pbf][tym][tx],
+                                       s1[pbf][typ][tx], l1m2, l1, fq1, c1m1, &c2);
+            if (!(inner_xy && k2 >= klo && k2 <= khi)) {
+                l2 = keep ? l1m1 : bv;
+                c2 = keep ? c1m1 : T(0);
+            }
+            s2[buf][ty][tx] = l2;
+            // ---- level 3 at plane k-2 from the level-2 plane written one iteration ago
+            const int k3 = k - 2;
+            if (k3 >= k0 && k3 < k1 && out_thread) {
+                T c3;
+                T l3 = var_update<T, MODE>(cf, l2m1, s2[pbf][ty][tx - 1], s2[pbf][ty][tx + 1], s2[pbf][ty - 1][tx],
+                                           s2[pbf][ty + 1][tx], l2m2, l2, fq2, c2m1, &c3);
+                if (!(inner_xy && k3 >= klo && k3 <= khi)) {
+                    l3 = keep ? l2m1 : bv;
+                    c3 = keep ? c2m1 : T(0);
+                }
+                const size_t o = (size_t)k3 * pb;
+                bstore<T>(l3, plane_rsrc((const char*)uout + o, range), oo);
+                if constexpr (KH) bstore<T>(c3, plane_rsrc((const char*)cout + o, range), oo);
+            }
+            l2m2 = l2m1;
+            l2m1 = l2;
+            c2m1 = c2;
+            l1m2 = l1m1;
+            l1m1 = l1;
+            c1m1 = c1;
+            fq2 = fq1;
+            fq1 = f;
+            fup = f.u;
+            am = ac;
+        }
+    }
+}
+
+// Multi-step tiles: steps per sweep (2 or 3), workgroup tile, z planes per
+// workgroup, prefetch depth.  The two-step entries keep their indices.
+struct TbCfg {
+    int steps, tx, ty, zc, d;
+};
+static const TbCfg kTbCfgs[] = {
+    {2, 64, 16, 64, 2}, {2, 64, 16, 128, 2}, {2, 64, 8, 64, 2},  {2, 128, 8, 64, 2}, {2, 64, 16, 64, 4},
+    {2, 128, 4, 64, 2}, {2, 64, 8, 64, 4},   {2, 64, 8, 32, 2},  {2, 64, 8, 128, 2}, {2, 64, 4, 64, 2},
+    {3, 64, 16, 64, 2}, {3, 64, 16, 32, 2},  {3, 64, 16, 128, 2}, {3, 64, 12, 64, 2}, {3, 64, 16, 64, 4},
+    {3, 128, 8, 64, 2}, {3, 64, 8, 64, 2},   {3, 64, 12, 32, 2}, {3, 64, 12, 64, 4}, {3, 64, 12, 96, 2},
+    {3, 64, 10, 64, 2}, {3, 64, 14, 64, 2},
+};
+static constexpr int kNumTb = sizeof(kTbCfgs) / sizeof(kTbCfgs[0]);
+
+template <int STEPS, typename T, int MODE, int TX, int TY, int D>
+static int tb_launch_t(Ctx* c, const Geo& g, const void* u, const void* cin, const void* kap, void* uo, void* co,
+                       const VarCoef& cf, int zc)
+{
+    constexpr int RING = STEPS - 1;  // ring width of the thread tile on each side
+    const int tiles_x = ceil_div(g.nx, TX - 2 * RING), tiles_y = ceil_div(g.ny, TY - 2 * RING);
+    const int tiles_z = ceil_div(g.nz, zc);
     const long nb = (long)tiles_x * tiles_y * tiles_z;
     if (nb > 0x7fffffffL) return fail(SCAFES_ERR_ARG, "grid too large");
-    heatvar_tb2_kernel<T, MODE, TX, TY, D><<<dim3((unsigned)nb), dim3(TX, TY), 0, c->stream>>>(
-        g, (const T*)u, (const T*)cin, (const T*)kap, (T*)uo, (T*)co, cf, zc, tiles_x, tiles_y);
+    if constexpr (STEPS == 2)
+        heatvar_tb2_kernel<T, MODE, TX, TY, D><<<dim3((unsigned)nb), dim3(TX, TY), 0, c->stream>>>(
+            g, (const T*)u, (const T*)cin, (const T*)kap, (T*)uo, (T*)co, cf, zc, tiles_x, tiles_y);
+    else
+        heatvar_tb3_kernel<T, MODE, TX, TY, D><<<dim3((unsigned)nb), dim3(TX, TY), 0, c->stream>>>(
+            g, (const T*)u, (const T*)cin, (const T*)kap, (T*)uo, (T*)co, cf, zc, tiles_x, tiles_y);
     SCAFES_HIP(hipGetLastError());
     return SCAFES_OK;
 }
 
 template <typename T, int MODE>
-static int tb2_dispatch(Ctx* c, const Geo& g, const void* u, const void* cin, const void* kap, void* uo, void* co,
-                        const VarCoef& cf, int cfg)
+static int tb_dispatch(Ctx* c, const Geo& g, const void* u, const void* cin, const void* kap, void* uo, void* co,
+                       const VarCoef& cf, int cfg)
 {
-    const Tb2Cfg t = kTb2Cfgs[cfg];
-    switch (t.tx * 100000 + t.ty * 100 + t.d) {
-    case 6401602: return tb2_launch_t<T, MODE, 64, 16, 2>(c, g, u, cin, kap, uo, co, cf, t.zc);
-    case 6400802: return tb2_launch_t<T, MODE, 64, 8, 2>(c, g, u, cin, kap, uo, co, cf, t.zc);
-    case 6400804: return tb2_launch_t<T, MODE, 64, 8, 4>(c, g, u, cin, kap, uo, co, cf, t.zc);
-    case 6400402: return tb2_launch_t<T, MODE, 64, 4, 2>(c, g, u, cin, kap, uo, co, cf, t.zc);
-    case 12800802: return tb2_launch_t<T, MODE, 128, 8, 2>(c, g, u, cin, kap, uo, co, cf, t.zc);
-    case 6401604: return tb2_launch_t<T, MODE, 64, 16, 4>(c, g, u, cin, kap, uo, co, cf, t.zc);
-    case 12800402: return tb2_launch_t<T, MODE, 128, 4, 2>(c, g, u, cin, kap, uo, co, cf, t.zc);
-    default: return fail(SCAFES_ERR_ARG, "bad two-step tile configuration");
+    const TbCfg t = kTbCfgs[cfg];
+#define SCAFES_TB(S, X, Y, DD)                                                                    \
+    case S * 10000000 + X * 100000 + Y * 100 + DD:                                                \
+        return tb_launch_t<S, T, MODE, X, Y, DD>(c, g, u, cin, kap, uo, co, cf, t.zc);
+    switch (t.steps * 10000000 + t.tx * 100000 + t.ty * 100 + t.d) {
+        SCAFES_TB(2, 64, 16, 2)
+        SCAFES_TB(2, 64, 8, 2)
+        SCAFES_TB(2, 64, 8, 4)
+        SCAFES_TB(2, 64, 4, 2)
+        SCAFES_TB(2, 128, 8, 2)
+        SCAFES_TB(2, 64, 16, 4)
+        SCAFES_TB(2, 128, 4, 2)
+        SCAFES_TB(3, 64, 16, 2)
+        SCAFES_TB(3, 64, 12, 2)
+        SCAFES_TB(3, 64, 16, 4)
+        SCAFES_TB(3, 128, 8, 2)
+        SCAFES_TB(3, 64, 8, 2)
+        SCAFES_TB(3, 64, 12, 4)
+        SCAFES_TB(3, 64, 10, 2)
+        SCAFES_TB(3, 64, 14, 2)
+    default: return fail(SCAFES_ERR_ARG, "bad multi-step tile configuration");
     }
+#undef SCAFES_TB
 }
 
-int heatvar_tb2_num_cfgs() { return kNumTb2; }
+int heatvar_tb2_num_cfgs() { return kNumTb; }
 
-void heatvar_tb2_cfg(int cfg, int* tx, int* ty, int* zc, int* d)
+void heatvar_tb2_cfg(int cfg, int* tx, int* ty, int* zc, int* d, int* steps)
 {
-    const Tb2Cfg t = kTb2Cfgs[cfg];
+    const TbCfg t = kTbCfgs[cfg];
     *tx = t.tx;
     *ty = t.ty;
     *zc = t.zc;
     *d = t.d;
+    *steps = t.steps;
 }
 
 bool heatvar_tb2_supported(const Geo& geo, int mode)
@@ -317,13 +507,13 @@ bool heatvar_tb2_supported(const Geo& geo, int mode)
     return geo.dim == 3 && (geo.base + geo.e01) * bytes < (1L << 32);
 }
 
-// Two steps u -> uo (and Kahan compensation cin -> co).
+// The configuration's number of steps u -> uo (and Kahan compensation cin -> co).
 int heatvar_tb2_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* uo,
                        const void* cin, void* co, const void* kap, double tau, int border_mode, double border_value,
                        int cfg)
 {
-    if (cfg < 0 || cfg >= kNumTb2) return fail(SCAFES_ERR_ARG, "bad two-step tile configuration");
-    if (!heatvar_tb2_supported(geo, mode)) return fail(SCAFES_ERR_UNSUPPORTED, "two-step sweep needs a 3-D grid");
+    if (cfg < 0 || cfg >= kNumTb) return fail(SCAFES_ERR_ARG, "bad multi-step tile configuration");
+    if (!heatvar_tb2_supported(geo, mode)) return fail(SCAFES_ERR_UNSUPPORTED, "multi-step sweep needs a 3-D grid");
     VarCoef cf;
     double hh[3];
     for (int p = 0; p < 3; ++p) hh[p] = g->h[p] * g->h[p];
@@ -336,9 +526,9 @@ int heatvar_tb2_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, c
     cf.c2 = (float)(0.5 * tau / hh[2]);
     cf.border_value = border_value;
     cf.border_mode = border_mode;
-    if (mode == SCAFES_HEATVAR_F64) return tb2_dispatch<double, 0>(c, geo, u, nullptr, kap, uo, nullptr, cf, cfg);
-    if (mode == SCAFES_HEATVAR_F32) return tb2_dispatch<float, 1>(c, geo, u, nullptr, kap, uo, nullptr, cf, cfg);
-    return tb2_dispatch<float, 2>(c, geo, u, cin, kap, uo, co, cf, cfg);
+    if (mode == SCAFES_HEATVAR_F64) return tb_dispatch<double, 0>(c, geo, u, nullptr, kap, uo, nullptr, cf, cfg);
+    if (mode == SCAFES_HEATVAR_F32) return tb_dispatch<float, 1>(c, geo, u, nullptr, kap, uo, nullptr, cf, cfg);
+    return tb_dispatch<float, 2>(c, geo, u, cin, kap, uo, co, cf, cfg);
 }
 
 }  // namespace scafes

@@ -102,9 +102,10 @@ int pennes_adj2_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, 
                        const void* w, const void* t1, const void* t1lo, const void* t2, const void* t2lo,
                        double* grad, const scafes_pennes_params* p);
 
-// Two-steps-per-sweep variable-kappa heat (defined in heatvar_tb2.hip).
+// Multi-step (two or three steps per sweep) variable-kappa heat (defined in
+// heatvar_tb2.hip); heatvar_tb2_launch advances the configuration's steps.
 int heatvar_tb2_num_cfgs();
-void heatvar_tb2_cfg(int cfg, int* tx, int* ty, int* zc, int* d);
+void heatvar_tb2_cfg(int cfg, int* tx, int* ty, int* zc, int* d, int* steps);
 bool heatvar_tb2_supported(const Geo& geo, int mode);  // 3-D grids with 32-bit in-plane offsets
 int heatvar_tb2_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* uo,
                        const void* cin, void* co, const void* kap, double tau, int border_mode, double border_value,

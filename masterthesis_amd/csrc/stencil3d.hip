@@ -368,10 +368,10 @@ static constexpr int kDefaultCfg = 3;
 // (F64 64x8 zc128 d6, F32 128x2 zc64 d4, F32+Kahan 64x16 zc128 d4); used
 // for the odd last step behind two-steps-per-sweep launches.
 static constexpr int kOneStepCfgByMode[3] = {12, 4, 9};
-// Default per mode: the fastest two-steps-per-sweep tile at 512^3 on an
-// MI355X (indices into heatvar_tb2.hip's table): F64 64x8 zc64 d4 (6),
-// F32 64x16 zc64 d2 (0), F32+Kahan 64x8 zc32 d2 (7).
-static constexpr int kDefaultCfgByMode[3] = {kNumCfgs + 6, kNumCfgs + 0, kNumCfgs + 7};
+// Default per mode: the fastest multi-step tile at 512^3 on an MI355X
+// (indices into heatvar_tb2.hip's table): F64 two steps per sweep, 64x8
+// zc64 d4 (6); F32 and F32+Kahan three steps per sweep, 64x12 zc64 d2 (13).
+static constexpr int kDefaultCfgByMode[3] = {kNumCfgs + 6, kNumCfgs + 13, kNumCfgs + 13};
 
 template <class Op, typename T, int TX, int TY, int D>
 static int launch_t(Ctx* c, const Op& op, const Geo& g, int zc)
@@ -663,8 +663,7 @@ int scafes_heat7var_cfg_desc(int cfg, int* tx, int* ty, int* zchunk, int* depth,
         d = kCfgs[cfg].d;
         st = 1;
     } else {
-        heatvar_tb2_cfg(cfg - kNumCfgs, &a, &b, &z, &d);
-        st = 2;
+        heatvar_tb2_cfg(cfg - kNumCfgs, &a, &b, &z, &d, &st);
     }
     if (tx) *tx = a;
     if (ty) *ty = b;
@@ -719,14 +718,16 @@ int scafes_heat7var_run(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode,
     int launches = 0;
     int s = 0;
     if (cfg >= kNumCfgs) {
-        for (; s + 2 <= nsteps; s += 2, ++launches) {
+        int tx_, ty_, zc_, d_, st = 1;
+        heatvar_tb2_cfg(cfg - kNumCfgs, &tx_, &ty_, &zc_, &d_, &st);
+        for (; s + st <= nsteps; s += st, ++launches) {
             rc = heatvar_tb2_launch(c, geo, g, mode, a, b, ca, cb, kappa, tau, border_mode, border_value,
                                     cfg - kNumCfgs);
             if (rc) return rc;
             std::swap(a, b);
             std::swap(ca, cb);
         }
-        cfg = kOneStepCfgByMode[mode];  // odd remainder
+        cfg = kOneStepCfgByMode[mode];  // remaining steps one per sweep
     }
     for (; s < nsteps; ++s, ++launches) {
         rc = heat7var_launch(c, geo, g, mode, a, b, ca, cb, kappa, tau, border_mode, border_value, cfg);

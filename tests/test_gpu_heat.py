@@ -219,13 +219,13 @@ def test_heatvar_f64_bitexact_all_tiles(ctx, cfg):
     np.testing.assert_array_equal(O.nodes_view(og, got), O.nodes_view(og, ref))
 
 
-TB2_CFGS = [c for c in range(NUM_CFGS) if cfg_steps_per_sweep(c) == 2]
+TB_CFGS = [c for c in range(NUM_CFGS) if cfg_steps_per_sweep(c) >= 2]
 
 
 @pytest.mark.parametrize("mode", [capi.HEATVAR_F32, capi.HEATVAR_F32_KAHAN, capi.HEATVAR_F64])
-@pytest.mark.parametrize("cfg", TB2_CFGS)
-def test_heatvar_two_step_sweep_matches_one_step(ctx, cfg, mode):
-    """Two steps per sweep perform the one-step kernel's arithmetic node by node:
+@pytest.mark.parametrize("cfg", TB_CFGS)
+def test_heatvar_multi_step_sweep_matches_one_step(ctx, cfg, mode):
+    """Two or three steps per sweep perform the one-step kernel's arithmetic node by node:
     identical bits in every precision, including the Kahan compensation, with
     nonzero kept borders, several z chunks and a tile grid that does not
     divide the domain."""
