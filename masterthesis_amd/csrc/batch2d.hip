@@ -8,16 +8,19 @@
 // tests/system/HeatEqnFDM/HeatEqnFDM.hpp:204-218, coefficient kappa_i):
 //   u' = u + (tau kappa / h^2) ((W + E - 2u) + (N + S - 2u))   on inner nodes.
 //
-// Mapping: one 32x16-thread workgroup per CU owns a whole problem at a
-// time.  Thread (tx, ty) keeps a BX x BY block of nodes (BX = ceil(n/32),
-// BY = ceil(n/16), so BX*BY <= 128 for n <= 256) in registers for all steps
-// (2 waves per SIMD leave 256 registers per lane).  Per step it publishes
-// its block's four edges to LDS (component-major, bank-conflict free) and
-// reads its neighbours' edges back: LDS traffic is one value per edge node
-// instead of five per node, and HBM is touched only to load and store each
-// problem once.  Problems are taken in descending BX*BY cost in a snake
-// order over workgroups (longest processing time first; no atomics and no
-// counters to reset between graph replays).
+// Mapping: one 32x32-thread workgroup per CU owns a whole problem at a
+// time.  Thread (tx, ty) keeps a B x B block of nodes (B = ceil(n/32) <= 8
+// for n <= 256) in registers for all steps (4 waves per SIMD leave 128
+// registers per lane).  Per step it publishes its block's four edges to LDS
+// (component-major, bank-conflict free) and reads its neighbours' edges
+// back: LDS traffic is one value per edge node instead of five per node,
+// and HBM is touched only to load and store each problem once.  Border
+// nodes take the update with a zero coefficient (fma(0, lap, u) = u).
+// Each block size B is a separate non-inlined function, so each gets its
+// own register allocation: inlined into one kernel, the B = 8 body's
+// pressure would spill in every smaller body too.  Problems are taken in
+// descending B in a snake order over workgroups (longest processing time
+// first; no atomics and no counters to reset between graph replays).
 //
 // Gradient: with kappa scalar and tau fixed, the step operator
 // A = I + tau kappa L (L symmetric on inner nodes with zero border) commutes
@@ -28,16 +31,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <type_traits>
 #include <vector>
 
 #include "scafes_common.hpp"
 
 namespace scafes {
 
-constexpr int kTX = 32, kTY = 16;  // workgroup shape
+constexpr int kTX = 32, kTY = 32;  // workgroup shape
 constexpr int kNmax = 256;         // largest problem edge
-constexpr int kBXmax = kNmax / kTX, kBYmax = kNmax / kTY;
+constexpr int kBmax = kNmax / kTX;
 
 struct BatchArgs {
     const int* n;
@@ -55,136 +57,139 @@ struct BatchArgs {
 
 // Block edges published per step (LDS): top / bottom rows indexed
 // [column][ty][tx], left / right columns [row][ty][tx].
-__shared__ float eT[kBXmax][kTY][kTX];
-__shared__ float eB[kBXmax][kTY][kTX];
-__shared__ float eL[kBYmax][kTY][kTX];
-__shared__ float eR[kBYmax][kTY][kTX];
-__shared__ double red[2 * kTX * kTY];
+__shared__ float eT[kBmax][kTY][kTX];
+__shared__ float eB[kBmax][kTY][kTX];
+__shared__ float eL[kBmax][kTY][kTX];
+__shared__ float eR[kBmax][kTY][kTX];
+__shared__ double red[2][kTX * kTY / 64];
 
-template <int BX, int BY>
-__device__ void run_problem(const BatchArgs& a, int p)
+template <int B>
+__device__ __attribute__((noinline)) void run_problem(const BatchArgs& a, int p)
 {
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int n = a.n[p];
     const double h = 1.0 / (n - 1);
     const float ck = (float)((double)a.tau[p] * (double)a.kappa[p] / (h * h));
     float* __restrict__ u = a.u + a.off[p];
-    const int ox = tx * BX, oy = ty * BY;
+    const int ox = tx * B, oy = ty * B;
     // bit c of inx / iny: column ox+c / row oy+c is inner (1..n-2); vx / vy: is a node (< n)
     unsigned inx = 0, iny = 0, vx = 0, vy = 0;
 #pragma unroll
-    for (int c = 0; c < BX; ++c) {
+    for (int c = 0; c < B; ++c) {
         inx |= (ox + c >= 1 && ox + c <= n - 2) ? (1u << c) : 0u;
         vx |= (ox + c < n) ? (1u << c) : 0u;
+        iny |= (oy + c >= 1 && oy + c <= n - 2) ? (1u << c) : 0u;
+        vy |= (oy + c < n) ? (1u << c) : 0u;
     }
+    float s[B][B];
 #pragma unroll
-    for (int r = 0; r < BY; ++r) {
-        iny |= (oy + r >= 1 && oy + r <= n - 2) ? (1u << r) : 0u;
-        vy |= (oy + r < n) ? (1u << r) : 0u;
-    }
-    float s[BY][BX];
-#pragma unroll
-    for (int r = 0; r < BY; ++r) {
+    for (int r = 0; r < B; ++r) {
         const float* __restrict__ row = u + (long)(oy + r) * n + ox;
 #pragma unroll
-        for (int c = 0; c < BX; ++c) s[r][c] = ((vy >> r) & (vx >> c) & 1u) ? row[c] : 0.0f;
+        for (int c = 0; c < B; ++c) s[r][c] = ((vy >> r) & (vx >> c) & 1u) ? row[c] : 0.0f;
     }
-    double gsum = 0.0, jsum = 0.0;
     const bool want_grad = a.target != nullptr;
-    const float* __restrict__ tgt = want_grad ? a.target + a.off[p] : nullptr;
-    // One explicit step on the register block; GRAD also forms the gradient
-    // and misfit terms of the final step.  Rows are updated top-down in
-    // place: prev[] keeps the old row above, `wold` the old left neighbour.
-    auto step = [&](auto grad_tag) {
-        constexpr bool GRAD = decltype(grad_tag)::value;
-        // Re-derive the per-node inner predicates from the bit masks every
-        // step: hoisted out of the time loop they would pin one mask per
-        // node in registers for the whole problem.
+    // One explicit step on the register block, rows top-down in place:
+    // prev[] keeps the old row above, `wold` the old left neighbour.  `last`
+    // (the final step when a gradient is wanted) stores u^N to HBM and keeps
+    // the inner nodes' L u^{N-1} terms (zero elsewhere) in the block.
+    auto step = [&](const bool last) {
+        // the masks live in VGPRs: scalar copies would be re-derived per use
         asm volatile("" : "+v"(inx), "+v"(iny));
 #pragma unroll
-        for (int c = 0; c < BX; ++c) {
+        for (int c = 0; c < B; ++c) {
             eT[c][ty][tx] = s[0][c];
-            eB[c][ty][tx] = s[BY - 1][c];
+            eB[c][ty][tx] = s[B - 1][c];
         }
 #pragma unroll
-        for (int r = 0; r < BY; ++r) {
+        for (int r = 0; r < B; ++r) {
             eL[r][ty][tx] = s[r][0];
-            eR[r][ty][tx] = s[r][BX - 1];
+            eR[r][ty][tx] = s[r][B - 1];
         }
         __syncthreads();
-        float prev[BX];
+        float prev[B];
 #pragma unroll
-        for (int c = 0; c < BX; ++c) prev[c] = ty > 0 ? eB[c][ty - 1][tx] : 0.0f;
+        for (int c = 0; c < B; ++c) prev[c] = ty > 0 ? eB[c][ty - 1][tx] : 0.0f;
 #pragma unroll
-        for (int r = 0; r < BY; ++r) {
+        for (int r = 0; r < B; ++r) {
             float wold = tx > 0 ? eR[r][ty][tx - 1] : 0.0f;
-            const bool rin = (iny >> r) & 1u;
+            const float ckr = ((iny >> r) & 1u) ? ck : 0.0f;
+            float* __restrict__ urow = u + (long)(oy + r) * n + ox;
 #pragma unroll
-            for (int c = 0; c < BX; ++c) {
+            for (int c = 0; c < B; ++c) {
                 const float uc = s[r][c];
-                const float e = c == BX - 1 ? (tx < kTX - 1 ? eL[r][ty][tx + 1] : 0.0f) : s[r][c + 1];
-                const float sn = r == BY - 1 ? (ty < kTY - 1 ? eT[c][ty + 1][tx] : 0.0f) : s[r + 1][c];
-                if (rin && ((inx >> c) & 1u)) {
-                    const float lap = ((wold + e) - 2.0f * uc) + ((prev[c] + sn) - 2.0f * uc);
-                    const float v = __builtin_fmaf(ck, lap, uc);
-                    s[r][c] = v;
-                    if constexpr (GRAD) {
-                        const double res = (double)v - (double)tgt[(long)(oy + r) * n + ox + c];
-                        gsum += res * (double)lap;
-                        jsum += res * res;
-                    }
-                }
+                const float e = c == B - 1 ? (tx < kTX - 1 ? eL[r][ty][tx + 1] : 0.0f) : s[r][c + 1];
+                const float sn = r == B - 1 ? (ty < kTY - 1 ? eT[c][ty + 1][tx] : 0.0f) : s[r + 1][c];
+                const float lap = ((wold + e) - 2.0f * uc) + ((prev[c] + sn) - 2.0f * uc);
+                const float k = ((inx >> c) & 1u) ? ckr : 0.0f;
+                const float v = __builtin_fmaf(k, lap, uc);
+                if (last && ((vy >> r) & (vx >> c) & 1u)) urow[c] = v;
+                s[r][c] = last ? (k != 0.0f ? lap : 0.0f) : v;
                 prev[c] = uc;
                 wold = uc;
             }
         }
         __syncthreads();
     };
-    for (int st = 0; st + 1 < a.nsteps; ++st) step(std::false_type{});
-    if (want_grad) step(std::true_type{});
-    else step(std::false_type{});
-    asm volatile("" : "+v"(vx), "+v"(vy));
-#pragma unroll
-    for (int r = 0; r < BY; ++r) {
-        float* __restrict__ row = u + (long)(oy + r) * n + ox;
-#pragma unroll
-        for (int c = 0; c < BX; ++c)
-            if ((vy >> r) & (vx >> c) & 1u) row[c] = s[r][c];
-    }
+    for (int st = 0; st + 1 < a.nsteps; ++st) step(false);
+    // The final step is the same code with a run-time flag: a second copy of
+    // the unrolled step (compile-time flag) would raise register pressure.
+    int lastg = want_grad ? 1 : 0;
+    asm volatile("" : "+v"(lastg));
+    step(lastg != 0);
     if (want_grad) {
-        constexpr int NT = kTX * kTY;
-        const int t = tx + kTX * ty;
-        red[t] = gsum;
-        red[NT + t] = jsum;
-        __syncthreads();
-        for (int w = NT / 2; w > 0; w >>= 1) {
-            if (t < w) {
-                red[t] += red[t + w];
-                red[NT + t] += red[NT + t + w];
+        // the final state was stored by this thread; the barrier at the end
+        // of the step made the stores visible to its own reloads
+        const float* __restrict__ tgt = a.target + a.off[p];
+        double gsum = 0.0, jsum = 0.0;
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            const long ro = (long)(oy + r) * n + ox;
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                if ((iny >> r) & (inx >> c) & 1u) {
+                    const double res = (double)u[ro + c] - (double)tgt[ro + c];
+                    gsum += res * (double)s[r][c];
+                    jsum += res * res;
+                }
             }
-            __syncthreads();
         }
-        if (t == 0) {
-            // dJ/dkappa = N tau <r, L u^{N-1}>,  L u = lap / h^2
-            a.grad[p] = (double)a.nsteps * (double)a.tau[p] * red[0] / (h * h);
-            a.J[p] = 0.5 * red[NT];
+        // wave sums by lane shuffles, then the 16 waves' sums in order
+        for (int w = 32; w > 0; w >>= 1) {
+            gsum += __shfl_down(gsum, w, 64);
+            jsum += __shfl_down(jsum, w, 64);
+        }
+        const int t = tx + kTX * ty;
+        if ((t & 63) == 0) {
+            red[0][t >> 6] = gsum;
+            red[1][t >> 6] = jsum;
         }
         __syncthreads();
+        if (t == 0) {
+            double g = 0.0, jj = 0.0;
+            for (int w = 0; w < kTX * kTY / 64; ++w) {
+                g += red[0][w];
+                jj += red[1][w];
+            }
+            // dJ/dkappa = N tau <r, L u^{N-1}>,  L u = lap / h^2
+            a.grad[p] = (double)a.nsteps * (double)a.tau[p] * g / (h * h);
+            a.J[p] = 0.5 * jj;
+        }
+        __syncthreads();
+    } else {
+        asm volatile("" : "+v"(vx), "+v"(vy));
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            float* __restrict__ row = u + (long)(oy + r) * n + ox;
+#pragma unroll
+            for (int c = 0; c < B; ++c)
+                if ((vy >> r) & (vx >> c) & 1u) row[c] = s[r][c];
+        }
     }
 }
 
-// Per-thread block shape of a problem edge n: BX = ceil(n/32), BY = ceil(n/16).
-__host__ __device__ inline int batch_bx(int n) { return (n + kTX - 1) / kTX; }
-__host__ __device__ inline int batch_by(int n) { return (n + kTY - 1) / kTY; }
-
-template <int BX>
-__device__ void dispatch_by(const BatchArgs& a, int p, int by)
-{
-    // BY is 2*BX - 1 or 2*BX for every n (a single row block when n <= 16)
-    if (by == 2 * BX) run_problem<BX, 2 * BX>(a, p);
-    else if constexpr (BX > 1) run_problem<BX, 2 * BX - 1>(a, p);
-    else run_problem<1, 1>(a, p);
-}
+// Per-thread block edge of a problem edge n: B = ceil(n/32).
+__host__ __device__ inline int batch_b(int n) { return (n + kTX - 1) / kTX; }
 
 __global__ __launch_bounds__(kTX* kTY) void batch2d_kernel(BatchArgs a)
 {
@@ -193,17 +198,15 @@ __global__ __launch_bounds__(kTX* kTY) void batch2d_kernel(BatchArgs a)
         const long idx = (long)round * G + ((round & 1) ? (G - 1 - w) : w);
         if (idx >= a.nprob) break;  // uniform over the workgroup
         const int p = a.order[idx];
-        const int n = a.n[p];
-        const int bx = batch_bx(n), by = batch_by(n);
-        switch (bx) {
-        case 1: dispatch_by<1>(a, p, by); break;
-        case 2: dispatch_by<2>(a, p, by); break;
-        case 3: dispatch_by<3>(a, p, by); break;
-        case 4: dispatch_by<4>(a, p, by); break;
-        case 5: dispatch_by<5>(a, p, by); break;
-        case 6: dispatch_by<6>(a, p, by); break;
-        case 7: dispatch_by<7>(a, p, by); break;
-        default: dispatch_by<8>(a, p, by); break;
+        switch (batch_b(a.n[p])) {
+        case 1: run_problem<1>(a, p); break;
+        case 2: run_problem<2>(a, p); break;
+        case 3: run_problem<3>(a, p); break;
+        case 4: run_problem<4>(a, p); break;
+        case 5: run_problem<5>(a, p); break;
+        case 6: run_problem<6>(a, p); break;
+        case 7: run_problem<7>(a, p); break;
+        default: run_problem<8>(a, p); break;
         }
     }
 }

@@ -22,7 +22,7 @@ def batch2d_ensemble(nprob, nsteps, seed=0, sizes=None, nmin=32, nmax=256):
     taken at fixed tau), u0 = sin(pi x) sin(pi y) + 0.01 seeded noise on
     inner nodes, target = u0 * exp(-2 pi^2 * 1.1 kappa_i * nsteps tau_i)
     (the fundamental mode decayed with a 10 % larger diffusivity).
-    order: descending per-thread block cost ceil(n/32) * ceil(n/16)."""
+    order: descending per-thread block edge ceil(n/32) (a B x B block)."""
     rng = np.random.default_rng(seed)
     n = np.asarray(sizes if sizes is not None else rng.integers(nmin, nmax + 1, size=nprob), dtype=np.int32)
     nprob = len(n)
@@ -45,7 +45,7 @@ def batch2d_ensemble(nprob, nsteps, seed=0, sizes=None, nmin=32, nmax=256):
         dec = np.exp(-2 * np.pi ** 2 * 1.1 * float(kappa[i]) * nsteps * float(tau[i]))
         u0[off[i]:off[i] + m * m] = u.ravel()
         target[off[i]:off[i] + m * m] = (u * dec).ravel()
-    cost = ((n + 31) // 32) * ((n + 15) // 16)
+    cost = (n + 31) // 32
     order = np.argsort(-cost, kind="stable").astype(np.int32)
     return {"n": n, "off": off, "kappa": kappa, "tau": tau, "u0": u0, "target": target, "order": order}
 
