@@ -163,6 +163,36 @@ int scafes_autotune_heat7var(scafes_ctx* ctx, const scafes_geom* g, int hw, int 
 int scafes_heat7var_init(scafes_ctx* ctx, const scafes_geom* g, int hw, int dtype, void* u,
                          void* kappa, uint64_t seed, double amp);
 
+/* ---- known fields on the device ---------------------------------------
+ * Replaces evalKnownDfs (Problem.hpp:3433-3485) for the TimeLinSpaceLin
+ * system problems, whose evalInner/evalBorder hooks are closed forms of
+ * (x, t): heat tests/system/HeatEqnFDM/Dirichlet/TimeLinSpaceLin/
+ * TimeLinSpaceLin.hpp:75-123, Pennes tests/system/PennesBioheatEqnFDM/
+ * Dirichlet/TimeLinSpaceLin/TimeLinSpaceLin.hpp:75-124, and the flux-border
+ * variants .../{Neumann,Cauchy,Radiation}/TimeLinSpaceLin/TimeLinSpaceLin.hpp.
+ * With s = 1 + sum_p x_p (x_p = x0_p + h_p i_p, Grid.hpp:719-733):
+ *   inner:  F = s*rho_c [+ rhob_cb_w*(s*(1+t)) for the Pennes form], G = 0
+ *   border, FIELD (Dirichlet): F = 0, G = s*(1+t)
+ *   border, flux modes: F as inner; G = s*(1+t) on nodes with an index 0,
+ *     else NEUMANN (-lambda)(1+t); CAUCHY s*(alpha(1+t)) + lambda(1+t);
+ *     RADIATION (s(1+t))^4*(sigma*epsilon) + lambda(1+t)
+ *   Y (exact solution) = s*(1+t) everywhere.
+ * F, G: stencil-0 FP64 device arrays (x fastest); Y: FP64 memAll with hw_y
+ * ghost layers (0 for a stencil-0 array; the unknown field's width when Y
+ * is its reference solution for compErrLinf); each may be NULL, ghosts are
+ * not written.  Bit-identical to evaluating the hooks on the host. */
+enum { SCAFES_KNOWN_HEAT_TIMELIN = 1, SCAFES_KNOWN_PENNES_TIMELIN = 2 };
+typedef struct {
+    int form;          /* SCAFES_KNOWN_HEAT_TIMELIN or SCAFES_KNOWN_PENNES_TIMELIN */
+    int border_mode;   /* SCAFES_BORDER_FIELD / _NEUMANN / _CAUCHY / _RADIATION */
+    double x0[3];      /* coordNodeFirst */
+    double rho_c;      /* RHO*C */
+    double rhob_cb_w;  /* RHO_BLOOD*C_BLOOD*W (Pennes form) */
+    double lambda, alpha, epsilon, sigma;
+} scafes_timelin_params;
+int scafes_known_timelin_f64(scafes_ctx* ctx, const scafes_geom* g, const scafes_timelin_params* p,
+                             double t, double* F, double* G, double* Y, int hw_y);
+
 /* ---- reductions (DataField::compErrLinf DataField.hpp:1439-1530 and
  *      DataField::checkConv DataField.hpp:1532-1636) ---------------------
  * a has hw_a ghost layers, b has hw_b (0 for a stencil-0 known field).

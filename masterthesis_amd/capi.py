@@ -35,6 +35,15 @@ class Heat7Params(C.Structure):
                 ("lam", C.c_double), ("alpha", C.c_double), ("epsilon", C.c_double), ("sigma", C.c_double)]
 
 
+class TimelinParams(C.Structure):
+    _fields_ = [("form", C.c_int), ("border_mode", C.c_int), ("x0", C.c_double * 3), ("rho_c", C.c_double),
+                ("rhob_cb_w", C.c_double), ("lam", C.c_double), ("alpha", C.c_double), ("epsilon", C.c_double),
+                ("sigma", C.c_double)]
+
+
+KNOWN_HEAT_TIMELIN, KNOWN_PENNES_TIMELIN = 1, 2
+
+
 class PennesParams(C.Structure):
     _fields_ = [("tau", C.c_double), ("cdiff", C.c_double), ("cperf", C.c_double),
                 ("csrc", C.c_double), ("t_a", C.c_double)]
@@ -86,6 +95,7 @@ def load():
                                          C.c_int, ip, C.POINTER(C.c_float)])
     _sig(L, "scafes_heat7var_init", [vp, gp, C.c_int, C.c_int, vp, vp, C.c_uint64, C.c_double])
     _sig(L, "scafes_reduce", [vp, gp, C.c_int, vp, C.c_int, vp, C.c_int, C.c_int, C.c_int, dp])
+    _sig(L, "scafes_known_timelin_f64", [vp, gp, C.POINTER(TimelinParams), C.c_double, vp, vp, vp, C.c_int])
     _sig(L, "scafes_pennes_step", [vp, gp, C.c_int, C.c_int, vp, vp, vp, vp, C.POINTER(PennesParams)])
     _sig(L, "scafes_pennes_adjoint_step", [vp, gp, C.c_int, C.c_int, vp, vp, vp, vp, vp,
                                            C.POINTER(PennesParams)])

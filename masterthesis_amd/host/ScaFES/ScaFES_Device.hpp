@@ -50,6 +50,17 @@ enum class DeviceBorder {
     RADIATION = SCAFES_BORDER_RADIATION
 };
 
+// Known fields computed on the device instead of by the host hooks: the
+// TimeLinSpaceLin system problems' evalInner/evalBorder are closed forms of
+// (x, t), so from step 1 on the engine writes F_old, G (and the exact
+// solution for compErr) with scafes_known_timelin_f64 and runs no host loop
+// and no upload.  The host copies of the known fields keep their n = 0 values.
+enum class DeviceKnownGen {
+    NONE = 0,
+    HEAT_TIMELIN = SCAFES_KNOWN_HEAT_TIMELIN,
+    PENNES_TIMELIN = SCAFES_KNOWN_PENNES_TIMELIN
+};
+
 // Everything the engine needs to run a problem's update phase on the GPU.
 // Known-field indices refer to the problem's data-field list (the order of
 // the constructor's nameDatafield vector).
@@ -78,6 +89,10 @@ struct DeviceUpdate {
     // Re-evaluate and upload the known fields every step (evalKnownDfs each
     // time iterate, Problem.hpp:3433-3485); false when they do not depend on t.
     bool knownTimeDependent = true;
+    // Device evaluation of the known fields (HEAT7 only); F = s*rhoC
+    // [+ rhobCbW*s*(1+t)] with s = 1 + sum x, see scafes_known_timelin_f64.
+    DeviceKnownGen knownGen = DeviceKnownGen::NONE;
+    double rhoC = 1.0, rhobCbW = 0.0;
 };
 
 template <class OWNPRBLM>

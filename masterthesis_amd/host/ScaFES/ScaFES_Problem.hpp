@@ -261,6 +261,11 @@ private:
         if (!evalDf()) return;
         if (0 < n && !mDev.knownTimeDependent && !anyComputeError()) return;
         Timer t;
+        if (0 < n && mDev.knownGen != DeviceKnownGen::NONE) {
+            evalKnownOnDevice(n);
+            mWallClockTimes["evalKnownDf"] += t.elapsed();
+            return;
+        }
         const GridSub<DIM>& part = mGG.partition(0);
         OWNPRBLM* self = static_cast<OWNPRBLM*>(this);
         for (typename GridSub<DIM>::iterator it = part.begin(); it != part.end(); ++it) {
@@ -269,6 +274,36 @@ private:
         }
         if (0 < n) uploadKnown(n);
         mWallClockTimes["evalKnownDf"] += t.elapsed();
+    }
+    // The kernel reads F from the old known buffer and G from the new one
+    // (flux borders also G_old): the values the hooks produced at steps n-1
+    // and n, i.e. at time(n-1) and time(n).  The exact solution for compErr
+    // is the one at time(n).
+    void evalKnownOnDevice(const int& n)
+    {
+        scafes_timelin_params p;
+        p.form = (int)mDev.knownGen;
+        p.border_mode = (int)mDev.border;
+        const Ntuple<double, DIM> x0 = coordinates(ScaFES_IntNtuple(0));
+        for (int q = 0; q < 3; ++q) p.x0[q] = q < (int)DIM ? x0.elem(q) : 0.0;
+        p.rho_c = mDev.rhoC;
+        p.rhob_cb_w = mDev.rhobCbW;
+        p.lambda = mDev.lambda;
+        p.alpha = mDev.alpha;
+        p.epsilon = mDev.epsilon;
+        p.sigma = mDev.sigma;
+        const scafes_geom g = geom();
+        scafes_ctx* c = mCtx->get();
+        double* exact = 0;
+        const int u = unknownSlot(mDev.unknown);
+        if (u >= 0 && (std::size_t)u < mRef.size()) exact = (double*)mRef[u].get();
+        deviceCheck(scafes_known_timelin_f64(c, &g, &p, time(n - 1), (double*)mKnownA.get(),
+                                             fluxBorder() ? (double*)mKnownC.get() : 0, 0, 0),
+                    "known fields (old)");
+        // the reference solution has the unknown field's memAll layout
+        deviceCheck(scafes_known_timelin_f64(c, &g, &p, time(n), 0, (double*)mKnownB.get(), exact,
+                                             mStencilWidth[mDev.unknown]),
+                    "known fields (new)");
     }
     bool anyComputeError() const
     {
@@ -326,6 +361,13 @@ private:
             if (mDev.sourceKnown >= 0) known(mDev.sourceKnown, "sourceKnown");
             if (mDev.border == DeviceBorder::KNOWN_FIELD || fluxBorder()) known(mDev.borderKnown, "borderKnown");
             if (fluxBorder() && mDev.lambda == 0.0) throw std::runtime_error("DeviceUpdate: flux borders need lambda != 0");
+            if (mDev.knownGen != DeviceKnownGen::NONE) {
+                if (mDev.sourceKnown < 0 || !(mDev.border == DeviceBorder::KNOWN_FIELD || fluxBorder()))
+                    throw std::runtime_error("DeviceUpdate: device known fields need a source F and a G border field");
+                for (std::size_t ii = 0; ii < mNames.size(); ++ii)
+                    if (mIsKnownDf[ii] && mStencilWidth[ii] != 0)
+                        throw std::runtime_error("DeviceUpdate: device known fields have stencil width 0");
+            }
             break;
         case DeviceKernel::HEAT7VAR:
             unknown(mDev.unknown);
@@ -358,6 +400,8 @@ private:
         default:
             throw std::runtime_error("DeviceUpdate: no kernel selected");
         }
+        if (mDev.knownGen != DeviceKnownGen::NONE && mDev.kernel != DeviceKernel::HEAT7)
+            throw std::runtime_error("DeviceUpdate: device known fields are a HEAT7 option");
     }
 
     scafes_geom geom() const

@@ -8,7 +8,9 @@
 // Build variants: -DSCAFES_DIM=1|2|3, -DEXAMPLE_PENNES_FORM for the Pennes form,
 // -DEXAMPLE_BORDER=1|2|3 for the Neumann / Cauchy / radiation border classes
 // (tests/system/*/{Neumann,Cauchy,Radiation}/TimeLinSpaceLin; default 0 =
-// Dirichlet).
+// Dirichlet), -DEXAMPLE_DEVICE_KNOWN to have the engine evaluate F, G and
+// the exact solution on the device after step 0 (the hooks below then run
+// only at n = 0).
 #include <iostream>
 
 #include "ScaFES.hpp"
@@ -311,6 +313,15 @@ struct DeviceTraits<Equation<CT, DIM, Class>> {
         d.sigma = p.SIGMA;
         d.borderKnown = 1;
         d.knownTimeDependent = true;
+#ifdef EXAMPLE_DEVICE_KNOWN
+#ifdef EXAMPLE_PENNES_FORM
+        d.knownGen = ScaFES::DeviceKnownGen::PENNES_TIMELIN;
+#else
+        d.knownGen = ScaFES::DeviceKnownGen::HEAT_TIMELIN;
+#endif
+        d.rhoC = p.RHO * p.C;
+        d.rhobCbW = p.RHO_BLOOD * p.C_BLOOD * p.W;
+#endif
     }
 };
 }  // namespace ScaFES

@@ -265,6 +265,12 @@ static void eval_known_timelin(const oracle_grid* g, int known, int border, doub
     }
 }
 
+void oracle_known_timelin(const oracle_grid* g, int known, int border, double t,
+                          const oracle_heat7_coef* c, double* F, double* G, double* Y)
+{
+    eval_known_timelin(g, known, border, t, c, F, G, Y);
+}
+
 /* compErrLinf over all partition nodes (DataField.hpp:1439-1530); b is a
  * stencil-0 field when bnodes != 0. */
 static double linf_vs_nodes(const oracle_grid* g, const double* a, const double* b)

@@ -115,6 +115,12 @@ void oracle_heat7_step(const oracle_grid* g, const double* u_old, double* u_new,
 double oracle_heat7_run(const oracle_grid* g, double* u, int nsteps, double t0,
                         const oracle_heat7_coef* c, int known, int border, double* linf);
 
+/* The TimeLinSpaceLin known fields at time t (evalInner/evalBorder of the
+ * system problems, restated as used by oracle_heat7_run): F, G and the exact
+ * solution Y, stencil-0 arrays of oracle_nnodes(g) doubles. */
+void oracle_known_timelin(const oracle_grid* g, int known, int border, double t,
+                          const oracle_heat7_coef* c, double* F, double* G, double* Y);
+
 /* Variable-coefficient conservative 7-point heat: one step, FP64.
  * kappa has the same layout (and stencil width) as u. */
 void oracle_heat7var_step(const oracle_grid* g, const double* u_old, double* u_new,

@@ -84,6 +84,7 @@ def lib():
         L.oracle_heat7_step.argtypes = [gp, dp, dp, dp, dp, dp, C.c_int, C.c_double, C.POINTER(Heat7Coef)]
         L.oracle_heat7_run.restype = C.c_double
         L.oracle_heat7_run.argtypes = [gp, dp, C.c_int, C.c_double, C.POINTER(Heat7Coef), C.c_int, C.c_int, dp]
+        L.oracle_known_timelin.argtypes = [gp, C.c_int, C.c_int, C.c_double, C.POINTER(Heat7Coef), dp, dp, dp]
         L.oracle_heat7var_step.argtypes = [gp, dp, dp, dp, C.c_double, C.c_int, C.c_double]
         L.oracle_heat7var_run.argtypes = [gp, dp, dp, C.c_double, C.c_int, C.c_int, C.c_double, dp]
         L.oracle_pennes_step.argtypes = [gp, dp, dp, dp, dp, C.POINTER(PennesCoef)]
@@ -133,6 +134,14 @@ def heat7_run(g, u, nsteps, t0, coef, known, border=BORDER_FIELD):
     linf = np.zeros(nsteps + 1)
     t = lib().oracle_heat7_run(C.byref(g), ptr(u), nsteps, t0, C.byref(coef), known, border, ptr(linf))
     return u, linf, t
+
+
+def known_timelin(g, known, border, t, coef):
+    """F, G, Y (stencil-0, x fastest) of the TimeLinSpaceLin problems at time t."""
+    nn = int(np.prod(g.n))
+    F, G, Y = np.empty(nn), np.empty(nn), np.empty(nn)
+    lib().oracle_known_timelin(C.byref(g), known, border, t, C.byref(coef), ptr(F), ptr(G), ptr(Y))
+    return F, G, Y
 
 
 def heat7_step(g, uo, f_old=None, g_new=None, border=BORDER_VALUE, border_value=0.0, coef=None, g_old=None):

@@ -80,6 +80,38 @@ def test_flux_border_timelin_through_problem(tmp_path, hip, prog, n, pennes, bc)
     np.testing.assert_allclose(linf, rlinf, rtol=0, atol=1e-15)
 
 
+@pytest.mark.parametrize("prog,n,pennes,bc", [("heat_timelin_1d", (17,), False, O.BORDER_FIELD),
+                                              ("heat_timelin_3d", (10, 9, 8), False, O.BORDER_FIELD),
+                                              ("pennes_timelin_3d", (9, 10, 11), True, O.BORDER_FIELD),
+                                              ("heat_neumann_2d", (13, 11), False, O.BORDER_NEUMANN),
+                                              ("heat_cauchy_3d", (9, 11, 10), False, O.BORDER_CAUCHY),
+                                              ("pennes_radiation_3d", (9, 10, 8), True, O.BORDER_RADIATION)])
+def test_device_known_fields_through_problem(tmp_path, hip, prog, n, pennes, bc):
+    """The same problem classes with DeviceKnownGen set: after step 0 the
+    engine evaluates F, G and the exact solution on the device instead of
+    running the hooks and uploading.  Final field and every "Local Linf"
+    equal the host-evaluated program's output and the oracle's, bit for bit."""
+    nsteps = 30
+    dim = len(n)
+    g = O.make_grid(n)
+    h2 = sum(1.0 / O.lib().oracle_gridsize(g, p) ** 2 for p in range(dim))
+    hmin = min(O.lib().oracle_gridsize(g, p) for p in range(dim))
+    tau = 0.4 / (2.0 * h2 + 1.0 + (8.0 * (1.0 + dim) ** 3 / hmin if bc == O.BORDER_RADIATION else 0.0))
+    args = [f"--dim={dim}", "--nNodes=" + "x".join(map(str, n)), f"--nTimesteps={nsteps}", f"--tau={tau!r}"]
+    host_dump, dev_dump = str(tmp_path / "host.bin"), str(tmp_path / "dev.bin")
+    out_host = run(prog, args, host_dump)
+    out_dev = run("dk_" + prog, args, dev_dump)
+    pat = r"Local Linf error\(U, USolDom\): (\S+)"
+    assert re.findall(pat, out_dev) == re.findall(pat, out_host)
+    assert len(re.findall(pat, out_dev)) == nsteps + 1
+    got = np.fromfile(dev_dump, dtype=np.float64)
+    np.testing.assert_array_equal(got, np.fromfile(host_dump, dtype=np.float64))
+    coef = O.heat7_coef(tau, perf_c=1.0, perf_w=1.0, rhob_cb_w=1.0, has_perf=pennes, has_source=True)
+    ref, _, _ = O.heat7_run(g, np.zeros(O.memall_shape(g)), nsteps, 0.0, coef,
+                            O.KNOWN_PENNES_TIMELIN if pennes else O.KNOWN_HEAT_TIMELIN, bc)
+    np.testing.assert_array_equal(got, O.nodes_view(g, ref).ravel())
+
+
 def _pattern(n, a, b, c, m, d):
     """((a i + b j + c k) mod m) / d on an (nz, ny, nx) node block."""
     k, j, i = np.meshgrid(*(np.arange(x) for x in reversed(n)), indexing="ij")

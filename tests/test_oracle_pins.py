@@ -46,6 +46,43 @@ def test_heat_timelin_exact(dims):
     np.testing.assert_allclose(got, y, rtol=1e-13, atol=0)
 
 
+@pytest.mark.parametrize("form", [O.KNOWN_HEAT_TIMELIN, O.KNOWN_PENNES_TIMELIN])
+@pytest.mark.parametrize("border", [O.BORDER_FIELD, O.BORDER_NEUMANN, O.BORDER_CAUCHY, O.BORDER_RADIATION])
+def test_known_timelin_fields_closed_form(form, border):
+    """The known-field evaluator on its own, on a shifted 3-D domain with two
+    border layers: Y = (1+t)(1+sum x) everywhere, F = (1+sum x) [+ w (1+t)(1+sum x)]
+    (zero on Dirichlet border nodes), G = Y on Dirichlet border nodes and on
+    flux-border nodes with an index 0, the flux datum on the far faces (the
+    value the one-sided border update needs for y to stay exact: for
+    Neumann -lambda dy/dn = -lambda (1+t))."""
+    dims, x0, x1, nl, t = (7, 6, 5), (0.5, -1.0, 0.0), (2.0, 1.5, 0.8), 2, 0.75
+    g = O.make_grid(dims, hw=0, x0=x0, x1=x1, nlayers=nl)
+    lam, alpha, eps, sig, w = 1.3, 0.6, 0.9, 0.4, 0.7
+    coef = O.heat7_coef(1e-3, rhob_cb_w=w, lam=lam, alpha=alpha, epsilon=eps, sigma=sig)
+    F, G, Y = (a.reshape(tuple(reversed(dims))).T for a in O.known_timelin(g, form, border, t, coef))
+    ax = [np.linspace(x0[p], x1[p], dims[p]) for p in range(3)]
+    X = np.meshgrid(*ax, indexing="ij")
+    s = 1.0 + X[0] + X[1] + X[2]
+    y = s * (1.0 + t)
+    idx = np.meshgrid(*(np.arange(n) for n in dims), indexing="ij")
+    bord = np.zeros(dims, bool)
+    for p in range(3):
+        bord |= (idx[p] < nl) | (idx[p] > dims[p] - 1 - nl)
+    near = (idx[0] == 0) | (idx[1] == 0) | (idx[2] == 0)
+    f = s + (w * y if form == O.KNOWN_PENNES_TIMELIN else 0.0)
+    np.testing.assert_allclose(Y, y, rtol=1e-14)
+    if border == O.BORDER_FIELD:
+        np.testing.assert_allclose(F, np.where(bord, 0.0, f), rtol=1e-14)
+        np.testing.assert_allclose(G, np.where(bord, y, 0.0), rtol=1e-14)
+        return
+    np.testing.assert_allclose(F, f, rtol=1e-14)
+    datum = {O.BORDER_NEUMANN: -lam * (1.0 + t) + 0.0 * s,
+             O.BORDER_CAUCHY: alpha * y + lam * (1.0 + t),
+             O.BORDER_RADIATION: sig * eps * y ** 4 + lam * (1.0 + t)}[border]
+    want = np.where(bord, np.where(near, y, datum), 0.0)
+    np.testing.assert_allclose(G, want, rtol=1e-14)
+
+
 def test_heat_timelin_detects_missing_source():
     """A run without the source term drifts away from the exact solution."""
     g = O.make_grid((8, 8, 8))
