@@ -181,6 +181,45 @@ double oracle_linf_nodes(const oracle_grid* g, const double* a, const double* b)
 void oracle_set_threads(int n);
 int oracle_get_threads(void);
 
+/*---------------------------------------------------------------------------*/
+/* EMFDTD example: Yee leapfrog with a complete CPML and conductive media
+ * (examples/EMFDTD/EMFDTD.hpp, stencils/CompleteCPMLCurlStencil.hpp,
+ * CPMLParams.hpp).  Fields are n^3 node blocks, x fastest. */
+enum {
+    OEM_EX = 0, OEM_EY, OEM_EZ, OEM_HX, OEM_HY, OEM_HZ, OEM_DX, OEM_DY, OEM_DZ, OEM_IX, OEM_IY, OEM_IZ,
+    OEM_PSI_EXDY, OEM_PSI_EXDZ, OEM_PSI_EYDX, OEM_PSI_EYDZ, OEM_PSI_EZDX, OEM_PSI_EZDY,
+    OEM_PSI_HXDY, OEM_PSI_HXDZ, OEM_PSI_HYDX, OEM_PSI_HYDZ, OEM_PSI_HZDX, OEM_PSI_HZDY,
+    OEM_GA, OEM_GB, OEM_NFIELDS
+};
+typedef struct {
+    int n;                       /* nodes per axis (cubic grid) */
+    int d;                       /* PML depth: tables have d+1 entries */
+    const double *b, *c, *ook;   /* CPML tables, index 0 = outermost layer */
+    double cfln;                 /* Courant factor (0.5 in the reference) */
+    int src[3], sink[3];         /* node indices of the hard Ez source and the sink */
+} oracle_emfdtd;
+
+/* CPMLParams (CPMLParams.hpp:23-49), d >= 2. */
+void oracle_cpml_tables(int d, double m_a, double m, double k_max, double s_max, double a_max, double dt,
+                        double* b, double* c, double* ook);
+/* dt = cfln * ds / C_0 with ds = 1 (EMFDTD.hpp:191-193). */
+double oracle_emfdtd_dt(double cfln);
+/* ga = 1/(1 + sigma dt/eps0), gb = sigma dt/eps0 per node (EMFDTD.hpp:351-352). */
+void oracle_emfdtd_ga_gb(long count, const double* sigma, double dt, double* ga, double* gb);
+/* ModulatedGaussianSource (sources/ModulatedGaussianSource.hpp:25-34). */
+double oracle_mod_gauss(double t, double dt, double f_mod, double t0, double spread);
+/* IndicatorEllipsoid at point (x,y,z) (geometry/IndicatorEllipsoid.hpp:48-68). */
+double oracle_ellipsoid(double x, double y, double z, double cx, double cy, double cz, double a, double b,
+                        double c, double inside);
+/* One leapfrog step in place: H phase, then D/E/I phase with the hard source
+ * value at src; *sink (nullable) receives the new Ez at the sink node.
+ * f: OEM_NFIELDS arrays (GA/GB read only). */
+void oracle_emfdtd_step(const oracle_emfdtd* p, double* const* f, double src_value, double* sink);
+/* nsteps steps with the source evaluated at the old field's time
+ * t_start + (s-1) (tau = 1); sink: nsteps values (nullable). */
+void oracle_emfdtd_run(const oracle_emfdtd* p, double* const* f, int nsteps, double t_start, double f_mod,
+                       double t0, double spread, double* sink);
+
 #ifdef __cplusplus
 }
 #endif

@@ -84,6 +84,17 @@ def lib():
         L.oracle_heat7_step.argtypes = [gp, dp, dp, dp, dp, dp, C.c_int, C.c_double, C.POINTER(Heat7Coef)]
         L.oracle_heat7_run.restype = C.c_double
         L.oracle_heat7_run.argtypes = [gp, dp, C.c_int, C.c_double, C.POINTER(Heat7Coef), C.c_int, C.c_int, dp]
+        L.oracle_cpml_tables.argtypes = [C.c_int] + [C.c_double] * 6 + [dp, dp, dp]
+        L.oracle_emfdtd_dt.restype = C.c_double
+        L.oracle_emfdtd_dt.argtypes = [C.c_double]
+        L.oracle_emfdtd_ga_gb.argtypes = [C.c_long, dp, C.c_double, dp, dp]
+        L.oracle_mod_gauss.restype = C.c_double
+        L.oracle_mod_gauss.argtypes = [C.c_double] * 5
+        L.oracle_ellipsoid.restype = C.c_double
+        L.oracle_ellipsoid.argtypes = [C.c_double] * 10
+        L.oracle_emfdtd_step.argtypes = [C.POINTER(EmfdtdParams), C.POINTER(dp), C.c_double, dp]
+        L.oracle_emfdtd_run.argtypes = [C.POINTER(EmfdtdParams), C.POINTER(dp), C.c_int, C.c_double,
+                                        C.c_double, C.c_double, C.c_double, dp]
         L.oracle_known_timelin.argtypes = [gp, C.c_int, C.c_int, C.c_double, C.POINTER(Heat7Coef), dp, dp, dp]
         L.oracle_heat7var_step.argtypes = [gp, dp, dp, dp, C.c_double, C.c_int, C.c_double]
         L.oracle_heat7var_run.argtypes = [gp, dp, dp, C.c_double, C.c_int, C.c_int, C.c_double, dp]
@@ -106,6 +117,12 @@ def lib():
         L.oracle_get_threads.restype = C.c_int
         _lib = L
     return _lib
+
+
+class EmfdtdParams(C.Structure):
+    _fields_ = [("n", C.c_int), ("d", C.c_int), ("b", C.POINTER(C.c_double)), ("c", C.POINTER(C.c_double)),
+                ("ook", C.POINTER(C.c_double)), ("cfln", C.c_double), ("src", C.c_int * 3),
+                ("sink", C.c_int * 3)]
 
 
 def ptr(a):
@@ -223,3 +240,89 @@ def heat2d_grad(n, kappa, tau, nsteps, u0, target):
                                  ptr(np.ascontiguousarray(u0, dtype=np.float64)),
                                  ptr(np.ascontiguousarray(target, dtype=np.float64)), C.byref(d))
     return J, d.value
+
+
+# ---------------------------------------------------------------------------
+# EMFDTD example (Yee leapfrog + CPML + conductive media)
+
+EM_FIELDS = ["Ex", "Ey", "Ez", "Hx", "Hy", "Hz", "Dx", "Dy", "Dz", "Ix", "Iy", "Iz",
+             "Psi_ExDy", "Psi_ExDz", "Psi_EyDx", "Psi_EyDz", "Psi_EzDx", "Psi_EzDy",
+             "Psi_HxDy", "Psi_HxDz", "Psi_HyDx", "Psi_HyDz", "Psi_HzDx", "Psi_HzDy", "ga", "gb"]
+EM = {name: i for i, name in enumerate(EM_FIELDS)}
+EM_STATE = 24  # fields updated by the step (ga, gb are read only)
+
+
+def cpml_tables(d, m_a=1.0, m=3.0, k_max=1.0, s_max=0.01222, a_max=1e-3, dt=None):
+    """CPMLParams tables (b, c, 1/kappa), d+1 entries, index 0 = outermost."""
+    if dt is None:
+        dt = emfdtd_dt(0.5)
+    b, c, ook = np.empty(d + 1), np.empty(d + 1), np.empty(d + 1)
+    lib().oracle_cpml_tables(d, m_a, m, k_max, s_max, a_max, dt, ptr(b), ptr(c), ptr(ook))
+    return b, c, ook
+
+
+def emfdtd_dt(cfln=0.5):
+    return lib().oracle_emfdtd_dt(cfln)
+
+
+def ga_gb(sigma, dt):
+    sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+    ga, gb = np.empty_like(sigma), np.empty_like(sigma)
+    lib().oracle_emfdtd_ga_gb(sigma.size, ptr(sigma), dt, ptr(ga), ptr(gb))
+    return ga, gb
+
+
+def mod_gauss(t, dt, f_mod=0.0, t0=40.0, spread=8.0):
+    return lib().oracle_mod_gauss(t, dt, f_mod, t0, spread)
+
+
+def ellipsoid_sigma(n, center, radii, inside=3.546e7):
+    """SIGMA of EMFDTDng::initInner (EMFDTD.hpp:258-276): the indicator at the
+    node indices, (n,n,n) array indexed [k, j, i]."""
+    out = np.empty((n, n, n))
+    L = lib()
+    for k in range(n):
+        for j in range(n):
+            for i in range(n):
+                out[k, j, i] = L.oracle_ellipsoid(float(i), float(j), float(k), *map(float, center),
+                                                  *map(float, radii), inside)
+    return out
+
+
+class Emfdtd:
+    """State of the EMFDTD example on an n^3 grid: f[26, n, n, n] (x fastest)."""
+
+    def __init__(self, n, d=5, cfln=0.5, src=(10, 10, 10), sink=(10, 10, 14), sigma=None, tables=None, **pml):
+        self.n, self.d, self.cfln = n, d, cfln
+        self.dt = emfdtd_dt(cfln)
+        b, c, ook = tables if tables is not None else cpml_tables(d, dt=self.dt, **pml)
+        self.b, self.c, self.ook = (np.ascontiguousarray(x, dtype=np.float64) for x in (b, c, ook))
+        self.f = np.zeros((len(EM_FIELDS), n, n, n))
+        sig = np.zeros((n, n, n)) if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64)
+        ga, gb = ga_gb(sig.ravel(), self.dt)
+        self.f[EM["ga"]] = ga.reshape(n, n, n)
+        self.f[EM["gb"]] = gb.reshape(n, n, n)
+        self.p = EmfdtdParams()
+        self.p.n, self.p.d, self.p.cfln = n, d, cfln
+        self.p.b, self.p.c, self.p.ook = ptr(self.b), ptr(self.c), ptr(self.ook)
+        for q in range(3):
+            self.p.src[q], self.p.sink[q] = src[q], sink[q]
+
+    def _fp(self):
+        arr = (C.POINTER(C.c_double) * len(EM_FIELDS))()
+        for i in range(len(EM_FIELDS)):
+            arr[i] = self.f[i].ctypes.data_as(C.POINTER(C.c_double))
+        return arr
+
+    def step(self, src_value):
+        out = C.c_double(0.0)
+        lib().oracle_emfdtd_step(C.byref(self.p), self._fp(), src_value, C.byref(out))
+        return out.value
+
+    def run(self, nsteps, t_start=0.0, f_mod=0.0, t0=40.0, spread=8.0):
+        sink = np.zeros(max(nsteps, 1))
+        lib().oracle_emfdtd_run(C.byref(self.p), self._fp(), nsteps, t_start, f_mod, t0, spread, ptr(sink))
+        return sink[:nsteps]
+
+    def __getitem__(self, name):
+        return self.f[EM[name]]
