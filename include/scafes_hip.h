@@ -265,6 +265,52 @@ int scafes_yee_run(scafes_ctx* ctx, int ncell, int dtype, double s, void* f, voi
  * cell units (Ez(i,j,k) sits at (i, j, k+1/2)). */
 int scafes_yee_init_pulse(scafes_ctx* ctx, int ncell, int dtype, void* f, double sigma);
 
+/* ---- EMFDTD example: Yee leapfrog + complete CPML + conductive media -----
+ * Replaces the update phases of examples/EMFDTD/EMFDTD.hpp (EMFDTDng:
+ * updateInner = H from E, :295-313; updateInner2 = D from H, E = ga (D - I),
+ * I += gb E, hard Ez source, Ez sink, :330-386) with the CPML curl stencil
+ * examples/EMFDTD/stencils/CompleteCPMLCurlStencil.hpp:38-80.  FP64, in the
+ * reference's operation order (bit-identical to its CPU path).
+ * State: SCAFES_EM_NFIELDS arrays of n^3 doubles (x fastest, no ghosts) in
+ * the order below, one allocation; ga/gb are per-node coefficients from
+ * sigma (scafes_emfdtd_ga_gb) and are read only.  The reference requires a
+ * cubic grid and tau = 1 (time counts steps). */
+#define SCAFES_EMFDTD_MAX_D 64
+enum {
+    SCAFES_EM_EX = 0, SCAFES_EM_EY, SCAFES_EM_EZ, SCAFES_EM_HX, SCAFES_EM_HY, SCAFES_EM_HZ,
+    SCAFES_EM_DX, SCAFES_EM_DY, SCAFES_EM_DZ, SCAFES_EM_IX, SCAFES_EM_IY, SCAFES_EM_IZ,
+    SCAFES_EM_PSI_EXDY, SCAFES_EM_PSI_EXDZ, SCAFES_EM_PSI_EYDX, SCAFES_EM_PSI_EYDZ,
+    SCAFES_EM_PSI_EZDX, SCAFES_EM_PSI_EZDY, SCAFES_EM_PSI_HXDY, SCAFES_EM_PSI_HXDZ,
+    SCAFES_EM_PSI_HYDX, SCAFES_EM_PSI_HYDZ, SCAFES_EM_PSI_HZDX, SCAFES_EM_PSI_HZDY,
+    SCAFES_EM_GA, SCAFES_EM_GB, SCAFES_EM_NFIELDS
+};
+typedef struct {
+    int n;          /* nodes per axis */
+    int d;          /* PML layers (CPMLParams D), 2..SCAFES_EMFDTD_MAX_D */
+    double cfln;    /* Courant factor of the curl updates (EMFDTD.hpp:191: 0.5) */
+    double dt;      /* seconds per step, scafes_emfdtd_dt(cfln) */
+    double b[SCAFES_EMFDTD_MAX_D + 1], c[SCAFES_EMFDTD_MAX_D + 1], ook[SCAFES_EMFDTD_MAX_D + 1];
+    int src[3], sink[3];         /* node indices (i, j, k) of the source and the sink */
+    double f_mod, t0, spread;    /* ModulatedGaussianSource parameters */
+} scafes_emfdtd_params;
+/* CPMLParams (examples/EMFDTD/CPMLParams.hpp:23-49): d+1 entries, index 0 =
+ * outermost layer, index d = interior (b = c = 0, 1/kappa = 1).  Host only. */
+int scafes_emfdtd_tables(int d, double m_a, double m, double k_max, double s_max, double a_max,
+                         double dt, double* b, double* c, double* ook);
+/* dt = cfln * ds / C_0, ds = 1 (EMFDTD.hpp:191-193). */
+double scafes_emfdtd_dt(double cfln);
+/* ga = 1/(1 + sigma dt/eps0), gb = sigma dt/eps0 on host arrays, evaluated
+ * like EMFDTD.hpp:351-352 (long double, EPSILON_0 being a long double). */
+int scafes_emfdtd_ga_gb(size_t count, const double* sigma, double dt, double* ga, double* gb);
+/* ModulatedGaussianSource (sources/ModulatedGaussianSource.hpp:25-34). */
+double scafes_emfdtd_source(double t, double dt, double f_mod, double t0, double spread);
+/* nsteps leapfrog steps in place.  Step s sets Ez at the source node to the
+ * source at the old field's time t_start + (s - 1) (EMFDTD.hpp:370-371) and,
+ * when sink != NULL (device, nsteps doubles), stores the new Ez at the sink
+ * node in sink[s - 1] (EMFDTD.hpp:376-380). */
+int scafes_emfdtd_run(scafes_ctx* ctx, const scafes_emfdtd_params* p, double* state, int nsteps,
+                      double t_start, double* sink);
+
 /* ---- batched ensemble of 2-D heat problems -----------------------------
  * The reference runs one problem per process; this runs nprob independent
  * n_i x n_i problems (unit square, h_i = 1/(n_i-1), zero Dirichlet border,

@@ -43,6 +43,16 @@ class TimelinParams(C.Structure):
 
 KNOWN_HEAT_TIMELIN, KNOWN_PENNES_TIMELIN = 1, 2
 
+EMFDTD_MAX_D = 64
+EM_NFIELDS = 26
+
+
+class EmfdtdParams(C.Structure):
+    _fields_ = [("n", C.c_int), ("d", C.c_int), ("cfln", C.c_double), ("dt", C.c_double),
+                ("b", C.c_double * (EMFDTD_MAX_D + 1)), ("c", C.c_double * (EMFDTD_MAX_D + 1)),
+                ("ook", C.c_double * (EMFDTD_MAX_D + 1)), ("src", C.c_int * 3), ("sink", C.c_int * 3),
+                ("f_mod", C.c_double), ("t0", C.c_double), ("spread", C.c_double)]
+
 
 class PennesParams(C.Structure):
     _fields_ = [("tau", C.c_double), ("cdiff", C.c_double), ("cperf", C.c_double),
@@ -107,6 +117,12 @@ def load():
     if hasattr(L, "scafes_yee_run"):
         _sig(L, "scafes_yee_run", [vp, C.c_int, C.c_int, C.c_double, vp, vp, C.c_int, C.c_int, dp, ip])
         _sig(L, "scafes_yee_init_pulse", [vp, C.c_int, C.c_int, vp, C.c_double])
+    if hasattr(L, "scafes_emfdtd_run"):
+        _sig(L, "scafes_emfdtd_tables", [C.c_int] + [C.c_double] * 6 + [dp, dp, dp])
+        _sig(L, "scafes_emfdtd_dt", [C.c_double], C.c_double)
+        _sig(L, "scafes_emfdtd_ga_gb", [C.c_size_t, dp, C.c_double, dp, dp])
+        _sig(L, "scafes_emfdtd_source", [C.c_double] * 5, C.c_double)
+        _sig(L, "scafes_emfdtd_run", [vp, C.POINTER(EmfdtdParams), vp, C.c_int, C.c_double, vp])
     if hasattr(L, "scafes_batch2d_run"):
         _sig(L, "scafes_batch2d_run", [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                        C.c_int, C.c_int])
@@ -229,3 +245,30 @@ class DeviceArray:
         if self.ptr:
             check(self.ctx.L.scafes_free(self.ctx.h, self.ptr))
             self.ptr = None
+
+
+def emfdtd_params(n, d=5, cfln=0.5, src=(10, 10, 10), sink=(10, 10, 14), f_mod=0.0, t0=40.0, spread=8.0,
+                  m_a=1.0, m=3.0, k_max=1.0, s_max=0.01222, a_max=1e-3):
+    """scafes_emfdtd_params of the EMFDTD example ([PML] / [source] / [sink]
+    sections of examples/EMFDTD/testcase_small.ini as defaults)."""
+    L = load()
+    p = EmfdtdParams()
+    p.n, p.d, p.cfln = n, d, cfln
+    p.dt = L.scafes_emfdtd_dt(cfln)
+    dp = C.POINTER(C.c_double)
+    check(L.scafes_emfdtd_tables(d, m_a, m, k_max, s_max, a_max, p.dt, C.cast(p.b, dp), C.cast(p.c, dp),
+                                 C.cast(p.ook, dp)))
+    for q in range(3):
+        p.src[q], p.sink[q] = src[q], sink[q]
+    p.f_mod, p.t0, p.spread = f_mod, t0, spread
+    return p
+
+
+def emfdtd_ga_gb(sigma, dt):
+    """ga, gb of a host sigma array through the library's long double path."""
+    sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+    ga, gb = np.empty_like(sigma), np.empty_like(sigma)
+    dp = C.POINTER(C.c_double)
+    check(load().scafes_emfdtd_ga_gb(sigma.size, sigma.ctypes.data_as(dp), dt, ga.ctypes.data_as(dp),
+                                     gb.ctypes.data_as(dp)))
+    return ga, gb

@@ -1,0 +1,314 @@
+// emfdtd.hip -- the reference's EMFDTD example on the device: Yee leapfrog
+// with a complete CPML (convolutional PML with two auxiliary psi fields per
+// curl component) and conductive media (D -> E through ga, gb, and the
+// conduction current I), a hard Ez source and an Ez sink.  FP64, in the
+// reference's operation order (bit-identical to its CPU path).
+//
+//   H phase  examples/EMFDTD/EMFDTD.hpp:295-313 (updateInner)
+//   E phase  examples/EMFDTD/EMFDTD.hpp:330-386 (updateInner2)
+//   curl     examples/EMFDTD/stencils/CompleteCPMLCurlStencil.hpp:38-80
+//   tables   examples/EMFDTD/CPMLParams.hpp:23-49
+//   source   examples/EMFDTD/sources/ModulatedGaussianSource.hpp:25-34
+//   leapfrog src/ScaFES_Problem.hpp:2603-2619 (update phase, swap, update2)
+//
+// State: 26 arrays of n^3 doubles (x fastest, no ghosts) in the order of
+// the SCAFES_EM_* field indices.  Each phase reads only values it does not write,
+// except at the node itself, so both run in place.  The one read outside the
+// node block, Hz(i, -1, k) of the Dx stencil at j = 0, is the reference's
+// never-written ghost layer: 0.
+//
+// Two exact shortcuts cut the HBM traffic:
+//  * interior psi: where a stencil's PML index is the interior entry d,
+//    b[d] = c[d] = 0 and 1/kappa = 1, so the reference writes psi = +-0 and
+//    adds it to an unchanged derivative.  After one full step every interior
+//    psi is +-0, so later steps neither read nor write it (values differ from
+//    the reference at most in the sign of a zero).
+//  * vacuum: where gb = 0 (sigma = 0) and ga = 1 and I = 0, the reference
+//    computes E = 1 * (D - 0) = D and I += 0 * E: E = D, I untouched.  A
+//    check at the start of each call selects this path only when every
+//    gb = 0 node has ga = 1 and I = 0.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "scafes_common.hpp"
+
+namespace scafes {
+namespace {
+
+constexpr long double kEps0 = 8.85418781762e-12L;  // physical_constants.hpp:14
+constexpr double kC0 = 299792458.0;                 // physical_constants.hpp:12
+
+struct EmTables {
+    double b[SCAFES_EMFDTD_MAX_D + 1], c[SCAFES_EMFDTD_MAX_D + 1], ook[SCAFES_EMFDTD_MAX_D + 1];
+};
+
+struct EmArgs {
+    int n, d;
+    long nn;  // n^3
+    double s;  // Courant factor
+    long src, sink;
+};
+
+__device__ __forceinline__ long nidx(int n, int i, int j, int k)
+{
+    return (long)i + (long)n * ((long)j + (long)n * k);
+}
+
+// One CompleteCurlCPMLStencil evaluation.  OL / OR: offsets of the two
+// differenced fields; LO: first valid index per axis; END: last valid index
+// relative to n-1 (the stencil's I_, J_, K_ = n - 1 + END).  The PML index of
+// each term is the distance to the nearer end of the stencil's range along
+// the offset's axis, clamped to d.  FULL: read and write psi everywhere.
+template <int OL0, int OL1, int OL2, int OR0, int OR1, int OR2, int LO0, int LO1, int LO2, int E0, int E1, int E2,
+          bool FULL>
+__device__ __forceinline__ double curl(const EmArgs& a, const EmTables& t, int i, int j, int k, long m,
+                                       double dest_old, double cst, const double* __restrict__ from_l,
+                                       double* __restrict__ psi_l, const double* __restrict__ from_r,
+                                       double* __restrict__ psi_r)
+{
+    const int n = a.n;
+    const int I = n - 1 + E0, J = n - 1 + E1, K = n - 1 + E2;
+    if (i < LO0 || i > I || j < LO1 || j > J || k < LO2 || k > K) return 0.;
+    const int mi = min(i, I - i), mj = min(j, J - j), mk = min(k, K - k);
+    const int dlo = (OL0 ? mi : 0) + (OL1 ? mj : 0) + (OL2 ? mk : 0);
+    const int dro = (OR0 ? mi : 0) + (OR1 ? mj : 0) + (OR2 ? mk : 0);
+    const int dl = min(dlo, a.d), dr = min(dro, a.d);
+    const int li = i + OL0, lj = j + OL1, lk = k + OL2;
+    const int ri = i + OR0, rj = j + OR1, rk = k + OR2;
+    // only the Dx stencil reaches index -1 (j - 1 at j = 0): the zero ghost
+    const double fl = (OL1 < 0 && lj < 0) ? 0.0 : from_l[nidx(n, li, lj, lk)];
+    const double fr = (OR1 < 0 && rj < 0) ? 0.0 : from_r[nidx(n, ri, rj, rk)];
+    double dfdl = fl - from_l[m];
+    if (FULL || dl < a.d) {
+        const double pl = (t.b[dl] * psi_l[m]) + (t.c[dl] * dfdl);
+        dfdl = (t.ook[dl] * dfdl) + pl;
+        psi_l[m] = pl;
+    }
+    double dfdr = fr - from_r[m];
+    if (FULL || dr < a.d) {
+        const double pr = (t.b[dr] * psi_r[m]) + (t.c[dr] * dfdr);
+        dfdr = (t.ook[dr] * dfdr) + pr;
+        psi_r[m] = pr;
+    }
+    return dest_old + (cst * (dfdl - dfdr));
+}
+
+struct EmFields {
+    double *ex, *ey, *ez, *hx, *hy, *hz, *dx, *dy, *dz, *ix, *iy, *iz;
+    double *p_exdy, *p_exdz, *p_eydx, *p_eydz, *p_ezdx, *p_ezdy;
+    double *p_hxdy, *p_hxdz, *p_hydx, *p_hydz, *p_hzdx, *p_hzdy;
+    const double *ga, *gb;
+};
+
+__host__ EmFields fields_of(double* s, long nn)
+{
+    EmFields f;
+    double** p[24] = {&f.ex, &f.ey, &f.ez, &f.hx, &f.hy, &f.hz, &f.dx, &f.dy, &f.dz, &f.ix, &f.iy, &f.iz,
+                      &f.p_exdy, &f.p_exdz, &f.p_eydx, &f.p_eydz, &f.p_ezdx, &f.p_ezdy,
+                      &f.p_hxdy, &f.p_hxdz, &f.p_hydx, &f.p_hydz, &f.p_hzdx, &f.p_hzdy};
+    for (int q = 0; q < 24; ++q) *p[q] = s + (long)q * nn;
+    f.ga = s + 24L * nn;
+    f.gb = s + 25L * nn;
+    return f;
+}
+
+// updateInner: H from E (EMFDTD.hpp:295-313, stencil types :167-169,
+// ranges and constants :208-210).
+template <bool FULL>
+__global__ __launch_bounds__(256) void em_h_kernel(EmArgs a, EmTables t, EmFields f)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= a.n || j >= a.n) return;
+    const long m = nidx(a.n, i, j, k);
+    const double S = a.s;
+    f.hx[m] = curl<0, 0, 1, 0, 1, 0, 1, 0, 0, 0, -1, -1, FULL>(a, t, i, j, k, m, f.hx[m], S, f.ey, f.p_eydz, f.ez,
+                                                               f.p_ezdy);
+    f.hy[m] = curl<1, 0, 0, 0, 0, 1, 0, 1, 0, -1, 0, -1, FULL>(a, t, i, j, k, m, f.hy[m], S, f.ez, f.p_ezdx, f.ex,
+                                                               f.p_exdz);
+    f.hz[m] = curl<0, 1, 0, 1, 0, 0, 0, 0, 1, -1, -1, 0, FULL>(a, t, i, j, k, m, f.hz[m], S, f.ex, f.p_exdy, f.ey,
+                                                               f.p_eydx);
+}
+
+// updateInner2: D from H, E = ga (D - I), I += gb E, hard source, sink
+// (EMFDTD.hpp:330-386, stencil types :170-172, ranges and constants :211-213).
+template <bool FULL, bool VACUUM>
+__global__ __launch_bounds__(256) void em_e_kernel(EmArgs a, EmTables t, EmFields f, double src_value,
+                                                   double* __restrict__ sink)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= a.n || j >= a.n) return;
+    const long m = nidx(a.n, i, j, k);
+    const double S = -a.s;
+    const double Dx = curl<0, -1, 0, 0, 0, -1, 0, 0, 1, -1, 0, 0, FULL>(a, t, i, j, k, m, f.dx[m], S, f.hz,
+                                                                        f.p_hzdy, f.hy, f.p_hydz);
+    const double Dy = curl<0, 0, -1, -1, 0, 0, 1, 0, 1, 0, -1, 0, FULL>(a, t, i, j, k, m, f.dy[m], S, f.hx,
+                                                                        f.p_hxdz, f.hz, f.p_hzdx);
+    const double Dz = curl<-1, 0, 0, 0, -1, 0, 1, 1, 0, 0, 0, -1, FULL>(a, t, i, j, k, m, f.dz[m], S, f.hy,
+                                                                        f.p_hydx, f.hx, f.p_hxdy);
+    const double gb = f.gb[m];
+    double Ex, Ey, Ez;
+    if (VACUUM && gb == 0.0) {
+        // ga = 1, I = 0: E = 1 * (D - 0) = D, I unchanged
+        Ex = Dx;
+        Ey = Dy;
+        Ez = Dz;
+    } else {
+        const double ga = f.ga[m];
+        const double ix = f.ix[m], iy = f.iy[m], iz = f.iz[m];
+        Ex = ga * (Dx - ix);
+        Ey = ga * (Dy - iy);
+        Ez = ga * (Dz - iz);
+        f.ix[m] = ix + (gb * Ex);
+        f.iy[m] = iy + (gb * Ey);
+        f.iz[m] = iz + (gb * Ez);
+    }
+    f.dx[m] = Dx;
+    f.dy[m] = Dy;
+    f.dz[m] = Dz;
+    f.ex[m] = Ex;
+    f.ey[m] = Ey;
+    const double ez = m == a.src ? src_value : Ez;
+    f.ez[m] = ez;
+    if (sink && m == a.sink) *sink = ez;
+}
+
+// 1 when some gb = 0 node has ga != 1 or a nonzero conduction current.
+__global__ __launch_bounds__(256) void em_vacuum_check(long nn, EmFields f, int* __restrict__ bad)
+{
+    int any = 0;
+    for (long m = blockIdx.x * (long)blockDim.x + threadIdx.x; m < nn; m += (long)gridDim.x * blockDim.x)
+        if (f.gb[m] == 0.0 && (f.ga[m] != 1.0 || f.ix[m] != 0.0 || f.iy[m] != 0.0 || f.iz[m] != 0.0)) any = 1;
+    if (any) *bad = 1;  // every writer stores the same value
+}
+
+}  // namespace
+}  // namespace scafes
+
+using namespace scafes;
+
+extern "C" {
+
+int scafes_emfdtd_tables(int d, double m_a, double m, double k_max, double s_max, double a_max, double dt,
+                         double* b, double* c, double* ook)
+{
+    if (!b || !c || !ook) return fail(SCAFES_ERR_ARG, "emfdtd_tables: null pointer");
+    if (d < 2 || d > SCAFES_EMFDTD_MAX_D) return fail(SCAFES_ERR_ARG, "emfdtd_tables: d must be 2..64");
+    // CPMLParams ctor (CPMLParams.hpp:23-49): w < d filled, the first d
+    // entries reversed (index 0 = outermost), entry d = interior.  EPSILON_0
+    // is a long double literal, so b's exponent is evaluated in long double.
+    for (int w = 0; w <= d; ++w) {
+        b[w] = 0.0;
+        c[w] = 0.0;
+        ook[w] = 1.0;
+    }
+    const double dd = (double)d;
+    for (int w = 0; w < d; ++w) {
+        const double wd = std::pow((double)w / (dd - 1.), m);
+        const double dw_a = std::pow((dd - 1. - (double)w) / (dd - 1.), m_a);
+        const double s = wd * s_max;
+        const double k = 1. + ((k_max - 1.) * wd);
+        const double a_ = dw_a * a_max;
+        b[w] = (double)std::exp((long double)(-dt) * (((s / k) + a_) / kEps0));
+        c[w] = (s * (b[w] - 1.)) / ((s * k) + (k * k * a_));
+        if (b[w] == 1.) c[w] = 0.;
+        ook[w] = 1. / k;
+    }
+    for (int lo = 0, hi = d - 1; lo < hi; ++lo, --hi) {
+        std::swap(b[lo], b[hi]);
+        std::swap(c[lo], c[hi]);
+        std::swap(ook[lo], ook[hi]);
+    }
+    return SCAFES_OK;
+}
+
+double scafes_emfdtd_dt(double cfln)
+{
+    return (cfln * 1.0) / kC0;
+}
+
+int scafes_emfdtd_ga_gb(size_t count, const double* sigma, double dt, double* ga, double* gb)
+{
+    if (count && (!sigma || !ga || !gb)) return fail(SCAFES_ERR_ARG, "emfdtd_ga_gb: null pointer");
+    for (size_t q = 0; q < count; ++q) {
+        ga[q] = (double)(1. / (1. + (sigma[q]) * dt / kEps0));
+        gb[q] = (double)((sigma[q]) * dt / kEps0);
+    }
+    return SCAFES_OK;
+}
+
+double scafes_emfdtd_source(double t, double dt, double f_mod, double t0, double spread)
+{
+    double v = (t0 - t) / spread;
+    v *= v;
+    v *= -.5;
+    v = std::exp(v);
+    const double mod = std::cos(2. * M_PI * f_mod * t * dt);
+    return v * mod;
+}
+
+int scafes_emfdtd_run(scafes_ctx* ctx, const scafes_emfdtd_params* p, double* state, int nsteps, double t_start,
+                      double* sink)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || !p || !state) return fail(SCAFES_ERR_ARG, "emfdtd_run: null pointer");
+    if (p->n < 2 || p->n > 4096) return fail(SCAFES_ERR_ARG, "emfdtd_run: n must be 2..4096");
+    if (p->d < 2 || p->d > SCAFES_EMFDTD_MAX_D) return fail(SCAFES_ERR_ARG, "emfdtd_run: d must be 2..64");
+    if (nsteps < 0) return fail(SCAFES_ERR_ARG, "emfdtd_run: negative step count");
+    for (int q = 0; q < 3; ++q)
+        if (p->src[q] < 0 || p->src[q] >= p->n || p->sink[q] < 0 || p->sink[q] >= p->n)
+            return fail(SCAFES_ERR_ARG, "emfdtd_run: source or sink outside the grid");
+    if (nsteps == 0) return SCAFES_OK;
+    const int n = p->n;
+    EmArgs a;
+    a.n = n;
+    a.d = p->d;
+    a.nn = (long)n * n * n;
+    a.s = p->cfln;
+    a.src = (long)p->src[0] + (long)n * ((long)p->src[1] + (long)n * p->src[2]);
+    a.sink = (long)p->sink[0] + (long)n * ((long)p->sink[1] + (long)n * p->sink[2]);
+    EmTables t;
+    for (int w = 0; w <= SCAFES_EMFDTD_MAX_D; ++w) {
+        t.b[w] = w <= p->d ? p->b[w] : 0.0;
+        t.c[w] = w <= p->d ? p->c[w] : 0.0;
+        t.ook[w] = w <= p->d ? p->ook[w] : 1.0;
+    }
+    const EmFields f = fields_of(state, a.nn);
+    // vacuum shortcut allowed?
+    int* bad = nullptr;
+    SCAFES_HIP(hipMalloc((void**)&bad, sizeof(int)));
+    int hbad = 0;
+    hipError_t e = hipMemcpyAsync(bad, &hbad, sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        em_vacuum_check<<<(unsigned)std::min<long>((a.nn + 255) / 256, 4096), 256, 0, c->stream>>>(a.nn, f, bad);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(bad);
+    if (e != hipSuccess) return check_hip(e, "emfdtd vacuum check");
+    const bool vacuum = hbad == 0;
+    const dim3 blk(64, 4, 1);
+    const dim3 grid((unsigned)ceil_div(n, 64), (unsigned)ceil_div(n, 4), (unsigned)n);
+    const double dt = p->dt;
+    for (int s = 1; s <= nsteps; ++s) {
+        // the source reads the old field's time t_start + (s-1) tau, tau = 1
+        const double sv = scafes_emfdtd_source(t_start + (double)(s - 1) * 1.0, dt, p->f_mod, p->t0, p->spread);
+        double* sk = sink ? sink + (s - 1) : nullptr;
+        const bool full = s == 1;
+        if (full) em_h_kernel<true><<<grid, blk, 0, c->stream>>>(a, t, f);
+        else em_h_kernel<false><<<grid, blk, 0, c->stream>>>(a, t, f);
+        if (full && vacuum) em_e_kernel<true, true><<<grid, blk, 0, c->stream>>>(a, t, f, sv, sk);
+        else if (full) em_e_kernel<true, false><<<grid, blk, 0, c->stream>>>(a, t, f, sv, sk);
+        else if (vacuum) em_e_kernel<false, true><<<grid, blk, 0, c->stream>>>(a, t, f, sv, sk);
+        else em_e_kernel<false, false><<<grid, blk, 0, c->stream>>>(a, t, f, sv, sk);
+    }
+    SCAFES_HIP(hipGetLastError());
+    return SCAFES_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+"""GPU parity of the EMFDTD example (Yee leapfrog + complete CPML +
+conductive media, FP64) against the oracle, through the C ABI: every one of
+the 24 state fields and the sink series must be bit-identical (zeros may
+differ in sign only: numpy's array_equal treats -0.0 == 0.0).
+
+Cases: the reference's small configuration (testcase_small.ini: d = 5,
+source 10x10x10, sink 10x10x14, a conductive sphere), a deeper PML with a
+modulated source, a run split over two calls, a state whose conduction
+current is nonzero in vacuum (the general E path), and a non-cubic-looking
+off-centre source near the PML.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from masterthesis_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx(hip):
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+def _setup(n, d, src, sink, sphere=None, f_mod=0.0):
+    sigma = None
+    if sphere is not None:
+        sigma = O.ellipsoid_sigma(n, sphere[0], (sphere[1],) * 3)
+    ref = O.Emfdtd(n, d=d, src=src, sink=sink, sigma=sigma)
+    p = capi.emfdtd_params(n, d=d, src=src, sink=sink, f_mod=f_mod)
+    return ref, p
+
+
+def _run_device(ctx, p, state, nsteps, t_start):
+    dev = ctx.upload(np.ascontiguousarray(state))
+    sink = ctx.alloc(max(nsteps, 1), np.float64)
+    capi.check(ctx.L.scafes_emfdtd_run(ctx.h, C.byref(p), dev.ptr, nsteps, t_start, sink.ptr))
+    out = dev.read().reshape(state.shape)
+    s = sink.read()[:nsteps]
+    dev.free()
+    sink.free()
+    return out, s
+
+
+def _compare(got, ref_f, sink_got, sink_ref):
+    for q, name in enumerate(O.EM_FIELDS[:O.EM_STATE]):
+        a, b = got[q], ref_f[q]
+        assert np.array_equal(a, b), f"{name}: {np.count_nonzero(a != b)} nodes differ, max {np.max(np.abs(a - b))}"
+    assert np.array_equal(sink_got, sink_ref)
+
+
+@pytest.mark.parametrize("n,d,src,sink,sphere,f_mod,nsteps", [
+    (30, 5, (10, 10, 10), (10, 10, 14), ((20, 20, 20), 6.0), 0.0, 90),
+    (40, 10, (20, 18, 21), (26, 18, 21), ((12, 26, 20), 4.0), 3e7, 70),
+    (24, 3, (4, 19, 5), (12, 12, 12), None, 0.0, 60),
+])
+def test_emfdtd_bitexact(ctx, n, d, src, sink, sphere, f_mod, nsteps):
+    ref, p = _setup(n, d, src, sink, sphere, f_mod)
+    state0 = ref.f.copy()
+    got, s = _run_device(ctx, p, state0, nsteps, 0.0)
+    rs = ref.run(nsteps, f_mod=f_mod)
+    _compare(got, ref.f, s, rs)
+    assert np.abs(got[O.EM["Ez"]]).max() > 1e-6
+
+
+def test_emfdtd_split_calls(ctx):
+    """Two calls (the second with t_start = 50, starting with a full-psi
+    step again) equal one 80-step oracle run."""
+    ref, p = _setup(28, 6, (9, 14, 14), (15, 14, 14), ((19, 14, 14), 4.0))
+    state0 = ref.f.copy()
+    mid, s1 = _run_device(ctx, p, state0, 50, 0.0)
+    end, s2 = _run_device(ctx, p, mid, 30, 50.0)
+    rs = ref.run(80)
+    _compare(end, ref.f, np.r_[s1, s2], rs)
+
+
+def test_emfdtd_general_e_path(ctx):
+    """A nonzero conduction current where sigma = 0 rules out the vacuum
+    shortcut: the general path must still match."""
+    ref, p = _setup(22, 4, (11, 11, 11), (11, 11, 15))
+    rng = np.random.default_rng(5)
+    for f in ("Ix", "Iy", "Iz"):
+        ref.f[O.EM[f]] = rng.uniform(-1e-3, 1e-3, (22, 22, 22))
+    state0 = ref.f.copy()
+    got, s = _run_device(ctx, p, state0, 40, 0.0)
+    rs = ref.run(40)
+    _compare(got, ref.f, s, rs)
+
+
+def test_emfdtd_rejects_bad_arguments(ctx):
+    p = capi.emfdtd_params(16, d=4, src=(3, 3, 3), sink=(5, 5, 5))
+    buf = ctx.alloc(26 * 16 ** 3, np.float64)
+    p.src[0] = 16
+    assert ctx.L.scafes_emfdtd_run(ctx.h, C.byref(p), buf.ptr, 3, 0.0, None) != 0
+    p.src[0] = 3
+    p.d = 65
+    assert ctx.L.scafes_emfdtd_run(ctx.h, C.byref(p), buf.ptr, 3, 0.0, None) != 0
+    buf.free()
