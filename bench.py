@@ -188,6 +188,72 @@ def bench_yee(ctx, capi, ncell=384, nsteps=2000, energy_every=100):
             **_profile_fields("yee_tb2_384.md", ncell == 384)}
 
 
+# (first index, last index - (n-1), axis of the l term, axis of the r term)
+# of the six curl stencils (EMFDTD.hpp:167-172, 208-213)
+_EM_STENCILS = [((1, 0, 0), (0, -1, -1), 2, 1), ((0, 1, 0), (-1, 0, -1), 0, 2), ((0, 0, 1), (-1, -1, 0), 1, 0),
+                ((0, 0, 1), (-1, 0, 0), 1, 2), ((1, 0, 1), (0, -1, 0), 2, 0), ((1, 1, 0), (0, 0, -1), 0, 1)]
+
+
+def _emfdtd_psi_terms(n, d):
+    """Number of (node, psi term) pairs inside the PML per step: a term is
+    in the PML where the node is in its stencil's range and its distance to
+    the range's nearer end along the term's axis is below d."""
+    total = 0
+    for lo, end, al, ar in _EM_STENCILS:
+        hi = [n - 1 + e for e in end]
+        cnt = [hi[q] - lo[q] + 1 for q in range(3)]
+        for ax in (al, ar):
+            idx = np.arange(lo[ax], hi[ax] + 1)
+            pml = int(np.count_nonzero(np.minimum(idx, hi[ax] - idx) < d))
+            total += pml * cnt[(ax + 1) % 3] * cnt[(ax + 2) % 3]
+    return float(total)
+
+
+def bench_emfdtd(ctx, capi, n=256, nsteps=500):
+    """EMFDTD example (Yee leapfrog + complete CPML + conductive sphere) on
+    n^3 nodes, FP64: the [PML] / [source] / [sink] parameters of
+    examples/EMFDTD/testcase_normal.ini (d = 10, source 25x25x25, sink
+    30x30x30, unmodulated pulse t0 = 40, spread = 8) with a sphere of radius
+    50 at the centre."""
+    from masterthesis_amd.workloads import sphere_sigma
+    d = 10
+    p = capi.emfdtd_params(n, d=d, src=(25, 25, 25), sink=(30, 30, 30))
+    nn = n ** 3
+    sigma = sphere_sigma(n, (n / 2, n / 2, n / 2), 50.0).ravel()
+    ga, gb = capi.emfdtd_ga_gb(sigma, p.dt)
+    state = ctx.alloc(capi.EM_NFIELDS * nn, np.float64)
+    state.fill(0.0)
+    vp = capi.C.c_void_p
+    for q, arr in ((24, ga), (25, gb)):
+        capi.check(ctx.L.scafes_memcpy_h2d(ctx.h, vp(state.ptr.value + q * nn * 8), arr.ctypes.data_as(vp), nn * 8))
+    sink = ctx.alloc(nsteps, np.float64)
+    capi.check(ctx.L.scafes_emfdtd_run(ctx.h, capi.C.byref(p), state.ptr, 2, 0.0, sink.ptr))  # warm-up
+    state.fill(0.0)
+    for q, arr in ((24, ga), (25, gb)):
+        capi.check(ctx.L.scafes_memcpy_h2d(ctx.h, vp(state.ptr.value + q * nn * 8), arr.ctypes.data_as(vp), nn * 8))
+    ctx.sync()
+    ctx.record(0)
+    capi.check(ctx.L.scafes_emfdtd_run(ctx.h, capi.C.byref(p), state.ptr, nsteps, 0.0, sink.ptr))
+    ctx.record(1)
+    ms = ctx.elapsed_ms(0, 1)
+    sk = sink.read()
+    state.free()
+    sink.free()
+    # algorithmic bytes per node-step: H phase reads H, E and writes H (72 B);
+    # E phase reads D, H, gb and writes D, E (104 B) in vacuum, plus ga and I
+    # read / I written (56 B) in the conductor; each PML psi term adds 16 B
+    cond = float(np.count_nonzero(gb != 0.0))
+    alg = 176.0 + 56.0 * cond / nn + 16.0 * _emfdtd_psi_terms(n, d) / nn
+    per = ms / nsteps
+    rate = nn / (per * 1e-3)
+    return {"workload": f"EMFDTD {n}^3 nodes, CPML d={d}, conductive sphere r=50, {nsteps} leapfrog steps, FP64",
+            "kernels": "em_h_kernel + em_e_kernel (one pass each per step)",
+            "ms_per_step": round(per, 4), "gnode_steps_per_s": round(rate / 1e9, 3),
+            "algorithmic_bytes_per_node_step": round(alg, 1),
+            "hbm_frac_algorithmic": round(alg * rate / 1e9 / HBM_PEAK_GBS, 4),
+            "sink_peak": float(np.abs(sk).max())}
+
+
 def bench_pennes(ctx, capi, n=256, nsteps=2000, snapshots=(62, 1999)):
     """Pennes bioheat on n^3: k = 0.5, rho c = 1, T_a = 37, log-normal
     perfusion w (median 0.5, sigma 0.3, i.i.d. seeded), Gaussian source;
@@ -320,6 +386,7 @@ def main():
     ap.add_argument("--no-yee", action="store_true", help="skip the Yee FDTD leg")
     ap.add_argument("--no-pennes", action="store_true", help="skip the Pennes adjoint leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--no-emfdtd", action="store_true", help="skip the EMFDTD (CPML) leg")
     ap.add_argument("--calibrate-copy", action="store_true", help="also run a 1 GiB copy (counter calibration)")
     ap.add_argument("--leg-steps", type=int, default=0,
                     help="step count of the Yee / Pennes / batch legs (0 = their workloads' counts; "
@@ -448,6 +515,9 @@ def main():
     if not args.no_batch:
         out["batch2d"] = bench_batch(ctx, capi, **({"nsteps": args.leg_steps} if args.leg_steps else {}))
         log(f"[bench] batch {out['batch2d']}")
+    if not args.no_emfdtd:
+        out["emfdtd"] = bench_emfdtd(ctx, capi, **({"nsteps": args.leg_steps} if args.leg_steps else {}))
+        log(f"[bench] emfdtd {out['emfdtd']}")
     ctx.close()
     print(json.dumps(out), flush=True)
 

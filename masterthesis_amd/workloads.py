@@ -89,3 +89,15 @@ def heatvar_init_numpy(n, seed=0, amp=1e-3):
     u = np.where(inner, v + amp * (2.0 * un - 1.0), v)
     kap = 1.0 + 0.5 * np.sin(2 * np.pi * X) * np.sin(2 * np.pi * Y)
     return u, kap
+
+
+def sphere_sigma(n, center, radius, inside=3.546e7, h=1e-8):
+    """Conductivity of the EMFDTD example's initInner (EMFDTD.hpp:258-276:
+    the IndicatorEllipsoid of geometry/IndicatorEllipsoid.hpp:48-68 at the
+    node indices) for equal radii, vectorised: (n, n, n) array indexed
+    [k, j, i]."""
+    ax = np.arange(n, dtype=np.float64)
+    fz, fy, fx = np.meshgrid(ax - center[2], ax - center[1], ax - center[0], indexing="ij")
+    ro = np.sqrt((fx * fx + fy * fy) + fz * fz)
+    rn = float(radius)
+    return 0.5 * inside * ((ro + rn) / np.sqrt((ro + rn) ** 2 + h * h) - (ro - rn) / np.sqrt((ro - rn) ** 2 + h * h))
