@@ -292,7 +292,15 @@ typedef struct {
     double b[SCAFES_EMFDTD_MAX_D + 1], c[SCAFES_EMFDTD_MAX_D + 1], ook[SCAFES_EMFDTD_MAX_D + 1];
     int src[3], sink[3];         /* node indices (i, j, k) of the source and the sink */
     double f_mod, t0, spread;    /* ModulatedGaussianSource parameters */
+    int flags;                   /* SCAFES_EMFDTD_* below, 0 for a state of unknown origin */
 } scafes_emfdtd_params;
+/* flags: PSI_SETTLED = the state comes from a previous run with the same
+ * tables, so every interior psi is +-0 and the first step needs no full
+ * psi pass; VACUUM_OK = scafes_emfdtd_vacuum_ok reported 1 for this state
+ * (every gb = 0 node has ga = 1 and I = 0), so the check is skipped.  Both
+ * are exact shortcuts; the caller vouches for them. */
+#define SCAFES_EMFDTD_PSI_SETTLED 1
+#define SCAFES_EMFDTD_VACUUM_OK 2
 /* CPMLParams (examples/EMFDTD/CPMLParams.hpp:23-49): d+1 entries, index 0 =
  * outermost layer, index d = interior (b = c = 0, 1/kappa = 1).  Host only. */
 int scafes_emfdtd_tables(int d, double m_a, double m, double k_max, double s_max, double a_max,
@@ -310,6 +318,9 @@ double scafes_emfdtd_source(double t, double dt, double f_mod, double t0, double
  * node in sink[s - 1] (EMFDTD.hpp:376-380). */
 int scafes_emfdtd_run(scafes_ctx* ctx, const scafes_emfdtd_params* p, double* state, int nsteps,
                       double t_start, double* sink);
+/* *ok = 1 when every node with gb = 0 has ga = 1 and Ix = Iy = Iz = 0 (the
+ * condition for the vacuum shortcut; stays true under stepping). */
+int scafes_emfdtd_vacuum_ok(scafes_ctx* ctx, const scafes_emfdtd_params* p, const double* state, int* ok);
 
 /* ---- batched ensemble of 2-D heat problems -----------------------------
  * The reference runs one problem per process; this runs nprob independent

@@ -51,7 +51,10 @@ class EmfdtdParams(C.Structure):
     _fields_ = [("n", C.c_int), ("d", C.c_int), ("cfln", C.c_double), ("dt", C.c_double),
                 ("b", C.c_double * (EMFDTD_MAX_D + 1)), ("c", C.c_double * (EMFDTD_MAX_D + 1)),
                 ("ook", C.c_double * (EMFDTD_MAX_D + 1)), ("src", C.c_int * 3), ("sink", C.c_int * 3),
-                ("f_mod", C.c_double), ("t0", C.c_double), ("spread", C.c_double)]
+                ("f_mod", C.c_double), ("t0", C.c_double), ("spread", C.c_double), ("flags", C.c_int)]
+
+
+EMFDTD_PSI_SETTLED, EMFDTD_VACUUM_OK = 1, 2
 
 
 class PennesParams(C.Structure):
@@ -123,6 +126,7 @@ def load():
         _sig(L, "scafes_emfdtd_ga_gb", [C.c_size_t, dp, C.c_double, dp, dp])
         _sig(L, "scafes_emfdtd_source", [C.c_double] * 5, C.c_double)
         _sig(L, "scafes_emfdtd_run", [vp, C.POINTER(EmfdtdParams), vp, C.c_int, C.c_double, vp])
+        _sig(L, "scafes_emfdtd_vacuum_ok", [vp, C.POINTER(EmfdtdParams), vp, ip])
     if hasattr(L, "scafes_batch2d_run"):
         _sig(L, "scafes_batch2d_run", [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                        C.c_int, C.c_int])

@@ -22,11 +22,13 @@
 //    b[d] = c[d] = 0 and 1/kappa = 1, so the reference writes psi = +-0 and
 //    adds it to an unchanged derivative.  After one full step every interior
 //    psi is +-0, so later steps neither read nor write it (values differ from
-//    the reference at most in the sign of a zero).
+//    the reference at most in the sign of a zero).  A caller continuing a
+//    run says so with SCAFES_EMFDTD_PSI_SETTLED and skips the full step.
 //  * vacuum: where gb = 0 (sigma = 0) and ga = 1 and I = 0, the reference
 //    computes E = 1 * (D - 0) = D and I += 0 * E: E = D, I untouched.  A
-//    check at the start of each call selects this path only when every
-//    gb = 0 node has ga = 1 and I = 0.
+//    check at the start of each call (or the caller's SCAFES_EMFDTD_VACUUM_OK
+//    from an earlier check) selects this path only when every gb = 0 node
+//    has ga = 1 and I = 0.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -251,6 +253,29 @@ double scafes_emfdtd_source(double t, double dt, double f_mod, double t0, double
     return v * mod;
 }
 
+int scafes_emfdtd_vacuum_ok(scafes_ctx* ctx, const scafes_emfdtd_params* p, const double* state, int* ok)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || !p || !state || !ok) return fail(SCAFES_ERR_ARG, "emfdtd_vacuum_ok: null pointer");
+    if (p->n < 2 || p->n > 4096) return fail(SCAFES_ERR_ARG, "emfdtd_vacuum_ok: n must be 2..4096");
+    const long nn = (long)p->n * p->n * p->n;
+    const EmFields f = fields_of(const_cast<double*>(state), nn);
+    int* bad = nullptr;
+    SCAFES_HIP(hipMalloc((void**)&bad, sizeof(int)));
+    int hbad = 0;
+    hipError_t e = hipMemcpyAsync(bad, &hbad, sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        em_vacuum_check<<<(unsigned)std::min<long>((nn + 255) / 256, 4096), 256, 0, c->stream>>>(nn, f, bad);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(bad);
+    if (e != hipSuccess) return check_hip(e, "emfdtd vacuum check");
+    *ok = hbad == 0 ? 1 : 0;
+    return SCAFES_OK;
+}
+
 int scafes_emfdtd_run(scafes_ctx* ctx, const scafes_emfdtd_params* p, double* state, int nsteps, double t_start,
                       double* sink)
 {
@@ -279,19 +304,12 @@ int scafes_emfdtd_run(scafes_ctx* ctx, const scafes_emfdtd_params* p, double* st
     }
     const EmFields f = fields_of(state, a.nn);
     // vacuum shortcut allowed?
-    int* bad = nullptr;
-    SCAFES_HIP(hipMalloc((void**)&bad, sizeof(int)));
-    int hbad = 0;
-    hipError_t e = hipMemcpyAsync(bad, &hbad, sizeof(int), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) {
-        em_vacuum_check<<<(unsigned)std::min<long>((a.nn + 255) / 256, 4096), 256, 0, c->stream>>>(a.nn, f, bad);
-        e = hipGetLastError();
+    int ok = (p->flags & SCAFES_EMFDTD_VACUUM_OK) ? 1 : 0;
+    if (!ok) {
+        const int rc = scafes_emfdtd_vacuum_ok(ctx, p, state, &ok);
+        if (rc != SCAFES_OK) return rc;
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    (void)hipFree(bad);
-    if (e != hipSuccess) return check_hip(e, "emfdtd vacuum check");
-    const bool vacuum = hbad == 0;
+    const bool vacuum = ok != 0;
     const dim3 blk(64, 4, 1);
     const dim3 grid((unsigned)ceil_div(n, 64), (unsigned)ceil_div(n, 4), (unsigned)n);
     const double dt = p->dt;
@@ -299,7 +317,7 @@ int scafes_emfdtd_run(scafes_ctx* ctx, const scafes_emfdtd_params* p, double* st
         // the source reads the old field's time t_start + (s-1) tau, tau = 1
         const double sv = scafes_emfdtd_source(t_start + (double)(s - 1) * 1.0, dt, p->f_mod, p->t0, p->spread);
         double* sk = sink ? sink + (s - 1) : nullptr;
-        const bool full = s == 1;
+        const bool full = s == 1 && !(p->flags & SCAFES_EMFDTD_PSI_SETTLED);
         if (full) em_h_kernel<true><<<grid, blk, 0, c->stream>>>(a, t, f);
         else em_h_kernel<false><<<grid, blk, 0, c->stream>>>(a, t, f);
         if (full && vacuum) em_e_kernel<true, true><<<grid, blk, 0, c->stream>>>(a, t, f, sv, sk);

@@ -33,7 +33,8 @@ enum class DeviceKernel {
     HEAT7,     // HeatEqnFDM / PennesBioheatEqnFDM system form, FP64
     HEAT7VAR,  // conservative variable-kappa heat, FP64 / FP32 / FP32+Kahan
     PENNES,    // TumorHeatEqnFDM form with perfusion and source fields
-    YEE        // leapfrog Yee FDTD, PEC box, vacuum (updateInner = H, updateInner2 = E)
+    YEE,       // leapfrog Yee FDTD, PEC box, vacuum (updateInner = H, updateInner2 = E)
+    EMFDTD     // the EMFDTD example: Yee leapfrog + complete CPML + conductive media, FP64
 };
 enum class DevicePrecision { F64 = 0, F32 = 1, F32_KAHAN = 2 };
 // KEEP: the copied old value; KNOWN_FIELD: Dirichlet from G_new; VALUE:
@@ -86,6 +87,19 @@ struct DeviceUpdate {
     int yee[6] = {0, 1, 2, 3, 4, 5};
     double courant = 0.5;
     int energyEvery = 0;
+    // EMFDTD: data-field indices of SIGMA and the 24 updated fields in the
+    // reference's DFs order (EMFDTD.hpp:91-101: SIGMA, Ex..Ez, Hx..Hz, Dx..Dz,
+    // Ix..Iz, Psi_ExDy, Psi_ExDz, Psi_EyDx, Psi_EyDz, Psi_EzDx, Psi_EzDy,
+    // Psi_HxDy, Psi_HxDz, Psi_HyDx, Psi_HyDz, Psi_HzDx, Psi_HzDy); CPMLParams
+    // arguments; Courant factor cfln; source / sink node indices and the
+    // ModulatedGaussianSource parameters.  The sink series is recorded on the
+    // device and read with Problem::sinkData().
+    int em[25] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24};
+    int pmlD = 10;
+    double pmlMa = 1.0, pmlM = 3.0, pmlKmax = 1.0, pmlSmax = 0.01222, pmlAmax = 1e-3;
+    double cfln = 0.5;
+    int src[3] = {0, 0, 0}, sink[3] = {0, 0, 0};
+    double fMod = 0.0, t0 = 40.0, spread = 8.0;
     // Re-evaluate and upload the known fields every step (evalKnownDfs each
     // time iterate, Problem.hpp:3433-3485); false when they do not depend on t.
     bool knownTimeDependent = true;

@@ -64,7 +64,8 @@ public:
             else if (key == "asynchronMode") mAsynchronMode = toBool(val);
             else if (key == "computeGradients") mComputeGradients = toBool(val);
             else if (key == "device") mDevice = toInt(key, val);
-            else if (key == "typeDomainDecomposition" || key == "outputfile" || key == "configfile" ||
+            else if (key == "configfile") mNameConfigFile = val;
+            else if (key == "typeDomainDecomposition" || key == "outputfile" ||
                      key == "initfile" || key == "initKnownDfs" || key == "useBoostMpiSkeletonConcept" ||
                      key == "checkGradients" || key == "kindfile" || key == "writeKindFile") {
                 // accepted for command-line compatibility; no effect on one GPU
@@ -116,6 +117,8 @@ public:
     int rank() const { return 0; }
     int nProcesses() const { return 1; }
     int device() const { return mDevice; }
+    // --configfile (Parameters.hpp:516, 928-935): read by problem classes.
+    const std::string& nameConfigFile() const { return mNameConfigFile; }
 
     // Indented, level-gated output (Parameters.hpp:1726-1737).
     int indentDepth() const { return mOutputLevelCurr; }
@@ -197,6 +200,7 @@ private:
     int mCheckConvFirstAtIter, mCheckConvAtEveryNIter, mNsnapshots, mNlayersAtBorder;
     bool mEnabledAdolc, mAsynchronMode, mComputeGradients;
     int mRankOutput, mOutputLevelMax, mOutputLevelCurr, mDevice;
+    std::string mNameConfigFile;
 };
 
 }  // namespace ScaFES

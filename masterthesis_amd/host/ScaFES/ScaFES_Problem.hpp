@@ -171,6 +171,9 @@ public:
     const std::vector<double>& errorHistory() const { return mErrors; }
     // Discrete energies recorded by the Yee kernel (every energyEvery steps).
     const std::vector<double>& energyHistory() const { return mEnergies; }
+    // EMFDTD: Ez at the sink node after each step (the reference's sink_,
+    // EMFDTD.hpp:376-380), available after iterateOverTime().
+    const std::vector<double>& sinkData() const { return mSink; }
     // Time iterate at which the loop stopped (nTimesteps if it ran to the end).
     int lastTimeIter() const { return mLastIter; }
 
@@ -234,6 +237,8 @@ public:
         mKnownC = DeviceBuffer();
         mYee = DeviceBuffer();
         mYee2 = DeviceBuffer();
+        mEm = DeviceBuffer();
+        mEmSink = DeviceBuffer();
         mRef.clear();
         mCtx.reset();
     }
@@ -397,6 +402,21 @@ private:
             for (std::size_t p = 1; p < DIM; ++p)
                 if (nNodes(p) != nNodes(0)) throw std::runtime_error("DeviceUpdate: YEE needs a cubic grid");
             break;
+        case DeviceKernel::EMFDTD:
+            if (!mUseLeapfrog) throw std::runtime_error("DeviceUpdate: EMFDTD needs the leapfrog mode");
+            if (DIM != 3) throw std::runtime_error("DeviceUpdate: EMFDTD is three-dimensional");
+            if (mDev.precision != DevicePrecision::F64) throw std::runtime_error("DeviceUpdate: EMFDTD runs in FP64");
+            for (std::size_t p = 1; p < DIM; ++p)
+                if (nNodes(p) != nNodes(0)) throw std::runtime_error("DeviceUpdate: EMFDTD needs a cubic grid");
+            for (int q = 0; q < 25; ++q)
+                if (mDev.em[q] < 0 || mDev.em[q] >= nDf || mIsKnownDf[mDev.em[q]])
+                    throw std::runtime_error("DeviceUpdate: EMFDTD fields must be unknown data fields");
+            if (mDev.pmlD < 2 || mDev.pmlD > SCAFES_EMFDTD_MAX_D)
+                throw std::runtime_error("DeviceUpdate: EMFDTD needs 2 <= PML depth <= 64");
+            for (int q = 0; q < 3; ++q)
+                if (mDev.src[q] < 0 || mDev.src[q] >= nNodes(0) || mDev.sink[q] < 0 || mDev.sink[q] >= nNodes(0))
+                    throw std::runtime_error("DeviceUpdate: EMFDTD source or sink outside the grid");
+            break;
         default:
             throw std::runtime_error("DeviceUpdate: no kernel selected");
         }
@@ -443,7 +463,9 @@ private:
     {
         mCtx.reset(new DeviceContext(mParams.device()));
         const bool fl = f32();
-        if (mDev.kernel == DeviceKernel::YEE) {
+        if (mDev.kernel == DeviceKernel::EMFDTD) {
+            emSetup();
+        } else if (mDev.kernel == DeviceKernel::YEE) {
             const std::size_t nn = (std::size_t)nNodes().size();
             const std::size_t es = fl ? sizeof(float) : sizeof(double);
             mYee = DeviceBuffer(*mCtx, 6 * nn * es);
@@ -477,6 +499,41 @@ private:
         uploadKnown(0);
     }
 
+    // EMFDTD state: the 24 updated fields' node blocks, then ga and gb from
+    // SIGMA (long double, as the reference's hook evaluates them); CPML
+    // tables from the CPMLParams arguments.
+    void emSetup()
+    {
+        const std::size_t nn = (std::size_t)nNodes().size();
+        std::vector<double> tmp(26 * nn), sigma(nn);
+        for (int q = 1; q < 25; ++q) packNodes(mLumpUnknownNew[unknownSlot(mDev.em[q])], mDev.em[q], &tmp[(q - 1) * nn]);
+        packNodes(mLumpUnknownNew[unknownSlot(mDev.em[0])], mDev.em[0], sigma.data());
+        mEmP = scafes_emfdtd_params();
+        mEmP.n = nNodes(0);
+        mEmP.d = mDev.pmlD;
+        mEmP.cfln = mDev.cfln;
+        mEmP.dt = scafes_emfdtd_dt(mDev.cfln);
+        deviceCheck(scafes_emfdtd_tables(mDev.pmlD, mDev.pmlMa, mDev.pmlM, mDev.pmlKmax, mDev.pmlSmax, mDev.pmlAmax,
+                                         mEmP.dt, mEmP.b, mEmP.c, mEmP.ook),
+                    "CPML tables");
+        deviceCheck(scafes_emfdtd_ga_gb(nn, sigma.data(), mEmP.dt, &tmp[24 * nn], &tmp[25 * nn]), "ga/gb");
+        for (int q = 0; q < 3; ++q) {
+            mEmP.src[q] = mDev.src[q];
+            mEmP.sink[q] = mDev.sink[q];
+        }
+        mEmP.f_mod = mDev.fMod;
+        mEmP.t0 = mDev.t0;
+        mEmP.spread = mDev.spread;
+        mEmP.flags = 0;
+        mEm = DeviceBuffer(*mCtx, tmp.size() * sizeof(double));
+        mEm.upload(tmp.data(), tmp.size() * sizeof(double));
+        mEmSink = DeviceBuffer(*mCtx, (std::size_t)std::max(1, mParams.nTimesteps()) * sizeof(double));
+        mEmSteps = 0;
+        int ok = 0;
+        deviceCheck(scafes_emfdtd_vacuum_ok(mCtx->get(), &mEmP, (const double*)mEm.get(), &ok), "vacuum check");
+        if (ok) mEmP.flags |= SCAFES_EMFDTD_VACUUM_OK;
+    }
+
     // Known fields the kernel reads (and reference solutions for compErr).
     void uploadKnown(const int& n)
     {
@@ -505,7 +562,7 @@ private:
         // reference solutions of unknown fields, for compErrLinf on the device
         for (std::size_t u = 0; u < nUnknown(); ++u) {
             const int ii = mUnknownIndex[u];
-            if (mComputeError[ii] && !fl && mDev.kernel != DeviceKernel::YEE) {
+            if (mComputeError[ii] && !fl && mDev.kernel != DeviceKernel::YEE && mDev.kernel != DeviceKernel::EMFDTD) {
                 if (mRef.size() < nUnknown()) mRef.resize(nUnknown());
                 up(ii, mRef[u], false, false);
             }
@@ -595,6 +652,16 @@ private:
             deviceCheck(scafes_pennes_step(c, &g, hw, f32() ? SCAFES_F32 : SCAFES_F64, mUOld.get(), mUNew.get(),
                                            mKnownA.get(), mKnownB.get(), &p),
                         "pennes step");
+            break;
+        }
+        case DeviceKernel::EMFDTD: {
+            // updateInner (H) and updateInner2 (D, E, I, source, sink); the
+            // source reads the old Ez's time (EMFDTD.hpp:370-371)
+            const double tOld = mVectUnknownDfsDomOld[unknownSlot(mDev.em[3])].time();
+            deviceCheck(scafes_emfdtd_run(c, &mEmP, (double*)mEm.get(), 1, tOld, (double*)mEmSink.get() + mEmSteps),
+                        "emfdtd step");
+            mEmP.flags |= SCAFES_EMFDTD_PSI_SETTLED;
+            ++mEmSteps;
             break;
         }
         case DeviceKernel::YEE: {
@@ -706,6 +773,19 @@ private:
         const int ii = mUnknownIndex[u];
         std::vector<CT>& lump = df.elemData() == mLumpUnknownOld[u].data() ? mLumpUnknownOld[u] : mLumpUnknownNew[u];
         const bool isNew = (&df == &mVectUnknownDfsDomNew[u]);
+        if (mDev.kernel == DeviceKernel::EMFDTD) {
+            const std::size_t nn = (std::size_t)nNodes().size();
+            int q = -1;
+            for (int r = 1; r < 25; ++r)
+                if (mDev.em[r] == ii) q = r;
+            if (q < 0) return;  // SIGMA: never updated, both lumps hold it
+            std::vector<double> tmp(nn);
+            deviceCheck(scafes_memcpy_d2h(mCtx->get(), tmp.data(), (const double*)mEm.get() + (q - 1) * nn,
+                                          nn * sizeof(double)),
+                        "download");
+            unpackNodes(lump, ii, tmp.data());
+            return;
+        }
         if (mDev.kernel == DeviceKernel::YEE) {
             const std::size_t nn = (std::size_t)nNodes().size();
             int c = -1;
@@ -747,6 +827,11 @@ private:
     {
         if (!mCtx) return;
         mCtx->sync();
+        if (mDev.kernel == DeviceKernel::EMFDTD && mEmSteps > 0) {
+            mSink.assign((std::size_t)mEmSteps, 0.0);
+            deviceCheck(scafes_memcpy_d2h(mCtx->get(), mSink.data(), mEmSink.get(), mSink.size() * sizeof(double)),
+                        "sink download");
+        }
         for (std::size_t u = 0; u < nUnknown(); ++u) downloadUnknownTo(u, mVectUnknownDfsDomOld[u]);
     }
 
@@ -788,12 +873,14 @@ private:
     ScaFES_VectDf<CT> mVectUnknownDfsDomOld, mVectUnknownDfsDomNew, mVectKnownDfsDomOld, mVectKnownDfsDomNew;
     DataField<short, DIM> mKind;
     std::map<std::string, double> mWallClockTimes;
-    std::vector<double> mErrors, mEnergies;
+    std::vector<double> mErrors, mEnergies, mSink;
     int mLastIter = 0;
     // device state
     DeviceUpdate mDev;
     std::unique_ptr<DeviceContext> mCtx;
-    DeviceBuffer mUOld, mUNew, mCOld, mCNew, mKnownA, mKnownB, mKnownC, mYee, mYee2;
+    DeviceBuffer mUOld, mUNew, mCOld, mCNew, mKnownA, mKnownB, mKnownC, mYee, mYee2, mEm, mEmSink;
+    scafes_emfdtd_params mEmP;
+    int mEmSteps = 0;
     std::vector<DeviceBuffer> mRef;
 };
 

@@ -79,6 +79,38 @@ def test_emfdtd_split_calls(ctx):
     _compare(end, ref.f, np.r_[s1, s2], rs)
 
 
+def test_emfdtd_settled_flags(ctx):
+    """Single-step calls continuing a run with PSI_SETTLED | VACUUM_OK (as
+    the host Problem makes them) equal the oracle; vacuum_ok reports the
+    state's condition."""
+    ref, p = _setup(24, 5, (8, 12, 12), (14, 12, 12), ((16, 12, 12), 3.0))
+    state = ref.f.copy()
+    dev = ctx.upload(state)
+    ok = C.c_int(-1)
+    capi.check(ctx.L.scafes_emfdtd_vacuum_ok(ctx.h, C.byref(p), dev.ptr, C.byref(ok)))
+    assert ok.value == 1
+    sink = ctx.alloc(30, np.float64)
+    for s in range(30):
+        p.flags = 0 if s == 0 else capi.EMFDTD_PSI_SETTLED | capi.EMFDTD_VACUUM_OK
+        sp = C.c_void_p(sink.ptr.value + 8 * s)
+        capi.check(ctx.L.scafes_emfdtd_run(ctx.h, C.byref(p), dev.ptr, 1, float(s), sp))
+    got = dev.read().reshape(state.shape)
+    rs = ref.run(30)
+    _compare(got, ref.f, sink.read(), rs)
+    dev.free()
+    sink.free()
+
+
+def test_emfdtd_vacuum_ok_detects_current(ctx):
+    ref, p = _setup(12, 3, (6, 6, 6), (6, 6, 8))
+    ref.f[O.EM["Iy"]][3, 4, 5] = 1e-9
+    dev = ctx.upload(ref.f)
+    ok = C.c_int(-1)
+    capi.check(ctx.L.scafes_emfdtd_vacuum_ok(ctx.h, C.byref(p), dev.ptr, C.byref(ok)))
+    assert ok.value == 0
+    dev.free()
+
+
 def test_emfdtd_general_e_path(ctx):
     """A nonzero conduction current where sigma = 0 rules out the vacuum
     shortcut: the general path must still match."""

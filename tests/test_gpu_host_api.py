@@ -207,3 +207,53 @@ def test_yee_problem_through_host_api(tmp_path, hip, prog):
     else:
         assert np.abs(got - ref).max() < 1e-5 * np.abs(ref).max()
         np.testing.assert_allclose(energies, eref, rtol=1e-5)
+
+
+EM_INI = """[PML]
+d={d}
+m_a=1.
+m=3
+k_max=1.
+s_max=0.01222
+a_max=1.e-3
+
+[sink]
+location={sink}
+data_file = {sinkfile}
+
+[source]
+location={src}
+f_mod = {fmod}
+t0 = 40.
+spread = 8.
+
+[initialization]
+parameters =  {sphere}
+
+[optimization]
+reference_data_file = reference.txt
+"""
+
+
+@pytest.mark.parametrize("n,d,src,sink,sphere,fmod,nsteps", [
+    (50, 5, (10, 10, 10), (10, 10, 14), (30, 30, 30, 8, 8, 8), 0.0, 60),  # testcase_small.ini on RUN.sh's 50^3
+    (36, 8, (12, 18, 17), (12, 26, 17), (24, 18, 18, 5, 5, 5), 2e7, 60),
+])
+def test_emfdtd_example_through_problem(tmp_path, hip, n, d, src, sink, sphere, fmod, nsteps):
+    """The EMFDTD example class (25 data fields, leapfrog) through
+    Problem::iterateOverTime on the device: the sink file and every updated
+    field equal the oracle's bit for bit."""
+    sinkfile = tmp_path / "reference.txt"
+    ini = tmp_path / "case.ini"
+    ini.write_text(EM_INI.format(d=d, sink="x".join(map(str, sink)), src="x".join(map(str, src)), fmod=fmod,
+                                 sinkfile=sinkfile, sphere=" ".join(map(str, sphere))))
+    dump = str(tmp_path / "f")
+    run("emfdtd", ["--dim=3", f"--nNodes={n}x{n}x{n}", f"--nTimesteps={nsteps}", f"--configfile={ini}"], dump)
+    ref = O.Emfdtd(n, d=d, src=src, sink=sink, sigma=O.ellipsoid_sigma(n, sphere[:3], sphere[3:]))
+    rs = ref.run(nsteps, f_mod=fmod)
+    got_sink = np.array([float(x) for x in sinkfile.read_text().split()])
+    np.testing.assert_array_equal(got_sink, rs)
+    assert np.abs(rs).max() > 1e-6
+    for q, name in enumerate(O.EM_FIELDS[:O.EM_STATE]):
+        got = np.fromfile(dump + "." + name, dtype=np.float64)
+        assert np.array_equal(got, ref.f[q].ravel()), name
