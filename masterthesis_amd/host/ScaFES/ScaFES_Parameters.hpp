@@ -6,9 +6,9 @@
 // (endtime - starttime)/nTimesteps if both are given, else tau = 1.
 // Bad values abort with exit(-1) after a message, as Environment::abort
 // does without MPI (src/ScaFES_Environment.hpp:167-179).
-// Options for I/O, partitioning and ADOL-C that this single-GPU engine does
-// not implement are parsed; asking for ADOL-C or more than one partition
-// is an error.  --device selects the HIP device (new).
+// Options for partitioning and ADOL-C that this single-GPU engine does not
+// implement are parsed; asking for ADOL-C or more than one partition is an
+// error.  --device selects the HIP device (new).
 #ifndef SCAFES_HOST_PARAMETERS_HPP_
 #define SCAFES_HOST_PARAMETERS_HPP_
 
@@ -65,8 +65,10 @@ public:
             else if (key == "computeGradients") mComputeGradients = toBool(val);
             else if (key == "device") mDevice = toInt(key, val);
             else if (key == "configfile") mNameConfigFile = val;
-            else if (key == "typeDomainDecomposition" || key == "outputfile" ||
-                     key == "initfile" || key == "initKnownDfs" || key == "useBoostMpiSkeletonConcept" ||
+            else if (key == "outputfile") { mNameDataFile = val; mWriteDataFile = true; }
+            else if (key == "initfile") { mNameInitFile = val; mReadInitFile = true; }
+            else if (key == "initKnownDfs") mInitKnownDfs = toInt(key, val) == 1;
+            else if (key == "typeDomainDecomposition" || key == "useBoostMpiSkeletonConcept" ||
                      key == "checkGradients" || key == "kindfile" || key == "writeKindFile") {
                 // accepted for command-line compatibility; no effect on one GPU
             } else {
@@ -112,13 +114,20 @@ public:
     bool enabledAdolc() const { return mEnabledAdolc; }
     bool asynchronMode() const { return mAsynchronMode; }
     bool computeGradients() const { return mComputeGradients; }
-    bool readInitFile() const { return false; }
     const int& rankOutput() const { return mRankOutput; }
     int rank() const { return 0; }
     int nProcesses() const { return 1; }
     int device() const { return mDevice; }
     // --configfile (Parameters.hpp:516, 928-935): read by problem classes.
     const std::string& nameConfigFile() const { return mNameConfigFile; }
+    // --outputfile / --initfile / --initKnownDfs (Parameters.hpp:514-520,
+    // 917-963): snapshot file <outputfile>.nc (default EMPTYDATAFILE), initial
+    // state from <initfile>.nc instead of the init hooks, known fields too.
+    const std::string& nameDataFile() const { return mNameDataFile; }
+    const bool& writeDataFile() const { return mWriteDataFile; }
+    const std::string& nameInitFile() const { return mNameInitFile; }
+    const bool& readInitFile() const { return mReadInitFile; }
+    const bool& initKnownDfs() const { return mInitKnownDfs; }
 
     // Indented, level-gated output (Parameters.hpp:1726-1737).
     int indentDepth() const { return mOutputLevelCurr; }
@@ -201,6 +210,8 @@ private:
     bool mEnabledAdolc, mAsynchronMode, mComputeGradients;
     int mRankOutput, mOutputLevelMax, mOutputLevelCurr, mDevice;
     std::string mNameConfigFile;
+    std::string mNameDataFile = "EMPTYDATAFILE", mNameInitFile = "EMPTYINITFILE";
+    bool mWriteDataFile = false, mReadInitFile = false, mInitKnownDfs = false;
 };
 
 }  // namespace ScaFES

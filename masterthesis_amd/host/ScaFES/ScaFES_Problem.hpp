@@ -101,10 +101,10 @@ public:
         }
         for (std::size_t u = 0; u < mUnknownIndex.size(); ++u) {
             const int ii = mUnknownIndex[u];
-            mVectUnknownDfsDomOld.push_back(DataField<CT, DIM>(mNames[ii] + "Old", &mParams, mGG, mMemAll[ii],
+            mVectUnknownDfsDomOld.push_back(DataField<CT, DIM>(mNames[ii] + "OldDom", &mParams, mGG, mMemAll[ii],
                                                                mStencilWidth[ii], mLumpUnknownOld[u].data(), 1,
                                                                0, mWriteToFile[ii]));
-            mVectUnknownDfsDomNew.push_back(DataField<CT, DIM>(mNames[ii], &mParams, mGG, mMemAll[ii],
+            mVectUnknownDfsDomNew.push_back(DataField<CT, DIM>(mNames[ii] + "NewDom", &mParams, mGG, mMemAll[ii],
                                                                mStencilWidth[ii], mLumpUnknownNew[u].data(), 1,
                                                                0, mWriteToFile[ii]));
         }
@@ -114,7 +114,7 @@ public:
             CT* pn = mLumpKnownNew[ii].empty() ? 0 : mLumpKnownNew[ii].data();
             mVectKnownDfsDomOld.push_back(DataField<CT, DIM>(mNames[ii] + "OldSolDom", &mParams, mGG,
                                                              mMemAll[ii], sw, po, 1, 0, mWriteToFile[ii]));
-            mVectKnownDfsDomNew.push_back(DataField<CT, DIM>(mNames[ii] + "SolDom", &mParams, mGG,
+            mVectKnownDfsDomNew.push_back(DataField<CT, DIM>(mNames[ii] + "NewSolDom", &mParams, mGG,
                                                              mMemAll[ii], sw, pn, 1, 0, mWriteToFile[ii]));
         }
         // node-kind flags (reference setGridGlobalFlags :3902-3980)
@@ -134,6 +134,7 @@ public:
         }
         const char* names[] = {"trace", "write", "evalKnownDf", "sync", "update", "init", "compErr", "checkConv"};
         for (int i = 0; i < 8; ++i) mWallClockTimes[names[i]] = 0.0;
+        setupDataFiles();
     }
 
     virtual ~Problem() { releaseDevice(); }
@@ -204,6 +205,7 @@ public:
                 deviceSetup();
                 mWallClockTimes["init"] += t.elapsed();
             }
+            writeAllDfs(n);
             compErrOfUnknownDfs(n);
             cont = checkConvOfUnknownDfs(n);
             swapPointersOfKnownDfs();
@@ -215,6 +217,7 @@ public:
                     std::cout << mParams.getPrefix() << " * Convergence after " << n << " of "
                               << mParams.nTimesteps() << " timesteps." << std::endl;
                 mLastIter = n;
+                writeAllDfs(mParams.nTimesteps());  // reference :2650-2660, after the swaps
                 break;
             }
         }
@@ -335,15 +338,140 @@ private:
         OWNPRBLM* self = static_cast<OWNPRBLM*>(this);
         for (std::size_t u = 0; u < nUnknown(); ++u) mVectUnknownDfsDomNew[u].time() = mParams.timeIntervalStart();
         const GridSub<DIM>& part = mGG.partition(0);
-        for (typename GridSub<DIM>::iterator it = part.begin(); it != part.end(); ++it) {
-            if (mKind[*it] & GridGlobal<DIM>::SCAFES_GLOBAL_INNER_NODE)
-                self->template initInner<CT>(mVectUnknownDfsDomNew, mParamsCurr, *it, 0);
-            else
-                self->template initBorder<CT>(mVectUnknownDfsDomNew, mParamsCurr, *it, 0);
+        if (mParams.readInitFile()) {
+            readDfsFromFile();  // instead of the init hooks (reference :2763-2795)
+        } else {
+            for (typename GridSub<DIM>::iterator it = part.begin(); it != part.end(); ++it) {
+                if (mKind[*it] & GridGlobal<DIM>::SCAFES_GLOBAL_INNER_NODE)
+                    self->template initInner<CT>(mVectUnknownDfsDomNew, mParamsCurr, *it, 0);
+                else
+                    self->template initBorder<CT>(mVectUnknownDfsDomNew, mParamsCurr, *it, 0);
+            }
         }
         for (std::size_t u = 0; u < nUnknown(); ++u) mVectUnknownDfsDomOld[u].setValuesAtMemAll(mVectUnknownDfsDomNew[u]);
         for (std::size_t ii = 0; ii < mNames.size(); ++ii)
             if (!mLumpKnownNew[ii].empty()) mVectKnownDfsDomOld[ii].setValuesAtMemAll(mVectKnownDfsDomNew[ii]);
+    }
+
+    // ---------------------------------------------------- snapshot files
+    WriteHowOften policy(std::size_t ii) const
+    {
+        return ii < mWriteToFile.size() ? mWriteToFile[ii] : WriteHowOften::NEVER;
+    }
+    // Variables of <outputfile>.nc and <initfile>.nc (reference :1760-1995).
+    // Known fields are looked up in the all-fields vector of known views by
+    // a counter of known fields, as the reference does.
+    void setupDataFiles()
+    {
+        std::vector<std::string> names;
+        std::size_t ik = 0, iu = 0;
+        for (std::size_t ii = 0; ii < mNames.size(); ++ii) {
+            if (mIsKnownDf[ii]) {
+                if (policy(ii) != WriteHowOften::NEVER) names.push_back(mVectKnownDfsDomNew[ik].name());
+                ++ik;
+            } else {
+                if (policy(ii) != WriteHowOften::NEVER) names.push_back(mVectUnknownDfsDomNew[iu].name());
+                ++iu;
+            }
+        }
+        mCommonFile = DataFile<CT, DIM>(mParams.nameDataFile(), names, nNodes());
+        if (mParams.readInitFile()) {
+            std::vector<std::string> init;
+            for (std::size_t ii = 0; ii < mNames.size(); ++ii)
+                if (!mIsKnownDf[ii] || mParams.initKnownDfs()) init.push_back(mNames[ii]);
+            mInitFile = DataFile<CT, DIM>(mParams.nameInitFile(), init, nNodes());
+        }
+    }
+    // node block (x fastest) of a view / back into a view
+    std::vector<CT> nodeBlock(const DataField<CT, DIM>& df) const
+    {
+        std::vector<CT> v;
+        v.reserve((std::size_t)nNodes().size());
+        const GridSub<DIM>& part = mGG.partition(0);
+        for (typename GridSub<DIM>::iterator it = part.begin(); it != part.end(); ++it) v.push_back(df(*it));
+        return v;
+    }
+    // writeDfsToFile (reference :4256-4420): one record when some field's
+    // policy selects this time iterate; intervals from --nSnapshots.  The
+    // data of the k-th written known field is the k-th entry of the known
+    // views (the reference's counter advances on written known fields only).
+    void writeAllDfs(const int& n)
+    {
+        Timer t;
+        int interval = mParams.nTimesteps();
+        if (0 < mParams.nSnapshots() && mParams.nSnapshots() <= mParams.nTimesteps())
+            interval = mParams.nTimesteps() / mParams.nSnapshots();
+        if (interval < 1) interval = 1;
+        std::vector<bool> writeData;
+        std::vector<std::size_t> fieldOf;
+        for (std::size_t ii = 0; ii < mNames.size(); ++ii) {
+            const WriteHowOften w = policy(ii);
+            if (w == WriteHowOften::NEVER) continue;
+            bool on = false;
+            switch (w) {
+            case WriteHowOften::AT_START: on = n == 0; break;
+            case WriteHowOften::AT_END: on = n == mParams.nTimesteps(); break;
+            case WriteHowOften::AT_START_AND_END: on = n == 0 || n == mParams.nTimesteps(); break;
+            case WriteHowOften::LIKE_GIVEN_AT_CL: on = n % interval == 0; break;
+            default: on = true; break;
+            }
+            writeData.push_back(on);
+            fieldOf.push_back(ii);
+        }
+        bool any = false;
+        for (bool b : writeData) any = any || b;
+        if (!any) return;
+        std::vector<std::vector<CT>> blocks(writeData.size());
+        std::vector<const CT*> ptrs(writeData.size(), 0);
+        std::size_t ik = 0;
+        for (std::size_t k = 0; k < fieldOf.size(); ++k) {
+            const std::size_t ii = fieldOf[k];
+            if (mIsKnownDf[ii]) {
+                const std::size_t src = ik++;
+                if (!writeData[k]) continue;
+                if (!mVectKnownDfsDomNew[src].elemData())
+                    throw std::runtime_error("writeDfsToFile: known field " + mNames[ii] +
+                                             " maps to an unallocated reference solution (reference indexing)");
+                blocks[k] = nodeBlock(mVectKnownDfsDomNew[src]);
+            } else {
+                if (!writeData[k]) continue;
+                const std::size_t u = (std::size_t)unknownSlot((int)ii);
+                if (mCtx) downloadUnknownTo(u, mVectUnknownDfsDomNew[u]);
+                blocks[k] = nodeBlock(mVectUnknownDfsDomNew[u]);
+            }
+            ptrs[k] = blocks[k].data();
+        }
+        mCommonFile.write(ptrs, writeData);
+        mWallClockTimes["write"] += t.elapsed();
+    }
+    // readDfsFromFile (reference :4427-4520): record 0 of each unknown field
+    // (and known field with --initKnownDfs=1) by its plain name.
+    void readDfsFromFile()
+    {
+        const std::size_t nn = (std::size_t)nNodes().size();
+        std::vector<std::vector<CT>> blocks;
+        std::vector<CT*> ptrs;
+        std::vector<DataField<CT, DIM>*> dst;
+        std::size_t ik = 0, iu = 0;
+        for (std::size_t ii = 0; ii < mNames.size(); ++ii) {
+            if (!mIsKnownDf[ii]) {
+                dst.push_back(&mVectUnknownDfsDomNew[iu++]);
+            } else if (mParams.initKnownDfs()) {
+                DataField<CT, DIM>* d = &mVectKnownDfsDomNew[ik++];
+                if (!d->elemData())
+                    throw std::runtime_error("readDfsFromFile: known field " + mNames[ii] +
+                                             " maps to an unallocated reference solution (reference indexing)");
+                dst.push_back(d);
+            }
+        }
+        blocks.assign(dst.size(), std::vector<CT>(nn));
+        for (std::size_t k = 0; k < dst.size(); ++k) ptrs.push_back(blocks[k].data());
+        mInitFile.read(ptrs);
+        const GridSub<DIM>& part = mGG.partition(0);
+        for (std::size_t k = 0; k < dst.size(); ++k) {
+            std::size_t q = 0;
+            for (typename GridSub<DIM>::iterator it = part.begin(); it != part.end(); ++it) (*dst[k])(*it) = blocks[k][q++];
+        }
     }
 
     // ---------------------------------------------------- device set-up
@@ -422,6 +550,11 @@ private:
         }
         if (mDev.knownGen != DeviceKnownGen::NONE && mDev.kernel != DeviceKernel::HEAT7)
             throw std::runtime_error("DeviceUpdate: device known fields are a HEAT7 option");
+        if (mDev.knownGen != DeviceKnownGen::NONE)
+            for (std::size_t ii = 0; ii < mNames.size(); ++ii)
+                if (mIsKnownDf[ii] && policy(ii) != WriteHowOften::NEVER)
+                    throw std::runtime_error("DeviceUpdate: known fields evaluated on the device are not written to "
+                                             "file (their host copies keep the n = 0 values)");
     }
 
     scafes_geom geom() const
@@ -882,6 +1015,7 @@ private:
     scafes_emfdtd_params mEmP;
     int mEmSteps = 0;
     std::vector<DeviceBuffer> mRef;
+    DataFile<CT, DIM> mCommonFile, mInitFile;
 };
 
 }  // namespace ScaFES

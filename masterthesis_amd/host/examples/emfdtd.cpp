@@ -132,7 +132,7 @@ public:
 
     EMFDTDng(ScaFES::Parameters const& cl, ScaFES::GridGlobal<3> const& gg, const Ini& ini)
         : Super(cl, gg, true, names(), halo(), std::vector<bool>(NDF, false), std::vector<int>(NDF, 0),
-                std::vector<CT>(NDF, CT(0.)), std::vector<ScaFES::WriteHowOften>(NDF, ScaFES::WriteHowOften::NEVER),
+                std::vector<CT>(NDF, CT(0.)), writePolicy(),
                 std::vector<bool>(NDF, false), ini.list("initialization.parameters", ' ')),
           mIni(ini), cfln_(.5), ds_(1.), dt_((cfln_ * ds_) / C_0)
     {
@@ -261,6 +261,14 @@ public:
 
 private:
     static std::vector<std::string> names() { return std::vector<std::string>(kNames, kNames + NDF); }
+    // EMFDTD.hpp:105-119: SIGMA at the start, E like --nSnapshots, the rest never
+    static std::vector<ScaFES::WriteHowOften> writePolicy()
+    {
+        std::vector<ScaFES::WriteHowOften> v(NDF, ScaFES::WriteHowOften::NEVER);
+        v[SIGMA] = ScaFES::WriteHowOften::AT_START;
+        v[Ex] = v[Ey] = v[Ez] = ScaFES::WriteHowOften::LIKE_GIVEN_AT_CL;
+        return v;
+    }
     static std::vector<int> halo()
     {
         std::vector<int> v(NDF, 0);

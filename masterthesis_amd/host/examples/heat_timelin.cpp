@@ -10,7 +10,8 @@
 // (tests/system/*/{Neumann,Cauchy,Radiation}/TimeLinSpaceLin; default 0 =
 // Dirichlet), -DEXAMPLE_DEVICE_KNOWN to have the engine evaluate F, G and
 // the exact solution on the device after step 0 (the hooks below then run
-// only at n = 0).
+// only at n = 0), -DEXAMPLE_WRITE for snapshot output policies (--outputfile,
+// --nSnapshots).
 #include <iostream>
 
 #include "ScaFES.hpp"
@@ -337,6 +338,11 @@ int main(int argc, char* argv[])
     std::vector<int> nLayers(3, 0);
     std::vector<double> defaults(3, 0.0);
     std::vector<ScaFES::WriteHowOften> write(3, ScaFES::WriteHowOften::NEVER);
+#ifdef EXAMPLE_WRITE
+    // snapshots: G at the start, U at the --nSnapshots intervals (F never)
+    write[1] = ScaFES::WriteHowOften::AT_START;
+    write[2] = ScaFES::WriteHowOften::LIKE_GIVEN_AT_CL;
+#endif
     std::vector<bool> err = {false, false, true};
     TimeLinSpaceLin<double, DIM> ppp(params, gg, false, names, sw, known, nLayers, defaults, write, err);
     double sumH = 0.0;

@@ -12,11 +12,13 @@ import subprocess
 
 import numpy as np
 import pytest
+from scipy.io import netcdf_file
 
 import oracle_lib as O
 
 pytestmark = pytest.mark.gpu
 BIN = os.path.join(O.ROOT, "masterthesis_amd", "bin")
+NC_FILL = 9.969209968386869e+36
 
 
 def run(prog, args, dump):
@@ -37,7 +39,7 @@ def test_timelin_through_problem(tmp_path, hip, prog, n, pennes):
     dump = str(tmp_path / "u.bin")
     args = [f"--dim={len(n)}", "--nNodes=" + "x".join(map(str, n)), f"--nTimesteps={nsteps}", f"--tau={tau!r}"]
     out = run(prog, args, dump)
-    linf = [float(x) for x in re.findall(r"Local Linf error\(U, USolDom\): (\S+)", out)]
+    linf = [float(x) for x in re.findall(r"Local Linf error\(UNewDom, UNewSolDom\): (\S+)", out)]
     assert len(linf) == nsteps + 1
     assert max(linf) < 1e-12
     coef = O.heat7_coef(tau, perf_c=1.0, perf_w=1.0, rhob_cb_w=1.0, has_perf=pennes, has_source=True)
@@ -69,7 +71,7 @@ def test_flux_border_timelin_through_problem(tmp_path, hip, prog, n, pennes, bc)
     dump = str(tmp_path / "u.bin")
     args = [f"--dim={dim}", "--nNodes=" + "x".join(map(str, n)), f"--nTimesteps={nsteps}", f"--tau={tau!r}"]
     out = run(prog, args, dump)
-    linf = [float(x) for x in re.findall(r"Local Linf error\(U, USolDom\): (\S+)", out)]
+    linf = [float(x) for x in re.findall(r"Local Linf error\(UNewDom, UNewSolDom\): (\S+)", out)]
     assert len(linf) == nsteps + 1
     assert max(linf) < 1e-12
     coef = O.heat7_coef(tau, perf_c=1.0, perf_w=1.0, rhob_cb_w=1.0, has_perf=pennes, has_source=True)
@@ -101,7 +103,7 @@ def test_device_known_fields_through_problem(tmp_path, hip, prog, n, pennes, bc)
     host_dump, dev_dump = str(tmp_path / "host.bin"), str(tmp_path / "dev.bin")
     out_host = run(prog, args, host_dump)
     out_dev = run("dk_" + prog, args, dev_dump)
-    pat = r"Local Linf error\(U, USolDom\): (\S+)"
+    pat = r"Local Linf error\(UNewDom, UNewSolDom\): (\S+)"
     assert re.findall(pat, out_dev) == re.findall(pat, out_host)
     assert len(re.findall(pat, out_dev)) == nsteps + 1
     got = np.fromfile(dev_dump, dtype=np.float64)
@@ -248,12 +250,94 @@ def test_emfdtd_example_through_problem(tmp_path, hip, n, d, src, sink, sphere, 
     ini.write_text(EM_INI.format(d=d, sink="x".join(map(str, sink)), src="x".join(map(str, src)), fmod=fmod,
                                  sinkfile=sinkfile, sphere=" ".join(map(str, sphere))))
     dump = str(tmp_path / "f")
-    run("emfdtd", ["--dim=3", f"--nNodes={n}x{n}x{n}", f"--nTimesteps={nsteps}", f"--configfile={ini}"], dump)
-    ref = O.Emfdtd(n, d=d, src=src, sink=sink, sigma=O.ellipsoid_sigma(n, sphere[:3], sphere[3:]))
-    rs = ref.run(nsteps, f_mod=fmod)
+    out = str(tmp_path / "em")
+    run("emfdtd", ["--dim=3", f"--nNodes={n}x{n}x{n}", f"--nTimesteps={nsteps}", f"--configfile={ini}",
+                   f"--outputfile={out}", "--nSnapshots=3"], dump)
+    sigma = O.ellipsoid_sigma(n, sphere[:3], sphere[3:])
+    ref = O.Emfdtd(n, d=d, src=src, sink=sink, sigma=sigma)
+    # snapshots: SIGMA at n = 0, E every nsteps/3 steps (the reference's policy)
+    nc = netcdf_file(out + ".nc", "r", mmap=False)
+    every = nsteps // 3
+    assert nc.variables["SIGMANewDom"].shape[0] == 4
+    np.testing.assert_array_equal(nc.variables["SIGMANewDom"][0], sigma)
+    assert np.all(nc.variables["SIGMANewDom"][1:] == NC_FILL)
+    rs = []
+    for r in range(4):
+        for f in ("Ex", "Ey", "Ez"):
+            np.testing.assert_array_equal(nc.variables[f + "NewDom"][r], ref[f])
+        if r < 3:
+            rs.append(ref.run(every, t_start=float(r * every), f_mod=fmod))
+    nc.close()
+    rs = np.concatenate(rs)
     got_sink = np.array([float(x) for x in sinkfile.read_text().split()])
     np.testing.assert_array_equal(got_sink, rs)
     assert np.abs(rs).max() > 1e-6
     for q, name in enumerate(O.EM_FIELDS[:O.EM_STATE]):
         got = np.fromfile(dump + "." + name, dtype=np.float64)
         assert np.array_equal(got, ref.f[q].ravel()), name
+
+
+def _timelin_tau(n):
+    g = O.make_grid(n)
+    return g, 0.8 / (2.0 * sum(1.0 / O.lib().oracle_gridsize(g, p) ** 2 for p in range(len(n))))
+
+
+def test_snapshot_file_through_problem(tmp_path, hip):
+    """--outputfile / --nSnapshots: U at n = 0, 5, .., 20 equals the oracle's
+    iterates bit for bit; G (AT_START) takes the record of n = 0 and, as the
+    reference's writeDfsToFile indexes known fields by a counter of written
+    known fields, holds F's values (F is NEVER written and comes first)."""
+    n, nsteps = (9, 8, 7), 20
+    g, tau = _timelin_tau(n)
+    out = str(tmp_path / "snap")
+    args = ["--dim=3", "--nNodes=" + "x".join(map(str, n)), f"--nTimesteps={nsteps}", f"--tau={tau!r}",
+            f"--outputfile={out}", "--nSnapshots=4"]
+    run("heat_timelin_3d_nc", args, str(tmp_path / "u.bin"))
+    nc = netcdf_file(out + ".nc", "r", mmap=False)
+    assert list(nc.dimensions.items()) == [("nNodes_0", 9), ("nNodes_1", 8), ("nNodes_2", 7), ("time", None)]
+    assert nc.variables["version"].getValue() == 3
+    assert sorted(k for k in nc.variables if k != "version") == ["GNewSolDom", "UNewDom"]
+    U, G = nc.variables["UNewDom"], nc.variables["GNewSolDom"]
+    assert U.shape == (5, 7, 8, 9)
+    coef = O.heat7_coef(tau, has_source=True)
+    for r, steps in enumerate(range(0, nsteps + 1, 5)):
+        ref, _, _ = O.heat7_run(g, np.zeros(O.memall_shape(g)), steps, 0.0, coef, O.KNOWN_HEAT_TIMELIN)
+        np.testing.assert_array_equal(U[r], O.nodes_view(g, ref))
+    F0 = O.known_timelin(g, O.KNOWN_HEAT_TIMELIN, O.BORDER_FIELD, 0.0, coef)[0]
+    np.testing.assert_array_equal(G[0].ravel(), F0)
+    assert np.all(G[1:] == NC_FILL)
+    nc.close()
+
+
+def test_init_file_through_problem(tmp_path, hip):
+    """--initfile: the unknown field U starts from record 0 of variable "U"
+    in <initfile>.nc (written here by scipy) instead of the init hooks; the
+    run then equals the oracle's steps from that state."""
+    n, nsteps = (10, 9, 8), 15
+    g, tau = _timelin_tau(n)
+    rng = np.random.default_rng(7)
+    u0 = np.zeros(O.memall_shape(g))
+    x = [np.linspace(0.0, 1.0, m) for m in n]
+    Z, Y, X = np.meshgrid(x[2], x[1], x[0], indexing="ij")
+    y0 = 1.0 + X + Y + Z
+    y0[1:-1, 1:-1, 1:-1] += 1e-2 * rng.standard_normal((n[2] - 2, n[1] - 2, n[0] - 2))
+    O.nodes_view(g, u0)[...] = y0
+    base = str(tmp_path / "start")
+    f = netcdf_file(base + ".nc", "w", version=1)
+    f.createDimension("time", None)
+    for q in range(3):
+        f.createDimension(f"nNodes_{q}", n[q])
+    v = f.createVariable("U", "d", ("time", "nNodes_2", "nNodes_1", "nNodes_0"))
+    v[0] = y0
+    f.close()
+    dump = str(tmp_path / "u.bin")
+    args = ["--dim=3", "--nNodes=" + "x".join(map(str, n)), f"--nTimesteps={nsteps}", f"--tau={tau!r}",
+            f"--initfile={base}"]
+    run("heat_timelin_3d", args, dump)
+    coef = O.heat7_coef(tau, has_source=True)
+    u = u0
+    for s in range(1, nsteps + 1):
+        F_old = O.known_timelin(g, O.KNOWN_HEAT_TIMELIN, O.BORDER_FIELD, 0.0 + (s - 1) * tau, coef)[0]
+        G_new = O.known_timelin(g, O.KNOWN_HEAT_TIMELIN, O.BORDER_FIELD, 0.0 + s * tau, coef)[1]
+        u = O.heat7_step(g, u, f_old=F_old, g_new=G_new, border=O.BORDER_FIELD, coef=coef)
+    np.testing.assert_array_equal(np.fromfile(dump, dtype=np.float64), O.nodes_view(g, u).ravel())
