@@ -67,7 +67,8 @@ enum {
     SCAFES_BORDER_VALUE = 2,
     SCAFES_BORDER_NEUMANN = 3,
     SCAFES_BORDER_CAUCHY = 4,
-    SCAFES_BORDER_RADIATION = 5
+    SCAFES_BORDER_RADIATION = 5,
+    SCAFES_BORDER_TUMOR = 6   /* Pennes only: see scafes_pennes_params */
 };
 
 /* Discrete domain of one partition covering the whole grid
@@ -203,11 +204,22 @@ int scafes_reduce(scafes_ctx* ctx, const scafes_geom* g, int dtype, const void* 
 
 /* ---- Pennes bioheat with perfusion / source fields ---------------------
  * TumorHeatEqnFDM::updateInner form (examples/TumorHeatEqnFDM/TumorHeatEqnFDM.hpp:233-269)
- * with omega, Q as node fields; border nodes keep their value:
+ * with omega, Q as node fields:
  *   T' = T + sum_p (tau*cdiff*(W+E-2T))/h_p^2 + ((tau*cperf)*w)*(T_a-T) + (tau*csrc)*Q.
- * dtype F64: bit-identical to the FP64 CPU path; F32: FP32 stream. */
+ * border_mode KEEP (0): border nodes keep their value.  TUMOR: the
+ * example's updateBorder (TumorHeatEqnFDM.hpp:272-349): on each axis p, with
+ * A = (2 h_p)/k_cond, a node with index n_p - 1 replaces the missing
+ * neighbour by the inner one and a flux term,
+ *   last axis (p = dim-1), Cauchy:  ((W + W) - (A*h_conv)*(T - t_inf)) - 2T
+ *   other axes, Neumann:            ((W + W) - A*(-q_bc)) - 2T,
+ * a node with index 0 (Neumann)      ((E + A*q_bc) + E) - 2T,
+ * other axes central; then perfusion and source as on inner nodes.
+ * dtype F64: bit-identical to the FP64 CPU path; F32: FP32 stream.  The
+ * adjoint and gradient entry points require border_mode KEEP. */
 typedef struct {
     double tau, cdiff, cperf, csrc, t_a;
+    int border_mode;                      /* SCAFES_BORDER_KEEP or SCAFES_BORDER_TUMOR */
+    double k_cond, h_conv, t_inf, q_bc;   /* K, H, T_INF, Q_BC of the TUMOR border */
 } scafes_pennes_params;
 int scafes_pennes_step(scafes_ctx* ctx, const scafes_geom* g, int hw, int dtype,
                        const void* t_old, void* t_new, const void* w, const void* q,

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libscafes_hip.so")
 F32, F64, F32_KAHAN = 0, 1, 2
 BORDER_KEEP, BORDER_FIELD, BORDER_VALUE = 0, 1, 2
 BORDER_NEUMANN, BORDER_CAUCHY, BORDER_RADIATION = 3, 4, 5
+BORDER_TUMOR = 6
 HEATVAR_F64, HEATVAR_F32, HEATVAR_F32_KAHAN = 0, 1, 2
 RED_MAX_ABS_DIFF, RED_SUM_SQ_DIFF, RED_SUM_PROD = 0, 1, 2
 
@@ -59,7 +60,8 @@ EMFDTD_PSI_SETTLED, EMFDTD_VACUUM_OK = 1, 2
 
 class PennesParams(C.Structure):
     _fields_ = [("tau", C.c_double), ("cdiff", C.c_double), ("cperf", C.c_double),
-                ("csrc", C.c_double), ("t_a", C.c_double)]
+                ("csrc", C.c_double), ("t_a", C.c_double), ("border_mode", C.c_int),
+                ("k_cond", C.c_double), ("h_conv", C.c_double), ("t_inf", C.c_double), ("q_bc", C.c_double)]
 
 
 _lib = None

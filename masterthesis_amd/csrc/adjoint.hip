@@ -246,6 +246,8 @@ int scafes_pennes_gradient(scafes_ctx* ctx, const scafes_geom* g, int hw, int dt
     if (dtype != SCAFES_F32 && dtype != SCAFES_F64 && dtype != SCAFES_F32_KAHAN)
         return fail(SCAFES_ERR_ARG, "pennes gradient: bad dtype");
     if (hw < 1 || g->nlayers < 1) return fail(SCAFES_ERR_ARG, "pennes gradient: need hw >= 1 and nlayers >= 1");
+    if (p->border_mode != SCAFES_BORDER_KEEP)
+        return fail(SCAFES_ERR_UNSUPPORTED, "pennes gradient: only border_mode KEEP has an adjoint");
     Geo geo;
     int rc = make_geo(g, hw, &geo);
     if (rc) return rc;

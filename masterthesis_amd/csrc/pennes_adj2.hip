@@ -242,6 +242,8 @@ int pennes_adj2_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, 
                        double* grad, const scafes_pennes_params* p)
 {
     if (!pennes_adj2_supported(geo)) return fail(SCAFES_ERR_UNSUPPORTED, "two-step adjoint needs a 3-D grid");
+    if (p->border_mode != SCAFES_BORDER_KEEP)
+        return fail(SCAFES_ERR_UNSUPPORTED, "two-step adjoint: only border_mode KEEP has an adjoint");
     if ((t1lo == nullptr) != (t2lo == nullptr)) return fail(SCAFES_ERR_ARG, "two-step adjoint: lo parts mismatch");
     AdjCoef cf;
     double hh[3];

@@ -276,13 +276,56 @@ struct PennesOp {
     T* cout;
     double ctd, ctp, ctq, ta, hh0, hh1, hh2;  // FP64 path
     float fd0, fd1, fd2, fp, fq, fta;         // FP32 path
+    int tumor;                                // border rule of the Tumor example
+    double a2h[3], hconv, tinf, qbc;          // (2 h_p)/K, H, T_INF, Q_BC
+    // One axis of TumorHeatEqnFDM::updateBorder (TumorHeatEqnFDM.hpp:300-345):
+    // the missing neighbour replaced by the inner one plus a flux term.
+    __device__ T tumor_term(int idx, int n, bool last, int p, T wv, T ev, T c) const
+    {
+        const T a = (T)a2h[p];
+        if (idx == n - 1) {
+            if (last) return ((wv + wv) - ((a * (T)hconv) * (c - (T)tinf))) - (T)2 * c;
+            return ((wv + wv) - (a * ((T)-1 * (T)qbc))) - (T)2 * c;
+        }
+        if (idx == 0) return ((ev + (a * (T)qbc)) + ev) - (T)2 * c;
+        return (wv + ev) - (T)2 * c;
+    }
     __device__ void apply(const Geo& g, int i, int j, int k, const St7<T>& s, const St7<T>&,
                           const T* xv) const
     {
         const long m = at(g, i, j, k);
         if (is_border(g, i, j, k)) {
-            out[m] = s.c;
-            if constexpr (KAHAN) cout[m] = xv[2];
+            if (!tumor) {
+                out[m] = s.c;
+                if constexpr (KAHAN) cout[m] = xv[2];
+                return;
+            }
+            const T t0 = tumor_term(i, g.nx, g.dim == 1, 0, s.w, s.e, s.c);
+            const T t1 = g.dim > 1 ? tumor_term(j, g.ny, g.dim == 2, 1, s.s, s.n, s.c) : (T)0;
+            const T t2 = g.dim > 2 ? tumor_term(k, g.nz, true, 2, s.d, s.u, s.c) : (T)0;
+            if constexpr (sizeof(T) == 8) {
+                double v = s.c;
+                v += (ctd * t0) / hh0;
+                if (g.dim > 1) v += (ctd * t1) / hh1;
+                if (g.dim > 2) v += (ctd * t2) / hh2;
+                v += (ctp * xv[0]) * (ta - s.c);
+                v += ctq * xv[1];
+                out[m] = v;
+            } else {
+                float d = fd0 * t0;
+                if (g.dim > 1) d = __builtin_fmaf(fd1, t1, d);
+                if (g.dim > 2) d = __builtin_fmaf(fd2, t2, d);
+                d = __builtin_fmaf(fp * xv[0], fta - s.c, d);
+                d = __builtin_fmaf(fq, xv[1], d);
+                if constexpr (KAHAN) {
+                    const float y = d + xv[2];
+                    const float t = s.c + y;
+                    cout[m] = y - (t - s.c);
+                    out[m] = t;
+                } else {
+                    out[m] = s.c + d;
+                }
+            }
             return;
         }
         if constexpr (sizeof(T) == 8) {
@@ -473,6 +516,17 @@ int pennes_step_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, 
 {
     double hh[3];
     hh_of(g, hh);
+    if (p->border_mode != SCAFES_BORDER_KEEP && p->border_mode != SCAFES_BORDER_TUMOR)
+        return fail(SCAFES_ERR_ARG, "pennes: border_mode must be KEEP or TUMOR");
+    if (p->border_mode == SCAFES_BORDER_TUMOR && p->k_cond == 0.0)
+        return fail(SCAFES_ERR_ARG, "pennes: the TUMOR border needs k_cond != 0");
+    auto tumor = [&](auto& op) {
+        op.tumor = p->border_mode == SCAFES_BORDER_TUMOR ? 1 : 0;
+        for (int q = 0; q < 3; ++q) op.a2h[q] = (2.0 * g->h[q]) / p->k_cond;
+        op.hconv = p->h_conv;
+        op.tinf = p->t_inf;
+        op.qbc = p->q_bc;
+    };
     if (dtype == SCAFES_F64) {
         PennesOp<double> op{};
         op.a = (const double*)t_old;
@@ -486,6 +540,7 @@ int pennes_step_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, 
         op.hh0 = hh[0];
         op.hh1 = hh[1];
         op.hh2 = hh[2];
+        if (p->border_mode == SCAFES_BORDER_TUMOR) tumor(op);
         return launch<PennesOp<double>, double>(c, op, geo, -1);
     }
     auto fill32 = [&](auto& op) {
@@ -499,6 +554,8 @@ int pennes_step_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, 
         op.fp = (float)(p->tau * p->cperf);
         op.fq = (float)(p->tau * p->csrc);
         op.fta = (float)p->t_a;
+        op.tumor = 0;
+        if (p->border_mode == SCAFES_BORDER_TUMOR) tumor(op);
     };
     if (dtype == SCAFES_F32_KAHAN) {
         if (!cin || !cout) return fail(SCAFES_ERR_ARG, "pennes: compensated step needs both compensation fields");
@@ -517,6 +574,8 @@ int pennes_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, c
                       void* lam_new, const void* w, const void* tprev, double* grad,
                       const scafes_pennes_params* p, const void* tprev_lo)
 {
+    if (p->border_mode != SCAFES_BORDER_KEEP)
+        return fail(SCAFES_ERR_UNSUPPORTED, "pennes adjoint: only border_mode KEEP has an adjoint");
     double hh[3];
     hh_of(g, hh);
     if (dtype == SCAFES_F64) {
@@ -704,8 +763,8 @@ int scafes_heat7var_run(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode,
     if (mode < 0 || mode > 2) return fail(SCAFES_ERR_ARG, "heat7var: bad mode");
     if (mode == SCAFES_HEATVAR_F32_KAHAN && (!cmp || !cmp2))
         return fail(SCAFES_ERR_ARG, "heat7var: Kahan mode needs both compensation buffers");
-    if (border_mode == SCAFES_BORDER_FIELD)
-        return fail(SCAFES_ERR_UNSUPPORTED, "heat7var: border FIELD not supported");
+    if (border_mode != SCAFES_BORDER_KEEP && border_mode != SCAFES_BORDER_VALUE)
+        return fail(SCAFES_ERR_UNSUPPORTED, "heat7var: border must be KEEP or VALUE");
     if (nsteps < 0) return fail(SCAFES_ERR_ARG, "heat7var: negative step count");
     int rc = check_stencil_geo(g, hw);
     if (rc) return rc;

@@ -48,7 +48,8 @@ enum class DeviceBorder {
     VALUE = SCAFES_BORDER_VALUE,
     NEUMANN = SCAFES_BORDER_NEUMANN,
     CAUCHY = SCAFES_BORDER_CAUCHY,
-    RADIATION = SCAFES_BORDER_RADIATION
+    RADIATION = SCAFES_BORDER_RADIATION,
+    TUMOR = SCAFES_BORDER_TUMOR  // PENNES: the Tumor example's Neumann / Cauchy borders
 };
 
 // Known fields computed on the device instead of by the host hooks: the
@@ -82,6 +83,15 @@ struct DeviceUpdate {
     // PENNES: T' = T + sum_p tau*cdiff*(W+E-2T)/h_p^2 + tau*cperf*w*(T_a-T) + tau*csrc*Q
     int perfusionKnown = -1, sourceFieldKnown = -1;
     double cdiff = 1.0, cperf = 1.0, csrc = 1.0, tA = 0.0;
+    // PENNES with border TUMOR (TumorHeatEqnFDM.hpp:272-349): K, H, T_INF,
+    // Q_BC.  tumorFields: perfusion w and source Q chosen per node by the
+    // tumor sphere as the example's hooks do (omegaIn/qIn inside, where the
+    // squared distance of the node's coordinates to tumorCenter is <=
+    // tumorRadius^2), instead of known data fields.
+    double kCond = 1.0, hConv = 0.0, tInf = 0.0, qBc = 0.0;
+    bool tumorFields = false;
+    double tumorCenter[3] = {0.0, 0.0, 0.0};
+    double tumorRadius = 0.0, omegaIn = 0.0, omegaOut = 0.0, qIn = 0.0, qOut = 0.0;
     // YEE: fields Ex,Ey,Ez,Hx,Hy,Hz are data fields yee[0..5]; Courant number
     // tau/h; discrete energy recorded every energyEvery steps (0 = never)
     int yee[6] = {0, 1, 2, 3, 4, 5};

@@ -508,16 +508,23 @@ private:
             known(mDev.kappaKnown, "kappaKnown");
             if (mStencilWidth[mDev.kappaKnown] != mStencilWidth[mDev.unknown])
                 throw std::runtime_error("DeviceUpdate: kappa needs the stencil width of the updated field");
-            if (mDev.border == DeviceBorder::KNOWN_FIELD)
+            if (mDev.border != DeviceBorder::KEEP && mDev.border != DeviceBorder::VALUE)
                 throw std::runtime_error("DeviceUpdate: HEAT7VAR supports KEEP or VALUE borders");
             break;
         case DeviceKernel::PENNES:
             unknown(mDev.unknown);
-            known(mDev.perfusionKnown, "perfusionKnown");
-            known(mDev.sourceFieldKnown, "sourceFieldKnown");
-            if (mStencilWidth[mDev.perfusionKnown] != mStencilWidth[mDev.unknown] ||
-                mStencilWidth[mDev.sourceFieldKnown] != mStencilWidth[mDev.unknown])
-                throw std::runtime_error("DeviceUpdate: perfusion and source need the stencil width of the updated field");
+            if (!mDev.tumorFields) {
+                known(mDev.perfusionKnown, "perfusionKnown");
+                known(mDev.sourceFieldKnown, "sourceFieldKnown");
+                if (mStencilWidth[mDev.perfusionKnown] != mStencilWidth[mDev.unknown] ||
+                    mStencilWidth[mDev.sourceFieldKnown] != mStencilWidth[mDev.unknown])
+                    throw std::runtime_error(
+                        "DeviceUpdate: perfusion and source need the stencil width of the updated field");
+            }
+            if (mDev.border != DeviceBorder::KEEP && mDev.border != DeviceBorder::TUMOR)
+                throw std::runtime_error("DeviceUpdate: PENNES supports KEEP or TUMOR borders");
+            if (mDev.border == DeviceBorder::TUMOR && mDev.kCond == 0.0)
+                throw std::runtime_error("DeviceUpdate: the TUMOR border needs kCond != 0");
             if (mDev.precision == DevicePrecision::F32_KAHAN)
                 throw std::runtime_error("DeviceUpdate: PENNES runs in FP64 or FP32");
             break;
@@ -688,8 +695,12 @@ private:
             } else if (mDev.kernel == DeviceKernel::HEAT7VAR) {
                 up(mDev.kappaKnown, mKnownA, false, fl);
             } else if (mDev.kernel == DeviceKernel::PENNES) {
-                up(mDev.perfusionKnown, mKnownA, false, fl);
-                up(mDev.sourceFieldKnown, mKnownB, false, fl);
+                if (mDev.tumorFields) {
+                    if (n == 0) tumorFieldsUpload();
+                } else {
+                    up(mDev.perfusionKnown, mKnownA, false, fl);
+                    up(mDev.sourceFieldKnown, mKnownB, false, fl);
+                }
             }
         }
         // reference solutions of unknown fields, for compErrLinf on the device
@@ -700,6 +711,32 @@ private:
                 up(ii, mRef[u], false, false);
             }
         }
+    }
+    // omega and Q per node of the updated field's memAll, selected by the
+    // tumor sphere like TumorHeatEqnFDM::updateInner (:239-254): squared
+    // distance summed over the axes in order, compared with radius^2.
+    void tumorFieldsUpload()
+    {
+        const int ii = mDev.unknown;
+        const std::size_t ne = nodesOf(ii);
+        std::vector<CT> w(ne, CT(0)), q(ne, CT(0));
+        const GridSub<DIM>& part = mGG.partition(0);
+        for (typename GridSub<DIM>::iterator it = part.begin(); it != part.end(); ++it) {
+            const Ntuple<double, DIM> x = coordinates(*it);
+            CT distance = 0.0;
+            for (std::size_t pp = 0; pp < DIM; ++pp)
+                distance += (x[pp] - mDev.tumorCenter[pp]) * (x[pp] - mDev.tumorCenter[pp]);
+            const bool in = distance <= (mDev.tumorRadius * mDev.tumorRadius);
+            unsigned long s = 0;
+            mMemAll[ii].idxNodeTuple2Scalar(s, *it);
+            w[s] = in ? mDev.omegaIn : mDev.omegaOut;
+            q[s] = in ? mDev.qIn : mDev.qOut;
+        }
+        const bool fl = f32();
+        mKnownA = alloc(ne, fl);
+        mKnownB = alloc(ne, fl);
+        put(mKnownA, w, ne, fl);
+        put(mKnownB, q, ne, fl);
     }
     // Known lumps as seen through the (swapped) views.
     const std::vector<CT>& mLumpKnownOldView(int ii) const { return lumpOf(mVectKnownDfsDomOld[ii], ii); }
@@ -776,12 +813,17 @@ private:
             break;
         }
         case DeviceKernel::PENNES: {
-            scafes_pennes_params p;
+            scafes_pennes_params p = scafes_pennes_params();
             p.tau = mParams.tau();
             p.cdiff = mDev.cdiff;
             p.cperf = mDev.cperf;
             p.csrc = mDev.csrc;
             p.t_a = mDev.tA - shift();
+            p.border_mode = (int)mDev.border;
+            p.k_cond = mDev.kCond;
+            p.h_conv = mDev.hConv;
+            p.t_inf = mDev.tInf - shift();
+            p.q_bc = mDev.qBc;
             deviceCheck(scafes_pennes_step(c, &g, hw, f32() ? SCAFES_F32 : SCAFES_F64, mUOld.get(), mUNew.get(),
                                            mKnownA.get(), mKnownB.get(), &p),
                         "pennes step");

@@ -458,6 +458,79 @@ void oracle_pennes_step(const oracle_grid* g, const double* to, double* tn, cons
     }
 }
 
+/* Tumor selection of the hooks (TumorHeatEqnFDM.hpp:239-254 / :282-297):
+ * squared distance to the center summed over the axes in order. */
+static int tumor_inside(const oracle_grid* g, const oracle_tumor_coef* c, int i, int j, int k)
+{
+    double x[3] = {0, 0, 0};
+    coords(g, i, j, k, x);
+    double distance = 0.0;
+    for (int p = 0; p < g->dim; ++p) distance += (x[p] - c->center[p]) * (x[p] - c->center[p]);
+    return distance <= (c->radius * c->radius);
+}
+
+void oracle_tumor_fields(const oracle_grid* g, const oracle_tumor_coef* c, double* omega, double* q)
+{
+    memset(omega, 0, sizeof(double) * (size_t)oracle_nelems(g));
+    memset(q, 0, sizeof(double) * (size_t)oracle_nelems(g));
+    for (int k = 0; k < g->n[2]; ++k)
+        for (int j = 0; j < g->n[1]; ++j)
+            for (int i = 0; i < g->n[0]; ++i) {
+                const int in = tumor_inside(g, c, i, j, k);
+                omega[pos(g, i, j, k)] = in ? c->omega_tumor : c->omega_brain;
+                q[pos(g, i, j, k)] = in ? c->q_tumor : c->q_brain;
+            }
+}
+
+/* TumorHeatEqnFDM::updateInner (:233-269) and ::updateBorder (:272-349):
+ * on border nodes, per axis, the neighbour beyond a face with index n-1 is
+ * the inner one minus the Cauchy (last axis) or Neumann flux term; beyond a
+ * face with index 0 the inner one plus the Neumann term. */
+void oracle_tumor_step(const oracle_grid* g, const double* to, double* tn, const oracle_tumor_coef* c)
+{
+    memcpy(tn, to, sizeof(double) * (size_t)oracle_nelems(g));
+    double h[3];
+    long st[3];
+    for (int p = 0; p < 3; ++p) {
+        h[p] = p < g->dim ? oracle_gridsize(g, p) : 1.0;
+        st[p] = stride(g, p);
+    }
+    const int n0 = g->n[0], n1 = g->n[1], n2 = g->n[2];
+#pragma omp parallel for collapse(2) schedule(static) num_threads(NTHR)
+    for (int k = 0; k < n2; ++k) {
+        for (int j = 0; j < n1; ++j) {
+            for (int i = 0; i < n0; ++i) {
+                const long m = pos(g, i, j, k);
+                const int in = tumor_inside(g, c, i, j, k);
+                const double omega = in ? c->omega_tumor : c->omega_brain;
+                const double Q = in ? c->q_tumor : c->q_brain;
+                const int idx[3] = {i, j, k};
+                const double u = to[m];
+                double v = u;
+                const int border = is_border(g, i, j, k);
+                for (int p = 0; p < g->dim; ++p) {
+                    const double W = to[m - st[p]], E = to[m + st[p]];
+                    double t;
+                    if (border && idx[p] == g->n[p] - 1) {
+                        if (p == g->dim - 1)
+                            t = W + W - ((2.0 * h[p] / c->k) * c->h * (u - c->t_inf)) - 2.0 * u;
+                        else
+                            t = W + W - ((2.0 * h[p] / c->k) * (-1.0 * c->q_bc)) - 2.0 * u;
+                    } else if (border && idx[p] == 0) {
+                        t = E + ((2.0 * h[p] / c->k) * c->q_bc) + E - 2.0 * u;
+                    } else {
+                        t = W + E - 2.0 * u;
+                    }
+                    v += c->tau * (c->k / (c->rho * c->c)) * t / (h[p] * h[p]);
+                }
+                v += c->tau * ((c->rho_b * c->c_pb) / (c->rho * c->c)) * omega * (c->t_a - u);
+                v += c->tau * (1.0 / (c->rho * c->c)) * Q;
+                tn[m] = v;
+            }
+        }
+    }
+}
+
 void oracle_pennes_forward_all(const oracle_grid* g, double* states, const double* w,
                                const double* q, const oracle_pennes_coef* c, int nsteps)
 {

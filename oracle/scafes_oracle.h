@@ -182,6 +182,23 @@ void oracle_set_threads(int n);
 int oracle_get_threads(void);
 
 /*---------------------------------------------------------------------------*/
+/* TumorHeatEqnFDM example (examples/TumorHeatEqnFDM/TumorHeatEqnFDM.hpp):
+ * Pennes bioheat with perfusion and heat source chosen per node by a
+ * spherical tumor, Neumann (Q_BC) borders and a Cauchy (H, T_INF) border on
+ * the far face of the last axis. */
+typedef struct {
+    double tau, k, rho, c, rho_b, c_pb;
+    double omega_brain, omega_tumor, q_brain, q_tumor, t_a;
+    double h, t_inf, q_bc;
+    double radius, center[3];
+} oracle_tumor_coef;
+/* updateInner / updateBorder (TumorHeatEqnFDM.hpp:233-349), one step. */
+void oracle_tumor_step(const oracle_grid* g, const double* t_old, double* t_new, const oracle_tumor_coef* c);
+/* omega and Q as the hooks select them, at every node of a memAll field
+ * (ghosts 0). */
+void oracle_tumor_fields(const oracle_grid* g, const oracle_tumor_coef* c, double* omega, double* q);
+
+/*---------------------------------------------------------------------------*/
 /* EMFDTD example: Yee leapfrog with a complete CPML and conductive media
  * (examples/EMFDTD/EMFDTD.hpp, stencils/CompleteCPMLCurlStencil.hpp,
  * CPMLParams.hpp).  Fields are n^3 node blocks, x fastest. */

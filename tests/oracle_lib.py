@@ -95,6 +95,8 @@ def lib():
         L.oracle_emfdtd_step.argtypes = [C.POINTER(EmfdtdParams), C.POINTER(dp), C.c_double, dp]
         L.oracle_emfdtd_run.argtypes = [C.POINTER(EmfdtdParams), C.POINTER(dp), C.c_int, C.c_double,
                                         C.c_double, C.c_double, C.c_double, dp]
+        L.oracle_tumor_step.argtypes = [gp, dp, dp, C.POINTER(TumorCoef)]
+        L.oracle_tumor_fields.argtypes = [gp, C.POINTER(TumorCoef), dp, dp]
         L.oracle_known_timelin.argtypes = [gp, C.c_int, C.c_int, C.c_double, C.POINTER(Heat7Coef), dp, dp, dp]
         L.oracle_heat7var_step.argtypes = [gp, dp, dp, dp, C.c_double, C.c_int, C.c_double]
         L.oracle_heat7var_run.argtypes = [gp, dp, dp, C.c_double, C.c_int, C.c_int, C.c_double, dp]
@@ -117,6 +119,37 @@ def lib():
         L.oracle_get_threads.restype = C.c_int
         _lib = L
     return _lib
+
+
+class TumorCoef(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("tau", "k", "rho", "c", "rho_b", "c_pb", "omega_brain", "omega_tumor",
+                                          "q_brain", "q_tumor", "t_a", "h", "t_inf", "q_bc", "radius")] + \
+               [("center", C.c_double * 3)]
+
+
+def tumor_coef(tau, center, radius, k=0.6, rho=1040.0, c=3650.0, rho_b=1052.0, c_pb=3800.0, omega_brain=0.004,
+               omega_tumor=0.0007, q_brain=25000.0, q_tumor=25000.0, t_a=37.0, h=10.0, t_inf=22.4, q_bc=0.0):
+    """Coefficients of the TumorHeatEqnFDM example (Parameters.ini.in defaults)."""
+    t = TumorCoef()
+    for name, val in dict(tau=tau, k=k, rho=rho, c=c, rho_b=rho_b, c_pb=c_pb, omega_brain=omega_brain,
+                          omega_tumor=omega_tumor, q_brain=q_brain, q_tumor=q_tumor, t_a=t_a, h=h, t_inf=t_inf,
+                          q_bc=q_bc, radius=radius).items():
+        setattr(t, name, val)
+    for q in range(3):
+        t.center[q] = center[q] if q < len(center) else 0.0
+    return t
+
+
+def tumor_step(g, u, coef):
+    un = np.empty_like(u)
+    lib().oracle_tumor_step(C.byref(g), ptr(u), ptr(un), C.byref(coef))
+    return un
+
+
+def tumor_fields(g, coef):
+    w, q = np.empty(memall_shape(g)), np.empty(memall_shape(g))
+    lib().oracle_tumor_fields(C.byref(g), C.byref(coef), ptr(w), ptr(q))
+    return w, q
 
 
 class EmfdtdParams(C.Structure):
