@@ -252,6 +252,54 @@ int scafes_pennes_gradient(scafes_ctx* ctx, const scafes_geom* g, int hw, int dt
 /* Number of forward steps the treeverse schedule runs for (nsteps, snaps). */
 long scafes_checkpoint_forward_steps(int nsteps, int snaps);
 
+/* ---- Pennes bioheat with per-node materials (MRI data) ------------------
+ * Replaces the update phase of examples/MRIDataPBHEqnFDM (MRIDataPBHEqnFDM.hpp:
+ * 143-273, fields of main3D.cpp:17-31): nine known material fields (stencil
+ * width 0, node layout hw = 0) and the temperature T (layout hw).  Per node,
+ * with rc = rho*c:
+ *   T' = T + sum_p ((tau*(lambda/rc)) * d_p) / h_p^2
+ *          + ((tau*((rho_b*c_b)/rc))*omega) * (T_blood - T) + (tau*(1/rc))*q,
+ * d_p central on inner axes.  On border nodes, with A = (2 h_p)/lambda:
+ *   index 0 (Neumann)                 d_p = ((E + A*q_bc) + E) - 2T
+ *   index n_p-1, p < dim-1 (Neumann)  d_p = ((W + W) - A*(-q_bc)) - 2T
+ *   index n_p-1, p = dim-1, (int)surface == 1 (open skull):
+ *     d_p = (((W + W) - (A*h_conv)*(T - t_amb)) - ((A*epsilon)*sigma)*(Tr^4 - Ta^4)) - 2T
+ *     with Tr = (T + the axes before p's increments) + 273.15 (the hook reads
+ *     vNew there, MRIDataPBHEqnFDM.hpp:200) and Ta = t_amb + 273.15;
+ *   otherwise (closed skull)          d_p = ((W + W) - A*(-q_skull)) - 2T.
+ * FP64, bit-identical to the reference's CPU path. */
+enum {
+    SCAFES_MRI_RHO = 0,
+    SCAFES_MRI_C = 1,
+    SCAFES_MRI_LAMBDA = 2,
+    SCAFES_MRI_RHO_BLOOD = 3,
+    SCAFES_MRI_C_BLOOD = 4,
+    SCAFES_MRI_OMEGA = 5,
+    SCAFES_MRI_T_BLOOD = 6,
+    SCAFES_MRI_Q = 7,
+    SCAFES_MRI_SURFACE = 8,
+    SCAFES_MRI_NFIELDS = 9,
+    SCAFES_MRI_NCOEF = 4
+};
+typedef struct {
+    double tau;
+    double h_conv, t_amb;     /* H, T_AMB: open-skull convection */
+    double q_bc, q_skull;     /* Q_BC (brain faces), Q_SKULL (closed skull) */
+    double epsilon, sigma;    /* EPSILON, SIGMA (5.670373e-8): open-skull radiation */
+} scafes_mri_params;
+/* Folds the materials mat[0..7] (device pointers, node layout) for p->tau
+ * into SCAFES_MRI_NCOEF per-node coefficient fields, each laid out like T
+ * (layout hw), stored back to back in coef (4 * memAll(hw) doubles).  The
+ * materials are time-independent, so this runs once per run (again if tau or
+ * a material changes). */
+int scafes_mri_coef_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const double* const* mat,
+                        const scafes_mri_params* p, double* coef);
+/* One step t_old -> t_new (out of place).  mat is the host array of the nine
+ * material field pointers (the step reads lambda and surface on border
+ * nodes); coef comes from scafes_mri_coef_f64 with the same tau. */
+int scafes_mri_step_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const double* t_old, double* t_new,
+                        const double* const* mat, const double* coef, const scafes_mri_params* p);
+
 /* ---- Yee FDTD, PEC box, vacuum ------------------------------------------
  * Leapfrog of the reference's two-phase mode (updateInner then
  * updateInner2, Problem.hpp:2603-2619; curl stencil of

@@ -64,6 +64,13 @@ class PennesParams(C.Structure):
                 ("k_cond", C.c_double), ("h_conv", C.c_double), ("t_inf", C.c_double), ("q_bc", C.c_double)]
 
 
+class MriParams(C.Structure):
+    _fields_ = [("tau", C.c_double), ("h_conv", C.c_double), ("t_amb", C.c_double), ("q_bc", C.c_double),
+                ("q_skull", C.c_double), ("epsilon", C.c_double), ("sigma", C.c_double)]
+
+
+MRI_NFIELDS, MRI_NCOEF = 9, 4
+
 _lib = None
 vp = C.c_void_p
 
@@ -115,6 +122,9 @@ def load():
     _sig(L, "scafes_pennes_adjoint_step", [vp, gp, C.c_int, C.c_int, vp, vp, vp, vp, vp,
                                            C.POINTER(PennesParams)])
     _sig(L, "scafes_pennes_step_kahan", [vp, gp, C.c_int, vp, vp, vp, vp, vp, vp, C.POINTER(PennesParams)])
+    if hasattr(L, "scafes_mri_step_f64"):
+        _sig(L, "scafes_mri_coef_f64", [vp, gp, C.c_int, C.POINTER(vp), C.POINTER(MriParams), vp])
+        _sig(L, "scafes_mri_step_f64", [vp, gp, C.c_int, vp, vp, C.POINTER(vp), vp, C.POINTER(MriParams)])
     if hasattr(L, "scafes_pennes_gradient"):
         _sig(L, "scafes_pennes_gradient", [vp, gp, C.c_int, C.c_int, vp, vp, vp, vp,
                                            C.POINTER(PennesParams), C.c_int, C.c_int, vp, dp, dp])

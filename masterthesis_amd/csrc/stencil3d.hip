@@ -354,6 +354,100 @@ struct PennesOp {
     }
 };
 
+// Pennes bioheat with per-node materials, MRIDataPBHEqnFDM::updateInner /
+// updateBorder (examples/MRIDataPBHEqnFDM/MRIDataPBHEqnFDM.hpp:143-273), FP64
+// in the reference's operation order.  The materials do not change over a
+// run, so mri_coef_kernel folds them once into four per-node coefficients
+// with the same rounding as the hook's left-to-right products:
+//   x[0] = tau * (lambda/(rho c))                  (diffusion, every axis)
+//   x[1] = (tau * ((rho_b c_b)/(rho c))) * omega   (perfusion)
+//   x[2] = T_blood
+//   x[3] = (tau * (1/(rho c))) * q                 (metabolic source)
+// and a step streams T plus these four instead of nine material fields.
+// Border nodes also read lambda (for 2 h_p / lambda) and, on the far face of
+// the last axis, the surface flag, both in the stencil-0 node layout gk.
+struct MriOp {
+    static constexpr bool kTwo = false;
+    static constexpr int kNX = 4;
+    const double* a;  // T_old
+    const double* b;
+    const double* x[4];
+    double* out;
+    const double* lam;   // lambda, layout gk
+    const double* surf;  // surface flag, layout gk
+    Geo gk;
+    double hh0, hh1, hh2, h[3];
+    double hconv, tamb, qbc, qskull, epsilon, sigma, tamb4;  // tamb4 = (T_AMB + 273.15)^4
+    // One axis of updateBorder (MRIDataPBHEqnFDM.hpp:193-262): the missing
+    // neighbour replaced by the inner one plus the face's flux term.  On the
+    // far face of the last axis an open skull (surface == 1) has convection
+    // and radiation, where the radiated T^4 is taken at the value the update
+    // has accumulated so far (the hook reads vNew there, :200); a closed
+    // skull has the flux Q_SKULL.
+    __device__ double term(int idx, int n, bool last, double hp, double kk, double wv, double ev, double c,
+                           double acc, long q) const
+    {
+        const double a = (2.0 * hp) / kk;
+        if (idx == n - 1) {
+            if (last) {
+                if ((int)surf[q] == 1) {
+                    const double t = acc + 273.15;
+                    const double t4 = ((t * t) * t) * t;
+                    return (((wv + wv) - ((a * hconv) * (c - tamb))) - (((a * epsilon) * sigma) * (t4 - tamb4))) -
+                           2.0 * c;
+                }
+                return ((wv + wv) - (a * (-1.0 * qskull))) - 2.0 * c;
+            }
+            return ((wv + wv) - (a * (-1.0 * qbc))) - 2.0 * c;
+        }
+        if (idx == 0) return ((ev + (a * qbc)) + ev) - 2.0 * c;
+        return (wv + ev) - 2.0 * c;
+    }
+    __device__ void apply(const Geo& g, int i, int j, int k, const St7<double>& s, const St7<double>&,
+                          const double* xv) const
+    {
+        const long m = at(g, i, j, k);
+        double v = s.c;
+        if (!is_border(g, i, j, k)) {
+            v += (xv[0] * ((s.w + s.e) - 2.0 * s.c)) / hh0;
+            if (g.dim > 1) v += (xv[0] * ((s.s + s.n) - 2.0 * s.c)) / hh1;
+            if (g.dim > 2) v += (xv[0] * ((s.d + s.u) - 2.0 * s.c)) / hh2;
+        } else {
+            const long q = at(gk, i, j, k);
+            const double kk = lam[q];
+            v += (xv[0] * term(i, g.nx, g.dim == 1, h[0], kk, s.w, s.e, s.c, v, q)) / hh0;
+            if (g.dim > 1) v += (xv[0] * term(j, g.ny, g.dim == 2, h[1], kk, s.s, s.n, s.c, v, q)) / hh1;
+            if (g.dim > 2) v += (xv[0] * term(k, g.nz, true, h[2], kk, s.d, s.u, s.c, v, q)) / hh2;
+        }
+        v += xv[1] * (xv[2] - s.c);
+        v += xv[3];
+        out[m] = v;
+    }
+};
+
+// Per-node coefficients of MriOp from the nine material fields (layout gk)
+// into the temperature's layout g; ghost entries are left untouched.
+__global__ void mri_coef_kernel(Geo g, Geo gk, const double* __restrict__ rho, const double* __restrict__ cp,
+                                const double* __restrict__ lam, const double* __restrict__ rho_b,
+                                const double* __restrict__ c_b, const double* __restrict__ omega,
+                                const double* __restrict__ t_b, const double* __restrict__ qm, double tau,
+                                double* __restrict__ coef, long stride)
+{
+    const long nn = (long)g.nx * g.ny * g.nz;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < nn; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % g.nx);
+        const long r = t / g.nx;
+        const int j = (int)(r % g.ny);
+        const int k = (int)(r / g.ny);
+        const long q = at(gk, i, j, k), m = at(g, i, j, k);
+        const double rc = rho[q] * cp[q];
+        coef[m] = tau * (lam[q] / rc);
+        coef[stride + m] = (tau * ((rho_b[q] * c_b[q]) / rc)) * omega[q];
+        coef[2 * stride + m] = t_b[q];
+        coef[3 * stride + m] = (tau * (1.0 / rc)) * qm[q];
+    }
+}
+
 // Discrete adjoint of PennesOp (inner nodes), FP64 gradient accumulation.
 // LO: the forward state is a (hi, lo) pair from the compensated forward.
 template <typename T, bool LO = false>
@@ -888,6 +982,71 @@ int scafes_pennes_step_kahan(scafes_ctx* ctx, const scafes_geom* g, int hw, cons
     rc = make_geo(g, hw, &geo);
     if (rc) return rc;
     return pennes_step_launch(c, geo, g, SCAFES_F32_KAHAN, t_old, t_new, w, q, p, c_old, c_new);
+}
+
+int scafes_mri_coef_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const double* const* mat,
+                        const scafes_mri_params* p, double* coef)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || !g || !mat || !p || !coef) return fail(SCAFES_ERR_ARG, "mri coef: null pointer");
+    for (int f = 0; f < SCAFES_MRI_SURFACE; ++f)
+        if (!mat[f]) return fail(SCAFES_ERR_ARG, "mri coef: null material field");
+    Geo geo, gk;
+    int rc = make_geo(g, hw, &geo);
+    if (rc) return rc;
+    rc = make_geo(g, 0, &gk);
+    if (rc) return rc;
+    long ne = 1;
+    for (int q = 0; q < g->dim; ++q) ne *= (long)g->n[q] + 2L * hw;
+    const long nn = (long)geo.nx * geo.ny * geo.nz;
+    const int blocks = (int)std::min<long>(ceil_div(nn, 256L), 8192L);
+    mri_coef_kernel<<<blocks, 256, 0, c->stream>>>(geo, gk, mat[SCAFES_MRI_RHO], mat[SCAFES_MRI_C],
+                                                   mat[SCAFES_MRI_LAMBDA], mat[SCAFES_MRI_RHO_BLOOD],
+                                                   mat[SCAFES_MRI_C_BLOOD], mat[SCAFES_MRI_OMEGA],
+                                                   mat[SCAFES_MRI_T_BLOOD], mat[SCAFES_MRI_Q], p->tau, coef, ne);
+    SCAFES_HIP(hipGetLastError());
+    return SCAFES_OK;
+}
+
+int scafes_mri_step_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const double* t_old, double* t_new,
+                        const double* const* mat, const double* coef, const scafes_mri_params* p)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || !g || !t_old || !t_new || !mat || !coef || !p) return fail(SCAFES_ERR_ARG, "mri: null pointer");
+    if (!mat[SCAFES_MRI_LAMBDA] || !mat[SCAFES_MRI_SURFACE])
+        return fail(SCAFES_ERR_ARG, "mri: the border needs lambda and surface");
+    if (t_old == t_new) return fail(SCAFES_ERR_ARG, "mri: in-place update not allowed");
+    int rc = check_stencil_geo(g, hw);
+    if (rc) return rc;
+    Geo geo, gk;
+    rc = make_geo(g, hw, &geo);
+    if (rc) return rc;
+    rc = make_geo(g, 0, &gk);
+    if (rc) return rc;
+    long ne = 1;
+    for (int q = 0; q < g->dim; ++q) ne *= (long)g->n[q] + 2L * hw;
+    double hh[3];
+    hh_of(g, hh);
+    MriOp op{};
+    op.a = t_old;
+    op.out = t_new;
+    for (int f = 0; f < 4; ++f) op.x[f] = coef + f * ne;
+    op.lam = mat[SCAFES_MRI_LAMBDA];
+    op.surf = mat[SCAFES_MRI_SURFACE];
+    op.gk = gk;
+    op.hh0 = hh[0];
+    op.hh1 = hh[1];
+    op.hh2 = hh[2];
+    for (int q = 0; q < 3; ++q) op.h[q] = q < g->dim ? g->h[q] : 1.0;
+    op.hconv = p->h_conv;
+    op.tamb = p->t_amb;
+    op.qbc = p->q_bc;
+    op.qskull = p->q_skull;
+    op.epsilon = p->epsilon;
+    op.sigma = p->sigma;
+    const double ta = p->t_amb + 273.15;
+    op.tamb4 = ((ta * ta) * ta) * ta;
+    return launch<MriOp, double>(c, op, geo, -1);
 }
 
 int scafes_pennes_adjoint_step(scafes_ctx* ctx, const scafes_geom* g, int hw, int dtype, const void* lam,

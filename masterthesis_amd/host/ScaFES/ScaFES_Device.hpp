@@ -34,7 +34,8 @@ enum class DeviceKernel {
     HEAT7VAR,  // conservative variable-kappa heat, FP64 / FP32 / FP32+Kahan
     PENNES,    // TumorHeatEqnFDM form with perfusion and source fields
     YEE,       // leapfrog Yee FDTD, PEC box, vacuum (updateInner = H, updateInner2 = E)
-    EMFDTD     // the EMFDTD example: Yee leapfrog + complete CPML + conductive media, FP64
+    EMFDTD,    // the EMFDTD example: Yee leapfrog + complete CPML + conductive media, FP64
+    MRI        // the MRIDataPBHEqnFDM example: Pennes with nine per-node material fields, FP64
 };
 enum class DevicePrecision { F64 = 0, F32 = 1, F32_KAHAN = 2 };
 // KEEP: the copied old value; KNOWN_FIELD: Dirichlet from G_new; VALUE:
@@ -110,6 +111,13 @@ struct DeviceUpdate {
     double cfln = 0.5;
     int src[3] = {0, 0, 0}, sink[3] = {0, 0, 0};
     double fMod = 0.0, t0 = 40.0, spread = 8.0;
+    // MRI: data-field indices of the material fields rho, c, lambda,
+    // rho_blood, c_blood, omega, T_blood, q, surface (main3D.cpp:17-26; known,
+    // stencil width 0) and the border constants H (hConv), T_AMB (tInf),
+    // Q_BC (qBc), Q_SKULL, EPSILON (epsilon), SIGMA (sigma); see
+    // scafes_mri_step_f64 for the update.
+    int mri[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
+    double qSkull = 0.0;
     // Re-evaluate and upload the known fields every step (evalKnownDfs each
     // time iterate, Problem.hpp:3433-3485); false when they do not depend on t.
     bool knownTimeDependent = true;

@@ -242,6 +242,7 @@ public:
         mYee2 = DeviceBuffer();
         mEm = DeviceBuffer();
         mEmSink = DeviceBuffer();
+        mMat.clear();
         mRef.clear();
         mCtx.reset();
     }
@@ -552,6 +553,15 @@ private:
                 if (mDev.src[q] < 0 || mDev.src[q] >= nNodes(0) || mDev.sink[q] < 0 || mDev.sink[q] >= nNodes(0))
                     throw std::runtime_error("DeviceUpdate: EMFDTD source or sink outside the grid");
             break;
+        case DeviceKernel::MRI:
+            unknown(mDev.unknown);
+            if (mDev.precision != DevicePrecision::F64) throw std::runtime_error("DeviceUpdate: MRI runs in FP64");
+            for (int q = 0; q < SCAFES_MRI_NFIELDS; ++q) {
+                known(mDev.mri[q], "an MRI material field");
+                if (mStencilWidth[mDev.mri[q]] != 0)
+                    throw std::runtime_error("DeviceUpdate: MRI material fields have stencil width 0");
+            }
+            break;
         default:
             throw std::runtime_error("DeviceUpdate: no kernel selected");
         }
@@ -694,6 +704,8 @@ private:
                 if (fluxBorder()) up(mDev.borderKnown, mKnownC, n > 0, false);
             } else if (mDev.kernel == DeviceKernel::HEAT7VAR) {
                 up(mDev.kappaKnown, mKnownA, false, fl);
+            } else if (mDev.kernel == DeviceKernel::MRI) {
+                mriUpload();
             } else if (mDev.kernel == DeviceKernel::PENNES) {
                 if (mDev.tumorFields) {
                     if (n == 0) tumorFieldsUpload();
@@ -711,6 +723,39 @@ private:
                 up(ii, mRef[u], false, false);
             }
         }
+    }
+    // The nine material fields (node blocks, stencil width 0) and their
+    // per-node coefficients for this tau, laid out like the updated field.
+    void mriUpload()
+    {
+        if (mMat.size() != SCAFES_MRI_NFIELDS) mMat.resize(SCAFES_MRI_NFIELDS);
+        const double* ptr[SCAFES_MRI_NFIELDS];
+        for (int q = 0; q < SCAFES_MRI_NFIELDS; ++q) {
+            const int ii = mDev.mri[q];
+            const std::size_t nE = nodesOf(ii);
+            if (!mMat[q].get()) mMat[q] = alloc(nE, false);
+            put(mMat[q], mLumpKnownNewView(ii), nE, false);
+            ptr[q] = (const double*)mMat[q].get();
+        }
+        const std::size_t ne = nodesOf(mDev.unknown);
+        if (!mKnownA.get()) mKnownA = alloc(SCAFES_MRI_NCOEF * ne, false);
+        const scafes_geom g = geom();
+        const scafes_mri_params p = mriParams();
+        deviceCheck(scafes_mri_coef_f64(mCtx->get(), &g, mStencilWidth[mDev.unknown], ptr, &p,
+                                        (double*)mKnownA.get()),
+                    "mri coefficients");
+    }
+    scafes_mri_params mriParams() const
+    {
+        scafes_mri_params p;
+        p.tau = mParams.tau();
+        p.h_conv = mDev.hConv;
+        p.t_amb = mDev.tInf;
+        p.q_bc = mDev.qBc;
+        p.q_skull = mDev.qSkull;
+        p.epsilon = mDev.epsilon;
+        p.sigma = mDev.sigma;
+        return p;
     }
     // omega and Q per node of the updated field's memAll, selected by the
     // tumor sphere like TumorHeatEqnFDM::updateInner (:239-254): squared
@@ -827,6 +872,15 @@ private:
             deviceCheck(scafes_pennes_step(c, &g, hw, f32() ? SCAFES_F32 : SCAFES_F64, mUOld.get(), mUNew.get(),
                                            mKnownA.get(), mKnownB.get(), &p),
                         "pennes step");
+            break;
+        }
+        case DeviceKernel::MRI: {
+            const double* mat[SCAFES_MRI_NFIELDS];
+            for (int q = 0; q < SCAFES_MRI_NFIELDS; ++q) mat[q] = (const double*)mMat[q].get();
+            const scafes_mri_params p = mriParams();
+            deviceCheck(scafes_mri_step_f64(c, &g, hw, (const double*)mUOld.get(), (double*)mUNew.get(), mat,
+                                            (const double*)mKnownA.get(), &p),
+                        "mri step");
             break;
         }
         case DeviceKernel::EMFDTD: {
@@ -1056,7 +1110,7 @@ private:
     DeviceBuffer mUOld, mUNew, mCOld, mCNew, mKnownA, mKnownB, mKnownC, mYee, mYee2, mEm, mEmSink;
     scafes_emfdtd_params mEmP;
     int mEmSteps = 0;
-    std::vector<DeviceBuffer> mRef;
+    std::vector<DeviceBuffer> mRef, mMat;
     DataFile<CT, DIM> mCommonFile, mInitFile;
 };
 

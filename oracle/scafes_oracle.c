@@ -531,6 +531,68 @@ void oracle_tumor_step(const oracle_grid* g, const double* to, double* tn, const
     }
 }
 
+/* MRIDataPBHEqnFDM::updateInner (:143-168) and ::updateBorder (:176-260).
+ * The open-skull radiation term takes the node's value as accumulated so
+ * far in the new field (vNew after the axes before the last one, :200),
+ * not the old value; restated as written. */
+void oracle_mri_step(const oracle_grid* g, const double* to, double* tn, const double* const* mat,
+                     const oracle_mri_coef* c)
+{
+    memcpy(tn, to, sizeof(double) * (size_t)oracle_nelems(g));
+    double h[3];
+    long st[3];
+    for (int p = 0; p < 3; ++p) {
+        h[p] = p < g->dim ? oracle_gridsize(g, p) : 1.0;
+        st[p] = stride(g, p);
+    }
+    const int n0 = g->n[0], n1 = g->n[1], n2 = g->n[2];
+#pragma omp parallel for collapse(2) schedule(static) num_threads(NTHR)
+    for (int k = 0; k < n2; ++k) {
+        for (int j = 0; j < n1; ++j) {
+            for (int i = 0; i < n0; ++i) {
+                const long m = pos(g, i, j, k);
+                const long q0 = npos(g, i, j, k);
+                const double rho = mat[0][q0], cc = mat[1][q0], kk = mat[2][q0];
+                const double rho_blood = mat[3][q0], c_blood = mat[4][q0], omega = mat[5][q0];
+                const double t_blood = mat[6][q0], q = mat[7][q0];
+                const int trepanation = (int)mat[8][q0];
+                const int idx[3] = {i, j, k};
+                const double u = to[m];
+                double v = u;
+                const int border = is_border(g, i, j, k);
+                for (int p = 0; p < g->dim; ++p) {
+                    const double W = to[m - st[p]], E = to[m + st[p]];
+                    double t;
+                    if (border && idx[p] == g->n[p] - 1) {
+                        if (p == g->dim - 1) {
+                            if (trepanation == 1) {
+                                const double tempOld = v + 273.15;
+                                const double tempOldPow4 = tempOld * tempOld * tempOld * tempOld;
+                                const double tempAmb = c->t_amb + 273.15;
+                                const double tempAmbPow4 = tempAmb * tempAmb * tempAmb * tempAmb;
+                                t = W + W - ((2.0 * h[p] / kk) * c->h * (u - c->t_amb)) -
+                                    ((2.0 * h[p] / kk) * c->epsilon * c->sigma * (tempOldPow4 - tempAmbPow4)) - 2.0 * u;
+                            } else {
+                                t = W + W - ((2.0 * h[p] / kk) * (-1.0 * c->q_skull)) - 2.0 * u;
+                            }
+                        } else {
+                            t = W + W - ((2.0 * h[p] / kk) * (-1.0 * c->q_bc)) - 2.0 * u;
+                        }
+                    } else if (border && idx[p] == 0) {
+                        t = E + ((2.0 * h[p] / kk) * c->q_bc) + E - 2.0 * u;
+                    } else {
+                        t = W + E - 2.0 * u;
+                    }
+                    v += c->tau * (kk / (rho * cc)) * t / (h[p] * h[p]);
+                }
+                v += c->tau * ((rho_blood * c_blood) / (rho * cc)) * omega * (t_blood - u);
+                v += c->tau * (1.0 / (rho * cc)) * q;
+                tn[m] = v;
+            }
+        }
+    }
+}
+
 void oracle_pennes_forward_all(const oracle_grid* g, double* states, const double* w,
                                const double* q, const oracle_pennes_coef* c, int nsteps)
 {

@@ -199,6 +199,20 @@ void oracle_tumor_step(const oracle_grid* g, const double* t_old, double* t_new,
 void oracle_tumor_fields(const oracle_grid* g, const oracle_tumor_coef* c, double* omega, double* q);
 
 /*---------------------------------------------------------------------------*/
+/* MRIDataPBHEqnFDM example (examples/MRIDataPBHEqnFDM/MRIDataPBHEqnFDM.hpp):
+ * Pennes bioheat with per-node material fields mat[0..8] = rho, c, lambda,
+ * rho_blood, c_blood, omega, T_blood, q, surface (stencil-0 node blocks),
+ * Neumann (Q_BC) borders, and on the far face of the last axis either an
+ * open skull (surface == 1: convection H, T_inf and radiation epsilon,
+ * sigma in Kelvin) or a closed one (Neumann Q_SKULL). */
+typedef struct {
+    double tau, h, t_amb, q_bc, q_skull, epsilon, sigma;
+} oracle_mri_coef;
+/* updateInner / updateBorder (MRIDataPBHEqnFDM.hpp:143-260), one step. */
+void oracle_mri_step(const oracle_grid* g, const double* t_old, double* t_new, const double* const* mat,
+                     const oracle_mri_coef* c);
+
+/*---------------------------------------------------------------------------*/
 /* EMFDTD example: Yee leapfrog with a complete CPML and conductive media
  * (examples/EMFDTD/EMFDTD.hpp, stencils/CompleteCPMLCurlStencil.hpp,
  * CPMLParams.hpp).  Fields are n^3 node blocks, x fastest. */

@@ -95,6 +95,7 @@ def lib():
         L.oracle_emfdtd_step.argtypes = [C.POINTER(EmfdtdParams), C.POINTER(dp), C.c_double, dp]
         L.oracle_emfdtd_run.argtypes = [C.POINTER(EmfdtdParams), C.POINTER(dp), C.c_int, C.c_double,
                                         C.c_double, C.c_double, C.c_double, dp]
+        L.oracle_mri_step.argtypes = [gp, dp, dp, C.POINTER(dp), C.POINTER(MriCoef)]
         L.oracle_tumor_step.argtypes = [gp, dp, dp, C.POINTER(TumorCoef)]
         L.oracle_tumor_fields.argtypes = [gp, C.POINTER(TumorCoef), dp, dp]
         L.oracle_known_timelin.argtypes = [gp, C.c_int, C.c_int, C.c_double, C.POINTER(Heat7Coef), dp, dp, dp]
@@ -150,6 +151,28 @@ def tumor_fields(g, coef):
     w, q = np.empty(memall_shape(g)), np.empty(memall_shape(g))
     lib().oracle_tumor_fields(C.byref(g), C.byref(coef), ptr(w), ptr(q))
     return w, q
+
+
+class MriCoef(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("tau", "h", "t_amb", "q_bc", "q_skull", "epsilon", "sigma")]
+
+
+MRI_FIELDS = ["rho", "c", "lambda", "rho_blood", "c_blood", "omega", "T_blood", "q", "surface"]
+
+
+def mri_coef(tau, h=10.0, t_amb=22.0, q_bc=0.0, q_skull=0.0, epsilon=0.98, sigma=5.670373e-8):
+    m = MriCoef()
+    m.tau, m.h, m.t_amb, m.q_bc, m.q_skull, m.epsilon, m.sigma = tau, h, t_amb, q_bc, q_skull, epsilon, sigma
+    return m
+
+
+def mri_step(g, u, mats, coef):
+    """mats: 9 stencil-0 node arrays (x fastest) in MRI_FIELDS order."""
+    mats = [np.ascontiguousarray(x, dtype=np.float64).ravel() for x in mats]
+    arr = (C.POINTER(C.c_double) * 9)(*[ptr(x) for x in mats])
+    un = np.empty_like(u)
+    lib().oracle_mri_step(C.byref(g), ptr(u), ptr(un), arr, C.byref(coef))
+    return un
 
 
 class EmfdtdParams(C.Structure):
