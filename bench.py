@@ -297,9 +297,9 @@ def bench_pennes(ctx, capi, n=256, nsteps=2000, snapshots=(62, 1999)):
     out = {"workload": f"pennes {n}^3, {nsteps} steps, FP32 fields (deviation from T_a), FP64 gradient",
            "fwd_ms": round(t_fwd, 2), "fwd_GLUPS": round(float(n) ** 3 * nsteps / (t_fwd * 1e-3) / 1e9, 2),
            "fwd_kernel": "stream_kernel<PennesOp<float>> (16 B/LUP)",
-           "adjoint_kernel": "pennes_adj2_kernel<float> (two reverse steps per sweep, 18 B/step)",
+           "adjoint_kernel": "pennes_adjk_kernel<float> (up to four reverse steps per sweep, 11 B/step at four)",
            "fwd_profile": _profile_fields("pennes_fwd_256.md", n == 256),
-           "adjoint_profile": _profile_fields("pennes_adj2_256.md", n == 256), "gradient": []}
+           "adjoint_profile": _profile_fields("pennes_adjk_256.md", n == 256), "gradient": []}
     grad = ctx.alloc(ne, np.float64)
     for s in snapshots:
         J = capi.C.c_double(0)

@@ -86,7 +86,7 @@ struct Rev {
     void* lam[2];
     int lcur;
     int nsteps;
-    bool pair;  // two adjoint steps per sweep where both states are at hand
+    bool pair;  // several adjoint steps per sweep where their states are at hand
     long fwd_steps, adj_steps;
     double J;
 
@@ -145,14 +145,15 @@ struct Rev {
         lcur = 1 - lcur;
         return r;
     }
-    // Adjoint steps n and n-1 in one sweep (lam^n -> lam^{n-2}) with
-    // T^{n-1} = t1 and T^{n-2} = t2; n < nsteps.
-    int adjoint2(int n, const void* t1, const void* t2)
+    // Adjoint steps n, n-1, ..., n-K+1 in one sweep (lam^n -> lam^{n-K})
+    // with st[l] = T^{n-1-l}; n < nsteps.
+    int adjointk(int K, const void* const* st)
     {
-        (void)n;
-        adj_steps += 2;
-        int r = pennes_adj2_launch(c, geo, g, lam_dtype, lam[lcur], lam[1 - lcur], w, t1, lo(t1), t2, lo(t2), grad,
-                                   p);
+        adj_steps += K;
+        const void* los[kAdjMaxK];
+        for (int l = 0; l < K; ++l) los[l] = lo(st[l]);
+        int r = pennes_adjk_launch(c, geo, g, lam_dtype, K, lam[lcur], lam[1 - lcur], w, st, kahan() ? los : nullptr,
+                                   grad, p);
         lcur = 1 - lcur;
         return r;
     }
@@ -163,7 +164,7 @@ struct Rev {
         if (pair && s >= b - a - 1 && (long)slots.size() >= b - a - 1) {
             // Repetition number 1: every intermediate state T^{a+1..b-1} gets
             // a snapshot (b-a-1 forward steps, as the recursive split would
-            // take), then the steps are reversed two per sweep.
+            // take), then the steps are reversed up to kAdjMaxK per sweep.
             std::vector<const void*> st(1, base);  // st[q] = T^{a+q}
             int r = SCAFES_OK;
             for (long q = 1; q < b - a && !r; ++q) {
@@ -177,8 +178,11 @@ struct Rev {
                     r = adjoint((int)n, st[n - 1 - a]);
                     n -= 1;
                 } else {
-                    r = adjoint2((int)n, st[n - 1 - a], st[n - 2 - a]);
-                    n -= 2;
+                    const int K = (int)std::min<long>(kAdjMaxK, n - a);
+                    const void* sk[kAdjMaxK];
+                    for (int l = 0; l < K; ++l) sk[l] = st[n - 1 - l - a];
+                    r = adjointk(K, sk);
+                    n -= K;
                 }
             }
             for (size_t q = st.size() - 1; q >= 1; --q) slots.push_back(const_cast<void*>(st[q]));
@@ -271,7 +275,7 @@ int scafes_pennes_gradient(scafes_ctx* ctx, const scafes_geom* g, int hw, int dt
     R.grad = grad;
     R.lcur = 0;
     R.nsteps = nsteps;
-    R.pair = pennes_adj2_supported(geo);
+    R.pair = pennes_adjk_supported(geo);
     R.fwd_steps = R.adj_steps = 0;
     R.J = 0.0;
     R.work = nullptr;

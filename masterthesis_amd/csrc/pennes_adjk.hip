@@ -1,0 +1,339 @@
+// pennes_adjk.hip -- K reverse-time Pennes adjoint steps per HBM sweep
+// (lam^n -> lam^{n-K}, K = 2..kAdjMaxK) with the gradient contributions of
+// all K steps, for 3-D grids.  Per node the arithmetic is that of
+// PennesAdjOp in stencil3d.hip (the one-step adjoint), in the same order,
+// and the FP64 gradient receives the step-n term, then the step-(n-1) term,
+// ..., so the result is bit-identical to K one-step launches.
+//
+//   lam^{n-1}_i = lam^n_i + sum_p ctd*((lam_W+lam_E) - 2 lam_i)/h_p^2 - (ctp*w_i)*lam_i
+//   grad_i     += lam^n_i * (ctp*(T_a - T^{n-1}_i))        (inner nodes)
+//   lam^{n-1} = 0 on border nodes.
+//
+// A TX x TY thread tile walks a chunk of z planes.  In iteration k, level 0
+// (lam^n) is staged in LDS at plane k with a one-node ring; level L >= 1
+// (lam^{n-L}) is computed at plane k-(L-1) from the LDS plane level L-1 wrote
+// one iteration earlier (x, y neighbours) and from level L-1's register
+// history (z neighbours).  Each level loses one ring of the tile, so level K
+// is valid on the (TX-2(K-1)) x (TY-2(K-1)) output tile.  One barrier per
+// plane (double-buffered LDS planes).  The gradient terms of an output node
+// need lam^{n-L} at that node: level L's value, held in its register history
+// until the output plane is reached.  The states T^{n-1..n-K} and the
+// gradient are fetched directly for the output plane, D planes ahead.
+//
+// Per K steps HBM moves lam r+w, w, T^{n-1}..T^{n-K} and the FP64 gradient
+// r+w once: (28 + 4K)/K bytes per step for FP32 fields (18 at K = 2, 11 at
+// K = 4), against 32 for one-step launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "scafes_common.hpp"
+
+namespace scafes {
+
+namespace {
+
+using Rsrc = __amdgpu_buffer_rsrc_t;
+constexpr int kRsrcWord3 = 0x00020000;  // gfx9 family: 32-bit data format, raw (stride 0) buffer
+
+__device__ inline Rsrc plane_rsrc(const void* plane, uint32_t range)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(plane), (short)0, (int)range, kRsrcWord3);
+}
+
+template <typename T>
+__device__ inline T bload(Rsrc r, uint32_t off)
+{
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+
+template <typename T>
+__device__ inline void bstore(T v, Rsrc r, uint32_t off)
+{
+    if constexpr (sizeof(T) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)off, 0, 0);
+    } else {
+        typedef int v2i __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2i, v), r, (int)off, 0, 0);
+    }
+}
+
+struct AdjCoef {
+    double ctd, ctp, ta, hh0, hh1, hh2;  // FP64 path
+    float fd0, fd1, fd2, fp;             // FP32 path: ctd/h_p^2, ctp
+};
+
+// T^{n-1}, ..., T^{n-K} (and their lo parts for compensated states)
+struct AdjStates {
+    const void* t[kAdjMaxK];
+    const void* lo[kAdjMaxK];
+};
+
+// One adjoint step at an inner node (PennesAdjOp's order).
+template <typename T>
+__device__ inline T adj_update(const AdjCoef& cf, T c, T w_, T e, T s, T n, T d, T u, T wf)
+{
+    if constexpr (sizeof(T) == 8) {
+        double v = c;
+        v += (cf.ctd * ((w_ + e) - 2.0 * c)) / cf.hh0;
+        v += (cf.ctd * ((s + n) - 2.0 * c)) / cf.hh1;
+        v += (cf.ctd * ((d + u) - 2.0 * c)) / cf.hh2;
+        v -= (cf.ctp * wf) * c;
+        return v;
+    } else {
+        float dd = cf.fd0 * ((w_ + e) - 2.0f * c);
+        dd = __builtin_fmaf(cf.fd1, (s + n) - 2.0f * c, dd);
+        dd = __builtin_fmaf(cf.fd2, (d + u) - 2.0f * c, dd);
+        dd = __builtin_fmaf(-cf.fp * wf, c, dd);
+        return c + dd;
+    }
+}
+
+template <typename T, bool LO, int TX, int TY, int K, int D>
+__global__ __launch_bounds__(TX* TY) void pennes_adjk_kernel(Geo g, const T* __restrict__ lam, T* __restrict__ lamo,
+                                                              const T* __restrict__ wf, AdjStates st,
+                                                              double* __restrict__ grad, AdjCoef cf, int zc,
+                                                              int tiles_x, int tiles_y)
+{
+    static_assert(D >= 2 && D % 2 == 0, "prefetch depth must be even");
+    static_assert(K >= 2 && K <= kAdjMaxK, "2 <= K <= kAdjMaxK");
+    constexpr int R = K - 1;  // rings the tile loses over the sweep
+    constexpr int OX = TX - 2 * R, OY = TY - 2 * R;
+    static_assert(OX >= 1 && OY >= 1, "tile too small for K levels");
+    constexpr int H = K + 1;  // register history per level (age 0 = this iteration)
+    __shared__ T s0[2][TY + 2][TX + 2];  // lam^n plane with ring
+    __shared__ T sl[K - 1][2][TY][TX];   // levels 1..K-1
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int per_plane = tiles_x * tiles_y;
+    const int tz = lb / per_plane;
+    const int r = lb - tz * per_plane;
+    const int tyi = r / tiles_x;
+    const int txi = r - tyi * tiles_x;
+    const int i = txi * OX - R + tx, j = tyi * OY - R + ty;
+    const int ic = min(max(i, 0), g.nx - 1), jc = min(max(j, 0), g.ny - 1);
+    const bool inner_xy = i >= g.nl0 && i <= g.nx - 1 - g.nl0 && j >= g.nl1 && j <= g.ny - 1 - g.nl1;
+    const bool out_thread = tx >= R && tx <= TX - 1 - R && ty >= R && ty <= TY - 1 - R && i < g.nx && j < g.ny;
+    const int k0 = tz * zc, k1 = min(k0 + zc, g.nz);
+    const int kstart = k0 - R, kend = k1 + R - 1, kcen = k1 + R;
+    const int klo = g.nl2, khi = g.nz - 1 - g.nl2;
+    const bool hx = (tx == 0) || (tx == TX - 1);
+    const bool hy = (ty == 0) || (ty == TY - 1);
+    const int hxi = (tx == 0) ? max(ic - 1, 0) : min(ic + 1, g.nx - 1);
+    const int hyj = (ty == 0) ? max(jc - 1, 0) : min(jc + 1, g.ny - 1);
+    const int sxh = (tx == 0) ? 0 : TX + 1;
+    const int syh = (ty == 0) ? 0 : TY + 1;
+    constexpr uint32_t B = sizeof(T);
+    const uint32_t oc = (uint32_t)(g.base + ic + g.e0 * jc) * B;
+    const uint32_t ohx = (uint32_t)(g.base + hxi + g.e0 * jc) * B;
+    const uint32_t ohy = (uint32_t)(g.base + ic + g.e0 * hyj) * B;
+    const uint32_t range = (uint32_t)((g.base + g.e01) * B);
+    const uint32_t og = (uint32_t)(g.base + ic + g.e0 * jc) * 8u;  // FP64 gradient
+    const uint32_t grange = (uint32_t)((g.base + g.e01) * 8);
+    const size_t pb = (size_t)g.e01 * B, pg = (size_t)g.e01 * 8;
+    auto pl = [&](int kk) { return (size_t)min(max(kk, 0), g.nz - 1); };
+
+    T qa[D], qax[D], qay[D], qw[D], qt[K][D], qtl[K][D];
+    double qg[D];
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+        qa[s] = qax[s] = qay[s] = qw[s] = T(0);
+#pragma unroll
+        for (int l = 0; l < K; ++l) qt[l][s] = qtl[l][s] = T(0);
+        qg[s] = 0.0;
+    }
+    // slot <- level-0 input at plane kk (centre, halo, w) and the output
+    // plane kk - R's states and gradient
+    auto fetch = [&](const int slot, const int kk) {
+        if (kk <= kcen) {
+            const size_t p = pl(kk);
+            const Rsrc rl = plane_rsrc((const char*)lam + p * pb, range);
+            qa[slot] = bload<T>(rl, oc);
+            if (kk <= kend) {
+                qw[slot] = bload<T>(plane_rsrc((const char*)wf + p * pb, range), oc);
+                if (hx) qax[slot] = bload<T>(rl, ohx);
+                if (hy) qay[slot] = bload<T>(rl, ohy);
+            }
+            const int po = kk - R;
+            if (out_thread && po >= k0 && po < k1) {
+#pragma unroll
+                for (int l = 0; l < K; ++l) {
+                    qt[l][slot] = bload<T>(plane_rsrc((const char*)st.t[l] + (size_t)po * pb, range), oc);
+                    if constexpr (LO)
+                        qtl[l][slot] = bload<T>(plane_rsrc((const char*)st.lo[l] + (size_t)po * pb, range), oc);
+                }
+                qg[slot] = bload<double>(plane_rsrc((const char*)grad + (size_t)po * pg, grange), og);
+            }
+        }
+    };
+    T am = bload<T>(plane_rsrc((const char*)lam + pl(kstart - 1) * pb, range), oc);
+#pragma unroll
+    for (int s = 0; s < D; ++s) fetch(s, kstart + s);
+
+    T hist[K][H];  // hist[L][a]: level L at the plane it computed a iterations ago
+    T wh[K];       // w at planes k, k-1, ..., k-K+1
+#pragma unroll
+    for (int l = 0; l < K; ++l) {
+        wh[l] = T(0);
+#pragma unroll
+        for (int a = 0; a < H; ++a) hist[l][a] = T(0);
+    }
+    for (int kb = kstart; kb <= kend; kb += D) {
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+            const int k = kb + s;
+            if (k > kend) break;  // uniform over the workgroup
+            const int buf = s & 1;
+            const T ac = qa[s], ap = qa[(s + 1) % D];
+#pragma unroll
+            for (int l = 0; l < K; ++l) {
+#pragma unroll
+                for (int a = H - 1; a >= 1; --a) hist[l][a] = hist[l][a - 1];
+            }
+#pragma unroll
+            for (int a = K - 1; a >= 1; --a) wh[a] = wh[a - 1];
+            wh[0] = qw[s];
+            hist[0][0] = ac;
+            T tl[K], tll[K];
+#pragma unroll
+            for (int l = 0; l < K; ++l) {
+                tl[l] = qt[l][s];
+                tll[l] = qtl[l][s];
+            }
+            const double gl = qg[s];
+            s0[buf][ty + 1][tx + 1] = ac;
+            if (hx) s0[buf][ty + 1][sxh] = qax[s];
+            if (hy) s0[buf][syh][tx + 1] = qay[s];
+            fetch(s, k + D);
+            __syncthreads();
+            // ---- level 1: lam^{n-1} at plane k, every thread
+            {
+                const bool in1 = inner_xy && k >= klo && k <= khi;
+                T l1 = adj_update<T>(cf, ac, s0[buf][ty + 1][tx], s0[buf][ty + 1][tx + 2], s0[buf][ty][tx + 1],
+                                     s0[buf][ty + 2][tx + 1], am, ap, wh[0]);
+                if (!in1) l1 = T(0);
+                hist[1][0] = l1;
+                sl[0][buf][ty][tx] = l1;
+            }
+            // ---- levels 2..K: lam^{n-L} at plane k-(L-1) on the tile shrunk by L-1 rings
+#pragma unroll
+            for (int L = 2; L <= K; ++L) {
+                const int p = k - (L - 1);
+                const bool valid = tx >= L - 1 && tx <= TX - L && ty >= L - 1 && ty <= TY - L;
+                const bool inL = inner_xy && p >= klo && p <= khi;
+                T v = T(0);
+                if (valid && inL) {
+                    const int pbf = buf ^ 1;
+                    v = adj_update<T>(cf, hist[L - 1][1], sl[L - 2][pbf][ty][tx - 1], sl[L - 2][pbf][ty][tx + 1],
+                                      sl[L - 2][pbf][ty - 1][tx], sl[L - 2][pbf][ty + 1][tx], hist[L - 1][2],
+                                      hist[L - 1][0], wh[L - 1]);
+                }
+                if (L < K) {
+                    hist[L][0] = v;
+                    sl[L - 1][buf][ty][tx] = v;
+                } else if (out_thread && p >= k0) {
+                    // output plane: lam^{n-K} and the K gradient terms in step order
+                    if (inL) {
+                        double gv = gl;
+#pragma unroll
+                        for (int l = 0; l < K; ++l) {
+                            // lam^{n-l} at plane p: level l computed it K-1 (l = 0) or K-l iterations ago
+                            const T lv = hist[l][l == 0 ? K - 1 : K - l];
+                            double tp = (double)tl[l];
+                            if constexpr (LO) tp += (double)tll[l];
+                            gv += (double)lv * (cf.ctp * (cf.ta - tp));
+                        }
+                        bstore<double>(gv, plane_rsrc((const char*)grad + (size_t)p * pg, grange), og);
+                    }
+                    bstore<T>(v, plane_rsrc((const char*)lamo + (size_t)p * pb, range), oc);
+                }
+            }
+            am = ac;
+        }
+    }
+}
+
+// Tile: 64 x 12 threads, 32 z planes per workgroup, prefetch depth 2.
+// Measured on an MI355X (Pennes 256^3 FP32, us per adjoint step, best z
+// chunk of 8..64 planes): K = 4 64x12 64.3, 64x16 72.3; K = 3 64x8 70.4,
+// 64x16 74.1; K = 2 64x16 72.6, 64x8 74.9.  The sweeps are not HBM-bound
+// (2.9 TB/s algorithmic at K = 4): per step a level costs about what it
+// saves in traffic, so the step time changes little with K.
+constexpr int kKTX = 64, kKTY = 12, kKZC = 32, kKD = 2;
+
+template <typename T, bool LO, int K>
+int adjk_launch_t(Ctx* c, const Geo& g, const void* lam, void* lamo, const void* w, const AdjStates& st,
+                  double* grad, const AdjCoef& cf)
+{
+    constexpr int R = K - 1;
+    const int tiles_x = ceil_div(g.nx, kKTX - 2 * R), tiles_y = ceil_div(g.ny, kKTY - 2 * R);
+    const int tiles_z = ceil_div(g.nz, kKZC);
+    const long nb = (long)tiles_x * tiles_y * tiles_z;
+    if (nb > 0x7fffffffL) return fail(SCAFES_ERR_ARG, "grid too large");
+    pennes_adjk_kernel<T, LO, kKTX, kKTY, K, kKD><<<dim3((unsigned)nb), dim3(kKTX, kKTY), 0, c->stream>>>(
+        g, (const T*)lam, (T*)lamo, (const T*)w, st, grad, cf, kKZC, tiles_x, tiles_y);
+    SCAFES_HIP(hipGetLastError());
+    return SCAFES_OK;
+}
+
+template <typename T, bool LO>
+int adjk_launch_k(Ctx* c, const Geo& g, int K, const void* lam, void* lamo, const void* w, const AdjStates& st,
+                  double* grad, const AdjCoef& cf)
+{
+    switch (K) {
+    case 2: return adjk_launch_t<T, LO, 2>(c, g, lam, lamo, w, st, grad, cf);
+    case 3: return adjk_launch_t<T, LO, 3>(c, g, lam, lamo, w, st, grad, cf);
+    case 4: return adjk_launch_t<T, LO, 4>(c, g, lam, lamo, w, st, grad, cf);
+    default: return fail(SCAFES_ERR_ARG, "multi-step adjoint: K must be 2..4");
+    }
+}
+
+}  // namespace
+
+bool pennes_adjk_supported(const Geo& geo)
+{
+    // 3-D only; in-plane byte offsets of the FP64 gradient must fit 32 bits
+    return geo.dim == 3 && (geo.base + geo.e01) * 8L < (1L << 32);
+}
+
+int pennes_adjk_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, int K, const void* lam,
+                       void* lam_out, const void* w, const void* const* t, const void* const* tlo, double* grad,
+                       const scafes_pennes_params* p)
+{
+    if (!pennes_adjk_supported(geo)) return fail(SCAFES_ERR_UNSUPPORTED, "multi-step adjoint needs a 3-D grid");
+    if (K < 2 || K > kAdjMaxK) return fail(SCAFES_ERR_ARG, "multi-step adjoint: K must be 2..4");
+    if (p->border_mode != SCAFES_BORDER_KEEP)
+        return fail(SCAFES_ERR_UNSUPPORTED, "multi-step adjoint: only border_mode KEEP has an adjoint");
+    AdjStates st{};
+    for (int l = 0; l < K; ++l) {
+        if (!t[l]) return fail(SCAFES_ERR_ARG, "multi-step adjoint: missing state");
+        st.t[l] = t[l];
+        st.lo[l] = tlo ? tlo[l] : nullptr;
+        if (tlo && !tlo[l]) return fail(SCAFES_ERR_ARG, "multi-step adjoint: lo parts mismatch");
+    }
+    AdjCoef cf;
+    double hh[3];
+    for (int q = 0; q < 3; ++q) hh[q] = g->h[q] * g->h[q];
+    cf.ctd = p->tau * p->cdiff;
+    cf.ctp = p->tau * p->cperf;
+    cf.ta = p->t_a;
+    cf.hh0 = hh[0];
+    cf.hh1 = hh[1];
+    cf.hh2 = hh[2];
+    cf.fd0 = (float)(cf.ctd / hh[0]);
+    cf.fd1 = (float)(cf.ctd / hh[1]);
+    cf.fd2 = (float)(cf.ctd / hh[2]);
+    cf.fp = (float)cf.ctp;
+    if (dtype == SCAFES_F64) {
+        if (tlo) return fail(SCAFES_ERR_ARG, "multi-step adjoint: FP64 states have no lo part");
+        return adjk_launch_k<double, false>(c, geo, K, lam, lam_out, w, st, grad, cf);
+    }
+    if (tlo) return adjk_launch_k<float, true>(c, geo, K, lam, lam_out, w, st, grad, cf);
+    return adjk_launch_k<float, false>(c, geo, K, lam, lam_out, w, st, grad, cf);
+}
+
+}  // namespace scafes

@@ -95,12 +95,14 @@ int pennes_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, c
                       void* lam_new, const void* w, const void* tprev, double* grad,
                       const scafes_pennes_params* p, const void* tprev_lo);
 
-// Two reverse Pennes adjoint steps per sweep (defined in pennes_adj2.hip):
-// lam^n -> lam^{n-2} with T^{n-1} = t1 (+ t1lo), T^{n-2} = t2 (+ t2lo).
-bool pennes_adj2_supported(const Geo& geo);
-int pennes_adj2_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, const void* lam, void* lam_out,
-                       const void* w, const void* t1, const void* t1lo, const void* t2, const void* t2lo,
-                       double* grad, const scafes_pennes_params* p);
+// K = 2..kAdjMaxK reverse Pennes adjoint steps per sweep (defined in
+// pennes_adjk.hip): lam^n -> lam^{n-K} with t[l] = T^{n-1-l} (+ tlo[l] for
+// compensated states, else tlo = nullptr), l = 0..K-1.
+constexpr int kAdjMaxK = 4;
+bool pennes_adjk_supported(const Geo& geo);
+int pennes_adjk_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, int K, const void* lam,
+                       void* lam_out, const void* w, const void* const* t, const void* const* tlo, double* grad,
+                       const scafes_pennes_params* p);
 
 // Multi-step (two or three steps per sweep) variable-kappa heat (defined in
 // heatvar_tb2.hip); heatvar_tb2_launch advances the configuration's steps.

@@ -523,10 +523,21 @@ static int launch_t(Ctx* c, const Op& op, const Geo& g, int zc)
     return SCAFES_OK;
 }
 
+// Tile of the one-step launches that take no configuration (heat7, Pennes
+// forward and adjoint, MRI): 128 x 2 threads, 8 z planes per workgroup.
+// Short z chunks give enough workgroups to fill 256 CUs on small grids and
+// cost one extra halo plane per 4 streamed ones, served from L2.  Measured on
+// an MI355X (us per step, this tile vs the best of 15 tiles x 5 chunk
+// lengths): Pennes FP32 256^3 50.5 (best), 512^3 482 (best); Pennes FP64
+// 256^3 112 vs 109; MRI FP64 256^3 200 vs 192, 120x120x50 16.0 (best); the
+// previous 64 x 8, 128-plane default took 87, 515, 127, 263 and 102.
+constexpr int kAutoZc = 8;
+
 template <class Op, typename T>
 static int launch(Ctx* c, const Op& op, const Geo& g, int cfg)
 {
-    if (cfg < 0 || cfg >= kNumCfgs) cfg = kDefaultCfg;
+    if (cfg < 0) return launch_t<Op, T, 128, 2, 4>(c, op, g, kAutoZc);
+    if (cfg >= kNumCfgs) cfg = kDefaultCfg;
     const TileCfg t = kCfgs[cfg];
     switch (t.tx * 100000 + t.ty * 100 + t.d) {
     case 6400404: return launch_t<Op, T, 64, 4, 4>(c, op, g, t.zc);

@@ -72,11 +72,13 @@ def test_checkpointed_gradient_bitexact(ctx, snaps):
     assert stats[1] == N
 
 
-def test_two_step_reverse_sweep_bitexact_ragged(ctx):
-    """Store-all budget: the reverse sweep runs two adjoint steps per launch
-    over several z chunks and a tile grid that does not divide the box; odd
-    step count (one single step at the end of the pairs)."""
-    n, N = 70, 9
+@pytest.mark.parametrize("N", [9, 8, 7, 6])
+def test_multi_step_reverse_sweep_bitexact_ragged(ctx, N):
+    """Store-all budget: after the first (single) step the reverse sweep runs
+    up to four adjoint steps per launch over several z chunks and a tile grid
+    that does not divide the box.  N - 1 = 8, 7, 6, 5 steps give sweeps of
+    4+4, 4+3, 4+2 and 4 then a single step."""
+    n = 70
     g, w, q, coef, T0, target = setup(n, N, seed=5)
     st = O.pennes_forward_all(g, T0, w, q, coef, N)
     Jref, gref = O.pennes_grad(g, st, w, target, coef, N)
@@ -88,10 +90,11 @@ def test_two_step_reverse_sweep_bitexact_ragged(ctx):
 
 
 @pytest.mark.parametrize("mode", [capi.F32, capi.F32_KAHAN])
-def test_fp32_two_step_reverse_sweep_matches_one_step(ctx, mode):
+def test_fp32_multi_step_reverse_sweep_matches_one_step(ctx, mode):
     """FP32 paths: with no snapshots every adjoint step runs alone; with
-    store-all they run in pairs.  The forward recomputation is deterministic,
-    so both schedules must give identical gradient bits."""
+    store-all they run up to four per sweep (here 4 + 3).  The forward
+    recomputation is deterministic, so both schedules must give identical
+    gradient bits."""
     n, N = 70, 8
     g, w, q, coef, T0, target = setup(n, N, seed=6)
     J0, g0, s0 = device_grad(ctx, n, mode, T0, w, q, target, coef, N, 0, shift=37.0)
