@@ -157,6 +157,100 @@ def cpu_baseline(u_host, k_host, tau, n, budget_s=12.0):
                       f"{n}^3 input, {steps} steps in {dt:.1f} s"}
 
 
+def _oracle(threads=None):
+    """The oracle (restated reference CPU path) for the CPU baselines."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    threads = threads or min(os.cpu_count() or 1, 16)
+    O.lib().oracle_set_threads(threads)
+    return O, threads
+
+
+def _timed(step, budget_s, cap=200):
+    """Steps of `step` that fit a budget of about budget_s seconds: (count, seconds)."""
+    t0 = time.perf_counter()
+    step()
+    t1 = time.perf_counter() - t0
+    k = max(1, min(cap, int(budget_s / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    for _ in range(k):
+        step()
+    return k, time.perf_counter() - t0
+
+
+def cpu_yee(ncell=384, budget_s=5.0):
+    """Oracle FP64 Yee leapfrog steps (OpenMP) on the same ncell^3 box."""
+    O, threads = _oracle()
+    f = O.yee_alloc(ncell)
+    m = ncell // 2
+    f[2, m, m, m] = 1.0
+    k, dt = _timed(lambda: O.lib().oracle_yee_step(ncell, 0.5, O.ptr(f)), budget_s)
+    del f
+    return {"value": round(float(ncell) ** 3 * k / dt / 1e9, 4), "unit": "Gcell-steps/s", "cores": threads,
+            "kind": "port", "sample": f"oracle FP64 Yee leapfrog step (OpenMP) on the same {ncell}^3 cells, "
+                                      f"{k} steps in {dt:.1f} s"}
+
+
+def cpu_pennes(n=256, budget_s=5.0):
+    """Oracle FP64 Pennes forward steps (OpenMP) on the same n^3 workload."""
+    from masterthesis_amd.workloads import pennes_workload
+    O, threads = _oracle()
+    wl = pennes_workload(n, seed=0)
+    g = O.make_grid((n, n, n))
+    coef = O.pennes_coef(wl["tau"], 0.5, 1.0, 1.0, 0.0)
+    a = np.zeros(O.memall_shape(g))
+    b = np.zeros_like(a)
+    w, q = (np.ascontiguousarray(wl[k].reshape(a.shape)) for k in ("w", "q"))
+    st = [a, b]
+
+    def step():
+        O.lib().oracle_pennes_step(O.C.byref(g), O.ptr(st[0]), O.ptr(st[1]), O.ptr(w), O.ptr(q), O.C.byref(coef))
+        st.reverse()
+
+    k, dt = _timed(step, budget_s)
+    return {"value": round(float(n) ** 3 * k / dt / 1e9, 4), "unit": "GLUPS", "cores": threads, "kind": "port",
+            "sample": f"oracle FP64 Pennes forward step (OpenMP) on the same {n}^3 input, {k} steps in {dt:.1f} s"}
+
+
+def cpu_batch(ens, nsteps, nsample=256):
+    """Oracle FP64 2-D heat runs of the first nsample ensemble members (one
+    problem per thread)."""
+    from concurrent.futures import ThreadPoolExecutor
+    O, threads = _oracle(1)
+    L = O.lib()
+    members = []
+    for i in range(min(nsample, len(ens["n"]))):
+        m, o = int(ens["n"][i]), int(ens["off"][i])
+        u = ens["u0"][o:o + m * m].astype(np.float64)
+        members.append((m, float(ens["kappa"][i]), float(ens["tau"][i]), u, np.empty_like(u)))
+    pool = min(os.cpu_count() or 1, 16)
+
+    def run(mem):
+        m, kap, tau, u, up = mem
+        L.oracle_heat2d_run(m, 1.0 / (m - 1), kap, tau, nsteps, O.ptr(u), O.ptr(up))
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(pool) as ex:
+        list(ex.map(run, members))
+    dt = time.perf_counter() - t0
+    lups = sum(m * m for m, *_ in members) * float(nsteps)
+    return {"value": round(lups / dt / 1e9, 4), "unit": "GLUPS", "cores": pool, "kind": "port",
+            "sample": f"oracle FP64 2-D heat runs ({nsteps} steps) of the first {len(members)} of the "
+                      f"{len(ens['n'])} problems, one problem per thread, {dt:.1f} s"}
+
+
+def cpu_emfdtd(n=256, budget_s=6.0):
+    """Oracle FP64 EMFDTD steps (serial) on the same n^3 case."""
+    from masterthesis_amd.workloads import sphere_sigma
+    O, _ = _oracle()
+    ref = O.Emfdtd(n, d=10, src=(25, 25, 25), sink=(30, 30, 30), sigma=sphere_sigma(n, (n / 2, n / 2, n / 2), 50.0))
+    k, dt = _timed(lambda: ref.step(0.0), budget_s, cap=20)
+    del ref
+    return {"value": round(float(n) ** 3 * k / dt / 1e9, 4), "unit": "Gnode-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle FP64 EMFDTD leapfrog step (serial, as the reference's hooks run per rank) on the "
+                      f"same {n}^3 case, {k} steps in {dt:.1f} s"}
+
+
 def bench_yee(ctx, capi, ncell=384, nsteps=2000, energy_every=100):
     """Yee FDTD, PEC box of ncell^3 cells, vacuum, Courant 0.5, Gaussian Ez
     pulse (sigma = 8 cells) at the centre, FP32 fields, FP64 energy."""
@@ -506,17 +600,29 @@ def main():
     if args.calibrate_copy:
         out["copy_calibration"] = calibrate_copy(ctx, capi)
         log(f"[bench] copy {out['copy_calibration']}")
+    # each leg: the GPU measurement, then (unless --no-cpu) the oracle timed
+    # on a bounded sample of the same workload on the host cores
     if not args.no_yee:
         out["yee"] = bench_yee(ctx, capi, **({"nsteps": args.leg_steps} if args.leg_steps else {}))
+        if not args.no_cpu:
+            out["yee"]["cpu_baseline"] = cpu_yee()
         log(f"[bench] yee {out['yee']}")
     if not args.no_pennes:
         out["pennes_adjoint"] = bench_pennes(ctx, capi, **({"nsteps": args.leg_steps} if args.leg_steps else {}))
+        if not args.no_cpu:
+            out["pennes_adjoint"]["cpu_baseline"] = cpu_pennes()
         log(f"[bench] pennes {out['pennes_adjoint']}")
     if not args.no_batch:
         out["batch2d"] = bench_batch(ctx, capi, **({"nsteps": args.leg_steps} if args.leg_steps else {}))
+        if not args.no_cpu:
+            from masterthesis_amd.workloads import batch2d_ensemble
+            nb = args.leg_steps or 500
+            out["batch2d"]["cpu_baseline"] = cpu_batch(batch2d_ensemble(4096, nb, seed=0), nb)
         log(f"[bench] batch {out['batch2d']}")
     if not args.no_emfdtd:
         out["emfdtd"] = bench_emfdtd(ctx, capi, **({"nsteps": args.leg_steps} if args.leg_steps else {}))
+        if not args.no_cpu:
+            out["emfdtd"]["cpu_baseline"] = cpu_emfdtd()
         log(f"[bench] emfdtd {out['emfdtd']}")
     ctx.close()
     print(json.dumps(out), flush=True)
