@@ -169,24 +169,23 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb2_kernel(Geo g, const T* __r
     T qa[D], qb[D], qc[D], qax[D], qbx[D], qay[D], qby[D];
 #pragma unroll
     for (int s = 0; s < D; ++s) qa[s] = qb[s] = qc[s] = qax[s] = qbx[s] = qay[s] = qby[s] = T(0);
+    // Every iteration issues the same loads and stores; unneeded ones use
+    // offset `range` (out of range: loads read 0 without a memory access,
+    // stores are dropped), so the compiler can wait for exactly the slot a
+    // plane consumes instead of for every outstanding access.
     auto fetch = [&](const int slot, const int kk) {
-        if (kk <= kcen) {
-            const size_t o = plane(kk);
-            const Rsrc ru = plane_rsrc(ub + o, range), rk = plane_rsrc(kb8 + o, range);
-            qa[slot] = bload<T>(ru, oc);
-            qb[slot] = bload<T>(rk, oc);
-            if (kk <= k1) {
-                if constexpr (KH) qc[slot] = bload<T>(plane_rsrc(cb + o, range), oc);
-                if (hx) {
-                    qax[slot] = bload<T>(ru, ohx);
-                    qbx[slot] = bload<T>(rk, ohx);
-                }
-                if (hy) {
-                    qay[slot] = bload<T>(ru, ohy);
-                    qby[slot] = bload<T>(rk, ohy);
-                }
-            }
-        }
+        const size_t o = plane(kk);
+        const Rsrc ru = plane_rsrc(ub + o, range), rk = plane_rsrc(kb8 + o, range);
+        const bool cen = kk <= kcen, mid = kk <= k1;
+        const uint32_t occ = cen ? oc : range, ocm = mid ? oc : range;
+        const uint32_t ox = (mid && hx) ? ohx : range, oy = (mid && hy) ? ohy : range;
+        qa[slot] = bload<T>(ru, occ);
+        qb[slot] = bload<T>(rk, occ);
+        if constexpr (KH) qc[slot] = bload<T>(plane_rsrc(cb + o, range), ocm);
+        qax[slot] = bload<T>(ru, ox);
+        qbx[slot] = bload<T>(rk, ox);
+        qay[slot] = bload<T>(ru, oy);
+        qby[slot] = bload<T>(rk, oy);
     };
     T am = bload<T>(plane_rsrc(ub + plane(kstart - 1), range), oc);
     const T bm = bload<T>(plane_rsrc(kb8 + plane(kstart - 1), range), oc);
@@ -237,18 +236,22 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb2_kernel(Geo g, const T* __r
             s1[buf][ty][tx] = l1;
             // ---- level 2 at (i, j, k-1) from the level-1 plane of the previous iteration
             const int k2 = k - 1;
-            if (k2 >= k0 && out_thread) {
-                const int qb1 = buf ^ 1;
-                T cv;
-                T v = var_update<T, MODE>(cf, l1m1, s1[qb1][ty][tx - 1], s1[qb1][ty][tx + 1], s1[qb1][ty - 1][tx],
-                                          s1[qb1][ty + 1][tx], l1m2, l1, fp, c1m1, &cv);
-                if (!(inner_xy && k2 >= klo && k2 <= khi)) {
-                    v = keep ? l1m1 : bv;
-                    cv = keep ? c1m1 : T(0);
+            {
+                const bool st = k2 >= k0 && out_thread;
+                T v = T(0), cv = T(0);
+                if (st) {
+                    const int qb1 = buf ^ 1;
+                    v = var_update<T, MODE>(cf, l1m1, s1[qb1][ty][tx - 1], s1[qb1][ty][tx + 1], s1[qb1][ty - 1][tx],
+                                            s1[qb1][ty + 1][tx], l1m2, l1, fp, c1m1, &cv);
+                    if (!(inner_xy && k2 >= klo && k2 <= khi)) {
+                        v = keep ? l1m1 : bv;
+                        cv = keep ? c1m1 : T(0);
+                    }
                 }
-                const size_t o = (size_t)k2 * pb;
-                bstore<T>(v, plane_rsrc((const char*)uout + o, range), oo);
-                if constexpr (KH) bstore<T>(cv, plane_rsrc((const char*)cout + o, range), oo);
+                const size_t o = plane(k2);
+                const uint32_t os = st ? oo : range;
+                bstore<T>(v, plane_rsrc((const char*)uout + o, range), os);
+                if constexpr (KH) bstore<T>(cv, plane_rsrc((const char*)cout + o, range), os);
             }
             l1m2 = l1m1;
             l1m1 = l1;
@@ -321,24 +324,25 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb3_kernel(Geo g, const T* __r
     T qa[D], qb[D], qc[D], qax[D], qbx[D], qay[D], qby[D];
 #pragma unroll
     for (int s = 0; s < D; ++s) qa[s] = qb[s] = qc[s] = qax[s] = qbx[s] = qay[s] = qby[s] = T(0);
+    // Every iteration issues the same loads (and stores, below): a load that
+    // is not needed reads at offset `range`, which the buffer unit answers
+    // with 0 without a memory access.  With a fixed count of memory
+    // instructions per plane the compiler waits only for the slot a plane
+    // consumes (vmcnt(n) instead of vmcnt(0)), so the prefetch ring hides the
+    // latency of the loads issued D planes ahead.
     auto fetch = [&](const int slot, const int kk) {
-        if (kk <= kcen) {
-            const size_t o = plane(kk);
-            const Rsrc ru = plane_rsrc(ub + o, range), rk = plane_rsrc(kb8 + o, range);
-            qa[slot] = bload<T>(ru, oc);
-            qb[slot] = bload<T>(rk, oc);
-            if (kk <= klast) {
-                if constexpr (KH) qc[slot] = bload<T>(plane_rsrc(cb + o, range), oc);
-                if (hx) {
-                    qax[slot] = bload<T>(ru, ohx);
-                    qbx[slot] = bload<T>(rk, ohx);
-                }
-                if (hy) {
-                    qay[slot] = bload<T>(ru, ohy);
-                    qby[slot] = bload<T>(rk, ohy);
-                }
-            }
-        }
+        const size_t o = plane(kk);
+        const Rsrc ru = plane_rsrc(ub + o, range), rk = plane_rsrc(kb8 + o, range);
+        const bool cen = kk <= kcen, mid = kk <= klast;
+        const uint32_t occ = cen ? oc : range, ocm = mid ? oc : range;
+        const uint32_t ox = (mid && hx) ? ohx : range, oy = (mid && hy) ? ohy : range;
+        qa[slot] = bload<T>(ru, occ);
+        qb[slot] = bload<T>(rk, occ);
+        if constexpr (KH) qc[slot] = bload<T>(plane_rsrc(cb + o, range), ocm);
+        qax[slot] = bload<T>(ru, ox);
+        qbx[slot] = bload<T>(rk, ox);
+        qay[slot] = bload<T>(ru, oy);
+        qby[slot] = bload<T>(rk, oy);
     };
     T am = bload<T>(plane_rsrc(ub + plane(kfirst - 1), range), oc);
     const T bm = bload<T>(plane_rsrc(kb8 + plane(kfirst - 1), range), oc);
@@ -351,12 +355,17 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb3_kernel(Geo g, const T* __r
     T fup = face<T, MODE>(qb[0], bm);
     const T bv = (T)cf.border_value;
     const bool keep = cf.border_mode != SCAFES_BORDER_VALUE;
-    for (int kb = kfirst; kb <= klast; kb += D) {
+    // The z queues below (faces, level values) shift every plane with period
+    // 3; a body of lcm(D, 3) planes (6 at D = 2) lets the register allocator
+    // rename instead of move.  s is the prefetch slot, buf the LDS parity.
+    constexpr int U = (D == 2) ? 6 : D;
+    for (int kb = kfirst; kb <= klast; kb += U) {
 #pragma unroll
-        for (int s = 0; s < D; ++s) {
-            const int k = kb + s;
+        for (int uu = 0; uu < U; ++uu) {
+            const int s = uu % D;
+            const int k = kb + uu;
             if (k > klast) break;  // uniform over the workgroup
-            const int buf = s & 1, pbf = buf ^ 1;
+            const int buf = uu & 1, pbf = buf ^ 1;
             const T ac = qa[s], ap = qa[(s + 1) % D];
             const T bc = qb[s], bp = qb[(s + 1) % D];
             const T c0 = qc[s];
@@ -400,17 +409,23 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb3_kernel(Geo g, const T* __r
             s2[buf][ty][tx] = l2;
             // ---- level 3 at plane k-2 from the level-2 plane written one iteration ago
             const int k3 = k - 2;
-            if (k3 >= k0 && k3 < k1 && out_thread) {
-                T c3;
-                T l3 = var_update<T, MODE>(cf, l2m1, s2[pbf][ty][tx - 1], s2[pbf][ty][tx + 1], s2[pbf][ty - 1][tx],
-                                           s2[pbf][ty + 1][tx], l2m2, l2, fq2, c2m1, &c3);
-                if (!(inner_xy && k3 >= klo && k3 <= khi)) {
-                    l3 = keep ? l2m1 : bv;
-                    c3 = keep ? c2m1 : T(0);
+            {
+                // computed by output threads on output planes; the store is
+                // issued by every thread (out of range elsewhere, so dropped)
+                const bool st = k3 >= k0 && k3 < k1 && out_thread;
+                T l3 = T(0), c3 = T(0);
+                if (st) {
+                    l3 = var_update<T, MODE>(cf, l2m1, s2[pbf][ty][tx - 1], s2[pbf][ty][tx + 1], s2[pbf][ty - 1][tx],
+                                             s2[pbf][ty + 1][tx], l2m2, l2, fq2, c2m1, &c3);
+                    if (!(inner_xy && k3 >= klo && k3 <= khi)) {
+                        l3 = keep ? l2m1 : bv;
+                        c3 = keep ? c2m1 : T(0);
+                    }
                 }
-                const size_t o = (size_t)k3 * pb;
-                bstore<T>(l3, plane_rsrc((const char*)uout + o, range), oo);
-                if constexpr (KH) bstore<T>(c3, plane_rsrc((const char*)cout + o, range), oo);
+                const size_t o = plane(k3);
+                const uint32_t os = st ? oo : range;
+                bstore<T>(l3, plane_rsrc((const char*)uout + o, range), os);
+                if constexpr (KH) bstore<T>(c3, plane_rsrc((const char*)cout + o, range), os);
             }
             l2m2 = l2m1;
             l2m1 = l2;
@@ -504,7 +519,7 @@ bool heatvar_tb2_supported(const Geo& geo, int mode)
 {
     // 3-D only; in-plane byte offsets (ghost planes included) must fit 32 bits
     const long bytes = mode == SCAFES_HEATVAR_F64 ? 8 : 4;
-    return geo.dim == 3 && (geo.base + geo.e01) * bytes < (1L << 32);
+    return geo.dim == 3 && (geo.base + geo.e01) * bytes < kOobLimit;
 }
 
 // The configuration's number of steps u -> uo (and Kahan compensation cin -> co).

@@ -70,6 +70,13 @@ __device__ inline bool is_border(const Geo& g, int i, int j, int k)
 // Maps the hardware dispatch order (round-robin over the 8 XCDs of an
 // MI355X) to a logical tile order in which each XCD owns a contiguous run
 // of tiles, so y-neighbouring tiles share that XCD's L2.
+// Largest plane range in bytes of the buffer-resource kernels.  They issue
+// the same loads and stores on every iteration and send unneeded ones to the
+// out-of-range offset `range` (a load there returns 0 without a memory
+// access, a store is dropped), so the compiler can count outstanding
+// accesses and wait only for the ones a plane consumes.
+constexpr long kOobLimit = 0xFFFFFF00L;
+
 __device__ inline int xcd_remap(int b, int nb)
 {
     constexpr int kXcd = 8;

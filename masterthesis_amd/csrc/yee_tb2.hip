@@ -132,25 +132,25 @@ __global__ __launch_bounds__(TX* TY) void yee_tb2_kernel(YeeDims d, T s, const T
     for (int q = 0; q < D; ++q) qex[q] = qey[q] = qez[q] = qhx[q] = qhy[q] = qhz[q] = rxy[q] = rxz[q] = ryx[q] =
         ryz[q] = T(0);
     const int klast = k1 + 3;  // last plane of E^n read
+    // Every iteration issues the same loads and stores; unneeded ones use
+    // offset `range` (out of range: loads read 0 without a memory access,
+    // stores are dropped), so the compiler can wait for exactly the slot a
+    // plane consumes instead of for every outstanding access.
     auto fetch = [&](const int slot, const int kk) {
-        if (kk <= klast) {
-            const char* p = a8 + plane(kk);
-            const Rsrc rEx = rsrc(p, range), rEy = rsrc(p + fs, range), rEz = rsrc(p + 2 * fs, range);
-            qex[slot] = bld<T>(rEx, oc);
-            qey[slot] = bld<T>(rEy, oc);
-            qez[slot] = bld<T>(rEz, oc);
-            qhx[slot] = bld<T>(rsrc(p + 3 * fs, range), oc);
-            qhy[slot] = bld<T>(rsrc(p + 4 * fs, range), oc);
-            qhz[slot] = bld<T>(rsrc(p + 5 * fs, range), oc);
-            if (rx) {
-                rxy[slot] = bld<T>(rEy, ox);
-                rxz[slot] = bld<T>(rEz, ox);
-            }
-            if (ry) {
-                ryx[slot] = bld<T>(rEx, oy);
-                ryz[slot] = bld<T>(rEz, oy);
-            }
-        }
+        const char* p = a8 + plane(kk);
+        const Rsrc rEx = rsrc(p, range), rEy = rsrc(p + fs, range), rEz = rsrc(p + 2 * fs, range);
+        const bool in = kk <= klast;
+        const uint32_t o0 = in ? oc : range, o1 = (in && rx) ? ox : range, o2 = (in && ry) ? oy : range;
+        qex[slot] = bld<T>(rEx, o0);
+        qey[slot] = bld<T>(rEy, o0);
+        qez[slot] = bld<T>(rEz, o0);
+        qhx[slot] = bld<T>(rsrc(p + 3 * fs, range), o0);
+        qhy[slot] = bld<T>(rsrc(p + 4 * fs, range), o0);
+        qhz[slot] = bld<T>(rsrc(p + 5 * fs, range), o0);
+        rxy[slot] = bld<T>(rEy, o1);
+        rxz[slot] = bld<T>(rEz, o1);
+        ryx[slot] = bld<T>(rEx, o2);
+        ryz[slot] = bld<T>(rEz, o2);
     };
     const int kfirst = k0 - 2;
 #pragma unroll
@@ -207,24 +207,30 @@ __global__ __launch_bounds__(TX* TY) void yee_tb2_kernel(YeeDims d, T s, const T
             L.h2[0][buf][ty][tx] = h2.x;
             L.h2[1][buf][ty][tx] = h2.y;
             L.h2[2][buf][ty][tx] = h2.z;
-            if (out && kc >= k0 && kc < k1) {
-                const size_t o = (size_t)kc * pb;
-                bst<T>(h2.x, rsrc(b8 + o + 3 * fs, range), oc);
-                bst<T>(h2.y, rsrc(b8 + o + 4 * fs, range), oc);
-                bst<T>(h2.z, rsrc(b8 + o + 5 * fs, range), oc);
-                if (ep1)
+            {
+                const bool st = out && kc >= k0 && kc < k1;
+                const size_t o = plane(kc);
+                const uint32_t os = st ? oc : range;
+                bst<T>(h2.x, rsrc(b8 + o + 3 * fs, range), os);
+                bst<T>(h2.y, rsrc(b8 + o + 4 * fs, range), os);
+                bst<T>(h2.z, rsrc(b8 + o + 5 * fs, range), os);
+                if (st && ep1)
                     es1 += 0.5 * ((double)e1m1.x * e1m1.x + (double)e1m1.y * e1m1.y + (double)e1m1.z * e1m1.z) +
                            0.5 * ((double)h1m2.x * h2.x + (double)h1m2.y * h2.y + (double)h1m2.z * h2.z);
             }
             // ---- D: E2 at plane kk-3 (H2 plane kk-3 from the previous iteration)
             const int kd = kk - 3;
-            if (out && kd >= k0 && kd < k1) {
-                const V3<T> e2 = e_update<T>(N, i, j, kd, s, e1m2, h2m1, h2m2x, h2m2y, L.h2[2][pbf][tym][tx],
-                                             L.h2[0][pbf][tym][tx], L.h2[2][pbf][ty][txm], L.h2[1][pbf][ty][txm]);
-                const size_t o = (size_t)kd * pb;
-                bst<T>(e2.x, rsrc(b8 + o, range), oc);
-                bst<T>(e2.y, rsrc(b8 + o + fs, range), oc);
-                bst<T>(e2.z, rsrc(b8 + o + 2 * fs, range), oc);
+            {
+                const bool st = out && kd >= k0 && kd < k1;
+                V3<T> e2 = {};
+                if (st)
+                    e2 = e_update<T>(N, i, j, kd, s, e1m2, h2m1, h2m2x, h2m2y, L.h2[2][pbf][tym][tx],
+                                     L.h2[0][pbf][tym][tx], L.h2[2][pbf][ty][txm], L.h2[1][pbf][ty][txm]);
+                const size_t o = plane(kd);
+                const uint32_t os = st ? oc : range;
+                bst<T>(e2.x, rsrc(b8 + o, range), os);
+                bst<T>(e2.y, rsrc(b8 + o + fs, range), os);
+                bst<T>(e2.z, rsrc(b8 + o + 2 * fs, range), os);
             }
             // ---- shift the z queues
             h2m2x = h2m1.x;
@@ -274,7 +280,7 @@ struct TileOf {
 
 }  // namespace
 
-bool yee_tb2_supported(const YeeDims& d, int elem_bytes) { return d.M2 * (long)elem_bytes < (1L << 32); }
+bool yee_tb2_supported(const YeeDims& d, int elem_bytes) { return d.M2 * (long)elem_bytes < kOobLimit; }
 
 // Work split: one workgroup per xy tile and z chunk; the chunk count is the
 // smallest that keeps every CU busy for about the same time (a chunk costs
