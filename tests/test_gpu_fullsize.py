@@ -1,0 +1,308 @@
+"""Parity at the full workload sizes, directly where the oracle finishes in
+seconds and otherwise through properties that do not depend on the size:
+
+ * var-kappa heat 512^3 (the headline workload's input): six FP64 steps
+   bit-identical to the oracle; the default multi-step sweeps (three steps per
+   launch for FP32 and FP32+Kahan) bit-identical to one-step launches over 7
+   steps (two three-step launches and a remainder); FP32+Kahan within 1e-6
+   (relative to the field's maximum) of FP64 after 30 steps.
+ * Yee 384^3: two FP64 leapfrog steps of the centred pulse bit-identical to
+   the oracle; the FP32 two-steps-per-sweep path keeps the discrete energy to
+   1e-6 relative over 100 steps.
+ * Pennes 256^3: the FP32 gradient with 10 snapshots and with store-all (99)
+   bit-identical (checkpointing changes the schedule, not the arithmetic);
+   the FP64 gradient against a central finite difference of J along a random
+   direction (relative 1e-6).
+ * The 4096-problem 2-D batch, 500 steps: hipGraph replay identical to the
+   direct launch; 8 sampled members within FP32 tolerance of the oracle.
+ * EMFDTD 256^3 (the bench case, CPML d = 10, conductive sphere): three
+   steps bit-identical to the oracle.
+ * The MRI example on the reference's 120 x 120 x 50 brain grid: 20 steps
+   bit-identical to the oracle.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from masterthesis_amd import capi
+from masterthesis_amd.workloads import batch2d_ensemble, pennes_workload
+
+pytestmark = pytest.mark.gpu
+N3 = 512
+
+
+@pytest.fixture(scope="module")
+def ctx(hip):
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def heat512(ctx):
+    g = capi.geom((N3, N3, N3))
+    ne = capi.memall_count(g, 1)
+    u0, k0 = ctx.alloc(ne), ctx.alloc(ne)
+    u0.fill(0.0)
+    k0.fill(0.0)
+    capi.check(ctx.L.scafes_heat7var_init(ctx.h, C.byref(g), 1, capi.F64, u0.ptr, k0.ptr, 0, 1e-3))
+    h = 1.0 / (N3 - 1)
+    tau = 0.9 * h * h / (6.0 * 1.5)
+    yield g, ne, u0, k0, tau
+    u0.free()
+    k0.free()
+
+
+def run_heat(ctx, g, ne, u0, k0, tau, mode, nsteps, cfg=-1):
+    """(u, c) after nsteps from the FP64 initial state converted to the mode's type."""
+    dt = np.float64 if mode == capi.HEATVAR_F64 else np.float32
+    code = capi.F64 if mode == capi.HEATVAR_F64 else capi.F32
+    a, b, k = ctx.alloc(ne, dt), ctx.alloc(ne, dt), ctx.alloc(ne, dt)
+    for d, src in ((a, u0), (b, u0), (k, k0)):
+        capi.check(ctx.L.scafes_convert(ctx.h, d.ptr, code, src.ptr, capi.F64, ne))
+    c1 = c2 = None
+    if mode == capi.HEATVAR_F32_KAHAN:
+        c1, c2 = ctx.alloc(ne, np.float32), ctx.alloc(ne, np.float32)
+        c1.fill(0.0)
+        c2.fill(0.0)
+    which = C.c_int(0)
+    capi.check(ctx.L.scafes_heat7var_run(ctx.h, C.byref(g), 1, mode, a.ptr, b.ptr, c1.ptr if c1 else None,
+                                         c2.ptr if c2 else None, k.ptr, tau, nsteps, capi.BORDER_VALUE, 0.0, cfg,
+                                         C.byref(which)))
+    u = (b if which.value else a).read()
+    c = ((c2 if which.value else c1).read() if c1 is not None else None)
+    for d in (a, b, k, c1, c2):
+        if d is not None:
+            d.free()
+    return u, c
+
+
+def one_step_cfg(ctx):
+    st, tx, ty, zc, d = (C.c_int() for _ in range(5))
+    for cfg in range(ctx.L.scafes_heat7var_num_cfgs()):
+        capi.check(ctx.L.scafes_heat7var_cfg_desc(cfg, C.byref(tx), C.byref(ty), C.byref(zc), C.byref(d), C.byref(st)))
+        if st.value == 1:
+            return cfg
+    raise AssertionError("no one-step configuration")
+
+
+def test_heat512_fp64_matches_oracle(ctx, heat512):
+    g, ne, u0, k0, tau = heat512
+    u, _ = run_heat(ctx, g, ne, u0, k0, tau, capi.HEATVAR_F64, 6)
+    og = O.make_grid((N3, N3, N3))
+    O.lib().oracle_set_threads(16)
+    ref = O.heat7var_run(og, u0.read().reshape(O.memall_shape(og)), k0.read().reshape(O.memall_shape(og)), tau, 6,
+                         border=O.BORDER_VALUE, border_value=0.0)
+    np.testing.assert_array_equal(O.nodes_view(og, u.reshape(ref.shape)), O.nodes_view(og, ref))
+
+
+@pytest.mark.parametrize("mode", [capi.HEATVAR_F32_KAHAN, capi.HEATVAR_F32])
+def test_heat512_multistep_equals_one_step(ctx, heat512, mode):
+    g, ne, u0, k0, tau = heat512
+    um, cm = run_heat(ctx, g, ne, u0, k0, tau, mode, 7)
+    u1, c1 = run_heat(ctx, g, ne, u0, k0, tau, mode, 7, cfg=one_step_cfg(ctx))
+    np.testing.assert_array_equal(um, u1)
+    if cm is not None:
+        np.testing.assert_array_equal(cm, c1)
+    assert np.abs(um - u0.read().astype(np.float32)).max() > 1e-6  # the state moved
+
+
+def test_heat512_kahan_close_to_fp64(ctx, heat512):
+    g, ne, u0, k0, tau = heat512
+    u64, _ = run_heat(ctx, g, ne, u0, k0, tau, capi.HEATVAR_F64, 30)
+    uk, ck = run_heat(ctx, g, ne, u0, k0, tau, capi.HEATVAR_F32_KAHAN, 30)
+    err = np.abs(uk.astype(np.float64) + ck.astype(np.float64) - u64).max() / np.abs(u64).max()
+    assert err < 1e-6, err
+
+
+def test_yee384_fp64_matches_oracle_and_fp32_energy(ctx):
+    N = 384
+    nf = (N + 1) ** 3
+    f = ctx.alloc(6 * nf, np.float64)
+    f2 = ctx.alloc(6 * nf, np.float64)
+    capi.check(ctx.L.scafes_yee_init_pulse(ctx.h, N, capi.F64, f.ptr, 8.0))
+    f0 = f.read().reshape(6, N + 1, N + 1, N + 1)
+    second = C.c_int(-1)
+    capi.check(ctx.L.scafes_yee_run(ctx.h, N, capi.F64, 0.5, f.ptr, f2.ptr, 2, 0, None, C.byref(second)))
+    got = (f2 if second.value else f).read().reshape(f0.shape)
+    O.lib().oracle_set_threads(16)
+    ref = O.yee_step(N, 0.5, f0.copy(), 2)
+    np.testing.assert_array_equal(got, ref)
+    f.free()
+    f2.free()
+    g32, g32b = ctx.alloc(6 * nf, np.float32), ctx.alloc(6 * nf, np.float32)
+    capi.check(ctx.L.scafes_yee_init_pulse(ctx.h, N, capi.F32, g32.ptr, 8.0))
+    e = np.zeros(10)
+    capi.check(ctx.L.scafes_yee_run(ctx.h, N, capi.F32, 0.5, g32.ptr, g32b.ptr, 100, 10,
+                                    e.ctypes.data_as(C.POINTER(C.c_double)), C.byref(second)))
+    g32.free()
+    g32b.free()
+    assert e[0] > 0 and np.abs(e - e[0]).max() / e[0] < 1e-6, e
+
+
+def _pennes_device(ctx, n, dt):
+    wl = pennes_workload(n, seed=0)
+    g = capi.geom((n, n, n))
+    ne = capi.memall_count(g, 1)
+    w = ctx.upload(wl["w"].astype(dt))
+    q = ctx.upload(wl["q"].astype(dt))
+    pp = capi.PennesParams(tau=wl["tau"], cdiff=0.5, cperf=1.0, csrc=1.0, t_a=0.0)
+    return wl, g, ne, w, q, pp
+
+
+def _forward(ctx, g, ne, dt, code, w, q, pp, nsteps):
+    a, b = ctx.alloc(ne, dt), ctx.alloc(ne, dt)
+    a.fill(0.0)
+    b.fill(0.0)
+    for _ in range(nsteps):
+        capi.check(ctx.L.scafes_pennes_step(ctx.h, C.byref(g), 1, code, a.ptr, b.ptr, w.ptr, q.ptr, C.byref(pp)))
+        a, b = b, a
+    out = a.read()
+    a.free()
+    b.free()
+    return out
+
+
+def _gradient(ctx, g, ne, dt, code, w, q, tgt, pp, nsteps, snaps):
+    t0 = ctx.alloc(ne, dt)
+    t0.fill(0.0)
+    grad = ctx.alloc(ne, np.float64)
+    J = C.c_double(0)
+    capi.check(ctx.L.scafes_pennes_gradient(ctx.h, C.byref(g), 1, code, t0.ptr, w.ptr, q.ptr, tgt.ptr, C.byref(pp),
+                                            nsteps, snaps, grad.ptr, C.byref(J), None))
+    out = grad.read()
+    t0.free()
+    grad.free()
+    return J.value, out
+
+
+def test_pennes256_checkpointing_does_not_change_gradient(ctx):
+    n, N = 256, 100
+    wl, g, ne, w, q, pp = _pennes_device(ctx, n, np.float32)
+    wp = ctx.upload((wl["w"] * wl["pert"]).astype(np.float32))
+    tgt = ctx.upload(_forward(ctx, g, ne, np.float32, capi.F32, wp, q, pp, N))
+    J1, g1 = _gradient(ctx, g, ne, np.float32, capi.F32, w, q, tgt, pp, N, 10)
+    J2, g2 = _gradient(ctx, g, ne, np.float32, capi.F32, w, q, tgt, pp, N, N - 1)
+    np.testing.assert_array_equal(g1, g2)
+    assert J1 == J2 and J1 > 0
+    assert np.abs(g1).max() > 0
+    for d in (w, q, wp, tgt):
+        d.free()
+
+
+def test_pennes256_fp64_gradient_vs_finite_difference(ctx):
+    n, N = 256, 40
+    wl, g, ne, w, q, pp = _pennes_device(ctx, n, np.float64)
+    og = O.make_grid((n, n, n))
+    inner = np.zeros(O.memall_shape(og), bool)
+    O.nodes_view(og, inner)[1:-1, 1:-1, 1:-1] = True
+    inner = inner.ravel()
+    wp = ctx.upload(wl["w"] * wl["pert"])
+    target = _forward(ctx, g, ne, np.float64, capi.F64, wp, q, pp, N)
+    tgt = ctx.upload(target)
+    J, grad = _gradient(ctx, g, ne, np.float64, capi.F64, w, q, tgt, pp, N, N - 1)
+    rng = np.random.default_rng(3)
+    v = np.where(inner, rng.standard_normal(ne), 0.0)
+    eps = 1e-4
+
+    def Jof(wv):
+        dw = ctx.upload(wv)
+        T = _forward(ctx, g, ne, np.float64, capi.F64, dw, q, pp, N)
+        dw.free()
+        r = np.where(inner, T - target, 0.0)
+        return 0.5 * float(np.dot(r, r))
+
+    fd = (Jof(wl["w"] + eps * v) - Jof(wl["w"] - eps * v)) / (2 * eps)
+    ad = float(np.dot(grad, v))
+    assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad)
+    assert abs(Jof(wl["w"]) - J) <= 1e-10 * J
+    for d in (w, q, wp, tgt):
+        d.free()
+
+
+def test_batch4096_graph_and_members(ctx):
+    nsteps = 500
+    ens = batch2d_ensemble(4096, nsteps, seed=0)
+
+    def run(graph):
+        d = {k: ctx.upload(ens[k]) for k in ("n", "off", "kappa", "tau", "order", "target")}
+        u = ctx.upload(ens["u0"])
+        J, gr = ctx.alloc(4096), ctx.alloc(4096)
+        capi.check(ctx.L.scafes_batch2d_run(ctx.h, 4096, d["n"].ptr, d["off"].ptr, d["kappa"].ptr, d["tau"].ptr,
+                                            d["order"].ptr, u.ptr, d["target"].ptr, J.ptr, gr.ptr, nsteps, graph))
+        out = (u.read(), J.read(), gr.read())
+        for x in list(d.values()) + [u, J, gr]:
+            x.free()
+        return out
+
+    a, b = run(0), run(1)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    u = a[0]
+    for i in np.random.default_rng(1).choice(4096, 8, replace=False):
+        n, o = int(ens["n"][i]), int(ens["off"][i])
+        u0 = ens["u0"][o:o + n * n].reshape(n, n).astype(np.float64)
+        ref, _ = O.heat2d_run(n, float(ens["kappa"][i]), float(ens["tau"][i]), nsteps, u0)
+        got = u[o:o + n * n].reshape(n, n)
+        assert np.abs(got - ref).max() <= 2e-5 * max(np.abs(ref).max(), 1e-30), (n, i)
+
+
+def test_emfdtd256_matches_oracle(ctx):
+    """The EMFDTD leg's case (256^3 nodes, CPML d = 10, conductive sphere of
+    radius 50, source 25^3, sink 30^3): three leapfrog steps bit-identical to
+    the oracle in every state field and the sink series."""
+    from masterthesis_amd.workloads import sphere_sigma
+    n, d, N = 256, 10, 3
+    sigma = sphere_sigma(n, (n / 2, n / 2, n / 2), 50.0)
+    ref = O.Emfdtd(n, d=d, src=(25, 25, 25), sink=(30, 30, 30), sigma=sigma)
+    p = capi.emfdtd_params(n, d=d, src=(25, 25, 25), sink=(30, 30, 30))
+    dev = ctx.upload(ref.f)
+    sink = ctx.alloc(N)
+    capi.check(ctx.L.scafes_emfdtd_run(ctx.h, C.byref(p), dev.ptr, N, 0.0, sink.ptr))
+    got = dev.read().reshape(ref.f.shape)
+    s = sink.read()
+    dev.free()
+    sink.free()
+    rs = ref.run(N)
+    for q, name in enumerate(O.EM_FIELDS[:O.EM_STATE]):
+        assert np.array_equal(got[q], ref.f[q]), name
+    np.testing.assert_array_equal(s, rs)
+
+
+def test_mri_brain_grid_matches_oracle(ctx):
+    """The MRI example on the reference configuration's 120 x 120 x 50 brain
+    grid (0.12 x 0.12 x 0.06 m): random materials, an open-skull disc, 20
+    FP64 steps bit-identical to the oracle."""
+    n = (120, 120, 50)
+    ext = (0.12, 0.12, 0.06)
+    og = O.make_grid(n, x0=[0.0] * 3, x1=list(ext))
+    h = [O.lib().oracle_gridsize(C.byref(og), q) for q in range(3)]
+    rng = np.random.default_rng(21)
+    shape = tuple(reversed(n))
+    mats = [1040.0 * (1 + 0.1 * rng.random(shape)), 3650.0 * (1 + 0.1 * rng.random(shape)),
+            0.5 + 0.2 * rng.random(shape), np.full(shape, 1052.0), np.full(shape, 3800.0),
+            0.004 + 0.006 * rng.random(shape), 35.0 + rng.random(shape), 25000.0 * rng.random(shape),
+            np.zeros(shape)]
+    y, x = np.meshgrid(np.arange(n[1]) * h[1] - 0.06, np.arange(n[0]) * h[0] - 0.06, indexing="ij")
+    mats[8][-1] = (x * x + y * y) <= 0.023 ** 2
+    tau = 0.5 / sum(2.0 * 0.7 / (1040.0 * 3650.0) / hp ** 2 for hp in h)
+    mc = O.mri_coef(tau, h=10.0, t_amb=22.0, q_bc=0.0, q_skull=0.0, epsilon=0.98)
+    u = np.zeros(O.memall_shape(og))
+    O.nodes_view(og, u)[...] = 37.0
+    g = capi.geom(n, h=h)
+    p = capi.MriParams(tau=tau, h_conv=10.0, t_amb=22.0, q_bc=0.0, q_skull=0.0, epsilon=0.98, sigma=5.670373e-8)
+    dm = [ctx.upload(np.ascontiguousarray(m)) for m in mats]
+    ptrs = (capi.vp * 9)(*[m.ptr for m in dm])
+    coef = ctx.alloc(capi.MRI_NCOEF * u.size)
+    capi.check(ctx.L.scafes_mri_coef_f64(ctx.h, C.byref(g), 1, ptrs, C.byref(p), coef.ptr))
+    a, b = ctx.upload(u), ctx.upload(u)
+    ref = u
+    for _ in range(20):
+        capi.check(ctx.L.scafes_mri_step_f64(ctx.h, C.byref(g), 1, a.ptr, b.ptr, ptrs, coef.ptr, C.byref(p)))
+        a, b = b, a
+        ref = O.mri_step(og, ref, mats, mc)
+    got = a.read().reshape(u.shape)
+    np.testing.assert_array_equal(O.nodes_view(og, got), O.nodes_view(og, ref))
+    assert O.nodes_view(og, ref)[-1].min() < 36.9  # the open skull cooled
