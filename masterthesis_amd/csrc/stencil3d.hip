@@ -506,9 +506,10 @@ static constexpr int kDefaultCfg = 3;
 // for the odd last step behind two-steps-per-sweep launches.
 static constexpr int kOneStepCfgByMode[3] = {12, 4, 9};
 // Default per mode: the fastest multi-step tile at 512^3 on an MI355X
-// (indices into heatvar_tb2.hip's table): F64 two steps per sweep, 64x8
-// zc64 d4 (6); F32 and F32+Kahan three steps per sweep, 64x12 zc64 d2 (13).
-static constexpr int kDefaultCfgByMode[3] = {kNumCfgs + 6, kNumCfgs + 13, kNumCfgs + 13};
+// (indices into heatvar_tb2.hip's table), all three steps per sweep: F64
+// 64x16 zc64 d2 (10; 620 us/step against 658 for the best two-step tile),
+// F32 and F32+Kahan 64x12 zc64 d2 (13; 234 and 280 us/step).
+static constexpr int kDefaultCfgByMode[3] = {kNumCfgs + 10, kNumCfgs + 13, kNumCfgs + 13};
 
 template <class Op, typename T, int TX, int TY, int D>
 static int launch_t(Ctx* c, const Op& op, const Geo& g, int zc)
