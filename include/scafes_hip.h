@@ -251,6 +251,26 @@ int scafes_pennes_gradient(scafes_ctx* ctx, const scafes_geom* g, int hw, int dt
                            double* grad, double* J, double* stats);
 /* Number of forward steps the treeverse schedule runs for (nsteps, snaps). */
 long scafes_checkpoint_forward_steps(int nsteps, int snaps);
+/* Gradient of J = 0.5 sum_inner (u^N - target)^2 w.r.t. the kappa field of
+ * the variable-kappa heat (the scafes_heat7var_run step, border KEEP or VALUE)
+ * by the reverse-time discrete adjoint with the same checkpointing as
+ * scafes_pennes_gradient.  No counterpart in the reference, whose gradients
+ * are ADOL-C tapes of its hooks w.r.t. a few scalar parameters
+ * (Problem.hpp:5841-5967, 6632-6747); this is the field gradient of the
+ * conservative var-kappa update:
+ *   lam^N = u^N - target on inner nodes, 0 on border nodes; for n = N..1
+ *   grad_k += 0.5 sum_p (tau/h_p^2) sum_{nb = k -+ e_p in the grid}
+ *             (lam_k - lam_nb)(u^{n-1}_nb - u^{n-1}_k)     (every node)
+ *   lam^{n-1} = the step's operator applied to lam^n, 0 on border nodes.
+ * mode SCAFES_HEATVAR_F64: bit-identical to the FP64 CPU restatement;
+ * F32 and F32_KAHAN: u0, kappa, target FP32 (u0 without compensation), the
+ * adjoint field FP32, the gradient FP64 (the compensated forward's lo parts
+ * do not enter the adjoint).  grad has the layout of u (layout hw) and is
+ * overwritten; stats as for scafes_pennes_gradient. */
+int scafes_heat7var_gradient(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode, const void* u0,
+                             const void* kappa, const void* target, double tau, int border_mode,
+                             double border_value, int nsteps, int max_snapshots, double* grad, double* J,
+                             double* stats);
 
 /* ---- Pennes bioheat with per-node materials (MRI data) ------------------
  * Replaces the update phase of examples/MRIDataPBHEqnFDM (MRIDataPBHEqnFDM.hpp:

@@ -129,6 +129,9 @@ def load():
         _sig(L, "scafes_pennes_gradient", [vp, gp, C.c_int, C.c_int, vp, vp, vp, vp,
                                            C.POINTER(PennesParams), C.c_int, C.c_int, vp, dp, dp])
         _sig(L, "scafes_checkpoint_forward_steps", [C.c_int, C.c_int], C.c_long)
+    if hasattr(L, "scafes_heat7var_gradient"):
+        _sig(L, "scafes_heat7var_gradient", [vp, gp, C.c_int, C.c_int, vp, vp, vp, C.c_double, C.c_int,
+                                             C.c_double, C.c_int, C.c_int, vp, dp, dp])
     if hasattr(L, "scafes_yee_run"):
         _sig(L, "scafes_yee_run", [vp, C.c_int, C.c_int, C.c_double, vp, vp, C.c_int, C.c_int, dp, ip])
         _sig(L, "scafes_yee_init_pulse", [vp, C.c_int, C.c_int, vp, C.c_double])

@@ -1,5 +1,6 @@
-// adjoint.hip -- gradient of a final-time misfit w.r.t. the Pennes perfusion
-// field by the reverse-time discrete adjoint with binomial checkpointing.
+// adjoint.hip -- gradient of a final-time misfit w.r.t. a coefficient field
+// (the Pennes perfusion w, or the variable-kappa heat's kappa) by the
+// reverse-time discrete adjoint with binomial checkpointing.
 //
 // The reference differentiates with ADOL-C: one tape of the update phase
 // replayed in forward vector mode w.r.t. a few geometry parameters
@@ -67,7 +68,16 @@ __global__ void lam_init_kernel(Geo g, const T* __restrict__ t, const T* __restr
 
 // A forward state is one array (F32, F64) or a (hi, lo) pair stored as two
 // consecutive arrays of ne elements (F32_KAHAN).
+enum { FAMILY_PENNES = 0, FAMILY_HEATVAR = 1 };
+
 struct Rev {
+    int family = FAMILY_PENNES;
+    // FAMILY_HEATVAR: SCAFES_HEATVAR_* mode of the forward, kappa field,
+    // time step and border of the variable-kappa step
+    int heat_mode = 0;
+    const void* kappa = nullptr;
+    double tau = 0.0, border_value = 0.0;
+    int border_mode = SCAFES_BORDER_KEEP;
     Ctx* c;
     Geo geo;
     const scafes_geom* g;
@@ -97,6 +107,9 @@ struct Rev {
     int step(const void* a, void* b)
     {
         ++fwd_steps;
+        if (family == FAMILY_HEATVAR)
+            return heat7var_step_one(c, geo, g, heat_mode, a, b, lo(a), lo(b), kappa, tau, border_mode,
+                                     border_value);
         return pennes_step_launch(c, geo, g, dtype, a, b, w, q, p, lo(a), lo(b));
     }
     int copy(void* dst, const void* src)
@@ -141,7 +154,11 @@ struct Rev {
             J *= 0.5;
         }
         ++adj_steps;
-        int r = pennes_adj_launch(c, geo, g, lam_dtype, lam[lcur], lam[1 - lcur], w, tprev, grad, p, lo(tprev));
+        // the var-kappa adjoint reads the forward state's hi part only
+        int r = family == FAMILY_HEATVAR
+                    ? heatvar_adj_launch(c, geo, g, lam_dtype, lam[lcur], lam[1 - lcur], kappa, tprev, grad, tau)
+                    : pennes_adj_launch(c, geo, g, lam_dtype, lam[lcur], lam[1 - lcur], w, tprev, grad, p,
+                                        lo(tprev));
         lcur = 1 - lcur;
         return r;
     }
@@ -231,6 +248,86 @@ static long count_forward(long a, long b, int s, int nsteps)
 
 using namespace scafes;
 
+// Reverse sweep with binomial checkpointing for one family: R carries the
+// family's fields; t0 is the initial state (FP32 fields in the FP32 modes).
+static int run_gradient(Rev& R, Ctx* c, const scafes_geom* g, int hw, int dtype, const void* t0,
+                        const void* target, int nsteps, int max_snapshots, double* grad, double* J, double* stats)
+{
+    Geo geo;
+    int rc = make_geo(g, hw, &geo);
+    if (rc) return rc;
+    long ne = 1;
+    for (int p2 = 0; p2 < 3; ++p2) ne *= (p2 < g->dim) ? (g->n[p2] + 2 * hw) : 1;
+    const size_t es = dtype == SCAFES_F64 ? 8 : 4;
+    const size_t abytes = (size_t)ne * es;  // one array
+    const int snaps = std::max(0, std::min(max_snapshots, nsteps - 1));
+    R.c = c;
+    R.geo = geo;
+    R.g = g;
+    R.dtype = dtype;
+    R.lam_dtype = dtype == SCAFES_F64 ? SCAFES_F64 : SCAFES_F32;
+    R.ne = ne;
+    R.sbytes = dtype == SCAFES_F32_KAHAN ? 2 * abytes : abytes;
+    R.target = target;
+    R.grad = grad;
+    R.lcur = 0;
+    R.nsteps = nsteps;
+    R.pair = R.family == FAMILY_PENNES && pennes_adjk_supported(geo);
+    R.fwd_steps = R.adj_steps = 0;
+    R.J = 0.0;
+    R.work = nullptr;
+    std::vector<void*> bufs;
+    auto take = [&](void** out, size_t bytes) -> int {
+        hipError_t e = hipMalloc(out, bytes);
+        if (e != hipSuccess) return check_hip(e, "gradient buffers");
+        bufs.push_back(*out);
+        return SCAFES_OK;
+    };
+    // the initial state as a state (a (t0, 0) pair in the compensated mode)
+    const void* base = t0;
+    void* base_buf = nullptr;
+    if (dtype == SCAFES_F32_KAHAN) {
+        rc = take(&base_buf, R.sbytes);
+        if (!rc) {
+            hipError_t e = hipMemcpyAsync(base_buf, t0, abytes, hipMemcpyDeviceToDevice, c->stream);
+            if (e == hipSuccess) e = hipMemsetAsync((char*)base_buf + abytes, 0, abytes, c->stream);
+            if (e != hipSuccess) rc = check_hip(e, "gradient init");
+        }
+        base = base_buf;
+    }
+    if (!rc) rc = take(&R.tmp[0], R.sbytes);
+    if (!rc) rc = take(&R.tmp[1], R.sbytes);
+    if (!rc) rc = take(&R.work, R.sbytes);
+    if (!rc) rc = take(&R.lam[0], abytes);
+    if (!rc) rc = take(&R.lam[1], abytes);
+    // one allocation for the whole snapshot pool (a store-all budget of a
+    // 256^3 x 2000-step FP32 run is 134 GB; HBM holds 288 GB)
+    if (!rc && snaps > 0) {
+        void* pool = nullptr;
+        rc = take(&pool, (size_t)snaps * R.sbytes);
+        for (int s = 0; !rc && s < snaps; ++s) R.slots.push_back((char*)pool + (size_t)s * R.sbytes);
+    }
+    if (!rc) {
+        hipError_t e = hipMemsetAsync(grad, 0, (size_t)ne * sizeof(double), c->stream);
+        if (e != hipSuccess) rc = check_hip(e, "gradient zero");
+    }
+    if (!rc) rc = R.treeverse(0, nsteps, (int)R.slots.size(), base);
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = check_hip(e, "gradient");
+    }
+    for (size_t i = 0; i < bufs.size(); ++i) (void)hipFree(bufs[i]);
+    if (rc) return rc;
+    *J = R.J;
+    if (stats) {
+        stats[0] = (double)R.fwd_steps;
+        stats[1] = (double)R.adj_steps;
+        stats[2] = (double)snaps;
+        stats[3] = (double)snaps * (double)R.sbytes;
+    }
+    return SCAFES_OK;
+}
+
 extern "C" {
 
 long scafes_checkpoint_forward_steps(int nsteps, int snaps)
@@ -252,83 +349,39 @@ int scafes_pennes_gradient(scafes_ctx* ctx, const scafes_geom* g, int hw, int dt
     if (hw < 1 || g->nlayers < 1) return fail(SCAFES_ERR_ARG, "pennes gradient: need hw >= 1 and nlayers >= 1");
     if (p->border_mode != SCAFES_BORDER_KEEP)
         return fail(SCAFES_ERR_UNSUPPORTED, "pennes gradient: only border_mode KEEP has an adjoint");
-    Geo geo;
-    int rc = make_geo(g, hw, &geo);
-    if (rc) return rc;
-    long ne = 1;
-    for (int p2 = 0; p2 < 3; ++p2) ne *= (p2 < g->dim) ? (g->n[p2] + 2 * hw) : 1;
-    const size_t es = dtype == SCAFES_F64 ? 8 : 4;
-    const size_t abytes = (size_t)ne * es;  // one array
-    const int snaps = std::max(0, std::min(max_snapshots, nsteps - 1));
     Rev R;
-    R.c = c;
-    R.geo = geo;
-    R.g = g;
-    R.dtype = dtype;
-    R.lam_dtype = dtype == SCAFES_F64 ? SCAFES_F64 : SCAFES_F32;
-    R.ne = ne;
-    R.sbytes = dtype == SCAFES_F32_KAHAN ? 2 * abytes : abytes;
+    R.family = FAMILY_PENNES;
     R.w = w;
     R.q = q;
-    R.target = target;
     R.p = p;
-    R.grad = grad;
-    R.lcur = 0;
-    R.nsteps = nsteps;
-    R.pair = pennes_adjk_supported(geo);
-    R.fwd_steps = R.adj_steps = 0;
-    R.J = 0.0;
-    R.work = nullptr;
-    std::vector<void*> bufs;
-    auto take = [&](void** out, size_t bytes) -> int {
-        hipError_t e = hipMalloc(out, bytes);
-        if (e != hipSuccess) return check_hip(e, "pennes gradient buffers");
-        bufs.push_back(*out);
-        return SCAFES_OK;
-    };
-    // the initial state as a state (a (t0, 0) pair in the compensated mode)
-    const void* base = t0;
-    void* base_buf = nullptr;
-    if (dtype == SCAFES_F32_KAHAN) {
-        rc = take(&base_buf, R.sbytes);
-        if (!rc) {
-            hipError_t e = hipMemcpyAsync(base_buf, t0, abytes, hipMemcpyDeviceToDevice, c->stream);
-            if (e == hipSuccess) e = hipMemsetAsync((char*)base_buf + abytes, 0, abytes, c->stream);
-            if (e != hipSuccess) rc = check_hip(e, "pennes gradient init");
-        }
-        base = base_buf;
-    }
-    if (!rc) rc = take(&R.tmp[0], R.sbytes);
-    if (!rc) rc = take(&R.tmp[1], R.sbytes);
-    if (!rc) rc = take(&R.work, R.sbytes);
-    if (!rc) rc = take(&R.lam[0], abytes);
-    if (!rc) rc = take(&R.lam[1], abytes);
-    // one allocation for the whole snapshot pool (a store-all budget of a
-    // 256^3 x 2000-step FP32 run is 134 GB; HBM holds 288 GB)
-    if (!rc && snaps > 0) {
-        void* pool = nullptr;
-        rc = take(&pool, (size_t)snaps * R.sbytes);
-        for (int s = 0; !rc && s < snaps; ++s) R.slots.push_back((char*)pool + (size_t)s * R.sbytes);
-    }
-    if (!rc) {
-        hipError_t e = hipMemsetAsync(grad, 0, (size_t)ne * sizeof(double), c->stream);
-        if (e != hipSuccess) rc = check_hip(e, "pennes gradient zero");
-    }
-    if (!rc) rc = R.treeverse(0, nsteps, (int)R.slots.size(), base);
-    if (!rc) {
-        hipError_t e = hipStreamSynchronize(c->stream);
-        if (e != hipSuccess) rc = check_hip(e, "pennes gradient");
-    }
-    for (size_t i = 0; i < bufs.size(); ++i) (void)hipFree(bufs[i]);
-    if (rc) return rc;
-    *J = R.J;
-    if (stats) {
-        stats[0] = (double)R.fwd_steps;
-        stats[1] = (double)R.adj_steps;
-        stats[2] = (double)snaps;
-        stats[3] = (double)snaps * (double)R.sbytes;
-    }
-    return SCAFES_OK;
+    return run_gradient(R, c, g, hw, dtype, t0, target, nsteps, max_snapshots, grad, J, stats);
+}
+
+int scafes_heat7var_gradient(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode, const void* u0,
+                             const void* kappa, const void* target, double tau, int border_mode,
+                             double border_value, int nsteps, int max_snapshots, double* grad, double* J,
+                             double* stats)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || !g || !u0 || !kappa || !target || !grad || !J)
+        return fail(SCAFES_ERR_ARG, "heat7var gradient: null pointer");
+    if (nsteps < 1) return fail(SCAFES_ERR_ARG, "heat7var gradient: need nsteps >= 1");
+    if (mode < SCAFES_HEATVAR_F64 || mode > SCAFES_HEATVAR_F32_KAHAN)
+        return fail(SCAFES_ERR_ARG, "heat7var gradient: bad mode");
+    if (hw < 1 || g->nlayers < 1) return fail(SCAFES_ERR_ARG, "heat7var gradient: need hw >= 1 and nlayers >= 1");
+    if (border_mode != SCAFES_BORDER_KEEP && border_mode != SCAFES_BORDER_VALUE)
+        return fail(SCAFES_ERR_UNSUPPORTED, "heat7var gradient: border must be KEEP or VALUE");
+    static const int kDtype[3] = {SCAFES_F64, SCAFES_F32, SCAFES_F32_KAHAN};
+    Rev R;
+    R.family = FAMILY_HEATVAR;
+    R.heat_mode = mode;
+    R.kappa = kappa;
+    R.tau = tau;
+    R.border_mode = border_mode;
+    R.border_value = border_value;
+    R.w = R.q = nullptr;
+    R.p = nullptr;
+    return run_gradient(R, c, g, hw, kDtype[mode], u0, target, nsteps, max_snapshots, grad, J, stats);
 }
 
 }  // extern "C"

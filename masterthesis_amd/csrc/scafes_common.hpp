@@ -102,6 +102,16 @@ int pennes_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, c
                       void* lam_new, const void* w, const void* tprev, double* grad,
                       const scafes_pennes_params* p, const void* tprev_lo);
 
+// One variable-kappa step with the one-step tile of the mode (defined in
+// stencil3d.hip); mode is a SCAFES_HEATVAR_* value.
+int heat7var_step_one(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* un,
+                      const void* cin, void* cout, const void* kap, double tau, int border_mode, double border_value);
+// Adjoint step of the variable-kappa heat w.r.t. kappa (defined in
+// stencil3d.hip): lam^n -> lam^{n-1}, grad += the step's kappa terms with
+// u^{n-1} = uprev; dtype SCAFES_F64 or SCAFES_F32 (fields), grad FP64.
+int heatvar_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, const void* lam, void* lam_new,
+                       const void* kap, const void* uprev, double* grad, double tau);
+
 // K = 2..kAdjMaxK reverse Pennes adjoint steps per sweep (defined in
 // pennes_adjk.hip): lam^n -> lam^{n-K} with t[l] = T^{n-1-l} (+ tlo[l] for
 // compensated states, else tlo = nullptr), l = 0..K-1.

@@ -34,18 +34,22 @@ struct St7 {
 // plane k (its centre, the halo value it stages for the tile, and the
 // centre-only fields of the op) is loaded D planes ahead into a register
 // ring, so each plane's barrier waits on loads issued D iterations before.
-// Op interface: kTwo (second stencil field b, e.g. kappa), kNX centre-only
-// fields x[0..kNX-1] of type T, apply(g, i, j, k, a7, b7, xv).
+// Op interface: kTwo (second stencil field b, e.g. kappa), kThree (third
+// stencil field c, with kTwo), kNX centre-only fields x[0..kNX-1] of type T,
+// apply(g, i, j, k, a7, b7, c7, xv).
 template <class Op, typename T, int TX, int TY, int D>
 __global__ __launch_bounds__(TX* TY) void stream_kernel(Op op, Geo g, int zc, int tiles_x,
                                                          int tiles_y)
 {
     static_assert(D >= 2 && D % 2 == 0, "prefetch depth must be even");
     constexpr bool TWO = Op::kTwo;
+    constexpr bool THREE = Op::kThree;
+    static_assert(!THREE || TWO, "a third stencil field needs the second");
     constexpr int NX = Op::kNX;
     constexpr int NXA = NX > 0 ? NX : 1;
     __shared__ T sa[2][TY + 2][TX + 2];
     __shared__ T sb[TWO ? 2 : 1][TWO ? TY + 2 : 1][TWO ? TX + 2 : 1];
+    __shared__ T sc[THREE ? 2 : 1][THREE ? TY + 2 : 1][THREE ? TX + 2 : 1];
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     const int per_plane = tiles_x * tiles_y;
@@ -68,11 +72,12 @@ __global__ __launch_bounds__(TX* TY) void stream_kernel(Op op, Geo g, int zc, in
     const long oc = at(g, ic, jc, 0), ohx = at(g, hxi, jc, 0), ohy = at(g, ic, hyj, 0);
     const T* __restrict__ A = op.a;
     const T* __restrict__ B = op.b;
+    const T* __restrict__ Cf = op.c3;
 
-    T qa[D], qb[D], qax[D], qbx[D], qay[D], qby[D], qx[NXA][D];
+    T qa[D], qb[D], qax[D], qbx[D], qay[D], qby[D], qc[D], qcx[D], qcy[D], qx[NXA][D];
 #pragma unroll
     for (int s = 0; s < D; ++s) {
-        qa[s] = qb[s] = qax[s] = qbx[s] = qay[s] = qby[s] = T(0);
+        qa[s] = qb[s] = qax[s] = qbx[s] = qay[s] = qby[s] = qc[s] = qcx[s] = qcy[s] = T(0);
 #pragma unroll
         for (int x = 0; x < NXA; ++x) qx[x][s] = T(0);
     }
@@ -81,14 +86,17 @@ __global__ __launch_bounds__(TX* TY) void stream_kernel(Op op, Geo g, int zc, in
             const long o = g.e01 * kk;
             qa[slot] = A[oc + o];
             if constexpr (TWO) qb[slot] = B[oc + o];
+            if constexpr (THREE) qc[slot] = Cf[oc + o];
             if (kk < k1) {
                 if (hx) {
                     qax[slot] = A[ohx + o];
                     if constexpr (TWO) qbx[slot] = B[ohx + o];
+                    if constexpr (THREE) qcx[slot] = Cf[ohx + o];
                 }
                 if (hy) {
                     qay[slot] = A[ohy + o];
                     if constexpr (TWO) qby[slot] = B[ohy + o];
+                    if constexpr (THREE) qcy[slot] = Cf[ohy + o];
                 }
 #pragma unroll
                 for (int x = 0; x < NX; ++x) qx[x][slot] = op.x[x][oc + o];
@@ -97,8 +105,9 @@ __global__ __launch_bounds__(TX* TY) void stream_kernel(Op op, Geo g, int zc, in
     };
 
     T am = A[oc + g.e01 * max(k0 - 1, 0)];
-    T bm = T(0);
+    T bm = T(0), cm = T(0);
     if constexpr (TWO) bm = B[oc + g.e01 * max(k0 - 1, 0)];
+    if constexpr (THREE) cm = Cf[oc + g.e01 * max(k0 - 1, 0)];
 #pragma unroll
     for (int s = 0; s < D; ++s) fetch(s, k0 + s);
 
@@ -110,6 +119,7 @@ __global__ __launch_bounds__(TX* TY) void stream_kernel(Op op, Geo g, int zc, in
             const int buf = s & 1;
             const T ac = qa[s], ap = qa[(s + 1) % D];
             const T bc = qb[s], bp = qb[(s + 1) % D];
+            const T cc = qc[s], cp = qc[(s + 1) % D];
             T xv[NXA];
 #pragma unroll
             for (int x = 0; x < NXA; ++x) xv[x] = qx[x][s];
@@ -121,6 +131,11 @@ __global__ __launch_bounds__(TX* TY) void stream_kernel(Op op, Geo g, int zc, in
                 if (hx) sb[buf][ty + 1][sxh] = qbx[s];
                 if (hy) sb[buf][syh][tx + 1] = qby[s];
             }
+            if constexpr (THREE) {
+                sc[buf][ty + 1][tx + 1] = cc;
+                if (hx) sc[buf][ty + 1][sxh] = qcx[s];
+                if (hy) sc[buf][syh][tx + 1] = qcy[s];
+            }
             fetch(s, k + D);
             __syncthreads();
             if (active) {
@@ -130,10 +145,15 @@ __global__ __launch_bounds__(TX* TY) void stream_kernel(Op op, Geo g, int zc, in
                 if constexpr (TWO)
                     sB = St7<T>{bc, sb[buf][ty + 1][tx], sb[buf][ty + 1][tx + 2], sb[buf][ty][tx + 1],
                                 sb[buf][ty + 2][tx + 1], bm, bp};
-                op.apply(g, i, j, k, sA, sB, xv);
+                St7<T> sC{};
+                if constexpr (THREE)
+                    sC = St7<T>{cc, sc[buf][ty + 1][tx], sc[buf][ty + 1][tx + 2], sc[buf][ty][tx + 1],
+                                sc[buf][ty + 2][tx + 1], cm, cp};
+                op.apply(g, i, j, k, sA, sB, sC, xv);
             }
             am = ac;
             bm = bc;
+            cm = cc;
         }
     }
 }
@@ -144,9 +164,11 @@ __global__ __launch_bounds__(TX* TY) void stream_kernel(Op op, Geo g, int zc, in
 // Dirichlet.hpp:124-130).  The library is built with -ffp-contract=off.
 struct HeatConstOp {
     static constexpr bool kTwo = false;
+    static constexpr bool kThree = false;
     static constexpr int kNX = 0;
     const double* a;
     const double* b;
+    const double* c3;
     const double* x[1];
     double* out;
     const double* f_old;
@@ -173,7 +195,7 @@ struct HeatConstOp {
         return t - 2.0 * u;
     }
     __device__ void apply(const Geo& g, int i, int j, int k, const St7<double>& s,
-                          const St7<double>&, const double*) const
+                          const St7<double>&, const St7<double>&, const double*) const
     {
         const long m = at(g, i, j, k);
         if (!is_border(g, i, j, k)) {
@@ -216,9 +238,11 @@ struct HeatConstOp {
 template <typename T, int MODE>
 struct HeatVarOp {
     static constexpr bool kTwo = true;
+    static constexpr bool kThree = false;
     static constexpr int kNX = MODE == 2 ? 1 : 0;
     const T* a;  // u
     const T* b;  // kappa
+    const T* c3;
     const T* x[1];  // x[0] = incoming compensation (Kahan mode)
     T* out;
     T* cout;
@@ -227,7 +251,7 @@ struct HeatVarOp {
     T border_value;
     int border_mode;
     __device__ void apply(const Geo& g, int i, int j, int k, const St7<T>& s,
-                          const St7<T>& q, const T* xv) const
+                          const St7<T>& q, const St7<T>&, const T* xv) const
     {
         const long m = at(g, i, j, k);
         if (!is_border(g, i, j, k)) {
@@ -268,9 +292,11 @@ struct HeatVarOp {
 template <typename T, bool KAHAN = false>
 struct PennesOp {
     static constexpr bool kTwo = false;
+    static constexpr bool kThree = false;
     static constexpr int kNX = KAHAN ? 3 : 2;
     const T* a;  // T_old
     const T* b;
+    const T* c3;
     const T* x[3];  // perfusion w, source Q, incoming compensation (KAHAN)
     T* out;
     T* cout;
@@ -291,7 +317,7 @@ struct PennesOp {
         return (wv + ev) - (T)2 * c;
     }
     __device__ void apply(const Geo& g, int i, int j, int k, const St7<T>& s, const St7<T>&,
-                          const T* xv) const
+                          const St7<T>&, const T* xv) const
     {
         const long m = at(g, i, j, k);
         if (is_border(g, i, j, k)) {
@@ -368,9 +394,11 @@ struct PennesOp {
 // the last axis, the surface flag, both in the stencil-0 node layout gk.
 struct MriOp {
     static constexpr bool kTwo = false;
+    static constexpr bool kThree = false;
     static constexpr int kNX = 4;
     const double* a;  // T_old
     const double* b;
+    const double* c3;
     const double* x[4];
     double* out;
     const double* lam;   // lambda, layout gk
@@ -404,7 +432,7 @@ struct MriOp {
         return (wv + ev) - 2.0 * c;
     }
     __device__ void apply(const Geo& g, int i, int j, int k, const St7<double>& s, const St7<double>&,
-                          const double* xv) const
+                          const St7<double>&, const double* xv) const
     {
         const long m = at(g, i, j, k);
         double v = s.c;
@@ -448,21 +476,85 @@ __global__ void mri_coef_kernel(Geo g, Geo gk, const double* __restrict__ rho, c
     }
 }
 
+// Discrete adjoint of HeatVarOp w.r.t. the kappa field (a = lam^n, b =
+// kappa, c3 = u^{n-1}; FP64 gradient):
+//   grad_k += 0.5 * sum_p th_p sum_{nb = k -+ e_p in the grid} (lam_k - lam_nb)(u_nb - u_k)
+// on every node (kappa at a border node enters the faces of its inner
+// neighbours), th_p = tau/h_p^2; lam^{n-1} = HeatVarOp's update applied to
+// lam on inner nodes (the operator is symmetric), 0 on border nodes.  FP64
+// fields: the oracle's operation order; FP32 fields: the FP32 update of
+// HeatVarOp and the gradient terms in FP64 from the FP32 values.
+template <typename T>
+struct HeatVarAdjOp {
+    static constexpr bool kTwo = true;
+    static constexpr bool kThree = true;
+    static constexpr int kNX = 0;
+    const T* a;   // lam^n
+    const T* b;   // kappa
+    const T* c3;  // u^{n-1}
+    const T* x[1];
+    T* out;       // lam^{n-1}
+    double* grad;
+    double tau, hh0, hh1, hh2, th0, th1, th2;  // th_p = tau / hh_p
+    float c0, c1, c2;                          // 0.5*tau/h_p^2 (FP32 update)
+    __device__ static double face_sum(bool lo, bool hi, double lc, double lm, double lp, double uc, double um,
+                                      double up)
+    {
+        double sp = 0.0;
+        if (lo) sp += (lc - lm) * (um - uc);
+        if (hi) sp += (lc - lp) * (up - uc);
+        return sp;
+    }
+    __device__ void apply(const Geo& g, int i, int j, int k, const St7<T>& l, const St7<T>& q, const St7<T>& u,
+                          const T*) const
+    {
+        const long m = at(g, i, j, k);
+        double t = 0.0;
+        t += th0 * face_sum(i > 0, i < g.nx - 1, (double)l.c, (double)l.w, (double)l.e, (double)u.c, (double)u.w,
+                            (double)u.e);
+        if (g.dim > 1)
+            t += th1 * face_sum(j > 0, j < g.ny - 1, (double)l.c, (double)l.s, (double)l.n, (double)u.c,
+                                (double)u.s, (double)u.n);
+        if (g.dim > 2)
+            t += th2 * face_sum(k > 0, k < g.nz - 1, (double)l.c, (double)l.d, (double)l.u, (double)u.c,
+                                (double)u.d, (double)u.u);
+        grad[m] += 0.5 * t;
+        if (is_border(g, i, j, k)) {
+            out[m] = T(0);
+            return;
+        }
+        if constexpr (sizeof(T) == 8) {
+            double v = l.c;
+            v += (tau * (0.5 * (q.e + q.c) * (l.e - l.c) - 0.5 * (q.c + q.w) * (l.c - l.w))) / hh0;
+            if (g.dim > 1) v += (tau * (0.5 * (q.n + q.c) * (l.n - l.c) - 0.5 * (q.c + q.s) * (l.c - l.s))) / hh1;
+            if (g.dim > 2) v += (tau * (0.5 * (q.u + q.c) * (l.u - l.c) - 0.5 * (q.c + q.d) * (l.c - l.d))) / hh2;
+            out[m] = v;
+        } else {
+            float d = c0 * __builtin_fmaf(q.e + q.c, l.e - l.c, -((q.c + q.w) * (l.c - l.w)));
+            if (g.dim > 1) d = __builtin_fmaf(c1, __builtin_fmaf(q.n + q.c, l.n - l.c, -((q.c + q.s) * (l.c - l.s))), d);
+            if (g.dim > 2) d = __builtin_fmaf(c2, __builtin_fmaf(q.u + q.c, l.u - l.c, -((q.c + q.d) * (l.c - l.d))), d);
+            out[m] = l.c + d;
+        }
+    }
+};
+
 // Discrete adjoint of PennesOp (inner nodes), FP64 gradient accumulation.
 // LO: the forward state is a (hi, lo) pair from the compensated forward.
 template <typename T, bool LO = false>
 struct PennesAdjOp {
     static constexpr bool kTwo = false;
+    static constexpr bool kThree = false;
     static constexpr int kNX = LO ? 3 : 2;
     const T* a;  // lambda^n
     const T* b;
+    const T* c3;
     const T* x[3];  // perfusion w, previous forward state T^{n-1} (hi, lo)
     T* out;         // lambda^{n-1}
     double* grad;
     double ctd, ctp, ta, hh0, hh1, hh2;
     float fd0, fd1, fd2, fp;
     __device__ void apply(const Geo& g, int i, int j, int k, const St7<T>& s, const St7<T>&,
-                          const T* xv) const
+                          const St7<T>&, const T* xv) const
     {
         const long m = at(g, i, j, k);
         if (is_border(g, i, j, k)) {
@@ -674,6 +766,49 @@ int pennes_step_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, 
     PennesOp<float> op{};
     fill32(op);
     return launch<PennesOp<float>, float>(c, op, geo, -1);
+}
+
+int heat7var_step_one(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* un,
+                      const void* cin, void* cout, const void* kap, double tau, int border_mode, double border_value)
+{
+    return heat7var_launch(c, geo, g, mode, u, un, cin, cout, kap, tau, border_mode, border_value,
+                           kOneStepCfgByMode[mode]);
+}
+
+int heatvar_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, const void* lam, void* lam_new,
+                       const void* kap, const void* uprev, double* grad, double tau)
+{
+    double hh[3];
+    hh_of(g, hh);
+    auto fill = [&](auto& op) {
+        op.tau = tau;
+        op.hh0 = hh[0];
+        op.hh1 = hh[1];
+        op.hh2 = hh[2];
+        op.th0 = tau / hh[0];
+        op.th1 = tau / hh[1];
+        op.th2 = tau / hh[2];
+        op.c0 = (float)(0.5 * tau / hh[0]);
+        op.c1 = (float)(0.5 * tau / hh[1]);
+        op.c2 = (float)(0.5 * tau / hh[2]);
+        op.grad = grad;
+    };
+    if (dtype == SCAFES_F64) {
+        HeatVarAdjOp<double> op{};
+        op.a = (const double*)lam;
+        op.b = (const double*)kap;
+        op.c3 = (const double*)uprev;
+        op.out = (double*)lam_new;
+        fill(op);
+        return launch<HeatVarAdjOp<double>, double>(c, op, geo, -1);
+    }
+    HeatVarAdjOp<float> op{};
+    op.a = (const float*)lam;
+    op.b = (const float*)kap;
+    op.c3 = (const float*)uprev;
+    op.out = (float*)lam_new;
+    fill(op);
+    return launch<HeatVarAdjOp<float>, float>(c, op, geo, -1);
 }
 
 int pennes_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, const void* lam,

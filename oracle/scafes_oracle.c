@@ -421,6 +421,85 @@ void oracle_heat7var_run(const oracle_grid* g, double* u, const double* kap, dou
     if (a != u) memcpy(u, a, sizeof(double) * (size_t)ne);
 }
 
+/* Gradient of J = 0.5 sum_inner (u^N - target)^2 w.r.t. the kappa field of
+ * the variable-kappa step, by the reverse-time discrete adjoint.  The
+ * reference differentiates its hooks with ADOL-C (Problem.hpp:5841-5967,
+ * 6632-6747); this restates the derivative of the step above:
+ *   u'_i = u_i + sum_p (tau/h_p^2) [a_iE (u_E - u_i) - a_iW (u_i - u_W)],
+ *   a_ij = (kappa_i + kappa_j)/2, on inner nodes; border nodes depend on
+ *   neither u nor kappa (kept or set to the border value).
+ * lam^N = u^N - target on inner nodes, 0 on border nodes; per step n = N..1
+ *   grad_k += 0.5 * sum_p (tau/h_p^2) sum_{nb = k -+ e_p in the grid}
+ *                 (lam_k - lam_nb) (u_nb - u_k),       u = u^{n-1}, lam = lam^n
+ *   lam^{n-1} = the step's operator applied to lam^n (it is symmetric), 0 on
+ *   border nodes.
+ * states holds u^0..u^N (nsteps+1 memAll arrays). */
+double oracle_heat7var_grad(const oracle_grid* g, const double* states, const double* kap,
+                            const double* target, double tau, int nsteps, double* grad)
+{
+    const long ne = oracle_nelems(g);
+    double* lam = (double*)calloc((size_t)ne, sizeof(double));
+    double* lam2 = (double*)calloc((size_t)ne, sizeof(double));
+    double hh[3];
+    long st[3];
+    for (int p = 0; p < 3; ++p) {
+        hh[p] = p < g->dim ? oracle_gridsize(g, p) * oracle_gridsize(g, p) : 1.0;
+        st[p] = stride(g, p);
+    }
+    const int n0 = g->n[0], n1 = g->n[1], n2 = g->n[2];
+    const double* uN = states + (size_t)nsteps * ne;
+    double J = 0.0;
+    for (int k = 0; k < n2; ++k)
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n0; ++i) {
+                const long m = pos(g, i, j, k);
+                if (is_border(g, i, j, k)) continue;
+                const double r = uN[m] - target[m];
+                lam[m] = r;
+                J += r * r;
+            }
+    J *= 0.5;
+    memset(grad, 0, sizeof(double) * (size_t)ne);
+    for (int n = nsteps; n >= 1; --n) {
+        const double* u = states + (size_t)(n - 1) * ne;
+#pragma omp parallel for collapse(2) schedule(static) num_threads(NTHR)
+        for (int k = 0; k < n2; ++k) {
+            for (int j = 0; j < n1; ++j) {
+                for (int i = 0; i < n0; ++i) {
+                    const long m = pos(g, i, j, k);
+                    const int idx[3] = {i, j, k};
+                    const double lc = lam[m], uc = u[m];
+                    double t = 0.0;
+                    for (int p = 0; p < g->dim; ++p) {
+                        double sp = 0.0;
+                        if (idx[p] > 0) sp += (lc - lam[m - st[p]]) * (u[m - st[p]] - uc);
+                        if (idx[p] < g->n[p] - 1) sp += (lc - lam[m + st[p]]) * (u[m + st[p]] - uc);
+                        t += (tau / hh[p]) * sp;
+                    }
+                    grad[m] += 0.5 * t;
+                    double v = 0.0;
+                    if (!is_border(g, i, j, k)) {
+                        const double kc = kap[m];
+                        v = lc;
+                        for (int p = 0; p < g->dim; ++p) {
+                            const double lw = lam[m - st[p]], le = lam[m + st[p]];
+                            const double kw = kap[m - st[p]], ke = kap[m + st[p]];
+                            v += tau * (0.5 * (ke + kc) * (le - lc) - 0.5 * (kc + kw) * (lc - lw)) / hh[p];
+                        }
+                    }
+                    lam2[m] = v;
+                }
+            }
+        }
+        double* t = lam;
+        lam = lam2;
+        lam2 = t;
+    }
+    free(lam);
+    free(lam2);
+    return J;
+}
+
 /*---------------------------------------------------------------------------*/
 /* Pennes bioheat, TumorHeatEqnFDM::updateInner form
  * (examples/TumorHeatEqnFDM/TumorHeatEqnFDM.hpp:233-269) with the

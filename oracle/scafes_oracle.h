@@ -126,6 +126,10 @@ void oracle_known_timelin(const oracle_grid* g, int known, int border, double t,
 void oracle_heat7var_step(const oracle_grid* g, const double* u_old, double* u_new,
                           const double* kappa, double tau, int border_mode,
                           double border_value);
+/* Gradient of J = 0.5 sum_inner (u^N - target)^2 w.r.t. kappa (discrete
+ * adjoint); states = u^0..u^N.  Returns J. */
+double oracle_heat7var_grad(const oracle_grid* g, const double* states, const double* kappa,
+                            const double* target, double tau, int nsteps, double* grad);
 void oracle_heat7var_run(const oracle_grid* g, double* u, const double* kappa,
                          double tau, int nsteps, int border_mode, double border_value,
                          double* scratch);

@@ -101,6 +101,8 @@ def lib():
         L.oracle_known_timelin.argtypes = [gp, C.c_int, C.c_int, C.c_double, C.POINTER(Heat7Coef), dp, dp, dp]
         L.oracle_heat7var_step.argtypes = [gp, dp, dp, dp, C.c_double, C.c_int, C.c_double]
         L.oracle_heat7var_run.argtypes = [gp, dp, dp, C.c_double, C.c_int, C.c_int, C.c_double, dp]
+        L.oracle_heat7var_grad.argtypes = [gp, dp, dp, dp, C.c_double, C.c_int, dp]
+        L.oracle_heat7var_grad.restype = C.c_double
         L.oracle_pennes_step.argtypes = [gp, dp, dp, dp, dp, C.POINTER(PennesCoef)]
         L.oracle_pennes_forward_all.argtypes = [gp, dp, dp, dp, C.POINTER(PennesCoef), C.c_int]
         L.oracle_pennes_grad.restype = C.c_double
@@ -236,6 +238,25 @@ def heat7var_run(g, u, kappa, tau, nsteps, border=BORDER_VALUE, border_value=0.0
     lib().oracle_heat7var_run(C.byref(g), ptr(u), ptr(kappa), tau, nsteps, border,
                               border_value, ptr(scratch))
     return u
+
+
+def heat7var_forward_all(g, u0, kappa, tau, nsteps, border=BORDER_VALUE, border_value=0.0):
+    """u^0..u^nsteps of the variable-kappa step, shape (nsteps+1,) + u0.shape."""
+    st = np.empty((nsteps + 1,) + u0.shape)
+    st[0] = u0
+    kap = np.ascontiguousarray(kappa, dtype=np.float64)
+    for n in range(nsteps):
+        st[n + 1] = heat7var_step(g, np.ascontiguousarray(st[n]), kap, tau, border, border_value)
+    return st
+
+
+def heat7var_grad(g, states, kappa, target, tau, nsteps):
+    """(J, dJ/dkappa) for J = 0.5 sum_inner (u^N - target)^2."""
+    grad = np.zeros(states[0].shape)
+    J = lib().oracle_heat7var_grad(C.byref(g), ptr(np.ascontiguousarray(states)),
+                                   ptr(np.ascontiguousarray(kappa, dtype=np.float64)),
+                                   ptr(np.ascontiguousarray(target, dtype=np.float64)), tau, nsteps, ptr(grad))
+    return J, grad
 
 
 def pennes_coef(tau, cdiff, cperf, csrc, t_a):

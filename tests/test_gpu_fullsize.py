@@ -19,6 +19,9 @@ seconds and otherwise through properties that do not depend on the size:
    steps bit-identical to the oracle.
  * The MRI example on the reference's 120 x 120 x 50 brain grid: 20 steps
    bit-identical to the oracle.
+ * The var-kappa field gradient: on the 512^3 input, 3 snapshots and
+   store-all bit-identical (FP32+Kahan); on 128^3, FP64 against a central
+   finite difference of J.
 """
 import ctypes as C
 
@@ -306,3 +309,82 @@ def test_mri_brain_grid_matches_oracle(ctx):
     got = a.read().reshape(u.shape)
     np.testing.assert_array_equal(O.nodes_view(og, got), O.nodes_view(og, ref))
     assert O.nodes_view(og, ref)[-1].min() < 36.9  # the open skull cooled
+
+
+def test_heat512_kappa_gradient_checkpointing_invariance(ctx, heat512):
+    """The var-kappa field gradient on the 512^3 headline input (FP32+Kahan
+    forward, 20 steps): 3 snapshots and store-all give identical bits."""
+    g, ne, u0, k0, tau = heat512
+    u32, k32 = ctx.alloc(ne, np.float32), ctx.alloc(ne, np.float32)
+    capi.check(ctx.L.scafes_convert(ctx.h, u32.ptr, capi.F32, u0.ptr, capi.F64, ne))
+    capi.check(ctx.L.scafes_convert(ctx.h, k32.ptr, capi.F32, k0.ptr, capi.F64, ne))
+    tgt = ctx.alloc(ne, np.float32)
+    tgt.fill(0.0)
+    out = []
+    for snaps in (3, 19):
+        grad = ctx.alloc(ne)
+        J = C.c_double(0)
+        capi.check(ctx.L.scafes_heat7var_gradient(ctx.h, C.byref(g), 1, capi.HEATVAR_F32_KAHAN, u32.ptr, k32.ptr,
+                                                  tgt.ptr, tau, capi.BORDER_VALUE, 0.0, 20, snaps, grad.ptr,
+                                                  C.byref(J), None))
+        out.append((J.value, grad.read()))
+        grad.free()
+    for d in (u32, k32, tgt):
+        d.free()
+    assert out[0][0] == out[1][0] > 0
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    assert np.abs(out[0][1]).max() > 0
+
+
+def test_kappa_gradient_fp64_vs_finite_difference_128(ctx):
+    """FP64 var-kappa field gradient on 128^3 (the headline's kappa and sine
+    input at 128^3, 30 steps) against a central difference of J along a
+    random direction."""
+    n, N = 128, 30
+    g = capi.geom((n, n, n))
+    ne = capi.memall_count(g, 1)
+    u0, k0 = ctx.alloc(ne), ctx.alloc(ne)
+    u0.fill(0.0)
+    k0.fill(0.0)
+    capi.check(ctx.L.scafes_heat7var_init(ctx.h, C.byref(g), 1, capi.F64, u0.ptr, k0.ptr, 0, 1e-3))
+    h = 1.0 / (n - 1)
+    tau = 0.9 * h * h / (6.0 * 1.5)
+    og = O.make_grid((n, n, n))
+    inner = np.zeros(O.memall_shape(og), bool)
+    O.nodes_view(og, inner)[1:-1, 1:-1, 1:-1] = True
+    inner = inner.ravel()
+    kap = k0.read()
+    target = 0.9 * u0.read()
+    tgt = ctx.upload(target)
+
+    def J_of(kv):
+        dk = ctx.upload(kv)
+        a, b = ctx.alloc(ne), ctx.alloc(ne)
+        capi.check(ctx.L.scafes_memcpy_d2d(ctx.h, a.ptr, u0.ptr, ne * 8))
+        capi.check(ctx.L.scafes_memcpy_d2d(ctx.h, b.ptr, u0.ptr, ne * 8))
+        which = C.c_int(0)
+        capi.check(ctx.L.scafes_heat7var_run(ctx.h, C.byref(g), 1, capi.HEATVAR_F64, a.ptr, b.ptr, None, None, dk.ptr,
+                                             tau, N, capi.BORDER_VALUE, 0.0, -1, C.byref(which)))
+        uN = (b if which.value else a).read()
+        for d in (dk, a, b):
+            d.free()
+        r = np.where(inner, uN - target, 0.0)
+        return 0.5 * float(np.dot(r, r))
+
+    grad = ctx.alloc(ne)
+    J = C.c_double(0)
+    capi.check(ctx.L.scafes_heat7var_gradient(ctx.h, C.byref(g), 1, capi.HEATVAR_F64, u0.ptr, k0.ptr, tgt.ptr, tau,
+                                              capi.BORDER_VALUE, 0.0, N, N - 1, grad.ptr, C.byref(J), None))
+    gv = grad.read()
+    rng = np.random.default_rng(11)
+    v = np.zeros(ne)
+    nodes = np.zeros(O.memall_shape(og), bool)
+    O.nodes_view(og, nodes)[...] = True
+    v[nodes.ravel()] = rng.standard_normal(int(nodes.sum()))
+    eps = 1e-5
+    fd = (J_of(kap + eps * v) - J_of(kap - eps * v)) / (2 * eps)
+    ad = float(np.dot(gv, v))
+    assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad)
+    assert abs(J_of(kap) - J.value) <= 1e-10 * J.value
+    for d in (u0, k0, tgt, grad):
+        d.free()
