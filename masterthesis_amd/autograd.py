@@ -1,11 +1,16 @@
-"""torch.autograd.Function over the Pennes adjoint of the C ABI.
+"""torch.autograd.Function over the adjoint gradients of the C ABI.
 
-J(w) = 0.5 * sum_inner (T^N(w) - target)^2 for the Pennes bioheat forward
-model with perfusion field w; backward returns dJ/dw from the reverse-time
-adjoint with binomial checkpointing (scafes_pennes_gradient).  Tensors are
-host (CPU) tensors in the memAll layout (ghost width 1, shape (z, y, x)
-with n+2 per axis); the device work runs on the given capi.Context.
+PennesMisfit: J(w) = 0.5 * sum_inner (T^N(w) - target)^2 for the Pennes
+bioheat forward model with perfusion field w; backward returns dJ/dw from
+the reverse-time adjoint with binomial checkpointing (scafes_pennes_gradient).
 The FP32 mode streams the deviation theta = T - T_a on the device.
+
+KappaMisfit: J(kappa) = 0.5 * sum_inner (u^N(kappa) - target)^2 for the
+variable-kappa heat; backward returns dJ/dkappa (scafes_heat7var_gradient).
+
+Tensors are host (CPU) tensors in the memAll layout (ghost width 1, shape
+(z, y, x) with n+2 per axis); the device work runs on the given
+capi.Context.
 """
 import numpy as np
 import torch
@@ -66,3 +71,57 @@ class PennesMisfit:
 
     def __call__(self, w, t0, q, target):
         return _PennesJ.apply(w, t0, q, target, self)
+
+
+class _KappaJ(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, kappa, u0, target, runner):
+        J, grad = runner._run(kappa.detach().cpu().numpy(), u0.detach().cpu().numpy(),
+                              target.detach().cpu().numpy())
+        ctx.save_for_backward(torch.from_numpy(grad))
+        return torch.tensor(J, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, gout):
+        (grad,) = ctx.saved_tensors
+        return gout * grad, None, None, None
+
+
+class KappaMisfit:
+    """Callable J(kappa, u0, target) of the variable-kappa heat with an
+    adjoint backward pass."""
+
+    def __init__(self, context, n, tau, nsteps, snapshots=16, dtype="f32_kahan", border="keep",
+                 border_value=0.0):
+        """dtype: "f64", "f32" or "f32_kahan"; border: "keep" or "value"."""
+        self.ctx = context
+        self.n = tuple(n)
+        self.tau = float(tau)
+        self.nsteps = int(nsteps)
+        self.snapshots = int(snapshots)
+        self.mode = {"f64": capi.HEATVAR_F64, "f32": capi.HEATVAR_F32, "f32_kahan": capi.HEATVAR_F32_KAHAN}[dtype]
+        self.border = {"keep": capi.BORDER_KEEP, "value": capi.BORDER_VALUE}[border]
+        self.border_value = float(border_value)
+        self.geom = capi.geom(self.n)
+        self.last_stats = None
+
+    def _run(self, kappa, u0, target):
+        dt = np.float64 if self.mode == capi.HEATVAR_F64 else np.float32
+        bufs = [self.ctx.upload(np.ascontiguousarray(a, dtype=dt)) for a in (u0, kappa, target)]
+        g = self.ctx.alloc(u0.size, np.float64)
+        J = capi.C.c_double(0.0)
+        stats = np.zeros(4)
+        try:
+            capi.check(self.ctx.L.scafes_heat7var_gradient(
+                self.ctx.h, capi.C.byref(self.geom), 1, self.mode, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, self.tau,
+                self.border, self.border_value, self.nsteps, self.snapshots, g.ptr, capi.C.byref(J),
+                stats.ctypes.data_as(capi.C.POINTER(capi.C.c_double))))
+            grad = g.read().reshape(u0.shape)
+        finally:
+            for b in bufs + [g]:
+                b.free()
+        self.last_stats = stats
+        return J.value, grad
+
+    def __call__(self, kappa, u0, target):
+        return _KappaJ.apply(kappa, u0, target, self)

@@ -9,7 +9,9 @@ test_oracle_heatvar_grad.py):
    the FP64 oracle (measured values printed);
  * FP32: checkpointing changes the schedule, not the arithmetic: the
    gradient with 3 snapshots and with store-all are bit-identical;
- * argument errors.
+ * argument errors;
+ * the torch.autograd.Function wrapper (KappaMisfit) against central
+   finite differences.
 """
 import ctypes as C
 
@@ -115,3 +117,30 @@ def test_argument_errors(ctx):
                                       C.byref(J), None) != 0
     assert L.scafes_heat7var_gradient(ctx.h, C.byref(g), 1, 0, None, a.ptr, a.ptr, 1e-3, 0, 0.0, 2, 1, a.ptr,
                                       C.byref(J), None) != 0
+
+
+def test_autograd_function_vs_finite_differences(ctx):
+    """KappaMisfit (torch.autograd.Function): J.backward() gives the adjoint
+    gradient; a central difference of J along random directions agrees to
+    1e-6 (FP64) and to 1e-3 with FP32+Kahan fields."""
+    torch = pytest.importorskip("torch")
+    from masterthesis_amd.autograd import KappaMisfit
+    n, N = (20, 18, 16), 25
+    g, h, u0, kap, tau, target = _case(n, seed=12)
+    for dtype, tol, eps in (("f64", 1e-6, 1e-5), ("f32_kahan", 1e-3, 1e-2)):
+        fn = KappaMisfit(ctx, n, tau, N, snapshots=5, dtype=dtype)
+        kt = torch.tensor(kap, dtype=torch.float64, requires_grad=True)
+        J = fn(kt, torch.tensor(u0), torch.tensor(target))
+        J.backward()
+        gk = kt.grad.numpy()
+        rng = np.random.default_rng(2)
+        nodes = np.zeros(kap.shape, bool)
+        O.nodes_view(g, nodes)[...] = True
+        v = np.where(nodes, rng.standard_normal(kap.shape), 0.0)
+        with torch.no_grad():
+            jp = fn(torch.tensor(kap + eps * v), torch.tensor(u0), torch.tensor(target)).item()
+            jm = fn(torch.tensor(kap - eps * v), torch.tensor(u0), torch.tensor(target)).item()
+        fd = (jp - jm) / (2 * eps)
+        ad = float(np.sum(gk * v))
+        print(f"\n[kappa autograd {dtype}] adjoint {ad:.10e} fd {fd:.10e} rel {abs(fd - ad) / abs(ad):.2e}")
+        assert abs(fd - ad) <= tol * abs(ad)
