@@ -251,6 +251,75 @@ def cpu_emfdtd(n=256, budget_s=6.0):
                       f"same {n}^3 case, {k} steps in {dt:.1f} s"}
 
 
+def bench_heat64(ctx, capi, n=64, nsteps=100):
+    """HeatEqnFDM plain method: u_t = Laplace(u) on the unit cube, n^3 nodes,
+    FP64, tau = 0.9 h^2/6, zero Dirichlet border, u0 = prod sin(pi x_i):
+    GLUPS, the error against the analytic solution exp(-3 pi^2 t) prod
+    sin(pi x_i), and against the discrete scheme's closed form
+    (1 - tau lambda_h)^n prod sin(pi x_i)."""
+    h = 1.0 / (n - 1)
+    tau = 0.9 * h * h / 6.0
+    x = np.linspace(0.0, 1.0, n)
+    s = np.sin(np.pi * x)
+    mode = s[None, None, :] * s[None, :, None] * s[:, None, None]
+    u0 = np.zeros((n + 2,) * 3)
+    u0[1:-1, 1:-1, 1:-1] = mode
+    g = capi.geom((n, n, n))
+    p = capi.Heat7Params(tau=tau, coeff_a=1.0, rhoc_inv=1.0, border_mode=capi.BORDER_VALUE)
+    a, b = ctx.upload(u0), ctx.upload(u0)
+    gp = capi.C.byref(g)
+
+    def run(k, x0, x1):
+        for _ in range(k):
+            capi.check(ctx.L.scafes_heat7_step_f64(ctx.h, gp, 1, x0.ptr, x1.ptr, None, None, None, capi.C.byref(p)))
+            x0, x1 = x1, x0
+        return x0
+
+    run(10, a, b)
+    a.write(u0)
+    b.write(u0)
+    ctx.sync()
+    ctx.record(0)
+    res = run(nsteps, a, b)
+    ctx.record(1)
+    ms = ctx.elapsed_ms(0, 1)
+    u = res.read().reshape(u0.shape)[1:-1, 1:-1, 1:-1]
+    a.free()
+    b.free()
+    t = nsteps * tau
+    lam_h = 3 * (4.0 / (h * h)) * np.sin(np.pi * h / 2) ** 2
+    exact_discrete = (1.0 - tau * lam_h) ** nsteps * mode
+    analytic = np.exp(-3 * np.pi ** 2 * t) * mode
+    return {"workload": f"heat {n}^3 FP64, sine mode, zero Dirichlet, tau = 0.9 h^2/6, {nsteps} steps",
+            "kernel": "stream_kernel<HeatConstOp> (one step per launch)", "ms": round(ms, 4),
+            "GLUPS": round(float(n) ** 3 * nsteps / (ms * 1e-3) / 1e9, 3),
+            "rel_err_vs_discrete_closed_form": float(np.abs(u - exact_discrete).max() / np.abs(exact_discrete).max()),
+            "rel_err_vs_analytic": float(np.abs(u - analytic).max() / np.abs(analytic).max())}
+
+
+def cpu_heat64(n=64, budget_s=3.0):
+    """Oracle FP64 constant-coefficient heat steps (OpenMP) on the same n^3 case."""
+    O, threads = _oracle()
+    g = O.make_grid((n, n, n))
+    h = 1.0 / (n - 1)
+    coef = O.heat7_coef(0.9 * h * h / 6.0)
+    a = np.zeros(O.memall_shape(g))
+    x = np.linspace(0.0, 1.0, n)
+    s = np.sin(np.pi * x)
+    O.nodes_view(g, a)[...] = s[None, None, :] * s[None, :, None] * s[:, None, None]
+    b = a.copy()
+    st = [a, b]
+
+    def step():
+        O.lib().oracle_heat7_step(O.C.byref(g), O.ptr(st[0]), O.ptr(st[1]), None, None, None, O.BORDER_VALUE, 0.0,
+                                  O.C.byref(coef))
+        st.reverse()
+
+    k, dt = _timed(step, budget_s, cap=2000)
+    return {"value": round(float(n) ** 3 * k / dt / 1e9, 4), "unit": "GLUPS", "cores": threads, "kind": "port",
+            "sample": f"oracle FP64 heat step (OpenMP) on the same {n}^3 case, {k} steps in {dt:.1f} s"}
+
+
 def bench_yee(ctx, capi, ncell=384, nsteps=2000, energy_every=100):
     """Yee FDTD, PEC box of ncell^3 cells, vacuum, Courant 0.5, Gaussian Ez
     pulse (sigma = 8 cells) at the centre, FP32 fields, FP64 energy."""
@@ -602,6 +671,10 @@ def main():
         log(f"[bench] copy {out['copy_calibration']}")
     # each leg: the GPU measurement, then (unless --no-cpu) the oracle timed
     # on a bounded sample of the same workload on the host cores
+    out["heat64"] = bench_heat64(ctx, capi)
+    if not args.no_cpu:
+        out["heat64"]["cpu_baseline"] = cpu_heat64()
+    log(f"[bench] heat64 {out['heat64']}")
     if not args.no_yee:
         out["yee"] = bench_yee(ctx, capi, **({"nsteps": args.leg_steps} if args.leg_steps else {}))
         if not args.no_cpu:
