@@ -191,6 +191,29 @@ def cpu_yee(ncell=384, budget_s=5.0):
                                       f"{k} steps in {dt:.1f} s"}
 
 
+def cpu_yee_eps(ncell=128, nsteps=80):
+    """Oracle FP64 permittivity gradient (forward states + reverse sweep,
+    OpenMP) on a ncell^3 box of the same kind: gradient cell-steps per s."""
+    O, threads = _oracle()
+    m = ncell + 1
+    f0 = O.yee_alloc(ncell)
+    f0[2, m // 2, m // 2, m // 2] = 1.0
+    ax = np.arange(m)
+    r2 = (ax[:, None, None] - 0.6 * ncell) ** 2 + (ax[None, :, None] - 0.6 * ncell) ** 2 + \
+        (ax[None, None, :] - 0.5 * ncell) ** 2
+    eps = np.where(r2 <= (ncell / 4.0) ** 2, 4.0, 1.0)
+    target = np.zeros((3, m, m, m))
+    t0 = time.perf_counter()
+    st = O.yee_eps_forward_all(ncell, 0.5, eps, f0, nsteps)
+    O.yee_eps_grad(ncell, 0.5, eps, st, target, nsteps)
+    dt = time.perf_counter() - t0
+    del st
+    return {"value": round(float(ncell) ** 3 * nsteps / dt / 1e9, 4), "unit": "gradient Gcell-steps/s",
+            "cores": threads, "kind": "port",
+            "sample": f"oracle FP64 Yee eps forward (all states kept) + adjoint gradient (OpenMP), {ncell}^3 cells, "
+                      f"{nsteps} steps in {dt:.1f} s"}
+
+
 def cpu_pennes(n=256, budget_s=5.0):
     """Oracle FP64 Pennes forward steps (OpenMP) on the same n^3 workload."""
     from masterthesis_amd.workloads import pennes_workload
@@ -349,6 +372,79 @@ def bench_yee(ctx, capi, ncell=384, nsteps=2000, energy_every=100):
             "hbm_frac_algorithmic": round(48 * cells / (per * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "energy_rel_drift_fp64": float(np.abs(e - e[0]).max() / e[0]), "energy_samples": int(nrec),
             **_profile_fields("yee_tb2_384.md", ncell == 384)}
+
+
+def bench_yee_eps(ctx, capi, ncell=384, nsteps=200, snapshots=(20, 100)):
+    """Yee FDTD with a permittivity field: PEC box of ncell^3 cells, Courant
+    0.5, Gaussian Ez pulse (sigma = 8 cells) at the centre, a dielectric
+    sphere (radius ncell/4, eps = 4) off centre; forward time (eps sweeps and
+    the vacuum two-steps-per-sweep kernel) vs the adjoint gradient of the
+    final-E misfit against the E field of a sphere with eps = 4.4 (FP32
+    fields, FP64 gradient) for two snapshot budgets."""
+    m = ncell + 1
+    nf = m ** 3
+    dt = np.float32
+    ax = np.arange(m, dtype=np.float32)
+    c0 = 0.6 * ncell
+    r2 = (ax[:, None, None] - c0) ** 2 + (ax[None, :, None] - c0) ** 2 + (ax[None, None, :] - 0.5 * ncell) ** 2
+    inside = r2 <= (ncell / 4.0) ** 2
+    eps = ctx.upload(np.where(inside, 4.0, 1.0).astype(dt))
+    eps_t = ctx.upload(np.where(inside, 4.4, 1.0).astype(dt))
+    del r2, inside
+    f0 = ctx.alloc(6 * nf, dt)
+    capi.check(ctx.L.scafes_yee_init_pulse(ctx.h, ncell, capi.F32, f0.ptr, 8.0))
+    f, w = ctx.alloc(6 * nf, dt), ctx.alloc(6 * nf, dt)
+
+    def eps_forward(ebuf, steps):
+        capi.check(ctx.L.scafes_memcpy_d2d(ctx.h, f.ptr, f0.ptr, 6 * nf * 4))
+        capi.check(ctx.L.scafes_yee_eps_run(ctx.h, ncell, capi.F32, 0.5, ebuf.ptr, f.ptr, w.ptr, steps))
+
+    eps_forward(eps_t, nsteps)
+    target = ctx.alloc(3 * nf, dt)
+    capi.check(ctx.L.scafes_memcpy_d2d(ctx.h, target.ptr, f.ptr, 3 * nf * 4))
+    eps_t.free()
+    eps_forward(eps, 4)
+    ctx.sync()
+    ctx.record(0)
+    eps_forward(eps, nsteps)
+    ctx.record(1)
+    t_eps = ctx.elapsed_ms(0, 1)
+    capi.check(ctx.L.scafes_memcpy_d2d(ctx.h, f.ptr, f0.ptr, 6 * nf * 4))
+    ctx.sync()
+    ctx.record(0)
+    capi.check(ctx.L.scafes_yee_run(ctx.h, ncell, capi.F32, 0.5, f.ptr, w.ptr, nsteps, 0, None, None))
+    ctx.record(1)
+    t_vac = ctx.elapsed_ms(0, 1)
+    cells = float(ncell) ** 3
+    out = {"workload": f"yee {ncell}^3 cells PEC, dielectric sphere eps=4 (r={ncell // 4}), Courant 0.5, "
+                       f"{nsteps} steps, FP32 fields, FP64 gradient",
+           "fwd_eps_ms": round(t_eps, 2), "fwd_eps_gcell_steps_per_s": round(cells * nsteps / (t_eps * 1e-3) / 1e9, 2),
+           "fwd_vacuum_ms": round(t_vac, 2),
+           "fwd_kernels": "h_sweep<float,false> + e_sweep<float,false> (76 B/cell-step moved, 52 B algorithmic)",
+           "adjoint_kernels": "h_sweep<float,true> + e_sweep<float,true> (164 B/cell-step moved)",
+           "gradient": []}
+    grad = ctx.alloc(nf, np.float64)
+    for s in snapshots:
+        J = capi.C.c_double(0)
+        stats = np.zeros(4)
+        ctx.sync()
+        t0 = time.perf_counter()
+        rc = ctx.L.scafes_yee_eps_gradient(ctx.h, ncell, capi.F32, 0.5, f0.ptr, eps.ptr, target.ptr, nsteps, s,
+                                           grad.ptr, capi.C.byref(J),
+                                           stats.ctypes.data_as(capi.C.POINTER(capi.C.c_double)))
+        t_grad = (time.perf_counter() - t0) * 1e3
+        if rc != 0:
+            out["gradient"].append({"snapshots": s, "error": ctx.L.scafes_last_error().decode()})
+            continue
+        out["gradient"].append({"snapshots": int(stats[2]), "checkpoint_GB": round(stats[3] / 1e9, 2),
+                                "forward_steps": int(stats[0]), "adjoint_steps": int(stats[1]),
+                                "grad_ms": round(t_grad, 2),
+                                "grad_gcell_steps_per_s": round(cells * nsteps / (t_grad * 1e-3) / 1e9, 3),
+                                "grad_over_fwd_eps": round(t_grad / t_eps, 3),
+                                "grad_over_fwd_vacuum": round(t_grad / t_vac, 3), "J": J.value})
+    for d in (eps, f0, f, w, target, grad):
+        d.free()
+    return out
 
 
 # (first index, last index - (n-1), axis of the l term, axis of the r term)
@@ -548,6 +644,7 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="autotune tiles before timing")
     ap.add_argument("--no-yee", action="store_true", help="skip the Yee FDTD leg")
     ap.add_argument("--no-pennes", action="store_true", help="skip the Pennes adjoint leg")
+    ap.add_argument("--no-yee-eps", action="store_true", help="skip the Yee permittivity-gradient leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-emfdtd", action="store_true", help="skip the EMFDTD (CPML) leg")
     ap.add_argument("--calibrate-copy", action="store_true", help="also run a 1 GiB copy (counter calibration)")
@@ -680,6 +777,12 @@ def main():
         if not args.no_cpu:
             out["yee"]["cpu_baseline"] = cpu_yee()
         log(f"[bench] yee {out['yee']}")
+    if not args.no_yee_eps:
+        out["yee_eps_adjoint"] = bench_yee_eps(ctx, capi, **({"nsteps": args.leg_steps, "snapshots": (4,)}
+                                                             if args.leg_steps else {}))
+        if not args.no_cpu:
+            out["yee_eps_adjoint"]["cpu_baseline"] = cpu_yee_eps()
+        log(f"[bench] yee eps {out['yee_eps_adjoint']}")
     if not args.no_pennes:
         out["pennes_adjoint"] = bench_pennes(ctx, capi, **({"nsteps": args.leg_steps} if args.leg_steps else {}))
         if not args.no_cpu:

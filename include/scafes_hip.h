@@ -344,6 +344,31 @@ int scafes_yee_run(scafes_ctx* ctx, int ncell, int dtype, double s, void* f, voi
  * Ez = exp(-r^2/(2 sigma^2)) at inner Ez nodes, r from the box centre in
  * cell units (Ez(i,j,k) sits at (i, j, k+1/2)). */
 int scafes_yee_init_pulse(scafes_ctx* ctx, int ncell, int dtype, void* f, double sigma);
+/* Yee leapfrog with a relative permittivity field eps ((N+1)^3 values in
+ * the state dtype, one per node index m, shared by Ex(m), Ey(m), Ez(m);
+ * eps > 0): the E half step becomes D += s curl H, E = D / eps, i.e.
+ * E += (s / eps) * (backward differences of H) -- the per-node D -> E
+ * scaling of the reference's EMFDTD update (examples/EMFDTD/EMFDTD.hpp:351-356,
+ * there 1/(1 + sigma dt/eps0)).  eps = 1 gives scafes_yee_run's bits.
+ * f (six arrays) is advanced nsteps in place; work is a second state
+ * buffer (out-of-place half steps).  F32 or F64. */
+int scafes_yee_eps_run(scafes_ctx* ctx, int ncell, int dtype, double s, const void* eps, void* f, void* work,
+                       int nsteps);
+/* Gradient of J = 0.5 sum over the E entries the step updates of
+ * (E^N - target)^2 (target: three E arrays) w.r.t. eps, by the reverse-time
+ * discrete adjoint of scafes_yee_eps_run's step with the checkpointing of
+ * scafes_pennes_gradient (max_snapshots stored states of six arrays).  No
+ * counterpart in the reference (its EMFDTD optimisation tapes a few
+ * geometry parameters with ADOL-C, Problem.hpp:5841-5967).  The adjoint is
+ * the pair of half steps run backwards:
+ *   mu_H = lam_H - Hform(ce lam_E),  lam_E <- lam_E - s Eform(mu_H),
+ *   grad(m) -= (ce/eps)(m) sum_c lam_E_c(m) Eform_c(H^{n-1/2})(m),  ce = s/eps.
+ * F64: bit-identical to the FP64 CPU restatement; F32: f0, eps, target and
+ * the adjoint fields FP32, grad FP64.  grad ((N+1)^3 FP64) is overwritten;
+ * stats as for scafes_pennes_gradient. */
+int scafes_yee_eps_gradient(scafes_ctx* ctx, int ncell, int dtype, double s, const void* f0, const void* eps,
+                            const void* target, int nsteps, int max_snapshots, double* grad, double* J,
+                            double* stats);
 
 /* ---- EMFDTD example: Yee leapfrog + complete CPML + conductive media -----
  * Replaces the update phases of examples/EMFDTD/EMFDTD.hpp (EMFDTDng:

@@ -8,9 +8,13 @@ The FP32 mode streams the deviation theta = T - T_a on the device.
 KappaMisfit: J(kappa) = 0.5 * sum_inner (u^N(kappa) - target)^2 for the
 variable-kappa heat; backward returns dJ/dkappa (scafes_heat7var_gradient).
 
-Tensors are host (CPU) tensors in the memAll layout (ghost width 1, shape
-(z, y, x) with n+2 per axis); the device work runs on the given
-capi.Context.
+PermittivityMisfit: J(eps) = 0.5 * sum (E^N(eps) - target)^2 over the E
+entries the Yee step updates; backward returns dJ/deps
+(scafes_yee_eps_gradient).
+
+Tensors are host (CPU) tensors: node fields in the memAll layout (ghost
+width 1, shape (z, y, x) with n+2 per axis), Yee fields as (N+1)^3 arrays
+per component; the device work runs on the given capi.Context.
 """
 import numpy as np
 import torch
@@ -125,3 +129,54 @@ class KappaMisfit:
 
     def __call__(self, kappa, u0, target):
         return _KappaJ.apply(kappa, u0, target, self)
+
+
+class _PermittivityJ(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, eps, f0, target, runner):
+        J, grad = runner._run(eps.detach().cpu().numpy(), f0.detach().cpu().numpy(),
+                              target.detach().cpu().numpy())
+        ctx.save_for_backward(torch.from_numpy(grad))
+        return torch.tensor(J, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, gout):
+        (grad,) = ctx.saved_tensors
+        return gout * grad, None, None, None
+
+
+class PermittivityMisfit:
+    """Callable J(eps, f0, target) of the Yee leapfrog in a PEC box of ncell
+    cells with a relative permittivity field eps ((N+1)^3), J = 0.5 sum over
+    the updated E entries of (E^N - target)^2, with an adjoint backward
+    pass.  f0: the six Yee arrays; target: the three E arrays."""
+
+    def __init__(self, context, ncell, s, nsteps, snapshots=16, dtype="f32"):
+        """dtype: "f64" or "f32" (FP32 fields, FP64 gradient)."""
+        self.ctx = context
+        self.ncell = int(ncell)
+        self.s = float(s)
+        self.nsteps = int(nsteps)
+        self.snapshots = int(snapshots)
+        self.code = {"f64": capi.F64, "f32": capi.F32}[dtype]
+        self.last_stats = None
+
+    def _run(self, eps, f0, target):
+        dt = np.float64 if self.code == capi.F64 else np.float32
+        bufs = [self.ctx.upload(np.ascontiguousarray(a, dtype=dt)) for a in (f0, eps, target)]
+        g = self.ctx.alloc(eps.size, np.float64)
+        J = capi.C.c_double(0.0)
+        stats = np.zeros(4)
+        try:
+            capi.check(self.ctx.L.scafes_yee_eps_gradient(
+                self.ctx.h, self.ncell, self.code, self.s, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, self.nsteps,
+                self.snapshots, g.ptr, capi.C.byref(J), stats.ctypes.data_as(capi.C.POINTER(capi.C.c_double))))
+            grad = g.read().reshape(eps.shape)
+        finally:
+            for b in bufs + [g]:
+                b.free()
+        self.last_stats = stats
+        return J.value, grad
+
+    def __call__(self, eps, f0, target):
+        return _PermittivityJ.apply(eps, f0, target, self)

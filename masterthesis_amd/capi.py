@@ -135,6 +135,10 @@ def load():
     if hasattr(L, "scafes_yee_run"):
         _sig(L, "scafes_yee_run", [vp, C.c_int, C.c_int, C.c_double, vp, vp, C.c_int, C.c_int, dp, ip])
         _sig(L, "scafes_yee_init_pulse", [vp, C.c_int, C.c_int, vp, C.c_double])
+    if hasattr(L, "scafes_yee_eps_run"):
+        _sig(L, "scafes_yee_eps_run", [vp, C.c_int, C.c_int, C.c_double, vp, vp, vp, C.c_int])
+        _sig(L, "scafes_yee_eps_gradient", [vp, C.c_int, C.c_int, C.c_double, vp, vp, vp, C.c_int, C.c_int, vp,
+                                            dp, dp])
     if hasattr(L, "scafes_emfdtd_run"):
         _sig(L, "scafes_emfdtd_tables", [C.c_int] + [C.c_double] * 6 + [dp, dp, dp])
         _sig(L, "scafes_emfdtd_dt", [C.c_double], C.c_double)

@@ -1,5 +1,6 @@
 // adjoint.hip -- gradient of a final-time misfit w.r.t. a coefficient field
-// (the Pennes perfusion w, or the variable-kappa heat's kappa) by the
+// (the Pennes perfusion w, the variable-kappa heat's kappa, or the Yee
+// permittivity eps) by the
 // reverse-time discrete adjoint with binomial checkpointing.
 //
 // The reference differentiates with ADOL-C: one tape of the update phase
@@ -68,7 +69,7 @@ __global__ void lam_init_kernel(Geo g, const T* __restrict__ t, const T* __restr
 
 // A forward state is one array (F32, F64) or a (hi, lo) pair stored as two
 // consecutive arrays of ne elements (F32_KAHAN).
-enum { FAMILY_PENNES = 0, FAMILY_HEATVAR = 1 };
+enum { FAMILY_PENNES = 0, FAMILY_HEATVAR = 1, FAMILY_YEE_EPS = 2 };
 
 struct Rev {
     int family = FAMILY_PENNES;
@@ -78,6 +79,15 @@ struct Rev {
     const void* kappa = nullptr;
     double tau = 0.0, border_value = 0.0;
     int border_mode = SCAFES_BORDER_KEEP;
+    // FAMILY_YEE_EPS: box, Courant number, ce = s/eps and eps fields, H'
+    // scratch (three arrays), J partials and J on the device
+    YeeDims yd{};
+    double yee_s = 0.0;
+    const void* ce = nullptr;
+    const void* eps = nullptr;
+    void* hp = nullptr;
+    double* partial = nullptr;
+    double* jdev = nullptr;
     Ctx* c;
     Geo geo;
     const scafes_geom* g;
@@ -107,6 +117,7 @@ struct Rev {
     int step(const void* a, void* b)
     {
         ++fwd_steps;
+        if (family == FAMILY_YEE_EPS) return yee_eps_step(c, yd, dtype, yee_s, ce, a, b);
         if (family == FAMILY_HEATVAR)
             return heat7var_step_one(c, geo, g, heat_mode, a, b, lo(a), lo(b), kappa, tau, border_mode,
                                      border_value);
@@ -134,6 +145,16 @@ struct Rev {
     // Adjoint step n (lam^n -> lam^{n-1}) with T^{n-1} = tprev.
     int adjoint(int n, const void* tprev)
     {
+        if (family == FAMILY_YEE_EPS) {
+            // in place on lam[0]; J^N goes to the device and is read at the end
+            if (n == nsteps) {
+                int r = step(tprev, tmp[0]);
+                if (!r) r = yee_eps_lam_init(c, yd, dtype, tmp[0], target, lam[0], partial, jdev);
+                if (r) return r;
+            }
+            ++adj_steps;
+            return yee_eps_adjoint(c, yd, dtype, yee_s, ce, eps, tprev, lam[0], hp, grad);
+        }
         if (n == nsteps) {
             // first leaf: T^N from T^{N-1}, then lam^N and J
             int r = step(tprev, tmp[0]);
@@ -248,31 +269,24 @@ static long count_forward(long a, long b, int s, int nsteps)
 
 using namespace scafes;
 
-// Reverse sweep with binomial checkpointing for one family: R carries the
-// family's fields; t0 is the initial state (FP32 fields in the FP32 modes).
-static int run_gradient(Rev& R, Ctx* c, const scafes_geom* g, int hw, int dtype, const void* t0,
-                        const void* target, int nsteps, int max_snapshots, double* grad, double* J, double* stats)
+// Reverse sweep with binomial checkpointing: R carries the family's fields;
+// t0 is the initial state (FP32 fields in the FP32 modes); a state is one
+// array of abytes (two in the compensated mode), an adjoint field abytes,
+// the gradient ngrad FP64 values.
+static int run_schedule(Rev& R, Ctx* c, int dtype, const void* t0, size_t abytes, long ngrad, const void* target,
+                        int nsteps, int max_snapshots, double* grad, double* J, double* stats)
 {
-    Geo geo;
-    int rc = make_geo(g, hw, &geo);
-    if (rc) return rc;
-    long ne = 1;
-    for (int p2 = 0; p2 < 3; ++p2) ne *= (p2 < g->dim) ? (g->n[p2] + 2 * hw) : 1;
-    const size_t es = dtype == SCAFES_F64 ? 8 : 4;
-    const size_t abytes = (size_t)ne * es;  // one array
+    int rc = SCAFES_OK;
     const int snaps = std::max(0, std::min(max_snapshots, nsteps - 1));
     R.c = c;
-    R.geo = geo;
-    R.g = g;
     R.dtype = dtype;
     R.lam_dtype = dtype == SCAFES_F64 ? SCAFES_F64 : SCAFES_F32;
-    R.ne = ne;
     R.sbytes = dtype == SCAFES_F32_KAHAN ? 2 * abytes : abytes;
     R.target = target;
     R.grad = grad;
     R.lcur = 0;
     R.nsteps = nsteps;
-    R.pair = R.family == FAMILY_PENNES && pennes_adjk_supported(geo);
+    R.pair = R.family == FAMILY_PENNES && pennes_adjk_supported(R.geo);
     R.fwd_steps = R.adj_steps = 0;
     R.J = 0.0;
     R.work = nullptr;
@@ -308,7 +322,7 @@ static int run_gradient(Rev& R, Ctx* c, const scafes_geom* g, int hw, int dtype,
         for (int s = 0; !rc && s < snaps; ++s) R.slots.push_back((char*)pool + (size_t)s * R.sbytes);
     }
     if (!rc) {
-        hipError_t e = hipMemsetAsync(grad, 0, (size_t)ne * sizeof(double), c->stream);
+        hipError_t e = hipMemsetAsync(grad, 0, (size_t)ngrad * sizeof(double), c->stream);
         if (e != hipSuccess) rc = check_hip(e, "gradient zero");
     }
     if (!rc) rc = R.treeverse(0, nsteps, (int)R.slots.size(), base);
@@ -326,6 +340,23 @@ static int run_gradient(Rev& R, Ctx* c, const scafes_geom* g, int hw, int dtype,
         stats[3] = (double)snaps * (double)R.sbytes;
     }
     return SCAFES_OK;
+}
+
+// Node-field families (Pennes, variable kappa): one state array per field
+// of the grid g with halo hw.
+static int run_gradient(Rev& R, Ctx* c, const scafes_geom* g, int hw, int dtype, const void* t0,
+                        const void* target, int nsteps, int max_snapshots, double* grad, double* J, double* stats)
+{
+    Geo geo;
+    int rc = make_geo(g, hw, &geo);
+    if (rc) return rc;
+    long ne = 1;
+    for (int p2 = 0; p2 < 3; ++p2) ne *= (p2 < g->dim) ? (g->n[p2] + 2 * hw) : 1;
+    const size_t es = dtype == SCAFES_F64 ? 8 : 4;
+    R.geo = geo;
+    R.g = g;
+    R.ne = ne;
+    return run_schedule(R, c, dtype, t0, (size_t)ne * es, ne, target, nsteps, max_snapshots, grad, J, stats);
 }
 
 extern "C" {
@@ -382,6 +413,54 @@ int scafes_heat7var_gradient(scafes_ctx* ctx, const scafes_geom* g, int hw, int 
     R.w = R.q = nullptr;
     R.p = nullptr;
     return run_gradient(R, c, g, hw, kDtype[mode], u0, target, nsteps, max_snapshots, grad, J, stats);
+}
+
+int scafes_yee_eps_gradient(scafes_ctx* ctx, int ncell, int dtype, double s, const void* f0, const void* eps,
+                            const void* target, int nsteps, int max_snapshots, double* grad, double* J,
+                            double* stats)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || !f0 || !eps || !target || !grad || !J) return fail(SCAFES_ERR_ARG, "yee eps gradient: null pointer");
+    YeeDims d;
+    if (!yee_dims(ncell, &d)) return fail(SCAFES_ERR_ARG, "yee eps gradient: need 2 <= ncell <= 1400");
+    if (nsteps < 1) return fail(SCAFES_ERR_ARG, "yee eps gradient: need nsteps >= 1");
+    if (!(s > 0.0) || s > 1.0 / 1.7320508075688772)
+        return fail(SCAFES_ERR_ARG, "yee eps gradient: Courant number must satisfy 0 < s <= 1/sqrt(3)");
+    if (dtype != SCAFES_F32 && dtype != SCAFES_F64) return fail(SCAFES_ERR_ARG, "yee eps gradient: bad dtype");
+    const size_t es = dtype == SCAFES_F64 ? 8 : 4;
+    Rev R;
+    R.family = FAMILY_YEE_EPS;
+    R.yd = d;
+    R.yee_s = s;
+    R.eps = eps;
+    R.ne = 6 * d.nf;
+    R.w = R.q = nullptr;
+    R.p = nullptr;
+    void* ce = nullptr;
+    void* hp = nullptr;
+    void* aux = nullptr;  // J partials, then J
+    const size_t naux = (size_t)yee_eps_partials() + 1;
+    hipError_t e = hipMalloc(&ce, (size_t)d.nf * es);
+    if (e == hipSuccess) e = hipMalloc(&hp, 3 * (size_t)d.nf * es);
+    if (e == hipSuccess) e = hipMalloc(&aux, naux * sizeof(double));
+    int rc = e == hipSuccess ? SCAFES_OK : check_hip(e, "yee eps gradient buffers");
+    if (!rc) rc = yee_eps_coef(c, d, dtype, s, eps, ce);
+    if (!rc) {
+        R.ce = ce;
+        R.hp = hp;
+        R.partial = (double*)aux;
+        R.jdev = (double*)aux + yee_eps_partials();
+        rc = run_schedule(R, c, dtype, f0, 6 * (size_t)d.nf * es, d.nf, target, nsteps, max_snapshots, grad, J,
+                          stats);
+    }
+    if (!rc) {
+        e = hipMemcpy(J, R.jdev, sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = check_hip(e, "yee eps gradient J");
+    }
+    (void)hipFree(ce);
+    (void)hipFree(hp);
+    (void)hipFree(aux);
+    return rc;
 }
 
 }  // extern "C"

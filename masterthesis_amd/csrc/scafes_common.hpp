@@ -67,9 +67,6 @@ __device__ inline bool is_border(const Geo& g, int i, int j, int k)
            k < g.nl2 || k > g.nz - 1 - g.nl2;
 }
 
-// Maps the hardware dispatch order (round-robin over the 8 XCDs of an
-// MI355X) to a logical tile order in which each XCD owns a contiguous run
-// of tiles, so y-neighbouring tiles share that XCD's L2.
 // Largest plane range in bytes of the buffer-resource kernels.  They issue
 // the same loads and stores on every iteration and send unneeded ones to the
 // out-of-range offset `range` (a load there returns 0 without a memory
@@ -77,6 +74,9 @@ __device__ inline bool is_border(const Geo& g, int i, int j, int k)
 // accesses and wait only for the ones a plane consumes.
 constexpr long kOobLimit = 0xFFFFFF00L;
 
+// Maps the hardware dispatch order (round-robin over the 8 XCDs of an
+// MI355X) to a logical tile order in which each XCD owns a contiguous run
+// of tiles, so y-neighbouring tiles share that XCD's L2.
 __device__ inline int xcd_remap(int b, int nb)
 {
     constexpr int kXcd = 8;
@@ -136,6 +136,17 @@ struct YeeDims {
     long M, M2;  // N+1, (N+1)^2
     long nf;     // (N+1)^3
 };
+// Yee with a relative permittivity field and its adjoint (defined in
+// yee_eps.hip).  ce = s / eps per node (state arithmetic); states are the six
+// Yee arrays; lam holds lam_E then lam_H; hp three arrays of scratch.
+bool yee_dims(int ncell, YeeDims* d);
+long yee_eps_partials();
+int yee_eps_coef(Ctx* c, const YeeDims& d, int dtype, double s, const void* eps, void* ce);
+int yee_eps_step(Ctx* c, const YeeDims& d, int dtype, double s, const void* ce, const void* a, void* b);
+int yee_eps_lam_init(Ctx* c, const YeeDims& d, int dtype, const void* fN, const void* target, void* lam,
+                     double* partial, double* J);
+int yee_eps_adjoint(Ctx* c, const YeeDims& d, int dtype, double s, const void* ce, const void* eps, const void* st,
+                    void* lam, void* hp, double* grad);
 // Two leapfrog steps per sweep, out of place (defined in yee_tb2.hip); ep0 /
 // ep1 (nullable): per-workgroup FP64 energy partials of the two steps.
 struct Tb2Plan {

@@ -795,6 +795,148 @@ double oracle_yee_energy(int N, const double* f, const double* h_prev)
     for (long m = 0; m < 3 * nf; ++m) h += f[3 * nf + m] * h_prev[m];
     return 0.5 * e + 0.5 * h;
 }
+
+/* Yee with a relative permittivity field eps (one value per node index m,
+ * shared by Ex(m), Ey(m), Ez(m)): D += s * (backward differences of H),
+ * E = D / eps, i.e. E += (s / eps) * (backward differences of H).  The
+ * reference's EMFDTD scales D to E per node by its material coefficient
+ * ga = 1 / (1 + sigma dt / eps0) (examples/EMFDTD/EMFDTD.hpp:351-356);
+ * 1 / eps takes that role here.  ce = s / eps is formed once per node. */
+static int yee_emask(int N, int c, int i, int j, int k)
+{
+    if (c == 0) return i < N && j >= 1 && j < N && k >= 1 && k < N;
+    if (c == 1) return j < N && i >= 1 && i < N && k >= 1 && k < N;
+    return k < N && i >= 1 && i < N && j >= 1 && j < N;
+}
+
+void oracle_yee_eps_half_e(int N, double s, const double* eps, double* f)
+{
+    const long M = N + 1, nf = M * M * M;
+    double *Ex = f, *Ey = f + nf, *Ez = f + 2 * nf, *Hx = f + 3 * nf, *Hy = f + 4 * nf,
+           *Hz = f + 5 * nf;
+#pragma omp parallel for schedule(static) num_threads(NTHR)
+    for (int k = 0; k <= N; ++k)
+        for (int j = 0; j <= N; ++j)
+            for (int i = 0; i <= N; ++i) {
+                const long m = YI(i, j, k);
+                const double ce = s / eps[m];
+                if (yee_emask(N, 0, i, j, k))
+                    Ex[m] = Ex[m] + ce * ((Hz[m] - Hz[YI(i, j - 1, k)]) - (Hy[m] - Hy[YI(i, j, k - 1)]));
+                if (yee_emask(N, 1, i, j, k))
+                    Ey[m] = Ey[m] + ce * ((Hx[m] - Hx[YI(i, j, k - 1)]) - (Hz[m] - Hz[YI(i - 1, j, k)]));
+                if (yee_emask(N, 2, i, j, k))
+                    Ez[m] = Ez[m] + ce * ((Hy[m] - Hy[YI(i - 1, j, k)]) - (Hx[m] - Hx[YI(i, j - 1, k)]));
+            }
+}
+
+void oracle_yee_eps_step(int N, double s, const double* eps, double* f)
+{
+    oracle_yee_half_h(N, s, f);
+    oracle_yee_eps_half_e(N, s, eps, f);
+}
+
+/* Gradient of J = 0.5 sum_{updated E entries} (E^N - target)^2 w.r.t. eps.
+ * The step is linear in (E, H): H' = H + s A E, E' = E + C B H' with
+ * A = -curl_f, B = curl_b, C = diag(ce) on the updated E entries; as
+ * curl_f^T = curl_b on the masked entries, its adjoint is again a pair of
+ * Yee half steps run backwards:
+ *   mu_H  = lam_H' - Hform(ce * lam_E')     (H-step stencil, coefficient -1)
+ *   lam_E = lam_E' - s * Eform(mu_H)        (E-step stencil, coefficient -s)
+ * with lam_E kept zero on entries the E step does not update, and
+ *   dJ/deps(m) += -(ce/eps)(m) * sum_c lam_E'_c(m) * Eform_c(H')(m),
+ * H' = H^{n-1} + s A E^{n-1} recomputed from the stored state.
+ * states: (nsteps+1) consecutive states (6 (N+1)^3 values each); target:
+ * three E arrays; grad: (N+1)^3 values.  Returns J. */
+double oracle_yee_eps_grad(int N, double s, const double* eps, const double* states, const double* target,
+                           int nsteps, double* grad)
+{
+    const long M = N + 1, nf = M * M * M;
+    double* lam = (double*)calloc((size_t)(6 * nf), sizeof(double));
+    double* hp = (double*)malloc((size_t)(3 * nf) * sizeof(double));
+    const double* fN = states + (long)nsteps * 6 * nf;
+    double J = 0.0;
+    for (int c = 0; c < 3; ++c)
+        for (int k = 0; k <= N; ++k)
+            for (int j = 0; j <= N; ++j)
+                for (int i = 0; i <= N; ++i) {
+                    const long m = YI(i, j, k);
+                    if (!yee_emask(N, c, i, j, k)) continue;
+                    const double r = fN[c * nf + m] - target[c * nf + m];
+                    lam[c * nf + m] = r;
+                    J += r * r;
+                }
+    for (long m = 0; m < nf; ++m) grad[m] = 0.0;
+    double *lEx = lam, *lEy = lam + nf, *lEz = lam + 2 * nf, *lHx = lam + 3 * nf, *lHy = lam + 4 * nf,
+           *lHz = lam + 5 * nf;
+    double *Px = hp, *Py = hp + nf, *Pz = hp + 2 * nf;
+    for (int n = nsteps; n >= 1; --n) {
+        const double* f = states + (long)(n - 1) * 6 * nf;
+        const double *Ex = f, *Ey = f + nf, *Ez = f + 2 * nf, *Hx = f + 3 * nf, *Hy = f + 4 * nf,
+                     *Hz = f + 5 * nf;
+        /* H' and mu_H (in place in lam_H; read only at its own entry) */
+#pragma omp parallel for schedule(static) num_threads(NTHR)
+        for (int k = 0; k <= N; ++k)
+            for (int j = 0; j <= N; ++j)
+                for (int i = 0; i <= N; ++i) {
+                    const long m = YI(i, j, k);
+#define W(F, ii, jj, kk) ((s / eps[YI(ii, jj, kk)]) * F[YI(ii, jj, kk)])
+                    Px[m] = Hx[m];
+                    Py[m] = Hy[m];
+                    Pz[m] = Hz[m];
+                    if (j < N && k < N) {
+                        Px[m] = Hx[m] + s * ((Ey[YI(i, j, k + 1)] - Ey[m]) - (Ez[YI(i, j + 1, k)] - Ez[m]));
+                        lHx[m] = lHx[m] - ((W(lEy, i, j, k + 1) - W(lEy, i, j, k)) -
+                                           (W(lEz, i, j + 1, k) - W(lEz, i, j, k)));
+                    }
+                    if (i < N && k < N) {
+                        Py[m] = Hy[m] + s * ((Ez[YI(i + 1, j, k)] - Ez[m]) - (Ex[YI(i, j, k + 1)] - Ex[m]));
+                        lHy[m] = lHy[m] - ((W(lEz, i + 1, j, k) - W(lEz, i, j, k)) -
+                                           (W(lEx, i, j, k + 1) - W(lEx, i, j, k)));
+                    }
+                    if (i < N && j < N) {
+                        Pz[m] = Hz[m] + s * ((Ex[YI(i, j + 1, k)] - Ex[m]) - (Ey[YI(i + 1, j, k)] - Ey[m]));
+                        lHz[m] = lHz[m] - ((W(lEx, i, j + 1, k) - W(lEx, i, j, k)) -
+                                           (W(lEy, i + 1, j, k) - W(lEy, i, j, k)));
+                    }
+#undef W
+                }
+        /* lam_E and the gradient (lam_E read and written only at its own
+         * entry; mu_H and H' at m and the -x/-y/-z neighbours) */
+#pragma omp parallel for schedule(static) num_threads(NTHR)
+        for (int k = 0; k <= N; ++k)
+            for (int j = 0; j <= N; ++j)
+                for (int i = 0; i <= N; ++i) {
+                    const long m = YI(i, j, k);
+                    double acc = 0.0;
+                    int any = 0;
+                    if (yee_emask(N, 0, i, j, k)) {
+                        const double l = lEx[m];
+                        acc = acc + l * ((Pz[m] - Pz[YI(i, j - 1, k)]) - (Py[m] - Py[YI(i, j, k - 1)]));
+                        lEx[m] = l - s * ((lHz[m] - lHz[YI(i, j - 1, k)]) - (lHy[m] - lHy[YI(i, j, k - 1)]));
+                        any = 1;
+                    }
+                    if (yee_emask(N, 1, i, j, k)) {
+                        const double l = lEy[m];
+                        acc = acc + l * ((Px[m] - Px[YI(i, j, k - 1)]) - (Pz[m] - Pz[YI(i - 1, j, k)]));
+                        lEy[m] = l - s * ((lHx[m] - lHx[YI(i, j, k - 1)]) - (lHz[m] - lHz[YI(i - 1, j, k)]));
+                        any = 1;
+                    }
+                    if (yee_emask(N, 2, i, j, k)) {
+                        const double l = lEz[m];
+                        acc = acc + l * ((Py[m] - Py[YI(i - 1, j, k)]) - (Px[m] - Px[YI(i, j - 1, k)]));
+                        lEz[m] = l - s * ((lHy[m] - lHy[YI(i - 1, j, k)]) - (lHx[m] - lHx[YI(i, j - 1, k)]));
+                        any = 1;
+                    }
+                    if (any) {
+                        const double ce = s / eps[m];
+                        grad[m] = grad[m] - (ce / eps[m]) * acc;
+                    }
+                }
+    }
+    free(lam);
+    free(hp);
+    return 0.5 * J;
+}
 #undef YI
 
 /*---------------------------------------------------------------------------*/

@@ -28,6 +28,10 @@
  *   oracle_yee_*        pinned: exact PEC cavity eigenmode with the Yee
  *                       dispersion relation, discrete energy invariant,
  *                       discrete Gauss law.
+ *   oracle_yee_eps_*    pinned: eps = 1 reduces to oracle_yee_step bit for
+ *                       bit; constant eps = a numpy-assembled step; the
+ *                       eps-weighted energy invariant; gradient vs central
+ *                       finite differences (every component, tiny grid).
  *   oracle_heat2d_*     pinned: 2-D sine eigenmode closed form, gradient vs
  *                       central finite differences.
  *   ADOL-C forward-mode gradients of the reference: parity unpinned (ADOL-C
@@ -168,6 +172,18 @@ void oracle_yee_step(int ncell, double s, double* f);
  * E^n and H^{n+1/2} (state right after an H half step), h_prev the three
  * H components before it.  Invariant of the scheme in exact arithmetic. */
 double oracle_yee_energy(int ncell, const double* f, const double* h_prev);
+/* Yee with a relative permittivity field eps ((N+1)^3 values, one per node
+ * index, shared by the three E components there): the E half step becomes
+ * E += (s / eps) * (backward differences of H), i.e. D += s curl H with
+ * E = D / eps (the per-node D -> E scaling of examples/EMFDTD/EMFDTD.hpp:351-356).
+ * eps = 1 gives oracle_yee_step bit for bit. */
+void oracle_yee_eps_half_e(int ncell, double s, const double* eps, double* f);
+void oracle_yee_eps_step(int ncell, double s, const double* eps, double* f);
+/* Discrete adjoint gradient of J = 0.5 sum over updated E entries of
+ * (E^N - target)^2 w.r.t. eps; states holds the nsteps+1 states, target the
+ * three E arrays.  Writes grad ((N+1)^3), returns J. */
+double oracle_yee_eps_grad(int ncell, double s, const double* eps, const double* states, const double* target,
+                           int nsteps, double* grad);
 
 /* Batched-ensemble member: 2-D heat, scalar kappa, zero Dirichlet border,
  * n x n nodes (no ghost layer), u_new = u + tau*kappa*lap(u) on inner nodes

@@ -112,6 +112,10 @@ def lib():
         L.oracle_yee_step.argtypes = [C.c_int, C.c_double, dp]
         L.oracle_yee_energy.restype = C.c_double
         L.oracle_yee_energy.argtypes = [C.c_int, dp, dp]
+        L.oracle_yee_eps_half_e.argtypes = [C.c_int, C.c_double, dp, dp]
+        L.oracle_yee_eps_step.argtypes = [C.c_int, C.c_double, dp, dp]
+        L.oracle_yee_eps_grad.restype = C.c_double
+        L.oracle_yee_eps_grad.argtypes = [C.c_int, C.c_double, dp, dp, dp, C.c_int, dp]
         L.oracle_heat2d_run.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp]
         L.oracle_heat2d_grad.restype = C.c_double
         L.oracle_heat2d_grad.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
@@ -302,6 +306,43 @@ def yee_half_e(ncell, s, f):
 
 def yee_energy(ncell, f, h_prev):
     return lib().oracle_yee_energy(ncell, ptr(f), ptr(np.ascontiguousarray(h_prev)))
+
+
+def yee_eps_step(ncell, s, eps, f, nsteps=1):
+    """f: (6, N+1, N+1, N+1) state [field, k, j, i]; eps: (N+1)^3."""
+    eps = np.ascontiguousarray(eps, dtype=np.float64)
+    for _ in range(nsteps):
+        lib().oracle_yee_eps_step(ncell, s, ptr(eps), ptr(f))
+    return f
+
+
+def yee_eps_forward_all(ncell, s, eps, f0, nsteps):
+    """(nsteps+1, 6, N+1, N+1, N+1): the states 0..nsteps."""
+    st = np.empty((nsteps + 1,) + f0.shape)
+    st[0] = f0
+    for n in range(nsteps):
+        st[n + 1] = st[n]
+        yee_eps_step(ncell, s, eps, st[n + 1])
+    return st
+
+
+def yee_eps_grad(ncell, s, eps, states, target, nsteps):
+    """(J, dJ/deps) with target the three E arrays (3, N+1, N+1, N+1)."""
+    eps = np.ascontiguousarray(eps, dtype=np.float64)
+    grad = np.zeros(eps.shape)
+    J = lib().oracle_yee_eps_grad(ncell, s, ptr(eps), ptr(np.ascontiguousarray(states)),
+                                  ptr(np.ascontiguousarray(target, dtype=np.float64)), nsteps, ptr(grad))
+    return J, grad
+
+
+def yee_emask(ncell):
+    """(3, N+1, N+1, N+1) bool: the E entries the E half step updates
+    (tangential E on the PEC walls and the padding entries stay fixed)."""
+    m = ncell + 1
+    k, j, i = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
+    inner = lambda a: (a >= 1) & (a < ncell)
+    return np.stack([(i < ncell) & inner(j) & inner(k), (j < ncell) & inner(i) & inner(k),
+                     (k < ncell) & inner(i) & inner(j)])
 
 
 def heat2d_run(n, kappa, tau, nsteps, u0):

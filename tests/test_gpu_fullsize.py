@@ -22,6 +22,9 @@ seconds and otherwise through properties that do not depend on the size:
  * The var-kappa field gradient: on the 512^3 input, 3 snapshots and
    store-all bit-identical (FP32+Kahan); on 128^3, FP64 against a central
    finite difference of J.
+ * The Yee permittivity gradient on 384^3 (pulse, dielectric sphere): two
+   FP64 steps bit-identical to the oracle (state and gradient), FP32 within
+   1e-4, FP32 snapshot budgets bit-identical.
 """
 import ctypes as C
 
@@ -388,3 +391,63 @@ def test_kappa_gradient_fp64_vs_finite_difference_128(ctx):
     assert abs(J_of(kap) - J.value) <= 1e-10 * J.value
     for d in (u0, k0, tgt, grad):
         d.free()
+
+
+def test_yee384_eps_gradient_matches_oracle(ctx):
+    """Permittivity gradient on the 384^3 Yee box (centred pulse, dielectric
+    sphere eps = 4, target from eps = 4.4): two FP64 steps -- forward state
+    and gradient bit-identical to the oracle; the FP32 gradient within 1e-4
+    relative L2 of it; FP32 with 1 snapshot and with store-all over 9 steps
+    bit-identical."""
+    N, s = 384, 0.5
+    m = N + 1
+    nf = m ** 3
+    ax = np.arange(m, dtype=np.float64)
+    r2 = (ax[:, None, None] - 0.6 * N) ** 2 + (ax[None, :, None] - 0.6 * N) ** 2 + (ax[None, None, :] - 0.5 * N) ** 2
+    inside = r2 <= (N / 4.0) ** 2
+    del r2
+    eps = np.where(inside, 4.0, 1.0)
+    eps_t = np.where(inside, 4.4, 1.0)
+    del inside
+    f = ctx.alloc(6 * nf, np.float64)
+    capi.check(ctx.L.scafes_yee_init_pulse(ctx.h, N, capi.F64, f.ptr, 8.0))
+    f0 = f.read().reshape(6, m, m, m)
+    f.free()
+    O.lib().oracle_set_threads(16)
+    target = np.ascontiguousarray(O.yee_eps_step(N, s, eps_t, f0.copy(), 2)[:3])
+    del eps_t
+    st = O.yee_eps_forward_all(N, s, eps, f0, 2)
+    Jref, gref = O.yee_eps_grad(N, s, eps, st, target, 2)
+    # forward
+    d, de, w = ctx.upload(f0), ctx.upload(eps), ctx.alloc(6 * nf)
+    capi.check(ctx.L.scafes_yee_eps_run(ctx.h, N, capi.F64, s, de.ptr, d.ptr, w.ptr, 2))
+    np.testing.assert_array_equal(d.read().reshape(f0.shape), st[2])
+    del st
+    for b in (d, de, w):
+        b.free()
+
+    def grad(dt, nsteps, snaps):
+        bufs = [ctx.upload(a.astype(dt)) for a in (f0, eps, target)]
+        g = ctx.alloc(nf)
+        J = C.c_double(0)
+        capi.check(ctx.L.scafes_yee_eps_gradient(ctx.h, N, capi.F64 if dt == np.float64 else capi.F32, s,
+                                                 bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, nsteps, snaps, g.ptr,
+                                                 C.byref(J), None))
+        out = g.read().reshape(eps.shape)
+        for b in bufs + [g]:
+            b.free()
+        return J.value, out
+
+    J64, g64 = grad(np.float64, 2, 1)
+    np.testing.assert_array_equal(g64, gref)
+    # J sums 1.7e8 squares in another order (block tree vs sequential): the
+    # rounding of a sequential sum of n terms grows like sqrt(n) * 1.1e-16
+    assert abs(J64 - Jref) <= 1e-10 * Jref
+    J32, g32 = grad(np.float32, 2, 1)
+    rel = np.linalg.norm(g32 - gref) / np.linalg.norm(gref)
+    print(f"\n[yee eps gradient 384^3 x 2, FP32] rel L2 err vs FP64 oracle {rel:.2e}")
+    assert rel < 1e-4 and np.abs(gref).max() > 0
+    Ja, ga = grad(np.float32, 9, 1)
+    Jb, gb = grad(np.float32, 9, 8)
+    np.testing.assert_array_equal(ga, gb)
+    assert Ja == Jb > 0
