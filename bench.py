@@ -420,8 +420,10 @@ def bench_yee_eps(ctx, capi, ncell=384, nsteps=200, snapshots=(20, 100)):
                        f"{nsteps} steps, FP32 fields, FP64 gradient",
            "fwd_eps_ms": round(t_eps, 2), "fwd_eps_gcell_steps_per_s": round(cells * nsteps / (t_eps * 1e-3) / 1e9, 2),
            "fwd_vacuum_ms": round(t_vac, 2),
-           "fwd_kernels": "h_sweep<float,false> + e_sweep<float,false> (76 B/cell-step moved, 52 B algorithmic)",
-           "adjoint_kernels": "h_sweep<float,true> + e_sweep<float,true> (164 B/cell-step moved)",
+           "fwd_kernels": "yee_tb2_kernel<float, 64, 16, 4, true> (two steps per sweep, 26 B/cell-step algorithmic); "
+                          "yee_fused_kernel<float, 64, 8, 2, true> for a single step (52 B)",
+           "adjoint_kernel": "adj_fused<float, 64, 16, 2> (one reverse step per sweep, 96 B/cell-step algorithmic)",
+           **({"adjoint_profile": _profile_fields("yee_eps_adjoint_384.md", True)} if ncell == 384 else {}),
            "gradient": []}
     grad = ctx.alloc(nf, np.float64)
     for s in snapshots:

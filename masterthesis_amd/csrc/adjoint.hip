@@ -79,13 +79,12 @@ struct Rev {
     const void* kappa = nullptr;
     double tau = 0.0, border_value = 0.0;
     int border_mode = SCAFES_BORDER_KEEP;
-    // FAMILY_YEE_EPS: box, Courant number, ce = s/eps and eps fields, H'
-    // scratch (three arrays), J partials and J on the device
+    // FAMILY_YEE_EPS: box, Courant number, ce = s/eps and eps fields, J
+    // partials and J on the device
     YeeDims yd{};
     double yee_s = 0.0;
     const void* ce = nullptr;
     const void* eps = nullptr;
-    void* hp = nullptr;
     double* partial = nullptr;
     double* jdev = nullptr;
     Ctx* c;
@@ -129,14 +128,24 @@ struct Rev {
         return e == hipSuccess ? SCAFES_OK : check_hip(e, "checkpoint copy");
     }
     // State after n steps from src, written to dst (src is never written).
+    // The Yee family advances two steps per sweep where two remain.
     int advance(const void* src, long n, void* dst)
     {
         if (n == 0) return src == dst ? SCAFES_OK : copy(dst, src);
+        const bool two = family == FAMILY_YEE_EPS && yee_eps_two_step_supported(yd, dtype);
         const void* x = src;
-        for (long s = 1; s <= n; ++s) {
-            void* out = (s == n) ? dst : tmp[s & 1];
-            if (out == x) out = tmp[(s + 1) & 1];
-            int r = step(x, out);
+        for (long done = 0; done < n;) {
+            const int k = (two && n - done >= 2) ? 2 : 1;
+            done += k;
+            void* out = (done == n) ? dst : (x == tmp[0] ? tmp[1] : tmp[0]);
+            if (out == x) out = (x == tmp[0]) ? tmp[1] : tmp[0];
+            int r;
+            if (k == 2) {
+                fwd_steps += 2;
+                r = yee_eps_step2(c, yd, dtype, yee_s, ce, x, out);
+            } else {
+                r = step(x, out);
+            }
             if (r) return r;
             x = out;
         }
@@ -146,14 +155,16 @@ struct Rev {
     int adjoint(int n, const void* tprev)
     {
         if (family == FAMILY_YEE_EPS) {
-            // in place on lam[0]; J^N goes to the device and is read at the end
+            // J^N goes to the device and is read at the end
             if (n == nsteps) {
                 int r = step(tprev, tmp[0]);
-                if (!r) r = yee_eps_lam_init(c, yd, dtype, tmp[0], target, lam[0], partial, jdev);
+                if (!r) r = yee_eps_lam_init(c, yd, dtype, tmp[0], target, lam[lcur], partial, jdev);
                 if (r) return r;
             }
             ++adj_steps;
-            return yee_eps_adjoint(c, yd, dtype, yee_s, ce, eps, tprev, lam[0], hp, grad);
+            const int r = yee_eps_adjoint(c, yd, dtype, yee_s, ce, eps, tprev, lam[lcur], lam[1 - lcur], grad);
+            lcur = 1 - lcur;
+            return r;
         }
         if (n == nsteps) {
             // first leaf: T^N from T^{N-1}, then lam^N and J
@@ -437,17 +448,14 @@ int scafes_yee_eps_gradient(scafes_ctx* ctx, int ncell, int dtype, double s, con
     R.w = R.q = nullptr;
     R.p = nullptr;
     void* ce = nullptr;
-    void* hp = nullptr;
     void* aux = nullptr;  // J partials, then J
     const size_t naux = (size_t)yee_eps_partials() + 1;
     hipError_t e = hipMalloc(&ce, (size_t)d.nf * es);
-    if (e == hipSuccess) e = hipMalloc(&hp, 3 * (size_t)d.nf * es);
     if (e == hipSuccess) e = hipMalloc(&aux, naux * sizeof(double));
     int rc = e == hipSuccess ? SCAFES_OK : check_hip(e, "yee eps gradient buffers");
     if (!rc) rc = yee_eps_coef(c, d, dtype, s, eps, ce);
     if (!rc) {
         R.ce = ce;
-        R.hp = hp;
         R.partial = (double*)aux;
         R.jdev = (double*)aux + yee_eps_partials();
         rc = run_schedule(R, c, dtype, f0, 6 * (size_t)d.nf * es, d.nf, target, nsteps, max_snapshots, grad, J,
@@ -458,7 +466,6 @@ int scafes_yee_eps_gradient(scafes_ctx* ctx, int ncell, int dtype, double s, con
         if (e != hipSuccess) rc = check_hip(e, "yee eps gradient J");
     }
     (void)hipFree(ce);
-    (void)hipFree(hp);
     (void)hipFree(aux);
     return rc;
 }

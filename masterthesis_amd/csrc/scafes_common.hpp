@@ -138,15 +138,23 @@ struct YeeDims {
 };
 // Yee with a relative permittivity field and its adjoint (defined in
 // yee_eps.hip).  ce = s / eps per node (state arithmetic); states are the six
-// Yee arrays; lam holds lam_E then lam_H; hp three arrays of scratch.
+// Yee arrays; adjoint states hold lam_E then lam_H.
 bool yee_dims(int ncell, YeeDims* d);
 long yee_eps_partials();
 int yee_eps_coef(Ctx* c, const YeeDims& d, int dtype, double s, const void* eps, void* ce);
 int yee_eps_step(Ctx* c, const YeeDims& d, int dtype, double s, const void* ce, const void* a, void* b);
+// Two steps a -> b in one sweep (the two-step kernel with ce); only where
+// yee_eps_two_step_supported.
+bool yee_eps_two_step_supported(const YeeDims& d, int dtype);
+int yee_eps_step2(Ctx* c, const YeeDims& d, int dtype, double s, const void* ce, const void* a, void* b);
 int yee_eps_lam_init(Ctx* c, const YeeDims& d, int dtype, const void* fN, const void* target, void* lam,
                      double* partial, double* J);
+// One adjoint step: st = state n-1, la = (lam_E', lam_H') -> lb, grad +=.
 int yee_eps_adjoint(Ctx* c, const YeeDims& d, int dtype, double s, const void* ce, const void* eps, const void* st,
-                    void* lam, void* hp, double* grad);
+                    const void* la, void* lb, double* grad);
+// One fused leapfrog step a -> b with the E coefficient field ce (yee.hip).
+template <typename T>
+int yee_fused_eps_launch(Ctx* c, const YeeDims& d, T s, const T* a, T* b, const T* ce);
 // Two leapfrog steps per sweep, out of place (defined in yee_tb2.hip); ep0 /
 // ep1 (nullable): per-workgroup FP64 energy partials of the two steps.
 struct Tb2Plan {
@@ -156,7 +164,8 @@ struct Tb2Plan {
 bool yee_tb2_supported(const YeeDims& d, int elem_bytes);
 Tb2Plan yee_tb2_plan(const YeeDims& d, int elem_bytes, int cu_count);
 template <typename T>
-int yee_tb2_launch(Ctx* c, const YeeDims& d, const Tb2Plan& pl, T s, const T* a, T* b, double* ep0, double* ep1);
+int yee_tb2_launch(Ctx* c, const YeeDims& d, const Tb2Plan& pl, T s, const T* a, T* b, double* ep0, double* ep1,
+                   const T* ce = nullptr);
 
 // Reductions (defined in scafes_capi.hip).
 enum RedOp { RED_MAX_ABS_DIFF = 0, RED_SUM_SQ_DIFF = 1, RED_SUM_PROD = 2 };

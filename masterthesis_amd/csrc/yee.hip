@@ -169,11 +169,13 @@ __device__ inline void yst(T v, YRsrc r, uint32_t off)
 // their nodes and write every field value there (updated or copied), so fb
 // is a complete state.  Per cell and step HBM moves 6 reads + 6 writes.
 // The arithmetic per value is that of yee_h_kernel / yee_e_kernel, so the
-// result is bit-identical to the two-pass in-place step.
-template <typename T, int TX, int TY, int D>
+// result is bit-identical to the two-pass in-place step.  EPS: the E half
+// step uses the per-node coefficient ce = s / eps (queued along z with the
+// fields) in place of s.
+template <typename T, int TX, int TY, int D, bool EPS = false>
 __global__ __launch_bounds__(TX* TY) void yee_fused_kernel(YeeDims d, T s, const T* __restrict__ fa,
                                                             T* __restrict__ fb, int zc, int tiles_x, int tiles_y,
-                                                            double* __restrict__ epart)
+                                                            double* __restrict__ epart, const T* __restrict__ ce)
 {
     static_assert(D >= 2 && D % 2 == 0, "prefetch depth must be even");
     constexpr int OX = TX - 1, OY = TY - 1;
@@ -200,13 +202,14 @@ __global__ __launch_bounds__(TX* TY) void yee_fused_kernel(YeeDims d, T s, const
     char* b8 = (char*)fb;
     auto plane = [&](int kk) { return (size_t)min(max(kk, 0), N) * pb; };
 
-    T qex[D], qey[D], qez[D], qhx[D], qhy[D], qhz[D], rxy[D], rxz[D], ryx[D], ryz[D];
+    T qex[D], qey[D], qez[D], qhx[D], qhy[D], qhz[D], rxy[D], rxz[D], ryx[D], ryz[D], qce[D];
 #pragma unroll
     for (int q = 0; q < D; ++q) qex[q] = qey[q] = qez[q] = qhx[q] = qhy[q] = qhz[q] = rxy[q] = rxz[q] = ryx[q] =
-        ryz[q] = T(0);
+        ryz[q] = qce[q] = T(0);
     auto fetch = [&](const int slot, const int kk) {
         if (kk <= k1 + 1) {
             const char* p = a8 + plane(kk);
+            if constexpr (EPS) qce[slot] = yld<T>(yrsrc((const char*)ce + plane(kk), range), oc);
             const YRsrc rEx = yrsrc(p, range), rEy = yrsrc(p + fs, range), rEz = yrsrc(p + 2 * fs, range);
             qex[slot] = yld<T>(rEx, oc);
             qey[slot] = yld<T>(rEy, oc);
@@ -232,6 +235,7 @@ __global__ __launch_bounds__(TX* TY) void yee_fused_kernel(YeeDims d, T s, const
     T pex = T(0), pey = T(0), pez = T(0);  // E^n at plane k-1
     T phx = T(0), phy = T(0), phz = T(0);  // H^{n+1/2} at plane k-1
     T qqx = T(0), qqy = T(0);              // Hx, Hy^{n+1/2} at plane k-2
+    T pc = s;                              // E coefficient at plane k-1
     double esum = 0.0;
     for (int kb = k0 - 1; kb <= k1; kb += D) {
 #pragma unroll
@@ -243,6 +247,7 @@ __global__ __launch_bounds__(TX* TY) void yee_fused_kernel(YeeDims d, T s, const
             const T ex = qex[q], ey = qey[q], ez = qez[q];
             const T exu = qex[q1], eyu = qey[q1];
             const T hx = qhx[q], hy = qhy[q], hz = qhz[q];
+            const T c0 = qce[q];  // slot q is refilled below
             sEx[buf][ty][tx] = ex;
             sEy[buf][ty][tx] = ey;
             sEz[buf][ty][tx] = ez;
@@ -285,9 +290,9 @@ __global__ __launch_bounds__(TX* TY) void yee_fused_kernel(YeeDims d, T s, const
                 const T hz_xm = sHz[pbf][ty][tx - 1], hy_xm = sHy[pbf][ty][tx - 1];
                 const bool in_k = km >= 1 && km < N, in_j = j >= 1 && j < N, in_i = i >= 1 && i < N;
                 T exn = pex, eyn = pey, ezn = pez;
-                if (i < N && in_j && in_k) exn = pex + s * ((phz - hz_ym) - (phy - qqy));
-                if (j < N && in_i && in_k) eyn = pey + s * ((phx - qqx) - (phz - hz_xm));
-                if (km < N && in_i && in_j) ezn = pez + s * ((phy - hy_xm) - (phx - hx_ym));
+                if (i < N && in_j && in_k) exn = pex + pc * ((phz - hz_ym) - (phy - qqy));
+                if (j < N && in_i && in_k) eyn = pey + pc * ((phx - qqx) - (phz - hz_xm));
+                if (km < N && in_i && in_j) ezn = pez + pc * ((phy - hy_xm) - (phx - hx_ym));
                 const size_t o = (size_t)km * pb;
                 yst<T>(exn, yrsrc(b8 + o, range), oc);
                 yst<T>(eyn, yrsrc(b8 + o + fs, range), oc);
@@ -301,6 +306,7 @@ __global__ __launch_bounds__(TX* TY) void yee_fused_kernel(YeeDims d, T s, const
             pex = ex;
             pey = ey;
             pez = ez;
+            if constexpr (EPS) pc = c0;
         }
     }
     if (epart) {
@@ -399,7 +405,7 @@ static int yee_run_t(Ctx* c, int N, double s, T* f, T* f2, int nsteps, int energ
         const bool r = rec(st);
         if (fused) {
             yee_fused_kernel<T, kFTX, kFTY, kFD><<<dim3((unsigned)nb), blk, 0, c->stream>>>(
-                d, (T)s, a, b, ZC, tiles_x, tiles_y, r ? epart : nullptr);
+                d, (T)s, a, b, ZC, tiles_x, tiles_y, r ? epart : nullptr, nullptr);
             std::swap(a, b);
         } else {
             yee_h_kernel<T, kYTX, kYTY><<<dim3((unsigned)nb), blk, 0, c->stream>>>(d, (T)s, f, ZC, tiles_x,
@@ -424,6 +430,22 @@ static int yee_run_t(Ctx* c, int N, double s, T* f, T* f2, int nsteps, int energ
     }
     return rc;
 }
+
+template <typename T>
+int yee_fused_eps_launch(Ctx* c, const YeeDims& d, T s, const T* a, T* b, const T* ce)
+{
+    if (d.M2 * (long)sizeof(T) >= (1L << 32)) return fail(SCAFES_ERR_ARG, "yee: plane too large");
+    const int tiles_x = ceil_div(d.N + 1, kFTX - 1), tiles_y = ceil_div(d.N + 1, kFTY - 1);
+    const long nb = (long)tiles_x * tiles_y * ceil_div(d.N + 1, kFZC);
+    if (nb > 0x7fffffffL) return fail(SCAFES_ERR_ARG, "yee: grid too large");
+    yee_fused_kernel<T, kFTX, kFTY, kFD, true><<<dim3((unsigned)nb), dim3(kFTX, kFTY), 0, c->stream>>>(
+        d, s, a, b, kFZC, tiles_x, tiles_y, nullptr, ce);
+    SCAFES_HIP(hipGetLastError());
+    return SCAFES_OK;
+}
+
+template int yee_fused_eps_launch<float>(Ctx*, const YeeDims&, float, const float*, float*, const float*);
+template int yee_fused_eps_launch<double>(Ctx*, const YeeDims&, double, const double*, double*, const double*);
 
 }  // namespace scafes
 

@@ -14,9 +14,11 @@
 //   lam_E = lam_E' - s * Eform(mu_H)       (E-step stencil, coefficient -s)
 //   grad(m) -= (ce/eps)(m) * sum_c lam_E'_c(m) * Eform_c(H')(m)
 // with H' recomputed from the stored state and lam_E zero on entries the E
-// step never updates.  Both halves are z-streaming column sweeps of the same
-// shape as the forward half steps; the adjoint fields are updated in place
-// (each is written only at the node that reads it there).
+// step never updates.  One adjoint step is one out-of-place sweep
+// (adj_fused below).  Forward steps use the Yee kernels with the ce field:
+// two steps per sweep (yee_tb2.hip) where two remain, else the fused
+// one-step sweep (yee.hip); their per-value arithmetic is that of the
+// half steps above.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,10 +28,6 @@
 namespace scafes {
 
 namespace {
-
-constexpr int kETX = 64, kETY = 4, kEZC = 16;
-
-__device__ inline long ye(const YeeDims& d, int i, int j, int k) { return i + d.M * j + d.M2 * k; }
 
 // E entries the E half step updates (tangential E on the PEC walls and the
 // padding entries at index N stay fixed).
@@ -47,143 +45,207 @@ __global__ void ce_kernel(long nf, double s, const T* __restrict__ eps, T* __res
         ce[m] = (T)(s / (double)eps[m]);
 }
 
-// H-shaped sweep (forward differences in x, y, z).  Writes H' = H + s A E
-// of the state fa into hout (every node; the entries the H step does not
-// update are copied).  ADJ: also mu_H = lam_H - Hform(ce * lam_E) in place
-// in lam_H on the updated entries.
-template <typename T, bool ADJ>
-__global__ __launch_bounds__(kETX* kETY) void h_sweep(YeeDims d, T s, const T* __restrict__ fa,
-                                                       T* __restrict__ hout, const T* __restrict__ ce,
-                                                       T* lam, int zc, int tiles_x, int tiles_y)
+// One adjoint step in one out-of-place sweep: st = state n-1 (E, H), la =
+// (lam_E', lam_H') -> lb = (lam_E, lam_H), grad += the step's term.  The
+// layout follows the fused forward step (yee.hip): a TX x TY workgroup
+// whose -x/-y edge threads only compute; per z plane k
+//   level 1 at plane k:   H'(k) = H + s Hform(E) and mu(k) = lam_H' -
+//                         Hform(w), w = ce * lam_E' (E and w planes in LDS
+//                         with a +x/+y ring; plane k+1 from the z queue);
+//                         lam_H(k) = mu(k) is stored;
+//   level 2 at plane k-1: lam_E(k-1) = lam_E' - s Eform(mu) and the
+//                         gradient term lam_E' . Eform(H') (H' and mu planes
+//                         in LDS for the -x/-y neighbours, plane k-2 in
+//                         registers).
+// Every owned node gets every output value (lam_E = 0 off the E mask,
+// lam_H = lam_H' off the H mask), so lb is a complete adjoint state.  The
+// per-value arithmetic is that of the oracle's two half-step loops.
+template <typename T, int TX, int TY, int D>
+__global__ __launch_bounds__(TX* TY) void adj_fused(YeeDims d, T s, const T* __restrict__ st,
+                                                     const T* __restrict__ la, T* __restrict__ lb,
+                                                     const T* __restrict__ ce, const T* __restrict__ eps,
+                                                     double* __restrict__ grad, int zc, int tiles_x, int tiles_y)
 {
-    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    static_assert(D >= 2 && D % 2 == 0, "prefetch depth must be even");
+    constexpr int OX = TX - 1, OY = TY - 1;
+    __shared__ T sE[3][2][TY + 1][TX + 1];  // E^{n-1} plane with the +x/+y ring
+    __shared__ T sW[3][2][TY + 1][TX + 1];  // w = ce * lam_E' plane with the ring
+    __shared__ T sP[3][2][TY][TX];          // H' plane
+    __shared__ T sM[3][2][TY][TX];          // mu plane
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int lbk = xcd_remap(blockIdx.x, gridDim.x);
     const int per = tiles_x * tiles_y;
-    const int tz = lb / per, r = lb - tz * per, tyi = r / tiles_x, txi = r - tyi * tiles_x;
+    const int tz = lbk / per, r = lbk - tz * per, tyi = r / tiles_x, txi = r - tyi * tiles_x;
     const int N = d.N;
-    const int i = txi * kETX + (int)threadIdx.x, j = tyi * kETY + (int)threadIdx.y;
-    if (i > N || j > N) return;
-    const int ip = min(i + 1, N), jp = min(j + 1, N);
-    const int k0 = tz * zc, k1 = min(k0 + zc, N + 1);
-    const T* __restrict__ Ex = fa;
-    const T* __restrict__ Ey = fa + d.nf;
-    const T* __restrict__ Ez = fa + 2 * d.nf;
-    const T* __restrict__ Hx = fa + 3 * d.nf;
-    const T* __restrict__ Hy = fa + 4 * d.nf;
-    const T* __restrict__ Hz = fa + 5 * d.nf;
-    // lam: lam_E (x, y, z) then lam_H (x, y, z), nf values each
     const long nf = d.nf;
-    long m = ye(d, i, j, k0);
-    T ex = Ex[m], ey = Ey[m], ez = Ez[m];
-    T wx = T(0), wy = T(0), wz = T(0);
-    if constexpr (ADJ) {
-        const T c = ce[m];
-        wx = c * lam[m];
-        wy = c * lam[nf + m];
-        wz = c * lam[2 * nf + m];
-    }
-    for (int k = k0; k < k1; ++k) {
-        const long mz = ye(d, i, j, min(k + 1, N)), mx = ye(d, ip, j, k), my = ye(d, i, jp, k);
-        const T exn = Ex[mz], eyn = Ey[mz], ezn = Ez[mz];
-        const T ez_xp = Ez[mx], ey_xp = Ey[mx];
-        const T ez_yp = Ez[my], ex_yp = Ex[my];
-        const bool ux = j < N && k < N, uy = i < N && k < N, uz = i < N && j < N;
-        const T hx = Hx[m], hy = Hy[m], hz = Hz[m];
-        hout[m] = ux ? hx + s * ((eyn - ey) - (ez_yp - ez)) : hx;
-        hout[d.nf + m] = uy ? hy + s * ((ez_xp - ez) - (exn - ex)) : hy;
-        hout[2 * d.nf + m] = uz ? hz + s * ((ex_yp - ex) - (ey_xp - ey)) : hz;
-        T wxn = T(0), wyn = T(0), wzn = T(0);
-        if constexpr (ADJ) {
-            const T cz = ce[mz], cx = ce[mx], cy = ce[my];
-            wxn = cz * lam[mz];
-            wyn = cz * lam[nf + mz];
-            wzn = cz * lam[2 * nf + mz];
-            const T wz_xp = cx * lam[2 * nf + mx], wy_xp = cx * lam[nf + mx];
-            const T wz_yp = cy * lam[2 * nf + my], wx_yp = cy * lam[my];
-            if (ux) lam[3 * nf + m] = lam[3 * nf + m] - ((wyn - wy) - (wz_yp - wz));
-            if (uy) lam[4 * nf + m] = lam[4 * nf + m] - ((wz_xp - wz) - (wxn - wx));
-            if (uz) lam[5 * nf + m] = lam[5 * nf + m] - ((wx_yp - wx) - (wy_xp - wy));
-        }
-        ex = exn;
-        ey = eyn;
-        ez = ezn;
-        wx = wxn;
-        wy = wyn;
-        wz = wzn;
-        m = mz;
-    }
-}
+    const int i = txi * OX - 1 + tx, j = tyi * OY - 1 + ty;
+    const int ic = min(max(i, 0), N), jc = min(max(j, 0), N);
+    const int ip = min(ic + 1, N), jp = min(jc + 1, N);
+    const bool out = tx >= 1 && ty >= 1 && i <= N && j <= N;
+    const bool rx = tx == TX - 1, ry = ty == TY - 1;
+    const int k0 = tz * zc, k1 = min(k0 + zc, N + 1);
+    // in-plane offsets fit 32 bits (ncell <= 1400); plane bases are uniform
+    const int oc = ic + (int)d.M * jc, ox = ip + (int)d.M * jc, oy = ic + (int)d.M * jp;
+    auto plane = [&](int kk) { return (long)min(max(kk, 0), N) * d.M2; };
 
-// E-shaped sweep (backward differences).  Forward (ADJ false): eout =
-// ein + ce * Eform(hin) on the updated entries, ein elsewhere.  Adjoint: e
-// holds lam_E' and becomes lam_E = lam_E' - s * Eform(mu) on the updated
-// entries; grad(m) -= (ce/eps)(m) * sum_c lam_E'_c Eform_c(hin) in FP64.
-template <typename T, bool ADJ>
-__global__ __launch_bounds__(kETX* kETY) void e_sweep(YeeDims d, T s, const T* __restrict__ ein,
-                                                       const T* __restrict__ hin, T* __restrict__ e,
-                                                       const T* __restrict__ ce, const T* __restrict__ eps,
-                                                       const T* __restrict__ mu, double* __restrict__ grad, int zc,
-                                                       int tiles_x, int tiles_y)
-{
-    const int lb = xcd_remap(blockIdx.x, gridDim.x);
-    const int per = tiles_x * tiles_y;
-    const int tz = lb / per, r = lb - tz * per, tyi = r / tiles_x, txi = r - tyi * tiles_x;
-    const int N = d.N;
-    const int i = txi * kETX + (int)threadIdx.x, j = tyi * kETY + (int)threadIdx.y;
-    if (i > N || j > N) return;
-    const int im = max(i - 1, 0), jm = max(j - 1, 0);
-    const int k0 = tz * zc, k1 = min(k0 + zc, N + 1);
-    const T* __restrict__ Px = hin;
-    const T* __restrict__ Py = hin + d.nf;
-    const T* __restrict__ Pz = hin + 2 * d.nf;
-    const long nf = d.nf;
-    const long m0 = ye(d, i, j, max(k0 - 1, 0));
-    T pxm = Px[m0], pym = Py[m0];
-    T mxm = T(0), mym = T(0);
-    if constexpr (ADJ) {
-        mxm = mu[m0];
-        mym = mu[nf + m0];
+    // z queue: slot q holds plane kb + q
+    T qE[3][D], qH[3][D], qL[3][D], qG[3][D], qc[D], qe[D];
+    T xE[2][D], xL[2][D], xc[D], yE[2][D], yL[2][D], yc[D];  // +x ring: y, z; +y ring: x, z
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+        for (int a = 0; a < 3; ++a) qE[a][q] = qH[a][q] = qL[a][q] = qG[a][q] = T(0);
+        for (int a = 0; a < 2; ++a) xE[a][q] = xL[a][q] = yE[a][q] = yL[a][q] = T(0);
+        qc[q] = qe[q] = xc[q] = yc[q] = T(0);
     }
-    for (int k = k0; k < k1; ++k) {
-        const long m = ye(d, i, j, k), my = ye(d, i, jm, k), mx = ye(d, im, j, k);
-        const T px = Px[m], py = Py[m], pz = Pz[m];
-        const T pz_ym = Pz[my], px_ym = Px[my];
-        const T pz_xm = Pz[mx], py_xm = Py[mx];
-        const bool ux = emask(N, 0, i, j, k), uy = emask(N, 1, i, j, k), uz = emask(N, 2, i, j, k);
-        const T dx = (pz - pz_ym) - (py - pym);
-        const T dy = (px - pxm) - (pz - pz_xm);
-        const T dz = (py - py_xm) - (px - px_ym);
-        if constexpr (!ADJ) {
-            const T c = ce[m];
-            const T ex = ein[m], ey = ein[d.nf + m], ez = ein[2 * d.nf + m];
-            e[m] = ux ? ex + c * dx : ex;
-            e[d.nf + m] = uy ? ey + c * dy : ey;
-            e[2 * d.nf + m] = uz ? ez + c * dz : ez;
-        } else {
-            const T mx0 = mu[m], my0 = mu[nf + m], mz0 = mu[2 * nf + m];
-            const T mz_ym = mu[2 * nf + my], mx_ym = mu[my];
-            const T mz_xm = mu[2 * nf + mx], my_xm = mu[nf + mx];
-            double acc = 0.0;
-            if (ux) {
-                const T l = e[m];
-                acc = acc + (double)l * (double)dx;
-                e[m] = l - s * ((mz0 - mz_ym) - (my0 - mym));
+    auto fetch = [&](const int slot, const int kk) {
+        if (kk <= k1 + 1) {
+            const long p = plane(kk);
+            const T* sp = st + p;
+            const T* lp = la + p;
+            const T* cp = ce + p;
+            for (int a = 0; a < 3; ++a) {
+                qE[a][slot] = sp[a * nf + oc];
+                qL[a][slot] = lp[a * nf + oc];
             }
-            if (uy) {
-                const T l = e[d.nf + m];
-                acc = acc + (double)l * (double)dy;
-                e[d.nf + m] = l - s * ((mx0 - mxm) - (mz0 - mz_xm));
+            qc[slot] = cp[oc];
+            if (kk <= k1) {
+                for (int a = 0; a < 3; ++a) {
+                    qH[a][slot] = sp[(3 + a) * nf + oc];
+                    qG[a][slot] = lp[(3 + a) * nf + oc];
+                }
+                qe[slot] = eps[p + oc];
+                if (rx) {
+                    xE[0][slot] = sp[nf + ox];
+                    xE[1][slot] = sp[2 * nf + ox];
+                    xL[0][slot] = lp[nf + ox];
+                    xL[1][slot] = lp[2 * nf + ox];
+                    xc[slot] = cp[ox];
+                }
+                if (ry) {
+                    yE[0][slot] = sp[oy];
+                    yE[1][slot] = sp[2 * nf + oy];
+                    yL[0][slot] = lp[oy];
+                    yL[1][slot] = lp[2 * nf + oy];
+                    yc[slot] = cp[oy];
+                }
             }
-            if (uz) {
-                const T l = e[2 * d.nf + m];
-                acc = acc + (double)l * (double)dz;
-                e[2 * d.nf + m] = l - s * ((my0 - my_xm) - (mx0 - mx_ym));
-            }
-            if (ux || uy || uz) grad[m] = grad[m] - ((double)ce[m] / (double)eps[m]) * acc;
-            mxm = mx0;
-            mym = my0;
         }
-        pxm = px;
-        pym = py;
+    };
+#pragma unroll
+    for (int q = 0; q < D; ++q) fetch(q, k0 - 1 + q);
+
+    // level-2 inputs of plane k-1 (and k-2 for the z differences)
+    T pPx = T(0), pPy = T(0), pPz = T(0), pMx = T(0), pMy = T(0), pMz = T(0);
+    T qPx = T(0), qPy = T(0), qMx = T(0), qMy = T(0);
+    T pLx = T(0), pLy = T(0), pLz = T(0), pc = T(0), pe = T(1);
+    for (int kb = k0 - 1; kb <= k1; kb += D) {
+#pragma unroll
+        for (int q = 0; q < D; ++q) {
+            const int k = kb + q;
+            if (k > k1) break;  // uniform over the workgroup
+            const int buf = q & 1, pbf = buf ^ 1;
+            const int q1 = (q + 1) % D;
+            const T ex = qE[0][q], ey = qE[1][q], ez = qE[2][q];
+            const T exu = qE[0][q1], eyu = qE[1][q1];
+            const T hx = qH[0][q], hy = qH[1][q], hz = qH[2][q];
+            const T lx = qL[0][q], ly = qL[1][q], lz = qL[2][q];
+            const T gx = qG[0][q], gy = qG[1][q], gz = qG[2][q];
+            const T c0 = qc[q], e0 = qe[q];
+            const T wx = c0 * lx, wy = c0 * ly, wz = c0 * lz;
+            const T wxu = qc[q1] * qL[0][q1], wyu = qc[q1] * qL[1][q1];
+            sE[0][buf][ty][tx] = ex;
+            sE[1][buf][ty][tx] = ey;
+            sE[2][buf][ty][tx] = ez;
+            sW[0][buf][ty][tx] = wx;
+            sW[1][buf][ty][tx] = wy;
+            sW[2][buf][ty][tx] = wz;
+            if (rx) {
+                sE[1][buf][ty][TX] = xE[0][q];
+                sE[2][buf][ty][TX] = xE[1][q];
+                sW[1][buf][ty][TX] = xc[q] * xL[0][q];
+                sW[2][buf][ty][TX] = xc[q] * xL[1][q];
+            }
+            if (ry) {
+                sE[0][buf][TY][tx] = yE[0][q];
+                sE[2][buf][TY][tx] = yE[1][q];
+                sW[0][buf][TY][tx] = yc[q] * yL[0][q];
+                sW[2][buf][TY][tx] = yc[q] * yL[1][q];
+            }
+            fetch(q, k + D);
+            __syncthreads();
+            // ---- level 1 at plane k
+            const bool ux = j < N && k < N, uy = i < N && k < N, uz = i < N && j < N;
+            const T ey_xp = sE[1][buf][ty][tx + 1], ez_xp = sE[2][buf][ty][tx + 1];
+            const T ex_yp = sE[0][buf][ty + 1][tx], ez_yp = sE[2][buf][ty + 1][tx];
+            const T wy_xp = sW[1][buf][ty][tx + 1], wz_xp = sW[2][buf][ty][tx + 1];
+            const T wx_yp = sW[0][buf][ty + 1][tx], wz_yp = sW[2][buf][ty + 1][tx];
+            const T Px = ux ? hx + s * ((eyu - ey) - (ez_yp - ez)) : hx;
+            const T Py = uy ? hy + s * ((ez_xp - ez) - (exu - ex)) : hy;
+            const T Pz = uz ? hz + s * ((ex_yp - ex) - (ey_xp - ey)) : hz;
+            const T Mx = ux ? gx - ((wyu - wy) - (wz_yp - wz)) : gx;
+            const T My = uy ? gy - ((wz_xp - wz) - (wxu - wx)) : gy;
+            const T Mz = uz ? gz - ((wx_yp - wx) - (wy_xp - wy)) : gz;
+            sP[0][buf][ty][tx] = Px;
+            sP[1][buf][ty][tx] = Py;
+            sP[2][buf][ty][tx] = Pz;
+            sM[0][buf][ty][tx] = Mx;
+            sM[1][buf][ty][tx] = My;
+            sM[2][buf][ty][tx] = Mz;
+            if (out && k >= k0 && k < k1) {
+                T* o = lb + (long)k * d.M2;
+                o[3 * nf + oc] = Mx;
+                o[4 * nf + oc] = My;
+                o[5 * nf + oc] = Mz;
+            }
+            // ---- level 2 at plane k-1 (H' and mu planes written one iteration ago)
+            const int km = k - 1;
+            if (out && km >= k0) {
+                const T pz_ym = sP[2][pbf][ty - 1][tx], px_ym = sP[0][pbf][ty - 1][tx];
+                const T pz_xm = sP[2][pbf][ty][tx - 1], py_xm = sP[1][pbf][ty][tx - 1];
+                const T mz_ym = sM[2][pbf][ty - 1][tx], mx_ym = sM[0][pbf][ty - 1][tx];
+                const T mz_xm = sM[2][pbf][ty][tx - 1], my_xm = sM[1][pbf][ty][tx - 1];
+                const bool vx = emask(N, 0, i, j, km), vy = emask(N, 1, i, j, km), vz = emask(N, 2, i, j, km);
+                const T dx = (pPz - pz_ym) - (pPy - qPy);
+                const T dy = (pPx - qPx) - (pPz - pz_xm);
+                const T dz = (pPy - py_xm) - (pPx - px_ym);
+                double acc = 0.0;
+                T nx = T(0), ny = T(0), nz = T(0);
+                if (vx) {
+                    acc = acc + (double)pLx * (double)dx;
+                    nx = pLx - s * ((pMz - mz_ym) - (pMy - qMy));
+                }
+                if (vy) {
+                    acc = acc + (double)pLy * (double)dy;
+                    ny = pLy - s * ((pMx - qMx) - (pMz - mz_xm));
+                }
+                if (vz) {
+                    acc = acc + (double)pLz * (double)dz;
+                    nz = pLz - s * ((pMy - my_xm) - (pMx - mx_ym));
+                }
+                const long pk = (long)km * d.M2;
+                T* o = lb + pk;
+                o[oc] = nx;
+                o[nf + oc] = ny;
+                o[2 * nf + oc] = nz;
+                double* go = grad + pk;
+                if (vx || vy || vz) go[oc] = go[oc] - ((double)pc / (double)pe) * acc;
+            }
+            qPx = pPx;
+            qPy = pPy;
+            qMx = pMx;
+            qMy = pMy;
+            pPx = Px;
+            pPy = Py;
+            pPz = Pz;
+            pMx = Mx;
+            pMy = My;
+            pMz = Mz;
+            pLx = lx;
+            pLy = ly;
+            pLz = lz;
+            pc = c0;
+            pe = e0;
+        }
     }
 }
 
@@ -236,41 +298,25 @@ __global__ __launch_bounds__(256) void sum_partials(const double* __restrict__ p
 
 constexpr int kInitBlocks = 2048;
 
-struct Sweep {
-    dim3 grid, block;
-    int tiles_x, tiles_y;
+// FP32: 64 x 16 workgroups with 32-plane z chunks (measured at 384^3 on an
+// MI355X: 1.27-1.48 ms per adjoint step against 1.51-1.62 for 64 x 8 with
+// 32 or 64 planes, 64 x 4 and a four-plane queue); FP64: 64 x 4 (its four LDS
+// planes take twice the space) with 64-plane chunks.
+template <typename T>
+struct AdjTile {
+    static constexpr int TX = 64, TY = sizeof(T) == 4 ? 16 : 4, D = 2, ZC = sizeof(T) == 4 ? 32 : 64;
 };
 
-Sweep sweep_shape(const YeeDims& d)
-{
-    Sweep w;
-    w.tiles_x = ceil_div(d.M, kETX);
-    w.tiles_y = ceil_div(d.M, kETY);
-    const long nb = (long)w.tiles_x * w.tiles_y * ceil_div(d.M, kEZC);
-    w.grid = dim3((unsigned)nb);
-    w.block = dim3(kETX, kETY);
-    return w;
-}
-
 template <typename T>
-int step_t(Ctx* c, const YeeDims& d, double s, const T* ce, const T* a, T* b)
+int adj_t(Ctx* c, const YeeDims& d, double s, const T* ce, const T* eps, const T* st, const T* la, T* lb,
+          double* grad)
 {
-    const Sweep w = sweep_shape(d);
-    h_sweep<T, false><<<w.grid, w.block, 0, c->stream>>>(d, (T)s, a, b + 3 * d.nf, nullptr, nullptr, kEZC,
-                                                           w.tiles_x, w.tiles_y);
-    e_sweep<T, false><<<w.grid, w.block, 0, c->stream>>>(d, (T)s, a, b + 3 * d.nf, b, ce, nullptr, nullptr,
-                                                           nullptr, kEZC, w.tiles_x, w.tiles_y);
-    SCAFES_HIP(hipGetLastError());
-    return SCAFES_OK;
-}
-
-template <typename T>
-int adj_t(Ctx* c, const YeeDims& d, double s, const T* ce, const T* eps, const T* st, T* lam, T* hp, double* grad)
-{
-    const Sweep w = sweep_shape(d);
-    h_sweep<T, true><<<w.grid, w.block, 0, c->stream>>>(d, (T)s, st, hp, ce, lam, kEZC, w.tiles_x, w.tiles_y);
-    e_sweep<T, true><<<w.grid, w.block, 0, c->stream>>>(d, (T)s, nullptr, hp, lam, ce, eps, lam + 3 * d.nf, grad,
-                                                          kEZC, w.tiles_x, w.tiles_y);
+    typedef AdjTile<T> A;
+    const int tiles_x = ceil_div(d.M, A::TX - 1), tiles_y = ceil_div(d.M, A::TY - 1);
+    const long nb = (long)tiles_x * tiles_y * ceil_div(d.M, A::ZC);
+    if (nb > 0x7fffffffL) return fail(SCAFES_ERR_ARG, "yee eps: grid too large");
+    adj_fused<T, A::TX, A::TY, A::D><<<dim3((unsigned)nb), dim3(A::TX, A::TY), 0, c->stream>>>(
+        d, (T)s, st, la, lb, ce, eps, grad, A::ZC, tiles_x, tiles_y);
     SCAFES_HIP(hipGetLastError());
     return SCAFES_OK;
 }
@@ -302,8 +348,24 @@ int yee_eps_coef(Ctx* c, const YeeDims& d, int dtype, double s, const void* eps,
 
 int yee_eps_step(Ctx* c, const YeeDims& d, int dtype, double s, const void* ce, const void* a, void* b)
 {
-    if (dtype == SCAFES_F64) return step_t<double>(c, d, s, (const double*)ce, (const double*)a, (double*)b);
-    return step_t<float>(c, d, s, (const float*)ce, (const float*)a, (float*)b);
+    if (dtype == SCAFES_F64)
+        return yee_fused_eps_launch<double>(c, d, s, (const double*)a, (double*)b, (const double*)ce);
+    return yee_fused_eps_launch<float>(c, d, (float)s, (const float*)a, (float*)b, (const float*)ce);
+}
+
+bool yee_eps_two_step_supported(const YeeDims& d, int dtype)
+{
+    return yee_tb2_supported(d, dtype == SCAFES_F64 ? 8 : 4);
+}
+
+int yee_eps_step2(Ctx* c, const YeeDims& d, int dtype, double s, const void* ce, const void* a, void* b)
+{
+    const int es = dtype == SCAFES_F64 ? 8 : 4;
+    const Tb2Plan pl = yee_tb2_plan(d, es, c->cu_count);
+    if (dtype == SCAFES_F64)
+        return yee_tb2_launch<double>(c, d, pl, s, (const double*)a, (double*)b, nullptr, nullptr,
+                                      (const double*)ce);
+    return yee_tb2_launch<float>(c, d, pl, (float)s, (const float*)a, (float*)b, nullptr, nullptr, (const float*)ce);
 }
 
 int yee_eps_lam_init(Ctx* c, const YeeDims& d, int dtype, const void* fN, const void* target, void* lam,
@@ -321,12 +383,12 @@ int yee_eps_lam_init(Ctx* c, const YeeDims& d, int dtype, const void* fN, const 
 }
 
 int yee_eps_adjoint(Ctx* c, const YeeDims& d, int dtype, double s, const void* ce, const void* eps, const void* st,
-                    void* lam, void* hp, double* grad)
+                    const void* la, void* lb, double* grad)
 {
     if (dtype == SCAFES_F64)
-        return adj_t<double>(c, d, s, (const double*)ce, (const double*)eps, (const double*)st, (double*)lam,
-                             (double*)hp, grad);
-    return adj_t<float>(c, d, s, (const float*)ce, (const float*)eps, (const float*)st, (float*)lam, (float*)hp,
+        return adj_t<double>(c, d, s, (const double*)ce, (const double*)eps, (const double*)st, (const double*)la,
+                             (double*)lb, grad);
+    return adj_t<float>(c, d, s, (const float*)ce, (const float*)eps, (const float*)st, (const float*)la, (float*)lb,
                         grad);
 }
 
@@ -351,11 +413,15 @@ int scafes_yee_eps_run(scafes_ctx* ctx, int ncell, int dtype, double s, const vo
     void* ce = nullptr;
     SCAFES_HIP(hipMallocAsync(&ce, (size_t)d.nf * es, c->stream));
     int rc = yee_eps_coef(c, d, dtype, s, eps, ce);
+    // pairs of steps in one sweep, an odd remainder by the two half-step sweeps
+    const bool two = yee_eps_two_step_supported(d, dtype);
     void* a = f;
     void* b = work;
-    for (int n = 0; rc == SCAFES_OK && n < nsteps; ++n) {
-        rc = yee_eps_step(c, d, dtype, s, ce, a, b);
+    for (int n = 0; rc == SCAFES_OK && n < nsteps;) {
+        const int k = (two && nsteps - n >= 2) ? 2 : 1;
+        rc = k == 2 ? yee_eps_step2(c, d, dtype, s, ce, a, b) : yee_eps_step(c, d, dtype, s, ce, a, b);
         std::swap(a, b);
+        n += k;
     }
     if (rc == SCAFES_OK && a != f) {
         hipError_t e = hipMemcpyAsync(f, a, 6 * (size_t)d.nf * es, hipMemcpyDeviceToDevice, c->stream);

@@ -93,10 +93,12 @@ struct Tb2Lds {
     T h2[3][2][TY][TX];
 };
 
-template <typename T, int TX, int TY, int D>
+// EPS: the E levels use the per-node coefficient ce = s / eps (one value
+// per node index, queued along z with the fields) in place of s.
+template <typename T, int TX, int TY, int D, bool EPS>
 __global__ __launch_bounds__(TX* TY) void yee_tb2_kernel(YeeDims d, T s, const T* __restrict__ fa, T* __restrict__ fb,
-                                                          int zc, int tiles_x, int tiles_y, double* __restrict__ ep0,
-                                                          double* __restrict__ ep1)
+                                                          const T* __restrict__ ce, int zc, int tiles_x, int tiles_y,
+                                                          double* __restrict__ ep0, double* __restrict__ ep1)
 {
     static_assert(D >= 2 && D % 2 == 0, "prefetch depth must be even");
     static_assert((TX * TY) % 64 == 0, "whole waves");
@@ -125,12 +127,13 @@ __global__ __launch_bounds__(TX* TY) void yee_tb2_kernel(YeeDims d, T s, const T
     const size_t fs = (size_t)d.nf * B, pb = (size_t)d.M2 * B;
     const char* a8 = (const char*)fa;
     char* b8 = (char*)fb;
+    const char* c8 = (const char*)ce;
     auto plane = [&](int kk) { return (size_t)min(max(kk, 0), N) * pb; };
 
-    T qex[D], qey[D], qez[D], qhx[D], qhy[D], qhz[D], rxy[D], rxz[D], ryx[D], ryz[D];
+    T qex[D], qey[D], qez[D], qhx[D], qhy[D], qhz[D], rxy[D], rxz[D], ryx[D], ryz[D], qce[D];
 #pragma unroll
     for (int q = 0; q < D; ++q) qex[q] = qey[q] = qez[q] = qhx[q] = qhy[q] = qhz[q] = rxy[q] = rxz[q] = ryx[q] =
-        ryz[q] = T(0);
+        ryz[q] = qce[q] = T(0);
     const int klast = k1 + 3;  // last plane of E^n read
     // Every iteration issues the same loads and stores; unneeded ones use
     // offset `range` (out of range: loads read 0 without a memory access,
@@ -151,6 +154,7 @@ __global__ __launch_bounds__(TX* TY) void yee_tb2_kernel(YeeDims d, T s, const T
         rxz[slot] = bld<T>(rEz, o1);
         ryx[slot] = bld<T>(rEx, o2);
         ryz[slot] = bld<T>(rEz, o2);
+        if constexpr (EPS) qce[slot] = bld<T>(rsrc(c8 + plane(kk), range), o0);
     };
     const int kfirst = k0 - 2;
 #pragma unroll
@@ -158,6 +162,7 @@ __global__ __launch_bounds__(TX* TY) void yee_tb2_kernel(YeeDims d, T s, const T
 
     V3<T> e0m1 = {}, h1m1 = {}, h1m2 = {}, e1m1 = {}, e1m2 = {}, h2m1 = {};
     T h2m2x = T(0), h2m2y = T(0);
+    T cm1 = s, cm2 = s, cm3 = s;  // E coefficient at planes kk-1, kk-2, kk-3
     double es0 = 0.0, es1 = 0.0;
     const int kend = k1 + 2;
     for (int kb = kfirst; kb <= kend; kb += D) {
@@ -170,6 +175,7 @@ __global__ __launch_bounds__(TX* TY) void yee_tb2_kernel(YeeDims d, T s, const T
             const V3<T> e0 = {qex[q], qey[q], qez[q]};
             const V3<T> h0 = {qhx[q], qhy[q], qhz[q]};
             const T exu = qex[q1], eyu = qey[q1];
+            const T c0 = qce[q];  // E coefficient at plane kk (slot q is refilled below)
             L.e0[0][buf][ty][tx] = e0.x;
             L.e0[1][buf][ty][tx] = e0.y;
             L.e0[2][buf][ty][tx] = e0.z;
@@ -195,7 +201,7 @@ __global__ __launch_bounds__(TX* TY) void yee_tb2_kernel(YeeDims d, T s, const T
             }
             // ---- B: E1 at plane kk-1 (H1 plane kk-1 from the previous iteration)
             const int kb1 = kk - 1;
-            const V3<T> e1 = e_update<T>(N, i, j, kb1, s, e0m1, h1m1, h1m2.x, h1m2.y, L.h1[2][pbf][tym][tx],
+            const V3<T> e1 = e_update<T>(N, i, j, kb1, cm1, e0m1, h1m1, h1m2.x, h1m2.y, L.h1[2][pbf][tym][tx],
                                          L.h1[0][pbf][tym][tx], L.h1[2][pbf][ty][txm], L.h1[1][pbf][ty][txm]);
             L.e1[0][buf][ty][tx] = e1.x;
             L.e1[1][buf][ty][tx] = e1.y;
@@ -224,7 +230,7 @@ __global__ __launch_bounds__(TX* TY) void yee_tb2_kernel(YeeDims d, T s, const T
                 const bool st = out && kd >= k0 && kd < k1;
                 V3<T> e2 = {};
                 if (st)
-                    e2 = e_update<T>(N, i, j, kd, s, e1m2, h2m1, h2m2x, h2m2y, L.h2[2][pbf][tym][tx],
+                    e2 = e_update<T>(N, i, j, kd, cm3, e1m2, h2m1, h2m2x, h2m2y, L.h2[2][pbf][tym][tx],
                                      L.h2[0][pbf][tym][tx], L.h2[2][pbf][ty][txm], L.h2[1][pbf][ty][txm]);
                 const size_t o = plane(kd);
                 const uint32_t os = st ? oc : range;
@@ -241,6 +247,11 @@ __global__ __launch_bounds__(TX* TY) void yee_tb2_kernel(YeeDims d, T s, const T
             h1m2 = h1m1;
             h1m1 = h1;
             e0m1 = e0;
+            if constexpr (EPS) {
+                cm3 = cm2;
+                cm2 = cm1;
+                cm1 = c0;
+            }
         }
     }
     if (ep0 || ep1) {
@@ -309,19 +320,25 @@ Tb2Plan yee_tb2_plan(const YeeDims& d, int elem_bytes, int cu_count)
 }
 
 template <typename T>
-int yee_tb2_launch(Ctx* c, const YeeDims& d, const Tb2Plan& pl, T s, const T* a, T* b, double* ep0, double* ep1)
+int yee_tb2_launch(Ctx* c, const YeeDims& d, const Tb2Plan& pl, T s, const T* a, T* b, double* ep0, double* ep1,
+                   const T* ce)
 {
     typedef TileOf<T> Tl;
     if (pl.nb > 0x7fffffffL) return fail(SCAFES_ERR_ARG, "yee: grid too large");
-    yee_tb2_kernel<T, Tl::TX, Tl::TY, Tl::D><<<dim3((unsigned)pl.nb), dim3(Tl::TX, Tl::TY), 0, c->stream>>>(
-        d, s, a, b, pl.zc, pl.tiles_x, pl.tiles_y, ep0, ep1);
+    const dim3 grid((unsigned)pl.nb), blk(Tl::TX, Tl::TY);
+    if (ce)
+        yee_tb2_kernel<T, Tl::TX, Tl::TY, Tl::D, true><<<grid, blk, 0, c->stream>>>(d, s, a, b, ce, pl.zc, pl.tiles_x,
+                                                                                     pl.tiles_y, ep0, ep1);
+    else
+        yee_tb2_kernel<T, Tl::TX, Tl::TY, Tl::D, false><<<grid, blk, 0, c->stream>>>(
+            d, s, a, b, nullptr, pl.zc, pl.tiles_x, pl.tiles_y, ep0, ep1);
     SCAFES_HIP(hipGetLastError());
     return SCAFES_OK;
 }
 
 template int yee_tb2_launch<float>(Ctx*, const YeeDims&, const Tb2Plan&, float, const float*, float*, double*,
-                                   double*);
+                                   double*, const float*);
 template int yee_tb2_launch<double>(Ctx*, const YeeDims&, const Tb2Plan&, double, const double*, double*, double*,
-                                    double*);
+                                    double*, const double*);
 
 }  // namespace scafes
