@@ -374,6 +374,22 @@ def bench_yee(ctx, capi, ncell=384, nsteps=2000, energy_every=100):
             **_profile_fields("yee_tb2_384.md", ncell == 384)}
 
 
+def _grad_calls(ctx, call):
+    """A gradient entry point called twice: (status, first call ms, second
+    call ms).  The first call sizes the workspace the context keeps (fresh
+    allocations, or memory the driver must clear after an earlier free); the
+    second is the cost of each further gradient, e.g. in an optimisation loop."""
+    times = []
+    for _ in range(2):
+        ctx.sync()
+        t0 = time.perf_counter()
+        rc = call()
+        times.append((time.perf_counter() - t0) * 1e3)
+        if rc != 0:
+            return rc, 0.0, 0.0
+    return 0, times[0], times[1]
+
+
 def bench_yee_eps(ctx, capi, ncell=384, nsteps=200, snapshots=(20, 100)):
     """Yee FDTD with a permittivity field: PEC box of ncell^3 cells, Courant
     0.5, Gaussian Ez pulse (sigma = 8 cells) at the centre, a dielectric
@@ -429,21 +445,22 @@ def bench_yee_eps(ctx, capi, ncell=384, nsteps=200, snapshots=(20, 100)):
     for s in snapshots:
         J = capi.C.c_double(0)
         stats = np.zeros(4)
-        ctx.sync()
-        t0 = time.perf_counter()
-        rc = ctx.L.scafes_yee_eps_gradient(ctx.h, ncell, capi.F32, 0.5, f0.ptr, eps.ptr, target.ptr, nsteps, s,
-                                           grad.ptr, capi.C.byref(J),
-                                           stats.ctypes.data_as(capi.C.POINTER(capi.C.c_double)))
-        t_grad = (time.perf_counter() - t0) * 1e3
+
+        def call():
+            return ctx.L.scafes_yee_eps_gradient(ctx.h, ncell, capi.F32, 0.5, f0.ptr, eps.ptr, target.ptr, nsteps,
+                                                 s, grad.ptr, capi.C.byref(J),
+                                                 stats.ctypes.data_as(capi.C.POINTER(capi.C.c_double)))
+        rc, t_first, t_grad = _grad_calls(ctx, call)
         if rc != 0:
             out["gradient"].append({"snapshots": s, "error": ctx.L.scafes_last_error().decode()})
             continue
-        out["gradient"].append({"snapshots": int(stats[2]), "checkpoint_GB": round(stats[3] / 1e9, 2),
+        out["gradient"].append({"snapshots": int(stats[2]), "checkpoint_GB": round(float(stats[3]) / 1e9, 2),
                                 "forward_steps": int(stats[0]), "adjoint_steps": int(stats[1]),
-                                "grad_ms": round(t_grad, 2),
+                                "grad_ms": round(t_grad, 2), "grad_first_call_ms": round(t_first, 2),
                                 "grad_gcell_steps_per_s": round(cells * nsteps / (t_grad * 1e-3) / 1e9, 3),
                                 "grad_over_fwd_eps": round(t_grad / t_eps, 3),
                                 "grad_over_fwd_vacuum": round(t_grad / t_vac, 3), "J": J.value})
+    capi.check(ctx.L.scafes_ctx_release_workspace(ctx.h))
     for d in (eps, f0, f, w, target, grad):
         d.free()
     return out
@@ -565,19 +582,20 @@ def bench_pennes(ctx, capi, n=256, nsteps=2000, snapshots=(62, 1999)):
     for s in snapshots:
         J = capi.C.c_double(0)
         stats = np.zeros(4)
-        ctx.sync()
-        t0 = time.perf_counter()
-        rc = ctx.L.scafes_pennes_gradient(ctx.h, gp, 1, capi.F32, th0.ptr, w.ptr, q.ptr, target.ptr,
-                                          capi.C.byref(pp), nsteps, s, grad.ptr, capi.C.byref(J),
-                                          stats.ctypes.data_as(capi.C.POINTER(capi.C.c_double)))
-        t_grad = (time.perf_counter() - t0) * 1e3
+
+        def call():
+            return ctx.L.scafes_pennes_gradient(ctx.h, gp, 1, capi.F32, th0.ptr, w.ptr, q.ptr, target.ptr,
+                                                capi.C.byref(pp), nsteps, s, grad.ptr, capi.C.byref(J),
+                                                stats.ctypes.data_as(capi.C.POINTER(capi.C.c_double)))
+        rc, t_first, t_grad = _grad_calls(ctx, call)
         if rc != 0:
             out["gradient"].append({"snapshots": s, "error": ctx.L.scafes_last_error().decode()})
             continue
-        out["gradient"].append({"snapshots": int(stats[2]), "checkpoint_GB": round(stats[3] / 1e9, 2),
+        out["gradient"].append({"snapshots": int(stats[2]), "checkpoint_GB": round(float(stats[3]) / 1e9, 2),
                                 "forward_steps": int(stats[0]), "adjoint_steps": int(stats[1]),
                                 "grad_ms": round(t_grad, 2), "grad_over_fwd": round(t_grad / t_fwd, 3),
-                                "J": J.value})
+                                "grad_first_call_ms": round(t_first, 2), "J": J.value})
+    capi.check(ctx.L.scafes_ctx_release_workspace(ctx.h))
     for d in (th0, w, q, a, b, target, grad):
         d.free()
     return out

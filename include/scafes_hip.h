@@ -85,6 +85,13 @@ const char* scafes_last_error(void);
 int scafes_device_count(int* n);
 int scafes_ctx_create(int device, scafes_ctx** out);
 int scafes_ctx_destroy(scafes_ctx* ctx);
+/* The gradient entry points (scafes_*_gradient) keep their workspace --
+ * the forward and adjoint states and the snapshot pool, up to the largest
+ * call's size -- in the context for the next call, since re-allocating
+ * freed device memory makes the driver clear it first (about 0.5 s per
+ * 137 GB store-all pool).  This frees it (scafes_ctx_destroy does too).  No
+ * counterpart in the reference, which keeps its ADOL-C tapes per Problem. */
+int scafes_ctx_release_workspace(scafes_ctx* ctx);
 int scafes_ctx_sync(scafes_ctx* ctx);
 int scafes_ctx_info(scafes_ctx* ctx, int* cu_count, char* name, int name_len);
 /* Device-side wall clock on the context's stream (ScaFES::Timer restart/

@@ -96,6 +96,7 @@ def load():
     _sig(L, "scafes_device_count", [ip])
     _sig(L, "scafes_ctx_create", [C.c_int, C.POINTER(vp)])
     _sig(L, "scafes_ctx_destroy", [vp])
+    _sig(L, "scafes_ctx_release_workspace", [vp])
     _sig(L, "scafes_ctx_sync", [vp])
     _sig(L, "scafes_ctx_info", [vp, ip, C.c_char_p, C.c_int])
     _sig(L, "scafes_event_record", [vp, C.c_int])

@@ -301,37 +301,44 @@ static int run_schedule(Rev& R, Ctx* c, int dtype, const void* t0, size_t abytes
     R.fwd_steps = R.adj_steps = 0;
     R.J = 0.0;
     R.work = nullptr;
-    std::vector<void*> bufs;
-    auto take = [&](void** out, size_t bytes) -> int {
-        hipError_t e = hipMalloc(out, bytes);
-        if (e != hipSuccess) return check_hip(e, "gradient buffers");
-        bufs.push_back(*out);
-        return SCAFES_OK;
-    };
+    // Workspace: [compensated initial state] tmp[0] tmp[1] work lam[0]
+    // lam[1] and the snapshot pool (a store-all budget of a 256^3 x
+    // 2000-step FP32 run is 134 GB; HBM holds 288 GB), carved from one
+    // buffer the context keeps for the next gradient call: re-allocating
+    // freed memory makes the driver clear it first (about 0.5 s per 137 GB).
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const bool kahan0 = dtype == SCAFES_F32_KAHAN;
+    const size_t sb = up(R.sbytes), lb = up(abytes);
+    const size_t need = (kahan0 ? sb : 0) + 3 * sb + 2 * lb + (size_t)snaps * sb;
+    if (c->grad_ws_bytes < need) {
+        if (c->grad_ws) (void)hipFree(c->grad_ws);
+        c->grad_ws = nullptr;
+        c->grad_ws_bytes = 0;
+        hipError_t e = hipMalloc(&c->grad_ws, need);
+        if (e != hipSuccess) {
+            c->grad_ws = nullptr;
+            return check_hip(e, "gradient workspace");
+        }
+        c->grad_ws_bytes = need;
+    }
+    char* ws = (char*)c->grad_ws;
     // the initial state as a state (a (t0, 0) pair in the compensated mode)
     const void* base = t0;
-    void* base_buf = nullptr;
-    if (dtype == SCAFES_F32_KAHAN) {
-        rc = take(&base_buf, R.sbytes);
-        if (!rc) {
-            hipError_t e = hipMemcpyAsync(base_buf, t0, abytes, hipMemcpyDeviceToDevice, c->stream);
-            if (e == hipSuccess) e = hipMemsetAsync((char*)base_buf + abytes, 0, abytes, c->stream);
-            if (e != hipSuccess) rc = check_hip(e, "gradient init");
-        }
+    if (kahan0) {
+        void* base_buf = ws;
+        ws += sb;
+        hipError_t e = hipMemcpyAsync(base_buf, t0, abytes, hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemsetAsync((char*)base_buf + abytes, 0, abytes, c->stream);
+        if (e != hipSuccess) rc = check_hip(e, "gradient init");
         base = base_buf;
     }
-    if (!rc) rc = take(&R.tmp[0], R.sbytes);
-    if (!rc) rc = take(&R.tmp[1], R.sbytes);
-    if (!rc) rc = take(&R.work, R.sbytes);
-    if (!rc) rc = take(&R.lam[0], abytes);
-    if (!rc) rc = take(&R.lam[1], abytes);
-    // one allocation for the whole snapshot pool (a store-all budget of a
-    // 256^3 x 2000-step FP32 run is 134 GB; HBM holds 288 GB)
-    if (!rc && snaps > 0) {
-        void* pool = nullptr;
-        rc = take(&pool, (size_t)snaps * R.sbytes);
-        for (int s = 0; !rc && s < snaps; ++s) R.slots.push_back((char*)pool + (size_t)s * R.sbytes);
-    }
+    R.tmp[0] = ws;
+    R.tmp[1] = ws + sb;
+    R.work = ws + 2 * sb;
+    R.lam[0] = ws + 3 * sb;
+    R.lam[1] = ws + 3 * sb + lb;
+    ws += 3 * sb + 2 * lb;
+    for (int s = 0; s < snaps; ++s) R.slots.push_back(ws + (size_t)s * sb);
     if (!rc) {
         hipError_t e = hipMemsetAsync(grad, 0, (size_t)ngrad * sizeof(double), c->stream);
         if (e != hipSuccess) rc = check_hip(e, "gradient zero");
@@ -341,7 +348,6 @@ static int run_schedule(Rev& R, Ctx* c, int dtype, const void* t0, size_t abytes
         hipError_t e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) rc = check_hip(e, "gradient");
     }
-    for (size_t i = 0; i < bufs.size(); ++i) (void)hipFree(bufs[i]);
     if (rc) return rc;
     *J = R.J;
     if (stats) {

@@ -241,6 +241,18 @@ int scafes_event_elapsed(scafes_ctx* ctx, int a, int b, float* ms)
     return SCAFES_OK;
 }
 
+int scafes_ctx_release_workspace(scafes_ctx* ctx)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c) return fail(SCAFES_ERR_ARG, "null context");
+    SCAFES_HIP(hipSetDevice(c->device));
+    SCAFES_HIP(hipStreamSynchronize(c->stream));
+    if (c->grad_ws) SCAFES_HIP(hipFree(c->grad_ws));
+    c->grad_ws = nullptr;
+    c->grad_ws_bytes = 0;
+    return SCAFES_OK;
+}
+
 int scafes_ctx_destroy(scafes_ctx* ctx)
 {
     Ctx* c = as_ctx(ctx);
@@ -249,6 +261,7 @@ int scafes_ctx_destroy(scafes_ctx* ctx)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int i = 0; i < 8; ++i)
         if (c->events[i]) (void)hipEventDestroy(c->events[i]);
+    if (c->grad_ws) (void)hipFree(c->grad_ws);
     if (c->red_partial) (void)hipFree(c->red_partial);
     if (c->red_result) (void)hipFree(c->red_result);
     if (c->host_result) (void)hipHostFree(c->host_result);

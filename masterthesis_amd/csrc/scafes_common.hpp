@@ -40,6 +40,11 @@ struct Ctx {
     // autotuned tile choice per (kernel family, dtype); -1 = not tuned yet
     int tuned[16];
     hipEvent_t events[8] = {};
+    // gradient workspace (states, adjoint fields, snapshot pool) kept between
+    // gradient calls; grows on demand, freed by scafes_ctx_release_workspace
+    // and scafes_ctx_destroy
+    void* grad_ws = nullptr;
+    size_t grad_ws_bytes = 0;
 };
 
 Ctx* as_ctx(scafes_ctx* c);

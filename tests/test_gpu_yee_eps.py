@@ -137,6 +137,23 @@ def test_gradient_fp32_checkpointing_does_not_change_bits(ctx):
     assert J1 == J2 > 0
 
 
+def test_workspace_kept_and_released(ctx):
+    """The context keeps the gradient workspace between calls; a smaller
+    call reuses it, a release frees it, and the results do not depend on
+    either."""
+    N, s, nsteps = 24, 0.5, 11
+    f0, eps, target = _case(N, seed=31)
+    J1, g1, _ = _grad(ctx, N, s, eps, f0, target, nsteps, 10, np.float64)
+    J2, g2, _ = _grad(ctx, N, s, eps, f0, target, nsteps, 2, np.float64)
+    capi.check(ctx.L.scafes_ctx_release_workspace(ctx.h))
+    J3, g3, _ = _grad(ctx, N, s, eps, f0, target, nsteps, 2, np.float64)
+    np.testing.assert_array_equal(g1, g2)
+    np.testing.assert_array_equal(g2, g3)
+    assert J1 == J2 == J3
+    capi.check(ctx.L.scafes_ctx_release_workspace(ctx.h))
+    capi.check(ctx.L.scafes_ctx_release_workspace(ctx.h))
+
+
 def test_argument_errors(ctx):
     N = 8
     a = ctx.alloc(6 * (N + 1) ** 3)
