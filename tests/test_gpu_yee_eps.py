@@ -6,7 +6,9 @@ oracle (pinned in test_oracle_yee_eps.py):
    vacuum scafes_yee_run in FP64 and FP32; FP32 within 1e-5 relative L2 of
    the FP64 oracle;
  * gradient FP64: bit-identical to the oracle for every snapshot budget and
-   box sizes that leave ragged tiles; J to 1e-12;
+   box sizes that leave ragged tiles, down to N = 2 and one step; J to 1e-12;
+ * the gradient workspace the context keeps: reuse and release do not change
+   the bits;
  * gradient FP32 (FP32 fields, FP64 gradient): within 1e-4 relative L2 of
    the FP64 oracle (measured value printed); the snapshot budget changes
    the schedule, not the bits;
@@ -113,6 +115,23 @@ def test_gradient_fp64_bitexact(ctx, N, snaps):
     assert abs(J - Jref) <= 1e-12 * abs(Jref)
     assert stats[0] == ctx.L.scafes_checkpoint_forward_steps(nsteps, snaps) and stats[1] == nsteps
     assert np.abs(gref).max() > 1e-6
+
+
+@pytest.mark.parametrize("N,nsteps,snaps", [(2, 3, 1), (3, 1, 0), (64, 2, 50), (63, 5, 1)])
+def test_gradient_edge_cases_bitexact(ctx, N, nsteps, snaps):
+    """Smallest boxes (N = 2: one updated node per E component), one step,
+    a snapshot budget above nsteps - 1, and tiles that end one node past a
+    64-wide (N = 64) or exactly on one (N = 63)."""
+    s = 0.5
+    f0, eps, target = _case(N, seed=100 + N)
+    st = O.yee_eps_forward_all(N, s, eps, f0, nsteps)
+    Jref, gref = O.yee_eps_grad(N, s, eps, st, target, nsteps)
+    J, g, stats = _grad(ctx, N, s, eps, f0, target, nsteps, snaps, np.float64)
+    np.testing.assert_array_equal(g, gref)
+    assert abs(J - Jref) <= 1e-12 * abs(Jref)
+    assert stats[2] == min(snaps, nsteps - 1)
+    assert np.count_nonzero(gref) > 0
+    np.testing.assert_array_equal(_run(ctx, N, s, eps, f0, nsteps, np.float64), st[nsteps])
 
 
 def test_gradient_fp32_vs_fp64_oracle(ctx):
