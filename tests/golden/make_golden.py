@@ -106,6 +106,24 @@ def case_yee():
     return {"ncell": N, "nsteps": steps, "courant": s}, {"f0": f0, "f": f}
 
 
+def case_yee_eps():
+    """Yee with a relative permittivity field, PEC box of 10^3 cells, Courant
+    0.5, random valid fields, eps in [1, 4), 8 steps: final state and the
+    gradient of J = 0.5 sum over the updated E entries of (E^N - target)^2
+    w.r.t. eps."""
+    N, steps, s = 10, 8, 0.5
+    rng = np.random.default_rng(13)
+    m = N + 1
+    f0 = rng.standard_normal((6, m, m, m))
+    f0[:3] *= O.yee_emask(N)
+    eps = 1.0 + 3.0 * rng.random((m, m, m))
+    target = 0.3 * rng.standard_normal((3, m, m, m))
+    st = O.yee_eps_forward_all(N, s, eps, f0, steps)
+    J, grad = O.yee_eps_grad(N, s, eps, st, target, steps)
+    return ({"ncell": N, "nsteps": steps, "courant": s, "J": J},
+            {"f0": f0, "eps": eps, "target": target, "f": st[-1], "grad": grad})
+
+
 def case_heat2d():
     """2-D heat problems of the batched ensemble (n = 9, 17, 33), 40 steps:
     final fields, J and dJ/dkappa."""
@@ -205,6 +223,7 @@ CASES = {
     "heat7var_16": case_heat7var,
     "pennes_grad_12": case_pennes,
     "yee_12": case_yee,
+    "yee_eps_10": case_yee_eps,
     "heat2d_batch": case_heat2d,
     "tumor_13x9x8": case_tumor,
     "mri_11x9x8": case_mri,

@@ -92,6 +92,24 @@ def test_yee_fp64(ctx):
     np.testing.assert_array_equal(got, f["f"])
 
 
+def test_yee_eps_fp64(ctx):
+    """Yee with a permittivity field: the forward state and dJ/deps bit for
+    bit (the gradient with 3 snapshots), J to 1e-12."""
+    par, f = fixture("yee_eps_10")
+    N, n = par["ncell"], par["nsteps"]
+    d, de, w = ctx.upload(f["f0"]), ctx.upload(f["eps"]), ctx.alloc(f["f0"].size)
+    capi.check(ctx.L.scafes_yee_eps_run(ctx.h, N, capi.F64, par["courant"], de.ptr, d.ptr, w.ptr, n))
+    np.testing.assert_array_equal(d.read().reshape(f["f0"].shape), f["f"])
+    d0, dt, dg = ctx.upload(f["f0"]), ctx.upload(f["target"]), ctx.alloc(f["eps"].size)
+    J = C.c_double(0)
+    capi.check(ctx.L.scafes_yee_eps_gradient(ctx.h, N, capi.F64, par["courant"], d0.ptr, de.ptr, dt.ptr, n, 3,
+                                             dg.ptr, C.byref(J), None))
+    np.testing.assert_array_equal(dg.read().reshape(f["eps"].shape), f["grad"])
+    assert abs(J.value - par["J"]) <= 1e-12 * abs(par["J"])
+    for b in (d, de, w, d0, dt, dg):
+        b.free()
+
+
 def test_heat2d_batch_fp32(ctx):
     """FP32 state against the FP64 fixture: fields within 2e-5 of their
     maximum; J and dJ/dkappa within 1e-3 relative (the misfit is a small
