@@ -358,7 +358,8 @@ int scafes_yee_init_pulse(scafes_ctx* ctx, int ncell, int dtype, void* f, double
  * scaling of the reference's EMFDTD update (examples/EMFDTD/EMFDTD.hpp:351-356,
  * there 1/(1 + sigma dt/eps0)).  eps = 1 gives scafes_yee_run's bits.
  * f (six arrays) is advanced nsteps in place; work is a second state
- * buffer (out-of-place half steps).  F32 or F64. */
+ * buffer (out-of-place sweeps: two steps per sweep, a single fused step for
+ * an odd remainder).  F32 or F64; 2 <= ncell <= 1400. */
 int scafes_yee_eps_run(scafes_ctx* ctx, int ncell, int dtype, double s, const void* eps, void* f, void* work,
                        int nsteps);
 /* Gradient of J = 0.5 sum over the E entries the step updates of
