@@ -374,6 +374,46 @@ def bench_yee(ctx, capi, ncell=384, nsteps=2000, energy_every=100):
             **_profile_fields("yee_tb2_384.md", ncell == 384)}
 
 
+def bench_kappa_grad(ctx, capi, g, n, u0, k0, tau, nsteps, t_fwd_ms, snapshots=(30, 100)):
+    """dJ/dkappa of the headline problem (the same n^3 input, kappa field,
+    tau and zero Dirichlet border; FP32+Kahan forward, FP32 adjoint field,
+    FP64 gradient), J = 0.5 sum_inner (u^N - u^0)^2, for two snapshot
+    budgets, against the headline forward time of the same nsteps."""
+    ne = capi.memall_count(g, 1)
+    uf, kf = ctx.alloc(ne, np.float32), ctx.alloc(ne, np.float32)
+    capi.check(ctx.L.scafes_convert(ctx.h, uf.ptr, capi.F32, u0.ptr, capi.F64, ne))
+    capi.check(ctx.L.scafes_convert(ctx.h, kf.ptr, capi.F32, k0.ptr, capi.F64, ne))
+    grad = ctx.alloc(ne, np.float64)
+    out = {"workload": f"heat7var {n}^3 dJ/dkappa, {nsteps} steps, FP32+Kahan forward, FP32 adjoint, FP64 gradient",
+           "fwd_ms": round(t_fwd_ms, 2),
+           "recompute": "the headline multi-step sweep where the schedule advances several steps, else one step",
+           "adjoint_kernel": "stream_kernel<HeatVarAdjOp<float>> (one reverse step per sweep, 32 B/node-step "
+                             "algorithmic: lam r+w, kappa, u^{n-1}, FP64 gradient r+w)",
+           "adjoint_profile": _profile_fields("heat7var_adjoint_512.md", n == 512),
+           "gradient": []}
+    for s in snapshots:
+        J = capi.C.c_double(0)
+        stats = np.zeros(4)
+
+        def call():
+            return ctx.L.scafes_heat7var_gradient(ctx.h, capi.C.byref(g), 1, capi.HEATVAR_F32_KAHAN, uf.ptr, kf.ptr,
+                                                  uf.ptr, tau, capi.BORDER_VALUE, 0.0, nsteps, s, grad.ptr,
+                                                  capi.C.byref(J),
+                                                  stats.ctypes.data_as(capi.C.POINTER(capi.C.c_double)))
+        rc, t_first, t_grad = _grad_calls(ctx, call)
+        if rc != 0:
+            out["gradient"].append({"snapshots": s, "error": ctx.L.scafes_last_error().decode()})
+            continue
+        out["gradient"].append({"snapshots": int(stats[2]), "checkpoint_GB": round(float(stats[3]) / 1e9, 2),
+                                "forward_steps": int(stats[0]), "adjoint_steps": int(stats[1]),
+                                "grad_ms": round(t_grad, 2), "grad_first_call_ms": round(t_first, 2),
+                                "grad_over_fwd": round(t_grad / t_fwd_ms, 3), "J": J.value})
+    capi.check(ctx.L.scafes_ctx_release_workspace(ctx.h))
+    for d in (uf, kf, grad):
+        d.free()
+    return out
+
+
 def _grad_calls(ctx, call):
     """A gradient entry point called twice: (status, first call ms, second
     call ms).  The first call sizes the workspace the context keeps (fresh
@@ -665,6 +705,7 @@ def main():
     ap.add_argument("--no-yee", action="store_true", help="skip the Yee FDTD leg")
     ap.add_argument("--no-pennes", action="store_true", help="skip the Pennes adjoint leg")
     ap.add_argument("--no-yee-eps", action="store_true", help="skip the Yee permittivity-gradient leg")
+    ap.add_argument("--no-kappa-grad", action="store_true", help="skip the headline kappa-gradient leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-emfdtd", action="store_true", help="skip the EMFDTD (CPML) leg")
     ap.add_argument("--calibrate-copy", action="store_true", help="also run a 1 GiB copy (counter calibration)")
@@ -781,6 +822,9 @@ def main():
         out["cpu_baseline"] = cpu_baseline(u_host, k_host, tau, n)
         del u_host, k_host
         log(f"[bench] cpu baseline {out['cpu_baseline']['value']} GLUPS")
+    if not args.no_kappa_grad:
+        out["kappa_gradient"] = bench_kappa_grad(ctx, capi, g, n, u0, k0, tau, args.steps, ms32)
+        log(f"[bench] kappa gradient {out['kappa_gradient']}")
     u0.free()
     k0.free()
     if args.calibrate_copy:

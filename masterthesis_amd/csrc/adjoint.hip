@@ -128,21 +128,27 @@ struct Rev {
         return e == hipSuccess ? SCAFES_OK : check_hip(e, "checkpoint copy");
     }
     // State after n steps from src, written to dst (src is never written).
-    // The Yee family advances two steps per sweep where two remain.
+    // The Yee and variable-kappa families advance several steps per sweep
+    // where that many remain (bit-identical to single steps).
     int advance(const void* src, long n, void* dst)
     {
         if (n == 0) return src == dst ? SCAFES_OK : copy(dst, src);
-        const bool two = family == FAMILY_YEE_EPS && yee_eps_two_step_supported(yd, dtype);
+        int multi = 1;
+        if (family == FAMILY_YEE_EPS && yee_eps_two_step_supported(yd, dtype)) multi = 2;
+        if (family == FAMILY_HEATVAR) multi = heat7var_sweep_steps(c, geo, heat_mode);
         const void* x = src;
         for (long done = 0; done < n;) {
-            const int k = (two && n - done >= 2) ? 2 : 1;
+            const int k = (multi > 1 && n - done >= multi) ? multi : 1;
             done += k;
             void* out = (done == n) ? dst : (x == tmp[0] ? tmp[1] : tmp[0]);
             if (out == x) out = (x == tmp[0]) ? tmp[1] : tmp[0];
             int r;
-            if (k == 2) {
-                fwd_steps += 2;
-                r = yee_eps_step2(c, yd, dtype, yee_s, ce, x, out);
+            if (k > 1) {
+                fwd_steps += k;
+                r = family == FAMILY_YEE_EPS
+                        ? yee_eps_step2(c, yd, dtype, yee_s, ce, x, out)
+                        : heat7var_sweep(c, geo, g, heat_mode, x, out, lo(x), lo(out), kappa, tau, border_mode,
+                                         border_value);
             } else {
                 r = step(x, out);
             }

@@ -111,6 +111,12 @@ int pennes_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, c
 // stencil3d.hip); mode is a SCAFES_HEATVAR_* value.
 int heat7var_step_one(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* un,
                       const void* cin, void* cout, const void* kap, double tau, int border_mode, double border_value);
+// Steps one sweep of the mode's default (or autotuned) tile advances on
+// this grid, and that sweep (stencil3d.hip); bit-identical to as many
+// one-step launches.
+int heat7var_sweep_steps(const Ctx* c, const Geo& geo, int mode);
+int heat7var_sweep(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* un, const void* cin,
+                   void* cout, const void* kap, double tau, int border_mode, double border_value);
 // Adjoint step of the variable-kappa heat w.r.t. kappa (defined in
 // stencil3d.hip): lam^n -> lam^{n-1}, grad += the step's kappa terms with
 // u^{n-1} = uprev; dtype SCAFES_F64 or SCAFES_F32 (fields), grad FP64.

@@ -1214,3 +1214,26 @@ int scafes_pennes_adjoint_step(scafes_ctx* ctx, const scafes_geom* g, int hw, in
 }
 
 }  // extern "C"
+
+namespace scafes {
+
+int heat7var_sweep_steps(const Ctx* c, const Geo& geo, int mode)
+{
+    const int cfg = resolve_heatvar_cfg(c, geo, mode, -1);
+    if (cfg < kNumCfgs) return 1;
+    int tx, ty, zc, d, st = 1;
+    heatvar_tb2_cfg(cfg - kNumCfgs, &tx, &ty, &zc, &d, &st);
+    return st;
+}
+
+int heat7var_sweep(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* un, const void* cin,
+                   void* cout, const void* kap, double tau, int border_mode, double border_value)
+{
+    const int cfg = resolve_heatvar_cfg(c, geo, mode, -1);
+    if (cfg < kNumCfgs)
+        return heat7var_step_one(c, geo, g, mode, u, un, cin, cout, kap, tau, border_mode, border_value);
+    return heatvar_tb2_launch(c, geo, g, mode, u, un, cin, cout, kap, tau, border_mode, border_value,
+                              cfg - kNumCfgs);
+}
+
+}  // namespace scafes
