@@ -178,6 +178,9 @@ public:
     // Time iterate at which the loop stopped (nTimesteps if it ran to the end).
     int lastTimeIter() const { return mLastIter; }
 
+    // Whether gradients were requested (reference :3870; --computeGradients).
+    bool computeGradients() const { return mParams.computeGradients(); }
+
     // -------------------------------------------------------------- driver
     void iterateOverTime()
     {
@@ -185,6 +188,15 @@ public:
             throw std::runtime_error(
                 "ScaFES: this problem declares no device update kernel (specialise "
                 "ScaFES::DeviceTraits for it); the engine has no CPU update path.");
+        // The reference records an ADOL-C tape of the hooks and propagates
+        // the derivatives w.r.t. geomparamsInit alongside the iterate
+        // (Problem.hpp:1391, :4598-4660).  This engine has no tape; its
+        // gradients are the field adjoints of the C ABI.
+        if (mParams.computeGradients())
+            throw std::runtime_error(
+                "ScaFES: --computeGradients (ADOL-C tapes of the hooks) is not available in this engine; "
+                "use the field gradients of the C ABI (scafes_pennes_gradient, scafes_heat7var_gradient, "
+                "scafes_yee_eps_gradient).");
         DeviceTraits<OWNPRBLM>::describe(static_cast<const OWNPRBLM&>(*this), mDev);
         validateDevice();
         Timer timerIterate;

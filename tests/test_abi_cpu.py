@@ -87,6 +87,15 @@ def test_parameters_reject_multi_partition():
     assert "one partition" in p.stderr
 
 
+def test_compute_gradients_option_is_refused():
+    """--computeGradients (ADOL-C tapes in the reference) is refused with a
+    message naming the C-ABI field gradients, not silently ignored."""
+    exe = os.path.join(ROOT, "masterthesis_amd", "bin", "heat_timelin_3d")
+    p = subprocess.run([exe, "--computeGradients=1"], capture_output=True, text=True, timeout=60)
+    assert p.returncode != 0
+    assert "computeGradients" in p.stderr and "scafes_heat7var_gradient" in p.stderr
+
+
 def test_problem_without_device_kernel_reports_it():
     """A problem class with no DeviceTraits specialisation compiles against
     the host API unchanged, and iterateOverTime() reports that it has no
