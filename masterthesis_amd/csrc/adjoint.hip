@@ -205,6 +205,13 @@ struct Rev {
     int adjointk(int K, const void* const* st)
     {
         adj_steps += K;
+        if (family == FAMILY_HEATVAR) {
+            // two steps per sweep; the var-kappa adjoint reads the states' hi parts
+            int r = heatvar_adj2_launch(c, geo, g, lam_dtype, lam[lcur], lam[1 - lcur], kappa, st[0], st[1], grad,
+                                        tau);
+            lcur = 1 - lcur;
+            return r;
+        }
         const void* los[kAdjMaxK];
         for (int l = 0; l < K; ++l) los[l] = lo(st[l]);
         int r = pennes_adjk_launch(c, geo, g, lam_dtype, K, lam[lcur], lam[1 - lcur], w, st, kahan() ? los : nullptr,
@@ -233,7 +240,8 @@ struct Rev {
                     r = adjoint((int)n, st[n - 1 - a]);
                     n -= 1;
                 } else {
-                    const int K = (int)std::min<long>(kAdjMaxK, n - a);
+                    const int kmax = family == FAMILY_HEATVAR ? 2 : kAdjMaxK;
+                    const int K = (int)std::min<long>(kmax, n - a);
                     const void* sk[kAdjMaxK];
                     for (int l = 0; l < K; ++l) sk[l] = st[n - 1 - l - a];
                     r = adjointk(K, sk);
@@ -303,7 +311,8 @@ static int run_schedule(Rev& R, Ctx* c, int dtype, const void* t0, size_t abytes
     R.grad = grad;
     R.lcur = 0;
     R.nsteps = nsteps;
-    R.pair = R.family == FAMILY_PENNES && pennes_adjk_supported(R.geo);
+    R.pair = (R.family == FAMILY_PENNES && pennes_adjk_supported(R.geo)) ||
+             (R.family == FAMILY_HEATVAR && heatvar_adj2_supported(R.geo));
     R.fwd_steps = R.adj_steps = 0;
     R.J = 0.0;
     R.work = nullptr;

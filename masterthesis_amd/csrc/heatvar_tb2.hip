@@ -441,6 +441,272 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb3_kernel(Geo g, const T* __r
     }
 }
 
+// ---------------------------------------------------------------------------
+// Two reverse steps of the variable-kappa adjoint per sweep (lam^n ->
+// lam^{n-2}) with both steps' kappa-gradient terms, 3-D grids.  Per node the
+// arithmetic is that of HeatVarAdjOp in stencil3d.hip, in the same order:
+//   lam^{n-1} = the forward step's operator applied to lam^n (var_update
+//               with the node's six face coefficients), 0 on border nodes;
+//   grad     += 0.5 * sum_p th_p * sum_{nb = -+e_p in the grid}
+//               (lam^n_c - lam^n_nb)(u^{n-1}_nb - u^{n-1}_c)   (every node),
+// and the FP64 gradient receives the step-n term, then the step-(n-1) term,
+// so the result is bit-identical to two one-step launches.  The layout is
+// the two-step forward's: level 1 (lam^{n-1}) at plane k for every thread
+// position (the output tile plus a one-node ring; lam^n and kappa come with
+// the ring), level 2 (lam^{n-2}) at plane k-1 on the output tile from the
+// level-1 plane written one iteration earlier.  The gradient terms are formed
+// on the output tile only, so u^{n-1} and u^{n-2} need no ring; the step-n
+// term of plane k rides in a register to the level-2 pass one plane later.
+// Per two reverse steps HBM moves lam r+w, kappa, u^{n-1}, u^{n-2} and the
+// FP64 gradient r+w once: 36 B per node for FP32 fields (18 per step,
+// against 32 for one-step launches).
+struct AdjVarCoef {
+    VarCoef v;                // lam update (the forward step's arithmetic)
+    double th0, th1, th2;     // tau / h_p^2
+};
+
+__device__ inline double adj_face_sum(bool lo, bool hi, double lc, double lm, double lp, double uc, double um,
+                                      double up)
+{
+    double sp = 0.0;
+    if (lo) sp += (lc - lm) * (um - uc);
+    if (hi) sp += (lc - lp) * (up - uc);
+    return sp;
+}
+
+template <typename T, int TX, int TY, int D>
+__global__ __launch_bounds__(TX* TY) void heatvar_adj2_kernel(Geo g, const T* __restrict__ lam,
+                                                               const T* __restrict__ kap, const T* __restrict__ u1,
+                                                               const T* __restrict__ u2, T* __restrict__ lamo,
+                                                               double* __restrict__ grad, AdjVarCoef cf, int zc,
+                                                               int tiles_x, int tiles_y)
+{
+    static_assert(D >= 2 && D % 2 == 0, "prefetch depth must be even");
+    constexpr int OX = TX - 2, OY = TY - 2;
+    constexpr int MODE = sizeof(T) == 8 ? 0 : 1;
+    __shared__ T s0[2][TY + 2][TX + 2];  // lam^n plane with ring
+    __shared__ T sk[2][TY + 2][TX + 2];  // kappa plane with ring
+    __shared__ T sa[2][TY][TX];          // u^{n-1} plane
+    __shared__ T s1[2][TY][TX];          // lam^{n-1} plane
+    __shared__ T sb[2][TY][TX];          // u^{n-2} plane
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int per_plane = tiles_x * tiles_y;
+    const int tz = lb / per_plane;
+    const int r = lb - tz * per_plane;
+    const int tyi = r / tiles_x;
+    const int txi = r - tyi * tiles_x;
+    const int i = txi * OX - 1 + tx, j = tyi * OY - 1 + ty;
+    const int ic = min(max(i, 0), g.nx - 1), jc = min(max(j, 0), g.ny - 1);
+    const bool inner_xy = i >= g.nl0 && i <= g.nx - 1 - g.nl0 && j >= g.nl1 && j <= g.ny - 1 - g.nl1;
+    const bool out_thread = tx >= 1 && tx <= TX - 2 && ty >= 1 && ty <= TY - 2 && i < g.nx && j < g.ny;
+    const int k0 = tz * zc, k1 = min(k0 + zc, g.nz);
+    const int kstart = k0 - 1;
+    const int kcen = k1 + 1;
+    const int klo = g.nl2, khi = g.nz - 1 - g.nl2;
+    const bool hx = (tx == 0) || (tx == TX - 1);
+    const bool hy = (ty == 0) || (ty == TY - 1);
+    const int hxi = (tx == 0) ? max(ic - 1, 0) : min(ic + 1, g.nx - 1);
+    const int hyj = (ty == 0) ? max(jc - 1, 0) : min(jc + 1, g.ny - 1);
+    const int sxh = (tx == 0) ? 0 : TX + 1;
+    const int syh = (ty == 0) ? 0 : TY + 1;
+    constexpr uint32_t B = sizeof(T);
+    const uint32_t oc = (uint32_t)(g.base + ic + g.e0 * jc) * B;
+    const uint32_t ohx = (uint32_t)(g.base + hxi + g.e0 * jc) * B;
+    const uint32_t ohy = (uint32_t)(g.base + ic + g.e0 * hyj) * B;
+    const uint32_t oo = (uint32_t)(g.base + i + g.e0 * j) * B;
+    const uint32_t og = (uint32_t)(g.base + (out_thread ? i : 0) + g.e0 * (out_thread ? j : 0)) * 8u;
+    const size_t pb = (size_t)g.e01 * B, pg = (size_t)g.e01 * 8;
+    const uint32_t range = (uint32_t)((g.base + g.e01) * B);
+    const uint32_t rangeg = (uint32_t)((g.base + g.e01) * 8);
+    const char* lb8 = (const char*)lam;
+    const char* kb8 = (const char*)kap;
+    const char* ab8 = (const char*)u1;
+    const char* bb8 = (const char*)u2;
+    const char* gb8 = (const char*)grad;
+    auto plane = [&](int kk) { return (size_t)min(max(kk, 0), g.nz - 1); };
+
+    T qa[D], qk[D], qu1[D], qu2[D], qax[D], qkx[D], qay[D], qky[D];
+    double qg[D];
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+        qa[s] = qk[s] = qu1[s] = qu2[s] = qax[s] = qkx[s] = qay[s] = qky[s] = T(0);
+        qg[s] = 0.0;
+    }
+    auto fetch = [&](const int slot, const int kk) {
+        const size_t pk = plane(kk);
+        const size_t o = pk * pb;
+        const Rsrc rl = plane_rsrc(lb8 + o, range), rk = plane_rsrc(kb8 + o, range);
+        const bool cen = kk <= kcen, mid = kk <= k1;
+        const uint32_t occ = cen ? oc : range, ocm = mid ? oc : range;
+        const uint32_t ox = (mid && hx) ? ohx : range, oy = (mid && hy) ? ohy : range;
+        qa[slot] = bload<T>(rl, occ);
+        qk[slot] = bload<T>(rk, occ);
+        qu1[slot] = bload<T>(plane_rsrc(ab8 + o, range), occ);
+        qu2[slot] = bload<T>(plane_rsrc(bb8 + o, range), ocm);
+        qax[slot] = bload<T>(rl, ox);
+        qkx[slot] = bload<T>(rk, ox);
+        qay[slot] = bload<T>(rl, oy);
+        qky[slot] = bload<T>(rk, oy);
+        qg[slot] = bload<double>(plane_rsrc(gb8 + pk * pg, rangeg), (mid && out_thread) ? og : rangeg);
+    };
+    T am = bload<T>(plane_rsrc(lb8 + plane(kstart - 1) * pb, range), oc);
+    T u1m = bload<T>(plane_rsrc(ab8 + plane(kstart - 1) * pb, range), oc);
+    const T km = bload<T>(plane_rsrc(kb8 + plane(kstart - 1) * pb, range), oc);
+#pragma unroll
+    for (int s = 0; s < D; ++s) fetch(s, kstart + s);
+
+    T l1m2 = T(0), l1m1 = T(0);     // level 1 at planes k-2, k-1
+    T u2m2 = T(0), u2m1 = T(0);     // u^{n-2} at planes k-2, k-1
+    double t1m1 = 0.0, gm1 = 0.0;   // step-n term and gradient of plane k-1
+    Faces<T> fp = {};
+    T fup = face<T, MODE>(qk[0], km);
+    for (int kb = kstart; kb <= k1; kb += D) {
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+            const int k = kb + s;
+            if (k > k1) break;  // uniform over the workgroup
+            const int buf = s & 1, pbf = buf ^ 1;
+            const T ac = qa[s], ap = qa[(s + 1) % D];
+            const T bc = qk[s], bp = qk[(s + 1) % D];
+            const T uc = qu1[s], up = qu1[(s + 1) % D];
+            const T vc = qu2[s];
+            const double gk = qg[s];
+            s0[buf][ty + 1][tx + 1] = ac;
+            sk[buf][ty + 1][tx + 1] = bc;
+            sa[buf][ty][tx] = uc;
+            if (hx) {
+                s0[buf][ty + 1][sxh] = qax[s];
+                sk[buf][ty + 1][sxh] = qkx[s];
+            }
+            if (hy) {
+                s0[buf][syh][tx + 1] = qay[s];
+                sk[buf][syh][tx + 1] = qky[s];
+            }
+            fetch(s, k + D);
+            __syncthreads();
+            // ---- level 1 at (i, j, k): lam^{n-1}, every position
+            const T lw = s0[buf][ty + 1][tx], le = s0[buf][ty + 1][tx + 2];
+            const T ls = s0[buf][ty][tx + 1], ln = s0[buf][ty + 2][tx + 1];
+            Faces<T> f;
+            f.e = face<T, MODE>(sk[buf][ty + 1][tx + 2], bc);
+            f.w = face<T, MODE>(bc, sk[buf][ty + 1][tx]);
+            f.n = face<T, MODE>(sk[buf][ty + 2][tx + 1], bc);
+            f.s = face<T, MODE>(bc, sk[buf][ty][tx + 1]);
+            f.u = face<T, MODE>(bp, bc);
+            f.d = fup;
+            T dummy;
+            T l1 = var_update<T, MODE>(cf.v, ac, lw, le, ls, ln, am, ap, f, T(0), &dummy);
+            if (!(inner_xy && k >= klo && k <= khi)) l1 = T(0);
+            // planes read one iteration later are written after the barrier
+            // (a write before it could overtake a slower thread still reading
+            // the same buffer two planes back)
+            s1[buf][ty][tx] = l1;
+            sb[buf][ty][tx] = vc;
+            // step-n gradient term at (i, j, k) (output positions: the in-plane
+            // neighbours are inside the thread tile)
+            double t1 = 0.0;
+            if (out_thread) {
+                const int txm = tx - 1, txp = tx + 1, tym = ty - 1, typ = ty + 1;
+                t1 += cf.th0 * adj_face_sum(i > 0, i < g.nx - 1, (double)ac, (double)lw, (double)le, (double)uc,
+                                            (double)sa[buf][ty][txm], (double)sa[buf][ty][txp]);
+                t1 += cf.th1 * adj_face_sum(j > 0, j < g.ny - 1, (double)ac, (double)ls, (double)ln, (double)uc,
+                                            (double)sa[buf][tym][tx], (double)sa[buf][typ][tx]);
+                t1 += cf.th2 * adj_face_sum(k > 0, k < g.nz - 1, (double)ac, (double)am, (double)ap, (double)uc,
+                                            (double)u1m, (double)up);
+            }
+            // ---- level 2 at (i, j, k-1): lam^{n-2} and the step-(n-1) term
+            const int k2 = k - 1;
+            {
+                const bool st = k2 >= k0 && out_thread;
+                T v = T(0);
+                double gv = 0.0;
+                if (st) {
+                    const T pw = s1[pbf][ty][tx - 1], pe = s1[pbf][ty][tx + 1];
+                    const T ps = s1[pbf][ty - 1][tx], pn = s1[pbf][ty + 1][tx];
+                    T dummy2;
+                    v = var_update<T, MODE>(cf.v, l1m1, pw, pe, ps, pn, l1m2, l1, fp, T(0), &dummy2);
+                    if (!(inner_xy && k2 >= klo && k2 <= khi)) v = T(0);
+                    double t2 = 0.0;
+                    t2 += cf.th0 * adj_face_sum(i > 0, i < g.nx - 1, (double)l1m1, (double)pw, (double)pe,
+                                                (double)u2m1, (double)sb[pbf][ty][tx - 1],
+                                                (double)sb[pbf][ty][tx + 1]);
+                    t2 += cf.th1 * adj_face_sum(j > 0, j < g.ny - 1, (double)l1m1, (double)ps, (double)pn,
+                                                (double)u2m1, (double)sb[pbf][ty - 1][tx],
+                                                (double)sb[pbf][ty + 1][tx]);
+                    t2 += cf.th2 * adj_face_sum(k2 > 0, k2 < g.nz - 1, (double)l1m1, (double)l1m2, (double)l1,
+                                                (double)u2m1, (double)u2m2, (double)vc);
+                    gv = gm1 + 0.5 * t1m1;
+                    gv = gv + 0.5 * t2;
+                }
+                const uint32_t os = st ? oo : range;
+                bstore<T>(v, plane_rsrc((const char*)lamo + plane(k2) * pb, range), os);
+                bstore<double>(gv, plane_rsrc((const char*)grad + plane(k2) * pg, rangeg), st ? og : rangeg);
+            }
+            l1m2 = l1m1;
+            l1m1 = l1;
+            u2m2 = u2m1;
+            u2m1 = vc;
+            t1m1 = t1;
+            gm1 = gk;
+            fp = f;
+            fup = f.u;
+            am = ac;
+            u1m = uc;
+        }
+    }
+}
+
+// FP32: 64 x 16 threads, FP64: 64 x 8 (its LDS planes take twice the
+// space); 32-plane z chunks; two planes in flight.
+template <typename T>
+struct Adj2Tile {
+    static constexpr int TX = 64, TY = sizeof(T) == 4 ? 16 : 8, D = 2, ZC = 32;
+};
+
+// 3-D grids whose in-plane byte offsets of the FP64 gradient fit 32 bits.
+bool heatvar_adj2_supported(const Geo& geo)
+{
+    return geo.dim == 3 && (geo.base + geo.e01) * 8 < kOobLimit;
+}
+
+template <typename T>
+static int adj2_launch_t(Ctx* c, const Geo& g, const void* lam, void* lamo, const void* kap, const void* u1,
+                         const void* u2, double* grad, const AdjVarCoef& cf)
+{
+    typedef Adj2Tile<T> A;
+    const int tiles_x = ceil_div(g.nx, A::TX - 2), tiles_y = ceil_div(g.ny, A::TY - 2);
+    const long nb = (long)tiles_x * tiles_y * ceil_div(g.nz, A::ZC);
+    if (nb > 0x7fffffffL) return fail(SCAFES_ERR_ARG, "grid too large");
+    heatvar_adj2_kernel<T, A::TX, A::TY, A::D><<<dim3((unsigned)nb), dim3(A::TX, A::TY), 0, c->stream>>>(
+        g, (const T*)lam, (const T*)kap, (const T*)u1, (const T*)u2, (T*)lamo, grad, cf, A::ZC, tiles_x, tiles_y);
+    SCAFES_HIP(hipGetLastError());
+    return SCAFES_OK;
+}
+
+int heatvar_adj2_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, const void* lam, void* lamo,
+                        const void* kap, const void* u1, const void* u2, double* grad, double tau)
+{
+    if (!heatvar_adj2_supported(geo)) return fail(SCAFES_ERR_UNSUPPORTED, "two-step adjoint needs a 3-D grid");
+    AdjVarCoef cf;
+    double hh[3];
+    for (int p = 0; p < 3; ++p) hh[p] = g->h[p] * g->h[p];
+    cf.v.tau = tau;
+    cf.v.hh0 = hh[0];
+    cf.v.hh1 = hh[1];
+    cf.v.hh2 = hh[2];
+    cf.v.c0 = (float)(0.5 * tau / hh[0]);
+    cf.v.c1 = (float)(0.5 * tau / hh[1]);
+    cf.v.c2 = (float)(0.5 * tau / hh[2]);
+    cf.v.border_value = 0.0;
+    cf.v.border_mode = SCAFES_BORDER_VALUE;
+    cf.th0 = tau / hh[0];
+    cf.th1 = tau / hh[1];
+    cf.th2 = tau / hh[2];
+    if (dtype == SCAFES_F64) return adj2_launch_t<double>(c, geo, lam, lamo, kap, u1, u2, grad, cf);
+    return adj2_launch_t<float>(c, geo, lam, lamo, kap, u1, u2, grad, cf);
+}
+
 // Multi-step tiles: steps per sweep (2 or 3), workgroup tile, z planes per
 // workgroup, prefetch depth.  The two-step entries keep their indices.
 struct TbCfg {

@@ -123,6 +123,13 @@ int heat7var_sweep(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const
 int heatvar_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, const void* lam, void* lam_new,
                        const void* kap, const void* uprev, double* grad, double tau);
 
+// Two reverse steps of the variable-kappa adjoint per sweep (defined in
+// heatvar_tb2.hip): lam^n -> lam^{n-2} with u1 = u^{n-1}, u2 = u^{n-2};
+// bit-identical to two heatvar_adj_launch calls.
+bool heatvar_adj2_supported(const Geo& geo);
+int heatvar_adj2_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, const void* lam, void* lamo,
+                        const void* kap, const void* u1, const void* u2, double* grad, double tau);
+
 // K = 2..kAdjMaxK reverse Pennes adjoint steps per sweep (defined in
 // pennes_adjk.hip): lam^n -> lam^{n-K} with t[l] = T^{n-1-l} (+ tlo[l] for
 // compensated states, else tlo = nullptr), l = 0..K-1.

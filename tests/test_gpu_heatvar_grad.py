@@ -4,7 +4,9 @@ oracle's discrete adjoint (pinned by finite differences in
 test_oracle_heatvar_grad.py):
 
  * FP64: gradient bit-identical to the oracle for every snapshot budget, both
-   border modes, one and two border layers, 1-D to 3-D; J to 1e-12;
+   border modes, one and two border layers, 1-D to 3-D; J to 1e-12; on 3-D
+   grids the store-all stretches run two reverse steps per sweep, also on
+   ragged tiles and several z chunks;
  * FP32 and FP32+Kahan fields (FP64 gradient): within 1e-4 relative L2 of
    the FP64 oracle (measured values printed);
  * FP32: checkpointing changes the schedule, not the arithmetic: the
@@ -77,6 +79,21 @@ def test_fp64_bitexact_vs_oracle(ctx, n, nlayers, border, snaps):
     assert abs(J - Jref) <= 1e-12 * abs(Jref)
     assert stats[0] == ctx.L.scafes_checkpoint_forward_steps(N, snaps) and stats[1] == N
     assert np.abs(gref).max() > 1e-6
+
+
+@pytest.mark.parametrize("snaps", [0, 8])
+def test_fp64_two_step_reverse_sweep_ragged_tiles(ctx, snaps):
+    """A 3-D grid with ragged x and y tiles and two z chunks of the
+    two-steps-per-sweep adjoint (store-all: every reverse step pair in one
+    sweep; no snapshots: one-step launches only): both bit-identical to the
+    oracle."""
+    n, N = (70, 33, 40), 9
+    g, h, u0, kap, tau, target = _case(n, seed=40 + snaps)
+    st = O.heat7var_forward_all(g, u0, kap, tau, N, O.BORDER_VALUE, 0.0)
+    Jref, gref = O.heat7var_grad(g, st, kap, target, tau, N)
+    J, grad, _ = _device(ctx, n, h, 1, capi.HEATVAR_F64, u0, kap, target, tau, capi.BORDER_VALUE, 0.0, N, snaps)
+    np.testing.assert_array_equal(grad, gref)
+    assert abs(J - Jref) <= 1e-12 * abs(Jref)
 
 
 @pytest.mark.parametrize("mode", [capi.HEATVAR_F32, capi.HEATVAR_F32_KAHAN])
