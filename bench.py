@@ -387,7 +387,7 @@ def bench_kappa_grad(ctx, capi, g, n, u0, k0, tau, nsteps, t_fwd_ms, snapshots=(
     out = {"workload": f"heat7var {n}^3 dJ/dkappa, {nsteps} steps, FP32+Kahan forward, FP32 adjoint, FP64 gradient",
            "fwd_ms": round(t_fwd_ms, 2),
            "recompute": "the headline multi-step sweep where the schedule advances several steps, else one step",
-           "adjoint_kernels": "heatvar_adj2_kernel<float, 64, 16, 2> (two reverse steps per sweep where the "
+           "adjoint_kernels": "heatvar_adj2_kernel<float, 64, 8, 2> (two reverse steps per sweep where the "
                               "schedule has both states at hand, 36 B/node per sweep: lam r+w, kappa, u^{n-1}, "
                               "u^{n-2}, FP64 gradient r+w); stream_kernel<HeatVarAdjOp<float>> (one step, 32 B) "
                               "elsewhere",

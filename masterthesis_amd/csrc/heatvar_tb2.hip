@@ -657,11 +657,13 @@ __global__ __launch_bounds__(TX* TY) void heatvar_adj2_kernel(Geo g, const T* __
     }
 }
 
-// FP32: 64 x 16 threads, FP64: 64 x 8 (its LDS planes take twice the
-// space); 32-plane z chunks; two planes in flight.
+// 64 x 8 threads, two planes in flight; FP32 64-plane z chunks, FP64 32.
+// Measured at 512^3 FP32 on an MI355X (us per two-step launch): 64x8 with
+// 64-plane chunks 1372, with 32 1395; 64x16 with 64 1408, with 32 1461;
+// 64x12 with 32 1724 (the results are bit-identical across tiles).
 template <typename T>
 struct Adj2Tile {
-    static constexpr int TX = 64, TY = sizeof(T) == 4 ? 16 : 8, D = 2, ZC = 32;
+    static constexpr int TX = 64, TY = 8, D = 2, ZC = sizeof(T) == 4 ? 64 : 32;
 };
 
 // 3-D grids whose in-plane byte offsets of the FP64 gradient fit 32 bits.
