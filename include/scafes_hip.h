@@ -121,7 +121,12 @@ int scafes_convert(scafes_ctx* ctx, void* dst, int dst_dtype, const void* src, i
  *                  [+ (tau*rhoc_inv)*F_old]
  * u_old/u_new: memAll with hw ghost layers; f_old/g_old/g_new: stencil-0
  * fields (NULL when unused; g_old is read by the flux border rules only).
- * Ghost layers of u_new are not written. */
+ * Ghost layers of u_new are not written.
+ * Stability: returns SCAFES_ERR_ARG unless
+ *   tau * (sum_p 2*coeff_a/h_p^2 [+ perf_c*perf_w]) <= 1,
+ * the positivity bound of the explicit step that the reference's drivers
+ * check (tests/system/HeatEqnFDM/Dirichlet/TimeLinSpaceLin/main3D.cpp:52-55);
+ * the border flux terms are not part of the bound. */
 typedef struct {
     double tau, coeff_a, rhoc_inv, perf_c, perf_w;
     int has_perf, has_source;
@@ -143,7 +148,13 @@ int scafes_heat7_step_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const d
  *       compensation field c: u_true ~ u + c).
  * Runs nsteps steps ping-ponging between (u, u2) and (c, c2); on return
  * *result_in_second = 1 when the final state is in u2/c2.  kappa has the
- * layout of u.  cfg: tile configuration index (-1 = autotuned/default). */
+ * layout of u.  cfg: tile configuration index (-1 = autotuned/default).
+ * A multi-step configuration runs nsteps / s sweeps of s steps; a
+ * remainder of two runs as one two-step sweep, a remainder of one as one
+ * step.  Stability: SCAFES_ERR_ARG unless tau * sum_p 2*kappa_max/h_p^2 <= 1
+ * (kappa_max = max |kappa| over all nodes, a device reduction kept per
+ * pointer until the library writes the field or scafes_coef_changed names
+ * it). */
 enum { SCAFES_HEATVAR_F64 = 0, SCAFES_HEATVAR_F32 = 1, SCAFES_HEATVAR_F32_KAHAN = 2 };
 int scafes_heat7var_run(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode, void* u,
                         void* u2, void* c, void* c2, const void* kappa, double tau, int nsteps,
@@ -159,8 +170,14 @@ int scafes_heat7var_cfg_desc(int cfg, int* tx, int* ty, int* zchunk, int* depth,
  * (autotuned choice, else the per-mode default; the two-steps-per-sweep
  * tiles need a 3-D grid). */
 int scafes_heat7var_active_cfg(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode, int* cfg);
-/* Project autotuner: times every configuration on the given buffers for
- * nsteps steps and stores the fastest as the default for this mode. */
+/* Project autotuner (no counterpart in the reference, whose device stub
+ * fixes BLOCK_SIZE 32, src/ScaFES_ProblemCuda.hpp:22): times every
+ * configuration on the given buffers over 6*ceil(nsteps/6) steps (a
+ * multiple of every configuration's steps per sweep, so no remainder sweep
+ * enters the timing) and keeps the fastest for this grid (extents, hw) and
+ * mode: later calls with cfg = -1 on the same grid use it (at most eight
+ * grids per context).  The buffers are overwritten: u/u2 (and c/c2) are
+ * advanced with border KEEP, an even number of sweeps per configuration. */
 int scafes_autotune_heat7var(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode, void* u,
                              void* u2, void* c, void* c2, const void* kappa, double tau,
                              int nsteps, int* best_cfg, float* best_ms);
@@ -208,6 +225,11 @@ int scafes_known_timelin_f64(scafes_ctx* ctx, const scafes_geom* g, const scafes
  * inner_only: restrict to GLOBAL_INNER nodes.  Result in FP64 on host. */
 int scafes_reduce(scafes_ctx* ctx, const scafes_geom* g, int dtype, const void* a, int hw_a,
                   const void* b, int hw_b, int op, int inner_only, double* out);
+/* Tells the stability checks that the coefficient field at p was written
+ * outside the library (by a kernel of the caller); the next step re-reads
+ * its maximum.  Writes through scafes_memcpy_*, scafes_fill, scafes_convert
+ * and scafes_free are seen without it. */
+int scafes_coef_changed(scafes_ctx* ctx, const void* p);
 
 /* ---- Pennes bioheat with perfusion / source fields ---------------------
  * TumorHeatEqnFDM::updateInner form (examples/TumorHeatEqnFDM/TumorHeatEqnFDM.hpp:233-269)
@@ -231,6 +253,11 @@ typedef struct {
 int scafes_pennes_step(scafes_ctx* ctx, const scafes_geom* g, int hw, int dtype,
                        const void* t_old, void* t_new, const void* w, const void* q,
                        const scafes_pennes_params* p);
+/* Stability of scafes_pennes_step(_kahan) and scafes_pennes_gradient
+ * (examples/TumorHeatEqnFDM/startSimulation.py:238-301): SCAFES_ERR_ARG
+ * unless tau * (sum_p 2*cdiff/h_p^2 + cperf*w_max) <= 1, with the
+ * convective face's 2*cdiff*h_conv/(k_cond*h_last) added for the TUMOR
+ * border; w_max = max |w| (cached per pointer as for heat7var). */
 /* FP32 step with Kahan-compensated accumulation: (t_old + c_old) ->
  * (t_new + c_new); the increment is formed from t_old. */
 int scafes_pennes_step_kahan(scafes_ctx* ctx, const scafes_geom* g, int hw, const float* t_old,

@@ -118,6 +118,7 @@ def load():
                                          C.c_int, ip, C.POINTER(C.c_float)])
     _sig(L, "scafes_heat7var_init", [vp, gp, C.c_int, C.c_int, vp, vp, C.c_uint64, C.c_double])
     _sig(L, "scafes_reduce", [vp, gp, C.c_int, vp, C.c_int, vp, C.c_int, C.c_int, C.c_int, dp])
+    _sig(L, "scafes_coef_changed", [vp, vp])
     _sig(L, "scafes_known_timelin_f64", [vp, gp, C.POINTER(TimelinParams), C.c_double, vp, vp, vp, C.c_int])
     _sig(L, "scafes_pennes_step", [vp, gp, C.c_int, C.c_int, vp, vp, vp, vp, C.POINTER(PennesParams)])
     _sig(L, "scafes_pennes_adjoint_step", [vp, gp, C.c_int, C.c_int, vp, vp, vp, vp, vp,

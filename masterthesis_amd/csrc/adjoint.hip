@@ -412,6 +412,15 @@ int scafes_pennes_gradient(scafes_ctx* ctx, const scafes_geom* g, int hw, int dt
     if (hw < 1 || g->nlayers < 1) return fail(SCAFES_ERR_ARG, "pennes gradient: need hw >= 1 and nlayers >= 1");
     if (p->border_mode != SCAFES_BORDER_KEEP)
         return fail(SCAFES_ERR_UNSUPPORTED, "pennes gradient: only border_mode KEEP has an adjoint");
+    {
+        Geo geo;
+        int rc = make_geo(g, hw, &geo);
+        if (rc) return rc;
+        // the perfusion field changes between the calls of an optimisation: no cache
+        rc = pennes_check_tau(c, geo, g, hw, dtype == SCAFES_F64 ? SCAFES_F64 : SCAFES_F32, w, p, false,
+                              "pennes gradient");
+        if (rc) return rc;
+    }
     Rev R;
     R.family = FAMILY_PENNES;
     R.w = w;
@@ -435,6 +444,16 @@ int scafes_heat7var_gradient(scafes_ctx* ctx, const scafes_geom* g, int hw, int 
     if (border_mode != SCAFES_BORDER_KEEP && border_mode != SCAFES_BORDER_VALUE)
         return fail(SCAFES_ERR_UNSUPPORTED, "heat7var gradient: border must be KEEP or VALUE");
     static const int kDtype[3] = {SCAFES_F64, SCAFES_F32, SCAFES_F32_KAHAN};
+    {
+        Geo geo;
+        int rc = make_geo(g, hw, &geo);
+        if (rc) return rc;
+        double kmax = 0.0;
+        rc = coef_abs_max(c, geo, hw, mode == SCAFES_HEATVAR_F64 ? SCAFES_F64 : SCAFES_F32, kappa, false, &kmax);
+        if (rc) return rc;
+        rc = check_explicit_tau(tau, diffusion_rate(g, kmax), "heat7var gradient");
+        if (rc) return rc;
+    }
     Rev R;
     R.family = FAMILY_HEATVAR;
     R.heat_mode = mode;

@@ -178,6 +178,74 @@ int reduce_nodes(Ctx* c, const Geo& ga, const Geo& gb, int dtype, const void* a,
     return SCAFES_OK;
 }
 
+int coef_abs_max(Ctx* c, const Geo& geo, int hw, int dtype, const void* p, bool use_cache, double* out)
+{
+    const long count = (long)geo.nx * geo.ny * geo.nz;
+    if (use_cache)
+        for (int i = 0; i < c->ncoef; ++i) {
+            const Ctx::CoefMax& e = c->coef[i];
+            if (e.p == p && e.count == count && e.dtype == dtype && e.hw == hw) {
+                *out = e.v;
+                return SCAFES_OK;
+            }
+        }
+    double v = 0.0;
+    const int rc = reduce_nodes(c, geo, geo, dtype, p, nullptr, RED_MAX_ABS_DIFF, 0, &v);
+    if (rc) return rc;
+    // memAll span of the field in bytes (ghost planes included)
+    const long h2 = geo.dim > 2 ? hw : 0;
+    const size_t span = (size_t)(geo.e01 * (geo.nz + 2 * h2)) * (dtype == SCAFES_F64 ? 8 : 4);
+    // newest entry first; the oldest falls out of the eight slots
+    const int n = c->ncoef < 8 ? c->ncoef + 1 : 8;
+    for (int i = n - 1; i > 0; --i) c->coef[i] = c->coef[i - 1];
+    c->coef[0] = Ctx::CoefMax{p, count, span, dtype, hw, v};
+    c->ncoef = n;
+    *out = v;
+    return SCAFES_OK;
+}
+
+void coef_invalidate(Ctx* c, const void* p, size_t bytes)
+{
+    if (!c) return;
+    if (bytes == 0) {
+        c->ncoef = 0;
+        return;
+    }
+    const char* lo = (const char*)p;
+    const char* hi = lo + bytes;
+    int k = 0;
+    for (int i = 0; i < c->ncoef; ++i) {
+        const Ctx::CoefMax& e = c->coef[i];
+        // a cached field spans at most its memAll elements from its pointer;
+        // any write starting before that span ends may touch it
+        const char* flo = (const char*)e.p;
+        const bool overlap = lo < flo + e.span && flo < hi;
+        if (!overlap) c->coef[k++] = e;
+    }
+    c->ncoef = k;
+}
+
+double diffusion_rate(const scafes_geom* g, double dmax)
+{
+    double r = 0.0;
+    for (int q = 0; q < g->dim; ++q) r += 2.0 * dmax / (g->h[q] * g->h[q]);
+    return r;
+}
+
+int check_explicit_tau(double tau, double rate, const char* who)
+{
+    if (!(tau > 0.0)) return fail(SCAFES_ERR_ARG, std::string(who) + ": tau must be positive");
+    if (tau * rate > 1.0 + 1e-12) {
+        char buf[256];
+        std::snprintf(buf, sizeof buf,
+                      "%s: explicit step unstable: tau = %.6e exceeds the bound 1 / (sum_p 2 d_max / h_p^2 + "
+                      "reaction) = %.6e",
+                      who, tau, 1.0 / rate);
+        return fail(SCAFES_ERR_ARG, buf);
+    }
+    return SCAFES_OK;
+}
+
 }  // namespace scafes
 
 using namespace scafes;
@@ -209,7 +277,6 @@ int scafes_ctx_create(int device, scafes_ctx** out)
     SCAFES_HIP(hipSetDevice(device));
     Ctx* c = new Ctx();
     c->device = device;
-    for (int i = 0; i < 16; ++i) c->tuned[i] = -1;
     hipDeviceProp_t prop;
     SCAFES_HIP(hipGetDeviceProperties(&prop, device));
     c->cu_count = prop.multiProcessorCount;
@@ -306,6 +373,7 @@ int scafes_free(scafes_ctx* ctx, void* dptr)
     Ctx* c = as_ctx(ctx);
     if (!c) return fail(SCAFES_ERR_ARG, "null context");
     if (!dptr) return SCAFES_OK;
+    coef_invalidate(c, nullptr, 0);  // a later allocation may reuse the address
     SCAFES_HIP(hipStreamSynchronize(c->stream));
     SCAFES_HIP(hipFree(dptr));
     return SCAFES_OK;
@@ -315,6 +383,7 @@ static int copy(Ctx* c, void* dst, const void* src, size_t bytes, hipMemcpyKind 
 {
     if (!c || (!dst && bytes) || (!src && bytes)) return fail(SCAFES_ERR_ARG, "null pointer");
     if (!bytes) return SCAFES_OK;
+    if (kind == hipMemcpyHostToDevice) coef_invalidate(c, dst, bytes);
     SCAFES_HIP(hipMemcpyAsync(dst, src, bytes, kind, c->stream));
     if (kind != hipMemcpyDeviceToDevice) SCAFES_HIP(hipStreamSynchronize(c->stream));
     return SCAFES_OK;
@@ -335,6 +404,7 @@ int scafes_memcpy_d2d(scafes_ctx* ctx, void* dst, const void* src, size_t bytes)
     Ctx* c = as_ctx(ctx);
     if (!c || ((!dst || !src) && bytes)) return fail(SCAFES_ERR_ARG, "null pointer");
     if (!bytes) return SCAFES_OK;
+    coef_invalidate(c, dst, bytes);
     const bool aligned = ((uintptr_t)dst % 16 == 0) && ((uintptr_t)src % 16 == 0);
     const size_t n16 = aligned ? bytes / 16 : 0;
     if (n16) {
@@ -357,6 +427,7 @@ int scafes_fill(scafes_ctx* ctx, void* dptr, int dtype, size_t count, double val
     Ctx* c = as_ctx(ctx);
     if (!c || (!dptr && count)) return fail(SCAFES_ERR_ARG, "null pointer");
     if (!count) return SCAFES_OK;
+    coef_invalidate(c, dptr, count * (dtype == SCAFES_F64 ? 8 : 4));
     const int blocks = (int)std::min<size_t>((count + 255) / 256, 65536);
     if (dtype == SCAFES_F64) fill_kernel<double><<<blocks, 256, 0, c->stream>>>((double*)dptr, count, value);
     else if (dtype == SCAFES_F32) fill_kernel<float><<<blocks, 256, 0, c->stream>>>((float*)dptr, count, (float)value);
@@ -371,6 +442,7 @@ int scafes_convert(scafes_ctx* ctx, void* dst, int dst_dtype, const void* src, i
     Ctx* c = as_ctx(ctx);
     if (!c || ((!dst || !src) && count)) return fail(SCAFES_ERR_ARG, "null pointer");
     if (!count) return SCAFES_OK;
+    coef_invalidate(c, dst, count * (dst_dtype == SCAFES_F64 ? 8 : 4));
     const int blocks = (int)std::min<size_t>((count + 255) / 256, 65536);
     if (dst_dtype == SCAFES_F32 && src_dtype == SCAFES_F64)
         convert_kernel<float, double><<<blocks, 256, 0, c->stream>>>((float*)dst, (const double*)src, count);
@@ -381,6 +453,14 @@ int scafes_convert(scafes_ctx* ctx, void* dst, int dst_dtype, const void* src, i
     else
         return fail(SCAFES_ERR_ARG, "convert: bad dtype");
     SCAFES_HIP(hipGetLastError());
+    return SCAFES_OK;
+}
+
+int scafes_coef_changed(scafes_ctx* ctx, const void* p)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c) return fail(SCAFES_ERR_ARG, "coef_changed: null context");
+    coef_invalidate(c, p, 1);
     return SCAFES_OK;
 }
 

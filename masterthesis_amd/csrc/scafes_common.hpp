@@ -37,8 +37,26 @@ struct Ctx {
     double* host_result = nullptr;  // pinned host mirror of red_result
     int red_capacity = 0;           // number of partial slots
     int cu_count = 0;
-    // autotuned tile choice per (kernel family, dtype); -1 = not tuned yet
-    int tuned[16];
+    // autotuned heat7var tile per grid and mode (scafes_autotune_heat7var)
+    struct Tuned {
+        int mode, nx, ny, nz;
+        long e0, e01;  // memAll strides: the ghost width of the tuned layout
+        int cfg;
+    };
+    Tuned tuned[8] = {};
+    int ntuned = 0;
+    // largest |value| of coefficient fields read by the explicit-step
+    // stability checks, per device pointer (invalidated by every write the
+    // library makes into the field, by scafes_free and by scafes_coef_changed)
+    struct CoefMax {
+        const void* p;
+        long count;   // nodes of the grid it was reduced over
+        size_t span;  // bytes of its memAll layout from p
+        int dtype, hw;
+        double v;
+    };
+    CoefMax coef[8] = {};
+    int ncoef = 0;
     hipEvent_t events[8] = {};
     // gradient workspace (states, adjoint fields, snapshot pool) kept between
     // gradient calls; grows on demand, freed by scafes_ctx_release_workspace
@@ -189,5 +207,27 @@ int yee_tb2_launch(Ctx* c, const YeeDims& d, const Tb2Plan& pl, T s, const T* a,
 enum RedOp { RED_MAX_ABS_DIFF = 0, RED_SUM_SQ_DIFF = 1, RED_SUM_PROD = 2 };
 int reduce_nodes(Ctx* c, const Geo& ga, const Geo& gb, int dtype, const void* a, const void* b,
                  int op, int inner_only, double* out);
+
+// Largest |value| over all nodes of a coefficient field with hw ghost
+// layers (FP64 device reduction).  use_cache: reuse the value of an earlier
+// call for the same pointer, grid and dtype unless the field was written
+// through the library since.
+int coef_abs_max(Ctx* c, const Geo& geo, int hw, int dtype, const void* p, bool use_cache, double* out);
+// Drops cached maxima of fields overlapping [p, p + bytes); bytes = 0 drops all.
+void coef_invalidate(Ctx* c, const void* p, size_t bytes);
+
+// Explicit-Euler stability of the diffusion updates, in the positivity form
+// the reference's drivers check (tests/system/HeatEqnFDM/Dirichlet/
+// TimeLinSpaceLin/main3D.cpp:52-55, examples/TumorHeatEqnFDM/
+// startSimulation.py:238-301): the new value is a nonnegative combination of
+// the old ones when tau * rate <= 1, rate = sum_p 2 d_max / h_p^2 (+ the
+// largest reaction coefficient).  Returns SCAFES_ERR_ARG with both numbers
+// otherwise.
+double diffusion_rate(const scafes_geom* g, double dmax);
+int check_explicit_tau(double tau, double rate, const char* who);
+// The Pennes bound (stencil3d.hip): diffusion + cperf * max|w| (+ the
+// Tumor border's convective face); w has the layout of T (hw).
+int pennes_check_tau(Ctx* c, const Geo& geo, const scafes_geom* g, int hw, int dtype, const void* w,
+                     const scafes_pennes_params* p, bool use_cache, const char* who);
 
 }  // namespace scafes

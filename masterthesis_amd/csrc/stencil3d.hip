@@ -602,6 +602,10 @@ static constexpr int kOneStepCfgByMode[3] = {12, 4, 9};
 // 64x16 zc64 d2 (10; 620 us/step against 658 for the best two-step tile),
 // F32 and F32+Kahan 64x12 zc64 d2 (13; 234 and 280 us/step).
 static constexpr int kDefaultCfgByMode[3] = {kNumCfgs + 10, kNumCfgs + 13, kNumCfgs + 13};
+// Two-step tile per mode for a remainder of two steps behind a three-step
+// configuration (F64 64x8 zc64 d4, F32 and F32+Kahan 64x8 zc64 d2): one
+// sweep instead of two one-step sweeps.
+static constexpr int kTwoStepCfgByMode[3] = {kNumCfgs + 6, kNumCfgs + 2, kNumCfgs + 2};
 
 template <class Op, typename T, int TX, int TY, int D>
 static int launch_t(Ctx* c, const Op& op, const Geo& g, int zc)
@@ -897,6 +901,25 @@ __global__ void heatvar_init_kernel(Geo g, double hx, double hy, double hz, T* u
 
 using namespace scafes;
 
+namespace scafes {
+
+int pennes_check_tau(Ctx* c, const Geo& geo, const scafes_geom* g, int hw, int dtype, const void* w,
+                     const scafes_pennes_params* p, bool use_cache, const char* who)
+{
+    double wmax = 0.0;
+    if (p->cperf != 0.0) {
+        const int rc = coef_abs_max(c, geo, hw, dtype == SCAFES_F64 ? SCAFES_F64 : SCAFES_F32, w, use_cache, &wmax);
+        if (rc) return rc;
+    }
+    double rate = diffusion_rate(g, fabs(p->cdiff)) + fabs(p->cperf) * wmax;
+    // the Cauchy face of the Tumor border (startSimulation.py:262-270)
+    if (p->border_mode == SCAFES_BORDER_TUMOR && p->k_cond != 0.0)
+        rate += 2.0 * fabs(p->cdiff * p->h_conv / p->k_cond) / g->h[g->dim - 1];
+    return check_explicit_tau(p->tau, rate, who);
+}
+
+}  // namespace scafes
+
 extern "C" {
 
 int scafes_heat7_step_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const double* u_old,
@@ -921,6 +944,10 @@ int scafes_heat7_step_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const d
     rc = make_geo(g, hw, &geo);
     if (rc) return rc;
     rc = make_geo(g, 0, &gk);
+    if (rc) return rc;
+    rc = check_explicit_tau(p->tau, diffusion_rate(g, fabs(p->coeff_a)) +
+                                        (p->has_perf ? fabs(p->perf_c * p->perf_w) : 0.0),
+                            "heat7");
     if (rc) return rc;
     double hh[3];
     hh_of(g, hh);
@@ -973,12 +1000,27 @@ int scafes_heat7var_cfg_desc(int cfg, int* tx, int* ty, int* zchunk, int* depth,
     return SCAFES_OK;
 }
 
-// cfg < 0: the autotuned choice for this mode, else the default; a
-// two-steps-per-sweep choice on a grid that kernel does not cover falls back
-// to the mode's one-step tile.
+// The autotuned configuration of this grid and mode, or -1.
+static int tuned_cfg(const Ctx* c, const Geo& geo, int mode)
+{
+    for (int i = 0; i < c->ntuned; ++i) {
+        const Ctx::Tuned& t = c->tuned[i];
+        if (t.mode == mode && t.nx == geo.nx && t.ny == geo.ny && t.nz == geo.nz && t.e0 == geo.e0 &&
+            t.e01 == geo.e01)
+            return t.cfg;
+    }
+    return -1;
+}
+
+// cfg < 0: the autotuned choice for this grid and mode, else the default; a
+// multi-step choice on a grid that kernel does not cover falls back to the
+// mode's one-step tile.
 static int resolve_heatvar_cfg(const Ctx* c, const Geo& geo, int mode, int cfg)
 {
-    if (cfg < 0) cfg = c->tuned[mode] >= 0 ? c->tuned[mode] : kDefaultCfgByMode[mode];
+    if (cfg < 0) {
+        cfg = tuned_cfg(c, geo, mode);
+        if (cfg < 0) cfg = kDefaultCfgByMode[mode];
+    }
     if (cfg >= kNumCfgs && !heatvar_tb2_supported(geo, mode)) cfg = kOneStepCfgByMode[mode];
     return cfg;
 }
@@ -1013,6 +1055,13 @@ int scafes_heat7var_run(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode,
     rc = make_geo(g, hw, &geo);
     if (rc) return rc;
     if (cfg >= scafes_heat7var_num_cfgs()) return fail(SCAFES_ERR_ARG, "heat7var: bad cfg");
+    if (nsteps > 0) {
+        double kmax = 0.0;
+        rc = coef_abs_max(c, geo, hw, mode == SCAFES_HEATVAR_F64 ? SCAFES_F64 : SCAFES_F32, kappa, true, &kmax);
+        if (rc) return rc;
+        rc = check_explicit_tau(tau, diffusion_rate(g, kmax), "heat7var");
+        if (rc) return rc;
+    }
     cfg = resolve_heatvar_cfg(c, geo, mode, cfg);
     void *a = u, *b = u2, *ca = cmp, *cb = cmp2;
     int launches = 0;
@@ -1027,7 +1076,17 @@ int scafes_heat7var_run(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode,
             std::swap(a, b);
             std::swap(ca, cb);
         }
-        cfg = kOneStepCfgByMode[mode];  // remaining steps one per sweep
+        // two steps left behind a three-step tile: one two-step sweep
+        if (nsteps - s == 2) {
+            rc = heatvar_tb2_launch(c, geo, g, mode, a, b, ca, cb, kappa, tau, border_mode, border_value,
+                                    kTwoStepCfgByMode[mode] - kNumCfgs);
+            if (rc) return rc;
+            std::swap(a, b);
+            std::swap(ca, cb);
+            s += 2;
+            ++launches;
+        }
+        cfg = kOneStepCfgByMode[mode];  // a last step one per sweep
     }
     for (; s < nsteps; ++s, ++launches) {
         rc = heat7var_launch(c, geo, g, mode, a, b, ca, cb, kappa, tau, border_mode, border_value, cfg);
@@ -1044,29 +1103,43 @@ int scafes_autotune_heat7var(scafes_ctx* ctx, const scafes_geom* g, int hw, int 
                              int nsteps, int* best_cfg, float* best_ms)
 {
     Ctx* c = as_ctx(ctx);
-    if (!c) return fail(SCAFES_ERR_ARG, "autotune: null context");
-    if (nsteps < 1) nsteps = 1;
+    if (!c || !g) return fail(SCAFES_ERR_ARG, "autotune: null pointer");
+    if (mode < 0 || mode > 2) return fail(SCAFES_ERR_ARG, "autotune: bad mode");
+    Geo geo;
+    int rc = make_geo(g, hw, &geo);
+    if (rc) return rc;
+    // a multiple of 6 runs every configuration (1, 2 or 3 steps per sweep)
+    // without a remainder sweep, and an even sweep count keeps u in place
+    const int steps = 6 * std::max(1, (nsteps + 5) / 6);
     hipEvent_t e0, e1;
     SCAFES_HIP(hipEventCreate(&e0));
     SCAFES_HIP(hipEventCreate(&e1));
     int best = -1;
     float bms = 1e30f;
     const int ncfg = scafes_heat7var_num_cfgs();
-    for (int cfg = 0; cfg < ncfg; ++cfg) {
+    for (int cfg = 0; cfg < ncfg && rc == SCAFES_OK; ++cfg) {
+        int st = 1, tx_, ty_, zc_, d_;
+        scafes_heat7var_cfg_desc(cfg, &tx_, &ty_, &zc_, &d_, &st);
+        if (st > 1 && !heatvar_tb2_supported(geo, mode)) continue;  // would run the one-step tile
         int which = 0;
-        // warm-up one step, then time nsteps (an even count keeps u in place)
-        int rc = scafes_heat7var_run(ctx, g, hw, mode, u, u2, cmp, cmp2, kappa, tau, 2, SCAFES_BORDER_KEEP,
-                                     0.0, cfg, &which);
-        if (rc) return rc;
-        SCAFES_HIP(hipEventRecord(e0, c->stream));
-        rc = scafes_heat7var_run(ctx, g, hw, mode, u, u2, cmp, cmp2, kappa, tau, 2 * ((nsteps + 1) / 2),
-                                 SCAFES_BORDER_KEEP, 0.0, cfg, &which);
-        if (rc) return rc;
-        SCAFES_HIP(hipEventRecord(e1, c->stream));
-        SCAFES_HIP(hipEventSynchronize(e1));
+        // warm-up: one launch of each sweep the timed run makes
+        rc = scafes_heat7var_run(ctx, g, hw, mode, u, u2, cmp, cmp2, kappa, tau, 2 * st, SCAFES_BORDER_KEEP, 0.0,
+                                 cfg, &which);
+        if (rc) break;
+        hipError_t e = hipEventRecord(e0, c->stream);
+        if (e == hipSuccess)
+            rc = scafes_heat7var_run(ctx, g, hw, mode, u, u2, cmp, cmp2, kappa, tau, steps, SCAFES_BORDER_KEEP, 0.0,
+                                     cfg, &which);
+        if (rc) break;
+        if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0.f;
-        SCAFES_HIP(hipEventElapsedTime(&ms, e0, e1));
-        ms /= (float)(2 * ((nsteps + 1) / 2));
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+            rc = check_hip(e, "autotune timing");
+            break;
+        }
+        ms /= (float)steps;
         if (ms < bms) {
             bms = ms;
             best = cfg;
@@ -1074,7 +1147,25 @@ int scafes_autotune_heat7var(scafes_ctx* ctx, const scafes_geom* g, int hw, int 
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    c->tuned[mode] = best;
+    if (rc) return rc;
+    if (best < 0) return fail(SCAFES_ERR_UNSUPPORTED, "autotune: no configuration runs on this grid");
+    // keep the choice for this grid and mode (replacing an earlier one)
+    int slot = -1;
+    for (int i = 0; i < c->ntuned; ++i) {
+        const Ctx::Tuned& t = c->tuned[i];
+        if (t.mode == mode && t.nx == geo.nx && t.ny == geo.ny && t.nz == geo.nz && t.e0 == geo.e0 &&
+            t.e01 == geo.e01)
+            slot = i;
+    }
+    if (slot < 0) {
+        if (c->ntuned < 8) {
+            slot = c->ntuned++;
+        } else {
+            for (int i = 0; i < 7; ++i) c->tuned[i] = c->tuned[i + 1];  // drop the oldest
+            slot = 7;
+        }
+    }
+    c->tuned[slot] = Ctx::Tuned{mode, geo.nx, geo.ny, geo.nz, geo.e0, geo.e01, best};
     if (best_cfg) *best_cfg = best;
     if (best_ms) *best_ms = bms;
     return SCAFES_OK;
@@ -1088,6 +1179,7 @@ int scafes_heat7var_init(scafes_ctx* ctx, const scafes_geom* g, int hw, int dtyp
     Geo geo;
     int rc = make_geo(g, hw, &geo);
     if (rc) return rc;
+    coef_invalidate(c, nullptr, 0);  // kappa is rewritten
     const long total = (long)geo.nx * geo.ny * geo.nz;
     const int blocks = (int)std::min<long>((total + 255) / 256, 65536);
     if (dtype == SCAFES_F64)
@@ -1112,6 +1204,8 @@ int scafes_pennes_step(scafes_ctx* ctx, const scafes_geom* g, int hw, int dtype,
     rc = make_geo(g, hw, &geo);
     if (rc) return rc;
     if (dtype != SCAFES_F32 && dtype != SCAFES_F64) return fail(SCAFES_ERR_ARG, "pennes: dtype F32 or F64");
+    rc = pennes_check_tau(c, geo, g, hw, dtype, w, p, true, "pennes");
+    if (rc) return rc;
     return pennes_step_launch(c, geo, g, dtype, t_old, t_new, w, q, p, nullptr, nullptr);
 }
 
@@ -1127,6 +1221,8 @@ int scafes_pennes_step_kahan(scafes_ctx* ctx, const scafes_geom* g, int hw, cons
     if (rc) return rc;
     Geo geo;
     rc = make_geo(g, hw, &geo);
+    if (rc) return rc;
+    rc = pennes_check_tau(c, geo, g, hw, SCAFES_F32, w, p, true, "pennes kahan");
     if (rc) return rc;
     return pennes_step_launch(c, geo, g, SCAFES_F32_KAHAN, t_old, t_new, w, q, p, c_old, c_new);
 }

@@ -17,6 +17,7 @@
 // Initial and coefficient values are small integer patterns divided by
 // powers of two, so another program can rebuild them bit for bit.  The
 // final unknown fields are dumped to $SCAFES_DUMP (suffix = field index).
+#include <cstdlib>
 #include <iomanip>
 #include <iostream>
 
@@ -283,8 +284,13 @@ int main(int argc, char* argv[])
     typedef YeePec<double, DIM> Prb;
 #endif
     const std::size_t nDf = names.size();
+    // EXAMPLE_WRITE_ALL=1: every field written at the --nSnapshots intervals
+    // (time-independent known fields included) to --outputfile
+    const char* wa = std::getenv("EXAMPLE_WRITE_ALL");
+    const ScaFES::WriteHowOften pol =
+        (wa && wa[0] == '1') ? ScaFES::WriteHowOften::LIKE_GIVEN_AT_CL : ScaFES::WriteHowOften::NEVER;
     Prb ppp(params, gg, leapfrog, names, sw, known, std::vector<int>(nDf, 0), std::vector<double>(nDf, 0.0),
-            std::vector<ScaFES::WriteHowOften>(nDf, ScaFES::WriteHowOften::NEVER), std::vector<bool>(nDf, false));
+            std::vector<ScaFES::WriteHowOften>(nDf, pol), std::vector<bool>(nDf, false));
     ppp.iterateOverTime();
     for (std::size_t u = 0; u < ppp.vectUnknownDfsDomOld().size(); ++u)
         dumpNodes(ppp.vectUnknownDfsDomOld()[u], "." + std::to_string(u));

@@ -341,3 +341,27 @@ def test_init_file_through_problem(tmp_path, hip):
         G_new = O.known_timelin(g, O.KNOWN_HEAT_TIMELIN, O.BORDER_FIELD, 0.0 + s * tau, coef)[1]
         u = O.heat7_step(g, u, f_old=F_old, g_new=G_new, border=O.BORDER_FIELD, coef=coef)
     np.testing.assert_array_equal(np.fromfile(dump, dtype=np.float64), O.nodes_view(g, u).ravel())
+
+
+def test_time_independent_known_field_written_every_record(tmp_path, hip):
+    """A known field the problem declares time independent (kappa of the
+    var-kappa class, DeviceUpdate::knownTimeDependent = false) is evaluated
+    once; every snapshot record of it, odd time iterates and the last one
+    included, holds those values (the reference re-evaluates it every step,
+    src/ScaFES_Problem.hpp:2596, with the same result)."""
+    n, nsteps = (9, 8, 7), 3
+    g = O.make_grid(n)
+    tau = 0.9 / (2.0 * 1.5 * sum((m - 1.0) ** 2 for m in n))
+    out = str(tmp_path / "snap")
+    args = _family_args(n, nsteps, tau) + [f"--outputfile={out}", f"--nSnapshots={nsteps}"]
+    env = dict(os.environ, SCAFES_DUMP=str(tmp_path / "u.bin"), EXAMPLE_WRITE_ALL="1")
+    p = subprocess.run([os.path.join(BIN, "heatvar_f64")] + args, capture_output=True, text=True, env=env,
+                       timeout=120)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    nc = netcdf_file(out + ".nc", "r", mmap=False)
+    kap = np.array(nc.variables["kappaNewSolDom"][:])
+    assert kap.shape == (nsteps + 1, n[2], n[1], n[0])
+    expect = 1.0 + _pattern(n, 1, 2, 3, 5, 8.0)[0]
+    for r in range(nsteps + 1):
+        np.testing.assert_array_equal(kap[r].ravel(), expect.ravel())
+    nc.close()
