@@ -374,6 +374,21 @@ int scafes_mri_step_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const dou
  * inside the H half step. */
 int scafes_yee_run(scafes_ctx* ctx, int ncell, int dtype, double s, void* f, void* f2, int nsteps,
                    int energy_every, double* energy, int* result_in_second);
+/* scafes_yee_run without any allocation, copy or wait: no counterpart in
+ * the reference; for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and results as scafes_yee_run (bit for bit), except that the
+ * energy records go to the caller's DEVICE array energy_dev
+ * (ceil(nsteps/energy_every) FP64 entries, written in stream order, not
+ * waited for).  The per-workgroup energy partials live in a static 1 MiB
+ * device array per GPU (2^17 doubles, two per workgroup of a launch: boxes
+ * up to about 1150 cells per axis); a larger box is refused with
+ * SCAFES_ERR_ARG before anything runs.  Nothing in the array needs clearing:
+ * every workgroup writes its slot before the slots are summed.  All calls on
+ * one GPU share that array: calls that overlap on one GPU (two contexts or
+ * streams at once) race and must be ordered by the caller. */
+int scafes_yee_run_noalloc(scafes_ctx* ctx, int ncell, int dtype, double s, void* f, void* f2, int nsteps,
+                           int energy_every, double* energy_dev, int* result_in_second);
 /* Initial state of the Yee workload: all fields zero except
  * Ez = exp(-r^2/(2 sigma^2)) at inner Ez nodes, r from the box centre in
  * cell units (Ez(i,j,k) sits at (i, j, k+1/2)). */

@@ -136,6 +136,7 @@ def load():
                                              C.c_double, C.c_int, C.c_int, vp, dp, dp])
     if hasattr(L, "scafes_yee_run"):
         _sig(L, "scafes_yee_run", [vp, C.c_int, C.c_int, C.c_double, vp, vp, C.c_int, C.c_int, dp, ip])
+        _sig(L, "scafes_yee_run_noalloc", [vp, C.c_int, C.c_int, C.c_double, vp, vp, C.c_int, C.c_int, vp, ip])
         _sig(L, "scafes_yee_init_pulse", [vp, C.c_int, C.c_int, vp, C.c_double])
     if hasattr(L, "scafes_yee_eps_run"):
         _sig(L, "scafes_yee_eps_run", [vp, C.c_int, C.c_int, C.c_double, vp, vp, vp, C.c_int])

@@ -286,6 +286,13 @@ int scafes_ctx_create(int device, scafes_ctx** out)
     SCAFES_HIP(hipMalloc(&c->red_result, sizeof(double) * 8));
     SCAFES_HIP(hipHostMalloc(&c->host_result, sizeof(double) * 8, hipHostMallocDefault));
     for (int i = 0; i < 8; ++i) SCAFES_HIP(hipEventCreate(&c->events[i]));
+    // device address of the no-allocation entry point's scratch array, looked
+    // up once per device with that device current
+    const int rc = yee_noalloc_setup(device);
+    if (rc) {
+        scafes_ctx_destroy(reinterpret_cast<scafes_ctx*>(c));
+        return rc;
+    }
     *out = reinterpret_cast<scafes_ctx*>(c);
     return SCAFES_OK;
 }

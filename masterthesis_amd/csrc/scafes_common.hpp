@@ -188,6 +188,10 @@ int yee_eps_lam_init(Ctx* c, const YeeDims& d, int dtype, const void* fN, const 
 // One adjoint step: st = state n-1, la = (lam_E', lam_H') -> lb, grad +=.
 int yee_eps_adjoint(Ctx* c, const YeeDims& d, int dtype, double s, const void* ce, const void* eps, const void* st,
                     const void* la, void* lb, double* grad);
+// Looks up, once per device and with that device current, the device
+// address of the scratch array of scafes_yee_run_noalloc (yee.hip); called
+// by scafes_ctx_create.
+int yee_noalloc_setup(int device);
 // One fused leapfrog step a -> b with the E coefficient field ce (yee.hip).
 template <typename T>
 int yee_fused_eps_launch(Ctx* c, const YeeDims& d, T s, const T* a, T* b, const T* ce);

@@ -24,6 +24,9 @@
 #include <algorithm>
 #include <vector>
 
+#include <cstdio>
+#include <mutex>
+
 #include "scafes_common.hpp"
 
 namespace scafes {
@@ -357,9 +360,14 @@ __global__ void yee_pulse_kernel(YeeDims d, T* f, double sigma)
 constexpr int kYTX = 64, kYTY = 4, kYZC = 64;         // two-pass in-place tiles
 constexpr int kFTX = 64, kFTY = 8, kFZC = 64, kFD = 2;  // fused out-of-place tile
 
+// energy: host array of the records (allocating path: the partials and the
+// device records come from the stream-ordered allocator and the records are
+// copied to the host at the end).  part != nullptr: the no-allocation path:
+// the partials live in part (part_slots doubles) and the records are written
+// to the device array edev_out; nothing is allocated, copied or waited for.
 template <typename T>
 static int yee_run_t(Ctx* c, int N, double s, T* f, T* f2, int nsteps, int energy_every, double* energy,
-                     int* result_in_second)
+                     int* result_in_second, double* part = nullptr, long part_slots = 0, double* edev_out = nullptr)
 {
     YeeDims d;
     d.N = N;
@@ -379,7 +387,18 @@ static int yee_run_t(Ctx* c, int N, double s, T* f, T* f2, int nsteps, int energ
     const int nrec = (energy_every > 0) ? (nsteps + energy_every - 1) / energy_every : 0;
     double* epart = nullptr;
     double* edev = nullptr;
-    if (nrec > 0) {
+    const bool noalloc = part != nullptr;
+    if (nrec > 0 && noalloc) {
+        if (2 * nbp > part_slots) {
+            char buf[200];
+            std::snprintf(buf, sizeof buf,
+                          "yee noalloc: %d cells per axis need %ld energy partial slots, the scratch array holds %ld",
+                          N, 2 * nbp, part_slots);
+            return fail(SCAFES_ERR_ARG, buf);
+        }
+        epart = part;
+        edev = edev_out;
+    } else if (nrec > 0) {
         SCAFES_HIP(hipMallocAsync((void**)&epart, sizeof(double) * 2 * nbp, c->stream));
         SCAFES_HIP(hipMallocAsync((void**)&edev, sizeof(double) * nrec, c->stream));
     }
@@ -419,7 +438,7 @@ static int yee_run_t(Ctx* c, int N, double s, T* f, T* f2, int nsteps, int energ
     }
     // every out-of-place launch ends with a swap: the final state is in a
     if (result_in_second) *result_in_second = (fused && a == f2) ? 1 : 0;
-    if (nrec > 0) {
+    if (nrec > 0 && !noalloc) {
         if (rc == SCAFES_OK) {
             hipError_t e = hipMemcpyAsync(energy, edev, sizeof(double) * nrec, hipMemcpyDeviceToHost, c->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -447,6 +466,39 @@ int yee_fused_eps_launch(Ctx* c, const YeeDims& d, T s, const T* a, T* b, const 
 template int yee_fused_eps_launch<float>(Ctx*, const YeeDims&, float, const float*, float*, const float*);
 template int yee_fused_eps_launch<double>(Ctx*, const YeeDims&, double, const double*, double*, const double*);
 
+// ---------------------------------------------------------------- no allocation
+// Energy partials of scafes_yee_run_noalloc: two slots per workgroup of a
+// launch, 2^17 doubles (1 MiB) per GPU, enough for Yee boxes of up to about
+// 1150 cells per axis.  Each GPU the library runs on holds its own copy
+// until the process ends.  Host code never passes the array's name to a
+// kernel (it is a host-side stand-in); it uses the device address looked up
+// once per device by yee_noalloc_setup.
+constexpr long kScratchSlots = 1L << 17;
+static __device__ __align__(256) double scratch[kScratchSlots];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+int yee_noalloc_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(SCAFES_ERR_ARG, "yee noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return SCAFES_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return check_hip(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[device] = (double*)p;
+    return SCAFES_OK;
+}
+
+static double* yee_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
 }  // namespace scafes
 
 using namespace scafes;
@@ -469,6 +521,27 @@ int scafes_yee_run(scafes_ctx* ctx, int ncell, int dtype, double s, void* f, voi
         return yee_run_t<double>(c, ncell, s, (double*)f, (double*)f2, nsteps, energy_every, energy,
                                  result_in_second);
     return fail(SCAFES_ERR_ARG, "yee: bad dtype");
+}
+
+int scafes_yee_run_noalloc(scafes_ctx* ctx, int ncell, int dtype, double s, void* f, void* f2, int nsteps,
+                           int energy_every, double* energy_dev, int* result_in_second)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || !f) return fail(SCAFES_ERR_ARG, "yee noalloc: null pointer");
+    if (f2 == f) return fail(SCAFES_ERR_ARG, "yee noalloc: second state buffer must differ from the first");
+    if (ncell < 2 || nsteps < 0) return fail(SCAFES_ERR_ARG, "yee noalloc: need ncell >= 2 and nsteps >= 0");
+    if (!(s > 0.0) || s > 1.0 / 1.7320508075688772)
+        return fail(SCAFES_ERR_ARG, "yee noalloc: Courant number must satisfy 0 < s <= 1/sqrt(3)");
+    if (energy_every > 0 && !energy_dev) return fail(SCAFES_ERR_ARG, "yee noalloc: energy output missing");
+    double* part = yee_scratch(c->device);
+    if (!part) return fail(SCAFES_ERR_ARG, "yee noalloc: scratch not set up for this device");
+    if (dtype == SCAFES_F32)
+        return yee_run_t<float>(c, ncell, s, (float*)f, (float*)f2, nsteps, energy_every, nullptr, result_in_second,
+                                part, kScratchSlots, energy_dev);
+    if (dtype == SCAFES_F64)
+        return yee_run_t<double>(c, ncell, s, (double*)f, (double*)f2, nsteps, energy_every, nullptr,
+                                 result_in_second, part, kScratchSlots, energy_dev);
+    return fail(SCAFES_ERR_ARG, "yee noalloc: bad dtype");
 }
 
 int scafes_yee_init_pulse(scafes_ctx* ctx, int ncell, int dtype, void* f, double sigma)
