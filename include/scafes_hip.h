@@ -94,6 +94,11 @@ int scafes_ctx_destroy(scafes_ctx* ctx);
 int scafes_ctx_release_workspace(scafes_ctx* ctx);
 int scafes_ctx_sync(scafes_ctx* ctx);
 int scafes_ctx_info(scafes_ctx* ctx, int* cu_count, char* name, int name_len);
+/* The context's HIP stream (a hipStream_t, default priority, non-blocking)
+ * for kernels the caller launches itself in stream order with the
+ * library's: the generic device update phase of a hipcc-compiled problem
+ * (the role of the reference's per-node hook loop, Problem.hpp:5443-5466). */
+int scafes_ctx_stream(scafes_ctx* ctx, void** stream);
 /* Device-side wall clock on the context's stream (ScaFES::Timer restart/
  * elapsed, src/ScaFES_Timer.hpp:171-214, for GPU work): record event slot
  * 0..7, then read the elapsed milliseconds between two recorded slots. */
@@ -490,7 +495,10 @@ int scafes_emfdtd_vacuum_ok(scafes_ctx* ctx, const scafes_emfdtd_params* p, cons
  * = N tau_i <u_i^N - target_i, L u_i^{N-1}> (exact discrete adjoint for a
  * scalar kappa with fixed tau: the step operator commutes with L).
  * n, off, kappa, tau, order, u, target, J, grad: device arrays.
- * use_graph != 0 captures the launch into a hipGraph and replays it. */
+ * use_graph != 0 captures the launch into a hipGraph the first time it sees
+ * an argument set and replays the instantiated graph on later calls with the
+ * same arguments (the context keeps the last one).  Both paths return
+ * without waiting (stream order); the sizes are checked on the host first. */
 int scafes_batch2d_run(scafes_ctx* ctx, int nprob, const int* n, const long* off,
                        const float* kappa, const float* tau, const int* order, float* u,
                        const float* target, double* J, double* grad, int nsteps, int use_graph);

@@ -99,6 +99,7 @@ def load():
     _sig(L, "scafes_ctx_release_workspace", [vp])
     _sig(L, "scafes_ctx_sync", [vp])
     _sig(L, "scafes_ctx_info", [vp, ip, C.c_char_p, C.c_int])
+    _sig(L, "scafes_ctx_stream", [vp, C.POINTER(vp)])
     _sig(L, "scafes_event_record", [vp, C.c_int])
     _sig(L, "scafes_event_elapsed", [vp, C.c_int, C.c_int, C.POINTER(C.c_float)])
     _sig(L, "scafes_malloc", [vp, C.c_size_t, C.POINTER(vp)])

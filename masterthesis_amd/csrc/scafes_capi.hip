@@ -336,6 +336,7 @@ int scafes_ctx_destroy(scafes_ctx* ctx)
     for (int i = 0; i < 8; ++i)
         if (c->events[i]) (void)hipEventDestroy(c->events[i]);
     if (c->grad_ws) (void)hipFree(c->grad_ws);
+    if (c->batch_exec) (void)hipGraphExecDestroy(c->batch_exec);
     if (c->red_partial) (void)hipFree(c->red_partial);
     if (c->red_result) (void)hipFree(c->red_result);
     if (c->host_result) (void)hipHostFree(c->host_result);
@@ -349,6 +350,14 @@ int scafes_ctx_sync(scafes_ctx* ctx)
     Ctx* c = as_ctx(ctx);
     if (!c) return fail(SCAFES_ERR_ARG, "null context");
     SCAFES_HIP(hipStreamSynchronize(c->stream));
+    return SCAFES_OK;
+}
+
+int scafes_ctx_stream(scafes_ctx* ctx, void** stream)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || !stream) return fail(SCAFES_ERR_ARG, "ctx_stream: null pointer");
+    *stream = (void*)c->stream;
     return SCAFES_OK;
 }
 

@@ -63,6 +63,10 @@ struct Ctx {
     // and scafes_ctx_destroy
     void* grad_ws = nullptr;
     size_t grad_ws_bytes = 0;
+    // batched 2-D ensemble: the instantiated graph of the last captured
+    // launch and the launch arguments it was captured with
+    hipGraphExec_t batch_exec = nullptr;
+    unsigned char batch_key[128] = {};
 };
 
 Ctx* as_ctx(scafes_ctx* c);

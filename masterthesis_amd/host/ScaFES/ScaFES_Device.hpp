@@ -35,7 +35,8 @@ enum class DeviceKernel {
     PENNES,    // TumorHeatEqnFDM form with perfusion and source fields
     YEE,       // leapfrog Yee FDTD, PEC box, vacuum (updateInner = H, updateInner2 = E)
     EMFDTD,    // the EMFDTD example: Yee leapfrog + complete CPML + conductive media, FP64
-    MRI        // the MRIDataPBHEqnFDM example: Pennes with nine per-node material fields, FP64
+    MRI,       // the MRIDataPBHEqnFDM example: Pennes with nine per-node material fields, FP64
+    HOOKS      // the problem's own update hooks compiled as device code (hipcc; no DeviceTraits)
 };
 enum class DevicePrecision { F64 = 0, F32 = 1, F32_KAHAN = 2 };
 // KEEP: the copied old value; KNOWN_FIELD: Dirichlet from G_new; VALUE:
@@ -153,6 +154,14 @@ public:
     DeviceContext& operator=(const DeviceContext&) = delete;
     scafes_ctx* get() const { return mCtx; }
     void sync() const { deviceCheck(scafes_ctx_sync(mCtx), "sync"); }
+    // The context's HIP stream (hipStream_t), for kernels a hipcc-compiled
+    // problem launches itself (the generic device update phase).
+    void* stream() const
+    {
+        void* s = 0;
+        deviceCheck(scafes_ctx_stream(mCtx, &s), "stream");
+        return s;
+    }
 
 private:
     scafes_ctx* mCtx;

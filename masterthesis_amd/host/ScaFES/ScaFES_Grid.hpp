@@ -9,6 +9,7 @@
 #ifndef SCAFES_HOST_GRID_HPP_
 #define SCAFES_HOST_GRID_HPP_
 
+#include "ScaFES_HostDevice.hpp"
 #include "ScaFES_Ntuple.hpp"
 
 namespace ScaFES {
@@ -18,9 +19,9 @@ class Grid {
 public:
     typedef Ntuple<int, DIM> Index;
 
-    Grid() : mFirst(0), mLast(1), mN(2), mX0(0.0), mX1(1.0), mH(1.0), mStyleC(false) {}
+    SCAFES_HD Grid() : mFirst(0), mLast(1), mN(2), mX0(0.0), mX1(1.0), mH(1.0), mStyleC(false) {}
 
-    Grid(const Index& first, const Index& last, Ntuple<double, DIM> x0, Ntuple<double, DIM> x1)
+    SCAFES_HD Grid(const Index& first, const Index& last, Ntuple<double, DIM> x0, Ntuple<double, DIM> x1)
         : mFirst(first), mLast(last), mN(last - first + Index(1)), mX0(x0), mX1(x1), mH(0.0),
           mStyleC(false)
     {
@@ -28,20 +29,20 @@ public:
             for (std::size_t p = 0; p < DIM; ++p) mH[p] = (x1.elem(p) - x0.elem(p)) / (mN.elem(p) - 1);
     }
 
-    const Index& idxNodeFirst() const { return mFirst; }
-    const Index& idxNodeLast() const { return mLast; }
-    const int& idxNodeFirst(const int& p) const { return mFirst.elem(p); }
-    const int& idxNodeLast(const int& p) const { return mLast.elem(p); }
-    const Index& nNodes() const { return mN; }
-    const int& nNodes(const int& p) const { return mN.elem(p); }
-    const Ntuple<double, DIM>& coordNodeFirst() const { return mX0; }
-    const Ntuple<double, DIM>& coordNodeLast() const { return mX1; }
-    const double& gridsize(const int& p) const { return mH.elem(p); }
-    const Ntuple<double, DIM>& gridsize() const { return mH; }
-    bool isNumberedStyleC() const { return mStyleC; }
-    bool valid() const { return mFirst <= mLast; }
+    SCAFES_HD const Index& idxNodeFirst() const { return mFirst; }
+    SCAFES_HD const Index& idxNodeLast() const { return mLast; }
+    SCAFES_HD const int& idxNodeFirst(const int& p) const { return mFirst.elem(p); }
+    SCAFES_HD const int& idxNodeLast(const int& p) const { return mLast.elem(p); }
+    SCAFES_HD const Index& nNodes() const { return mN; }
+    SCAFES_HD const int& nNodes(const int& p) const { return mN.elem(p); }
+    SCAFES_HD const Ntuple<double, DIM>& coordNodeFirst() const { return mX0; }
+    SCAFES_HD const Ntuple<double, DIM>& coordNodeLast() const { return mX1; }
+    SCAFES_HD const double& gridsize(const int& p) const { return mH.elem(p); }
+    SCAFES_HD const Ntuple<double, DIM>& gridsize() const { return mH; }
+    SCAFES_HD bool isNumberedStyleC() const { return mStyleC; }
+    SCAFES_HD bool valid() const { return mFirst <= mLast; }
 
-    Ntuple<double, DIM> coordinates(const Index& idx) const
+    SCAFES_HD Ntuple<double, DIM> coordinates(const Index& idx) const
     {
         Ntuple<double, DIM> c(0.0);
         for (std::size_t p = 0; p < DIM; ++p)
@@ -49,7 +50,7 @@ public:
         return c;
     }
 
-    Index neigh(const Index& idx, const int& dir) const
+    SCAFES_HD Index neigh(const Index& idx, const int& dir) const
     {
         Index nb(idx);
         const int p = dir / 2;
@@ -62,20 +63,20 @@ public:
         return nb;
     }
 
-    bool inside(const Index& idx) const { return mFirst <= idx && idx <= mLast; }
+    SCAFES_HD bool inside(const Index& idx) const { return mFirst <= idx && idx <= mLast; }
 
-    void idxNodeTuple2ScalarStyleFortran(unsigned long& s, const Index& idx) const
+    SCAFES_HD void idxNodeTuple2ScalarStyleFortran(unsigned long& s, const Index& idx) const
     {
         s = idx.elem(DIM - 1) - mFirst.elem(DIM - 1);
         for (int p = (int)DIM - 2; p >= 0; --p) s = s * mN.elem(p) + idx.elem(p) - mFirst.elem(p);
     }
-    unsigned long idxNodeTuple2ScalarStyleFortran(const Index& idx) const
+    SCAFES_HD unsigned long idxNodeTuple2ScalarStyleFortran(const Index& idx) const
     {
         unsigned long s = 0;
         idxNodeTuple2ScalarStyleFortran(s, idx);
         return s;
     }
-    void idxNodeScalar2TupleStyleFortran(Index& idx, const unsigned long& s) const
+    SCAFES_HD void idxNodeScalar2TupleStyleFortran(Index& idx, const unsigned long& s) const
     {
         unsigned long t = s;
         for (std::size_t p = 0; p < DIM; ++p) {
@@ -83,18 +84,18 @@ public:
             t /= mN.elem(p);
         }
     }
-    Index idxNodeScalar2TupleStyleFortran(const unsigned long& s) const
+    SCAFES_HD Index idxNodeScalar2TupleStyleFortran(const unsigned long& s) const
     {
         Index idx;
         idxNodeScalar2TupleStyleFortran(idx, s);
         return idx;
     }
-    void idxNodeTuple2ScalarStyleC(unsigned long& s, const Index& idx) const
+    SCAFES_HD void idxNodeTuple2ScalarStyleC(unsigned long& s, const Index& idx) const
     {
         s = idx.elem(0) - mFirst.elem(0);
         for (std::size_t p = 1; p < DIM; ++p) s = s * mN.elem(p) + idx.elem(p) - mFirst.elem(p);
     }
-    void idxNodeScalar2TupleStyleC(Index& idx, const unsigned long& s) const
+    SCAFES_HD void idxNodeScalar2TupleStyleC(Index& idx, const unsigned long& s) const
     {
         unsigned long t = s;
         for (std::size_t p = DIM - 1; p > 0; --p) {
@@ -104,12 +105,12 @@ public:
         idx[0] = (int)t + mFirst.elem(0);
     }
     // Default numbering (Fortran style unless switched).
-    void idxNodeTuple2Scalar(unsigned long& s, const Index& idx) const
+    SCAFES_HD void idxNodeTuple2Scalar(unsigned long& s, const Index& idx) const
     {
         if (mStyleC) idxNodeTuple2ScalarStyleC(s, idx);
         else idxNodeTuple2ScalarStyleFortran(s, idx);
     }
-    void idxNodeScalar2Tuple(Index& idx, const unsigned long& s) const
+    SCAFES_HD void idxNodeScalar2Tuple(Index& idx, const unsigned long& s) const
     {
         if (mStyleC) idxNodeScalar2TupleStyleC(idx, s);
         else idxNodeScalar2TupleStyleFortran(idx, s);

@@ -33,6 +33,7 @@ public:
     };
     typedef Ntuple<int, DIM> Index;
 
+    SCAFES_HD GridGlobal() {}
     explicit GridGlobal(const Parameters& params)
     {
         if ((int)DIM != params.dim()) {
@@ -51,9 +52,9 @@ public:
         mPartition = GridSub<DIM>(first, last, mDomain);
     }
 
-    const Grid<DIM>& discreteDomain() const { return mDomain; }
-    const GridSub<DIM>& partition(const int& /*rank*/) const { return mPartition; }
-    int nPartitions() const { return 1; }
+    SCAFES_HD const Grid<DIM>& discreteDomain() const { return mDomain; }
+    SCAFES_HD const GridSub<DIM>& partition(const int& /*rank*/) const { return mPartition; }
+    SCAFES_HD int nPartitions() const { return 1; }
     std::vector<int> neighbourPartitions(const int& /*rank*/) const { return std::vector<int>(); }
 
 private:

@@ -12,28 +12,28 @@ class GridSub {
 public:
     typedef Ntuple<int, DIM> Index;
 
-    GridSub() : mFirst(0), mLast(-1), mBase() {}
-    GridSub(const Index& first, const Index& last, const Grid<DIM>& base)
+    SCAFES_HD GridSub() : mFirst(0), mLast(-1), mBase() {}
+    SCAFES_HD GridSub(const Index& first, const Index& last, const Grid<DIM>& base)
         : mFirst(first), mLast(last), mBase(base)
     {
     }
 
-    const Index& idxNodeFirstSub() const { return mFirst; }
-    const Index& idxNodeLastSub() const { return mLast; }
-    Index nNodesSub() const
+    SCAFES_HD const Index& idxNodeFirstSub() const { return mFirst; }
+    SCAFES_HD const Index& idxNodeLastSub() const { return mLast; }
+    SCAFES_HD Index nNodesSub() const
     {
         Index n;
         for (std::size_t p = 0; p < DIM; ++p) n[p] = mLast.elem(p) - mFirst.elem(p) + 1;
         return n;
     }
-    int nNodesTotal() const
+    SCAFES_HD int nNodesTotal() const
     {
         for (std::size_t p = 0; p < DIM; ++p)
             if (mLast.elem(p) < mFirst.elem(p)) return 0;
         return nNodesSub().size();
     }
-    const Grid<DIM>& grid() const { return mBase; }
-    bool inside(const Index& idx) const { return mFirst <= idx && idx <= mLast; }
+    SCAFES_HD const Grid<DIM>& grid() const { return mBase; }
+    SCAFES_HD bool inside(const Index& idx) const { return mFirst <= idx && idx <= mLast; }
 
     // Forward iterator over the nodes in Fortran order (x fastest).
     class iterator {

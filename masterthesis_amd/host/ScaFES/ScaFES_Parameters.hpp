@@ -64,6 +64,7 @@ public:
             else if (key == "asynchronMode") mAsynchronMode = toBool(val);
             else if (key == "computeGradients") mComputeGradients = toBool(val);
             else if (key == "device") mDevice = toInt(key, val);
+            else if (key == "checkDeviceTraits") mCheckDeviceTraits = toBool(val);
             else if (key == "configfile") mNameConfigFile = val;
             else if (key == "outputfile") { mNameDataFile = val; mWriteDataFile = true; }
             else if (key == "initfile") { mNameInitFile = val; mReadInitFile = true; }
@@ -118,6 +119,11 @@ public:
     int rank() const { return 0; }
     int nProcesses() const { return 1; }
     int device() const { return mDevice; }
+    // --checkDeviceTraits (default on): after the first time step, compare the
+    // declared kernel family's result with the problem's own hooks evaluated
+    // on the host at sampled nodes (no counterpart in the reference, which
+    // runs only the hooks).
+    bool checkDeviceTraits() const { return mCheckDeviceTraits; }
     // --configfile (Parameters.hpp:516, 928-935): read by problem classes.
     const std::string& nameConfigFile() const { return mNameConfigFile; }
     // --outputfile / --initfile / --initKnownDfs (Parameters.hpp:514-520,
@@ -209,6 +215,7 @@ private:
     int mCheckConvFirstAtIter, mCheckConvAtEveryNIter, mNsnapshots, mNlayersAtBorder;
     bool mEnabledAdolc, mAsynchronMode, mComputeGradients;
     int mRankOutput, mOutputLevelMax, mOutputLevelCurr, mDevice;
+    bool mCheckDeviceTraits = true;
     std::string mNameConfigFile;
     std::string mNameDataFile = "EMPTYDATAFILE", mNameInitFile = "EMPTYINITFILE";
     bool mWriteDataFile = false, mReadInitFile = false, mInitKnownDfs = false;

@@ -9,9 +9,12 @@
 //                      [n>0 : update phase (+ update2 phase with leapfrog)]
 //   -> compErrLinf -> checkConv -> swap old/new -> times += tau.
 // The init and eval hooks run on the host (once, or every step when the
-// known fields depend on t).  The update phase runs on the GPU through the
-// C ABI, using the kernel family the problem declares through
-// ScaFES::DeviceTraits; there is no CPU update path.
+// known fields depend on t).  The update phase runs on the GPU: through the
+// C ABI with the kernel family the problem declares through
+// ScaFES::DeviceTraits, or, in a program compiled by hipcc with the problem
+// header between SCAFES_DEVICE_HOOKS_BEGIN / _END, as the problem's own
+// update hooks compiled for the device (ScaFES_DeviceHooks.hpp).  There is
+// no CPU update path.
 #ifndef SCAFES_HOST_PROBLEM_HPP_
 #define SCAFES_HOST_PROBLEM_HPP_
 
@@ -21,6 +24,7 @@
 #include <iostream>
 #include <map>
 #include <memory>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -28,8 +32,12 @@
 #include "ScaFES_DataField.hpp"
 #include "ScaFES_Device.hpp"
 #include "ScaFES_GridGlobal.hpp"
+#include "ScaFES_HostDevice.hpp"
 #include "ScaFES_Parameters.hpp"
 #include "ScaFES_Timer.hpp"
+#if defined(__HIPCC__)
+#include "ScaFES_DeviceHooks.hpp"
+#endif
 
 namespace ScaFES {
 
@@ -53,6 +61,10 @@ public:
           mDefaultValue(defaultValue), mWriteToFile(writeToFile), mComputeError(computeError),
           mParamsCurr(geomparamsInit), mCheckConvergence(checkConvergence)
     {
+        // plain copies the hook accessors read (also from a device copy of this object)
+        mDomainC = mGG.discreteDomain();
+        mTauC = mParams.tau();
+        mTimeStartC = mParams.timeIntervalStart();
         // The four update hooks are required even if unused (reference :2849-2862, :2895-2908).
         (void)&OWNPRBLM::template updateInner<CT>;
         (void)&OWNPRBLM::template updateBorder<CT>;
@@ -143,19 +155,42 @@ public:
     const Parameters& params() const { return mParams; }
     const GridGlobal<DIM>& globalGrid() const { return mGG; }
     const int& nTimesteps() const { return mParams.nTimesteps(); }
-    const double& tau() const { return mParams.tau(); }
-    const double& gridsize(const int& p) const { return mGG.discreteDomain().gridsize(p); }
-    const ScaFES_IntNtuple& nNodes() const { return mGG.discreteDomain().nNodes(); }
-    const int& nNodes(const int& p) const { return mGG.discreteDomain().nNodes(p); }
-    Ntuple<double, DIM> coordinates(const ScaFES_IntNtuple& idx) const { return mGG.discreteDomain().coordinates(idx); }
-    double time(const int& timestep) const { return mParams.timeIntervalStart() + timestep * mParams.tau(); }
-    ScaFES_IntNtuple connect(const ScaFES_IntNtuple& idx, const int& dir) { return mGG.discreteDomain().neigh(idx, dir); }
-    const short& kind(const ScaFES_IntNtuple& idx) const { return mKind[idx]; }
+    // The accessors update hooks call are host+device functions reading plain
+    // members (SCAFES_HD; plain functions under g++).
+    SCAFES_HD const double& tau() const { return mTauC; }
+    SCAFES_HD const double& gridsize(const int& p) const { return mDomainC.gridsize(p); }
+    SCAFES_HD const ScaFES_IntNtuple& nNodes() const { return mDomainC.nNodes(); }
+    SCAFES_HD const int& nNodes(const int& p) const { return mDomainC.nNodes(p); }
+    SCAFES_HD Ntuple<double, DIM> coordinates(const ScaFES_IntNtuple& idx) const { return mDomainC.coordinates(idx); }
+    SCAFES_HD double time(const int& timestep) const { return mTimeStartC + timestep * mTauC; }
+    SCAFES_HD ScaFES_IntNtuple connect(const ScaFES_IntNtuple& idx, const int& dir) { return mDomainC.neigh(idx, dir); }
+    SCAFES_HD const short& kind(const ScaFES_IntNtuple& idx) const
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return hookState()->kind[mDomainC.idxNodeTuple2ScalarStyleFortran(idx)];
+#else
+        return mKind[idx];
+#endif
+    }
     const DataField<short, DIM>& kind() const { return mKind; }
     const std::vector<CT>& vectParamsCurr() const { return mParamsCurr; }
-    const CT& knownDf(const int& idx, const ScaFES_IntNtuple& node) const { return mVectKnownDfsDomNew[idx](node); }
-    const CT& knownDfNew(const int& idx, const ScaFES_IntNtuple& node) const { return mVectKnownDfsDomNew[idx](node); }
-    const CT& knownDfOld(const int& idx, const ScaFES_IntNtuple& node) const { return mVectKnownDfsDomOld[idx](node); }
+    SCAFES_HD const CT& knownDf(const int& idx, const ScaFES_IntNtuple& node) const { return knownDfNew(idx, node); }
+    SCAFES_HD const CT& knownDfNew(const int& idx, const ScaFES_IntNtuple& node) const
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return hookState()->knownNew[idx](node).v;
+#else
+        return mVectKnownDfsDomNew[idx](node);
+#endif
+    }
+    SCAFES_HD const CT& knownDfOld(const int& idx, const ScaFES_IntNtuple& node) const
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return hookState()->knownOld[idx](node).v;
+#else
+        return mVectKnownDfsDomOld[idx](node);
+#endif
+    }
     const ScaFES_VectDf<CT>& vectSol() const { return mVectKnownDfsDomNew; }
     const ScaFES_VectDf<CT>& vectKnownDfsDomNew() const { return mVectKnownDfsDomNew; }
     const ScaFES_VectDf<CT>& vectKnownDfsDomOld() const { return mVectKnownDfsDomOld; }
@@ -184,10 +219,14 @@ public:
     // -------------------------------------------------------------- driver
     void iterateOverTime()
     {
+#if !defined(__HIPCC__)
         if (!DeviceTraits<OWNPRBLM>::available)
             throw std::runtime_error(
-                "ScaFES: this problem declares no device update kernel (specialise "
-                "ScaFES::DeviceTraits for it); the engine has no CPU update path.");
+                "ScaFES: this problem declares no device update kernel (specialise ScaFES::DeviceTraits for "
+                "it) and this program was not compiled by hipcc, so its hooks cannot run on the GPU (compile "
+                "it with hipcc, the problem header between SCAFES_DEVICE_HOOKS_BEGIN and SCAFES_DEVICE_HOOKS_END); "
+                "the engine has no CPU update path.");
+#endif
         // The reference records an ADOL-C tape of the hooks and propagates
         // the derivatives w.r.t. geomparamsInit alongside the iterate
         // (Problem.hpp:1391, :4598-4660).  This engine has no tape; its
@@ -197,7 +236,11 @@ public:
                 "ScaFES: --computeGradients (ADOL-C tapes of the hooks) is not available in this engine; "
                 "use the field gradients of the C ABI (scafes_pennes_gradient, scafes_heat7var_gradient, "
                 "scafes_yee_eps_gradient).");
-        DeviceTraits<OWNPRBLM>::describe(static_cast<const OWNPRBLM&>(*this), mDev);
+        mDev = DeviceUpdate();
+        if (DeviceTraits<OWNPRBLM>::available)
+            DeviceTraits<OWNPRBLM>::describe(static_cast<const OWNPRBLM&>(*this), mDev);
+        else
+            mDev.kernel = DeviceKernel::HOOKS;  // the problem's own hooks on the device
         validateDevice();
         Timer timerIterate;
         if (out()) std::cout << mParams.getPrefix() << " * Iterate over time..." << std::endl;
@@ -209,6 +252,7 @@ public:
             evalKnownDfs(n);
             if (0 < n) {
                 evalUpdatePhase(n);
+                if (n == 1) checkTraitsAgainstHooks(n);
             } else {
                 Timer t;
                 resetValsAtNodesOfUnknownDfs();
@@ -256,6 +300,14 @@ public:
         mEmSink = DeviceBuffer();
         mMat.clear();
         mRef.clear();
+        mHookOld.clear();
+        mHookNew.clear();
+        mHookKnownOld.clear();
+        mHookKnownNew.clear();
+        mHookKind = DeviceBuffer();
+        mHookStateBuf = DeviceBuffer();
+        mHookSelf = DeviceBuffer();
+        mDevHookState = 0;
         mCtx.reset();
     }
 
@@ -574,6 +626,12 @@ private:
                     throw std::runtime_error("DeviceUpdate: MRI material fields have stencil width 0");
             }
             break;
+        case DeviceKernel::HOOKS:
+            if (mNames.size() > kMaxDeviceFields)
+                throw std::runtime_error("device hooks: at most 32 data fields");
+            if (mDev.precision != DevicePrecision::F64)
+                throw std::runtime_error("device hooks: the problem's own hooks run in its CT");
+            break;
         default:
             throw std::runtime_error("DeviceUpdate: no kernel selected");
         }
@@ -624,6 +682,10 @@ private:
     void deviceSetup()
     {
         mCtx.reset(new DeviceContext(mParams.device()));
+        if (mDev.kernel == DeviceKernel::HOOKS) {
+            hooksSetup();
+            return;
+        }
         const bool fl = f32();
         if (mDev.kernel == DeviceKernel::EMFDTD) {
             emSetup();
@@ -700,6 +762,13 @@ private:
     void uploadKnown(const int& n)
     {
         if (!mCtx) return;
+        if (mDev.kernel == DeviceKernel::HOOKS) {
+            // every lump with memory, as evalKnownDfs just rewrote them on the host
+            for (std::size_t ii = 0; ii < mNames.size(); ++ii)
+                if (mHookKnownNew[ii].get())
+                    mHookKnownNew[ii].upload(mVectKnownDfsDomNew[ii].elemData(), nodesOf((int)ii) * sizeof(CT));
+            return;
+        }
         const bool fl = f32();
         auto up = [&](int ii, DeviceBuffer& d, bool old, bool asFloat) {
             const std::size_t nE = nodesOf(ii);
@@ -905,6 +974,21 @@ private:
             ++mEmSteps;
             break;
         }
+        case DeviceKernel::HOOKS: {
+            hooksPhase(n, 1);
+            if (mUseLeapfrog) {
+                // reference :2603-2619: swap old/new known and unknown fields
+                // but not their times, then the update2 phase
+                swapKeepingTimes(mVectKnownDfsDomNew, mVectKnownDfsDomOld);
+                std::swap(mHookKnownNew, mHookKnownOld);
+                swapKeepingTimes(mVectUnknownDfsDomNew, mVectUnknownDfsDomOld);
+                std::swap(mHookNew, mHookOld);
+                for (std::size_t u = 0; u < nUnknown(); ++u)
+                    mVectUnknownDfsDomNew[u].time() = mVectUnknownDfsDomOld[u].time() + mParams.tau();
+                hooksPhase(n, 2);
+            }
+            break;
+        }
         case DeviceKernel::YEE: {
             // updateInner (H) and, after the leapfrog swap, updateInner2 (E):
             // one fused leapfrog step on the device (reference :2603-2619)
@@ -927,6 +1011,186 @@ private:
         mWallClockTimes["update"] += t.elapsed();
     }
 
+    // ---------------------------------------------------- DeviceTraits check
+    // After the first step: the declared kernel family's new state of the
+    // updated field against the problem's own hooks run on the host (TT = CT)
+    // from the same old state, at sampled nodes (every combination of the
+    // per-axis indices 0, 1, 2, n/4, n/2, 3n/4, n-3, n-2, n-1, and about
+    // 2048 nodes spread over the grid).  A DeviceTraits that names the wrong
+    // family, coefficient, field or border rule throws here instead of
+    // silently running another update.  Tolerance relative to the largest
+    // sampled value: 1e-12 for FP64 kernels (only the operation order may
+    // differ), 1e-5 for the FP32 ones.  Not for leapfrog problems (the
+    // second phase needs the first one's neighbourhood) or device-evaluated
+    // known fields (their host copies are not current).
+    void checkTraitsAgainstHooks(const int& n)
+    {
+        if (!mParams.checkDeviceTraits()) return;
+        if (mDev.kernel == DeviceKernel::HOOKS || mUseLeapfrog || mDev.knownGen != DeviceKnownGen::NONE) return;
+        const int u = unknownSlot(mDev.unknown);
+        if (u < 0) return;
+        downloadUnknownTo((std::size_t)u, mVectUnknownDfsDomNew[u]);
+        // the hooks write into copies of the old lumps (new <- old first, as
+        // the reference's update phase does)
+        std::vector<std::vector<CT>> scratch(nUnknown());
+        ScaFES_VectDf<CT> vNew(mVectUnknownDfsDomNew);
+        for (std::size_t q = 0; q < nUnknown(); ++q) {
+            const DataField<CT, DIM>& o = mVectUnknownDfsDomOld[q];
+            scratch[q].assign(o.elemData(), o.elemData() + o.nElemsAllocated());
+            vNew[q].setElemData(scratch[q].data());
+        }
+        std::vector<ScaFES_IntNtuple> nodes;
+        int axis[DIM][9];
+        int nax[DIM];
+        for (std::size_t p = 0; p < DIM; ++p) {
+            const int m = nNodes((int)p), f = mDomainC.idxNodeFirst((int)p);
+            const int cand[9] = {0, 1, 2, m / 4, m / 2, (3 * m) / 4, m - 3, m - 2, m - 1};
+            nax[p] = 0;
+            for (int c = 0; c < 9; ++c) {
+                if (cand[c] < 0 || cand[c] >= m) continue;
+                bool dup = false;
+                for (int e = 0; e < nax[p]; ++e) dup = dup || axis[p][e] == f + cand[c];
+                if (!dup) axis[p][nax[p]++] = f + cand[c];
+            }
+        }
+        int pick[DIM] = {};
+        for (;;) {
+            ScaFES_IntNtuple x;
+            for (std::size_t p = 0; p < DIM; ++p) x[p] = axis[p][pick[p]];
+            nodes.push_back(x);
+            std::size_t p = 0;
+            while (p < DIM && ++pick[p] == nax[p]) pick[p++] = 0;
+            if (p == DIM) break;
+        }
+        const unsigned long total = (unsigned long)nNodes().size();
+        const unsigned long stride = total > 2048 ? total / 2048 + 1 : 1;
+        for (unsigned long t = 0; t < total; t += stride) nodes.push_back(mDomainC.idxNodeScalar2TupleStyleFortran(t));
+        OWNPRBLM* self = static_cast<OWNPRBLM*>(this);
+        const DataField<CT, DIM>& dev = mVectUnknownDfsDomNew[u];
+        double scale = 0.0, worst = -1.0, hw = 0.0, dw = 0.0;
+        ScaFES_IntNtuple at;
+        for (std::size_t k = 0; k < nodes.size(); ++k) {
+            const ScaFES_IntNtuple& x = nodes[k];
+            if (mKind[x] & GridGlobal<DIM>::SCAFES_GLOBAL_INNER_NODE)
+                self->template updateInner<CT>(vNew, mVectUnknownDfsDomOld, x, n);
+            else
+                self->template updateBorder<CT>(vNew, mVectUnknownDfsDomOld, x, n);
+            const double h = (double)vNew[u](x), d = (double)dev(x);
+            scale = std::max(scale, std::fabs(h));
+            const double e = std::fabs(h - d);
+            if (!(e <= worst)) {  // NaN counts as the worst
+                worst = e;
+                hw = h;
+                dw = d;
+                at = x;
+            }
+        }
+        const double tol = (f32() ? 1e-5 : 1e-12) * std::max(scale, 1e-300);
+        if (!(worst <= tol)) {
+            std::ostringstream msg;
+            msg << std::setprecision(17) << "ScaFES: the DeviceTraits kernel family disagrees with the problem's own "
+                << "update hooks after the first step at node " << at << ": device " << dw << ", hooks " << hw
+                << " (tolerance " << tol << "); fix the DeviceTraits specialisation, or pass "
+                << "--checkDeviceTraits=0 to skip this check.";
+            throw std::runtime_error(msg.str());
+        }
+    }
+
+    // ---------------------------------------------------- generic device update
+    // The problem's own updateInner/updateBorder(/2), instantiated with
+    // TT = DevReal<CT> and run by one GPU thread per node
+    // (ScaFES_DeviceHooks.hpp).  Device state: one memAll array per unknown
+    // field (old and new), per known lump (known fields and the reference
+    // solutions of compErr fields; old and new), the node kinds, the views
+    // of one launch (HookState) and a byte copy of this problem object.
+#if defined(__HIPCC__)
+    SCAFES_HD const HookState<CT, DIM>* hookState() const
+    {
+        return static_cast<const HookState<CT, DIM>*>(mDevHookState);
+    }
+    void hooksSetup()
+    {
+        const std::size_t es = sizeof(CT);
+        mHookOld.clear();
+        mHookNew.clear();
+        for (std::size_t u = 0; u < nUnknown(); ++u) {
+            const std::size_t n = nodesOf(mUnknownIndex[u]);
+            mHookOld.push_back(DeviceBuffer(*mCtx, n * es));
+            mHookNew.push_back(DeviceBuffer(*mCtx, n * es));
+            mHookOld.back().upload(mVectUnknownDfsDomOld[u].elemData(), n * es);
+            mHookNew.back().upload(mVectUnknownDfsDomNew[u].elemData(), n * es);
+        }
+        mHookKnownOld.clear();
+        mHookKnownNew.clear();
+        for (std::size_t ii = 0; ii < mNames.size(); ++ii) {
+            mHookKnownOld.push_back(DeviceBuffer());
+            mHookKnownNew.push_back(DeviceBuffer());
+            if (!mVectKnownDfsDomNew[ii].elemData()) continue;
+            const std::size_t n = nodesOf((int)ii);
+            mHookKnownOld.back() = DeviceBuffer(*mCtx, n * es);
+            mHookKnownNew.back() = DeviceBuffer(*mCtx, n * es);
+            mHookKnownOld.back().upload(mVectKnownDfsDomOld[ii].elemData(), n * es);
+            mHookKnownNew.back().upload(mVectKnownDfsDomNew[ii].elemData(), n * es);
+        }
+        mHookKind = DeviceBuffer(*mCtx, mLumpKind.size() * sizeof(short));
+        mHookKind.upload(mLumpKind.data(), mLumpKind.size() * sizeof(short));
+        mHookStateBuf = DeviceBuffer(*mCtx, sizeof(HookState<CT, DIM>));
+        mDevHookState = mHookStateBuf.get();
+        // the device copy of this object: its hook accessors read plain
+        // members and, on the device, the HookState above
+        mHookSelf = DeviceBuffer(*mCtx, sizeof(OWNPRBLM));
+        mHookSelf.upload(static_cast<OWNPRBLM*>(this), sizeof(OWNPRBLM));
+    }
+    // The views of the current old/new iterates (pointers, layouts, times).
+    void hooksUploadState()
+    {
+        typedef DataField<DevReal<CT>, DIM> View;
+        std::unique_ptr<HookState<CT, DIM>> st(new HookState<CT, DIM>());
+        st->gg = mGG;
+        st->domain = mDomainC;
+        st->nNodes = (long)nNodes().size();
+        st->kind = static_cast<const short*>(mHookKind.get());
+        const GridGlobal<DIM>* ggDev = reinterpret_cast<const GridGlobal<DIM>*>(
+            static_cast<char*>(mHookStateBuf.get()) + (reinterpret_cast<char*>(&st->gg) - reinterpret_cast<char*>(st.get())));
+        for (std::size_t u = 0; u < nUnknown(); ++u) {
+            const int ii = mUnknownIndex[u];
+            const long n = (long)nodesOf(ii);
+            st->vNew.push_back(View(static_cast<CT*>(mHookNew[u].get()), mMemAll[ii], mStencilWidth[ii], 1, n,
+                                    mVectUnknownDfsDomNew[u].time(), ggDev));
+            st->vOld.push_back(View(static_cast<CT*>(mHookOld[u].get()), mMemAll[ii], mStencilWidth[ii], 1, n,
+                                    mVectUnknownDfsDomOld[u].time(), ggDev));
+        }
+        for (std::size_t ii = 0; ii < mNames.size(); ++ii) {
+            if (!mHookKnownNew[ii].get()) continue;
+            const int sw = mIsKnownDf[ii] ? mStencilWidth[ii] : 0;
+            const long n = (long)nodesOf((int)ii);
+            st->knownNew[ii] = View(static_cast<CT*>(mHookKnownNew[ii].get()), mMemAll[ii], sw, 1, n,
+                                    mVectKnownDfsDomNew[ii].time(), ggDev);
+            st->knownOld[ii] = View(static_cast<CT*>(mHookKnownOld[ii].get()), mMemAll[ii], sw, 1, n,
+                                    mVectKnownDfsDomOld[ii].time(), ggDev);
+        }
+        mHookStateBuf.upload(st.get(), sizeof(HookState<CT, DIM>));
+    }
+    // One update phase (1: updateInner/updateBorder, 2: the leapfrog's
+    // updateInner2/updateBorder2): new <- old for every unknown field
+    // (setVectDfDepToDfIndep, reference :6226-6256), then the hooks.
+    void hooksPhase(int n, int phase)
+    {
+        scafes_ctx* c = mCtx->get();
+        for (std::size_t u = 0; u < nUnknown(); ++u)
+            deviceCheck(scafes_memcpy_d2d(c, mHookNew[u].get(), mHookOld[u].get(), mHookOld[u].bytes()),
+                        "old to new copy");
+        hooksUploadState();
+        if (launch_hooks<OWNPRBLM, CT, DIM>(mCtx->stream(), static_cast<OWNPRBLM*>(mHookSelf.get()),
+                                            static_cast<HookState<CT, DIM>*>(mHookStateBuf.get()),
+                                            (long)nNodes().size(), n, phase))
+            throw std::runtime_error("ScaFES device: launch of the problem's update hooks failed");
+    }
+#else
+    void hooksSetup() { throw std::logic_error("device hooks need a hipcc-compiled program"); }
+    void hooksPhase(int, int) { throw std::logic_error("device hooks need a hipcc-compiled program"); }
+#endif
+
     // ---------------------------------------------------- error / convergence
     void compErrOfUnknownDfs(const int& /*n*/)
     {
@@ -936,7 +1200,12 @@ private:
             if (!mComputeError[ii]) continue;
             double e = 0.0;
             const scafes_geom g = geom();
-            if (!f32() && mDev.kernel != DeviceKernel::YEE && ii == mDev.unknown) {
+            if (mDev.kernel == DeviceKernel::HOOKS) {
+                deviceCheck(scafes_reduce(mCtx->get(), &g, sizeof(CT) == 8 ? SCAFES_F64 : SCAFES_F32,
+                                          mHookNew[u].get(), mStencilWidth[ii], mHookKnownNew[ii].get(),
+                                          mStencilWidth[ii], 0, 0, &e),
+                            "compErrLinf");
+            } else if (!f32() && mDev.kernel != DeviceKernel::YEE && ii == mDev.unknown) {
                 deviceCheck(scafes_reduce(mCtx->get(), &g, SCAFES_F64, mUNew.get(), mStencilWidth[ii], mRef[u].get(),
                                           mStencilWidth[ii], 0, 0, &e),
                             "compErrLinf");
@@ -962,13 +1231,20 @@ private:
             if (!(mCheckConvergence[ii] && n >= mParams.checkConvFirstAtIter() &&
                   (n - mParams.checkConvFirstAtIter()) % mParams.checkConvAtEveryNIter() == 0))
                 continue;
-            if (mDev.kernel == DeviceKernel::YEE || ii != mDev.unknown)
-                throw std::runtime_error("checkConvergence is supported for the updated field of a heat/Pennes kernel");
             double d = 0.0;
             const scafes_geom g = geom();
-            deviceCheck(scafes_reduce(mCtx->get(), &g, f32() ? SCAFES_F32 : SCAFES_F64, mUNew.get(), mStencilWidth[ii],
-                                      mUOld.get(), mStencilWidth[ii], 0, 0, &d),
-                        "checkConv");
+            if (mDev.kernel == DeviceKernel::HOOKS) {
+                deviceCheck(scafes_reduce(mCtx->get(), &g, sizeof(CT) == 8 ? SCAFES_F64 : SCAFES_F32, mHookNew[u].get(),
+                                          mStencilWidth[ii], mHookOld[u].get(), mStencilWidth[ii], 0, 0, &d),
+                            "checkConv");
+            } else {
+                if (mDev.kernel == DeviceKernel::YEE || ii != mDev.unknown)
+                    throw std::runtime_error(
+                        "checkConvergence is supported for the updated field of a heat/Pennes kernel");
+                deviceCheck(scafes_reduce(mCtx->get(), &g, f32() ? SCAFES_F32 : SCAFES_F64, mUNew.get(),
+                                          mStencilWidth[ii], mUOld.get(), mStencilWidth[ii], 0, 0, &d),
+                            "checkConv");
+            }
             cont.push_back(d > mParams.threshold());
             if (out())
                 std::cout << mParams.getPrefix() << " * Is (" << mVectUnknownDfsDomNew[u].name() << ", "
@@ -986,6 +1262,7 @@ private:
     {
         for (std::size_t ii = 0; ii < mNames.size(); ++ii)
             mVectKnownDfsDomNew[ii].swapPointerToMemoryWith(mVectKnownDfsDomOld[ii]);
+        std::swap(mHookKnownNew, mHookKnownOld);
     }
     void swapPointersOfUnknownDfs()
     {
@@ -993,6 +1270,19 @@ private:
             mVectUnknownDfsDomNew[u].swapPointerToMemoryWith(mVectUnknownDfsDomOld[u]);
         mUOld.swap(mUNew);
         mCOld.swap(mCNew);
+        std::swap(mHookNew, mHookOld);
+    }
+    // Pointer swap that leaves each view's time where it was (the
+    // reference's swapPointers + swapTimes pair, :2604-2616).
+    static void swapKeepingTimes(ScaFES_VectDf<CT>& a, ScaFES_VectDf<CT>& b)
+    {
+        for (std::size_t k = 0; k < a.size(); ++k) {
+            if (!a[k].elemData() || !b[k].elemData()) continue;
+            const CT ta = a[k].time(), tb = b[k].time();
+            a[k].swapPointerToMemoryWith(b[k]);
+            a[k].time() = ta;
+            b[k].time() = tb;
+        }
     }
     void updateTimesOfKnownDfs()
     {
@@ -1014,6 +1304,10 @@ private:
         const int ii = mUnknownIndex[u];
         std::vector<CT>& lump = df.elemData() == mLumpUnknownOld[u].data() ? mLumpUnknownOld[u] : mLumpUnknownNew[u];
         const bool isNew = (&df == &mVectUnknownDfsDomNew[u]);
+        if (mDev.kernel == DeviceKernel::HOOKS) {
+            (isNew ? mHookNew[u] : mHookOld[u]).download(lump.data(), nodesOf(ii) * sizeof(CT));
+            return;
+        }
         if (mDev.kernel == DeviceKernel::EMFDTD) {
             const std::size_t nn = (std::size_t)nNodes().size();
             int q = -1;
@@ -1123,6 +1417,13 @@ private:
     scafes_emfdtd_params mEmP;
     int mEmSteps = 0;
     std::vector<DeviceBuffer> mRef, mMat;
+    // generic device update (HOOKS)
+    std::vector<DeviceBuffer> mHookOld, mHookNew, mHookKnownOld, mHookKnownNew;
+    DeviceBuffer mHookKind, mHookStateBuf, mHookSelf;
+    const void* mDevHookState = 0;  // device address of the HookState the device copy reads
+    // plain copies for the hook accessors (readable from the device copy)
+    Grid<DIM> mDomainC;
+    double mTauC = 0.0, mTimeStartC = 0.0;
     DataFile<CT, DIM> mCommonFile, mInitFile;
 };
 

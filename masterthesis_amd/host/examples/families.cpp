@@ -4,7 +4,8 @@
 // src/ScaFES_Problem.hpp:5397-5476), and a DeviceTraits specialisation names
 // the kernel family the update phase runs as on the GPU.
 //
-// Build variants (-DEXAMPLE_FAMILY=...):
+// Build variants (-DEXAMPLE_FAMILY=...; -DEXAMPLE_NO_TRAITS with hipcc: no
+// DeviceTraits, the generic device update runs the hooks below):
 //   HEATVAR  conservative variable-kappa heat, kappa a known field with the
 //            unknown's stencil width, zero Dirichlet border
 //            (-DEXAMPLE_PRECISION=F64 or F32_KAHAN)
@@ -20,6 +21,7 @@
 #include <cstdlib>
 #include <iomanip>
 #include <iostream>
+#include <stdexcept>
 
 #include "ScaFES.hpp"
 #include "example_util.hpp"
@@ -34,6 +36,10 @@
 #ifndef EXAMPLE_PRECISION
 #define EXAMPLE_PRECISION F64
 #endif
+
+// The problem classes are host+device code in a hipcc build (the
+// EXAMPLE_NO_TRAITS variants run their own hooks on the GPU).
+SCAFES_DEVICE_HOOKS_BEGIN
 
 template <typename CT, std::size_t DIM>
 using VDf = std::vector<ScaFES::DataField<CT, DIM>>;
@@ -212,7 +218,12 @@ public:
     }
 };
 
+SCAFES_DEVICE_HOOKS_END
+
 // ------------------------------------------------------------ device traits
+// (left out with -DEXAMPLE_NO_TRAITS: the update phase then runs the classes'
+// own hooks, which needs a hipcc build)
+#ifndef EXAMPLE_NO_TRAITS
 namespace ScaFES {
 template <typename CT, std::size_t DIM>
 struct DeviceTraits<HeatVar<CT, DIM>> {
@@ -224,7 +235,11 @@ struct DeviceTraits<HeatVar<CT, DIM>> {
         d.unknown = 1;
         d.kappaKnown = 0;
         d.border = DeviceBorder::VALUE;
+#ifdef EXAMPLE_BAD_TRAITS
+        d.borderValue = 0.5;  // a wrong border value: the hooks keep the border at 0
+#else
         d.borderValue = 0.0;
+#endif
         d.knownTimeDependent = false;
     }
 };
@@ -258,8 +273,23 @@ struct DeviceTraits<YeePec<CT, DIM>> {
     }
 };
 }  // namespace ScaFES
+#endif
 
+static int runFamily(int argc, char* argv[]);
+
+// Errors of the engine (a refused DeviceTraits, a device failure) end the
+// program with status 1 and the message on stderr.
 int main(int argc, char* argv[])
+{
+    try {
+        return runFamily(argc, argv);
+    } catch (const std::exception& e) {
+        std::cerr << e.what() << std::endl;
+        return 1;
+    }
+}
+
+static int runFamily(int argc, char* argv[])
 {
     const std::size_t DIM = 3;
     ScaFES::Parameters params(argc, argv);

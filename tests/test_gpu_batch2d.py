@@ -76,3 +76,36 @@ def test_batch_rejects_oversized_problem(ctx):
     ens = batch2d_ensemble(3, 5, seed=1, sizes=[32, 300, 40])
     with pytest.raises(capi.ScafesError):
         run_batch(ctx, ens, 5, False)
+
+
+def test_batch_graph_kept_and_replayed(ctx):
+    """The instantiated graph is kept per argument set: replays with the same
+    arguments (the state re-uploaded in between) and a re-capture after the
+    arguments change all equal direct launches."""
+    nsteps = 30
+    ens = batch2d_ensemble(120, nsteps, seed=11)
+    ref, Jr, gr_ref = run_batch(ctx, ens, nsteps, True, use_graph=0)
+    d = {k: ctx.upload(v) for k, v in (("n", ens["n"]), ("off", ens["off"]), ("kappa", ens["kappa"]),
+                                       ("tau", ens["tau"]), ("order", ens["order"]), ("u", ens["u0"]))}
+    tgt = ctx.upload(ens["target"])
+    J, gr = ctx.alloc(len(ens["n"]), np.float64), ctx.alloc(len(ens["n"]), np.float64)
+
+    def call(steps):
+        d["u"].write(ens["u0"])
+        capi.check(ctx.L.scafes_batch2d_run(ctx.h, len(ens["n"]), d["n"].ptr, d["off"].ptr, d["kappa"].ptr,
+                                            d["tau"].ptr, d["order"].ptr, d["u"].ptr, tgt.ptr, J.ptr, gr.ptr, steps, 1))
+        return d["u"].read(), J.read(), gr.read()
+
+    for _ in range(3):  # capture, then two replays of the kept graph
+        out, Jg, gg = call(nsteps)
+        np.testing.assert_array_equal(out, ref)
+        np.testing.assert_array_equal(Jg, Jr)
+        np.testing.assert_array_equal(gg, gr_ref)
+    ref2, Jr2, _ = run_batch(ctx, ens, nsteps - 7, True, use_graph=0)
+    out2, J2, _ = call(nsteps - 7)  # new arguments: captured again
+    np.testing.assert_array_equal(out2, ref2)
+    np.testing.assert_array_equal(J2, Jr2)
+    out3, _, _ = call(nsteps)  # and back
+    np.testing.assert_array_equal(out3, ref)
+    for b in list(d.values()) + [tgt, J, gr]:
+        b.free()
