@@ -37,28 +37,36 @@ namespace {
 using Rsrc = __amdgpu_buffer_rsrc_t;
 constexpr int kRsrcWord3 = 0x00020000;  // gfx9 family: 32-bit data format, raw (stride 0) buffer
 
-__device__ inline Rsrc plane_rsrc(const void* plane, uint32_t range)
+
+// Buffer access to a workgroup's z-chunk of one array: the resource is
+// built once per workgroup (base = the lowest plane the workgroup touches,
+// kRecords bytes); the plane enters each access as the instruction's
+// scalar offset (one scalar multiply per plane and plane stride instead of a
+// 64-bit base and a resource per array and plane).
+constexpr uint32_t kRecords = 0x7FFFFFFFu;
+
+__device__ inline Rsrc chunk_rsrc(const void* base)
 {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(plane), (short)0, (int)range, kRsrcWord3);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)kRecords, kRsrcWord3);
 }
 
 template <typename T>
-__device__ inline T bload(Rsrc r, uint32_t off)
+__device__ inline T cload(Rsrc r, uint32_t voff, uint32_t soff)
 {
     if constexpr (sizeof(T) == 4)
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
     else
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
 }
 
 template <typename T>
-__device__ inline void bstore(T v, Rsrc r, uint32_t off)
+__device__ inline void cstore(T v, Rsrc r, uint32_t voff, uint32_t soff)
 {
     if constexpr (sizeof(T) == 4) {
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)voff, (int)soff, 0);
     } else {
         typedef int v2i __attribute__((ext_vector_type(2)));
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2i, v), r, (int)off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2i, v), r, (int)voff, (int)soff, 0);
     }
 }
 
@@ -131,11 +139,21 @@ __global__ __launch_bounds__(TX* TY) void pennes_adjk_kernel(Geo g, const T* __r
     const uint32_t oc = (uint32_t)(g.base + ic + g.e0 * jc) * B;
     const uint32_t ohx = (uint32_t)(g.base + hxi + g.e0 * jc) * B;
     const uint32_t ohy = (uint32_t)(g.base + ic + g.e0 * hyj) * B;
-    const uint32_t range = (uint32_t)((g.base + g.e01) * B);
     const uint32_t og = (uint32_t)(g.base + ic + g.e0 * jc) * 8u;  // FP64 gradient
-    const uint32_t grange = (uint32_t)((g.base + g.e01) * 8);
-    const size_t pb = (size_t)g.e01 * B, pg = (size_t)g.e01 * 8;
-    auto pl = [&](int kk) { return (size_t)min(max(kk, 0), g.nz - 1); };
+    const uint32_t pb = (uint32_t)(g.e01 * B), pg = (uint32_t)(g.e01 * 8);
+    // every plane this workgroup reads or writes lies in [kbase, nz-1]
+    const int kbase = max(kstart - 1, 0);
+    auto pl = [&](int kk) { return (uint32_t)(min(max(kk, kbase), g.nz - 1) - kbase); };
+    const Rsrc rl = chunk_rsrc((const char*)lam + (size_t)kbase * pb);
+    const Rsrc rw = chunk_rsrc((const char*)wf + (size_t)kbase * pb);
+    const Rsrc rlo = chunk_rsrc((char*)lamo + (size_t)kbase * pb);
+    const Rsrc rg = chunk_rsrc((char*)grad + (size_t)kbase * pg);
+    Rsrc rt[K], rtl[K];
+#pragma unroll
+    for (int l = 0; l < K; ++l) {
+        rt[l] = chunk_rsrc((const char*)st.t[l] + (size_t)kbase * pb);
+        rtl[l] = chunk_rsrc(LO ? (const char*)st.lo[l] + (size_t)kbase * pb : nullptr);
+    }
 
     T qa[D], qax[D], qay[D], qw[D], qt[K][D], qtl[K][D];
     double qg[D];
@@ -150,27 +168,26 @@ __global__ __launch_bounds__(TX* TY) void pennes_adjk_kernel(Geo g, const T* __r
     // plane kk - R's states and gradient
     auto fetch = [&](const int slot, const int kk) {
         if (kk <= kcen) {
-            const size_t p = pl(kk);
-            const Rsrc rl = plane_rsrc((const char*)lam + p * pb, range);
-            qa[slot] = bload<T>(rl, oc);
+            const uint32_t p = pl(kk);
+            qa[slot] = cload<T>(rl, oc, p * pb);
             if (kk <= kend) {
-                qw[slot] = bload<T>(plane_rsrc((const char*)wf + p * pb, range), oc);
-                if (hx) qax[slot] = bload<T>(rl, ohx);
-                if (hy) qay[slot] = bload<T>(rl, ohy);
+                qw[slot] = cload<T>(rw, oc, p * pb);
+                if (hx) qax[slot] = cload<T>(rl, ohx, p * pb);
+                if (hy) qay[slot] = cload<T>(rl, ohy, p * pb);
             }
             const int po = kk - R;
             if (out_thread && po >= k0 && po < k1) {
+                const uint32_t q = pl(po);
 #pragma unroll
                 for (int l = 0; l < K; ++l) {
-                    qt[l][slot] = bload<T>(plane_rsrc((const char*)st.t[l] + (size_t)po * pb, range), oc);
-                    if constexpr (LO)
-                        qtl[l][slot] = bload<T>(plane_rsrc((const char*)st.lo[l] + (size_t)po * pb, range), oc);
+                    qt[l][slot] = cload<T>(rt[l], oc, q * pb);
+                    if constexpr (LO) qtl[l][slot] = cload<T>(rtl[l], oc, q * pb);
                 }
-                qg[slot] = bload<double>(plane_rsrc((const char*)grad + (size_t)po * pg, grange), og);
+                qg[slot] = cload<double>(rg, og, q * pg);
             }
         }
     };
-    T am = bload<T>(plane_rsrc((const char*)lam + pl(kstart - 1) * pb, range), oc);
+    T am = cload<T>(rl, oc, pl(kstart - 1) * pb);
 #pragma unroll
     for (int s = 0; s < D; ++s) fetch(s, kstart + s);
 
@@ -247,9 +264,9 @@ __global__ __launch_bounds__(TX* TY) void pennes_adjk_kernel(Geo g, const T* __r
                             if constexpr (LO) tp += (double)tll[l];
                             gv += (double)lv * (cf.ctp * (cf.ta - tp));
                         }
-                        bstore<double>(gv, plane_rsrc((const char*)grad + (size_t)p * pg, grange), og);
+                        cstore<double>(gv, rg, og, pl(p) * pg);
                     }
-                    bstore<T>(v, plane_rsrc((const char*)lamo + (size_t)p * pb, range), oc);
+                    cstore<T>(v, rlo, oc, pl(p) * pb);
                 }
             }
             am = ac;
@@ -261,8 +278,11 @@ __global__ __launch_bounds__(TX* TY) void pennes_adjk_kernel(Geo g, const T* __r
 // Measured on an MI355X (Pennes 256^3 FP32, us per adjoint step, best z
 // chunk of 8..64 planes): K = 4 64x12 64.3, 64x16 72.3; K = 3 64x8 70.4,
 // 64x16 74.1; K = 2 64x16 72.6, 64x8 74.9.  The sweeps are not HBM-bound
-// (2.9 TB/s algorithmic at K = 4): per step a level costs about what it
-// saves in traffic, so the step time changes little with K.
+// (2.9 TB/s algorithmic at K = 4): the SQ counters put the scalar unit at
+// 0.68 of its issue rate against 0.47 for the vector units, from the
+// per-plane, per-array resources.  With one resource per array and
+// workgroup and the plane in the scalar offset, K = 4 takes 241 us per
+// launch (60 us per step, from 259), K = 2 150 (from 157).
 constexpr int kKTX = 64, kKTY = 12, kKZC = 32, kKD = 2;
 
 template <typename T, bool LO, int K>
@@ -296,8 +316,10 @@ int adjk_launch_k(Ctx* c, const Geo& g, int K, const void* lam, void* lamo, cons
 
 bool pennes_adjk_supported(const Geo& geo)
 {
-    // 3-D only; in-plane byte offsets of the FP64 gradient must fit 32 bits
-    return geo.dim == 3 && (geo.base + geo.e01) * 8L < (1L << 32);
+    // 3-D only; a workgroup's chunk of the FP64 gradient (at most kKZC + 2K
+    // + kKD planes) plus the in-plane offsets must stay below 2^31 - 1 bytes
+    return geo.dim == 3 && ((long)(kKZC + 2 * kAdjMaxK + kKD + 2) * geo.e01 + geo.base + geo.e01) * 8L <
+                               (long)kRecords;
 }
 
 int pennes_adjk_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, int K, const void* lam,
