@@ -50,9 +50,13 @@ __device__ inline Rsrc chunk_rsrc(const void* base)
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)kRecords, kRsrcWord3);
 }
 
+// The plane offset is uniform; readfirstlane states it for accesses inside
+// divergent branches, where the compiler would otherwise keep it in a
+// vector register and wrap the access in a waterfall loop.
 template <typename T>
 __device__ inline T cload(Rsrc r, uint32_t voff, uint32_t soff)
 {
+    soff = __builtin_amdgcn_readfirstlane(soff);
     if constexpr (sizeof(T) == 4)
         return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
     else
@@ -62,6 +66,7 @@ __device__ inline T cload(Rsrc r, uint32_t voff, uint32_t soff)
 template <typename T>
 __device__ inline void cstore(T v, Rsrc r, uint32_t voff, uint32_t soff)
 {
+    soff = __builtin_amdgcn_readfirstlane(soff);
     if constexpr (sizeof(T) == 4) {
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)voff, (int)soff, 0);
     } else {
