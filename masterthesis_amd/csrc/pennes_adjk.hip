@@ -106,6 +106,12 @@ __device__ inline T adj_update(const AdjCoef& cf, T c, T w_, T e, T s, T n, T d,
     }
 }
 
+// voffset of an access a lane does not make: beyond kRecords, so the range
+// check answers a load with 0 and drops a store, whatever the plane offset.
+constexpr uint32_t kSkip = 0x80000000u;
+
+constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
+
 template <typename T, bool LO, int TX, int TY, int K, int D>
 __global__ __launch_bounds__(TX* TY) void pennes_adjk_kernel(Geo g, const T* __restrict__ lam, T* __restrict__ lamo,
                                                               const T* __restrict__ wf, AdjStates st,
@@ -114,13 +120,21 @@ __global__ __launch_bounds__(TX* TY) void pennes_adjk_kernel(Geo g, const T* __r
 {
     static_assert(D >= 2 && D % 2 == 0, "prefetch depth must be even");
     static_assert(K >= 2 && K <= kAdjMaxK, "2 <= K <= kAdjMaxK");
+    static_assert(TX == 64, "one tile row per wavefront: ty is wave-uniform");
     constexpr int R = K - 1;  // rings the tile loses over the sweep
     constexpr int OX = TX - 2 * R, OY = TY - 2 * R;
     static_assert(OX >= 1 && OY >= 1, "tile too small for K levels");
-    constexpr int H = K + 1;  // register history per level (age 0 = this iteration)
+    // Register rings: the value level l computed a iterations ago sits at
+    // ring(it - a), ring size H (ages up to max(2, K-1) are read).  The plane
+    // loop is unrolled by U, a multiple of H and D, so every ring and
+    // prefetch-slot index is a compile-time constant: no register moves.
+    constexpr int H = K > 3 ? K : 3;
+    constexpr int U = H / gcd_c(H, D) * D;
+    auto ring = [](int c) { return ((c % H) + H) % H; };
     __shared__ T s0[2][TY + 2][TX + 2];  // lam^n plane with ring
     __shared__ T sl[K - 1][2][TY][TX];   // levels 1..K-1
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int tx = threadIdx.x;
+    const int ty = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     const int per_plane = tiles_x * tiles_y;
     const int tz = lb / per_plane;
@@ -129,22 +143,33 @@ __global__ __launch_bounds__(TX* TY) void pennes_adjk_kernel(Geo g, const T* __r
     const int txi = r - tyi * tiles_x;
     const int i = txi * OX - R + tx, j = tyi * OY - R + ty;
     const int ic = min(max(i, 0), g.nx - 1), jc = min(max(j, 0), g.ny - 1);
-    const bool inner_xy = i >= g.nl0 && i <= g.nx - 1 - g.nl0 && j >= g.nl1 && j <= g.ny - 1 - g.nl1;
-    const bool out_thread = tx >= R && tx <= TX - 1 - R && ty >= R && ty <= TY - 1 - R && i < g.nx && j < g.ny;
+    // lane-dependent (x) and wave-uniform (y) parts of the node predicates
+    const bool inner_i = i >= g.nl0 && i <= g.nx - 1 - g.nl0;
+    const bool inner_j = j >= g.nl1 && j <= g.ny - 1 - g.nl1;
+    const bool col_out = tx >= R && tx <= TX - 1 - R && i < g.nx;
+    const bool row_out = ty >= R && ty <= TY - 1 - R && j < g.ny;
     const int k0 = tz * zc, k1 = min(k0 + zc, g.nz);
-    const int kstart = k0 - R, kend = k1 + R - 1, kcen = k1 + R;
+    const int kstart = k0 - R, kend = k1 + R - 1;
     const int klo = g.nl2, khi = g.nz - 1 - g.nl2;
     const bool hx = (tx == 0) || (tx == TX - 1);
     const bool hy = (ty == 0) || (ty == TY - 1);
     const int hxi = (tx == 0) ? max(ic - 1, 0) : min(ic + 1, g.nx - 1);
     const int hyj = (ty == 0) ? max(jc - 1, 0) : min(jc + 1, g.ny - 1);
-    const int sxh = (tx == 0) ? 0 : TX + 1;
+    // x ring: a lane off the ring reloads and rewrites its own centre value
+    const int sxc = hx ? ((tx == 0) ? 0 : TX + 1) : tx + 1;
     const int syh = (ty == 0) ? 0 : TY + 1;
+    // LDS neighbours of levels >= 2, clamped into the tile: only lanes whose
+    // result is discarded read a clamped position
+    const int txm = max(tx - 1, 0), txp = min(tx + 1, TX - 1);
+    const int tym = max(ty - 1, 0), typ = min(ty + 1, TY - 1);
     constexpr uint32_t B = sizeof(T);
     const uint32_t oc = (uint32_t)(g.base + ic + g.e0 * jc) * B;
-    const uint32_t ohx = (uint32_t)(g.base + hxi + g.e0 * jc) * B;
-    const uint32_t ohy = (uint32_t)(g.base + ic + g.e0 * hyj) * B;
-    const uint32_t og = (uint32_t)(g.base + ic + g.e0 * jc) * 8u;  // FP64 gradient
+    const uint32_t ohx = hx ? (uint32_t)(g.base + hxi + g.e0 * jc) * B : oc;
+    const uint32_t ohy = hy ? (uint32_t)(g.base + ic + g.e0 * hyj) * B : kSkip;
+    const bool out = col_out && row_out;
+    const uint32_t oo = out ? oc : kSkip;                                     // output nodes
+    const uint32_t og = out ? (uint32_t)(g.base + ic + g.e0 * jc) * 8u : kSkip;  // FP64 gradient
+    const uint32_t ogs = inner_i ? og : kSkip;  // the gradient is written at inner nodes
     const uint32_t pb = (uint32_t)(g.e01 * B), pg = (uint32_t)(g.e01 * 8);
     // every plane this workgroup reads or writes lies in [kbase, nz-1]
     const int kbase = max(kstart - 1, 0);
@@ -170,56 +195,47 @@ __global__ __launch_bounds__(TX* TY) void pennes_adjk_kernel(Geo g, const T* __r
         qg[s] = 0.0;
     }
     // slot <- level-0 input at plane kk (centre, halo, w) and the output
-    // plane kk - R's states and gradient
+    // plane kk - R's states and gradient.  Unconditional: planes past the
+    // chunk are clamped into the grid and their values never used; output
+    // planes outside [k0, k1) are skipped through the offset.
     auto fetch = [&](const int slot, const int kk) {
-        if (kk <= kcen) {
-            const uint32_t p = pl(kk);
-            qa[slot] = cload<T>(rl, oc, p * pb);
-            if (kk <= kend) {
-                qw[slot] = cload<T>(rw, oc, p * pb);
-                if (hx) qax[slot] = cload<T>(rl, ohx, p * pb);
-                if (hy) qay[slot] = cload<T>(rl, ohy, p * pb);
-            }
-            const int po = kk - R;
-            if (out_thread && po >= k0 && po < k1) {
-                const uint32_t q = pl(po);
+        const uint32_t p = pl(kk) * pb;
+        qa[slot] = cload<T>(rl, oc, p);
+        qw[slot] = cload<T>(rw, oc, p);
+        qax[slot] = cload<T>(rl, ohx, p);
+        qay[slot] = cload<T>(rl, ohy, p);
+        const int po = kk - R;
+        const bool need = po >= k0 && po < k1;
+        const uint32_t os = need ? oo : kSkip, osg = need ? og : kSkip;
+        const uint32_t q = pl(po);
 #pragma unroll
-                for (int l = 0; l < K; ++l) {
-                    qt[l][slot] = cload<T>(rt[l], oc, q * pb);
-                    if constexpr (LO) qtl[l][slot] = cload<T>(rtl[l], oc, q * pb);
-                }
-                qg[slot] = cload<double>(rg, og, q * pg);
-            }
+        for (int l = 0; l < K; ++l) {
+            qt[l][slot] = cload<T>(rt[l], os, q * pb);
+            if constexpr (LO) qtl[l][slot] = cload<T>(rtl[l], os, q * pb);
         }
+        qg[slot] = cload<double>(rg, osg, q * pg);
     };
-    T am = cload<T>(rl, oc, pl(kstart - 1) * pb);
 #pragma unroll
     for (int s = 0; s < D; ++s) fetch(s, kstart + s);
 
-    T hist[K][H];  // hist[L][a]: level L at the plane it computed a iterations ago
-    T wh[K];       // w at planes k, k-1, ..., k-K+1
+    T hist[K][H];  // hist[l][ring(it)]: level l at the plane it computed in iteration it
+    T wh[H];       // wh[ring(it)]: w at plane kstart + it
 #pragma unroll
-    for (int l = 0; l < K; ++l) {
-        wh[l] = T(0);
+    for (int a = 0; a < H; ++a) {
+        wh[a] = T(0);
 #pragma unroll
-        for (int a = 0; a < H; ++a) hist[l][a] = T(0);
+        for (int l = 0; l < K; ++l) hist[l][a] = T(0);
     }
-    for (int kb = kstart; kb <= kend; kb += D) {
+    hist[0][ring(-1)] = cload<T>(rl, oc, pl(kstart - 1) * pb);
+    for (int kb = kstart; kb <= kend; kb += U) {
 #pragma unroll
-        for (int s = 0; s < D; ++s) {
-            const int k = kb + s;
+        for (int u = 0; u < U; ++u) {
+            const int k = kb + u;
             if (k > kend) break;  // uniform over the workgroup
-            const int buf = s & 1;
-            const T ac = qa[s], ap = qa[(s + 1) % D];
-#pragma unroll
-            for (int l = 0; l < K; ++l) {
-#pragma unroll
-                for (int a = H - 1; a >= 1; --a) hist[l][a] = hist[l][a - 1];
-            }
-#pragma unroll
-            for (int a = K - 1; a >= 1; --a) wh[a] = wh[a - 1];
-            wh[0] = qw[s];
-            hist[0][0] = ac;
+            const int s = u % D, buf = u & 1;
+            const T ac = qa[s], ap = qa[(s + 1) % D], am = hist[0][ring(u - 1)];
+            hist[0][ring(u)] = ac;
+            wh[ring(u)] = qw[s];
             T tl[K], tll[K];
 #pragma unroll
             for (int l = 0; l < K; ++l) {
@@ -228,53 +244,52 @@ __global__ __launch_bounds__(TX* TY) void pennes_adjk_kernel(Geo g, const T* __r
             }
             const double gl = qg[s];
             s0[buf][ty + 1][tx + 1] = ac;
-            if (hx) s0[buf][ty + 1][sxh] = qax[s];
+            s0[buf][ty + 1][sxc] = qax[s];
             if (hy) s0[buf][syh][tx + 1] = qay[s];
             fetch(s, k + D);
             __syncthreads();
             // ---- level 1: lam^{n-1} at plane k, every thread
             {
-                const bool in1 = inner_xy && k >= klo && k <= khi;
+                const bool in1 = inner_i && inner_j && k >= klo && k <= khi;
                 T l1 = adj_update<T>(cf, ac, s0[buf][ty + 1][tx], s0[buf][ty + 1][tx + 2], s0[buf][ty][tx + 1],
-                                     s0[buf][ty + 2][tx + 1], am, ap, wh[0]);
-                if (!in1) l1 = T(0);
-                hist[1][0] = l1;
+                                     s0[buf][ty + 2][tx + 1], am, ap, wh[ring(u)]);
+                l1 = in1 ? l1 : T(0);
+                hist[1][ring(u)] = l1;
                 sl[0][buf][ty][tx] = l1;
             }
             // ---- levels 2..K: lam^{n-L} at plane k-(L-1) on the tile shrunk by L-1 rings
 #pragma unroll
             for (int L = 2; L <= K; ++L) {
                 const int p = k - (L - 1);
-                const bool valid = tx >= L - 1 && tx <= TX - L && ty >= L - 1 && ty <= TY - L;
-                const bool inL = inner_xy && p >= klo && p <= khi;
+                const bool inz = p >= klo && p <= khi;
                 T v = T(0);
-                if (valid && inL) {
+                if (ty >= L - 1 && ty <= TY - L && inner_j && inz) {  // wave-uniform
                     const int pbf = buf ^ 1;
-                    v = adj_update<T>(cf, hist[L - 1][1], sl[L - 2][pbf][ty][tx - 1], sl[L - 2][pbf][ty][tx + 1],
-                                      sl[L - 2][pbf][ty - 1][tx], sl[L - 2][pbf][ty + 1][tx], hist[L - 1][2],
-                                      hist[L - 1][0], wh[L - 1]);
+                    v = adj_update<T>(cf, hist[L - 1][ring(u - 1)], sl[L - 2][pbf][ty][txm],
+                                      sl[L - 2][pbf][ty][txp], sl[L - 2][pbf][tym][tx], sl[L - 2][pbf][typ][tx],
+                                      hist[L - 1][ring(u - 2)], hist[L - 1][ring(u)], wh[ring(u - (L - 1))]);
+                    v = (tx >= L - 1 && tx <= TX - L && inner_i) ? v : T(0);
                 }
                 if (L < K) {
-                    hist[L][0] = v;
+                    hist[L][ring(u)] = v;
                     sl[L - 1][buf][ty][tx] = v;
-                } else if (out_thread && p >= k0) {
+                } else if (row_out && p >= k0) {  // wave-uniform
                     // output plane: lam^{n-K} and the K gradient terms in step order
-                    if (inL) {
+                    if (inner_j && inz) {
                         double gv = gl;
 #pragma unroll
                         for (int l = 0; l < K; ++l) {
                             // lam^{n-l} at plane p: level l computed it K-1 (l = 0) or K-l iterations ago
-                            const T lv = hist[l][l == 0 ? K - 1 : K - l];
+                            const T lv = hist[l][ring(u - (l == 0 ? K - 1 : K - l))];
                             double tp = (double)tl[l];
                             if constexpr (LO) tp += (double)tll[l];
                             gv += (double)lv * (cf.ctp * (cf.ta - tp));
                         }
-                        cstore<double>(gv, rg, og, pl(p) * pg);
+                        cstore<double>(gv, rg, ogs, pl(p) * pg);
                     }
-                    cstore<T>(v, rlo, oc, pl(p) * pb);
+                    cstore<T>(v, rlo, oo, pl(p) * pb);
                 }
             }
-            am = ac;
         }
     }
 }
@@ -284,10 +299,14 @@ __global__ __launch_bounds__(TX* TY) void pennes_adjk_kernel(Geo g, const T* __r
 // chunk of 8..64 planes): K = 4 64x12 64.3, 64x16 72.3; K = 3 64x8 70.4,
 // 64x16 74.1; K = 2 64x16 72.6, 64x8 74.9.  The sweeps are not HBM-bound
 // (2.9 TB/s algorithmic at K = 4): the SQ counters put the scalar unit at
-// 0.68 of its issue rate against 0.47 for the vector units, from the
-// per-plane, per-array resources.  With one resource per array and
-// workgroup and the plane in the scalar offset, K = 4 takes 241 us per
-// launch (60 us per step, from 259), K = 2 150 (from 157).
+// 0.68 of its issue rate against 0.47 for the vector units.  One resource
+// per array and workgroup (the plane in the scalar offset) took K = 4 to
+// 241 us per launch; wave-uniform rows, skipped lanes through out-of-range
+// offsets instead of branches and register rings instead of shifts take it
+// to 207 us (K = 3 157, K = 2 139), scalar instructions per wave 4493 ->
+// 2624.  With the branch-free kernel 64x12 with prefetch depth 2 is still
+// the best tile: 64x16 is 8 % slower, 64x16 limited to 64 VGPRs (8 waves
+// per SIMD, spills) 44 %, prefetch depth 4 16 %, 64x8 24 %.
 constexpr int kKTX = 64, kKTY = 12, kKZC = 32, kKD = 2;
 
 template <typename T, bool LO, int K>
