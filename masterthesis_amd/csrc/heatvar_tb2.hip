@@ -125,12 +125,16 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb2_kernel(Geo g, const T* __r
                                                               int tiles_y)
 {
     static_assert(D >= 2 && D % 2 == 0, "prefetch depth must be even");
+    static_assert(TX % 64 == 0, "whole rows per wavefront: ty is wave-uniform");
     constexpr int OX = TX - 2, OY = TY - 2;
     constexpr bool KH = (MODE == 2);
-    __shared__ T s0[2][TY + 2][TX + 2];  // level-0 plane with ring
-    __shared__ T sk[2][TY + 2][TX + 2];  // kappa plane with ring
+    // level-0 and kappa planes with ring; row TY + 2 takes the y-ring writes
+    // of the rows off the ring (a sink that is never read)
+    __shared__ T s0[2][TY + 3][TX + 2];
+    __shared__ T sk[2][TY + 3][TX + 2];
     __shared__ T s1[2][TY][TX];          // level-1 plane
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int tx = threadIdx.x;
+    const int ty = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     const int per_plane = tiles_x * tiles_y;
     const int tz = lb / per_plane;
@@ -142,7 +146,9 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb2_kernel(Geo g, const T* __r
     // in-plane interior (also false outside the grid): border and outside
     // positions keep (or take the border value) at both levels
     const bool inner_xy = i >= g.nl0 && i <= g.nx - 1 - g.nl0 && j >= g.nl1 && j <= g.ny - 1 - g.nl1;
-    const bool out_thread = tx >= 1 && tx <= TX - 2 && ty >= 1 && ty <= TY - 2 && i < g.nx && j < g.ny;
+    const bool col_out = tx >= 1 && tx <= TX - 2 && i < g.nx;  // lane-dependent
+    const bool row_out = ty >= 1 && ty <= TY - 2 && j < g.ny;  // wave-uniform
+    const int txm = max(tx - 1, 0), txp = min(tx + 1, TX - 1);
     const int k0 = tz * zc, k1 = min(k0 + zc, g.nz);
     const int kstart = k0 - 1;  // first plane with a level-1 value
     const int kcen = k1 + 1;    // last plane whose centre is needed
@@ -152,15 +158,16 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb2_kernel(Geo g, const T* __r
     const bool hy = (ty == 0) || (ty == TY - 1);
     const int hxi = (tx == 0) ? max(ic - 1, 0) : min(ic + 1, g.nx - 1);
     const int hyj = (ty == 0) ? max(jc - 1, 0) : min(jc + 1, g.ny - 1);
-    const int sxh = (tx == 0) ? 0 : TX + 1;
-    const int syh = (ty == 0) ? 0 : TY + 1;
+    // x ring: a lane off the ring reloads and rewrites its own centre value
+    const int sxc = hx ? ((tx == 0) ? 0 : TX + 1) : tx + 1;
+    const int syc = hy ? ((ty == 0) ? 0 : TY + 1) : TY + 2;
     constexpr uint32_t B = sizeof(T);
     const uint32_t oc = (uint32_t)(g.base + ic + g.e0 * jc) * B;
-    const uint32_t ohx = (uint32_t)(g.base + hxi + g.e0 * jc) * B;
+    const uint32_t ohx = hx ? (uint32_t)(g.base + hxi + g.e0 * jc) * B : oc;
     const uint32_t ohy = (uint32_t)(g.base + ic + g.e0 * hyj) * B;
-    const uint32_t oo = (uint32_t)(g.base + i + g.e0 * j) * B;  // output node (out_thread only)
-    const size_t pb = (size_t)g.e01 * B;                        // plane stride in bytes
-    const uint32_t range = (uint32_t)((g.base + g.e01) * B);    // in-plane offsets stay below this
+    const size_t pb = (size_t)g.e01 * B;                      // plane stride in bytes
+    const uint32_t range = (uint32_t)((g.base + g.e01) * B);  // in-plane offsets stay below this
+    const uint32_t oo = col_out ? (uint32_t)(g.base + i + g.e0 * j) * B : range;  // output node
     const char* ub = (const char*)u;
     const char* kb8 = (const char*)kap;
     const char* cb = (const char*)cin;
@@ -178,7 +185,7 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb2_kernel(Geo g, const T* __r
         const Rsrc ru = plane_rsrc(ub + o, range), rk = plane_rsrc(kb8 + o, range);
         const bool cen = kk <= kcen, mid = kk <= k1;
         const uint32_t occ = cen ? oc : range, ocm = mid ? oc : range;
-        const uint32_t ox = (mid && hx) ? ohx : range, oy = (mid && hy) ? ohy : range;
+        const uint32_t ox = mid ? ohx : range, oy = (mid && hy) ? ohy : range;
         qa[slot] = bload<T>(ru, occ);
         qb[slot] = bload<T>(rk, occ);
         if constexpr (KH) qc[slot] = bload<T>(plane_rsrc(cb + o, range), ocm);
@@ -201,21 +208,19 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb2_kernel(Geo g, const T* __r
 #pragma unroll
         for (int s = 0; s < D; ++s) {
             const int k = kb + s;
-            if (k > k1) break;  // uniform over the workgroup
+            // uniform over the workgroup; nothing follows the plane loop, so
+            // the exit leaves the register queues dead (no copies on this path)
+            if (k > k1) return;
             const int buf = s & 1;
             const T ac = qa[s], ap = qa[(s + 1) % D];
             const T bc = qb[s], bp = qb[(s + 1) % D];
             const T c0 = qc[s];
             s0[buf][ty + 1][tx + 1] = ac;
             sk[buf][ty + 1][tx + 1] = bc;
-            if (hx) {
-                s0[buf][ty + 1][sxh] = qax[s];
-                sk[buf][ty + 1][sxh] = qbx[s];
-            }
-            if (hy) {
-                s0[buf][syh][tx + 1] = qay[s];
-                sk[buf][syh][tx + 1] = qby[s];
-            }
+            s0[buf][ty + 1][sxc] = qax[s];
+            sk[buf][ty + 1][sxc] = qbx[s];
+            s0[buf][syc][tx + 1] = qay[s];
+            sk[buf][syc][tx + 1] = qby[s];
             fetch(s, k + D);
             __syncthreads();
             // ---- level 1 at (i, j, k), computed for every position, kept where not interior
@@ -237,11 +242,14 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb2_kernel(Geo g, const T* __r
             // ---- level 2 at (i, j, k-1) from the level-1 plane of the previous iteration
             const int k2 = k - 1;
             {
-                const bool st = k2 >= k0 && out_thread;
+                // computed by the output rows on output planes (a wave-uniform
+                // branch); the store is issued by every thread, out of range
+                // off the output columns and planes, so dropped there
+                const bool st = k2 >= k0 && row_out;
                 T v = T(0), cv = T(0);
                 if (st) {
                     const int qb1 = buf ^ 1;
-                    v = var_update<T, MODE>(cf, l1m1, s1[qb1][ty][tx - 1], s1[qb1][ty][tx + 1], s1[qb1][ty - 1][tx],
+                    v = var_update<T, MODE>(cf, l1m1, s1[qb1][ty][txm], s1[qb1][ty][txp], s1[qb1][ty - 1][tx],
                                             s1[qb1][ty + 1][tx], l1m2, l1, fp, c1m1, &cv);
                     if (!(inner_xy && k2 >= klo && k2 <= khi)) {
                         v = keep ? l1m1 : bv;
@@ -271,6 +279,14 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb2_kernel(Geo g, const T* __r
 // an LDS plane written one iteration earlier (one barrier per plane, all
 // planes double-buffered).  The face coefficients of a node are the same
 // at every level, so levels 2 and 3 take them from a register queue.
+// A tile row is one wavefront (TX a multiple of 64), so row predicates are
+// wave-uniform branches; lane predicates never branch: off-ring lanes load
+// and rewrite their own centre, off-ring rows write the y ring into a sink
+// row, and lanes off the output columns store out of range.  Measured on an
+// MI355X at 512^3 (64x12, us per three-step sweep): F32 + Kahan 835 -> 813,
+// F32 701 -> 631, F64 (64x16) 1882 -> 1767 against the branching kernel;
+// SQ counters for F32 + Kahan: vector ALU 0.64 of its issue rate, scalar
+// 0.51 (0.66 / 0.65 before).
 template <typename T, int MODE, int TX, int TY, int D>
 __global__ __launch_bounds__(TX* TY) void heatvar_tb3_kernel(Geo g, const T* __restrict__ u, const T* __restrict__ cin,
                                                               const T* __restrict__ kap, T* __restrict__ uout,
@@ -278,13 +294,17 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb3_kernel(Geo g, const T* __r
                                                               int tiles_y)
 {
     static_assert(D >= 2 && D % 2 == 0, "prefetch depth must be even");
+    static_assert(TX % 64 == 0, "whole rows per wavefront: ty is wave-uniform");
     constexpr int OX = TX - 4, OY = TY - 4;
     constexpr bool KH = (MODE == 2);
-    __shared__ T s0[2][TY + 2][TX + 2];  // level-0 plane with ring
-    __shared__ T sk[2][TY + 2][TX + 2];  // kappa plane with ring
+    // level-0 and kappa planes with ring; row TY + 2 takes the y-ring writes
+    // of the rows off the ring (a sink that is never read)
+    __shared__ T s0[2][TY + 3][TX + 2];
+    __shared__ T sk[2][TY + 3][TX + 2];
     __shared__ T s1[2][TY][TX];          // level-1 plane
     __shared__ T s2[2][TY][TX];          // level-2 plane
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int tx = threadIdx.x;
+    const int ty = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     const int per_plane = tiles_x * tiles_y;
     const int tz = lb / per_plane;
@@ -294,7 +314,8 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb3_kernel(Geo g, const T* __r
     const int i = txi * OX - 2 + tx, j = tyi * OY - 2 + ty;
     const int ic = min(max(i, 0), g.nx - 1), jc = min(max(j, 0), g.ny - 1);
     const bool inner_xy = i >= g.nl0 && i <= g.nx - 1 - g.nl0 && j >= g.nl1 && j <= g.ny - 1 - g.nl1;
-    const bool out_thread = tx >= 2 && tx <= TX - 3 && ty >= 2 && ty <= TY - 3 && i < g.nx && j < g.ny;
+    const bool col_out = tx >= 2 && tx <= TX - 3 && i < g.nx;  // lane-dependent
+    const bool row_out = ty >= 2 && ty <= TY - 3 && j < g.ny;  // wave-uniform
     // clamped neighbour slots of levels 2 and 3 (clamped reads only feed
     // positions that are never stored)
     const int txm = max(tx - 1, 0), txp = min(tx + 1, TX - 1), tym = max(ty - 1, 0), typ = min(ty + 1, TY - 1);
@@ -307,15 +328,16 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb3_kernel(Geo g, const T* __r
     const bool hy = (ty == 0) || (ty == TY - 1);
     const int hxi = (tx == 0) ? max(ic - 1, 0) : min(ic + 1, g.nx - 1);
     const int hyj = (ty == 0) ? max(jc - 1, 0) : min(jc + 1, g.ny - 1);
-    const int sxh = (tx == 0) ? 0 : TX + 1;
-    const int syh = (ty == 0) ? 0 : TY + 1;
+    // x ring: a lane off the ring reloads and rewrites its own centre value
+    const int sxc = hx ? ((tx == 0) ? 0 : TX + 1) : tx + 1;
+    const int syc = hy ? ((ty == 0) ? 0 : TY + 1) : TY + 2;
     constexpr uint32_t B = sizeof(T);
     const uint32_t oc = (uint32_t)(g.base + ic + g.e0 * jc) * B;
-    const uint32_t ohx = (uint32_t)(g.base + hxi + g.e0 * jc) * B;
+    const uint32_t ohx = hx ? (uint32_t)(g.base + hxi + g.e0 * jc) * B : oc;
     const uint32_t ohy = (uint32_t)(g.base + ic + g.e0 * hyj) * B;
-    const uint32_t oo = (uint32_t)(g.base + i + g.e0 * j) * B;  // output node (out_thread only)
     const size_t pb = (size_t)g.e01 * B;
     const uint32_t range = (uint32_t)((g.base + g.e01) * B);
+    const uint32_t oo = col_out ? (uint32_t)(g.base + i + g.e0 * j) * B : range;  // output node
     const char* ub = (const char*)u;
     const char* kb8 = (const char*)kap;
     const char* cb = (const char*)cin;
@@ -335,7 +357,7 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb3_kernel(Geo g, const T* __r
         const Rsrc ru = plane_rsrc(ub + o, range), rk = plane_rsrc(kb8 + o, range);
         const bool cen = kk <= kcen, mid = kk <= klast;
         const uint32_t occ = cen ? oc : range, ocm = mid ? oc : range;
-        const uint32_t ox = (mid && hx) ? ohx : range, oy = (mid && hy) ? ohy : range;
+        const uint32_t ox = mid ? ohx : range, oy = (mid && hy) ? ohy : range;
         qa[slot] = bload<T>(ru, occ);
         qb[slot] = bload<T>(rk, occ);
         if constexpr (KH) qc[slot] = bload<T>(plane_rsrc(cb + o, range), ocm);
@@ -364,21 +386,19 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb3_kernel(Geo g, const T* __r
         for (int uu = 0; uu < U; ++uu) {
             const int s = uu % D;
             const int k = kb + uu;
-            if (k > klast) break;  // uniform over the workgroup
+            // uniform over the workgroup; nothing follows the plane loop, so
+            // the exit leaves the register queues dead (no copies on this path)
+            if (k > klast) return;
             const int buf = uu & 1, pbf = buf ^ 1;
             const T ac = qa[s], ap = qa[(s + 1) % D];
             const T bc = qb[s], bp = qb[(s + 1) % D];
             const T c0 = qc[s];
             s0[buf][ty + 1][tx + 1] = ac;
             sk[buf][ty + 1][tx + 1] = bc;
-            if (hx) {
-                s0[buf][ty + 1][sxh] = qax[s];
-                sk[buf][ty + 1][sxh] = qbx[s];
-            }
-            if (hy) {
-                s0[buf][syh][tx + 1] = qay[s];
-                sk[buf][syh][tx + 1] = qby[s];
-            }
+            s0[buf][ty + 1][sxc] = qax[s];
+            sk[buf][ty + 1][sxc] = qbx[s];
+            s0[buf][syc][tx + 1] = qay[s];
+            sk[buf][syc][tx + 1] = qby[s];
             fetch(s, k + D);
             __syncthreads();
             // ---- level 1 at plane k
@@ -397,25 +417,30 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb3_kernel(Geo g, const T* __r
                 c1 = keep ? c0 : T(0);
             }
             s1[buf][ty][tx] = l1;
-            // ---- level 2 at plane k-1 from the level-1 plane written one iteration ago
+            // ---- level 2 at plane k-1 from the level-1 plane written one
+            // iteration ago; level 3 reads it on rows 1..TY-2 only (a
+            // wave-uniform branch skips the outer rows)
             const int k2 = k - 1;
-            T c2;
-            T l2 = var_update<T, MODE>(cf, l1m1, s1[pbf][ty][txm], s1[pbf][ty][txp], s1[pbf][tym][tx],
-                                       s1[pbf][typ][tx], l1m2, l1, fq1, c1m1, &c2);
-            if (!(inner_xy && k2 >= klo && k2 <= khi)) {
-                l2 = keep ? l1m1 : bv;
-                c2 = keep ? c1m1 : T(0);
+            T c2 = T(0), l2 = T(0);
+            if (ty >= 1 && ty <= TY - 2) {
+                l2 = var_update<T, MODE>(cf, l1m1, s1[pbf][ty][txm], s1[pbf][ty][txp], s1[pbf][tym][tx],
+                                         s1[pbf][typ][tx], l1m2, l1, fq1, c1m1, &c2);
+                if (!(inner_xy && k2 >= klo && k2 <= khi)) {
+                    l2 = keep ? l1m1 : bv;
+                    c2 = keep ? c1m1 : T(0);
+                }
             }
             s2[buf][ty][tx] = l2;
             // ---- level 3 at plane k-2 from the level-2 plane written one iteration ago
             const int k3 = k - 2;
             {
-                // computed by output threads on output planes; the store is
-                // issued by every thread (out of range elsewhere, so dropped)
-                const bool st = k3 >= k0 && k3 < k1 && out_thread;
+                // computed by the output rows on output planes (a wave-uniform
+                // branch); the store is issued by every thread, out of range
+                // off the output columns and planes, so dropped there
+                const bool st = k3 >= k0 && k3 < k1 && row_out;
                 T l3 = T(0), c3 = T(0);
                 if (st) {
-                    l3 = var_update<T, MODE>(cf, l2m1, s2[pbf][ty][tx - 1], s2[pbf][ty][tx + 1], s2[pbf][ty - 1][tx],
+                    l3 = var_update<T, MODE>(cf, l2m1, s2[pbf][ty][txm], s2[pbf][ty][txp], s2[pbf][ty - 1][tx],
                                              s2[pbf][ty + 1][tx], l2m2, l2, fq2, c2m1, &c3);
                     if (!(inner_xy && k3 >= klo && k3 <= khi)) {
                         l3 = keep ? l2m1 : bv;

@@ -112,8 +112,11 @@ constexpr uint32_t kSkip = 0x80000000u;
 
 constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
 
-template <typename T, bool LO, int TX, int TY, int K, int D>
-__global__ __launch_bounds__(TX* TY) void pennes_adjk_kernel(Geo g, const T* __restrict__ lam, T* __restrict__ lamo,
+// W: waves per SIMD the register budget must allow.  Two workgroups of 12
+// waves per CU need at most 80 VGPRs (W = 6); the FP64 and compensated
+// variants would spill under that cap and run one workgroup per CU (W = 1).
+template <typename T, bool LO, int TX, int TY, int K, int D, int W = (sizeof(T) == 4 && !LO) ? 6 : 1>
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(W))) void pennes_adjk_kernel(Geo g, const T* __restrict__ lam, T* __restrict__ lamo,
                                                               const T* __restrict__ wf, AdjStates st,
                                                               double* __restrict__ grad, AdjCoef cf, int zc,
                                                               int tiles_x, int tiles_y)
@@ -231,7 +234,9 @@ __global__ __launch_bounds__(TX* TY) void pennes_adjk_kernel(Geo g, const T* __r
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int k = kb + u;
-            if (k > kend) break;  // uniform over the workgroup
+            // uniform over the workgroup; nothing follows the plane loop, so
+            // the exit leaves the register rings dead (no copies on this path)
+            if (k > kend) return;
             const int s = u % D, buf = u & 1;
             const T ac = qa[s], ap = qa[(s + 1) % D], am = hist[0][ring(u - 1)];
             hist[0][ring(u)] = ac;
