@@ -451,11 +451,13 @@ def bench_kappa_grad(ctx, capi, g, n, u0, k0, tau, nsteps, t_fwd_ms, snapshots=(
     grad = ctx.alloc(ne, np.float64)
     out = {"workload": f"heat7var {n}^3 dJ/dkappa, {nsteps} steps, FP32+Kahan forward, FP32 adjoint, FP64 gradient",
            "fwd_ms": round(t_fwd_ms, 2),
-           "recompute": "the headline multi-step sweep where the schedule advances several steps, else one step",
-           "adjoint_kernels": "heatvar_adj2_kernel<float, 64, 8, 2> (two reverse steps per sweep where the "
-                              "schedule has both states at hand, 36 B/node per sweep: lam r+w, kappa, u^{n-1}, "
-                              "u^{n-2}, FP64 gradient r+w); stream_kernel<HeatVarAdjOp<float>> (one step, 32 B) "
-                              "elsewhere",
+           "recompute": "the headline three-step sweep where the schedule advances several steps; where it "
+                        "stores every state, the same sweep writing the two states in between "
+                        "(heatvar_tb3_kernel<float, 2, 64, 12, 2, true>, 28 B/node per sweep); else one step",
+           "adjoint_kernels": "heatvar_adjk_kernel<float, 64, 16, K, 2> (K <= 4 reverse steps per sweep where the "
+                              "schedule has the states at hand, (28 + 4K) B/node per sweep: lam r+w, kappa, "
+                              "u^{n-1..n-K}, FP64 gradient r+w); stream_kernel<HeatVarAdjOp<float>> (one step, "
+                              "32 B) elsewhere",
            "adjoint_profile": _profile_fields("heat7var_adjoint_512.md", n == 512),
            "gradient": []}
     for s in snapshots:
@@ -639,12 +641,13 @@ def bench_emfdtd(ctx, capi, n=256, nsteps=500):
             "sink_peak": float(np.abs(sk).max())}
 
 
-def bench_pennes(ctx, capi, n=256, nsteps=2000, snapshots=(62, 1999)):
+def bench_pennes(ctx, capi, n=256, nsteps=2000, snapshots=(62, 500, 1999)):
     """Pennes bioheat on n^3: k = 0.5, rho c = 1, T_a = 37, log-normal
     perfusion w (median 0.5, sigma 0.3, i.i.d. seeded), Gaussian source;
-    forward time vs adjoint-gradient time (FP32 fields, FP64 gradient) for a
-    binomial snapshot budget (62 states, 4.3 GB) and for store-all (1999
-    states, 134 GB: fits in one MI355X's 288 GB)."""
+    forward time vs adjoint-gradient time (FP32 fields, FP64 gradient) for
+    binomial snapshot budgets of 62 states (4.3 GB) and 500 states (34 GB,
+    an eighth of one MI355X's 288 GB) and for store-all (1999 states,
+    137 GB)."""
     from masterthesis_amd.workloads import pennes_workload
     wl = pennes_workload(n, seed=0)
     g = capi.geom((n, n, n))

@@ -206,9 +206,12 @@ struct Rev {
     {
         adj_steps += K;
         if (family == FAMILY_HEATVAR) {
-            // two steps per sweep; the var-kappa adjoint reads the states' hi parts
-            int r = heatvar_adj2_launch(c, geo, g, lam_dtype, lam[lcur], lam[1 - lcur], kappa, st[0], st[1], grad,
-                                        tau);
+            // the var-kappa adjoint reads the states' hi parts
+            int r = heatvar_adjk_supported(geo)
+                        ? heatvar_adjk_launch(c, geo, g, lam_dtype, K, lam[lcur], lam[1 - lcur], kappa, st, grad,
+                                              tau)
+                        : heatvar_adj2_launch(c, geo, g, lam_dtype, lam[lcur], lam[1 - lcur], kappa, st[0], st[1],
+                                              grad, tau);
             lcur = 1 - lcur;
             return r;
         }
@@ -229,18 +232,38 @@ struct Rev {
             // take), then the steps are reversed up to kAdjMaxK per sweep.
             std::vector<const void*> st(1, base);  // st[q] = T^{a+q}
             int r = SCAFES_OK;
-            for (long q = 1; q < b - a && !r; ++q) {
+            // The variable-kappa family takes the m mod 3 first states one
+            // step at a time, the rest three per sweep with the two states in
+            // between written by the same sweep (value parts only: its adjoint
+            // reads no more), so that the last state T^{b-1} is whole.
+            const long m = b - a - 1;
+            long single = m;
+            if (family == FAMILY_HEATVAR && m >= 3 && heat7var_states_sweep_ok(c, geo, heat_mode)) single = m % 3;
+            for (long q = 1; q <= single && !r; ++q) {
                 void* sl = slots.back();
                 slots.pop_back();
                 r = step(st.back(), sl);
                 st.push_back(sl);
+            }
+            while (!r && (long)st.size() - 1 < m) {
+                void* s3[3];
+                for (int q = 0; q < 3; ++q) {
+                    s3[q] = slots.back();
+                    slots.pop_back();
+                }
+                void* mid[2] = {s3[0], s3[1]};
+                const void* x = st.back();
+                r = heat7var_sweep_states(c, geo, g, heat_mode, x, s3[2], lo(x), lo(s3[2]), mid, kappa, tau,
+                                          border_mode, border_value);
+                fwd_steps += 3;
+                for (int q = 0; q < 3; ++q) st.push_back(s3[q]);
             }
             for (long n = b; n > a && !r;) {
                 if (n == nsteps || n - a < 2) {
                     r = adjoint((int)n, st[n - 1 - a]);
                     n -= 1;
                 } else {
-                    const int kmax = family == FAMILY_HEATVAR ? 2 : kAdjMaxK;
+                    const int kmax = family == FAMILY_HEATVAR && !heatvar_adjk_supported(geo) ? 2 : kAdjMaxK;
                     const int K = (int)std::min<long>(kmax, n - a);
                     const void* sk[kAdjMaxK];
                     for (int l = 0; l < K; ++l) sk[l] = st[n - 1 - l - a];

@@ -27,63 +27,10 @@
 #include <algorithm>
 #include <cstdint>
 
+#include "heatvar_ops.hpp"
 #include "scafes_common.hpp"
 
 namespace scafes {
-
-struct VarCoef {
-    double tau, hh0, hh1, hh2;  // FP64 path (CPU operation order)
-    float c0, c1, c2;           // 0.5*tau/h_p^2, FP32 paths
-    double border_value;
-    int border_mode;
-};
-
-// Face coefficient between the centre and one neighbour: 0.5*(k_nb + k_c)
-// in the FP64 path (the CPU path's grouping), k_nb + k_c in FP32 (the 0.5 is
-// folded into c_p).  IEEE addition is commutative, so the face shared by two
-// nodes has one value.
-template <typename T, int MODE>
-__device__ inline T face(T a, T b)
-{
-    if constexpr (MODE == 0)
-        return 0.5 * (a + b);
-    else
-        return a + b;
-}
-
-template <typename T>
-struct Faces {
-    T e, w, n, s, u, d;
-};
-
-// One explicit step at one interior node from its six face coefficients.
-// MODE 0: FP64 in the CPU path's order; 1: FP32; 2: FP32 + Kahan (cin -> *cout).
-template <typename T, int MODE>
-__device__ inline T var_update(const VarCoef& cf, T uc, T uw, T ue, T us, T un, T ud, T uu, const Faces<T>& f, T cin,
-                               T* cout)
-{
-    if constexpr (MODE == 0) {
-        double v = uc;
-        v += (cf.tau * (f.e * (ue - uc) - f.w * (uc - uw))) / cf.hh0;
-        v += (cf.tau * (f.n * (un - uc) - f.s * (uc - us))) / cf.hh1;
-        v += (cf.tau * (f.u * (uu - uc) - f.d * (uc - ud))) / cf.hh2;
-        *cout = T(0);
-        return (T)v;
-    } else {
-        float d = cf.c0 * __builtin_fmaf(f.e, ue - uc, -(f.w * (uc - uw)));
-        d = __builtin_fmaf(cf.c1, __builtin_fmaf(f.n, un - uc, -(f.s * (uc - us))), d);
-        d = __builtin_fmaf(cf.c2, __builtin_fmaf(f.u, uu - uc, -(f.d * (uc - ud))), d);
-        if constexpr (MODE == 2) {
-            const float y = d + cin;
-            const float t = uc + y;
-            *cout = y - (t - uc);
-            return t;
-        } else {
-            *cout = 0.0f;
-            return uc + d;
-        }
-    }
-}
 
 // Buffer access to one z plane: a wave-uniform resource (base = the plane's
 // first memAll element, range = the in-plane extent incl. ghost planes below
@@ -287,10 +234,14 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb2_kernel(Geo g, const T* __r
 // F32 701 -> 631, F64 (64x16) 1882 -> 1767 against the branching kernel;
 // SQ counters for F32 + Kahan: vector ALU 0.64 of its issue rate, scalar
 // 0.51 (0.66 / 0.65 before).
-template <typename T, int MODE, int TX, int TY, int D>
+// MO: the states after one and two steps are written too (u1o, u2o; the
+// value part only in the compensated mode), at every node of the output
+// tile, as the one-step kernel would write them.
+template <typename T, int MODE, int TX, int TY, int D, bool MO = false>
 __global__ __launch_bounds__(TX* TY) void heatvar_tb3_kernel(Geo g, const T* __restrict__ u, const T* __restrict__ cin,
                                                               const T* __restrict__ kap, T* __restrict__ uout,
-                                                              T* __restrict__ cout, VarCoef cf, int zc, int tiles_x,
+                                                              T* __restrict__ cout, T* __restrict__ u1o,
+                                                              T* __restrict__ u2o, VarCoef cf, int zc, int tiles_x,
                                                               int tiles_y)
 {
     static_assert(D >= 2 && D % 2 == 0, "prefetch depth must be even");
@@ -417,6 +368,10 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb3_kernel(Geo g, const T* __r
                 c1 = keep ? c0 : T(0);
             }
             s1[buf][ty][tx] = l1;
+            if constexpr (MO) {
+                const uint32_t os1 = (k >= k0 && k < k1 && row_out) ? oo : range;
+                bstore<T>(l1, plane_rsrc((const char*)u1o + plane(k), range), os1);
+            }
             // ---- level 2 at plane k-1 from the level-1 plane written one
             // iteration ago; level 3 reads it on rows 1..TY-2 only (a
             // wave-uniform branch skips the outer rows)
@@ -431,6 +386,10 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb3_kernel(Geo g, const T* __r
                 }
             }
             s2[buf][ty][tx] = l2;
+            if constexpr (MO) {
+                const uint32_t os2 = (k2 >= k0 && k2 < k1 && row_out) ? oo : range;
+                bstore<T>(l2, plane_rsrc((const char*)u2o + plane(k2), range), os2);
+            }
             // ---- level 3 at plane k-2 from the level-2 plane written one iteration ago
             const int k3 = k - 2;
             {
@@ -485,20 +444,6 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb3_kernel(Geo g, const T* __r
 // Per two reverse steps HBM moves lam r+w, kappa, u^{n-1}, u^{n-2} and the
 // FP64 gradient r+w once: 36 B per node for FP32 fields (18 per step,
 // against 32 for one-step launches).
-struct AdjVarCoef {
-    VarCoef v;                // lam update (the forward step's arithmetic)
-    double th0, th1, th2;     // tau / h_p^2
-};
-
-__device__ inline double adj_face_sum(bool lo, bool hi, double lc, double lm, double lp, double uc, double um,
-                                      double up)
-{
-    double sp = 0.0;
-    if (lo) sp += (lc - lm) * (um - uc);
-    if (hi) sp += (lc - lp) * (up - uc);
-    return sp;
-}
-
 template <typename T, int TX, int TY, int D>
 __global__ __launch_bounds__(TX* TY) void heatvar_adj2_kernel(Geo g, const T* __restrict__ lam,
                                                                const T* __restrict__ kap, const T* __restrict__ u1,
@@ -748,9 +693,9 @@ static const TbCfg kTbCfgs[] = {
 };
 static constexpr int kNumTb = sizeof(kTbCfgs) / sizeof(kTbCfgs[0]);
 
-template <int STEPS, typename T, int MODE, int TX, int TY, int D>
+template <int STEPS, typename T, int MODE, int TX, int TY, int D, bool MO = false>
 static int tb_launch_t(Ctx* c, const Geo& g, const void* u, const void* cin, const void* kap, void* uo, void* co,
-                       const VarCoef& cf, int zc)
+                       const VarCoef& cf, int zc, void* const* mid = nullptr)
 {
     constexpr int RING = STEPS - 1;  // ring width of the thread tile on each side
     const int tiles_x = ceil_div(g.nx, TX - 2 * RING), tiles_y = ceil_div(g.ny, TY - 2 * RING);
@@ -761,8 +706,9 @@ static int tb_launch_t(Ctx* c, const Geo& g, const void* u, const void* cin, con
         heatvar_tb2_kernel<T, MODE, TX, TY, D><<<dim3((unsigned)nb), dim3(TX, TY), 0, c->stream>>>(
             g, (const T*)u, (const T*)cin, (const T*)kap, (T*)uo, (T*)co, cf, zc, tiles_x, tiles_y);
     else
-        heatvar_tb3_kernel<T, MODE, TX, TY, D><<<dim3((unsigned)nb), dim3(TX, TY), 0, c->stream>>>(
-            g, (const T*)u, (const T*)cin, (const T*)kap, (T*)uo, (T*)co, cf, zc, tiles_x, tiles_y);
+        heatvar_tb3_kernel<T, MODE, TX, TY, D, MO><<<dim3((unsigned)nb), dim3(TX, TY), 0, c->stream>>>(
+            g, (const T*)u, (const T*)cin, (const T*)kap, (T*)uo, (T*)co, MO ? (T*)mid[0] : nullptr,
+            MO ? (T*)mid[1] : nullptr, cf, zc, tiles_x, tiles_y);
     SCAFES_HIP(hipGetLastError());
     return SCAFES_OK;
 }
@@ -796,6 +742,27 @@ static int tb_dispatch(Ctx* c, const Geo& g, const void* u, const void* cin, con
 #undef SCAFES_TB
 }
 
+// Three-step sweeps that also write the two intermediate states: the
+// default three-step tiles only (F64 64x16, F32 64x12, prefetch depth 2).
+template <typename T, int MODE>
+static int tb_dispatch_mo(Ctx* c, const Geo& g, const void* u, const void* cin, const void* kap, void* uo, void* co,
+                          const VarCoef& cf, int cfg, void* const* mid)
+{
+    const TbCfg t = kTbCfgs[cfg];
+    if (t.steps == 3 && t.tx == 64 && t.ty == 16 && t.d == 2)
+        return tb_launch_t<3, T, MODE, 64, 16, 2, true>(c, g, u, cin, kap, uo, co, cf, t.zc, mid);
+    if (t.steps == 3 && t.tx == 64 && t.ty == 12 && t.d == 2)
+        return tb_launch_t<3, T, MODE, 64, 12, 2, true>(c, g, u, cin, kap, uo, co, cf, t.zc, mid);
+    return fail(SCAFES_ERR_UNSUPPORTED, "no state-writing variant of this multi-step tile");
+}
+
+bool heatvar_tb3_states_supported(int cfg)
+{
+    if (cfg < 0 || cfg >= kNumTb) return false;
+    const TbCfg t = kTbCfgs[cfg];
+    return t.steps == 3 && t.tx == 64 && (t.ty == 16 || t.ty == 12) && t.d == 2;
+}
+
 int heatvar_tb2_num_cfgs() { return kNumTb; }
 
 void heatvar_tb2_cfg(int cfg, int* tx, int* ty, int* zc, int* d, int* steps)
@@ -818,7 +785,7 @@ bool heatvar_tb2_supported(const Geo& geo, int mode)
 // The configuration's number of steps u -> uo (and Kahan compensation cin -> co).
 int heatvar_tb2_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* uo,
                        const void* cin, void* co, const void* kap, double tau, int border_mode, double border_value,
-                       int cfg)
+                       int cfg, void* const* mid)
 {
     if (cfg < 0 || cfg >= kNumTb) return fail(SCAFES_ERR_ARG, "bad multi-step tile configuration");
     if (!heatvar_tb2_supported(geo, mode)) return fail(SCAFES_ERR_UNSUPPORTED, "multi-step sweep needs a 3-D grid");
@@ -834,6 +801,13 @@ int heatvar_tb2_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, c
     cf.c2 = (float)(0.5 * tau / hh[2]);
     cf.border_value = border_value;
     cf.border_mode = border_mode;
+    if (mid) {
+        if (mode == SCAFES_HEATVAR_F64)
+            return tb_dispatch_mo<double, 0>(c, geo, u, nullptr, kap, uo, nullptr, cf, cfg, mid);
+        if (mode == SCAFES_HEATVAR_F32)
+            return tb_dispatch_mo<float, 1>(c, geo, u, nullptr, kap, uo, nullptr, cf, cfg, mid);
+        return tb_dispatch_mo<float, 2>(c, geo, u, cin, kap, uo, co, cf, cfg, mid);
+    }
     if (mode == SCAFES_HEATVAR_F64) return tb_dispatch<double, 0>(c, geo, u, nullptr, kap, uo, nullptr, cf, cfg);
     if (mode == SCAFES_HEATVAR_F32) return tb_dispatch<float, 1>(c, geo, u, nullptr, kap, uo, nullptr, cf, cfg);
     return tb_dispatch<float, 2>(c, geo, u, cin, kap, uo, co, cf, cfg);

@@ -139,6 +139,14 @@ int heat7var_step_one(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, co
 int heat7var_sweep_steps(const Ctx* c, const Geo& geo, int mode);
 int heat7var_sweep(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* un, const void* cin,
                    void* cout, const void* kap, double tau, int border_mode, double border_value);
+// Three steps u -> un (cin -> cout) in one sweep that also writes the
+// states after one and two steps to mid[0], mid[1] (value parts only in the
+// compensated mode), when the grid's three-step tile has such a variant;
+// returns SCAFES_ERR_UNSUPPORTED (nothing launched) otherwise.
+bool heat7var_states_sweep_ok(const Ctx* c, const Geo& geo, int mode);
+int heat7var_sweep_states(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* un,
+                          const void* cin, void* cout, void* const* mid, const void* kap, double tau,
+                          int border_mode, double border_value);
 // Adjoint step of the variable-kappa heat w.r.t. kappa (defined in
 // stencil3d.hip): lam^n -> lam^{n-1}, grad += the step's kappa terms with
 // u^{n-1} = uprev; dtype SCAFES_F64 or SCAFES_F32 (fields), grad FP64.
@@ -156,6 +164,12 @@ int heatvar_adj2_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype,
 // pennes_adjk.hip): lam^n -> lam^{n-K} with t[l] = T^{n-1-l} (+ tlo[l] for
 // compensated states, else tlo = nullptr), l = 0..K-1.
 constexpr int kAdjMaxK = 4;
+// K = 2..kAdjMaxK reverse variable-kappa adjoint steps per sweep (defined in
+// heatvar_adjk.hip): lam^n -> lam^{n-K} with u[l] = u^{n-1-l} (value parts),
+// bit-identical to K heatvar_adj_launch calls.
+bool heatvar_adjk_supported(const Geo& geo);
+int heatvar_adjk_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, int K, const void* lam,
+                        void* lamo, const void* kap, const void* const* u, double* grad, double tau);
 bool pennes_adjk_supported(const Geo& geo);
 int pennes_adjk_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, int K, const void* lam,
                        void* lam_out, const void* w, const void* const* t, const void* const* tlo, double* grad,
@@ -166,9 +180,12 @@ int pennes_adjk_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, 
 int heatvar_tb2_num_cfgs();
 void heatvar_tb2_cfg(int cfg, int* tx, int* ty, int* zc, int* d, int* steps);
 bool heatvar_tb2_supported(const Geo& geo, int mode);  // 3-D grids with 32-bit in-plane offsets
+// mid = nullptr, or the two intermediate state outputs of a three-step
+// configuration with a state-writing variant (heatvar_tb3_states_supported).
 int heatvar_tb2_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* uo,
                        const void* cin, void* co, const void* kap, double tau, int border_mode, double border_value,
-                       int cfg);
+                       int cfg, void* const* mid = nullptr);
+bool heatvar_tb3_states_supported(int cfg);
 
 // Yee box of N cells per axis; fields hold N+1 nodes per axis, x fastest.
 struct YeeDims {

@@ -1332,4 +1332,21 @@ int heat7var_sweep(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const
                               cfg - kNumCfgs);
 }
 
+bool heat7var_states_sweep_ok(const Ctx* c, const Geo& geo, int mode)
+{
+    const int cfg = resolve_heatvar_cfg(c, geo, mode, -1);
+    return cfg >= kNumCfgs && heatvar_tb3_states_supported(cfg - kNumCfgs);
+}
+
+int heat7var_sweep_states(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* un,
+                          const void* cin, void* cout, void* const* mid, const void* kap, double tau,
+                          int border_mode, double border_value)
+{
+    const int cfg = resolve_heatvar_cfg(c, geo, mode, -1);
+    if (cfg < kNumCfgs || !heatvar_tb3_states_supported(cfg - kNumCfgs))
+        return fail(SCAFES_ERR_UNSUPPORTED, "no state-writing three-step sweep for this grid");
+    return heatvar_tb2_launch(c, geo, g, mode, u, un, cin, cout, kap, tau, border_mode, border_value,
+                              cfg - kNumCfgs, mid);
+}
+
 }  // namespace scafes
