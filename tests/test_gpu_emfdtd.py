@@ -5,9 +5,10 @@ differ in sign only: numpy's array_equal treats -0.0 == 0.0).
 
 Cases: the reference's small configuration (testcase_small.ini: d = 5,
 source 10x10x10, sink 10x10x14, a conductive sphere), a deeper PML with a
-modulated source, a run split over two calls, a state whose conduction
-current is nonzero in vacuum (the general E path), and a non-cubic-looking
-off-centre source near the PML.
+modulated source, a run split over two calls, single-step calls, calls of
+settled steps only (the out-of-place sweeps alone, ragged tiles, several z
+chunks), a state whose conduction current is nonzero in vacuum (the general
+E path), and a non-cubic-looking off-centre source near the PML.
 """
 import ctypes as C
 
@@ -96,6 +97,28 @@ def test_emfdtd_settled_flags(ctx):
         capi.check(ctx.L.scafes_emfdtd_run(ctx.h, C.byref(p), dev.ptr, 1, float(s), sp))
     got = dev.read().reshape(state.shape)
     rs = ref.run(30)
+    _compare(got, ref.f, sink.read(), rs)
+    dev.free()
+    sink.free()
+
+
+@pytest.mark.parametrize("n,nsteps", [(26, 10), (71, 7), (130, 4)])
+def test_emfdtd_settled_out_of_place_steps(ctx, n, nsteps):
+    """A first full-psi step, then one call with PSI_SETTLED | VACUUM_OK:
+    an even count runs only out-of-place sweeps (state -> workspace ->
+    state), an odd count one in-place step before them; ragged x/y tiles
+    (63 x 7 owned nodes per workgroup) and several 64-plane z chunks; equal
+    to the oracle."""
+    c = n // 2
+    ref, p = _setup(n, 4, (c - 3, c, c), (c + 4, c, c), ((c + 2, c - 1, c), 3.0))
+    state = ref.f.copy()
+    dev = ctx.upload(state)
+    sink = ctx.alloc(nsteps + 1, np.float64)
+    capi.check(ctx.L.scafes_emfdtd_run(ctx.h, C.byref(p), dev.ptr, 1, 0.0, sink.ptr))
+    p.flags = capi.EMFDTD_PSI_SETTLED | capi.EMFDTD_VACUUM_OK
+    capi.check(ctx.L.scafes_emfdtd_run(ctx.h, C.byref(p), dev.ptr, nsteps, 1.0, C.c_void_p(sink.ptr.value + 8)))
+    got = dev.read().reshape(state.shape)
+    rs = ref.run(nsteps + 1)
     _compare(got, ref.f, sink.read(), rs)
     dev.free()
     sink.free()
