@@ -81,10 +81,6 @@ struct DevReal {
     SCAFES_HD DevReal operator+() const { return *this; }
 };
 
-template <typename T>
-struct IsDevReal : std::false_type {};
-template <typename T>
-struct IsDevReal<DevReal<T>> : std::true_type {};
 template <typename U>
 using EnableArith = typename std::enable_if<std::is_arithmetic<U>::value, int>::type;
 
