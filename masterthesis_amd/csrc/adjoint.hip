@@ -223,7 +223,51 @@ struct Rev {
         return r;
     }
     // Reverse steps a+1..b given T^a in `base`; s snapshots may be taken.
-    int treeverse(long a, long b, int s, const void* base)
+    int treeverse(long a, long b, int s, const void* base);
+
+    // Where snapshots are plentiful the binomial split takes one step off
+    // the left again and again: each time T^{a+1} goes into a snapshot, the
+    // right part is reversed with one snapshot less, then step a+1 alone.
+    // The same run of splits is taken here in one go (the same forward steps
+    // and snapshots); the peeled states stay in their snapshots while the
+    // right part is reversed, and their steps are then reversed together,
+    // up to kAdjMaxK per sweep (bit-identical to single steps).
+    int peel_run(long a, long b, int s, const void* base)
+    {
+        std::vector<const void*> chain(1, base);  // chain[q] = T^{a+q}
+        long aa = a;
+        int ss = s;
+        int r = SCAFES_OK;
+        while (!r && b - aa > 1 && ss > 0 && !slots.empty() && right_part(b - aa, ss) == b - aa - 1 &&
+               !(ss >= b - aa - 1 && (long)slots.size() >= b - aa - 1)) {
+            void* sl = slots.back();
+            slots.pop_back();
+            r = advance(chain.back(), 1, sl);
+            chain.push_back(sl);
+            ++aa;
+            --ss;
+        }
+        if (!r) r = treeverse(aa, b, ss, chain.back());
+        for (long n = aa; n > a && !r;) {
+            if (n == nsteps || n - a < 2) {
+                r = adjoint((int)n, chain[n - 1 - a]);
+                n -= 1;
+            } else {
+                const int kmax = family == FAMILY_HEATVAR && !heatvar_adjk_supported(geo) ? 2 : kAdjMaxK;
+                const int K = (int)std::min<long>(kmax, n - a);
+                const void* sk[kAdjMaxK];
+                for (int l = 0; l < K; ++l) sk[l] = chain[n - 1 - l - a];
+                r = adjointk(K, sk);
+                n -= K;
+            }
+        }
+        for (size_t q = chain.size() - 1; q >= 1; --q) slots.push_back(const_cast<void*>(chain[q]));
+        return r;
+    }
+};
+
+int Rev::treeverse(long a, long b, int s, const void* base)
+{
     {
         if (b - a == 1) return adjoint((int)b, base);
         if (pair && s >= b - a - 1 && (long)slots.size() >= b - a - 1) {
@@ -287,6 +331,7 @@ struct Rev {
             }
             return SCAFES_OK;
         }
+        if (pair && right_part(b - a, s) == b - a - 1) return peel_run(a, b, s, base);
         const long right = right_part(b - a, s);
         const long m = b - right;
         void* slot = slots.back();
@@ -297,7 +342,7 @@ struct Rev {
         if (r) return r;
         return treeverse(a, m, s, base);
     }
-};
+}
 
 // Counts the forward steps of the schedule without running it.
 static long count_forward(long a, long b, int s, int nsteps)
