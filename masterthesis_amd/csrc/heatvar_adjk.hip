@@ -46,19 +46,6 @@ struct AdjkStates {
     const void* u[kAdjMaxK];  // u[l] = u^{n-1-l}
 };
 
-// sp = 0; sp += (lc - lm)(um - uc) where the minus neighbour exists; sp +=
-// (lc - lp)(up - uc) where the plus neighbour exists (adj_face_sum's order;
-// adding 0.0 instead of skipping a term leaves sp unchanged, as sp is never
-// -0.0 after 0.0 + x).
-__device__ inline double face_sum_sel(bool lo, bool hi, double lc, double lm, double lp, double uc, double um,
-                                      double up)
-{
-    double sp = 0.0;
-    sp += lo ? (lc - lm) * (um - uc) : 0.0;
-    sp += hi ? (lc - lp) * (up - uc) : 0.0;
-    return sp;
-}
-
 template <typename T, int TX, int TY, int K, int D>
 __global__ __launch_bounds__(TX* TY) void heatvar_adjk_kernel(Geo g, const T* __restrict__ lam,
                                                                T* __restrict__ lamo, const T* __restrict__ kap,
@@ -233,13 +220,9 @@ __global__ __launch_bounds__(TX* TY) void heatvar_adjk_kernel(Geo g, const T* __
             }
             // ---- step-n term at plane k (output rows and planes)
             if (row_out && k >= k0 && k < k1) {  // wave-uniform
-                double t = 0.0;
-                t += cf.th0 * face_sum_sel(xlo, xhi, (double)ac, (double)lw, (double)le, (double)uc[0],
-                                           (double)su[0][buf][ty][txm], (double)su[0][buf][ty][txp]);
-                t += cf.th1 * face_sum_sel(ylo, yhi, (double)ac, (double)ls, (double)ln, (double)uc[0],
-                                           (double)su[0][buf][tym][tx], (double)su[0][buf][typ][tx]);
-                t += cf.th2 * face_sum_sel(k > 0, k < g.nz - 1, (double)ac, (double)am, (double)ap, (double)uc[0],
-                                           (double)um[0], (double)up[0]);
+                const double t = adj_term3<T>(cf, xlo, xhi, ylo, yhi, k > 0, k < g.nz - 1, ac, lw, le, ls, ln, am,
+                                              ap, uc[0], su[0][buf][ty][txm], su[0][buf][ty][txp],
+                                              su[0][buf][tym][tx], su[0][buf][typ][tx], um[0], up[0]);
                 sum[ring(u)] = gl + 0.5 * t;
             }
             // ---- levels 2..K: lam^{n-L} at plane k-(L-1), and the step-(n-L+1) term there
@@ -259,15 +242,11 @@ __global__ __launch_bounds__(TX* TY) void heatvar_adjk_kernel(Geo g, const T* __
                 if (row_out && p >= k0 && p < k1) {  // wave-uniform
                     // level L-1 (lam^{n-L+1}) at plane p with u^{n-L} = state L-1
                     const int l = L - 1;
-                    double t = 0.0;
-                    t += cf.th0 * face_sum_sel(xlo, xhi, (double)lc, (double)sl[l - 1][pbf][ty][txm],
-                                               (double)sl[l - 1][pbf][ty][txp], (double)uc[l],
-                                               (double)su[l][buf][ty][txm], (double)su[l][buf][ty][txp]);
-                    t += cf.th1 * face_sum_sel(ylo, yhi, (double)lc, (double)sl[l - 1][pbf][tym][tx],
-                                               (double)sl[l - 1][pbf][typ][tx], (double)uc[l],
-                                               (double)su[l][buf][tym][tx], (double)su[l][buf][typ][tx]);
-                    t += cf.th2 * face_sum_sel(p > 0, p < g.nz - 1, (double)lc, (double)lm, (double)lp,
-                                               (double)uc[l], (double)um[l], (double)up[l]);
+                    const double t =
+                        adj_term3<T>(cf, xlo, xhi, ylo, yhi, p > 0, p < g.nz - 1, lc, sl[l - 1][pbf][ty][txm],
+                                     sl[l - 1][pbf][ty][txp], sl[l - 1][pbf][tym][tx], sl[l - 1][pbf][typ][tx], lm,
+                                     lp, uc[l], su[l][buf][ty][txm], su[l][buf][ty][txp], su[l][buf][tym][tx],
+                                     su[l][buf][typ][tx], um[l], up[l]);
                     sum[ring(u - l)] = sum[ring(u - l)] + 0.5 * t;
                 }
                 if (L < K) {
@@ -291,16 +270,18 @@ __global__ __launch_bounds__(TX* TY) void heatvar_adjk_kernel(Geo g, const T* __
 // About 125 VGPRs per thread (the FP64 gradient terms of K steps in
 // flight) allow one workgroup per CU; the sweeps are bound by the FP64
 // gradient arithmetic (the same per step at every K), not by HBM.
-template <int K>
+// FP64 fields: 64 x 8 (a 1024-thread workgroup caps a thread at 128 VGPRs,
+// which the FP64 rings exceed).
+template <typename T, int K>
 struct AdjkTile {
-    static constexpr int TX = 64, TY = 16, D = 2, ZC = 32;
+    static constexpr int TX = 64, TY = sizeof(T) == 8 ? 8 : 16, D = 2, ZC = 32;
 };
 
 template <typename T, int K>
 int adjk_launch_t(Ctx* c, const Geo& g, const void* lam, void* lamo, const void* kap, const AdjkStates& st,
                   double* grad, const AdjVarCoef& cf)
 {
-    typedef AdjkTile<K> A;
+    typedef AdjkTile<T, K> A;
     constexpr int R = K - 1;
     const int tiles_x = ceil_div(g.nx, A::TX - 2 * R), tiles_y = ceil_div(g.ny, A::TY - 2 * R);
     const long nb = (long)tiles_x * tiles_y * ceil_div(g.nz, A::ZC);
@@ -360,6 +341,9 @@ int heatvar_adjk_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype,
     cf.th0 = tau / hh[0];
     cf.th1 = tau / hh[1];
     cf.th2 = tau / hh[2];
+    cf.f0 = (float)cf.th0;
+    cf.f1 = (float)cf.th1;
+    cf.f2 = (float)cf.th2;
     if (dtype == SCAFES_F64) return adjk_launch_k<double>(c, geo, K, lam, lamo, kap, st, grad, cf);
     return adjk_launch_k<float>(c, geo, K, lam, lamo, kap, st, grad, cf);
 }

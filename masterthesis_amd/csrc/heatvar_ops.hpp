@@ -66,6 +66,7 @@ __device__ inline T var_update(const VarCoef& cf, T uc, T uw, T ue, T us, T un, 
 struct AdjVarCoef {
     VarCoef v;                // lam update (the forward step's arithmetic)
     double th0, th1, th2;     // tau / h_p^2
+    float f0, f1, f2;         // the same, rounded to FP32
 };
 
 __device__ inline double adj_face_sum(bool lo, bool hi, double lc, double lm, double lp, double uc, double um,
@@ -75,6 +76,39 @@ __device__ inline double adj_face_sum(bool lo, bool hi, double lc, double lm, do
     if (lo) sp += (lc - lm) * (um - uc);
     if (hi) sp += (lc - lp) * (up - uc);
     return sp;
+}
+
+// The same face sum in FP32 (FP32 fields).
+__device__ inline float adj_face_sum_f(bool lo, bool hi, float lc, float lm, float lp, float uc, float um, float up)
+{
+    const float a = lo ? (lc - lm) * (um - uc) : 0.0f;
+    const float b = hi ? (lc - lp) * (up - uc) : 0.0f;
+    return a + b;
+}
+
+// Kappa-gradient term of one adjoint step at a node of a 3-D grid,
+//   t = sum_p th_p * sum_{nb = -+e_p in the grid} (lam_c - lam_nb)(u_nb - u_c);
+// the gradient adds 0.5 t in FP64.  FP64 fields: t in FP64 in the oracle's
+// order.  FP32 fields: t in FP32 from the FP32 values, th_p rounded to FP32
+// and the axes joined with fmaf (the one-step operator in stencil3d.hip uses
+// the same arithmetic, so every schedule gives the same gradient).
+template <typename T>
+__device__ inline double adj_term3(const AdjVarCoef& cf, bool xlo, bool xhi, bool ylo, bool yhi, bool zlo, bool zhi,
+                                   T lc, T lw, T le, T ls, T ln, T ld, T lu, T uc, T uw, T ue, T us, T un, T ud,
+                                   T uu)
+{
+    if constexpr (sizeof(T) == 8) {
+        double t = 0.0;
+        t += cf.th0 * adj_face_sum(xlo, xhi, lc, lw, le, uc, uw, ue);
+        t += cf.th1 * adj_face_sum(ylo, yhi, lc, ls, ln, uc, us, un);
+        t += cf.th2 * adj_face_sum(zlo, zhi, lc, ld, lu, uc, ud, uu);
+        return t;
+    } else {
+        float t = cf.f0 * adj_face_sum_f(xlo, xhi, lc, lw, le, uc, uw, ue);
+        t = __builtin_fmaf(cf.f1, adj_face_sum_f(ylo, yhi, lc, ls, ln, uc, us, un), t);
+        t = __builtin_fmaf(cf.f2, adj_face_sum_f(zlo, zhi, lc, ld, lu, uc, ud, uu), t);
+        return (double)t;
+    }
 }
 
 }  // namespace scafes

@@ -578,12 +578,9 @@ __global__ __launch_bounds__(TX* TY) void heatvar_adj2_kernel(Geo g, const T* __
             double t1 = 0.0;
             if (out_thread) {
                 const int txm = tx - 1, txp = tx + 1, tym = ty - 1, typ = ty + 1;
-                t1 += cf.th0 * adj_face_sum(i > 0, i < g.nx - 1, (double)ac, (double)lw, (double)le, (double)uc,
-                                            (double)sa[buf][ty][txm], (double)sa[buf][ty][txp]);
-                t1 += cf.th1 * adj_face_sum(j > 0, j < g.ny - 1, (double)ac, (double)ls, (double)ln, (double)uc,
-                                            (double)sa[buf][tym][tx], (double)sa[buf][typ][tx]);
-                t1 += cf.th2 * adj_face_sum(k > 0, k < g.nz - 1, (double)ac, (double)am, (double)ap, (double)uc,
-                                            (double)u1m, (double)up);
+                t1 = adj_term3<T>(cf, i > 0, i < g.nx - 1, j > 0, j < g.ny - 1, k > 0, k < g.nz - 1, ac, lw, le, ls,
+                                  ln, am, ap, uc, sa[buf][ty][txm], sa[buf][ty][txp], sa[buf][tym][tx],
+                                  sa[buf][typ][tx], u1m, up);
             }
             // ---- level 2 at (i, j, k-1): lam^{n-2} and the step-(n-1) term
             const int k2 = k - 1;
@@ -597,15 +594,10 @@ __global__ __launch_bounds__(TX* TY) void heatvar_adj2_kernel(Geo g, const T* __
                     T dummy2;
                     v = var_update<T, MODE>(cf.v, l1m1, pw, pe, ps, pn, l1m2, l1, fp, T(0), &dummy2);
                     if (!(inner_xy && k2 >= klo && k2 <= khi)) v = T(0);
-                    double t2 = 0.0;
-                    t2 += cf.th0 * adj_face_sum(i > 0, i < g.nx - 1, (double)l1m1, (double)pw, (double)pe,
-                                                (double)u2m1, (double)sb[pbf][ty][tx - 1],
-                                                (double)sb[pbf][ty][tx + 1]);
-                    t2 += cf.th1 * adj_face_sum(j > 0, j < g.ny - 1, (double)l1m1, (double)ps, (double)pn,
-                                                (double)u2m1, (double)sb[pbf][ty - 1][tx],
-                                                (double)sb[pbf][ty + 1][tx]);
-                    t2 += cf.th2 * adj_face_sum(k2 > 0, k2 < g.nz - 1, (double)l1m1, (double)l1m2, (double)l1,
-                                                (double)u2m1, (double)u2m2, (double)vc);
+                    const double t2 = adj_term3<T>(cf, i > 0, i < g.nx - 1, j > 0, j < g.ny - 1, k2 > 0,
+                                                   k2 < g.nz - 1, l1m1, pw, pe, ps, pn, l1m2, l1, u2m1,
+                                                   sb[pbf][ty][tx - 1], sb[pbf][ty][tx + 1], sb[pbf][ty - 1][tx],
+                                                   sb[pbf][ty + 1][tx], u2m2, vc);
                     gv = gm1 + 0.5 * t1m1;
                     gv = gv + 0.5 * t2;
                 }
@@ -675,6 +667,9 @@ int heatvar_adj2_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype,
     cf.th0 = tau / hh[0];
     cf.th1 = tau / hh[1];
     cf.th2 = tau / hh[2];
+    cf.f0 = (float)cf.th0;
+    cf.f1 = (float)cf.th1;
+    cf.f2 = (float)cf.th2;
     if (dtype == SCAFES_F64) return adj2_launch_t<double>(c, geo, lam, lamo, kap, u1, u2, grad, cf);
     return adj2_launch_t<float>(c, geo, lam, lamo, kap, u1, u2, grad, cf);
 }

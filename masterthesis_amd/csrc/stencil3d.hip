@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "heatvar_ops.hpp"
 #include "scafes_common.hpp"
 
 namespace scafes {
@@ -483,7 +484,8 @@ __global__ void mri_coef_kernel(Geo g, Geo gk, const double* __restrict__ rho, c
 // neighbours), th_p = tau/h_p^2; lam^{n-1} = HeatVarOp's update applied to
 // lam on inner nodes (the operator is symmetric), 0 on border nodes.  FP64
 // fields: the oracle's operation order; FP32 fields: the FP32 update of
-// HeatVarOp and the gradient terms in FP64 from the FP32 values.
+// HeatVarOp and each step's gradient term in FP32, added to the FP64
+// gradient.
 template <typename T>
 struct HeatVarAdjOp {
     static constexpr bool kTwo = true;
@@ -497,6 +499,7 @@ struct HeatVarAdjOp {
     double* grad;
     double tau, hh0, hh1, hh2, th0, th1, th2;  // th_p = tau / hh_p
     float c0, c1, c2;                          // 0.5*tau/h_p^2 (FP32 update)
+    float f0, f1, f2;                          // th_p rounded to FP32 (FP32 gradient terms)
     __device__ static double face_sum(bool lo, bool hi, double lc, double lm, double lp, double uc, double um,
                                       double up)
     {
@@ -510,14 +513,22 @@ struct HeatVarAdjOp {
     {
         const long m = at(g, i, j, k);
         double t = 0.0;
-        t += th0 * face_sum(i > 0, i < g.nx - 1, (double)l.c, (double)l.w, (double)l.e, (double)u.c, (double)u.w,
-                            (double)u.e);
-        if (g.dim > 1)
-            t += th1 * face_sum(j > 0, j < g.ny - 1, (double)l.c, (double)l.s, (double)l.n, (double)u.c,
-                                (double)u.s, (double)u.n);
-        if (g.dim > 2)
-            t += th2 * face_sum(k > 0, k < g.nz - 1, (double)l.c, (double)l.d, (double)l.u, (double)u.c,
-                                (double)u.d, (double)u.u);
+        if constexpr (sizeof(T) == 8) {
+            t += th0 * face_sum(i > 0, i < g.nx - 1, (double)l.c, (double)l.w, (double)l.e, (double)u.c,
+                                (double)u.w, (double)u.e);
+            if (g.dim > 1)
+                t += th1 * face_sum(j > 0, j < g.ny - 1, (double)l.c, (double)l.s, (double)l.n, (double)u.c,
+                                    (double)u.s, (double)u.n);
+            if (g.dim > 2)
+                t += th2 * face_sum(k > 0, k < g.nz - 1, (double)l.c, (double)l.d, (double)l.u, (double)u.c,
+                                    (double)u.d, (double)u.u);
+        } else {
+            // FP32 fields: the term in FP32 (adj_term3's arithmetic in heatvar_ops.hpp)
+            float tf = f0 * adj_face_sum_f(i > 0, i < g.nx - 1, l.c, l.w, l.e, u.c, u.w, u.e);
+            if (g.dim > 1) tf = __builtin_fmaf(f1, adj_face_sum_f(j > 0, j < g.ny - 1, l.c, l.s, l.n, u.c, u.s, u.n), tf);
+            if (g.dim > 2) tf = __builtin_fmaf(f2, adj_face_sum_f(k > 0, k < g.nz - 1, l.c, l.d, l.u, u.c, u.d, u.u), tf);
+            t = (double)tf;
+        }
         grad[m] += 0.5 * t;
         if (is_border(g, i, j, k)) {
             out[m] = T(0);
@@ -792,6 +803,9 @@ int heatvar_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, 
         op.th0 = tau / hh[0];
         op.th1 = tau / hh[1];
         op.th2 = tau / hh[2];
+        op.f0 = (float)op.th0;
+        op.f1 = (float)op.th1;
+        op.f2 = (float)op.th2;
         op.c0 = (float)(0.5 * tau / hh[0]);
         op.c1 = (float)(0.5 * tau / hh[1]);
         op.c2 = (float)(0.5 * tau / hh[2]);
