@@ -106,6 +106,7 @@ struct Rev {
     int lcur;
     int nsteps;
     bool pair;  // several adjoint steps per sweep where their states are at hand
+    bool lam_ready = false;  // lam^N and J set (first_leaf)
     long fwd_steps, adj_steps;
     double J;
 
@@ -172,8 +173,25 @@ struct Rev {
             lcur = 1 - lcur;
             return r;
         }
-        if (n == nsteps) {
-            // first leaf: T^N from T^{N-1}, then lam^N and J
+        if (n == nsteps && !lam_ready) {
+            const int r = first_leaf(tprev);
+            if (r) return r;
+        }
+        ++adj_steps;
+        // the var-kappa adjoint reads the forward state's hi part only
+        int r = family == FAMILY_HEATVAR
+                    ? heatvar_adj_launch(c, geo, g, lam_dtype, lam[lcur], lam[1 - lcur], kappa, tprev, grad, tau)
+                    : pennes_adj_launch(c, geo, g, lam_dtype, lam[lcur], lam[1 - lcur], w, tprev, grad, p,
+                                        lo(tprev));
+        lcur = 1 - lcur;
+        return r;
+    }
+    // T^N from T^{N-1} = tprev, then lam^N and J (the Pennes and
+    // variable-kappa families): after it the reverse steps from N on may run
+    // one or several per sweep.
+    int first_leaf(const void* tprev)
+    {
+        {
             int r = step(tprev, tmp[0]);
             if (r) return r;
             const long total = (long)geo.nx * geo.ny * geo.nz;
@@ -191,17 +209,11 @@ struct Rev {
             if (r) return r;
             J *= 0.5;
         }
-        ++adj_steps;
-        // the var-kappa adjoint reads the forward state's hi part only
-        int r = family == FAMILY_HEATVAR
-                    ? heatvar_adj_launch(c, geo, g, lam_dtype, lam[lcur], lam[1 - lcur], kappa, tprev, grad, tau)
-                    : pennes_adj_launch(c, geo, g, lam_dtype, lam[lcur], lam[1 - lcur], w, tprev, grad, p,
-                                        lo(tprev));
-        lcur = 1 - lcur;
-        return r;
+        lam_ready = true;
+        return SCAFES_OK;
     }
     // Adjoint steps n, n-1, ..., n-K+1 in one sweep (lam^n -> lam^{n-K})
-    // with st[l] = T^{n-1-l}; n < nsteps.
+    // with st[l] = T^{n-1-l}; lam^n is set (first_leaf for n = nsteps).
     int adjointk(int K, const void* const* st)
     {
         adj_steps += K;
@@ -249,7 +261,11 @@ struct Rev {
         }
         if (!r) r = treeverse(aa, b, ss, chain.back());
         for (long n = aa; n > a && !r;) {
-            if (n == nsteps || n - a < 2) {
+            if (n == nsteps && !lam_ready) {
+                r = first_leaf(chain[n - 1 - a]);
+                if (r) break;
+            }
+            if (n - a < 2) {
                 r = adjoint((int)n, chain[n - 1 - a]);
                 n -= 1;
             } else {
@@ -303,7 +319,11 @@ int Rev::treeverse(long a, long b, int s, const void* base)
                 for (int q = 0; q < 3; ++q) st.push_back(s3[q]);
             }
             for (long n = b; n > a && !r;) {
-                if (n == nsteps || n - a < 2) {
+                if (n == nsteps && !lam_ready) {
+                    r = first_leaf(st[n - 1 - a]);
+                    if (r) break;
+                }
+                if (n - a < 2) {
                     r = adjoint((int)n, st[n - 1 - a]);
                     n -= 1;
                 } else {
