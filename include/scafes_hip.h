@@ -249,7 +249,7 @@ int scafes_coef_changed(scafes_ctx* ctx, const void* p);
  * a node with index 0 (Neumann)      ((E + A*q_bc) + E) - 2T,
  * other axes central; then perfusion and source as on inner nodes.
  * dtype F64: bit-identical to the FP64 CPU path; F32: FP32 stream.  The
- * adjoint and gradient entry points require border_mode KEEP. */
+ * adjoint and gradient entry points take KEEP and TUMOR. */
 typedef struct {
     double tau, cdiff, cperf, csrc, t_a;
     int border_mode;                      /* SCAFES_BORDER_KEEP or SCAFES_BORDER_TUMOR */
@@ -271,13 +271,21 @@ int scafes_pennes_step_kahan(scafes_ctx* ctx, const scafes_geom* g, int hw, cons
 /* Discrete adjoint step (replaces ADOL-C fov_forward, Problem.hpp:6632-6747):
  *   lam'_i = lam_i + sum_p tau*cdiff*(lam_W+lam_E-2lam_i)/h_p^2 - tau*cperf*w_i*lam_i
  *   grad_i += lam_i * tau*cperf*(T_a - t_prev_i)          (inner nodes; FP64 grad)
- * lam' = 0 on border nodes. */
+ * lam' = 0 on border nodes (KEEP).  TUMOR: every node is updated with the
+ * transpose of the Tumor step's linear part: on axis p, lam_W counts twice
+ * where the W neighbour has index 0 and lam_E twice where the E neighbour
+ * has index n_p - 1, a neighbour beyond the grid counts 0, and a node on
+ * the far face of the last axis adds -(2 h_p/k_cond)*h_conv*lam_i (Cauchy);
+ * the gradient term is taken on every node. */
 int scafes_pennes_adjoint_step(scafes_ctx* ctx, const scafes_geom* g, int hw, int dtype,
                                const void* lam, void* lam_new, const void* w,
                                const void* t_prev, double* grad, const scafes_pennes_params* p);
 /* Gradient of J = 0.5 sum_inner (T^N - target)^2 w.r.t. the perfusion field
  * w by the reverse-time adjoint with binomial checkpointing (Griewank's
  * treeverse schedule) using at most max_snapshots stored states.
+ * border_mode TUMOR (the Tumor example's Neumann/Cauchy borders): J sums
+ * over every node, since every node is updated, and the gradient covers
+ * every node; its reverse steps run one per sweep.
  * dtype F64, F32, or F32_KAHAN (FP32 fields, compensated forward whose
  * checkpoints store (hi, lo) pairs; the adjoint field is plain FP32).  For
  * FP32 modes t0, w, q, target are FP32 (t0 without compensation).

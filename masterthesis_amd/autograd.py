@@ -3,7 +3,9 @@
 PennesMisfit: J(w) = 0.5 * sum_inner (T^N(w) - target)^2 for the Pennes
 bioheat forward model with perfusion field w; backward returns dJ/dw from
 the reverse-time adjoint with binomial checkpointing (scafes_pennes_gradient).
-The FP32 mode streams the deviation theta = T - T_a on the device.
+The FP32 mode streams the deviation theta = T - T_a on the device.  With the
+Tumor example's Neumann/Cauchy borders (TumorHeatEqnFDM.hpp:272-349) every
+node is updated, and J and dJ/dw cover every node.
 
 KappaMisfit: J(kappa) = 0.5 * sum_inner (u^N(kappa) - target)^2 for the
 variable-kappa heat; backward returns dJ/dkappa (scafes_heat7var_gradient).
@@ -39,8 +41,12 @@ class _PennesJ(torch.autograd.Function):
 class PennesMisfit:
     """Callable J(w, T0, Q, target) with an adjoint backward pass."""
 
-    def __init__(self, context, n, tau, cdiff, cperf, csrc, t_a, nsteps, snapshots=16, dtype="f32_kahan"):
-        """dtype: "f64", "f32" or "f32_kahan" (FP32 fields with a compensated forward)."""
+    def __init__(self, context, n, tau, cdiff, cperf, csrc, t_a, nsteps, snapshots=16, dtype="f32_kahan",
+                 tumor=None, h=None):
+        """dtype: "f64", "f32" or "f32_kahan" (FP32 fields with a compensated forward).
+        tumor: None (border nodes keep their values) or (k_cond, h_conv, t_inf,
+        q_bc) of the Tumor example's borders.  h: grid spacing per axis
+        (default 1/(n_p - 1))."""
         self.ctx = context
         self.n = tuple(n)
         self.nsteps = int(nsteps)
@@ -50,7 +56,12 @@ class PennesMisfit:
         self.t_a = float(t_a)
         self.shift = 0.0 if self.f64 else self.t_a
         self.params = capi.PennesParams(tau=tau, cdiff=cdiff, cperf=cperf, csrc=csrc, t_a=self.t_a - self.shift)
-        self.geom = capi.geom(self.n)
+        if tumor is not None:
+            k_cond, h_conv, t_inf, q_bc = (float(v) for v in tumor)
+            self.params.border_mode = capi.BORDER_TUMOR
+            self.params.k_cond, self.params.h_conv, self.params.q_bc = k_cond, h_conv, q_bc
+            self.params.t_inf = t_inf - self.shift
+        self.geom = capi.geom(self.n, h=h)
         self.last_stats = None
 
     def _run(self, w, t0, q, target):

@@ -51,7 +51,7 @@ static long right_part(long l, int s)
 template <typename T>
 __global__ void lam_init_kernel(Geo g, const T* __restrict__ t, const T* __restrict__ tlo,
                                 const T* __restrict__ target, T* __restrict__ lam, T* __restrict__ lam2,
-                                double* __restrict__ grad)
+                                double* __restrict__ grad, int every_node)
 {
     const long total = (long)g.nx * g.ny * g.nz;
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
@@ -61,7 +61,7 @@ __global__ void lam_init_kernel(Geo g, const T* __restrict__ t, const T* __restr
         const long m = at(g, i, j, k);
         double v = (double)t[m] - (double)target[m];
         if (tlo) v += (double)tlo[m];
-        lam[m] = is_border(g, i, j, k) ? T(0) : (T)v;
+        lam[m] = (every_node || !is_border(g, i, j, k)) ? (T)v : T(0);
         lam2[m] = T(0);
         grad[m] = 0.0;
     }
@@ -196,16 +196,19 @@ struct Rev {
             if (r) return r;
             const long total = (long)geo.nx * geo.ny * geo.nz;
             const int blocks = (int)std::min<long>((total + 255) / 256, 65536);
+            // J over the nodes the step updates: every node under the
+            // Tumor border, the inner nodes otherwise
+            const int every = family == FAMILY_PENNES && p->border_mode == SCAFES_BORDER_TUMOR;
             if (lam_dtype == SCAFES_F64)
                 lam_init_kernel<double><<<blocks, 256, 0, c->stream>>>(geo, (const double*)tmp[0], nullptr,
                                                                        (const double*)target, (double*)lam[0],
-                                                                       (double*)lam[1], grad);
+                                                                       (double*)lam[1], grad, every);
             else
                 lam_init_kernel<float><<<blocks, 256, 0, c->stream>>>(geo, (const float*)tmp[0],
                                                                       (const float*)lo(tmp[0]), (const float*)target,
-                                                                      (float*)lam[0], (float*)lam[1], grad);
+                                                                      (float*)lam[0], (float*)lam[1], grad, every);
             SCAFES_HIP(hipGetLastError());
-            r = reduce_nodes(c, geo, geo, lam_dtype, lam[0], nullptr, RED_SUM_SQ_DIFF, 1, &J);
+            r = reduce_nodes(c, geo, geo, lam_dtype, lam[0], nullptr, RED_SUM_SQ_DIFF, every ? 0 : 1, &J);
             if (r) return r;
             J *= 0.5;
         }
@@ -399,7 +402,9 @@ static int run_schedule(Rev& R, Ctx* c, int dtype, const void* t0, size_t abytes
     R.grad = grad;
     R.lcur = 0;
     R.nsteps = nsteps;
-    R.pair = (R.family == FAMILY_PENNES && pennes_adjk_supported(R.geo)) ||
+    // several reverse steps per sweep: the Keep-border Pennes and the
+    // var-kappa adjoints (the Tumor border's adjoint runs one step per sweep)
+    R.pair = (R.family == FAMILY_PENNES && R.p->border_mode == SCAFES_BORDER_KEEP && pennes_adjk_supported(R.geo)) ||
              (R.family == FAMILY_HEATVAR && heatvar_adj2_supported(R.geo));
     R.fwd_steps = R.adj_steps = 0;
     R.J = 0.0;
@@ -498,8 +503,10 @@ int scafes_pennes_gradient(scafes_ctx* ctx, const scafes_geom* g, int hw, int dt
     if (dtype != SCAFES_F32 && dtype != SCAFES_F64 && dtype != SCAFES_F32_KAHAN)
         return fail(SCAFES_ERR_ARG, "pennes gradient: bad dtype");
     if (hw < 1 || g->nlayers < 1) return fail(SCAFES_ERR_ARG, "pennes gradient: need hw >= 1 and nlayers >= 1");
-    if (p->border_mode != SCAFES_BORDER_KEEP)
-        return fail(SCAFES_ERR_UNSUPPORTED, "pennes gradient: only border_mode KEEP has an adjoint");
+    if (p->border_mode != SCAFES_BORDER_KEEP && p->border_mode != SCAFES_BORDER_TUMOR)
+        return fail(SCAFES_ERR_ARG, "pennes gradient: border_mode must be KEEP or TUMOR");
+    if (p->border_mode == SCAFES_BORDER_TUMOR && p->k_cond == 0.0)
+        return fail(SCAFES_ERR_ARG, "pennes gradient: the TUMOR border needs k_cond != 0");
     {
         Geo geo;
         int rc = make_geo(g, hw, &geo);

@@ -564,10 +564,31 @@ struct PennesAdjOp {
     double* grad;
     double ctd, ctp, ta, hh0, hh1, hh2;
     float fd0, fd1, fd2, fp;
+    int tumor;                   // the Tumor border rule (every node updated)
+    double a2h[3], hconv;        // (2 h_p)/K, H
+    // Axis p of the transposed Tumor step at a node with index idx of n:
+    // the row of the neighbour at idx-1 weighs this node twice when that
+    // neighbour lies on the face (index 0), the row of the one at idx+1
+    // twice when it has index n-1, and there is no neighbour beyond the
+    // grid (selected away, so ghost values never enter); on the Cauchy face
+    // (last axis, idx = n-1) the diagonal gets -(2 h/K) H besides -2.
+    __device__ T tumor_adj(int idx, int n, bool last, int p, T lw, T le, T l) const
+    {
+        lw = idx >= 1 ? lw : T(0);
+        le = idx <= n - 2 ? le : T(0);
+        if (idx == 1) lw = lw + lw;
+        if (idx == n - 2) le = le + le;
+        if (last && idx == n - 1) return ((lw + le) - ((T)a2h[p] * (T)hconv) * l) - (T)2 * l;
+        return (lw + le) - (T)2 * l;
+    }
     __device__ void apply(const Geo& g, int i, int j, int k, const St7<T>& s, const St7<T>&,
                           const St7<T>&, const T* xv) const
     {
         const long m = at(g, i, j, k);
+        if (tumor) {
+            apply_tumor(g, i, j, k, m, s, xv);
+            return;
+        }
         if (is_border(g, i, j, k)) {
             out[m] = T(0);
             return;
@@ -586,6 +607,29 @@ struct PennesAdjOp {
             float d = fd0 * ((s.w + s.e) - 2.0f * s.c);
             if (g.dim > 1) d = __builtin_fmaf(fd1, (s.s + s.n) - 2.0f * s.c, d);
             if (g.dim > 2) d = __builtin_fmaf(fd2, (s.d + s.u) - 2.0f * s.c, d);
+            d = __builtin_fmaf(-fp * xv[0], s.c, d);
+            out[m] = s.c + d;
+        }
+    }
+    __device__ void apply_tumor(const Geo& g, int i, int j, int k, long m, const St7<T>& s, const T* xv) const
+    {
+        double tp = (double)xv[1];
+        if constexpr (LO) tp += (double)xv[2];
+        grad[m] += (double)s.c * (ctp * (ta - tp));
+        const T t0 = tumor_adj(i, g.nx, g.dim == 1, 0, s.w, s.e, s.c);
+        const T t1 = g.dim > 1 ? tumor_adj(j, g.ny, g.dim == 2, 1, s.s, s.n, s.c) : T(0);
+        const T t2 = g.dim > 2 ? tumor_adj(k, g.nz, true, 2, s.d, s.u, s.c) : T(0);
+        if constexpr (sizeof(T) == 8) {
+            double v = s.c;
+            v += (ctd * t0) / hh0;
+            if (g.dim > 1) v += (ctd * t1) / hh1;
+            if (g.dim > 2) v += (ctd * t2) / hh2;
+            v -= (ctp * xv[0]) * s.c;
+            out[m] = v;
+        } else {
+            float d = fd0 * t0;
+            if (g.dim > 1) d = __builtin_fmaf(fd1, t1, d);
+            if (g.dim > 2) d = __builtin_fmaf(fd2, t2, d);
             d = __builtin_fmaf(-fp * xv[0], s.c, d);
             out[m] = s.c + d;
         }
@@ -833,10 +877,17 @@ int pennes_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, c
                       void* lam_new, const void* w, const void* tprev, double* grad,
                       const scafes_pennes_params* p, const void* tprev_lo)
 {
-    if (p->border_mode != SCAFES_BORDER_KEEP)
-        return fail(SCAFES_ERR_UNSUPPORTED, "pennes adjoint: only border_mode KEEP has an adjoint");
+    if (p->border_mode != SCAFES_BORDER_KEEP && p->border_mode != SCAFES_BORDER_TUMOR)
+        return fail(SCAFES_ERR_ARG, "pennes adjoint: border_mode must be KEEP or TUMOR");
+    if (p->border_mode == SCAFES_BORDER_TUMOR && p->k_cond == 0.0)
+        return fail(SCAFES_ERR_ARG, "pennes adjoint: the TUMOR border needs k_cond != 0");
     double hh[3];
     hh_of(g, hh);
+    auto tumor = [&](auto& op) {
+        op.tumor = p->border_mode == SCAFES_BORDER_TUMOR ? 1 : 0;
+        for (int q = 0; q < 3; ++q) op.a2h[q] = (2.0 * g->h[q]) / p->k_cond;
+        op.hconv = p->h_conv;
+    };
     if (dtype == SCAFES_F64) {
         PennesAdjOp<double> op{};
         op.a = (const double*)lam;
@@ -850,6 +901,7 @@ int pennes_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, c
         op.hh0 = hh[0];
         op.hh1 = hh[1];
         op.hh2 = hh[2];
+        tumor(op);
         return launch<PennesAdjOp<double>, double>(c, op, geo, -1);
     }
     auto fill32 = [&](auto& op) {
@@ -864,6 +916,7 @@ int pennes_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, c
         op.fd1 = (float)(p->tau * p->cdiff / hh[1]);
         op.fd2 = (float)(p->tau * p->cdiff / hh[2]);
         op.fp = (float)(p->tau * p->cperf);
+        tumor(op);
     };
     if (tprev_lo) {
         PennesAdjOp<float, true> op{};

@@ -733,6 +733,148 @@ double oracle_pennes_grad(const oracle_grid* g, const double* states, const doub
     return J;
 }
 
+/* oracle_pennes_step on node fields w, Q with the Tumor example's border
+ * rule (TumorHeatEqnFDM::updateBorder, TumorHeatEqnFDM.hpp:272-349): every
+ * node is updated; on each axis a node with index n_p - 1 replaces its
+ * missing outer neighbour by the inner one and the Cauchy (last axis) or
+ * Neumann flux term, a node with index 0 by the inner one and the Neumann
+ * term.  The operations and their order are those of oracle_tumor_step, so
+ * the fields of oracle_tumor_fields give its result bit for bit. */
+void oracle_pennes_tumor_step(const oracle_grid* g, const double* to, double* tn, const double* w,
+                              const double* q, const oracle_pennes_coef* c, const oracle_tumor_border* b)
+{
+    memcpy(tn, to, sizeof(double) * (size_t)oracle_nelems(g));
+    double h[3];
+    long st[3];
+    for (int p = 0; p < 3; ++p) {
+        h[p] = p < g->dim ? oracle_gridsize(g, p) : 1.0;
+        st[p] = stride(g, p);
+    }
+    const int n0 = g->n[0], n1 = g->n[1], n2 = g->n[2];
+#pragma omp parallel for collapse(2) schedule(static) num_threads(NTHR)
+    for (int k = 0; k < n2; ++k) {
+        for (int j = 0; j < n1; ++j) {
+            for (int i = 0; i < n0; ++i) {
+                const long m = pos(g, i, j, k);
+                const int idx[3] = {i, j, k};
+                const double u = to[m];
+                double v = u;
+                for (int p = 0; p < g->dim; ++p) {
+                    const double W = to[m - st[p]], E = to[m + st[p]];
+                    double t;
+                    if (idx[p] == g->n[p] - 1) {
+                        if (p == g->dim - 1)
+                            t = W + W - ((2.0 * h[p] / b->k) * b->h * (u - b->t_inf)) - 2.0 * u;
+                        else
+                            t = W + W - ((2.0 * h[p] / b->k) * (-1.0 * b->q_bc)) - 2.0 * u;
+                    } else if (idx[p] == 0) {
+                        t = E + ((2.0 * h[p] / b->k) * b->q_bc) + E - 2.0 * u;
+                    } else {
+                        t = W + E - 2.0 * u;
+                    }
+                    v += c->tau * c->cdiff * t / (h[p] * h[p]);
+                }
+                v += c->tau * c->cperf * w[m] * (c->t_a - u);
+                v += c->tau * c->csrc * q[m];
+                tn[m] = v;
+            }
+        }
+    }
+}
+
+/* Transpose of the linear part of oracle_pennes_tumor_step.  Node m's row
+ * weights its neighbour on axis p by 1, by 2 where m lies on a face (the
+ * inner neighbour stands in for the missing one) and by 0 beyond the grid,
+ * so the column of node i takes lam of the neighbour at index j-1 twice
+ * when that neighbour has index 0, lam of the one at j+1 twice when it has
+ * index n_p - 1, and no term beyond the grid; the diagonal gets the Cauchy
+ * face's -(2 h/K) H besides -2:
+ *   lam'_i = lam_i + sum_p tau*cdiff*t_p/h_p^2 - tau*cperf*w_i*lam_i,
+ *   t_p = (lW + lE) [- (2 h_p/K) H lam_i on the Cauchy face] - 2 lam_i. */
+void oracle_pennes_tumor_adjoint_step(const oracle_grid* g, const double* lam, double* lam_new, const double* w,
+                                      const oracle_pennes_coef* c, const oracle_tumor_border* b)
+{
+    double h[3];
+    long st[3];
+    for (int p = 0; p < 3; ++p) {
+        h[p] = p < g->dim ? oracle_gridsize(g, p) : 1.0;
+        st[p] = stride(g, p);
+    }
+    const int n0 = g->n[0], n1 = g->n[1], n2 = g->n[2];
+#pragma omp parallel for collapse(2) schedule(static) num_threads(NTHR)
+    for (int k = 0; k < n2; ++k) {
+        for (int j = 0; j < n1; ++j) {
+            for (int i = 0; i < n0; ++i) {
+                const long m = pos(g, i, j, k);
+                const int idx[3] = {i, j, k};
+                const double l = lam[m];
+                double v = l;
+                for (int p = 0; p < g->dim; ++p) {
+                    const int x = idx[p], n = g->n[p];
+                    double lw = x >= 1 ? lam[m - st[p]] : 0.0;
+                    double le = x <= n - 2 ? lam[m + st[p]] : 0.0;
+                    if (x == 1) lw = lw + lw;
+                    if (x == n - 2) le = le + le;
+                    double t;
+                    if (x == n - 1 && p == g->dim - 1)
+                        t = (lw + le) - ((2.0 * h[p] / b->k) * b->h) * l - 2.0 * l;
+                    else
+                        t = (lw + le) - 2.0 * l;
+                    v += c->tau * c->cdiff * t / (h[p] * h[p]);
+                }
+                v -= c->tau * c->cperf * w[m] * l;
+                lam_new[m] = v;
+            }
+        }
+    }
+}
+
+void oracle_pennes_tumor_forward_all(const oracle_grid* g, double* states, const double* w, const double* q,
+                                     const oracle_pennes_coef* c, const oracle_tumor_border* b, int nsteps)
+{
+    const long ne = oracle_nelems(g);
+    for (int n = 1; n <= nsteps; ++n)
+        oracle_pennes_tumor_step(g, states + (size_t)(n - 1) * ne, states + (size_t)n * ne, w, q, c, b);
+}
+
+/* J = 0.5 sum over all nodes (T^N - target)^2 (the Tumor border updates
+ * every node) and dJ/dw by the reverse-time discrete adjoint:
+ *   lam^N = T^N - target;  for n = N..1:
+ *   dJ/dw_i += lam^n_i * tau*cperf*(T_a - T^{n-1}_i);  lam^{n-1} = A^T lam^n. */
+double oracle_pennes_tumor_grad(const oracle_grid* g, const double* states, const double* w, const double* target,
+                                const oracle_pennes_coef* c, const oracle_tumor_border* b, int nsteps,
+                                double* grad)
+{
+    const long ne = oracle_nelems(g);
+    double* lam = (double*)calloc((size_t)ne, sizeof(double));
+    double* lam2 = (double*)calloc((size_t)ne, sizeof(double));
+    const double* tN = states + (size_t)nsteps * ne;
+    double J = 0.0;
+    for (long m = 0; m < ne; ++m) grad[m] = 0.0;
+    for (int k = 0; k < g->n[2]; ++k)
+        for (int j = 0; j < g->n[1]; ++j)
+            for (int i = 0; i < g->n[0]; ++i) {
+                const long m = pos(g, i, j, k);
+                const double r = tN[m] - target[m];
+                J += 0.5 * r * r;
+                lam[m] = r;
+            }
+    for (int n = nsteps; n >= 1; --n) {
+        const double* tp = states + (size_t)(n - 1) * ne;
+        for (int k = 0; k < g->n[2]; ++k)
+            for (int j = 0; j < g->n[1]; ++j)
+                for (int i = 0; i < g->n[0]; ++i) {
+                    const long m = pos(g, i, j, k);
+                    grad[m] += lam[m] * (c->tau * c->cperf * (c->t_a - tp[m]));
+                }
+        oracle_pennes_tumor_adjoint_step(g, lam, lam2, w, c, b);
+        double* t = lam; lam = lam2; lam2 = t;
+    }
+    free(lam);
+    free(lam2);
+    return J;
+}
+
 /*---------------------------------------------------------------------------*/
 /* Yee FDTD in a PEC box (normalised units).  The reference's leapfrog mode
  * runs updateInner (H from E) then swaps and runs updateInner2 (E from H)

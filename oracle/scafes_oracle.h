@@ -25,6 +25,11 @@
  *   oracle_pennes_*     pinned: reduction to heat7 (w=0,Q=0), Tumor-form
  *                       closed-form steady state, adjoint vs central
  *                       finite differences and dot-product test.
+ *   oracle_pennes_tumor_* pinned: the Tumor-border step equals
+ *                       oracle_tumor_step bit for bit on the example's
+ *                       fields; adjoint step by the dot-product test
+ *                       against the step's linear part; gradient vs central
+ *                       finite differences.
  *   oracle_yee_*        pinned: exact PEC cavity eigenmode with the Yee
  *                       dispersion relation, discrete energy invariant,
  *                       discrete Gauss law.
@@ -159,6 +164,26 @@ void oracle_pennes_forward_all(const oracle_grid* g, double* states, const doubl
 double oracle_pennes_grad(const oracle_grid* g, const double* states, const double* w,
                           const double* target, const oracle_pennes_coef* c, int nsteps,
                           double* grad);
+
+/* The Tumor example's border constants (TumorHeatEqnFDM.hpp:272-349):
+ * conductivity K, convection coefficient H, ambient T_INF, heat flux Q_BC. */
+typedef struct {
+    double k, h, t_inf, q_bc;
+} oracle_tumor_border;
+
+/* oracle_pennes_step with the Tumor border rule on every border node (all
+ * nodes updated); fields w, Q as for oracle_pennes_step. */
+void oracle_pennes_tumor_step(const oracle_grid* g, const double* t_old, double* t_new, const double* w,
+                              const double* q, const oracle_pennes_coef* c, const oracle_tumor_border* b);
+/* lam_new = A^T lam, A the linear part of oracle_pennes_tumor_step. */
+void oracle_pennes_tumor_adjoint_step(const oracle_grid* g, const double* lam, double* lam_new, const double* w,
+                                      const oracle_pennes_coef* c, const oracle_tumor_border* b);
+void oracle_pennes_tumor_forward_all(const oracle_grid* g, double* states, const double* w, const double* q,
+                                     const oracle_pennes_coef* c, const oracle_tumor_border* b, int nsteps);
+/* J = 0.5*sum_{all nodes}(T^N - target)^2 and dJ/dw (every node). */
+double oracle_pennes_tumor_grad(const oracle_grid* g, const double* states, const double* w, const double* target,
+                                const oracle_pennes_coef* c, const oracle_tumor_border* b, int nsteps,
+                                double* grad);
 
 /* Yee FDTD, vacuum, normalised (c=1, h=1, Courant s=tau/h), PEC box of
  * N cells per axis; fields stored on (N+1)^3 nodes each, no ghost layer.

@@ -107,6 +107,12 @@ def lib():
         L.oracle_pennes_forward_all.argtypes = [gp, dp, dp, dp, C.POINTER(PennesCoef), C.c_int]
         L.oracle_pennes_grad.restype = C.c_double
         L.oracle_pennes_grad.argtypes = [gp, dp, dp, dp, C.POINTER(PennesCoef), C.c_int, dp]
+        tb = C.POINTER(TumorBorder)
+        L.oracle_pennes_tumor_step.argtypes = [gp, dp, dp, dp, dp, C.POINTER(PennesCoef), tb]
+        L.oracle_pennes_tumor_adjoint_step.argtypes = [gp, dp, dp, dp, C.POINTER(PennesCoef), tb]
+        L.oracle_pennes_tumor_forward_all.argtypes = [gp, dp, dp, dp, C.POINTER(PennesCoef), tb, C.c_int]
+        L.oracle_pennes_tumor_grad.restype = C.c_double
+        L.oracle_pennes_tumor_grad.argtypes = [gp, dp, dp, dp, C.POINTER(PennesCoef), tb, C.c_int, dp]
         L.oracle_yee_half_h.argtypes = [C.c_int, C.c_double, dp]
         L.oracle_yee_half_e.argtypes = [C.c_int, C.c_double, dp]
         L.oracle_yee_step.argtypes = [C.c_int, C.c_double, dp]
@@ -282,6 +288,43 @@ def pennes_grad(g, states, w, target, coef, nsteps):
     grad = np.zeros_like(w)
     J = lib().oracle_pennes_grad(C.byref(g), ptr(states), ptr(w), ptr(target), C.byref(coef),
                                  nsteps, ptr(grad))
+    return J, grad
+
+
+class TumorBorder(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("k", "h", "t_inf", "q_bc")]
+
+
+def tumor_border(k, h, t_inf, q_bc):
+    b = TumorBorder()
+    b.k, b.h, b.t_inf, b.q_bc = k, h, t_inf, q_bc
+    return b
+
+
+def pennes_tumor_step(g, u, w, q, coef, border):
+    un = np.empty_like(u)
+    lib().oracle_pennes_tumor_step(C.byref(g), ptr(u), ptr(un), ptr(w), ptr(q), C.byref(coef), C.byref(border))
+    return un
+
+
+def pennes_tumor_adjoint_step(g, lam, w, coef, border):
+    out = np.zeros_like(lam)
+    lib().oracle_pennes_tumor_adjoint_step(C.byref(g), ptr(lam), ptr(out), ptr(w), C.byref(coef), C.byref(border))
+    return out
+
+
+def pennes_tumor_forward_all(g, t0, w, q, coef, border, nsteps):
+    st = np.empty((nsteps + 1,) + t0.shape)
+    st[0] = t0
+    lib().oracle_pennes_tumor_forward_all(C.byref(g), ptr(st), ptr(w), ptr(q), C.byref(coef), C.byref(border),
+                                          nsteps)
+    return st
+
+
+def pennes_tumor_grad(g, states, w, target, coef, border, nsteps):
+    grad = np.zeros_like(w)
+    J = lib().oracle_pennes_tumor_grad(C.byref(g), ptr(states), ptr(w), ptr(target), C.byref(coef),
+                                       C.byref(border), nsteps, ptr(grad))
     return J, grad
 
 
