@@ -267,7 +267,10 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(W))) voi
 // to 207 us (K = 3 157, K = 2 139), scalar instructions per wave 4493 ->
 // 2624.  With the branch-free kernel 64x12 with prefetch depth 2 is still
 // the best tile: 64x16 is 8 % slower, 64x16 limited to 64 VGPRs (8 waves
-// per SIMD, spills) 44 %, prefetch depth 4 16 %, 64x8 24 %.
+// per SIMD, spills) 44 %, prefetch depth 4 16 %, 64x8 24 %.  More steps per
+// sweep do not pay: K = 5 on 64x16 (9.6 B per step, 118 VGPRs, one
+// workgroup per CU) gave the 2000-step gradient with 62 snapshots in 302.6
+// ms against 300.4 for K = 4 on 64x12, and K = 6 (8.7 B per step) 335 ms.
 constexpr int kKTX = 64, kKTY = 12, kKZC = 32, kKD = 2;
 
 template <typename T, bool LO, int K>
