@@ -161,7 +161,7 @@ def _device_grad(ctx, n, h, nlayers, dtype, coef, t0, w, q, target, nsteps, snap
     return J.value, grad, stats
 
 
-@pytest.mark.parametrize("n,nlayers", [((12, 10, 9), 1), ((15, 11), 1), ((13, 9, 8), 2)])
+@pytest.mark.parametrize("n,nlayers", [((12, 10, 9), 1), ((15, 11), 1), ((31,), 1), ((13, 9, 8), 2)])
 @pytest.mark.parametrize("snaps", [0, 2, 5, 29])
 def test_tumor_gradient_fp64_bitexact(ctx, n, nlayers, snaps):
     """dJ/dw through the Tumor border: every checkpoint budget reproduces the
