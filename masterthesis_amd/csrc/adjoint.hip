@@ -46,13 +46,18 @@ static long right_part(long l, int s)
     return std::min(beta(s - 1, r), l - 1);
 }
 
-// lam^N = T^N - target on inner nodes (T^N = hi [+ lo]), 0 elsewhere; the
-// second adjoint buffer and the gradient start at zero.
+// lam^N = T^N - target on inner nodes (T^N = hi [+ lo]), or on every node
+// (every_node), 0 elsewhere; the second adjoint buffer starts at zero (the
+// gradient is cleared before).  partial[block] = the block's FP64 sum of
+// lam^N squared (2 J), to be summed by reduce_partials.
 template <typename T>
-__global__ void lam_init_kernel(Geo g, const T* __restrict__ t, const T* __restrict__ tlo,
-                                const T* __restrict__ target, T* __restrict__ lam, T* __restrict__ lam2,
-                                double* __restrict__ grad, int every_node)
+__global__ __launch_bounds__(256) void lam_init_kernel(Geo g, const T* __restrict__ t, const T* __restrict__ tlo,
+                                                       const T* __restrict__ target, T* __restrict__ lam,
+                                                       T* __restrict__ lam2, int every_node,
+                                                       double* __restrict__ partial)
 {
+    __shared__ double sh[256];
+    double acc = 0.0;
     const long total = (long)g.nx * g.ny * g.nz;
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
         const int i = (int)(q % g.nx);
@@ -61,10 +66,18 @@ __global__ void lam_init_kernel(Geo g, const T* __restrict__ t, const T* __restr
         const long m = at(g, i, j, k);
         double v = (double)t[m] - (double)target[m];
         if (tlo) v += (double)tlo[m];
-        lam[m] = (every_node || !is_border(g, i, j, k)) ? (T)v : T(0);
+        const T l = (every_node || !is_border(g, i, j, k)) ? (T)v : T(0);
+        lam[m] = l;
         lam2[m] = T(0);
-        grad[m] = 0.0;
+        acc += (double)l * (double)l;
     }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
 // A forward state is one array (F32, F64) or a (hi, lo) pair stored as two
@@ -195,20 +208,21 @@ struct Rev {
             int r = step(tprev, tmp[0]);
             if (r) return r;
             const long total = (long)geo.nx * geo.ny * geo.nz;
-            const int blocks = (int)std::min<long>((total + 255) / 256, 65536);
+            const int blocks = (int)std::min<long>((total + 255) / 256, c->red_capacity);
             // J over the nodes the step updates: every node under the
             // Tumor border, the inner nodes otherwise
             const int every = family == FAMILY_PENNES && p->border_mode == SCAFES_BORDER_TUMOR;
             if (lam_dtype == SCAFES_F64)
                 lam_init_kernel<double><<<blocks, 256, 0, c->stream>>>(geo, (const double*)tmp[0], nullptr,
                                                                        (const double*)target, (double*)lam[0],
-                                                                       (double*)lam[1], grad, every);
+                                                                       (double*)lam[1], every, c->red_partial);
             else
                 lam_init_kernel<float><<<blocks, 256, 0, c->stream>>>(geo, (const float*)tmp[0],
                                                                       (const float*)lo(tmp[0]), (const float*)target,
-                                                                      (float*)lam[0], (float*)lam[1], grad, every);
+                                                                      (float*)lam[0], (float*)lam[1], every,
+                                                                      c->red_partial);
             SCAFES_HIP(hipGetLastError());
-            r = reduce_nodes(c, geo, geo, lam_dtype, lam[0], nullptr, RED_SUM_SQ_DIFF, every ? 0 : 1, &J);
+            r = reduce_partials(c, blocks, &J);
             if (r) return r;
             J *= 0.5;
         }

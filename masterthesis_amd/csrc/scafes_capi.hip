@@ -153,6 +153,16 @@ __global__ __launch_bounds__(256) void reduce_final(const double* partial, int n
     if (threadIdx.x == 0) out[0] = sh[0];
 }
 
+int reduce_partials(Ctx* c, int nblocks, double* out)
+{
+    reduce_final<<<1, 256, 0, c->stream>>>(c->red_partial, nblocks, RED_SUM_SQ_DIFF, c->red_result);
+    SCAFES_HIP(hipGetLastError());
+    SCAFES_HIP(hipMemcpyAsync(c->host_result, c->red_result, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SCAFES_HIP(hipStreamSynchronize(c->stream));
+    *out = c->host_result[0];
+    return SCAFES_OK;
+}
+
 int reduce_nodes(Ctx* c, const Geo& ga, const Geo& gb, int dtype, const void* a, const void* b,
                  int op, int inner_only, double* out)
 {

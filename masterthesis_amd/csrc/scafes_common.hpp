@@ -230,6 +230,9 @@ int yee_tb2_launch(Ctx* c, const YeeDims& d, const Tb2Plan& pl, T s, const T* a,
 
 // Reductions (defined in scafes_capi.hip).
 enum RedOp { RED_MAX_ABS_DIFF = 0, RED_SUM_SQ_DIFF = 1, RED_SUM_PROD = 2 };
+// Sum of the first nblocks entries of c->red_partial (per-block FP64 partial
+// sums a kernel wrote there, nblocks <= c->red_capacity), waited for.
+int reduce_partials(Ctx* c, int nblocks, double* out);
 int reduce_nodes(Ctx* c, const Geo& ga, const Geo& gb, int dtype, const void* a, const void* b,
                  int op, int inner_only, double* out);
 
