@@ -274,9 +274,11 @@ __global__ __launch_bounds__(TX* TY) void heatvar_adjk_kernel(Geo g, const T* __
 // at most 80 VGPRs.
 // FP64 fields: 64 x 8 (a 1024-thread workgroup caps a thread at 128 VGPRs,
 // which the FP64 rings exceed).
+// FP32, K = 3: 64 x 12 (64 x 16 caps a thread at 128 VGPRs, which the
+// three-step rings exceed by a 16-byte spill).
 template <typename T, int K>
 struct AdjkTile {
-    static constexpr int TX = 64, TY = sizeof(T) == 8 ? 8 : 16, D = 2, ZC = 32;
+    static constexpr int TX = 64, TY = sizeof(T) == 8 ? 8 : (K == 3 ? 12 : 16), D = 2, ZC = 32;
 };
 
 template <typename T, int K>
