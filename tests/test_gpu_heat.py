@@ -306,6 +306,28 @@ def test_reductions(ctx):
     assert abs(out.value - np.sum((an[inner] - b[inner]) ** 2)) < 1e-10 * out.value
 
 
+@pytest.mark.parametrize("nbytes", [0, 1, 15, 16, 17, 1023, 4096, 65541, 1 << 20, (64 << 20) + 7])
+@pytest.mark.parametrize("offs", [(0, 0), (4, 4), (16, 48), (3, 0)])
+def test_device_copy_sizes_and_alignment(ctx, nbytes, offs):
+    """scafes_memcpy_d2d byte for byte against numpy: 16-byte path (aligned
+    ends, several elements per thread, grids from one block to 65536), the
+    byte tail, and unaligned ends (byte path only); bytes outside the target
+    range keep their values."""
+    so, do = offs
+    total = nbytes + 64
+    rng = np.random.default_rng(nbytes + 7 * so + do)
+    src_h = rng.integers(0, 256, total, dtype=np.uint8)
+    dst_h = rng.integers(0, 256, total, dtype=np.uint8)
+    src, dst = ctx.upload(src_h), ctx.upload(dst_h)
+    vp = capi.C.c_void_p
+    capi.check(ctx.L.scafes_memcpy_d2d(ctx.h, vp(dst.ptr.value + do), vp(src.ptr.value + so), nbytes))
+    want = dst_h.copy()
+    want[do:do + nbytes] = src_h[so:so + nbytes]
+    np.testing.assert_array_equal(dst.read(), want)
+    src.free()
+    dst.free()
+
+
 # ---------------------------------------------------------------------------
 # Pennes field form (forward and adjoint steps)
 
