@@ -329,8 +329,14 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb3_kernel(Geo g, const T* __r
     const T bv = (T)cf.border_value;
     const bool keep = cf.border_mode != SCAFES_BORDER_VALUE;
     // The z queues below (faces, level values) shift every plane with period
-    // 3; a body of lcm(D, 3) planes (6 at D = 2) lets the register allocator
-    // rename instead of move.  s is the prefetch slot, buf the LDS parity.
+    // 3; the body covers lcm(D, 3) planes (6 at D = 2).  The shifts still cost
+    // about 30 register moves per plane (the wave-uniform branches merge
+    // values), but this form needs 78 VGPRs (FP32 + Kahan): two 12-wave
+    // workgroups per CU.  Register rings indexed by the unrolled plane (no
+    // moves, 28 % fewer vector instructions) needed 102 VGPRs and took a
+    // three-step sweep at 512^3 from 815 to 880-960 us (one workgroup per CU;
+    // capped at 80 VGPRs they spill: 2100 us), the FP64 leg from 222 to 148
+    // GLUPS.  s is the prefetch slot, buf the LDS parity.
     constexpr int U = (D == 2) ? 6 : D;
     for (int kb = kfirst; kb <= klast; kb += U) {
 #pragma unroll
