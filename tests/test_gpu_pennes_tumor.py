@@ -18,6 +18,7 @@ import oracle_lib as O
 from masterthesis_amd import capi
 
 pytestmark = pytest.mark.gpu
+ERR_ARG = -1  # SCAFES_ERR_ARG
 BIN = os.path.join(O.ROOT, "masterthesis_amd", "bin")
 EXTENT = (0.06, 0.03, 0.03)
 
@@ -190,6 +191,25 @@ def test_tumor_gradient_fp32_deviation_state(ctx, dtype):
     err = np.linalg.norm(grad - gref) / np.linalg.norm(gref)
     assert err < 1e-4, err
     assert J == pytest.approx(Jref, rel=1e-4)
+
+
+def test_tumor_adjoint_rejects_zero_conductivity(ctx):
+    """k_cond = 0 would divide the flux terms by zero: the adjoint step and
+    the gradient refuse it with SCAFES_ERR_ARG before any launch."""
+    n = (12, 10, 9)
+    g, h, tau, coef = _case(n)
+    geo = capi.geom(n, h=h)
+    p = _params(coef)
+    p.k_cond = 0.0
+    ne = int(np.prod(O.memall_shape(g)))
+    bufs = [ctx.alloc(ne, np.float64) for _ in range(5)]
+    J = C.c_double(0)
+    rc = ctx.L.scafes_pennes_adjoint_step(ctx.h, C.byref(geo), 1, capi.F64, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr,
+                                          bufs[3].ptr, bufs[4].ptr, C.byref(p))
+    assert rc == ERR_ARG and b"k_cond" in ctx.L.scafes_last_error()
+    rc = ctx.L.scafes_pennes_gradient(ctx.h, C.byref(geo), 1, capi.F64, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr,
+                                      bufs[3].ptr, C.byref(p), 3, 2, bufs[4].ptr, C.byref(J), None)
+    assert rc == ERR_ARG and b"k_cond" in ctx.L.scafes_last_error()
 
 
 def test_tumor_autograd_vs_finite_differences(ctx):
