@@ -705,6 +705,44 @@ def bench_pennes(ctx, capi, n=256, nsteps=2000, snapshots=(62, 500, 1999)):
                                 "forward_steps": int(stats[0]), "adjoint_steps": int(stats[1]),
                                 "grad_ms": round(t_grad, 2), "grad_over_fwd": round(t_grad / t_fwd, 3),
                                 "grad_first_call_ms": round(t_first, 2), "J": J.value})
+    # The Tumor example's borders (Neumann flux on every face, Cauchy
+    # convection on the far z face; TumorHeatEqnFDM.hpp:272-349) on the same
+    # workload: k_cond = cdiff (rho c = 1), H = 10, ambient 15 K below T_a,
+    # no flux.  H adds 2 H / h_z = 2.6 % to the explicit-step rate, inside the
+    # workload's 10 % margin.  J and the gradient cover every node.
+    pt = capi.PennesParams(tau=wl["tau"], cdiff=0.5, cperf=1.0, csrc=1.0, t_a=0.0)
+    pt.border_mode, pt.k_cond, pt.h_conv, pt.t_inf, pt.q_bc = capi.BORDER_TUMOR, 0.5, 10.0, -15.0, 0.0
+    saved = pp
+    pp = pt
+    wp = ctx.upload((wl["w"] * wl["pert"]).astype(dt))
+    tgt_src = forward(wp, nsteps)
+    capi.check(ctx.L.scafes_memcpy_d2d(ctx.h, target.ptr, tgt_src.ptr, ne * 4))
+    wp.free()
+    forward(w, 5)
+    ctx.sync()
+    ctx.record(0)
+    forward(w, nsteps)
+    ctx.record(1)
+    t_fwd_t = ctx.elapsed_ms(0, 1)
+    J = capi.C.c_double(0)
+    stats = np.zeros(4)
+
+    def call_t():
+        return ctx.L.scafes_pennes_gradient(ctx.h, gp, 1, capi.F32, th0.ptr, w.ptr, q.ptr, target.ptr,
+                                            capi.C.byref(pt), nsteps, 500, grad.ptr, capi.C.byref(J),
+                                            stats.ctypes.data_as(capi.C.POINTER(capi.C.c_double)))
+    rc, t_first, t_grad = _grad_calls(ctx, call_t)
+    tum = {"border": "Tumor example: Neumann (q_bc = 0) on every face, Cauchy (H = 10, T_inf = T_a - 15) on z = n-1",
+           "fwd_ms": round(t_fwd_t, 2)}
+    if rc != 0:
+        tum["error"] = ctx.L.scafes_last_error().decode()
+    else:
+        tum.update({"snapshots": int(stats[2]), "checkpoint_GB": round(float(stats[3]) / 1e9, 2),
+                    "forward_steps": int(stats[0]), "adjoint_steps": int(stats[1]), "grad_ms": round(t_grad, 2),
+                    "grad_over_fwd": round(t_grad / t_fwd_t, 3), "grad_first_call_ms": round(t_first, 2),
+                    "J": J.value})
+    out["tumor_border"] = tum
+    pp = saved
     capi.check(ctx.L.scafes_ctx_release_workspace(ctx.h))
     for d in (th0, w, q, a, b, target, grad):
         d.free()

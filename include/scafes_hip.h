@@ -285,7 +285,7 @@ int scafes_pennes_adjoint_step(scafes_ctx* ctx, const scafes_geom* g, int hw, in
  * treeverse schedule) using at most max_snapshots stored states.
  * border_mode TUMOR (the Tumor example's Neumann/Cauchy borders): J sums
  * over every node, since every node is updated, and the gradient covers
- * every node; its reverse steps run one per sweep.
+ * every node.
  * dtype F64, F32, or F32_KAHAN (FP32 fields, compensated forward whose
  * checkpoints store (hi, lo) pairs; the adjoint field is plain FP32).  For
  * FP32 modes t0, w, q, target are FP32 (t0 without compensation).

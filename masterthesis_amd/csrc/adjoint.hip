@@ -416,9 +416,8 @@ static int run_schedule(Rev& R, Ctx* c, int dtype, const void* t0, size_t abytes
     R.grad = grad;
     R.lcur = 0;
     R.nsteps = nsteps;
-    // several reverse steps per sweep: the Keep-border Pennes and the
-    // var-kappa adjoints (the Tumor border's adjoint runs one step per sweep)
-    R.pair = (R.family == FAMILY_PENNES && R.p->border_mode == SCAFES_BORDER_KEEP && pennes_adjk_supported(R.geo)) ||
+    // several reverse steps per sweep where the states are at hand (3-D grids)
+    R.pair = (R.family == FAMILY_PENNES && pennes_adjk_supported(R.geo)) ||
              (R.family == FAMILY_HEATVAR && heatvar_adj2_supported(R.geo));
     R.fwd_steps = R.adj_steps = 0;
     R.J = 0.0;

@@ -8,6 +8,9 @@
 //   lam^{n-1}_i = lam^n_i + sum_p ctd*((lam_W+lam_E) - 2 lam_i)/h_p^2 - (ctp*w_i)*lam_i
 //   grad_i     += lam^n_i * (ctp*(T_a - T^{n-1}_i))        (inner nodes)
 //   lam^{n-1} = 0 on border nodes.
+// Under the Tumor border every node is updated with the transposed face
+// weights of the one-step operator (adj_update_tumor below) and the
+// gradient covers every node.
 //
 // A TX x TY thread tile walks a chunk of z planes.  In iteration k, level 0
 // (lam^n) is staged in LDS at plane k with a one-node ring; level L >= 1
@@ -40,6 +43,8 @@ using namespace chunk;
 struct AdjCoef {
     double ctd, ctp, ta, hh0, hh1, hh2;  // FP64 path
     float fd0, fd1, fd2, fp;             // FP32 path: ctd/h_p^2, ctp
+    double a2h2, hc;                     // Tumor border: (2 h_z)/K and H (Cauchy face z = nz-1)
+    float fa2h2, fhc;                    // the same rounded to FP32
 };
 
 // T^{n-1}, ..., T^{n-K} (and their lo parts for compensated states)
@@ -68,10 +73,56 @@ __device__ inline T adj_update(const AdjCoef& cf, T c, T w_, T e, T s, T n, T d,
     }
 }
 
+// One axis of the transposed Tumor step at a node with index idx of n
+// (PennesAdjOp::tumor_adj's order): the neighbour row at idx-1 weighs this
+// node twice when it lies on the face, the one at idx+1 twice when it has
+// index n-1, neighbours beyond the grid are selected away (their values are
+// never used); on the Cauchy face the diagonal gets -cau = -(2 h/K) H.
+template <typename T>
+__device__ inline T tumor_t(int idx, int n, bool cauchy, T cau, T lw, T le, T l)
+{
+    lw = idx >= 1 ? lw : T(0);
+    le = idx <= n - 2 ? le : T(0);
+    if (idx == 1) lw = lw + lw;
+    if (idx == n - 2) le = le + le;
+    if (cauchy && idx == n - 1) return ((lw + le) - cau * l) - (T)2 * l;
+    return (lw + le) - (T)2 * l;
+}
+
+// One adjoint step of the Tumor-border Pennes update at node (i, j, k) of a
+// 3-D grid (every node is updated; the Cauchy face is z = nz-1).
+template <typename T>
+__device__ inline T adj_update_tumor(const AdjCoef& cf, const Geo& g, int i, int j, int k, T c, T w_, T e, T s,
+                                     T n, T d, T u, T wf)
+{
+    const T t0 = tumor_t<T>(i, g.nx, false, T(0), w_, e, c);
+    const T t1 = tumor_t<T>(j, g.ny, false, T(0), s, n, c);
+    if constexpr (sizeof(T) == 8) {
+        const T t2 = tumor_t<T>(k, g.nz, true, cf.a2h2 * cf.hc, d, u, c);
+        double v = c;
+        v += (cf.ctd * t0) / cf.hh0;
+        v += (cf.ctd * t1) / cf.hh1;
+        v += (cf.ctd * t2) / cf.hh2;
+        v -= (cf.ctp * wf) * c;
+        return v;
+    } else {
+        const T t2 = tumor_t<T>(k, g.nz, true, cf.fa2h2 * cf.fhc, d, u, c);
+        float dd = cf.fd0 * t0;
+        dd = __builtin_fmaf(cf.fd1, t1, dd);
+        dd = __builtin_fmaf(cf.fd2, t2, dd);
+        dd = __builtin_fmaf(-cf.fp * wf, c, dd);
+        return c + dd;
+    }
+}
+
 // W: waves per SIMD the register budget must allow.  Two workgroups of 12
 // waves per CU need at most 80 VGPRs (W = 6); the FP64 and compensated
 // variants would spill under that cap and run one workgroup per CU (W = 1).
-template <typename T, bool LO, int TX, int TY, int K, int D, int W = (sizeof(T) == 4 && !LO) ? 6 : 1>
+// TU: the Tumor border (every node of the grid is updated, with the
+// transposed face weights of adj_update_tumor; the gradient covers every
+// node) instead of border nodes kept.
+template <typename T, bool LO, int TX, int TY, int K, int D, bool TU = false,
+          int W = (sizeof(T) == 4 && !LO) ? 6 : 1>
 __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(W))) void pennes_adjk_kernel(Geo g, const T* __restrict__ lam, T* __restrict__ lamo,
                                                               const T* __restrict__ wf, AdjStates st,
                                                               double* __restrict__ grad, AdjCoef cf, int zc,
@@ -103,13 +154,14 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(W))) voi
     const int i = txi * OX - R + tx, j = tyi * OY - R + ty;
     const int ic = min(max(i, 0), g.nx - 1), jc = min(max(j, 0), g.ny - 1);
     // lane-dependent (x) and wave-uniform (y) parts of the node predicates
-    const bool inner_i = i >= g.nl0 && i <= g.nx - 1 - g.nl0;
-    const bool inner_j = j >= g.nl1 && j <= g.ny - 1 - g.nl1;
+    // inner: updated by the step (every node of the grid under TU)
+    const bool inner_i = TU ? (i >= 0 && i <= g.nx - 1) : (i >= g.nl0 && i <= g.nx - 1 - g.nl0);
+    const bool inner_j = TU ? (j >= 0 && j <= g.ny - 1) : (j >= g.nl1 && j <= g.ny - 1 - g.nl1);
     const bool col_out = tx >= R && tx <= TX - 1 - R && i < g.nx;
     const bool row_out = ty >= R && ty <= TY - 1 - R && j < g.ny;
     const int k0 = tz * zc, k1 = min(k0 + zc, g.nz);
     const int kstart = k0 - R, kend = k1 + R - 1;
-    const int klo = g.nl2, khi = g.nz - 1 - g.nl2;
+    const int klo = TU ? 0 : g.nl2, khi = TU ? g.nz - 1 : g.nz - 1 - g.nl2;
     const bool hx = (tx == 0) || (tx == TX - 1);
     const bool hy = (ty == 0) || (ty == TY - 1);
     const int hxi = (tx == 0) ? max(ic - 1, 0) : min(ic + 1, g.nx - 1);
@@ -212,8 +264,13 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(W))) voi
             // ---- level 1: lam^{n-1} at plane k, every thread
             {
                 const bool in1 = inner_i && inner_j && k >= klo && k <= khi;
-                T l1 = adj_update<T>(cf, ac, s0[buf][ty + 1][tx], s0[buf][ty + 1][tx + 2], s0[buf][ty][tx + 1],
-                                     s0[buf][ty + 2][tx + 1], am, ap, wh[ring(u)]);
+                T l1;
+                if constexpr (TU)
+                    l1 = adj_update_tumor<T>(cf, g, i, j, k, ac, s0[buf][ty + 1][tx], s0[buf][ty + 1][tx + 2],
+                                             s0[buf][ty][tx + 1], s0[buf][ty + 2][tx + 1], am, ap, wh[ring(u)]);
+                else
+                    l1 = adj_update<T>(cf, ac, s0[buf][ty + 1][tx], s0[buf][ty + 1][tx + 2], s0[buf][ty][tx + 1],
+                                       s0[buf][ty + 2][tx + 1], am, ap, wh[ring(u)]);
                 l1 = in1 ? l1 : T(0);
                 hist[1][ring(u)] = l1;
                 sl[0][buf][ty][tx] = l1;
@@ -226,9 +283,15 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(W))) voi
                 T v = T(0);
                 if (ty >= L - 1 && ty <= TY - L && inner_j && inz) {  // wave-uniform
                     const int pbf = buf ^ 1;
-                    v = adj_update<T>(cf, hist[L - 1][ring(u - 1)], sl[L - 2][pbf][ty][txm],
-                                      sl[L - 2][pbf][ty][txp], sl[L - 2][pbf][tym][tx], sl[L - 2][pbf][typ][tx],
-                                      hist[L - 1][ring(u - 2)], hist[L - 1][ring(u)], wh[ring(u - (L - 1))]);
+                    if constexpr (TU)
+                        v = adj_update_tumor<T>(cf, g, i, j, p, hist[L - 1][ring(u - 1)], sl[L - 2][pbf][ty][txm],
+                                                sl[L - 2][pbf][ty][txp], sl[L - 2][pbf][tym][tx],
+                                                sl[L - 2][pbf][typ][tx], hist[L - 1][ring(u - 2)],
+                                                hist[L - 1][ring(u)], wh[ring(u - (L - 1))]);
+                    else
+                        v = adj_update<T>(cf, hist[L - 1][ring(u - 1)], sl[L - 2][pbf][ty][txm],
+                                          sl[L - 2][pbf][ty][txp], sl[L - 2][pbf][tym][tx], sl[L - 2][pbf][typ][tx],
+                                          hist[L - 1][ring(u - 2)], hist[L - 1][ring(u)], wh[ring(u - (L - 1))]);
                     v = (tx >= L - 1 && tx <= TX - L && inner_i) ? v : T(0);
                 }
                 if (L < K) {
@@ -273,7 +336,7 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(W))) voi
 // ms against 300.4 for K = 4 on 64x12, and K = 6 (8.7 B per step) 335 ms.
 constexpr int kKTX = 64, kKTY = 12, kKZC = 32, kKD = 2;
 
-template <typename T, bool LO, int K>
+template <typename T, bool LO, int K, bool TU>
 int adjk_launch_t(Ctx* c, const Geo& g, const void* lam, void* lamo, const void* w, const AdjStates& st,
                   double* grad, const AdjCoef& cf)
 {
@@ -282,20 +345,20 @@ int adjk_launch_t(Ctx* c, const Geo& g, const void* lam, void* lamo, const void*
     const int tiles_z = ceil_div(g.nz, kKZC);
     const long nb = (long)tiles_x * tiles_y * tiles_z;
     if (nb > 0x7fffffffL) return fail(SCAFES_ERR_ARG, "grid too large");
-    pennes_adjk_kernel<T, LO, kKTX, kKTY, K, kKD><<<dim3((unsigned)nb), dim3(kKTX, kKTY), 0, c->stream>>>(
+    pennes_adjk_kernel<T, LO, kKTX, kKTY, K, kKD, TU><<<dim3((unsigned)nb), dim3(kKTX, kKTY), 0, c->stream>>>(
         g, (const T*)lam, (T*)lamo, (const T*)w, st, grad, cf, kKZC, tiles_x, tiles_y);
     SCAFES_HIP(hipGetLastError());
     return SCAFES_OK;
 }
 
-template <typename T, bool LO>
+template <typename T, bool LO, bool TU>
 int adjk_launch_k(Ctx* c, const Geo& g, int K, const void* lam, void* lamo, const void* w, const AdjStates& st,
                   double* grad, const AdjCoef& cf)
 {
     switch (K) {
-    case 2: return adjk_launch_t<T, LO, 2>(c, g, lam, lamo, w, st, grad, cf);
-    case 3: return adjk_launch_t<T, LO, 3>(c, g, lam, lamo, w, st, grad, cf);
-    case 4: return adjk_launch_t<T, LO, 4>(c, g, lam, lamo, w, st, grad, cf);
+    case 2: return adjk_launch_t<T, LO, 2, TU>(c, g, lam, lamo, w, st, grad, cf);
+    case 3: return adjk_launch_t<T, LO, 3, TU>(c, g, lam, lamo, w, st, grad, cf);
+    case 4: return adjk_launch_t<T, LO, 4, TU>(c, g, lam, lamo, w, st, grad, cf);
     default: return fail(SCAFES_ERR_ARG, "multi-step adjoint: K must be 2..4");
     }
 }
@@ -316,8 +379,10 @@ int pennes_adjk_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, 
 {
     if (!pennes_adjk_supported(geo)) return fail(SCAFES_ERR_UNSUPPORTED, "multi-step adjoint needs a 3-D grid");
     if (K < 2 || K > kAdjMaxK) return fail(SCAFES_ERR_ARG, "multi-step adjoint: K must be 2..4");
-    if (p->border_mode != SCAFES_BORDER_KEEP)
-        return fail(SCAFES_ERR_UNSUPPORTED, "multi-step adjoint: only border_mode KEEP has an adjoint");
+    if (p->border_mode != SCAFES_BORDER_KEEP && p->border_mode != SCAFES_BORDER_TUMOR)
+        return fail(SCAFES_ERR_ARG, "multi-step adjoint: border_mode must be KEEP or TUMOR");
+    const bool tumor = p->border_mode == SCAFES_BORDER_TUMOR;
+    if (tumor && p->k_cond == 0.0) return fail(SCAFES_ERR_ARG, "multi-step adjoint: the TUMOR border needs k_cond != 0");
     AdjStates st{};
     for (int l = 0; l < K; ++l) {
         if (!t[l]) return fail(SCAFES_ERR_ARG, "multi-step adjoint: missing state");
@@ -338,12 +403,21 @@ int pennes_adjk_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, 
     cf.fd1 = (float)(cf.ctd / hh[1]);
     cf.fd2 = (float)(cf.ctd / hh[2]);
     cf.fp = (float)cf.ctp;
+    // the one-step Tumor adjoint's coefficients: (2 h_p)/K per axis, H
+    cf.a2h2 = tumor ? (2.0 * g->h[2]) / p->k_cond : 0.0;
+    cf.hc = tumor ? p->h_conv : 0.0;
+    cf.fa2h2 = (float)cf.a2h2;
+    cf.fhc = (float)cf.hc;
     if (dtype == SCAFES_F64) {
         if (tlo) return fail(SCAFES_ERR_ARG, "multi-step adjoint: FP64 states have no lo part");
-        return adjk_launch_k<double, false>(c, geo, K, lam, lam_out, w, st, grad, cf);
+        return tumor ? adjk_launch_k<double, false, true>(c, geo, K, lam, lam_out, w, st, grad, cf)
+                     : adjk_launch_k<double, false, false>(c, geo, K, lam, lam_out, w, st, grad, cf);
     }
-    if (tlo) return adjk_launch_k<float, true>(c, geo, K, lam, lam_out, w, st, grad, cf);
-    return adjk_launch_k<float, false>(c, geo, K, lam, lam_out, w, st, grad, cf);
+    if (tlo)
+        return tumor ? adjk_launch_k<float, true, true>(c, geo, K, lam, lam_out, w, st, grad, cf)
+                     : adjk_launch_k<float, true, false>(c, geo, K, lam, lam_out, w, st, grad, cf);
+    return tumor ? adjk_launch_k<float, false, true>(c, geo, K, lam, lam_out, w, st, grad, cf)
+                 : adjk_launch_k<float, false, false>(c, geo, K, lam, lam_out, w, st, grad, cf);
 }
 
 }  // namespace scafes

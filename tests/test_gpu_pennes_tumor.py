@@ -179,6 +179,22 @@ def test_tumor_gradient_fp64_bitexact(ctx, n, nlayers, snaps):
     assert stats[0] == ctx.L.scafes_checkpoint_forward_steps(N, snaps) and stats[1] == N
 
 
+@pytest.mark.parametrize("N,snaps", [(9, 8), (11, 10), (12, 3)])
+def test_tumor_multi_step_reverse_sweeps_ragged(ctx, N, snaps):
+    """The multi-step reverse sweeps under the Tumor border on a grid with
+    ragged x and y tiles and two z chunks: store-all (four steps per sweep,
+    then the remainder) and a binomial budget, FP64 bit for bit against the
+    oracle on every node."""
+    n = (70, 33, 40)
+    g, h, coef, w, q, oc, ob, t0, target = _grad_setup(n, N)
+    st = O.pennes_tumor_forward_all(g, t0, w, q, oc, ob, N)
+    Jref, gref = O.pennes_tumor_grad(g, st, w, target, oc, ob, N)
+    J, grad, stats = _device_grad(ctx, n, h, 1, capi.F64, coef, t0, w, q, target, N, snaps)
+    np.testing.assert_array_equal(grad, gref)
+    assert abs(J - Jref) <= 1e-12 * abs(Jref)
+    assert stats[1] == N
+
+
 @pytest.mark.parametrize("dtype", [capi.F32, capi.F32_KAHAN])
 def test_tumor_gradient_fp32_deviation_state(ctx, dtype):
     """FP32 fields on theta = T - T_a, FP64 gradient: within 1e-4 (relative
