@@ -321,7 +321,10 @@ struct PennesOp {
                           const St7<T>&, const T* xv) const
     {
         const long m = at(g, i, j, k);
-        if (is_border(g, i, j, k)) {
+        // Tumor border: every node takes the border form, a branch uniform
+        // over the launch (per node only selects); away from the faces it is
+        // the interior form term for term, so inner nodes get the same value.
+        if (tumor || is_border(g, i, j, k)) {
             if (!tumor) {
                 out[m] = s.c;
                 if constexpr (KAHAN) cout[m] = xv[2];
