@@ -290,6 +290,59 @@ def test_pennes256_fp64_gradient_vs_finite_difference(ctx):
         d.free()
 
 
+def _tumor(pp):
+    """The Tumor example's borders on the Pennes workload (k_cond = cdiff
+    with rho c = 1, H = 10, ambient 15 K below T_a, no flux)."""
+    pp.border_mode, pp.k_cond, pp.h_conv, pp.t_inf, pp.q_bc = capi.BORDER_TUMOR, 0.5, 10.0, -15.0, 0.0
+    return pp
+
+
+def test_pennes256_tumor_gradient_vs_finite_difference_and_budgets(ctx):
+    """Tumor borders at 256^3: the FP64 gradient (every node) against central
+    differences of the device forward along a random direction over every
+    node, J against the forward; FP32 binomial and store-all budgets give
+    the same gradient bit for bit."""
+    n, N = 256, 40
+    wl, g, ne, w, q, pp = _pennes_device(ctx, n, np.float64)
+    pp = _tumor(pp)
+    og = O.make_grid((n, n, n))
+    dom = np.zeros(O.memall_shape(og), bool)
+    O.nodes_view(og, dom)[...] = True
+    dom = dom.ravel()
+    wp = ctx.upload(wl["w"] * wl["pert"])
+    target = _forward(ctx, g, ne, np.float64, capi.F64, wp, q, pp, N)
+    tgt = ctx.upload(target)
+    J, grad = _gradient(ctx, g, ne, np.float64, capi.F64, w, q, tgt, pp, N, N - 1)
+    assert np.all(grad[~dom] == 0.0) and np.abs(grad[dom]).max() > 0
+    rng = np.random.default_rng(5)
+    v = np.where(dom, rng.standard_normal(ne), 0.0)
+    eps = 1e-4
+
+    def Jof(wv):
+        dw = ctx.upload(wv)
+        T = _forward(ctx, g, ne, np.float64, capi.F64, dw, q, pp, N)
+        dw.free()
+        r = np.where(dom, T - target, 0.0)
+        return 0.5 * float(np.dot(r, r))
+
+    fd = (Jof(wl["w"] + eps * v) - Jof(wl["w"] - eps * v)) / (2 * eps)
+    ad = float(np.dot(grad, v))
+    assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad)
+    assert abs(Jof(wl["w"]) - J) <= 1e-10 * J
+    for d in (w, q, wp, tgt):
+        d.free()
+    wl, g, ne, w, q, pp = _pennes_device(ctx, n, np.float32)
+    pp = _tumor(pp)
+    wp = ctx.upload((wl["w"] * wl["pert"]).astype(np.float32))
+    tgt = ctx.upload(_forward(ctx, g, ne, np.float32, capi.F32, wp, q, pp, N))
+    J1, g1 = _gradient(ctx, g, ne, np.float32, capi.F32, w, q, tgt, pp, N, 6)
+    J2, g2 = _gradient(ctx, g, ne, np.float32, capi.F32, w, q, tgt, pp, N, N - 1)
+    np.testing.assert_array_equal(g1, g2)
+    assert J1 == J2 > 0
+    for d in (w, q, wp, tgt):
+        d.free()
+
+
 def test_batch4096_graph_and_members(ctx):
     nsteps = 500
     ens = batch2d_ensemble(4096, nsteps, seed=0)
