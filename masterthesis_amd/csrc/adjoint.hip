@@ -199,13 +199,13 @@ struct Rev {
         lcur = 1 - lcur;
         return r;
     }
-    // T^N from T^{N-1} = tprev, then lam^N and J (the Pennes and
-    // variable-kappa families): after it the reverse steps from N on may run
-    // one or several per sweep.
+    // T^N from T^{N-1} = tprev (tprev = nullptr: T^N is in tmp[0] already),
+    // then lam^N and J (the Pennes and variable-kappa families): after it the
+    // reverse steps from N on may run one or several per sweep.
     int first_leaf(const void* tprev)
     {
         {
-            int r = step(tprev, tmp[0]);
+            int r = tprev ? step(tprev, tmp[0]) : SCAFES_OK;
             if (r) return r;
             const long total = (long)geo.nx * geo.ny * geo.nz;
             const int blocks = (int)std::min<long>((total + 255) / 256, c->red_capacity);
@@ -309,32 +309,47 @@ int Rev::treeverse(long a, long b, int s, const void* base)
             // take), then the steps are reversed up to kAdjMaxK per sweep.
             std::vector<const void*> st(1, base);  // st[q] = T^{a+q}
             int r = SCAFES_OK;
-            // The variable-kappa family takes the m mod 3 first states one
-            // step at a time, the rest three per sweep with the two states in
-            // between written by the same sweep (value parts only: its adjoint
-            // reads no more), so that the last state T^{b-1} is whole.
+            // The variable-kappa family takes the first states one step at a
+            // time, the rest three per sweep with the two states in between
+            // written by the same sweep (value parts only: its adjoint reads
+            // no more), so that the last state of each sweep is whole.  Where
+            // the stretch ends at N and lam^N is not set yet, T^N is the last
+            // sweep's output (into tmp[0]): (m + 1) mod 3 single steps instead
+            // of m mod 3 and one more for T^N.
             const long m = b - a - 1;
+            const bool sweeps = family == FAMILY_HEATVAR && heat7var_states_sweep_ok(c, geo, heat_mode);
+            const bool leaf = sweeps && b == nsteps && !lam_ready && m + 1 >= 3;
+            const long total = leaf ? m + 1 : m;  // states this stretch forms
             long single = m;
-            if (family == FAMILY_HEATVAR && m >= 3 && heat7var_states_sweep_ok(c, geo, heat_mode)) single = m % 3;
+            if (sweeps && total >= 3) single = total % 3;
             for (long q = 1; q <= single && !r; ++q) {
                 void* sl = slots.back();
                 slots.pop_back();
                 r = step(st.back(), sl);
                 st.push_back(sl);
             }
-            while (!r && (long)st.size() - 1 < m) {
+            while (!r && (long)st.size() - 1 < total) {
+                const bool to_leaf = leaf && (long)st.size() - 1 + 3 == total;
                 void* s3[3];
                 for (int q = 0; q < 3; ++q) {
-                    s3[q] = slots.back();
-                    slots.pop_back();
+                    if (q == 2 && to_leaf) {
+                        s3[q] = tmp[0];
+                    } else {
+                        s3[q] = slots.back();
+                        slots.pop_back();
+                    }
                 }
                 void* mid[2] = {s3[0], s3[1]};
                 const void* x = st.back();
                 r = heat7var_sweep_states(c, geo, g, heat_mode, x, s3[2], lo(x), lo(s3[2]), mid, kappa, tau,
                                           border_mode, border_value);
                 fwd_steps += 3;
-                for (int q = 0; q < 3; ++q) st.push_back(s3[q]);
+                st.push_back(s3[0]);
+                st.push_back(s3[1]);
+                if (to_leaf) break;
+                st.push_back(s3[2]);
             }
+            if (!r && leaf) r = first_leaf(nullptr);
             for (long n = b; n > a && !r;) {
                 if (n == nsteps && !lam_ready) {
                     r = first_leaf(st[n - 1 - a]);
