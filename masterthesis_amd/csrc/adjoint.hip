@@ -229,6 +229,17 @@ struct Rev {
         lam_ready = true;
         return SCAFES_OK;
     }
+    // Reverse steps the next sweep takes when `left` (>= 2) remain with
+    // their states at hand: as many as the kernel allows, but one fewer where
+    // that would leave a single step (a one-step launch costs about two steps
+    // of a multi-step sweep), so 5 -> 3 + 2 instead of 4 + 1.
+    int sweep_k(long left) const
+    {
+        const int kmax = family == FAMILY_HEATVAR && !heatvar_adjk_supported(geo) ? 2 : kAdjMaxK;
+        int K = (int)std::min<long>(kmax, left);
+        if (left - K == 1 && K >= 3) --K;
+        return K;
+    }
     // Adjoint steps n, n-1, ..., n-K+1 in one sweep (lam^n -> lam^{n-K})
     // with st[l] = T^{n-1-l}; lam^n is set (first_leaf for n = nsteps).
     int adjointk(int K, const void* const* st)
@@ -286,8 +297,7 @@ struct Rev {
                 r = adjoint((int)n, chain[n - 1 - a]);
                 n -= 1;
             } else {
-                const int kmax = family == FAMILY_HEATVAR && !heatvar_adjk_supported(geo) ? 2 : kAdjMaxK;
-                const int K = (int)std::min<long>(kmax, n - a);
+                const int K = sweep_k(n - a);
                 const void* sk[kAdjMaxK];
                 for (int l = 0; l < K; ++l) sk[l] = chain[n - 1 - l - a];
                 r = adjointk(K, sk);
@@ -359,8 +369,7 @@ int Rev::treeverse(long a, long b, int s, const void* base)
                     r = adjoint((int)n, st[n - 1 - a]);
                     n -= 1;
                 } else {
-                    const int kmax = family == FAMILY_HEATVAR && !heatvar_adjk_supported(geo) ? 2 : kAdjMaxK;
-                    const int K = (int)std::min<long>(kmax, n - a);
+                    const int K = sweep_k(n - a);
                     const void* sk[kAdjMaxK];
                     for (int l = 0; l < K; ++l) sk[l] = st[n - 1 - l - a];
                     r = adjointk(K, sk);
