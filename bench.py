@@ -546,7 +546,9 @@ def bench_yee_eps(ctx, capi, ncell=384, nsteps=200, snapshots=(20, 100)):
            "fwd_eps_ms": round(t_eps, 2), "fwd_eps_gcell_steps_per_s": round(cells * nsteps / (t_eps * 1e-3) / 1e9, 2),
            "fwd_vacuum_ms": round(t_vac, 2),
            "fwd_kernels": "yee_tb2_kernel<float, 64, 16, 4, true> (two steps per sweep, 26 B/cell-step algorithmic); "
-                          "yee_fused_kernel<float, 64, 8, 2, true> for a single step (52 B)",
+                          "yee_fused_kernel<float, 64, 8, 2, true> for a single step (52 B); in runs of one-step "
+                          "binomial splits the two-step sweep that also writes the state after one step "
+                          "(yee_tb2_kernel<float, 64, 16, 4, true, true>)",
            "adjoint_kernel": "adj_fused<float, 64, 16, 2> (one reverse step per sweep, 96 B/cell-step algorithmic)",
            **({"adjoint_profile": _profile_fields("yee_eps_adjoint_384.md", True)} if ncell == 384 else {}),
            "gradient": []}

@@ -204,6 +204,9 @@ int yee_eps_step(Ctx* c, const YeeDims& d, int dtype, double s, const void* ce, 
 // yee_eps_two_step_supported.
 bool yee_eps_two_step_supported(const YeeDims& d, int dtype);
 int yee_eps_step2(Ctx* c, const YeeDims& d, int dtype, double s, const void* ce, const void* a, void* b);
+// The same sweep also writing the state after one step to mid.
+int yee_eps_step2_states(Ctx* c, const YeeDims& d, int dtype, double s, const void* ce, const void* a, void* mid,
+                         void* b);
 int yee_eps_lam_init(Ctx* c, const YeeDims& d, int dtype, const void* fN, const void* target, void* lam,
                      double* partial, double* J);
 // One adjoint step: st = state n-1, la = (lam_E', lam_H') -> lb, grad +=.
@@ -224,9 +227,11 @@ struct Tb2Plan {
 };
 bool yee_tb2_supported(const YeeDims& d, int elem_bytes);
 Tb2Plan yee_tb2_plan(const YeeDims& d, int elem_bytes, int cu_count);
+// mid (nullable, with ce and without energy partials): the state after one
+// step is written there too.
 template <typename T>
 int yee_tb2_launch(Ctx* c, const YeeDims& d, const Tb2Plan& pl, T s, const T* a, T* b, double* ep0, double* ep1,
-                   const T* ce = nullptr);
+                   const T* ce = nullptr, T* mid = nullptr);
 
 // Reductions (defined in scafes_capi.hip).
 enum RedOp { RED_MAX_ABS_DIFF = 0, RED_SUM_SQ_DIFF = 1, RED_SUM_PROD = 2 };

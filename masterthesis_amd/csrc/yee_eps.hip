@@ -368,6 +368,18 @@ int yee_eps_step2(Ctx* c, const YeeDims& d, int dtype, double s, const void* ce,
     return yee_tb2_launch<float>(c, d, pl, (float)s, (const float*)a, (float*)b, nullptr, nullptr, (const float*)ce);
 }
 
+int yee_eps_step2_states(Ctx* c, const YeeDims& d, int dtype, double s, const void* ce, const void* a, void* mid,
+                         void* b)
+{
+    const int es = dtype == SCAFES_F64 ? 8 : 4;
+    const Tb2Plan pl = yee_tb2_plan(d, es, c->cu_count);
+    if (dtype == SCAFES_F64)
+        return yee_tb2_launch<double>(c, d, pl, s, (const double*)a, (double*)b, nullptr, nullptr,
+                                      (const double*)ce, (double*)mid);
+    return yee_tb2_launch<float>(c, d, pl, (float)s, (const float*)a, (float*)b, nullptr, nullptr, (const float*)ce,
+                                 (float*)mid);
+}
+
 int yee_eps_lam_init(Ctx* c, const YeeDims& d, int dtype, const void* fN, const void* target, void* lam,
                      double* partial, double* J)
 {

@@ -95,10 +95,14 @@ struct Tb2Lds {
 
 // EPS: the E levels use the per-node coefficient ce = s / eps (one value
 // per node index, queued along z with the fields) in place of s.
-template <typename T, int TX, int TY, int D, bool EPS>
+// MO: the state after one step (E1, H1) is written to fm too, at the nodes
+// whose E2, H2 the sweep writes (every node of the grid over all tiles), as a
+// one-step sweep would write it.
+template <typename T, int TX, int TY, int D, bool EPS, bool MO = false>
 __global__ __launch_bounds__(TX* TY) void yee_tb2_kernel(YeeDims d, T s, const T* __restrict__ fa, T* __restrict__ fb,
                                                           const T* __restrict__ ce, int zc, int tiles_x, int tiles_y,
-                                                          double* __restrict__ ep0, double* __restrict__ ep1)
+                                                          double* __restrict__ ep0, double* __restrict__ ep1,
+                                                          T* __restrict__ fm)
 {
     static_assert(D >= 2 && D % 2 == 0, "prefetch depth must be even");
     static_assert((TX * TY) % 64 == 0, "whole waves");
@@ -195,6 +199,14 @@ __global__ __launch_bounds__(TX* TY) void yee_tb2_kernel(YeeDims d, T s, const T
             L.h1[0][buf][ty][tx] = h1.x;
             L.h1[1][buf][ty][tx] = h1.y;
             L.h1[2][buf][ty][tx] = h1.z;
+            if constexpr (MO) {
+                const size_t o = plane(kk);
+                const uint32_t os = (out && kk >= k0 && kk < k1) ? oc : range;
+                char* m8 = (char*)fm;
+                bst<T>(h1.x, rsrc(m8 + o + 3 * fs, range), os);
+                bst<T>(h1.y, rsrc(m8 + o + 4 * fs, range), os);
+                bst<T>(h1.z, rsrc(m8 + o + 5 * fs, range), os);
+            }
             if (out && kk >= k0 && kk < k1 && ep0) {
                 es0 += 0.5 * ((double)e0.x * e0.x + (double)e0.y * e0.y + (double)e0.z * e0.z) +
                        0.5 * ((double)h0.x * h1.x + (double)h0.y * h1.y + (double)h0.z * h1.z);
@@ -206,6 +218,14 @@ __global__ __launch_bounds__(TX* TY) void yee_tb2_kernel(YeeDims d, T s, const T
             L.e1[0][buf][ty][tx] = e1.x;
             L.e1[1][buf][ty][tx] = e1.y;
             L.e1[2][buf][ty][tx] = e1.z;
+            if constexpr (MO) {
+                const size_t o = plane(kb1);
+                const uint32_t os = (out && kb1 >= k0 && kb1 < k1) ? oc : range;
+                char* m8 = (char*)fm;
+                bst<T>(e1.x, rsrc(m8 + o, range), os);
+                bst<T>(e1.y, rsrc(m8 + o + fs, range), os);
+                bst<T>(e1.z, rsrc(m8 + o + 2 * fs, range), os);
+            }
             // ---- C: H2 at plane kk-2 (E1 plane kk-2 from the previous iteration, E1(kk-1) in registers)
             const int kc = kk - 2;
             const V3<T> h2 = h_update<T>(N, i, j, kc, s, h1m2, e1m1, e1.x, e1.y, L.e1[1][pbf][ty][txp],
@@ -321,24 +341,28 @@ Tb2Plan yee_tb2_plan(const YeeDims& d, int elem_bytes, int cu_count)
 
 template <typename T>
 int yee_tb2_launch(Ctx* c, const YeeDims& d, const Tb2Plan& pl, T s, const T* a, T* b, double* ep0, double* ep1,
-                   const T* ce)
+                   const T* ce, T* mid)
 {
     typedef TileOf<T> Tl;
     if (pl.nb > 0x7fffffffL) return fail(SCAFES_ERR_ARG, "yee: grid too large");
+    if (mid && (!ce || ep0 || ep1)) return fail(SCAFES_ERR_ARG, "yee: the mid state needs the eps sweep, no energy");
     const dim3 grid((unsigned)pl.nb), blk(Tl::TX, Tl::TY);
-    if (ce)
+    if (mid)
+        yee_tb2_kernel<T, Tl::TX, Tl::TY, Tl::D, true, true><<<grid, blk, 0, c->stream>>>(
+            d, s, a, b, ce, pl.zc, pl.tiles_x, pl.tiles_y, nullptr, nullptr, mid);
+    else if (ce)
         yee_tb2_kernel<T, Tl::TX, Tl::TY, Tl::D, true><<<grid, blk, 0, c->stream>>>(d, s, a, b, ce, pl.zc, pl.tiles_x,
-                                                                                     pl.tiles_y, ep0, ep1);
+                                                                                     pl.tiles_y, ep0, ep1, nullptr);
     else
         yee_tb2_kernel<T, Tl::TX, Tl::TY, Tl::D, false><<<grid, blk, 0, c->stream>>>(
-            d, s, a, b, nullptr, pl.zc, pl.tiles_x, pl.tiles_y, ep0, ep1);
+            d, s, a, b, nullptr, pl.zc, pl.tiles_x, pl.tiles_y, ep0, ep1, nullptr);
     SCAFES_HIP(hipGetLastError());
     return SCAFES_OK;
 }
 
 template int yee_tb2_launch<float>(Ctx*, const YeeDims&, const Tb2Plan&, float, const float*, float*, double*,
-                                   double*, const float*);
+                                   double*, const float*, float*);
 template int yee_tb2_launch<double>(Ctx*, const YeeDims&, const Tb2Plan&, double, const double*, double*, double*,
-                                    double*, const double*);
+                                    double*, const double*, double*);
 
 }  // namespace scafes
