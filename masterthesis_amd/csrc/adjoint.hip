@@ -143,7 +143,8 @@ struct Rev {
     }
     // State after n steps from src, written to dst (src is never written).
     // The Yee and variable-kappa families advance several steps per sweep
-    // where that many remain (bit-identical to single steps).
+    // where that many remain (bit-identical to single steps); two steps left
+    // behind three-step sweeps take one two-step sweep.
     int advance(const void* src, long n, void* dst)
     {
         if (n == 0) return src == dst ? SCAFES_OK : copy(dst, src);
@@ -152,17 +153,22 @@ struct Rev {
         if (family == FAMILY_HEATVAR) multi = heat7var_sweep_steps(c, geo, heat_mode);
         const void* x = src;
         for (long done = 0; done < n;) {
-            const int k = (multi > 1 && n - done >= multi) ? multi : 1;
+            int k = (multi > 1 && n - done >= multi) ? multi : 1;
+            if (multi == 3 && n - done == 2) k = 2;
             done += k;
             void* out = (done == n) ? dst : (x == tmp[0] ? tmp[1] : tmp[0]);
             if (out == x) out = (x == tmp[0]) ? tmp[1] : tmp[0];
             int r;
             if (k > 1) {
                 fwd_steps += k;
-                r = family == FAMILY_YEE_EPS
-                        ? yee_eps_step2(c, yd, dtype, yee_s, ce, x, out)
-                        : heat7var_sweep(c, geo, g, heat_mode, x, out, lo(x), lo(out), kappa, tau, border_mode,
-                                         border_value);
+                if (family == FAMILY_YEE_EPS)
+                    r = yee_eps_step2(c, yd, dtype, yee_s, ce, x, out);
+                else if (k == 2 && multi == 3)
+                    r = heat7var_sweep2(c, geo, g, heat_mode, x, out, lo(x), lo(out), kappa, tau, border_mode,
+                                        border_value);
+                else
+                    r = heat7var_sweep(c, geo, g, heat_mode, x, out, lo(x), lo(out), kappa, tau, border_mode,
+                                       border_value);
             } else {
                 r = step(x, out);
             }

@@ -139,6 +139,11 @@ int heat7var_step_one(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, co
 int heat7var_sweep_steps(const Ctx* c, const Geo& geo, int mode);
 int heat7var_sweep(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* un, const void* cin,
                    void* cout, const void* kap, double tau, int border_mode, double border_value);
+// Two steps in one sweep with the mode's two-step tile, where the grid's
+// sweep takes three (heat7var_sweep_steps == 3): the two steps left behind
+// three-step sweeps; SCAFES_ERR_UNSUPPORTED otherwise.
+int heat7var_sweep2(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* un, const void* cin,
+                    void* cout, const void* kap, double tau, int border_mode, double border_value);
 // Three steps u -> un (cin -> cout) in one sweep that also writes the
 // states after one and two steps to mid[0], mid[1] (value parts only in the
 // compensated mode), when the grid's three-step tile has such a variant;

@@ -1402,6 +1402,15 @@ int heat7var_sweep(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const
                               cfg - kNumCfgs);
 }
 
+int heat7var_sweep2(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* un, const void* cin,
+                    void* cout, const void* kap, double tau, int border_mode, double border_value)
+{
+    if (heat7var_sweep_steps(c, geo, mode) != 3)
+        return fail(SCAFES_ERR_UNSUPPORTED, "no two-step sweep behind a three-step tile on this grid");
+    return heatvar_tb2_launch(c, geo, g, mode, u, un, cin, cout, kap, tau, border_mode, border_value,
+                              kTwoStepCfgByMode[mode] - kNumCfgs);
+}
+
 bool heat7var_states_sweep_ok(const Ctx* c, const Geo& geo, int mode)
 {
     const int cfg = resolve_heatvar_cfg(c, geo, mode, -1);
