@@ -328,19 +328,48 @@ struct Rev {
     // up to kAdjMaxK per sweep (bit-identical to single steps).
     int peel_run(long a, long b, int s, const void* base)
     {
+        long len = 0;  // splits in the run
+        {
+            long x = a;
+            int sx = s;
+            long free_slots = (long)slots.size();
+            while (b - x > 1 && sx > 0 && free_slots > 0 && right_part(b - x, sx) == b - x - 1 &&
+                   !(sx >= b - x - 1 && free_slots >= b - x - 1)) {
+                ++x;
+                --sx;
+                --free_slots;
+                ++len;
+            }
+        }
+        // The variable-kappa family forms the peeled states three per sweep
+        // where the grid's sweep can write the two states in between (value
+        // parts, all the adjoint reads), the len mod 3 first ones one step at
+        // a time, so the run's last state (the right part's base) is whole.
         std::vector<const void*> chain(1, base);  // chain[q] = T^{a+q}
-        long aa = a;
-        int ss = s;
+        const bool tri = family == FAMILY_HEATVAR && len >= 3 && heat7var_states_sweep_ok(c, geo, heat_mode);
+        const long single = tri ? len % 3 : len;
         int r = SCAFES_OK;
-        while (!r && b - aa > 1 && ss > 0 && !slots.empty() && right_part(b - aa, ss) == b - aa - 1 &&
-               !(ss >= b - aa - 1 && (long)slots.size() >= b - aa - 1)) {
+        for (long q = 0; q < single && !r; ++q) {
             void* sl = slots.back();
             slots.pop_back();
             r = advance(chain.back(), 1, sl);
             chain.push_back(sl);
-            ++aa;
-            --ss;
         }
+        while (!r && (long)chain.size() - 1 < len) {
+            void* s3[3];
+            for (int q = 0; q < 3; ++q) {
+                s3[q] = slots.back();
+                slots.pop_back();
+            }
+            void* mid[2] = {s3[0], s3[1]};
+            const void* x = chain.back();
+            r = heat7var_sweep_states(c, geo, g, heat_mode, x, s3[2], lo(x), lo(s3[2]), mid, kappa, tau, border_mode,
+                                      border_value);
+            fwd_steps += 3;
+            for (int q = 0; q < 3; ++q) chain.push_back(s3[q]);
+        }
+        const long aa = a + len;
+        const int ss = s - (int)len;
         if (!r) r = treeverse(aa, b, ss, chain.back());
         for (long n = aa; n > a && !r;) {
             if (n == nsteps && !lam_ready) {

@@ -81,6 +81,21 @@ def test_fp64_bitexact_vs_oracle(ctx, n, nlayers, border, snaps):
     assert np.abs(gref).max() > 1e-6
 
 
+@pytest.mark.parametrize("N,snaps", [(12, 8), (12, 6), (20, 12)])
+def test_fp64_peel_runs_bitexact(ctx, N, snaps):
+    """Budgets whose binomial schedule splits one step off the left again and
+    again (runs of 6, 3 and 8 peeled states, formed three per sweep where
+    they can be): FP64 bit for bit against the oracle's store-all adjoint."""
+    n = (14, 13, 12)
+    g, h, u0, kap, tau, target = _case(n, 1, seed=N + snaps)
+    st = O.heat7var_forward_all(g, u0, kap, tau, N, O.BORDER_VALUE, 0.0)
+    Jref, gref = O.heat7var_grad(g, st, kap, target, tau, N)
+    J, grad, stats = _device(ctx, n, h, 1, capi.HEATVAR_F64, u0, kap, target, tau, capi.BORDER_VALUE, 0.0, N, snaps)
+    np.testing.assert_array_equal(grad, gref)
+    assert abs(J - Jref) <= 1e-12 * abs(Jref)
+    assert stats[0] == ctx.L.scafes_checkpoint_forward_steps(N, snaps) and stats[1] == N
+
+
 @pytest.mark.parametrize("snaps", [0, 8])
 def test_fp64_two_step_reverse_sweep_ragged_tiles(ctx, snaps):
     """A 3-D grid with ragged x and y tiles and two z chunks of the
@@ -113,13 +128,15 @@ def test_fp32_modes_vs_fp64_oracle(ctx, mode):
 @pytest.mark.parametrize("N", [12, 11, 10])
 def test_fp32_checkpointing_does_not_change_gradient(ctx, N):
     """FP32 + Kahan: the gradient is the same bit for bit whatever the
-    schedule: one-step launches only (no snapshots), a binomial budget, and
-    store-all, whose reverse steps run 4+4+4, 4+4+3 and 4+4+2 per sweep at
-    N = 12, 11, 10."""
+    schedule: one-step launches only (no snapshots), a binomial budget (at
+    N = 12 with 8 snapshots a run of 6 peeled states, formed three per
+    sweep), and store-all, whose reverse steps run 4+4+4, 4+4+3 and 4+4+2
+    per sweep at N = 12, 11, 10."""
     n = (40, 36, 33)
     g, h, u0, kap, tau, target = _case(n, seed=4)
     J0, g0, _ = _device(ctx, n, h, 1, capi.HEATVAR_F32_KAHAN, u0, kap, target, tau, capi.BORDER_VALUE, 0.0, N, 0)
-    J1, g1, _ = _device(ctx, n, h, 1, capi.HEATVAR_F32_KAHAN, u0, kap, target, tau, capi.BORDER_VALUE, 0.0, N, 3)
+    J1, g1, _ = _device(ctx, n, h, 1, capi.HEATVAR_F32_KAHAN, u0, kap, target, tau, capi.BORDER_VALUE, 0.0, N,
+                        8 if N == 12 else 3)
     J2, g2, _ = _device(ctx, n, h, 1, capi.HEATVAR_F32_KAHAN, u0, kap, target, tau, capi.BORDER_VALUE, 0.0, N, N - 1)
     np.testing.assert_array_equal(g1, g0)
     np.testing.assert_array_equal(g2, g0)
