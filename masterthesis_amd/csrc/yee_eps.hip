@@ -300,8 +300,10 @@ constexpr int kInitBlocks = 2048;
 
 // FP32: 64 x 16 workgroups with 32-plane z chunks (measured at 384^3 on an
 // MI355X: 1.27-1.48 ms per adjoint step against 1.51-1.62 for 64 x 8 with
-// 32 or 64 planes, 64 x 4 and a four-plane queue); FP64: 64 x 4 (its four LDS
-// planes take twice the space) with 64-plane chunks.
+// 32 or 64 planes, 64 x 4 and a four-plane queue; the 200-step gradient with
+// 100 snapshots took 434 ms with 32-plane chunks, 441, 436 and 458 with 48,
+// 64 and 128); FP64: 64 x 4 (its four LDS planes take twice the space) with
+// 64-plane chunks.
 template <typename T>
 struct AdjTile {
     static constexpr int TX = 64, TY = sizeof(T) == 4 ? 16 : 4, D = 2, ZC = sizeof(T) == 4 ? 32 : 64;
