@@ -154,9 +154,13 @@ int scafes_heat7_step_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const d
  * Runs nsteps steps ping-ponging between (u, u2) and (c, c2); on return
  * *result_in_second = 1 when the final state is in u2/c2.  kappa has the
  * layout of u.  cfg: tile configuration index (-1 = autotuned/default).
- * A multi-step configuration runs nsteps / s sweeps of s steps; a
- * remainder of two runs as one two-step sweep, a remainder of one as one
- * step.  Stability: SCAFES_ERR_ARG unless tau * sum_p 2*kappa_max/h_p^2 <= 1
+ * A multi-step configuration given as cfg runs nsteps / s sweeps of s
+ * steps, a remainder of two as one two-step sweep, the rest one step per
+ * sweep.  With cfg = -1 the steps left behind three-step sweeps go to one
+ * two-step sweep, or, where the mode has a four-step tile of the same cost
+ * per step (FP32 + Kahan), to four-step sweeps (7 = 4 + 3, 20 = 2 x 4 +
+ * 4 x 3); a tuned four-step tile takes its remainder with the three-, two-
+ * and one-step tiles.  Stability: SCAFES_ERR_ARG unless tau * sum_p 2*kappa_max/h_p^2 <= 1
  * (kappa_max = max |kappa| over all nodes, a device reduction kept per
  * pointer until the library writes the field or scafes_coef_changed names
  * it). */
@@ -166,8 +170,7 @@ int scafes_heat7var_run(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode,
                         int border_mode, double border_value, int cfg, int* result_in_second);
 /* Number of tile configurations and a description of one: workgroup tile,
  * z planes per workgroup, prefetch depth in planes, and time steps per HBM
- * sweep (1, or 2 / 3 for the temporally blocked kernels; steps left over
- * at the end run one per sweep). */
+ * sweep (1, or 2 / 3 / 4 for the temporally blocked kernels). */
 int scafes_heat7var_num_cfgs(void);
 int scafes_heat7var_cfg_desc(int cfg, int* tx, int* ty, int* zchunk, int* depth,
                              int* steps_per_sweep);
@@ -175,9 +178,17 @@ int scafes_heat7var_cfg_desc(int cfg, int* tx, int* ty, int* zchunk, int* depth,
  * (autotuned choice, else the per-mode default; the two-steps-per-sweep
  * tiles need a 3-D grid). */
 int scafes_heat7var_active_cfg(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode, int* cfg);
+/* How scafes_heat7var_run with cfg = -1 splits nsteps into sweeps on this
+ * grid: the steps of each sweep in launch order (the first min(max, count)
+ * written to sweeps), their number in *count.  No counterpart in the
+ * reference, which updates one step per kernel launch
+ * (src/ScaFES_ProblemCuda.cu); callers that time the sweep kinds
+ * separately use it. */
+int scafes_heat7var_plan(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode, int nsteps, int* sweeps,
+                         int max, int* count);
 /* Project autotuner (no counterpart in the reference, whose device stub
  * fixes BLOCK_SIZE 32, src/ScaFES_ProblemCuda.hpp:22): times every
- * configuration on the given buffers over 6*ceil(nsteps/6) steps (a
+ * configuration on the given buffers over 24*ceil(nsteps/24) steps (a
  * multiple of every configuration's steps per sweep, so no remainder sweep
  * enters the timing) and keeps the fastest for this grid (extents, hw) and
  * mode: later calls with cfg = -1 on the same grid use it (at most eight

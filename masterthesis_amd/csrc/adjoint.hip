@@ -143,18 +143,17 @@ struct Rev {
     }
     // State after n steps from src, written to dst (src is never written).
     // The Yee and variable-kappa families advance several steps per sweep
-    // where that many remain (bit-identical to single steps); two steps left
-    // behind three-step sweeps take one two-step sweep.
+    // where that many remain (bit-identical to single steps); the
+    // variable-kappa family splits n into sweeps as its forward run does.
     int advance(const void* src, long n, void* dst)
     {
         if (n == 0) return src == dst ? SCAFES_OK : copy(dst, src);
-        int multi = 1;
-        if (family == FAMILY_YEE_EPS && yee_eps_two_step_supported(yd, dtype)) multi = 2;
-        if (family == FAMILY_HEATVAR) multi = heat7var_sweep_steps(c, geo, heat_mode);
+        const bool yee2 = family == FAMILY_YEE_EPS && yee_eps_two_step_supported(yd, dtype);
         const void* x = src;
         for (long done = 0; done < n;) {
-            int k = (multi > 1 && n - done >= multi) ? multi : 1;
-            if (multi == 3 && n - done == 2) k = 2;
+            int k = 1;
+            if (yee2 && n - done >= 2) k = 2;
+            if (family == FAMILY_HEATVAR) k = heat7var_next_sweep(c, geo, heat_mode, n - done);
             done += k;
             void* out = (done == n) ? dst : (x == tmp[0] ? tmp[1] : tmp[0]);
             if (out == x) out = (x == tmp[0]) ? tmp[1] : tmp[0];
@@ -163,12 +162,9 @@ struct Rev {
                 fwd_steps += k;
                 if (family == FAMILY_YEE_EPS)
                     r = yee_eps_step2(c, yd, dtype, yee_s, ce, x, out);
-                else if (k == 2 && multi == 3)
-                    r = heat7var_sweep2(c, geo, g, heat_mode, x, out, lo(x), lo(out), kappa, tau, border_mode,
-                                        border_value);
                 else
-                    r = heat7var_sweep(c, geo, g, heat_mode, x, out, lo(x), lo(out), kappa, tau, border_mode,
-                                       border_value);
+                    r = heat7var_sweep_k(c, geo, g, heat_mode, k, x, out, lo(x), lo(out), kappa, tau, border_mode,
+                                         border_value);
             } else {
                 r = step(x, out);
             }

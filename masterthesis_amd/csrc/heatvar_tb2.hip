@@ -431,6 +431,198 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb3_kernel(Geo g, const T* __r
     }
 }
 
+// Four steps per sweep: the three-step scheme with one more level.  Threads
+// cover the output tile plus a three-node ring; level L (u after L steps) is
+// valid L-1 nodes inside the thread tile, level 4 on the (TX-6) x (TY-6)
+// output tile.  Iteration kk computes level L at plane kk-(L-1); levels 2..4
+// read the in-plane neighbours of the level below from its LDS plane written
+// one iteration earlier and the z neighbours from register queues (one
+// barrier per plane, planes double-buffered).  The faces of a node ride in a
+// three-deep register queue for levels 2..4.  Same branch-free form as the
+// three-step kernel: wave-uniform row branches, off-ring rows writing the y
+// ring into a sink row, lanes off the output columns storing out of range.
+// Per four steps HBM moves one sweep of u, kappa (and c) plus the ring.
+template <typename T, int MODE, int TX, int TY, int D>
+__global__ __launch_bounds__(TX* TY) void heatvar_tb4_kernel(Geo g, const T* __restrict__ u, const T* __restrict__ cin,
+                                                              const T* __restrict__ kap, T* __restrict__ uout,
+                                                              T* __restrict__ cout, VarCoef cf, int zc, int tiles_x,
+                                                              int tiles_y)
+{
+    static_assert(D >= 2 && D % 2 == 0, "prefetch depth must be even");
+    static_assert(TX % 64 == 0, "whole rows per wavefront: ty is wave-uniform");
+    static_assert(TY >= 8, "the output tile needs rows");
+    constexpr int OX = TX - 6, OY = TY - 6;
+    constexpr bool KH = (MODE == 2);
+    __shared__ T s0[2][TY + 3][TX + 2];
+    __shared__ T sk[2][TY + 3][TX + 2];
+    __shared__ T s1[2][TY][TX];  // level-1 plane
+    __shared__ T s2[2][TY][TX];  // level-2 plane
+    __shared__ T s3[2][TY][TX];  // level-3 plane
+    const int tx = threadIdx.x;
+    const int ty = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int per_plane = tiles_x * tiles_y;
+    const int tz = lb / per_plane;
+    const int r = lb - tz * per_plane;
+    const int tyi = r / tiles_x;
+    const int txi = r - tyi * tiles_x;
+    const int i = txi * OX - 3 + tx, j = tyi * OY - 3 + ty;
+    const int ic = min(max(i, 0), g.nx - 1), jc = min(max(j, 0), g.ny - 1);
+    const bool inner_xy = i >= g.nl0 && i <= g.nx - 1 - g.nl0 && j >= g.nl1 && j <= g.ny - 1 - g.nl1;
+    const bool col_out = tx >= 3 && tx <= TX - 4 && i < g.nx;  // lane-dependent
+    const bool row_out = ty >= 3 && ty <= TY - 4 && j < g.ny;  // wave-uniform
+    // clamped neighbour slots of level 2 (clamped reads only feed positions
+    // that are never used); levels 3 and 4 run on rows whose neighbours exist
+    const int txm = max(tx - 1, 0), txp = min(tx + 1, TX - 1), tym = max(ty - 1, 0), typ = min(ty + 1, TY - 1);
+    const int k0 = tz * zc, k1 = min(k0 + zc, g.nz);
+    const int kfirst = k0 - 3;  // first plane with a level-1 value
+    const int klast = k1 + 2;   // last iteration (level 1 at k1 + 2)
+    const int kcen = k1 + 3;    // last plane whose centre is read
+    const int klo = g.nl2, khi = g.nz - 1 - g.nl2;
+    const bool hx = (tx == 0) || (tx == TX - 1);
+    const bool hy = (ty == 0) || (ty == TY - 1);
+    const int hxi = (tx == 0) ? max(ic - 1, 0) : min(ic + 1, g.nx - 1);
+    const int hyj = (ty == 0) ? max(jc - 1, 0) : min(jc + 1, g.ny - 1);
+    const int sxc = hx ? ((tx == 0) ? 0 : TX + 1) : tx + 1;
+    const int syc = hy ? ((ty == 0) ? 0 : TY + 1) : TY + 2;
+    constexpr uint32_t B = sizeof(T);
+    const uint32_t oc = (uint32_t)(g.base + ic + g.e0 * jc) * B;
+    const uint32_t ohx = hx ? (uint32_t)(g.base + hxi + g.e0 * jc) * B : oc;
+    const uint32_t ohy = (uint32_t)(g.base + ic + g.e0 * hyj) * B;
+    const size_t pb = (size_t)g.e01 * B;
+    const uint32_t range = (uint32_t)((g.base + g.e01) * B);
+    const uint32_t oo = col_out ? (uint32_t)(g.base + i + g.e0 * j) * B : range;  // output node
+    const char* ub = (const char*)u;
+    const char* kb8 = (const char*)kap;
+    const char* cb = (const char*)cin;
+    auto plane = [&](int kk) { return (size_t)min(max(kk, 0), g.nz - 1) * pb; };
+
+    T qa[D], qb[D], qc[D], qax[D], qbx[D], qay[D], qby[D];
+#pragma unroll
+    for (int s = 0; s < D; ++s) qa[s] = qb[s] = qc[s] = qax[s] = qbx[s] = qay[s] = qby[s] = T(0);
+    // the same fixed set of loads every plane (unneeded ones read at offset
+    // `range`, answered with 0 without a memory access)
+    auto fetch = [&](const int slot, const int kk) {
+        const size_t o = plane(kk);
+        const Rsrc ru = plane_rsrc(ub + o, range), rk = plane_rsrc(kb8 + o, range);
+        const bool cen = kk <= kcen, mid = kk <= klast;
+        const uint32_t occ = cen ? oc : range, ocm = mid ? oc : range;
+        const uint32_t ox = mid ? ohx : range, oy = (mid && hy) ? ohy : range;
+        qa[slot] = bload<T>(ru, occ);
+        qb[slot] = bload<T>(rk, occ);
+        if constexpr (KH) qc[slot] = bload<T>(plane_rsrc(cb + o, range), ocm);
+        qax[slot] = bload<T>(ru, ox);
+        qbx[slot] = bload<T>(rk, ox);
+        qay[slot] = bload<T>(ru, oy);
+        qby[slot] = bload<T>(rk, oy);
+    };
+    T am = bload<T>(plane_rsrc(ub + plane(kfirst - 1), range), oc);
+    const T bm = bload<T>(plane_rsrc(kb8 + plane(kfirst - 1), range), oc);
+#pragma unroll
+    for (int s = 0; s < D; ++s) fetch(s, kfirst + s);
+
+    T l1m1 = T(0), l1m2 = T(0), c1m1 = T(0);  // level 1 at kk-1, kk-2; compensation at kk-1
+    T l2m1 = T(0), l2m2 = T(0), c2m1 = T(0);  // level 2 at kk-2, kk-3; compensation at kk-2
+    T l3m1 = T(0), l3m2 = T(0), c3m1 = T(0);  // level 3 at kk-3, kk-4; compensation at kk-3
+    Faces<T> fq1 = {}, fq2 = {}, fq3 = {};      // faces of planes kk-1, kk-2, kk-3
+    T fup = face<T, MODE>(qb[0], bm);
+    const T bv = (T)cf.border_value;
+    const bool keep = cf.border_mode != SCAFES_BORDER_VALUE;
+    constexpr int U = (D == 2) ? 6 : D;
+    for (int kb = kfirst; kb <= klast; kb += U) {
+#pragma unroll
+        for (int uu = 0; uu < U; ++uu) {
+            const int s = uu % D;
+            const int k = kb + uu;
+            if (k > klast) return;  // uniform over the workgroup
+            const int buf = uu & 1, pbf = buf ^ 1;
+            const T ac = qa[s], ap = qa[(s + 1) % D];
+            const T bc = qb[s], bp = qb[(s + 1) % D];
+            const T c0 = qc[s];
+            s0[buf][ty + 1][tx + 1] = ac;
+            sk[buf][ty + 1][tx + 1] = bc;
+            s0[buf][ty + 1][sxc] = qax[s];
+            sk[buf][ty + 1][sxc] = qbx[s];
+            s0[buf][syc][tx + 1] = qay[s];
+            sk[buf][syc][tx + 1] = qby[s];
+            fetch(s, k + D);
+            __syncthreads();
+            // ---- level 1 at plane k, every thread position
+            Faces<T> f;
+            f.e = face<T, MODE>(sk[buf][ty + 1][tx + 2], bc);
+            f.w = face<T, MODE>(bc, sk[buf][ty + 1][tx]);
+            f.n = face<T, MODE>(sk[buf][ty + 2][tx + 1], bc);
+            f.s = face<T, MODE>(bc, sk[buf][ty][tx + 1]);
+            f.u = face<T, MODE>(bp, bc);
+            f.d = fup;
+            T c1;
+            T l1 = var_update<T, MODE>(cf, ac, s0[buf][ty + 1][tx], s0[buf][ty + 1][tx + 2], s0[buf][ty][tx + 1],
+                                       s0[buf][ty + 2][tx + 1], am, ap, f, c0, &c1);
+            if (!(inner_xy && k >= klo && k <= khi)) {
+                l1 = keep ? ac : bv;
+                c1 = keep ? c0 : T(0);
+            }
+            s1[buf][ty][tx] = l1;
+            // ---- level 2 at plane k-1 on rows 1..TY-2
+            const int k2 = k - 1;
+            T c2 = T(0), l2 = T(0);
+            if (ty >= 1 && ty <= TY - 2) {
+                l2 = var_update<T, MODE>(cf, l1m1, s1[pbf][ty][txm], s1[pbf][ty][txp], s1[pbf][tym][tx],
+                                         s1[pbf][typ][tx], l1m2, l1, fq1, c1m1, &c2);
+                if (!(inner_xy && k2 >= klo && k2 <= khi)) {
+                    l2 = keep ? l1m1 : bv;
+                    c2 = keep ? c1m1 : T(0);
+                }
+            }
+            s2[buf][ty][tx] = l2;
+            // ---- level 3 at plane k-2 on rows 2..TY-3
+            const int k3 = k - 2;
+            T c3 = T(0), l3 = T(0);
+            if (ty >= 2 && ty <= TY - 3) {
+                l3 = var_update<T, MODE>(cf, l2m1, s2[pbf][ty][txm], s2[pbf][ty][txp], s2[pbf][ty - 1][tx],
+                                         s2[pbf][ty + 1][tx], l2m2, l2, fq2, c2m1, &c3);
+                if (!(inner_xy && k3 >= klo && k3 <= khi)) {
+                    l3 = keep ? l2m1 : bv;
+                    c3 = keep ? c2m1 : T(0);
+                }
+            }
+            s3[buf][ty][tx] = l3;
+            // ---- level 4 at plane k-3 on the output rows and planes
+            const int k4 = k - 3;
+            {
+                const bool st = k4 >= k0 && k4 < k1 && row_out;
+                T l4 = T(0), c4 = T(0);
+                if (st) {
+                    l4 = var_update<T, MODE>(cf, l3m1, s3[pbf][ty][txm], s3[pbf][ty][txp], s3[pbf][ty - 1][tx],
+                                             s3[pbf][ty + 1][tx], l3m2, l3, fq3, c3m1, &c4);
+                    if (!(inner_xy && k4 >= klo && k4 <= khi)) {
+                        l4 = keep ? l3m1 : bv;
+                        c4 = keep ? c3m1 : T(0);
+                    }
+                }
+                const size_t o = plane(k4);
+                const uint32_t os = st ? oo : range;
+                bstore<T>(l4, plane_rsrc((const char*)uout + o, range), os);
+                if constexpr (KH) bstore<T>(c4, plane_rsrc((const char*)cout + o, range), os);
+            }
+            l3m2 = l3m1;
+            l3m1 = l3;
+            c3m1 = c3;
+            l2m2 = l2m1;
+            l2m1 = l2;
+            c2m1 = c2;
+            l1m2 = l1m1;
+            l1m1 = l1;
+            c1m1 = c1;
+            fq3 = fq2;
+            fq2 = fq1;
+            fq1 = f;
+            fup = f.u;
+            am = ac;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Two reverse steps of the variable-kappa adjoint per sweep (lam^n ->
 // lam^{n-2}) with both steps' kappa-gradient terms, 3-D grids.  Per node the
@@ -680,8 +872,8 @@ int heatvar_adj2_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype,
     return adj2_launch_t<float>(c, geo, lam, lamo, kap, u1, u2, grad, cf);
 }
 
-// Multi-step tiles: steps per sweep (2 or 3), workgroup tile, z planes per
-// workgroup, prefetch depth.  The two-step entries keep their indices.
+// Multi-step tiles: steps per sweep (2, 3 or 4), workgroup tile, z planes
+// per workgroup, prefetch depth.  Earlier entries keep their indices.
 struct TbCfg {
     int steps, tx, ty, zc, d;
 };
@@ -691,6 +883,7 @@ static const TbCfg kTbCfgs[] = {
     {3, 64, 16, 64, 2}, {3, 64, 16, 32, 2},  {3, 64, 16, 128, 2}, {3, 64, 12, 64, 2}, {3, 64, 16, 64, 4},
     {3, 128, 8, 64, 2}, {3, 64, 8, 64, 2},   {3, 64, 12, 32, 2}, {3, 64, 12, 64, 4}, {3, 64, 12, 96, 2},
     {3, 64, 10, 64, 2}, {3, 64, 14, 64, 2},
+    {4, 64, 16, 64, 2}, {4, 64, 16, 32, 2},  {4, 64, 16, 128, 2}, {4, 64, 16, 96, 2}, {4, 64, 16, 64, 4},
 };
 static constexpr int kNumTb = sizeof(kTbCfgs) / sizeof(kTbCfgs[0]);
 
@@ -705,6 +898,9 @@ static int tb_launch_t(Ctx* c, const Geo& g, const void* u, const void* cin, con
     if (nb > 0x7fffffffL) return fail(SCAFES_ERR_ARG, "grid too large");
     if constexpr (STEPS == 2)
         heatvar_tb2_kernel<T, MODE, TX, TY, D><<<dim3((unsigned)nb), dim3(TX, TY), 0, c->stream>>>(
+            g, (const T*)u, (const T*)cin, (const T*)kap, (T*)uo, (T*)co, cf, zc, tiles_x, tiles_y);
+    else if constexpr (STEPS == 4)
+        heatvar_tb4_kernel<T, MODE, TX, TY, D><<<dim3((unsigned)nb), dim3(TX, TY), 0, c->stream>>>(
             g, (const T*)u, (const T*)cin, (const T*)kap, (T*)uo, (T*)co, cf, zc, tiles_x, tiles_y);
     else
         heatvar_tb3_kernel<T, MODE, TX, TY, D, MO><<<dim3((unsigned)nb), dim3(TX, TY), 0, c->stream>>>(
@@ -738,6 +934,8 @@ static int tb_dispatch(Ctx* c, const Geo& g, const void* u, const void* cin, con
         SCAFES_TB(3, 64, 12, 4)
         SCAFES_TB(3, 64, 10, 2)
         SCAFES_TB(3, 64, 14, 2)
+        SCAFES_TB(4, 64, 16, 2)
+        SCAFES_TB(4, 64, 16, 4)
     default: return fail(SCAFES_ERR_ARG, "bad multi-step tile configuration");
     }
 #undef SCAFES_TB

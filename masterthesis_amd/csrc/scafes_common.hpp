@@ -133,17 +133,14 @@ int pennes_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, c
 // stencil3d.hip); mode is a SCAFES_HEATVAR_* value.
 int heat7var_step_one(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* un,
                       const void* cin, void* cout, const void* kap, double tau, int border_mode, double border_value);
-// Steps one sweep of the mode's default (or autotuned) tile advances on
-// this grid, and that sweep (stencil3d.hip); bit-identical to as many
-// one-step launches.
-int heat7var_sweep_steps(const Ctx* c, const Geo& geo, int mode);
-int heat7var_sweep(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* un, const void* cin,
-                   void* cout, const void* kap, double tau, int border_mode, double border_value);
-// Two steps in one sweep with the mode's two-step tile, where the grid's
-// sweep takes three (heat7var_sweep_steps == 3): the two steps left behind
-// three-step sweeps; SCAFES_ERR_UNSUPPORTED otherwise.
-int heat7var_sweep2(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* un, const void* cin,
-                    void* cout, const void* kap, double tau, int border_mode, double border_value);
+// Steps of the next sweep of a run with `left` steps to go on this grid
+// (the mode's default or autotuned tile, the remainders of its multi-step
+// sweeps taken by the two-, three- or four-step tile of the mode), and
+// that sweep of k steps (stencil3d.hip); bit-identical to k one-step
+// launches.
+int heat7var_next_sweep(const Ctx* c, const Geo& geo, int mode, long left);
+int heat7var_sweep_k(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, int k, const void* u, void* un,
+                     const void* cin, void* cout, const void* kap, double tau, int border_mode, double border_value);
 // Three steps u -> un (cin -> cout) in one sweep that also writes the
 // states after one and two steps to mid[0], mid[1] (value parts only in the
 // compensated mode), when the grid's three-step tile has such a variant;

@@ -664,6 +664,15 @@ static constexpr int kDefaultCfgByMode[3] = {kNumCfgs + 10, kNumCfgs + 13, kNumC
 // configuration (F64 64x8 zc64 d4, F32 and F32+Kahan 64x8 zc64 d2): one
 // sweep instead of two one-step sweeps.
 static constexpr int kTwoStepCfgByMode[3] = {kNumCfgs + 6, kNumCfgs + 2, kNumCfgs + 2};
+// Four-step tile per mode, or -1: F32 + Kahan 64x16 zc128 d2.  Measured on
+// two MI355X boxes at 512^3 (us per sweep): 1084-1185 for four steps
+// against 800-813 for three with the default tile (271-290 against 267-271
+// us per step) and 776-780 for the two-step tile, so 20 steps as 2 x 4 +
+// 4 x 3 took 5.42-5.61 ms against 5.64-5.66 as 6 x 3 + 2.  FP32 (963
+// against 638 for three steps: 241 and 213 us per step) and FP64 (2660
+// against 1790: 665 and 597) lose more per step with four steps and keep
+// the two-step remainder.
+static constexpr int kFourStepCfgByMode[3] = {-1, -1, kNumCfgs + 24};
 
 template <class Op, typename T, int TX, int TY, int D>
 static int launch_t(Ctx* c, const Op& op, const Geo& g, int zc)
@@ -1107,6 +1116,26 @@ int scafes_heat7var_active_cfg(scafes_ctx* ctx, const scafes_geom* g, int hw, in
     return SCAFES_OK;
 }
 
+int scafes_heat7var_plan(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode, int nsteps, int* sweeps, int max,
+                         int* count)
+{
+    Ctx* c = as_ctx(ctx);
+    if (!c || !g || !count || (max > 0 && !sweeps)) return fail(SCAFES_ERR_ARG, "heat7var_plan: null pointer");
+    if (mode < 0 || mode > 2) return fail(SCAFES_ERR_ARG, "heat7var_plan: bad mode");
+    if (nsteps < 0) return fail(SCAFES_ERR_ARG, "heat7var_plan: negative step count");
+    Geo geo;
+    const int rc = make_geo(g, hw, &geo);
+    if (rc) return rc;
+    int n = 0;
+    for (long s = 0; s < nsteps; ++n) {
+        const int k = heat7var_next_sweep(c, geo, mode, nsteps - s);
+        if (n < max) sweeps[n] = k;
+        s += k;
+    }
+    *count = n;
+    return SCAFES_OK;
+}
+
 int scafes_heat7var_run(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode, void* u, void* u2,
                         void* cmp, void* cmp2, const void* kappa, double tau, int nsteps,
                         int border_mode, double border_value, int cfg, int* result_in_second)
@@ -1132,11 +1161,22 @@ int scafes_heat7var_run(scafes_ctx* ctx, const scafes_geom* g, int hw, int mode,
         rc = check_explicit_tau(tau, diffusion_rate(g, kmax), "heat7var");
         if (rc) return rc;
     }
+    const bool planned = cfg < 0;  // the default or autotuned tile
     cfg = resolve_heatvar_cfg(c, geo, mode, cfg);
     void *a = u, *b = u2, *ca = cmp, *cb = cmp2;
     int launches = 0;
     int s = 0;
-    if (cfg >= kNumCfgs) {
+    if (planned && cfg >= kNumCfgs) {
+        while (s < nsteps) {
+            const int k = heat7var_next_sweep(c, geo, mode, nsteps - s);
+            rc = heat7var_sweep_k(c, geo, g, mode, k, a, b, ca, cb, kappa, tau, border_mode, border_value);
+            if (rc) return rc;
+            std::swap(a, b);
+            std::swap(ca, cb);
+            s += k;
+            ++launches;
+        }
+    } else if (cfg >= kNumCfgs) {
         int tx_, ty_, zc_, d_, st = 1;
         heatvar_tb2_cfg(cfg - kNumCfgs, &tx_, &ty_, &zc_, &d_, &st);
         for (; s + st <= nsteps; s += st, ++launches) {
@@ -1178,9 +1218,10 @@ int scafes_autotune_heat7var(scafes_ctx* ctx, const scafes_geom* g, int hw, int 
     Geo geo;
     int rc = make_geo(g, hw, &geo);
     if (rc) return rc;
-    // a multiple of 6 runs every configuration (1, 2 or 3 steps per sweep)
-    // without a remainder sweep, and an even sweep count keeps u in place
-    const int steps = 6 * std::max(1, (nsteps + 5) / 6);
+    // a multiple of 24 runs every configuration (1, 2, 3 or 4 steps per
+    // sweep) without a remainder sweep, and an even sweep count keeps u in
+    // place
+    const int steps = 24 * std::max(1, (nsteps + 23) / 24);
     hipEvent_t e0, e1;
     SCAFES_HIP(hipEventCreate(&e0));
     SCAFES_HIP(hipEventCreate(&e1));
@@ -1383,32 +1424,46 @@ int scafes_pennes_adjoint_step(scafes_ctx* ctx, const scafes_geom* g, int hw, in
 
 namespace scafes {
 
-int heat7var_sweep_steps(const Ctx* c, const Geo& geo, int mode)
+// Steps per sweep of a tile configuration index (one-step tiles: 1).
+static int cfg_steps(int cfg)
 {
-    const int cfg = resolve_heatvar_cfg(c, geo, mode, -1);
     if (cfg < kNumCfgs) return 1;
     int tx, ty, zc, d, st = 1;
     heatvar_tb2_cfg(cfg - kNumCfgs, &tx, &ty, &zc, &d, &st);
     return st;
 }
 
-int heat7var_sweep(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* un, const void* cin,
-                   void* cout, const void* kap, double tau, int border_mode, double border_value)
+// Behind a three-step tile, a remainder of two steps takes one two-step
+// sweep; where the mode has a four-step tile of the same cost per step,
+// step counts that are not a multiple of three are made of three- and
+// four-step sweeps instead (every count from 6 on: 7 = 4 + 3, 8 = 4 + 4),
+// so no short sweep pays for a whole pass over HBM.  A four-step tile
+// takes a remainder of three with the three-step default tile.
+int heat7var_next_sweep(const Ctx* c, const Geo& geo, int mode, long left)
 {
-    const int cfg = resolve_heatvar_cfg(c, geo, mode, -1);
-    if (cfg < kNumCfgs)
-        return heat7var_step_one(c, geo, g, mode, u, un, cin, cout, kap, tau, border_mode, border_value);
-    return heatvar_tb2_launch(c, geo, g, mode, u, un, cin, cout, kap, tau, border_mode, border_value,
-                              cfg - kNumCfgs);
+    const int st = cfg_steps(resolve_heatvar_cfg(c, geo, mode, -1));
+    if (left <= 1 || st == 1) return 1;
+    if (st == 2) return 2;
+    if (st == 4) return left >= 4 ? 4 : (int)left;
+    if (kFourStepCfgByMode[mode] >= 0 && left >= 4 && (left % 3 == 1 || (left % 3 == 2 && left >= 8))) return 4;
+    return left >= 3 ? 3 : 2;
 }
 
-int heat7var_sweep2(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* un, const void* cin,
-                    void* cout, const void* kap, double tau, int border_mode, double border_value)
+int heat7var_sweep_k(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, int k, const void* u, void* un,
+                     const void* cin, void* cout, const void* kap, double tau, int border_mode, double border_value)
 {
-    if (heat7var_sweep_steps(c, geo, mode) != 3)
-        return fail(SCAFES_ERR_UNSUPPORTED, "no two-step sweep behind a three-step tile on this grid");
+    if (k == 1) return heat7var_step_one(c, geo, g, mode, u, un, cin, cout, kap, tau, border_mode, border_value);
+    int cfg = resolve_heatvar_cfg(c, geo, mode, -1);
+    if (cfg_steps(cfg) != k) {
+        if (k == 2) cfg = kTwoStepCfgByMode[mode];
+        else if (k == 3) cfg = kDefaultCfgByMode[mode];
+        else if (k == 4) cfg = kFourStepCfgByMode[mode];
+        else cfg = -1;
+    }
+    if (cfg < kNumCfgs || cfg_steps(cfg) != k || !heatvar_tb2_supported(geo, mode))
+        return fail(SCAFES_ERR_UNSUPPORTED, "no sweep of this many steps for this grid and mode");
     return heatvar_tb2_launch(c, geo, g, mode, u, un, cin, cout, kap, tau, border_mode, border_value,
-                              kTwoStepCfgByMode[mode] - kNumCfgs);
+                              cfg - kNumCfgs);
 }
 
 bool heat7var_states_sweep_ok(const Ctx* c, const Geo& geo, int mode)

@@ -33,13 +33,13 @@ def test_host_side_argument_checks_need_no_gpu():
     tx = capi.C.c_int()
     assert L.scafes_heat7var_cfg_desc(999, capi.C.byref(tx), None, None, None, None) == -1
     assert b"bad cfg" in L.scafes_last_error()
-    # one-step tiles first, then the two- and three-steps-per-sweep tiles
+    # one-step tiles first, then the two-, three- and four-steps-per-sweep tiles
     st = capi.C.c_int()
     spans = []
     for cfg in range(L.scafes_heat7var_num_cfgs()):
         assert L.scafes_heat7var_cfg_desc(cfg, None, None, None, None, capi.C.byref(st)) == 0
         spans.append(st.value)
-    assert set(spans) == {1, 2, 3} and spans == sorted(spans)
+    assert set(spans) == {1, 2, 3, 4} and spans == sorted(spans)
     # null context is rejected before any HIP call
     assert L.scafes_fill(None, None, capi.F32, 10, 0.0) == -1
     n = capi.C.c_int(-7)

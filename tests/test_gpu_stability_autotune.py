@@ -211,16 +211,52 @@ def test_autotune_picks_valid_cfg_kept_per_grid(ctx, mode):
             d.free()
 
 
-@pytest.mark.parametrize("nsteps", [2, 5, 8, 20])
-def test_two_step_remainder_bit_identical(ctx, nsteps):
-    """A three-step configuration with two steps left runs them as one
-    two-step sweep: the result equals one-step launches bit for bit."""
+@pytest.mark.parametrize("nsteps", [1, 2, 4, 5, 7, 8, 10, 11, 20])
+def test_default_sweep_plan_bit_identical(ctx, nsteps):
+    """The default run splits the steps into three-step sweeps, a two-step
+    sweep for two left over, and (FP32 + Kahan) four-step sweeps for step
+    counts that are not a multiple of three: the result equals one-step
+    launches bit for bit in every mode."""
     n = (40, 36, 30)
     g = capi.geom(n)
     og, u0, kap, tau = _heatvar_case(n, 21)
-    for mode in (capi.HEATVAR_F32_KAHAN, capi.HEATVAR_F64):
+    for mode in (capi.HEATVAR_F32_KAHAN, capi.HEATVAR_F32, capi.HEATVAR_F64):
         multi, cm = _run(ctx, g, mode, u0, kap, tau, nsteps)
         one, c1 = _run(ctx, g, mode, u0, kap, tau, nsteps, cfg=0)
         np.testing.assert_array_equal(multi, one)
         if cm is not None:
             np.testing.assert_array_equal(cm, c1)
+
+
+def _plan(ctx, g, mode, nsteps, cap=64):
+    cnt = C.c_int(0)
+    buf = (C.c_int * cap)()
+    capi.check(ctx.L.scafes_heat7var_plan(ctx.h, C.byref(g), 1, mode, nsteps, buf, cap, C.byref(cnt)))
+    return list(buf[:min(cnt.value, cap)]), cnt.value
+
+
+def test_sweep_plan(ctx):
+    """scafes_heat7var_plan reports the default run's sweeps: they add up to
+    the step count; FP32 + Kahan covers every count from 6 on with three-
+    and four-step sweeps (at most two four-step ones); FP32 and FP64 take a
+    remainder of two as one two-step sweep and of one as one step; a 2-D
+    grid has no multi-step tile."""
+    g = capi.geom((40, 36, 30))
+    K, F32, F64 = capi.HEATVAR_F32_KAHAN, capi.HEATVAR_F32, capi.HEATVAR_F64
+    for n in range(0, 40):
+        p, c = _plan(ctx, g, K, n)
+        assert sum(p) == n and c == len(p)
+        if n >= 6:
+            assert set(p) <= {3, 4} and p.count(4) <= 2, (n, p)
+        for mode in (F32, F64):
+            q, _ = _plan(ctx, g, mode, n)
+            assert sum(q) == n and q[: n // 3] == [3] * (n // 3), (n, q)
+            assert q[n // 3:] == ([2] if n % 3 == 2 else [1] if n % 3 == 1 else []), (n, q)
+    assert _plan(ctx, g, K, 20)[0] == [4, 4, 3, 3, 3, 3]
+    assert _plan(ctx, g, K, 1000, cap=400)[0] == [4] + [3] * 332
+    assert _plan(ctx, g, K, 5)[0] == [3, 2]
+    assert _plan(ctx, g, K, 4)[0] == [4]
+    assert _plan(ctx, capi.geom((40, 36)), K, 5)[0] == [1] * 5
+    # the count is reported in full when the array is shorter
+    p, c = _plan(ctx, g, K, 20, cap=2)
+    assert p == [4, 4] and c == 6
