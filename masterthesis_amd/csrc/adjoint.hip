@@ -136,10 +136,10 @@ struct Rev {
                                      border_value);
         return pennes_step_launch(c, geo, g, dtype, a, b, w, q, p, lo(a), lo(b));
     }
+    // the library's device copy kernel (about 1.8x the runtime's copy rate)
     int copy(void* dst, const void* src)
     {
-        hipError_t e = hipMemcpyAsync(dst, src, sbytes, hipMemcpyDeviceToDevice, c->stream);
-        return e == hipSuccess ? SCAFES_OK : check_hip(e, "checkpoint copy");
+        return scafes_memcpy_d2d(reinterpret_cast<scafes_ctx*>(c), dst, src, sbytes);
     }
     // State after n steps from src, written to dst (src is never written).
     // The Yee and variable-kappa families advance several steps per sweep
@@ -315,6 +315,27 @@ struct Rev {
         return r;
     }
 
+    // How the variable-kappa family forms `total` consecutive states:
+    // `single` one-step sweeps first, then `q4` four-step and the rest
+    // three-step state-writing sweeps.  Four-step sweeps (where the mode has
+    // them) replace the one or two single steps a total that is not a
+    // multiple of three would leave (4 = 4, 8 = 4 + 4, 10 = 4 + 3 + 3).
+    void states_plan(long total, long* single, long* q4) const
+    {
+        *single = total;
+        *q4 = 0;
+        if (total < 3 || !heat7var_states_sweep_ok(c, geo, heat_mode)) return;
+        *single = total % 3;
+        if (!heat7var_states4_ok(c, geo, heat_mode)) return;
+        if (*single == 1 && total >= 4) {
+            *q4 = 1;
+            *single = 0;
+        } else if (*single == 2 && total >= 8) {
+            *q4 = 2;
+            *single = 0;
+        }
+    }
+
     // Where snapshots are plentiful the binomial split takes one step off
     // the left again and again: each time T^{a+1} goes into a snapshot, the
     // right part is reversed with one snapshot less, then step a+1 alone.
@@ -337,13 +358,13 @@ struct Rev {
                 ++len;
             }
         }
-        // The variable-kappa family forms the peeled states three per sweep
-        // where the grid's sweep can write the two states in between (value
-        // parts, all the adjoint reads), the len mod 3 first ones one step at
-        // a time, so the run's last state (the right part's base) is whole.
+        // The variable-kappa family forms the peeled states three or four per
+        // sweep where the grid's sweeps can write the states in between
+        // (value parts, all the adjoint reads), the rest first one step at a
+        // time, so the run's last state (the right part's base) is whole.
         std::vector<const void*> chain(1, base);  // chain[q] = T^{a+q}
-        const bool tri = family == FAMILY_HEATVAR && len >= 3 && heat7var_states_sweep_ok(c, geo, heat_mode);
-        const long single = tri ? len % 3 : len;
+        long single = len, q4 = 0;
+        if (family == FAMILY_HEATVAR) states_plan(len, &single, &q4);
         int r = SCAFES_OK;
         for (long q = 0; q < single && !r; ++q) {
             void* sl = slots.back();
@@ -352,17 +373,18 @@ struct Rev {
             chain.push_back(sl);
         }
         while (!r && (long)chain.size() - 1 < len) {
-            void* s3[3];
-            for (int q = 0; q < 3; ++q) {
-                s3[q] = slots.back();
+            const int k = q4 > 0 ? 4 : 3;
+            if (q4 > 0) --q4;
+            void* sk[4];
+            for (int q = 0; q < k; ++q) {
+                sk[q] = slots.back();
                 slots.pop_back();
             }
-            void* mid[2] = {s3[0], s3[1]};
             const void* x = chain.back();
-            r = heat7var_sweep_states(c, geo, g, heat_mode, x, s3[2], lo(x), lo(s3[2]), mid, kappa, tau, border_mode,
-                                      border_value);
-            fwd_steps += 3;
-            for (int q = 0; q < 3; ++q) chain.push_back(s3[q]);
+            r = heat7var_sweep_states(c, geo, g, heat_mode, k, x, sk[k - 1], lo(x), lo(sk[k - 1]), sk, kappa, tau,
+                                      border_mode, border_value);
+            fwd_steps += k;
+            for (int q = 0; q < k; ++q) chain.push_back(sk[q]);
         }
         const long aa = a + len;
         const int ss = s - (int)len;
@@ -399,18 +421,18 @@ int Rev::treeverse(long a, long b, int s, const void* base)
             std::vector<const void*> st(1, base);  // st[q] = T^{a+q}
             int r = SCAFES_OK;
             // The variable-kappa family takes the first states one step at a
-            // time, the rest three per sweep with the two states in between
-            // written by the same sweep (value parts only: its adjoint reads
-            // no more), so that the last state of each sweep is whole.  Where
-            // the stretch ends at N and lam^N is not set yet, T^N is the last
-            // sweep's output (into tmp[0]): (m + 1) mod 3 single steps instead
-            // of m mod 3 and one more for T^N.
+            // time, the rest three or four per sweep with the states in
+            // between written by the same sweep (value parts only: its
+            // adjoint reads no more), so that the last state of each sweep is
+            // whole.  Where the stretch ends at N and lam^N is not set yet,
+            // T^N is the last sweep's output (into tmp[0]): the m + 1 states
+            // are split instead of m and one more step for T^N.
             const long m = b - a - 1;
             const bool sweeps = family == FAMILY_HEATVAR && heat7var_states_sweep_ok(c, geo, heat_mode);
             const bool leaf = sweeps && b == nsteps && !lam_ready && m + 1 >= 3;
             const long total = leaf ? m + 1 : m;  // states this stretch forms
-            long single = m;
-            if (sweeps && total >= 3) single = total % 3;
+            long single = m, q4 = 0;
+            if (family == FAMILY_HEATVAR) states_plan(total, &single, &q4);
             for (long q = 1; q <= single && !r; ++q) {
                 void* sl = slots.back();
                 slots.pop_back();
@@ -418,25 +440,25 @@ int Rev::treeverse(long a, long b, int s, const void* base)
                 st.push_back(sl);
             }
             while (!r && (long)st.size() - 1 < total) {
-                const bool to_leaf = leaf && (long)st.size() - 1 + 3 == total;
-                void* s3[3];
-                for (int q = 0; q < 3; ++q) {
-                    if (q == 2 && to_leaf) {
-                        s3[q] = tmp[0];
+                const int k = q4 > 0 ? 4 : 3;
+                if (q4 > 0) --q4;
+                const bool to_leaf = leaf && (long)st.size() - 1 + k == total;
+                void* sk[4];
+                for (int q = 0; q < k; ++q) {
+                    if (q == k - 1 && to_leaf) {
+                        sk[q] = tmp[0];
                     } else {
-                        s3[q] = slots.back();
+                        sk[q] = slots.back();
                         slots.pop_back();
                     }
                 }
-                void* mid[2] = {s3[0], s3[1]};
                 const void* x = st.back();
-                r = heat7var_sweep_states(c, geo, g, heat_mode, x, s3[2], lo(x), lo(s3[2]), mid, kappa, tau,
-                                          border_mode, border_value);
-                fwd_steps += 3;
-                st.push_back(s3[0]);
-                st.push_back(s3[1]);
+                r = heat7var_sweep_states(c, geo, g, heat_mode, k, x, sk[k - 1], lo(x), lo(sk[k - 1]), sk, kappa,
+                                          tau, border_mode, border_value);
+                fwd_steps += k;
+                for (int q = 0; q < k - 1; ++q) st.push_back(sk[q]);
                 if (to_leaf) break;
-                st.push_back(s3[2]);
+                st.push_back(sk[k - 1]);
             }
             if (!r && leaf) r = first_leaf(nullptr);
             for (long n = b; n > a && !r;) {
@@ -554,9 +576,11 @@ static int run_schedule(Rev& R, Ctx* c, int dtype, const void* t0, size_t abytes
     if (kahan0) {
         void* base_buf = ws;
         ws += sb;
-        hipError_t e = hipMemcpyAsync(base_buf, t0, abytes, hipMemcpyDeviceToDevice, c->stream);
-        if (e == hipSuccess) e = hipMemsetAsync((char*)base_buf + abytes, 0, abytes, c->stream);
-        if (e != hipSuccess) rc = check_hip(e, "gradient init");
+        rc = scafes_memcpy_d2d(reinterpret_cast<scafes_ctx*>(c), base_buf, t0, abytes);
+        if (!rc) {
+            hipError_t e = hipMemsetAsync((char*)base_buf + abytes, 0, abytes, c->stream);
+            if (e != hipSuccess) rc = check_hip(e, "gradient init");
+        }
         base = base_buf;
     }
     R.tmp[0] = ws;

@@ -442,11 +442,15 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb3_kernel(Geo g, const T* __r
 // three-step kernel: wave-uniform row branches, off-ring rows writing the y
 // ring into a sink row, lanes off the output columns storing out of range.
 // Per four steps HBM moves one sweep of u, kappa (and c) plus the ring.
-template <typename T, int MODE, int TX, int TY, int D>
+// MO: the states after one, two and three steps are written too (u1o, u2o,
+// u3o; the value part only in the compensated mode) at every node of the
+// output tile, as the one-step kernel would write them.
+template <typename T, int MODE, int TX, int TY, int D, bool MO = false>
 __global__ __launch_bounds__(TX* TY) void heatvar_tb4_kernel(Geo g, const T* __restrict__ u, const T* __restrict__ cin,
                                                               const T* __restrict__ kap, T* __restrict__ uout,
-                                                              T* __restrict__ cout, VarCoef cf, int zc, int tiles_x,
-                                                              int tiles_y)
+                                                              T* __restrict__ cout, T* __restrict__ u1o,
+                                                              T* __restrict__ u2o, T* __restrict__ u3o, VarCoef cf,
+                                                              int zc, int tiles_x, int tiles_y)
 {
     static_assert(D >= 2 && D % 2 == 0, "prefetch depth must be even");
     static_assert(TX % 64 == 0, "whole rows per wavefront: ty is wave-uniform");
@@ -563,6 +567,10 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb4_kernel(Geo g, const T* __r
                 c1 = keep ? c0 : T(0);
             }
             s1[buf][ty][tx] = l1;
+            if constexpr (MO) {
+                const uint32_t os1 = (k >= k0 && k < k1 && row_out) ? oo : range;
+                bstore<T>(l1, plane_rsrc((const char*)u1o + plane(k), range), os1);
+            }
             // ---- level 2 at plane k-1 on rows 1..TY-2
             const int k2 = k - 1;
             T c2 = T(0), l2 = T(0);
@@ -575,6 +583,10 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb4_kernel(Geo g, const T* __r
                 }
             }
             s2[buf][ty][tx] = l2;
+            if constexpr (MO) {
+                const uint32_t os2 = (k2 >= k0 && k2 < k1 && row_out) ? oo : range;
+                bstore<T>(l2, plane_rsrc((const char*)u2o + plane(k2), range), os2);
+            }
             // ---- level 3 at plane k-2 on rows 2..TY-3
             const int k3 = k - 2;
             T c3 = T(0), l3 = T(0);
@@ -587,6 +599,10 @@ __global__ __launch_bounds__(TX* TY) void heatvar_tb4_kernel(Geo g, const T* __r
                 }
             }
             s3[buf][ty][tx] = l3;
+            if constexpr (MO) {
+                const uint32_t os3 = (k3 >= k0 && k3 < k1 && row_out) ? oo : range;
+                bstore<T>(l3, plane_rsrc((const char*)u3o + plane(k3), range), os3);
+            }
             // ---- level 4 at plane k-3 on the output rows and planes
             const int k4 = k - 3;
             {
@@ -900,8 +916,9 @@ static int tb_launch_t(Ctx* c, const Geo& g, const void* u, const void* cin, con
         heatvar_tb2_kernel<T, MODE, TX, TY, D><<<dim3((unsigned)nb), dim3(TX, TY), 0, c->stream>>>(
             g, (const T*)u, (const T*)cin, (const T*)kap, (T*)uo, (T*)co, cf, zc, tiles_x, tiles_y);
     else if constexpr (STEPS == 4)
-        heatvar_tb4_kernel<T, MODE, TX, TY, D><<<dim3((unsigned)nb), dim3(TX, TY), 0, c->stream>>>(
-            g, (const T*)u, (const T*)cin, (const T*)kap, (T*)uo, (T*)co, cf, zc, tiles_x, tiles_y);
+        heatvar_tb4_kernel<T, MODE, TX, TY, D, MO><<<dim3((unsigned)nb), dim3(TX, TY), 0, c->stream>>>(
+            g, (const T*)u, (const T*)cin, (const T*)kap, (T*)uo, (T*)co, MO ? (T*)mid[0] : nullptr,
+            MO ? (T*)mid[1] : nullptr, MO ? (T*)mid[2] : nullptr, cf, zc, tiles_x, tiles_y);
     else
         heatvar_tb3_kernel<T, MODE, TX, TY, D, MO><<<dim3((unsigned)nb), dim3(TX, TY), 0, c->stream>>>(
             g, (const T*)u, (const T*)cin, (const T*)kap, (T*)uo, (T*)co, MO ? (T*)mid[0] : nullptr,
@@ -941,8 +958,9 @@ static int tb_dispatch(Ctx* c, const Geo& g, const void* u, const void* cin, con
 #undef SCAFES_TB
 }
 
-// Three-step sweeps that also write the two intermediate states: the
-// default three-step tiles only (F64 64x16, F32 64x12, prefetch depth 2).
+// Multi-step sweeps that also write the intermediate states: the default
+// three-step tiles (F64 64x16, F32 64x12, prefetch depth 2) and the 64x16
+// four-step tile with prefetch depth 2.
 template <typename T, int MODE>
 static int tb_dispatch_mo(Ctx* c, const Geo& g, const void* u, const void* cin, const void* kap, void* uo, void* co,
                           const VarCoef& cf, int cfg, void* const* mid)
@@ -952,14 +970,17 @@ static int tb_dispatch_mo(Ctx* c, const Geo& g, const void* u, const void* cin, 
         return tb_launch_t<3, T, MODE, 64, 16, 2, true>(c, g, u, cin, kap, uo, co, cf, t.zc, mid);
     if (t.steps == 3 && t.tx == 64 && t.ty == 12 && t.d == 2)
         return tb_launch_t<3, T, MODE, 64, 12, 2, true>(c, g, u, cin, kap, uo, co, cf, t.zc, mid);
+    if (t.steps == 4 && t.tx == 64 && t.ty == 16 && t.d == 2)
+        return tb_launch_t<4, T, MODE, 64, 16, 2, true>(c, g, u, cin, kap, uo, co, cf, t.zc, mid);
     return fail(SCAFES_ERR_UNSUPPORTED, "no state-writing variant of this multi-step tile");
 }
 
-bool heatvar_tb3_states_supported(int cfg)
+bool heatvar_tb_states_supported(int cfg)
 {
     if (cfg < 0 || cfg >= kNumTb) return false;
     const TbCfg t = kTbCfgs[cfg];
-    return t.steps == 3 && t.tx == 64 && (t.ty == 16 || t.ty == 12) && t.d == 2;
+    return (t.steps == 3 && t.tx == 64 && (t.ty == 16 || t.ty == 12) && t.d == 2) ||
+           (t.steps == 4 && t.tx == 64 && t.ty == 16 && t.d == 2);
 }
 
 int heatvar_tb2_num_cfgs() { return kNumTb; }

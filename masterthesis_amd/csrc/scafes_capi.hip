@@ -92,7 +92,10 @@ __global__ void convert_kernel(D* d, const S* s, size_t n)
         d[i] = (D)s[i];
 }
 
-// Per-block FP64 partials of a node reduction; each thread walks x-rows.
+// Per-block FP64 partials of a node reduction.  Work items are 256-node
+// chunks of x-rows: a block takes items blockIdx.x, + gridDim.x, ..., one
+// node per thread, so the row and plane indices are formed once per chunk
+// (uniform over the block) rather than by 64-bit divisions per node.
 template <typename T>
 __global__ __launch_bounds__(256) void reduce_kernel(Geo ga, Geo gb, const T* __restrict__ a,
                                                      const T* __restrict__ b, int op, int inner_only,
@@ -100,14 +103,14 @@ __global__ __launch_bounds__(256) void reduce_kernel(Geo ga, Geo gb, const T* __
 {
     __shared__ double sh[256];
     double acc = 0.0;
-    const long nrows = (long)ga.ny * ga.nz;
-    const long total = nrows * ga.nx;
-    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total;
-         t += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(t % ga.nx);
-        const long r = t / ga.nx;
+    const int cpr = (ga.nx + 255) / 256;  // chunks per x-row
+    const long items = (long)ga.ny * ga.nz * cpr;
+    for (long w = blockIdx.x; w < items; w += gridDim.x) {
+        const long r = w / cpr;
+        const int i = (int)(w - r * cpr) * 256 + (int)threadIdx.x;
         const int j = (int)(r % ga.ny);
         const int k = (int)(r / ga.ny);
+        if (i >= ga.nx) continue;
         if (inner_only && is_border(ga, i, j, k)) continue;
         const double x = (double)a[at(ga, i, j, k)];
         const double y = b ? (double)b[at(gb, i, j, k)] : 0.0;

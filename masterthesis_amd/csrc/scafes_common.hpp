@@ -141,12 +141,15 @@ int heat7var_step_one(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, co
 int heat7var_next_sweep(const Ctx* c, const Geo& geo, int mode, long left);
 int heat7var_sweep_k(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, int k, const void* u, void* un,
                      const void* cin, void* cout, const void* kap, double tau, int border_mode, double border_value);
-// Three steps u -> un (cin -> cout) in one sweep that also writes the
-// states after one and two steps to mid[0], mid[1] (value parts only in the
-// compensated mode), when the grid's three-step tile has such a variant;
-// returns SCAFES_ERR_UNSUPPORTED (nothing launched) otherwise.
+// k = 3 or 4 steps u -> un (cin -> cout) in one sweep that also writes the
+// states after 1..k-1 steps to mid[0..k-2] (value parts only in the
+// compensated mode): k = 3 with the grid's three-step tile where it has such
+// a variant (heat7var_states_sweep_ok), k = 4 with the mode's four-step tile
+// behind it (heat7var_states4_ok); SCAFES_ERR_UNSUPPORTED (nothing
+// launched) otherwise.
 bool heat7var_states_sweep_ok(const Ctx* c, const Geo& geo, int mode);
-int heat7var_sweep_states(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* un,
+bool heat7var_states4_ok(const Ctx* c, const Geo& geo, int mode);
+int heat7var_sweep_states(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, int k, const void* u, void* un,
                           const void* cin, void* cout, void* const* mid, const void* kap, double tau,
                           int border_mode, double border_value);
 // Adjoint step of the variable-kappa heat w.r.t. kappa (defined in
@@ -177,17 +180,18 @@ int pennes_adjk_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, 
                        void* lam_out, const void* w, const void* const* t, const void* const* tlo, double* grad,
                        const scafes_pennes_params* p);
 
-// Multi-step (two or three steps per sweep) variable-kappa heat (defined in
-// heatvar_tb2.hip); heatvar_tb2_launch advances the configuration's steps.
+// Multi-step (two, three or four steps per sweep) variable-kappa heat
+// (defined in heatvar_tb2.hip); heatvar_tb2_launch advances the
+// configuration's steps.
 int heatvar_tb2_num_cfgs();
 void heatvar_tb2_cfg(int cfg, int* tx, int* ty, int* zc, int* d, int* steps);
 bool heatvar_tb2_supported(const Geo& geo, int mode);  // 3-D grids with 32-bit in-plane offsets
-// mid = nullptr, or the two intermediate state outputs of a three-step
-// configuration with a state-writing variant (heatvar_tb3_states_supported).
+// mid = nullptr, or the steps - 1 intermediate state outputs of a
+// configuration with a state-writing variant (heatvar_tb_states_supported).
 int heatvar_tb2_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* uo,
                        const void* cin, void* co, const void* kap, double tau, int border_mode, double border_value,
                        int cfg, void* const* mid = nullptr);
-bool heatvar_tb3_states_supported(int cfg);
+bool heatvar_tb_states_supported(int cfg);
 
 // Yee box of N cells per axis; fields hold N+1 nodes per axis, x fastest.
 struct YeeDims {

@@ -1469,16 +1469,23 @@ int heat7var_sweep_k(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, int
 bool heat7var_states_sweep_ok(const Ctx* c, const Geo& geo, int mode)
 {
     const int cfg = resolve_heatvar_cfg(c, geo, mode, -1);
-    return cfg >= kNumCfgs && heatvar_tb3_states_supported(cfg - kNumCfgs);
+    return cfg >= kNumCfgs && cfg_steps(cfg) == 3 && heatvar_tb_states_supported(cfg - kNumCfgs);
 }
 
-int heat7var_sweep_states(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, const void* u, void* un,
+bool heat7var_states4_ok(const Ctx* c, const Geo& geo, int mode)
+{
+    const int f = kFourStepCfgByMode[mode];
+    return heat7var_states_sweep_ok(c, geo, mode) && f >= kNumCfgs && heatvar_tb_states_supported(f - kNumCfgs);
+}
+
+int heat7var_sweep_states(Ctx* c, const Geo& geo, const scafes_geom* g, int mode, int k, const void* u, void* un,
                           const void* cin, void* cout, void* const* mid, const void* kap, double tau,
                           int border_mode, double border_value)
 {
-    const int cfg = resolve_heatvar_cfg(c, geo, mode, -1);
-    if (cfg < kNumCfgs || !heatvar_tb3_states_supported(cfg - kNumCfgs))
-        return fail(SCAFES_ERR_UNSUPPORTED, "no state-writing three-step sweep for this grid");
+    int cfg = -1;
+    if (k == 3 && heat7var_states_sweep_ok(c, geo, mode)) cfg = resolve_heatvar_cfg(c, geo, mode, -1);
+    if (k == 4 && heat7var_states4_ok(c, geo, mode)) cfg = kFourStepCfgByMode[mode];
+    if (cfg < kNumCfgs) return fail(SCAFES_ERR_UNSUPPORTED, "no state-writing sweep of this many steps for this grid");
     return heatvar_tb2_launch(c, geo, g, mode, u, un, cin, cout, kap, tau, border_mode, border_value,
                               cfg - kNumCfgs, mid);
 }

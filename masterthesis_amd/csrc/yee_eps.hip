@@ -437,10 +437,7 @@ int scafes_yee_eps_run(scafes_ctx* ctx, int ncell, int dtype, double s, const vo
         std::swap(a, b);
         n += k;
     }
-    if (rc == SCAFES_OK && a != f) {
-        hipError_t e = hipMemcpyAsync(f, a, 6 * (size_t)d.nf * es, hipMemcpyDeviceToDevice, c->stream);
-        if (e != hipSuccess) rc = check_hip(e, "yee eps copy");
-    }
+    if (rc == SCAFES_OK && a != f) rc = scafes_memcpy_d2d(reinterpret_cast<scafes_ctx*>(c), f, a, 6 * (size_t)d.nf * es);
     (void)hipFreeAsync(ce, c->stream);
     return rc;
 }

@@ -125,18 +125,21 @@ def test_fp32_modes_vs_fp64_oracle(ctx, mode):
     assert abs(J - Jref) < 1e-4 * Jref
 
 
-@pytest.mark.parametrize("N", [12, 11, 10])
+@pytest.mark.parametrize("N", [12, 11, 10, 13, 8, 20])
 def test_fp32_checkpointing_does_not_change_gradient(ctx, N):
     """FP32 + Kahan: the gradient is the same bit for bit whatever the
     schedule: one-step launches only (no snapshots), a binomial budget (at
     N = 12 with 8 snapshots a run of 6 peeled states, formed three per
-    sweep), and store-all, whose reverse steps run 4+4+4, 4+4+3 and 4+4+2
-    per sweep at N = 12, 11, 10."""
+    sweep; at N = 20 with 12 a run of 8, formed by two four-step sweeps),
+    and store-all, whose reverse steps run 4+4+4, 4+4+3 and 4+4+2 per sweep
+    at N = 12, 11, 10 and whose forward states (T^N included) come from
+    three- and four-step sweeps: 3+3+3+3 at N = 12, 4+4+3 at 11, 4+3+3 at
+    10, 4+3+3+3 at 13, 4+4 at 8 (T^N from a four-step sweep)."""
     n = (40, 36, 33)
     g, h, u0, kap, tau, target = _case(n, seed=4)
     J0, g0, _ = _device(ctx, n, h, 1, capi.HEATVAR_F32_KAHAN, u0, kap, target, tau, capi.BORDER_VALUE, 0.0, N, 0)
     J1, g1, _ = _device(ctx, n, h, 1, capi.HEATVAR_F32_KAHAN, u0, kap, target, tau, capi.BORDER_VALUE, 0.0, N,
-                        8 if N == 12 else 3)
+                        {12: 8, 20: 12}.get(N, 3))
     J2, g2, _ = _device(ctx, n, h, 1, capi.HEATVAR_F32_KAHAN, u0, kap, target, tau, capi.BORDER_VALUE, 0.0, N, N - 1)
     np.testing.assert_array_equal(g1, g0)
     np.testing.assert_array_equal(g2, g0)
