@@ -480,9 +480,11 @@ def bench_kappa_grad(ctx, capi, g, n, u0, k0, tau, nsteps, t_fwd_ms, snapshots=(
     grad = ctx.alloc(ne, np.float64)
     out = {"workload": f"heat7var {n}^3 dJ/dkappa, {nsteps} steps, FP32+Kahan forward, FP32 adjoint, FP64 gradient",
            "fwd_ms": round(t_fwd_ms, 2),
-           "recompute": "the headline three-step sweep where the schedule advances several steps; where it "
-                        "stores every state, the same sweep writing the two states in between "
-                        "(heatvar_tb3_kernel<float, 2, 64, 12, 2, true>, 28 B/node per sweep); else one step",
+           "recompute": "the headline run's sweeps where the schedule advances several steps (three-step "
+                        "sweeps, four-step ones where a count is not a multiple of three); where it stores "
+                        "every state, the same sweeps writing the states in between "
+                        "(heatvar_tb3_kernel<float, 2, 64, 12, 2, true>, 28 B/node per sweep; "
+                        "heatvar_tb4_kernel<float, 2, 64, 16, 2, true>, 32 B/node per sweep); else one step",
            "adjoint_kernels": "heatvar_adjk_kernel<float, 64, 16, K, 2> (K <= 4 reverse steps per sweep where the "
                               "schedule has the states at hand, (28 + 4K) B/node per sweep: lam r+w, kappa, "
                               "u^{n-1..n-K}, FP64 gradient r+w); stream_kernel<HeatVarAdjOp<float>> (one step, "
