@@ -223,7 +223,7 @@ __global__ __launch_bounds__(TX* TY) void heatvar_adjk_kernel(Geo g, const T* __
                 const double t = adj_term3<T>(cf, xlo, xhi, ylo, yhi, k > 0, k < g.nz - 1, ac, lw, le, ls, ln, am,
                                               ap, uc[0], su[0][buf][ty][txm], su[0][buf][ty][txp],
                                               su[0][buf][tym][tx], su[0][buf][typ][tx], um[0], up[0]);
-                sum[ring(u)] = gl + 0.5 * t;
+                sum[ring(u)] = gl + t;
             }
             // ---- levels 2..K: lam^{n-L} at plane k-(L-1), and the step-(n-L+1) term there
 #pragma unroll
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(TX* TY) void heatvar_adjk_kernel(Geo g, const T* __
                                      sl[l - 1][pbf][ty][txp], sl[l - 1][pbf][tym][tx], sl[l - 1][pbf][typ][tx], lm,
                                      lp, uc[l], su[l][buf][ty][txm], su[l][buf][ty][txp], su[l][buf][tym][tx],
                                      su[l][buf][typ][tx], um[l], up[l]);
-                    sum[ring(u - l)] = sum[ring(u - l)] + 0.5 * t;
+                    sum[ring(u - l)] = sum[ring(u - l)] + t;
                 }
                 if (L < K) {
                     hist[L][ring(u)] = v;
@@ -345,9 +345,9 @@ int heatvar_adjk_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype,
     cf.th0 = tau / hh[0];
     cf.th1 = tau / hh[1];
     cf.th2 = tau / hh[2];
-    cf.f0 = (float)cf.th0;
-    cf.f1 = (float)cf.th1;
-    cf.f2 = (float)cf.th2;
+    cf.f0 = (float)(0.5 * cf.th0);
+    cf.f1 = (float)(0.5 * cf.th1);
+    cf.f2 = (float)(0.5 * cf.th2);
     if (dtype == SCAFES_F64) return adjk_launch_k<double>(c, geo, K, lam, lamo, kap, st, grad, cf);
     return adjk_launch_k<float>(c, geo, K, lam, lamo, kap, st, grad, cf);
 }

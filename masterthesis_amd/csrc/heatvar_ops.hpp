@@ -66,7 +66,7 @@ __device__ inline T var_update(const VarCoef& cf, T uc, T uw, T ue, T us, T un, 
 struct AdjVarCoef {
     VarCoef v;                // lam update (the forward step's arithmetic)
     double th0, th1, th2;     // tau / h_p^2
-    float f0, f1, f2;         // the same, rounded to FP32
+    float f0, f1, f2;         // 0.5 * th_p rounded to FP32 (exactly half of th_p rounded)
 };
 
 __device__ inline double adj_face_sum(bool lo, bool hi, double lc, double lm, double lp, double uc, double um,
@@ -88,10 +88,14 @@ __device__ inline float adj_face_sum_f(bool lo, bool hi, float lc, float lm, flo
 
 // Kappa-gradient term of one adjoint step at a node of a 3-D grid,
 //   t = sum_p th_p * sum_{nb = -+e_p in the grid} (lam_c - lam_nb)(u_nb - u_c);
-// the gradient adds 0.5 t in FP64.  FP64 fields: t in FP64 in the oracle's
-// order.  FP32 fields: t in FP32 from the FP32 values, th_p rounded to FP32
-// and the axes joined with fmaf (the one-step operator in stencil3d.hip uses
-// the same arithmetic, so every schedule gives the same gradient).
+// returns the increment 0.5 t the FP64 gradient adds.  FP64 fields: t in
+// FP64 in the oracle's order, then 0.5 t.  FP32 fields: 0.5 t in FP32 from
+// the FP32 values with the coefficients 0.5 th_p rounded to FP32 and the
+// axes joined with fmaf, which is 0.5 times the term formed with th_p
+// rounded to FP32 (scaling by 2^-1 is exact at every rounding step away
+// from the subnormal range), so no FP64 multiply is left per term (the
+// one-step operator in stencil3d.hip uses the same arithmetic, so every
+// schedule gives the same gradient).
 template <typename T>
 __device__ inline double adj_term3(const AdjVarCoef& cf, bool xlo, bool xhi, bool ylo, bool yhi, bool zlo, bool zhi,
                                    T lc, T lw, T le, T ls, T ln, T ld, T lu, T uc, T uw, T ue, T us, T un, T ud,
@@ -102,12 +106,12 @@ __device__ inline double adj_term3(const AdjVarCoef& cf, bool xlo, bool xhi, boo
         t += cf.th0 * adj_face_sum(xlo, xhi, lc, lw, le, uc, uw, ue);
         t += cf.th1 * adj_face_sum(ylo, yhi, lc, ls, ln, uc, us, un);
         t += cf.th2 * adj_face_sum(zlo, zhi, lc, ld, lu, uc, ud, uu);
-        return t;
+        return 0.5 * t;
     } else {
         float t = cf.f0 * adj_face_sum_f(xlo, xhi, lc, lw, le, uc, uw, ue);
         t = __builtin_fmaf(cf.f1, adj_face_sum_f(ylo, yhi, lc, ls, ln, uc, us, un), t);
         t = __builtin_fmaf(cf.f2, adj_face_sum_f(zlo, zhi, lc, ld, lu, uc, ud, uu), t);
-        return (double)t;
+        return (double)t;  // 0.5 t: the halved coefficients scale every rounding step by 2^-1 exactly
     }
 }
 

@@ -812,8 +812,8 @@ __global__ __launch_bounds__(TX* TY) void heatvar_adj2_kernel(Geo g, const T* __
                                                    k2 < g.nz - 1, l1m1, pw, pe, ps, pn, l1m2, l1, u2m1,
                                                    sb[pbf][ty][tx - 1], sb[pbf][ty][tx + 1], sb[pbf][ty - 1][tx],
                                                    sb[pbf][ty + 1][tx], u2m2, vc);
-                    gv = gm1 + 0.5 * t1m1;
-                    gv = gv + 0.5 * t2;
+                    gv = gm1 + t1m1;
+                    gv = gv + t2;
                 }
                 const uint32_t os = st ? oo : range;
                 bstore<T>(v, plane_rsrc((const char*)lamo + plane(k2) * pb, range), os);
@@ -881,9 +881,9 @@ int heatvar_adj2_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype,
     cf.th0 = tau / hh[0];
     cf.th1 = tau / hh[1];
     cf.th2 = tau / hh[2];
-    cf.f0 = (float)cf.th0;
-    cf.f1 = (float)cf.th1;
-    cf.f2 = (float)cf.th2;
+    cf.f0 = (float)(0.5 * cf.th0);
+    cf.f1 = (float)(0.5 * cf.th1);
+    cf.f2 = (float)(0.5 * cf.th2);
     if (dtype == SCAFES_F64) return adj2_launch_t<double>(c, geo, lam, lamo, kap, u1, u2, grad, cf);
     return adj2_launch_t<float>(c, geo, lam, lamo, kap, u1, u2, grad, cf);
 }

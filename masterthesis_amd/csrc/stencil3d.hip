@@ -502,7 +502,7 @@ struct HeatVarAdjOp {
     double* grad;
     double tau, hh0, hh1, hh2, th0, th1, th2;  // th_p = tau / hh_p
     float c0, c1, c2;                          // 0.5*tau/h_p^2 (FP32 update)
-    float f0, f1, f2;                          // th_p rounded to FP32 (FP32 gradient terms)
+    float f0, f1, f2;                          // 0.5 th_p rounded to FP32 (FP32 gradient terms)
     __device__ static double face_sum(bool lo, bool hi, double lc, double lm, double lp, double uc, double um,
                                       double up)
     {
@@ -515,7 +515,7 @@ struct HeatVarAdjOp {
                           const T*) const
     {
         const long m = at(g, i, j, k);
-        double t = 0.0;
+        double t = 0.0;  // the gradient increment 0.5 * (the step's term)
         if constexpr (sizeof(T) == 8) {
             t += th0 * face_sum(i > 0, i < g.nx - 1, (double)l.c, (double)l.w, (double)l.e, (double)u.c,
                                 (double)u.w, (double)u.e);
@@ -525,14 +525,15 @@ struct HeatVarAdjOp {
             if (g.dim > 2)
                 t += th2 * face_sum(k > 0, k < g.nz - 1, (double)l.c, (double)l.d, (double)l.u, (double)u.c,
                                     (double)u.d, (double)u.u);
+            t = 0.5 * t;
         } else {
-            // FP32 fields: the term in FP32 (adj_term3's arithmetic in heatvar_ops.hpp)
+            // FP32 fields: half the term in FP32 (adj_term3's arithmetic in heatvar_ops.hpp)
             float tf = f0 * adj_face_sum_f(i > 0, i < g.nx - 1, l.c, l.w, l.e, u.c, u.w, u.e);
             if (g.dim > 1) tf = __builtin_fmaf(f1, adj_face_sum_f(j > 0, j < g.ny - 1, l.c, l.s, l.n, u.c, u.s, u.n), tf);
             if (g.dim > 2) tf = __builtin_fmaf(f2, adj_face_sum_f(k > 0, k < g.nz - 1, l.c, l.d, l.u, u.c, u.d, u.u), tf);
             t = (double)tf;
         }
-        grad[m] += 0.5 * t;
+        grad[m] += t;
         if (is_border(g, i, j, k)) {
             out[m] = T(0);
             return;
@@ -859,9 +860,9 @@ int heatvar_adj_launch(Ctx* c, const Geo& geo, const scafes_geom* g, int dtype, 
         op.th0 = tau / hh[0];
         op.th1 = tau / hh[1];
         op.th2 = tau / hh[2];
-        op.f0 = (float)op.th0;
-        op.f1 = (float)op.th1;
-        op.f2 = (float)op.th2;
+        op.f0 = (float)(0.5 * op.th0);
+        op.f1 = (float)(0.5 * op.th1);
+        op.f2 = (float)(0.5 * op.th2);
         op.c0 = (float)(0.5 * tau / hh[0]);
         op.c1 = (float)(0.5 * tau / hh[1]);
         op.c2 = (float)(0.5 * tau / hh[2]);
