@@ -142,6 +142,15 @@ typedef struct {
 int scafes_heat7_step_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const double* u_old,
                           double* u_new, const double* f_old, const double* g_old,
                           const double* g_new, const scafes_heat7_params* p);
+/* nsteps of the same update without known fields (no source, border KEEP
+ * or VALUE and the other rules that read no G), ping-ponging u -> u2 ->
+ * u ...: the loop of iterateOverTime (Problem.hpp:2558-2715) over the
+ * update phase for problems whose known fields do not enter the update,
+ * one launch per step and no host work in between.  *result_in_second =
+ * 1 when the final state is in u2.  Bit-identical to nsteps calls of
+ * scafes_heat7_step_f64. */
+int scafes_heat7_run_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, double* u, double* u2, int nsteps,
+                         const scafes_heat7_params* p, int* result_in_second);
 
 /* ---- variable-coefficient conservative 7-point heat -----------------------
  *   inner: u' = u + sum_p tau*(0.5(kE+kC)(uE-uC) - 0.5(kC+kW)(uC-uW))/h_p^2

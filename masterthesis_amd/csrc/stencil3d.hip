@@ -696,12 +696,29 @@ static int launch_t(Ctx* c, const Op& op, const Geo& g, int zc)
 // lengths): Pennes FP32 256^3 50.5 (best), 512^3 482 (best); Pennes FP64
 // 256^3 112 vs 109; MRI FP64 256^3 200 vs 192, 120x120x50 16.0 (best); the
 // previous 64 x 8, 128-plane default took 87, 515, 127, 263 and 102.
+// Rows of at most 64 nodes take 64 x 4 threads instead (a 128-wide row
+// would leave half its lanes idle), and where 8-plane chunks would give
+// fewer workgroups than CUs the chunks shrink towards four workgroups per
+// CU (heat 64^3 FP64, 100 steps: 7.2 us per step with 128 x 2 and 8-plane
+// chunks, 256 workgroups).
 constexpr int kAutoZc = 8;
+
+template <class Op, typename T>
+static int launch_auto(Ctx* c, const Op& op, const Geo& g)
+{
+    if (g.nx > 64) return launch_t<Op, T, 128, 2, 4>(c, op, g, kAutoZc);
+    const long per_plane = ceil_div(g.ny, 4);
+    const long cus = c->cu_count > 0 ? c->cu_count : 256;
+    int zc = kAutoZc;
+    if (per_plane * ceil_div(g.nz, zc) < cus)
+        zc = (int)std::max(1L, std::min((long)kAutoZc, per_plane * g.nz / (4 * cus)));
+    return launch_t<Op, T, 64, 4, 4>(c, op, g, zc);
+}
 
 template <class Op, typename T>
 static int launch(Ctx* c, const Op& op, const Geo& g, int cfg)
 {
-    if (cfg < 0) return launch_t<Op, T, 128, 2, 4>(c, op, g, kAutoZc);
+    if (cfg < 0) return launch_auto<Op, T>(c, op, g);
     if (cfg >= kNumCfgs) cfg = kDefaultCfg;
     const TileCfg t = kCfgs[cfg];
     switch (t.tx * 100000 + t.ty * 100 + t.d) {
@@ -1002,11 +1019,12 @@ int pennes_check_tau(Ctx* c, const Geo& geo, const scafes_geom* g, int hw, int d
 
 extern "C" {
 
-int scafes_heat7_step_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const double* u_old,
-                          double* u_new, const double* f_old, const double* g_old, const double* g_new,
-                          const scafes_heat7_params* p)
+// Argument checks and the operator of one constant-coefficient heat step
+// (shared by the step and the run entry points).
+static int heat7_const_op(Ctx* c, const scafes_geom* g, int hw, const double* u_old, double* u_new,
+                          const double* f_old, const double* g_old, const double* g_new,
+                          const scafes_heat7_params* p, Geo* geo, HeatConstOp* op)
 {
-    Ctx* c = as_ctx(ctx);
     if (!c || !g || !u_old || !u_new || !p) return fail(SCAFES_ERR_ARG, "heat7: null pointer");
     if (u_old == u_new) return fail(SCAFES_ERR_ARG, "heat7: in-place update not allowed");
     int rc = check_stencil_geo(g, hw);
@@ -1020,8 +1038,8 @@ int scafes_heat7_step_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const d
         return fail(SCAFES_ERR_ARG, "heat7: flux border needs G_old and G_new");
     if (p->border_mode >= SCAFES_BORDER_NEUMANN && !(p->lambda != 0.0))
         return fail(SCAFES_ERR_ARG, "heat7: flux border needs lambda != 0");
-    Geo geo, gk;
-    rc = make_geo(g, hw, &geo);
+    Geo gk;
+    rc = make_geo(g, hw, geo);
     if (rc) return rc;
     rc = make_geo(g, 0, &gk);
     if (rc) return rc;
@@ -1031,28 +1049,63 @@ int scafes_heat7_step_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const d
     if (rc) return rc;
     double hh[3];
     hh_of(g, hh);
-    HeatConstOp op{};
-    op.a = u_old;
-    op.out = u_new;
-    op.f_old = f_old;
-    op.g_old = g_old;
-    op.g_new = g_new;
-    op.gk = gk;
-    for (int q = 0; q < 3; ++q) op.a2[q] = q < g->dim ? 2.0 * g->h[q] / p->lambda : 0.0;
-    op.alpha = p->alpha;
-    op.epsilon = p->epsilon;
-    op.sigma = p->sigma;
-    op.ctau_a = p->tau * p->coeff_a;
-    op.ctw = (p->tau * p->perf_c) * p->perf_w;
-    op.ctf = p->tau * p->rhoc_inv;
-    op.hh0 = hh[0];
-    op.hh1 = hh[1];
-    op.hh2 = hh[2];
-    op.has_perf = p->has_perf;
-    op.has_source = p->has_source;
-    op.border_mode = p->border_mode;
-    op.border_value = p->border_value;
+    *op = HeatConstOp{};
+    op->a = u_old;
+    op->out = u_new;
+    op->f_old = f_old;
+    op->g_old = g_old;
+    op->g_new = g_new;
+    op->gk = gk;
+    for (int q = 0; q < 3; ++q) op->a2[q] = q < g->dim ? 2.0 * g->h[q] / p->lambda : 0.0;
+    op->alpha = p->alpha;
+    op->epsilon = p->epsilon;
+    op->sigma = p->sigma;
+    op->ctau_a = p->tau * p->coeff_a;
+    op->ctw = (p->tau * p->perf_c) * p->perf_w;
+    op->ctf = p->tau * p->rhoc_inv;
+    op->hh0 = hh[0];
+    op->hh1 = hh[1];
+    op->hh2 = hh[2];
+    op->has_perf = p->has_perf;
+    op->has_source = p->has_source;
+    op->border_mode = p->border_mode;
+    op->border_value = p->border_value;
+    return SCAFES_OK;
+}
+
+int scafes_heat7_step_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, const double* u_old,
+                          double* u_new, const double* f_old, const double* g_old, const double* g_new,
+                          const scafes_heat7_params* p)
+{
+    Ctx* c = as_ctx(ctx);
+    Geo geo;
+    HeatConstOp op;
+    const int rc = heat7_const_op(c, g, hw, u_old, u_new, f_old, g_old, g_new, p, &geo, &op);
+    if (rc) return rc;
     return launch<HeatConstOp, double>(c, op, geo, -1);
+}
+
+int scafes_heat7_run_f64(scafes_ctx* ctx, const scafes_geom* g, int hw, double* u, double* u2, int nsteps,
+                         const scafes_heat7_params* p, int* result_in_second)
+{
+    Ctx* c = as_ctx(ctx);
+    if (nsteps < 0) return fail(SCAFES_ERR_ARG, "heat7 run: negative step count");
+    if (p && (p->has_source || p->border_mode == SCAFES_BORDER_FIELD || p->border_mode >= SCAFES_BORDER_NEUMANN))
+        return fail(SCAFES_ERR_ARG, "heat7 run: known fields change per step; call scafes_heat7_step_f64");
+    Geo geo;
+    HeatConstOp op;
+    int rc = heat7_const_op(c, g, hw, u, u2, nullptr, nullptr, nullptr, p, &geo, &op);
+    if (rc) return rc;
+    double *a = u, *b = u2;
+    for (int s = 0; s < nsteps; ++s) {
+        op.a = a;
+        op.out = b;
+        rc = launch<HeatConstOp, double>(c, op, geo, -1);
+        if (rc) return rc;
+        std::swap(a, b);
+    }
+    if (result_in_second) *result_in_second = nsteps % 2;
+    return SCAFES_OK;
 }
 
 // Configuration index space: [0, kNumCfgs) one-step tiles, then the

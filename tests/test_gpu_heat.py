@@ -158,6 +158,44 @@ def test_sine_mode_64cube_bitexact_and_analytic(ctx):
     assert rel < 1e-3
 
 
+@pytest.mark.parametrize("nsteps", [0, 1, 37])
+def test_heat7_run_f64_equals_steps(ctx, nsteps):
+    """scafes_heat7_run_f64: nsteps launches in one call, bit-identical to the
+    oracle (and so to nsteps single-step calls), the result in u2 after an
+    odd count; update rules that read known fields are refused."""
+    n = (33, 30, 28)
+    og = O.make_grid(n)
+    g = capi.geom(n)
+    h = 1.0 / (n[0] - 1)
+    tau = 0.9 * h * h / 6.0
+    u0, _ = sine_state(og)
+    u0 = u0 + 0.01 * np.random.default_rng(nsteps).random(u0.shape)
+    coef = O.heat7_coef(tau, coeff_a=1.0)
+    C = capi.C
+    for bm, bv in ((capi.BORDER_VALUE, 0.25), (capi.BORDER_KEEP, 0.0)):
+        ref = u0
+        for _ in range(nsteps):
+            ref = O.heat7_step(og, ref, border=bm, border_value=bv, coef=coef)
+        a, b = ctx.upload(u0), ctx.upload(u0)
+        p = capi.Heat7Params(tau=tau, coeff_a=1.0, rhoc_inv=1.0, has_perf=0, has_source=0, border_mode=bm,
+                             border_value=bv)
+        which = C.c_int(-1)
+        capi.check(ctx.L.scafes_heat7_run_f64(ctx.h, C.byref(g), 1, a.ptr, b.ptr, nsteps, C.byref(p),
+                                              C.byref(which)))
+        assert which.value == nsteps % 2
+        got = (b if which.value else a).read().reshape(u0.shape)
+        np.testing.assert_array_equal(O.nodes_view(og, got), O.nodes_view(og, ref))
+        a.free()
+        b.free()
+    a, b = ctx.upload(u0), ctx.upload(u0)
+    err_arg = -1  # SCAFES_ERR_ARG
+    for kw in ({"has_source": 1}, {"border_mode": capi.BORDER_NEUMANN, "lam": 1.0}):
+        p = capi.Heat7Params(tau=tau, coeff_a=1.0, rhoc_inv=1.0, **kw)
+        assert ctx.L.scafes_heat7_run_f64(ctx.h, C.byref(g), 1, a.ptr, b.ptr, 3, C.byref(p), None) == err_arg
+    a.free()
+    b.free()
+
+
 def heatvar_inputs(n, seed=0, amp=1e-3):
     og = O.make_grid(n)
     xs = _xs(n)
