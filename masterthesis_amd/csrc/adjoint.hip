@@ -237,7 +237,14 @@ struct Rev {
     // of a multi-step sweep), so 5 -> 3 + 2 instead of 4 + 1.
     int sweep_k(long left) const
     {
-        const int kmax = family == FAMILY_HEATVAR && !heatvar_adjk_supported(geo) ? 2 : kAdjMaxK;
+        int kmax = family == FAMILY_HEATVAR && !heatvar_adjk_supported(geo) ? 2 : kAdjMaxK;
+        // Under the Tumor border the four-step sweep needs more than the 80
+        // VGPRs of two workgroups per CU and spills (267 us per launch at
+        // 256^3 FP32, 67 us per step), the three-step one does not (161 us,
+        // 54 per step): the 2000-step gradient with 500 snapshots took 293
+        // ms with at most three reverse steps per sweep against 312 with four
+        // (355 with four at one workgroup per CU).
+        if (family == FAMILY_PENNES && p && p->border_mode == SCAFES_BORDER_TUMOR) kmax = 3;
         int K = (int)std::min<long>(kmax, left);
         if (left - K == 1 && K >= 3) --K;
         return K;
