@@ -1492,11 +1492,12 @@ static int cfg_steps(int cfg)
 }
 
 // Behind a three-step tile, a remainder of two steps takes one two-step
-// sweep; where the mode has a four-step tile of the same cost per step,
-// step counts that are not a multiple of three are made of three- and
-// four-step sweeps instead (every count from 6 on: 7 = 4 + 3, 8 = 4 + 4),
-// so no short sweep pays for a whole pass over HBM.  A four-step tile
-// takes a remainder of three with the three-step default tile.
+// sweep; where the mode has a four-step tile of about the same cost per
+// step, step counts that are not a multiple of three are made of three-
+// and four-step sweeps instead (4 = 4, 7 = 4 + 3, 8 = 4 + 4, 20 = 4 + 4 +
+// 4 x 3; only 5 keeps 3 + 2), so no short sweep pays for a whole pass over
+// HBM.  A tuned four-step tile takes a remainder of three with the mode's
+// three-step default tile, of two with its two-step tile.
 int heat7var_next_sweep(const Ctx* c, const Geo& geo, int mode, long left)
 {
     const int st = cfg_steps(resolve_heatvar_cfg(c, geo, mode, -1));
