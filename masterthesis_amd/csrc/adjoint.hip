@@ -58,11 +58,14 @@ __global__ __launch_bounds__(256) void lam_init_kernel(Geo g, const T* __restric
 {
     __shared__ double sh[256];
     double acc = 0.0;
-    const long total = (long)g.nx * g.ny * g.nz;
-    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(q % g.nx);
-        const long r = q / g.nx;
+    // work items: 256-node chunks of x-rows (indices formed once per chunk)
+    const int cpr = (g.nx + 255) / 256;
+    const long items = (long)g.ny * g.nz * cpr;
+    for (long w = blockIdx.x; w < items; w += gridDim.x) {
+        const long r = w / cpr;
+        const int i = (int)(w - r * cpr) * 256 + (int)threadIdx.x;
         const int j = (int)(r % g.ny), k = (int)(r / g.ny);
+        if (i >= g.nx) continue;
         const long m = at(g, i, j, k);
         double v = (double)t[m] - (double)target[m];
         if (tlo) v += (double)tlo[m];
