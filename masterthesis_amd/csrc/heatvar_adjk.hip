@@ -263,7 +263,10 @@ __global__ __launch_bounds__(TX* TY) void heatvar_adjk_kernel(Geo g, const T* __
     }
 }
 
-// Tile: 64 x 16 threads, prefetch depth 2, 32-plane z chunks.  Measured on
+// Tile: 64 x 16 threads, prefetch depth 2, 64-plane z chunks for FP32 (the
+// 20-step gradient at 512^3 with every state stored: 18.9-19.4 ms with 64-
+// and 128-plane chunks, 19.5-19.9 with 48 and 96, 19.6-19.9 with 32, 20.9-21.4
+// with 16), 32 for FP64.  Measured on
 // an MI355X (512^3 FP32 + Kahan, 20 steps store-all, whole gradient in ms):
 // K <= 4 64x16 24.0, 64x12 27.9; K <= 3 64x12 24.6, 64x16 25.2; K = 2
 // 64x8 24.5, 64x16 24.1; the two-step kernel of heatvar_tb2.hip 25.0.
@@ -278,7 +281,7 @@ __global__ __launch_bounds__(TX* TY) void heatvar_adjk_kernel(Geo g, const T* __
 // three-step rings exceed by a 16-byte spill).
 template <typename T, int K>
 struct AdjkTile {
-    static constexpr int TX = 64, TY = sizeof(T) == 8 ? 8 : (K == 3 ? 12 : 16), D = 2, ZC = 32;
+    static constexpr int TX = 64, TY = sizeof(T) == 8 ? 8 : (K == 3 ? 12 : 16), D = 2, ZC = sizeof(T) == 8 ? 32 : 64;
 };
 
 template <typename T, int K>
@@ -315,7 +318,7 @@ bool heatvar_adjk_supported(const Geo& geo)
     // 3-D only; a workgroup's chunk of the FP64 gradient (at most the z chunk
     // + 2K + the prefetch depth + 2 planes) plus the in-plane offsets must
     // stay below kRecords bytes
-    constexpr long kPlanes = 32 + 2 * kAdjMaxK + 2 + 2;
+    constexpr long kPlanes = 64 + 2 * kAdjMaxK + 2 + 2;
     return geo.dim == 3 && (kPlanes * geo.e01 + geo.base + geo.e01) * 8L < (long)kRecords;
 }
 
