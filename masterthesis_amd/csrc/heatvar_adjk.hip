@@ -271,9 +271,9 @@ __global__ __launch_bounds__(TX* TY) void heatvar_adjk_kernel(Geo g, const T* __
 // K <= 4 64x16 24.0, 64x12 27.9; K <= 3 64x12 24.6, 64x16 25.2; K = 2
 // 64x8 24.5, 64x16 24.1; the two-step kernel of heatvar_tb2.hip 25.0.
 // Those figures were taken with FP64 gradient terms.  With FP32 terms
-// (adj_term3) the K = 4 sweep takes 2.14 ms at 512^3 and still needs 123
-// VGPRs: one workgroup per CU, bound by vector issue (0.60 of its rate),
-// not by HBM (0.42 of 8 TB/s).  Two 12-wave workgroups per CU would need
+// (adj_term3) the K = 4 sweep takes 2.14 ms at 512^3 (2.01 with 64-plane
+// chunks) and still needs 123 VGPRs: one workgroup per CU, bound by vector
+// issue (0.60 of its rate), not by HBM (0.41 of 8 TB/s).  Two 12-wave workgroups per CU would need
 // at most 80 VGPRs.
 // FP64 fields: 64 x 8 (a 1024-thread workgroup caps a thread at 128 VGPRs,
 // which the FP64 rings exceed).
